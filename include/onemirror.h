@@ -1,0 +1,194 @@
+/*
+ * onemirror.h -- C ABI of the MI355X (gfx950) heliostat-field Monte-Carlo ray tracer.
+ *
+ * The reference (Hu_smooth/ClassedSunRay) exposes its tracer as C++ free functions and
+ * static methods over a singleton scene.  Each entry point below replaces one of them;
+ * the comment names the reference interface (file:line, root ClassedSunRay/).
+ *
+ * Conventions
+ *   - Every function returns an om_status; nothing calls exit().  om_last_error() gives a
+ *     message for the last failure on the calling thread.
+ *   - Device pointers are plain addresses in the memory of the context's GPU; sizes are
+ *     element counts.  Stream arguments are hipStream_t passed as void* (NULL = default).
+ *   - Trace and smoothing calls are ordered on the given stream and do not synchronise;
+ *     om_results_with_given_normal is synchronous, like the reference driver.
+ *   - Handles are thread-compatible: one thread per scene at a time.
+ */
+#ifndef ONEMIRROR_H
+#define ONEMIRROR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    OM_OK = 0,
+    OM_ERR_INVALID = 1,     /* bad argument                                     */
+    OM_ERR_IO = 2,          /* file could not be opened                          */
+    OM_ERR_PARSE = 3,       /* malformed scene file                              */
+    OM_ERR_HIP = 4,         /* HIP runtime error                                 */
+    OM_ERR_NOMEM = 5,       /* device or host allocation failed                  */
+    OM_ERR_UNSUPPORTED = 6, /* configuration outside what the library supports   */
+    OM_ERR_SIZE = 7         /* caller buffer too small                           */
+} om_status;
+
+typedef struct om_ctx om_ctx;
+typedef struct om_scene om_scene;
+
+/* Scene description: what a .scn file carries (scene_file_proc.cpp:41-286) plus the sun
+ * of SunRay (sunray.h:4-60).  World frame x = east, y = up, z = south (main.cpp:71-73). */
+typedef struct {
+    int          n_helio;
+    const float *helio_pos;    /* [n][3] heliostat centre                                 */
+    const float *helio_size;   /* [n][3] width (x), thickness (y), height (z)             */
+    const float *helio_gap;    /* [n][2] gap between facets                               */
+    const int   *helio_matrix; /* [n][2] facet rows, facet columns                        */
+    int   recv_type;           /* 0 rectangle (receiver.cuh:66-101)                       */
+    float recv_pos[3];
+    float recv_size[3];
+    float recv_normal[3];
+    int   recv_face;           /* 0..3 (receiver.cpp:30-48)                               */
+    float recv_pixel_length;   /* solarenergy::receiver_pixel_length (common_var.cpp:12)  */
+    float grid_pos[3];         /* RectGrid pos_/size_/interval_ (grid.h:25-49)            */
+    float grid_size[3];
+    float grid_interval[3];
+    float sun_dir[3];          /* direction the light travels                             */
+    float dni;                 /* W/m^2                                                   */
+} om_scene_desc;
+
+enum { OM_SUN_PILLBOX = 0, OM_SUN_BUIE = 1 };
+enum { OM_MICRO_LATTICE = 0, OM_MICRO_POISSON = 1 };
+
+/* Tracing parameters: the solarenergy:: globals (common_var.cpp:9-27) plus the per-ray
+ * Philox sampling that replaces the cuRAND pools (scene_instance_process.cu:6-226). */
+typedef struct {
+    uint64_t seed;             /* Philox key                                              */
+    uint32_t scene_index;      /* Philox stream of this scene (batched runs)              */
+    int   sunshape;            /* OM_SUN_*                                                */
+    float pillbox_half_angle;  /* rad                                                     */
+    float csr;                 /* Buie circumsolar ratio                                  */
+    int   buie_bins;           /* CDF bins on [0, 4.65] mrad (reference: 1024)            */
+    float slope_sigma;         /* rad (disturb_std)                                       */
+    int   micro_mode;          /* OM_MICRO_*; the reference default is the lattice        */
+    float helio_pixel_length;  /* microsample pitch (helio_pixel_length)                  */
+    int   poisson_k;           /* dart trials per cell (steps_for_raytracing.h:11)        */
+    int   rays_per_sample;     /* Nc, sun rays per microsample                            */
+    float reflectivity;        /* reflected_rate                                          */
+    int   smooth_radius;       /* trimmed-mean window radius (Gtest.cu:50)                */
+    float smooth_ratio;        /* trimmed fraction (Gtest.cu:51)                          */
+} om_params;
+
+/* Output maps of one receiver, row-major [res_y][res_x], all device memory. */
+typedef struct {
+    uint32_t *counts;          /* rays per pixel                                          */
+    uint64_t *energy;          /* sum of |s.n| per pixel, fixed point in 2^-31 units      */
+    float    *flux;            /* W/m^2, NULL to skip                                     */
+    float    *flux_smooth;     /* trimmed-mean smoothed W/m^2, NULL to skip               */
+    int64_t   capacity;        /* elements available in each non-NULL array               */
+} om_flux;
+
+typedef struct {
+    int     n_helio;
+    int     res_x, res_y;
+    int     grid_num[3];
+    int64_t grid_cells, grid_nnz;
+    int64_t total_micro;       /* microsamples over all heliostats                        */
+    int64_t total_rays;        /* rays one full trace draws                               */
+    int     max_micro;
+    float   pixel_length;      /* receiver pixel edge                                     */
+    float   flux_scale;        /* DNI * Ssub * rho / Nc / Srec (Gtest.cu:37-46)           */
+    float   total_energy;      /* sum |n.s| DNI A rho / Srec (Gtest.cu:29,44)             */
+} om_scene_info;
+
+/* Host-side result of the full driver. */
+typedef struct {
+    float *flux;               /* host [res_y*res_x], may be NULL                         */
+    float *flux_smooth;        /* host [res_y*res_x], may be NULL                         */
+    float  ratio;              /* intercept ratio, unsmoothed (Gtest.cu:47)               */
+    float  ratio_smooth;       /* intercept ratio, smoothed (Gtest.cu:59)                 */
+    float  total_energy;
+    int64_t rays;
+} om_result;
+
+const char *om_status_string(om_status s);
+const char *om_last_error(void);
+void        om_default_params(om_params *p);
+
+/* Context on one GPU. (Replaces the process-global cuRAND generator set in main.cpp:51-54
+ * and the implicit current device.) */
+om_status om_ctx_create(int device, om_ctx **out);
+om_status om_ctx_destroy(om_ctx *ctx);
+
+/* Scene from a .scn file + SolarScene::InitContent: SceneFileProc::SceneFileRead
+ * (scene_file_proc.cpp:41) then sun, grid, receiver, heliostat aiming
+ * (solar_scene.cpp:83-99).  sun_dir and dni are the solarenergy globals. */
+om_status om_scene_from_scn(om_ctx *ctx, const char *path, const float sun_dir[3], float dni,
+                            const om_params *params, om_scene **out);
+
+/* Parse a .scn file into a description whose arrays live until om_scene_desc_free. */
+om_status om_scn_parse(const char *path, om_scene_desc *desc);
+void      om_scene_desc_free(om_scene_desc *desc);
+
+/* Scene from arrays (generated fields) + InitContent (solar_scene.cpp:83-99). */
+om_status om_scene_create(om_ctx *ctx, const om_scene_desc *desc, const om_params *params,
+                          om_scene **out);
+om_status om_scene_destroy(om_scene *scene);
+om_status om_scene_get_info(const om_scene *scene, om_scene_info *info);
+
+/* New sun position: renormalise the sun and re-aim every heliostat at the receiver focus
+ * (set_sunray_content + set_helio_content, scene_instance_process.cpp:21-28). */
+om_status om_scene_set_sun(om_scene *scene, const float sun_dir[3], float dni);
+
+/* Copy scene arrays to host buffers for inspection (parity tests).  what:
+ *  "helio_normal" [n][3] f32   "helio_v0" / "helio_v1" / "helio_v3" [n][3] f32
+ *  "grid_start" [cells+1] i32  "grid_items" [nnz] i32   "receiver" [4][3] f32 + focus[3]
+ *  "buie" cdf_x[b+1], cdf_y[b+1], slope[b] f32         "micro_count" [n] i32
+ *  "ray_offset" [n+1] i64      "poisson" [n][max_micro][2] f32 (zero padded)
+ *  "sun" sun[3] + buie constants int465, A, B, C, 1/(gamma+1)            */
+om_status om_scene_export(const om_scene *scene, const char *what, void *host, size_t bytes);
+
+/* Full trace of the field into caller maps (cleared first), then flux scaling and, when
+ * flux_smooth is set, trimmed-mean smoothing with params.smooth_radius/ratio.  This is the
+ * device part of results_with_given_normal (Gtest.cu:10-63): the per-heliostat loop of
+ * recthelio_ray_tracing (recthelio_tracing.cpp:46-79) -> ray_tracing (ray_tracing.cu:3-32)
+ * -> map_tracing (rectgrid_raytracing.cu:155-196) in one launch. */
+om_status om_trace_plain(om_scene *scene, const om_flux *out, void *stream);
+
+/* Trace heliostats [h0, h1) and accumulate (no clear) into counts/energy: the per-heliostat
+ * entry recthelio_ray_tracing (recthelio_tracing.cpp:46-79; recthelio_tracing.h:16). */
+om_status om_trace_heliostats(om_scene *scene, int h0, int h1, const om_flux *out, void *stream);
+
+/* Per-ray records for parity checks: for every ray of heliostats [h0, h1) (indexed by
+ * ray_offset[h] - ray_offset[h0] + m*Nc + l) the blocking heliostat of the sun ray (shade),
+ * of the reflected ray (block; -2 when not traced) and the receiver pixel (-1 for none).
+ * Device int32 arrays of at least ray_offset[h1]-ray_offset[h0] elements. */
+om_status om_trace_records(om_scene *scene, int h0, int h1, int32_t *shade, int32_t *block,
+                           int32_t *pixel, int64_t capacity, void *stream);
+
+/* Energy/fixed point -> flux maps (the scaling loop of Gtest.cu:37-46) for a scene. */
+om_status om_flux_finalize(om_scene *scene, const om_flux *out, void *stream);
+
+/* ImageSmoother::image_smooth (image_smooth.cuh:20, image_smooth.cu:60-115): trimmed
+ * mean over a (2r+1)^2 clipped window with the two-heap update of heap.cuh.  d_in and
+ * d_out may be the same buffer (the reference works in place). */
+om_status om_smooth_trimmed(om_ctx *ctx, const float *d_in, float *d_out, int width, int height,
+                            int kernel_radius, float trimmed_ratio, void *stream);
+
+/* results_with_given_normal (Gtest.cu:10-63; Gtest.cuh:8): trace, scale, intercept ratio,
+ * smooth, scale, ratio.  Synchronous; host outputs. */
+om_status om_results_with_given_normal(om_scene *scene, om_result *result);
+
+/* Intercept ratio of a host flux map (ImageSaver::savetxt, image_save.cpp:26-47). */
+float om_intercept_ratio(const float *flux, int width, int height, float total_energy);
+
+/* Device self-test of the float math (parity of the kernels' arithmetic with the host):
+ * fn as in the oracle's om_math_batch; in/out device arrays of n floats. */
+om_status om_math_eval(om_ctx *ctx, int fn, const float *d_in, float *d_out, int n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

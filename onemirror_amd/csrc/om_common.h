@@ -1,0 +1,292 @@
+// om_common.h -- single-precision geometry, sampling and RNG shared by the gfx950 kernels
+// and the host-side scene build.  Every function is one fixed IEEE operation sequence:
+// the library is compiled with -ffp-contract=off, divisions and square roots are
+// correctly rounded, and transcendental functions are project-written polynomials, so a
+// value computed here is the same bits on the host and on the GPU.
+//
+// Reference semantics (Hu_smooth/ClassedSunRay):
+//   vector algebra            vector_arithmetic.cuh:341-413 (host rsqrtf = 1/sqrtf, :51-54)
+//   frames and intersection   global_function.cuh:14-105, :175-180
+//   DDA helpers               rectgrid_raytracing.cu:3-17
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define OM_HD __host__ __device__ __forceinline__
+#else
+#define OM_HD inline
+#endif
+
+namespace om {
+
+// ---- the reference's Epsilon is the double 1e-6 (global_constant.h:6); a float x
+// compared against it is promoted to double.  Equivalent float thresholds:
+//   x <  1e-6  <=>  x <  EPS_LT        x <= 1e-6  <=>  x <= EPS_LE
+constexpr float EPS_LT = 0x1.0c6f7cp-20f;  // smallest float above 1e-6
+constexpr float EPS_LE = 0x1.0c6f7ap-20f;  // largest float below 1e-6
+constexpr float INT_MAX_F = 2147483648.0f; // float(INT_MAX), absDivide's sentinel
+
+OM_HD float fabs_(float x) { return __builtin_fabsf(x); }
+OM_HD float sqrt_(float x) { return __builtin_sqrtf(x); }
+OM_HD float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+OM_HD float rint_(float x) { return __builtin_rintf(x); }
+OM_HD float floor_(float x) { return __builtin_floorf(x); }
+OM_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+OM_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+struct v3 {
+    float x, y, z;
+};
+OM_HD v3 mk(float x, float y, float z) { return v3{x, y, z}; }
+OM_HD v3 operator+(v3 a, v3 b) { return v3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+OM_HD v3 operator-(v3 a, v3 b) { return v3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+OM_HD v3 operator-(v3 a) { return v3{-a.x, -a.y, -a.z}; }
+OM_HD v3 operator*(v3 a, float s) { return v3{a.x * s, a.y * s, a.z * s}; }
+OM_HD v3 operator*(float s, v3 a) { return v3{s * a.x, s * a.y, s * a.z}; }
+OM_HD float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+OM_HD v3 cross(v3 a, v3 b)
+{
+    return v3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// normalize with the host rsqrtf: v * (1 / sqrtf(v.v))
+OM_HD v3 normalize(v3 v)
+{
+    float inv = 1.0f / sqrt_(dot(v, v));
+    return v * inv;
+}
+// reflect(i, n) = i - (2 n) dot(n, i)
+OM_HD v3 reflect(v3 i, v3 n)
+{
+    float d = dot(n, i);
+    return i - (2.0f * n) * d;
+}
+
+// float -> int truncation, clamped to +-2^30 so that every platform agrees (NaN -> -2^30)
+OM_HD int trunc_i(float x)
+{
+    if (!(x > -1073741824.0f)) x = -1073741824.0f;
+    if (x > 1073741824.0f) x = 1073741824.0f;
+    return (int)x;
+}
+
+// ---- orthonormal frame of local2world (global_function.cuh:14-52): u = norm((0,1,0) x n),
+// v = norm(u x n); identity when |n.x| and |n.z| are below Epsilon.
+struct frame {
+    v3 u, n, v;
+    int identity;
+};
+OM_HD frame make_frame(v3 n)
+{
+    frame f;
+    f.n = n;
+    f.identity = (fabs_(n.x) < EPS_LT && fabs_(n.z) < EPS_LT) ? 1 : 0;
+    if (f.identity) {
+        f.u = mk(1.0f, 0.0f, 0.0f);
+        f.v = mk(0.0f, 0.0f, 1.0f);
+        return f;
+    }
+    f.u = normalize(cross(mk(0.0f, 1.0f, 0.0f), n));
+    f.v = normalize(cross(f.u, n));
+    return f;
+}
+OM_HD v3 to_world(v3 d, const frame &f)
+{
+    if (f.identity) return d;
+    return v3{d.x * f.u.x + d.y * f.n.x + d.z * f.v.x,
+              d.x * f.u.y + d.y * f.n.y + d.z * f.v.y,
+              d.x * f.u.z + d.y * f.n.z + d.z * f.v.z};
+}
+
+// ---- rotateY (global_function.cuh:61-74)
+OM_HD v3 rotate_y(v3 o, v3 old_dir, v3 new_dir)
+{
+    int dir = (cross(old_dir, new_dir).y > 0) ? 1 : -1;
+    float c = dot(old_dir, new_dir);
+    float s = (float)dir * sqrt_(1.0f - c * c);
+    return v3{c * o.x + s * o.z, o.y, -s * o.x + c * o.z};
+}
+
+// ---- rayParallelogramIntersect (global_function.cuh:78-105) with E1 = B - A and
+// E2 = C - A precomputed by the caller.
+OM_HD bool parallelogram(v3 orig, v3 dir, v3 A, v3 E1, v3 E2, float &t, float &u, float &v)
+{
+    v3 pvec = cross(dir, E2);
+    float det = dot(E1, pvec);
+    if (fabs_(det) < EPS_LT) return false;
+    float invDet = 1.0f / det;
+    v3 T = orig - A;
+    u = dot(T, pvec) * invDet;
+    if (u < 0.0f || u > 1.0f) return false;
+    v3 qvec = cross(T, E1);
+    v = dot(dir, qvec) * invDet;
+    if (v < 0.0f || v > 1.0f) return false;
+    t = dot(E2, qvec) * invDet;
+    if (t < EPS_LT) return false;
+    return true;
+}
+
+// ---- DDA helpers: calTMax and absDivide (rectgrid_raytracing.cu:3-17)
+OM_HD float cal_tmax(float dir, float interval, int idx, float cur)
+{
+    if (dir >= 0.0f) return (float)(idx + 1) * interval - cur;
+    return cur - (float)idx * interval;
+}
+OM_HD float abs_divide(float den, float num)
+{
+    if (num <= EPS_LE && num >= -EPS_LE) return INT_MAX_F;
+    return fabs_(den / num);
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+struct u4 {
+    uint32_t x, y, z, w;
+};
+OM_HD u4 philox(u4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+// counter word 3: scene index << 4 | purpose
+enum : uint32_t { PURPOSE_RAY = 0, PURPOSE_POISSON = 1 };
+
+// uniform on (0,1): (2 (x >> 9) + 1) 2^-24, exact in float
+OM_HD float u01(uint32_t x) { return (float)((x >> 9) * 2u + 1u) * 5.9604644775390625e-08f; }
+
+// sin and cos of 2 pi u: quadrant of t = 4u, remainder in [-1/2, 1/2] (both exact), angle
+// a = r pi/2, Taylor series of sin to a^9 and of cos to a^10.
+OM_HD void sincos2pi(float u, float &s, float &c)
+{
+    float t = u * 4.0f;
+    float q = rint_(t);
+    float r = t - q;
+    float a = r * 1.57079637f;
+    float a2 = a * a;
+    float ps = fma_(a2, 2.75573192e-06f, -1.98412698e-04f);
+    ps = fma_(a2, ps, 8.33333333e-03f);
+    ps = fma_(a2, ps, -1.66666667e-01f);
+    float sn = fma_(a * a2, ps, a);
+    float pc = fma_(a2, -2.75573192e-07f, 2.48015873e-05f);
+    pc = fma_(a2, pc, -1.38888889e-03f);
+    pc = fma_(a2, pc, 4.16666667e-02f);
+    pc = fma_(a2, pc, -0.5f);
+    float cs = fma_(a2, pc, 1.0f);
+    int qi = (int)q & 3;
+    float s0 = (qi & 1) ? cs : sn;
+    float c0 = (qi & 1) ? sn : cs;
+    s = (qi & 2) ? -s0 : s0;
+    c = ((qi + 1) & 2) ? -c0 : c0;
+}
+
+// sin and cos of a small polar angle: Taylor series below 0.25 rad, else sincos2pi
+OM_HD void sincos_small(float x, float &s, float &c)
+{
+    if (fabs_(x) < 0.25f) {
+        float x2 = x * x;
+        float ps = fma_(x2, -1.98412698e-04f, 8.33333333e-03f);
+        ps = fma_(x2, ps, -1.66666667e-01f);
+        s = fma_(x * x2, ps, x);
+        float pc = fma_(x2, 2.48015873e-05f, -1.38888889e-03f);
+        pc = fma_(x2, pc, 4.16666667e-02f);
+        pc = fma_(x2, pc, -0.5f);
+        c = fma_(x2, pc, 1.0f);
+    } else {
+        float u = x * 0.159154943f;
+        u = u - floor_(u);
+        sincos2pi(u, s, c);
+    }
+}
+
+// natural log of a positive normal float: m in [sqrt(1/2), sqrt(2)), 2 atanh((m-1)/(m+1))
+// as an odd series to s^11, plus e ln2 in two parts.
+OM_HD float log_(float x)
+{
+    uint32_t ix = f2u(x);
+    int e = (int)(ix >> 23) - 127;
+    float m = u2f((ix & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421354f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    float f = m - 1.0f;
+    float s = f / (m + 1.0f);
+    float s2 = s * s;
+    float p = fma_(s2, 0.0909090909f, 0.111111111f);
+    p = fma_(s2, p, 0.142857143f);
+    p = fma_(s2, p, 0.2f);
+    p = fma_(s2, p, 0.333333333f);
+    float at = fma_(s * s2, p, s);
+    float lm = at * 2.0f;
+    float fe = (float)e;
+    return fma_(fe, 0.693145752f, fma_(fe, 1.42860677e-06f, lm));
+}
+
+// exp for |x| < 87: 2^k exp(r), r = x - k ln2 (two parts), Taylor to r^8
+OM_HD float exp_(float x)
+{
+    float k = rint_(x * 1.44269504f);
+    float r = fma_(-k, 0.693145752f, x);
+    r = fma_(-k, 1.42860677e-06f, r);
+    float p = fma_(r, 2.48015873e-05f, 1.98412698e-04f);
+    p = fma_(r, p, 1.38888889e-03f);
+    p = fma_(r, p, 8.33333333e-03f);
+    p = fma_(r, p, 4.16666667e-02f);
+    p = fma_(r, p, 1.66666667e-01f);
+    p = fma_(r, p, 0.5f);
+    p = fma_(r, p, 1.0f);
+    p = fma_(r, p, 1.0f);
+    int ki = (int)k;
+    return p * u2f((uint32_t)(ki + 127) << 23);
+}
+
+OM_HD float pow_(float x, float y) { return exp_(y * log_(x)); }
+
+// erfinv in single precision: M. Giles, "Approximating the erfinv function",
+// GPU Computing Gems Jade Edition (2011): polynomial in w = -log(1 - x^2), two branches.
+OM_HD float erfinv_(float x)
+{
+    float w = -log_((1.0f - x) * (1.0f + x));
+    float p;
+    if (w < 5.0f) {
+        w = w - 2.5f;
+        p = 2.81022636e-08f;
+        p = fma_(p, w, 3.43273939e-07f);
+        p = fma_(p, w, -3.5233877e-06f);
+        p = fma_(p, w, -4.39150654e-06f);
+        p = fma_(p, w, 0.00021858087f);
+        p = fma_(p, w, -0.00125372503f);
+        p = fma_(p, w, -0.00417768164f);
+        p = fma_(p, w, 0.246640727f);
+        p = fma_(p, w, 1.50140941f);
+    } else {
+        w = sqrt_(w) - 3.0f;
+        p = -0.000200214257f;
+        p = fma_(p, w, 0.000100950558f);
+        p = fma_(p, w, 0.00134934322f);
+        p = fma_(p, w, -0.00367342844f);
+        p = fma_(p, w, 0.00573950773f);
+        p = fma_(p, w, -0.0076224613f);
+        p = fma_(p, w, 0.00943887047f);
+        p = fma_(p, w, 1.00167406f);
+        p = fma_(p, w, 2.83297682f);
+    }
+    return p * x;
+}
+
+// standard normal deviate: sqrt(2) erfinv(2u - 1)
+OM_HD float gauss_(float u) { return erfinv_(2.0f * u - 1.0f) * 1.41421356f; }
+
+// angle2xyz (global_function.cuh:175-180) from the sines and cosines
+OM_HD v3 angle2xyz(float st, float ct, float sp, float cp) { return v3{st * cp, ct, st * sp}; }
+
+} // namespace om

@@ -1,0 +1,153 @@
+// om_internal.h -- scene layout in HBM and the host/device glue of the tracer library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/onemirror.h"
+#include "om_common.h"
+
+namespace om {
+
+// Per-heliostat record, 32 words.  One workgroup traces rays of one heliostat, so the
+// record is read with uniform (scalar) loads.
+struct alignas(16) HelioRec {
+    float pos[3];
+    float n[3];
+    float fu[3];            // frame of local2world around n (global_function.cuh:41-44)
+    float fv[3];
+    int ident;              // frame is the identity (|n.x|, |n.z| < Epsilon)
+    float sx2, sy2, sz2;    // size / 2
+    float gx, gy;           // facet gaps
+    float subrow_len;       // subhelio_rowlength_collength.x (steps_for_raytracing.cu:75)
+    float subcol_len;       // subhelio_rowlength_collength.y (:76)
+    int sub_rows, sub_cols; // microsamples per facet row / column (:79-80)
+    int cols_all;           // facet columns * sub_cols
+    int mcount;             // microsamples of this heliostat
+    int ray_off_lo, ray_off_hi; // global ray id of (m=0, l=0), for records
+    int rows, cols;         // facet rows, facet columns
+    int pad[4];
+};
+static_assert(sizeof(HelioRec) == 128, "HelioRec is 32 words");
+
+// Kernel arguments of the trace kernels (passed by value).
+struct TraceArgs {
+    // sun (scene_instance_process.cu:6-226)
+    v3 sun_u, sun_n, sun_v;
+    int sun_ident;
+    float pillbox;
+    float buie_int465, buie_A, buie_B, buie_C, buie_inv_gp1;
+    int nbins;
+    const float *cdf_x, *cdf_y, *slope;
+    float slope_sigma;
+    uint32_t key0, key1, w3;
+    int L;
+    float pitch, pitch2;
+    // grid (grid.h:25-49) + occupied index range
+    float gpx, gpy, gpz, gix, giy, giz;
+    int gnx, gny, gnz;
+    int occ_lo[3], occ_hi[3];
+    const int *cell_start;
+    const int *cell_items;
+    const float4 *geo;      // [3H]: (v0, 0), (v1 - v0, 0), (v3 - v0, 0)
+    // receiver (receiver.cuh:79-82)
+    v3 rA, rE1, rE2;
+    int res_x, res_y;
+    // heliostats and work
+    const HelioRec *helio;
+    const float2 *poisson;
+    int poisson_cap;
+    const int2 *tiles;      // (heliostat, first ray of the tile)
+    int tile_base;          // first tile of this launch
+    int ntiles;             // tiles of this launch
+    int rays_per_thread;
+    // outputs
+    uint32_t *counts;
+    unsigned long long *energy;
+    int32_t *rec_shade, *rec_block, *rec_pixel;
+    long long rec_base;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+} // namespace om
+
+struct om_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    om::DevBuf scratch;     // raw maps for finalize/smooth
+    om::DevBuf scratch2;
+};
+
+struct om_scene {
+    om_ctx *ctx = nullptr;
+    om_params prm{};
+    // host copy of the description
+    int H = 0;
+    std::vector<float> pos, size, gap;
+    std::vector<int> matrix;
+    int recv_type = 0;
+    float recv_pos[3], recv_size[3], recv_normal_in[3];
+    int recv_face = 0;
+    float recv_pixel_length = 0.01f;
+    float grid_pos[3], grid_size[3], grid_interval[3];
+    float sun_in[3];
+    float dni = 1000.0f;
+    // derived host state
+    om::v3 sun;
+    std::vector<om::v3> normal, v0, v1, v2, v3;
+    int gn[3] = {0, 0, 0};
+    long long cells = 0, nnz = 0;
+    int occ_lo[3], occ_hi[3];
+    std::vector<int> cell_start, cell_items;
+    om::v3 rv[4], recv_normal, focus;
+    int geo_px = 0, res_x = 0, res_y = 0;
+    float recv_pixel = 0.0f;
+    int nbins = 0;
+    std::vector<float> cdf_x, cdf_y, slope;
+    float buie_int465 = 0, buie_A = 0, buie_B = 0, buie_C = 0, buie_inv_gp1 = 0, buie_gamma = 0;
+    std::vector<float> subrow_len, subcol_len;
+    std::vector<int> sub_rows, sub_cols, mcount;
+    std::vector<long long> ray_offset;
+    long long total_micro = 0, total_rays = 0;
+    int max_micro = 0;
+    int poisson_cap = 0;
+    float scale = 0.0f, total_energy = 0.0f;
+    std::vector<om::HelioRec> recs;
+    // device state
+    om::HelioRec *d_helio = nullptr;
+    float4 *d_geo = nullptr;
+    int *d_cell_start = nullptr, *d_cell_items = nullptr;
+    float *d_cdf_x = nullptr, *d_cdf_y = nullptr, *d_slope = nullptr;
+    float2 *d_poisson = nullptr;
+    int *d_mcount = nullptr;
+    std::vector<int2> tiles;
+    int2 *d_tiles = nullptr;
+    int rays_per_thread = 8;
+};
+
+namespace om {
+void set_error(const std::string &msg);
+om_status hip_fail(hipError_t e, const char *what);
+TraceArgs make_trace_args(const om_scene *s);
+// kernels (om_kernels.hip)
+om_status launch_poisson(om_scene *s, hipStream_t st);
+om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st);
+om_status launch_finalize(const unsigned long long *energy, float *raw, float *flux, float scale,
+                          int n, hipStream_t st);
+om_status launch_smooth(const float *in, float *out, int w, int h, int radius, float ratio,
+                        float scale, hipStream_t st);
+om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st);
+om_status ensure_buf(DevBuf &b, size_t bytes);
+} // namespace om
+
+#define OM_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess) return om::hip_fail(_e, #call);                              \
+    } while (0)

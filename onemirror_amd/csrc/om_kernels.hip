@@ -1,0 +1,701 @@
+// om_kernels.hip -- gfx950 kernels of the heliostat-field ray tracer and the C ABI entry
+// points that launch them.
+//
+//   trace_kernel    map_tracing (rectgrid_raytracing.cu:155-196) for a whole field in one
+//                   launch: Philox sampling of sunshape and slope error, shading DDA,
+//                   reflection, blocking DDA, receiver hit, fixed-point accumulation.
+//   poisson_kernel  poisson_sample + copy_if (poisson_disk.cu:4-201, compact_sample.cu:6-19),
+//                   one workgroup per heliostat, cell grid in LDS.
+//   finalize_kernel fixed point -> FP32 flux (Gtest.cu:37-46).
+//   smooth_kernel   trimmed_mean (image_smooth.cu:10-58) with the heaps of heap.cuh.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "om_internal.h"
+
+namespace om {
+
+constexpr int WG = 256;
+
+// Workgroup b of an n-workgroup launch runs on XCD b % 8.  Give each XCD a contiguous
+// range of tiles (neighbouring heliostats share blockers and grid cells in its L2).
+__device__ __forceinline__ int xcd_chunk(int b, int n)
+{
+    int q = n >> 3, rmd = n & 7;
+    int x = b & 7, i = b >> 3;
+    return x * q + min(x, rmd) + i;
+}
+
+// samplelights::max_less_index (scene_instance_process.cu:158-173)
+__device__ __forceinline__ int max_less_index(const float *cdf_y, float value, int n)
+{
+    int left = 0, right = n;
+    while (left <= right) {
+        int mid = (left + right) >> 1;
+        float c = cdf_y[mid];
+        if (value > c) left = mid + 1;
+        else if (value < c) right = mid - 1;
+        else return mid;
+    }
+    return right;
+}
+
+// NotCollision + Intersect (rectgrid_raytracing.cu:21-117): Amanatides-Woo walk from the
+// cell of the origin (which is never tested), first candidate in CSR order whose
+// parallelogram the ray hits.  The ray's own heliostat is skipped (the reference rejects it
+// through t < 1e-6, which holds in exact arithmetic).  The walk also ends, with the same
+// result, once the cell index on an axis has passed the occupied index range in the
+// direction of travel: every later cell is empty.
+__device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
+{
+    float rx = o.x - a.gpx, ry = o.y - a.gpy, rz = o.z - a.gpz;
+    int gx = trunc_i(rx / a.gix), gy = trunc_i(ry / a.giy), gz = trunc_i(rz / a.giz);
+    int sx = (d.x >= 0.0f) ? 1 : -1, sy = (d.y >= 0.0f) ? 1 : -1, sz = (d.z >= 0.0f) ? 1 : -1;
+    float mx = abs_divide(cal_tmax(d.x, a.gix, gx, rx), d.x);
+    float my = abs_divide(cal_tmax(d.y, a.giy, gy, ry), d.y);
+    float mz = abs_divide(cal_tmax(d.z, a.giz, gz, rz), d.z);
+    float dx = abs_divide(a.gix, d.x), dy = abs_divide(a.giy, d.y), dz = abs_divide(a.giz, d.z);
+    // admissible index window per axis: inside the grid and not past the occupied range
+    int lox = sx > 0 ? 0 : a.occ_lo[0], hix = sx > 0 ? a.occ_hi[0] : a.gnx - 1;
+    int loy = sy > 0 ? 0 : a.occ_lo[1], hiy = sy > 0 ? a.occ_hi[1] : a.gny - 1;
+    int loz = sz > 0 ? 0 : a.occ_lo[2], hiz = sz > 0 ? a.occ_hi[2] : a.gnz - 1;
+    if ((sx > 0 ? gx > hix : gx < lox) || (sy > 0 ? gy > hiy : gy < loy) ||
+        (sz > 0 ? gz > hiz : gz < loz))
+        return -1;
+    for (;;) {
+        if (mx < my) {
+            if (mx < mz) {
+                gx += sx;
+                if (gx < lox || gx > hix) return -1;
+                mx += dx;
+            } else {
+                gz += sz;
+                if (gz < loz || gz > hiz) return -1;
+                mz += dz;
+            }
+        } else {
+            if (my < mz) {
+                gy += sy;
+                if (gy < loy || gy > hiy) return -1;
+                my += dy;
+            } else {
+                gz += sz;
+                if (gz < loz || gz > hiz) return -1;
+                mz += dz;
+            }
+        }
+        int addr = (gx * a.gny + gy) * a.gnz + gz;
+        int i0 = a.cell_start[addr], i1 = a.cell_start[addr + 1];
+        for (int i = i0; i < i1; ++i) {
+            int h = a.cell_items[i];
+            if (h == self) continue;
+            float4 A = a.geo[3 * h], E1 = a.geo[3 * h + 1], E2 = a.geo[3 * h + 2];
+            float t, u, v;
+            if (parallelogram(o, d, mk(A.x, A.y, A.z), mk(E1.x, E1.y, E1.z), mk(E2.x, E2.y, E2.z),
+                              t, u, v))
+                return h;
+        }
+    }
+}
+
+template <int SUN, bool POISSON, bool RECORD>
+__global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
+{
+    extern __shared__ float lds[];
+    float *s_cdf_y = lds;
+    float *s_slope = lds + (a.nbins + 1);
+    float *s_cdf_x = lds + (2 * a.nbins + 1);
+    if (SUN == OM_SUN_BUIE) {
+        for (int i = threadIdx.x; i <= a.nbins; i += WG) {
+            s_cdf_y[i] = a.cdf_y[i];
+            s_cdf_x[i] = a.cdf_x[i];
+            if (i < a.nbins) s_slope[i] = a.slope[i];
+        }
+        __syncthreads();
+    }
+    const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
+    const int2 tl = a.tiles[tile];
+    const int h = tl.x;
+    const HelioRec *hr = a.helio + h;
+    frame hf;
+    hf.n = mk(hr->n[0], hr->n[1], hr->n[2]);
+    hf.u = mk(hr->fu[0], hr->fu[1], hr->fu[2]);
+    hf.v = mk(hr->fv[0], hr->fv[1], hr->fv[2]);
+    hf.identity = hr->ident;
+    frame sf;
+    sf.n = a.sun_n;
+    sf.u = a.sun_u;
+    sf.v = a.sun_v;
+    sf.identity = a.sun_ident;
+    const v3 hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
+    const int L = a.L;
+    const int nr = hr->mcount * L;
+    for (int k = 0; k < a.rays_per_thread; ++k) {
+        const int r = tl.y + k * WG + (int)threadIdx.x;
+        if (r >= nr) break;
+        const int m = r / L;
+        const int l = r - m * L;
+        // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
+        v3 loc;
+        if (POISSON) {
+            float2 pt = a.poisson[(size_t)h * a.poisson_cap + m];
+            int br = trunc_i(pt.y / hr->subrow_len);
+            int bc = trunc_i(pt.x / hr->subcol_len);
+            loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
+        } else {
+            int row = m / hr->cols_all, col = m - row * hr->cols_all;
+            int br = row / hr->sub_rows, bc = col / hr->sub_cols;
+            loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
+                     (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
+        }
+        const v3 orig = to_world(loc, hf) + hpos;
+        // draws: sunshape theta, sunshape phi, slope-error normal deviate, slope-error phi
+        u4 x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3}, a.key0, a.key1);
+        float th;
+        if (SUN == OM_SUN_PILLBOX) {
+            th = a.pillbox * sqrt_(u01(x.x));
+        } else {
+            float v = u01(x.x) * a.buie_A;
+            if (v < a.buie_int465) {
+                int id = max_less_index(s_cdf_y, v, a.nbins);
+                th = (v - s_cdf_y[id]) * s_slope[id] + s_cdf_x[id];
+            } else {
+                th = pow_((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
+            }
+            th = th / 1000.0f;
+        }
+        float st, ct, sp, cp;
+        sincos_small(th, st, ct);
+        sincos2pi(u01(x.y), sp, cp);
+        const v3 to_sun = -to_world(angle2xyz(st, ct, sp, cp), sf);
+        long long rid = 0;
+        if (RECORD) rid = (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
+        int sh = dda(a, orig, to_sun, h);
+        if (sh >= 0) {
+            if (RECORD) {
+                a.rec_shade[rid] = sh;
+                a.rec_block[rid] = -2;
+                a.rec_pixel[rid] = -1;
+            }
+            continue;
+        }
+        float ths = gauss_(u01(x.z)) * a.slope_sigma;
+        sincos_small(ths, st, ct);
+        sincos2pi(u01(x.w), sp, cp);
+        const v3 nrm = normalize(to_world(angle2xyz(st, ct, sp, cp), hf));
+        const v3 refl = normalize(reflect(-to_sun, nrm));
+        int bl = dda(a, orig, refl, h);
+        int px = -1;
+        if (bl < 0) {
+            float t, u, v;
+            if (parallelogram(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) {
+                int row = trunc_i(u * (float)a.res_y), col = trunc_i(v * (float)a.res_x);
+                row = min(row, a.res_y - 1);
+                col = min(col, a.res_x - 1);
+                px = row * a.res_x + col;
+            }
+        }
+        if (RECORD) {
+            a.rec_shade[rid] = -1;
+            a.rec_block[rid] = bl;
+            a.rec_pixel[rid] = px;
+        }
+        if (px >= 0) {
+            float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
+            uint32_t q = (uint32_t)(e * 2147483648.0f);
+            atomicAdd(a.counts + px, 1u);
+            atomicAdd(a.energy + px, (unsigned long long)q);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Poisson disk
+// ---------------------------------------------------------------------------------------
+struct PoissonArgs {
+    const HelioRec *helio;
+    float r, cell;
+    int k;
+    int cap;
+    uint32_t key0, key1, w3;
+    float2 *out;
+    int *count;
+};
+
+__global__ __launch_bounds__(WG) void poisson_kernel(PoissonArgs a)
+{
+    extern __shared__ float2 g[];
+    __shared__ int s_scan[WG];
+    const int h = blockIdx.x;
+    const HelioRec *hr = a.helio + h;
+    const float width = hr->subcol_len * (float)hr->cols;
+    const float height = hr->subrow_len * (float)hr->rows;
+    const float cell = a.cell, r = a.r;
+    const int wi = (int)(__builtin_ceilf(width / cell) + 2.0f);
+    const int hi = (int)(__builtin_ceilf(height / cell) + 2.0f);
+    const int ncell = wi * hi; // <= cap, checked on the host
+    const int pw = (wi % 3 != 0) ? wi / 3 + 1 : wi / 3;
+    const int ph = (hi % 3 != 0) ? hi / 3 + 1 : hi / 3;
+    for (int i = threadIdx.x; i < ncell; i += WG) g[i] = make_float2(-2 * cell, -2 * cell);
+    __syncthreads();
+    // Nine phases; cells of one phase are 3 apart and a candidate only reads cells within
+    // 2 of its own, so updating in place equals reading the pre-phase snapshot.
+    for (int phase = 0; phase < 9; ++phase) {
+        const int xo = phase % 3, yo = phase / 3;
+        for (int idx = threadIdx.x; idx < pw * ph; idx += WG) {
+            const int tx = idx % pw, ty = idx / pw;
+            const int my_indx = ty * wi / 3 + tx;
+            const int x = tx * 3 + xo, y = ty * 3 + yo;
+            if (x > wi - 1 || y > hi - 1) continue;
+            if (phase == 0) {
+                u4 o = philox(u4{(uint32_t)my_indx, 0u, (uint32_t)h, a.w3}, a.key0, a.key1);
+                g[y * wi + x] = make_float2((u01(o.x) + (float)x - 1.0f) * cell,
+                                            (u01(o.y) + (float)y - 1.0f) * cell);
+                continue;
+            }
+            for (int i = 0; i < a.k; ++i) {
+                u4 o = philox(u4{(uint32_t)(my_indx * a.k + i), (uint32_t)phase, (uint32_t)h, a.w3},
+                              a.key0, a.key1);
+                const float px = (u01(o.x) + (float)x - 1.0f) * cell;
+                const float py = (u01(o.y) + (float)y - 1.0f) * cell;
+                const int c0 = trunc_i((px - r) / cell + 1.0f), c1 = trunc_i((px + r) / cell + 1.0f);
+                const int r0 = trunc_i((py - r) / cell + 1.0f), r1 = trunc_i((py + r) / cell + 1.0f);
+                bool cannot = false;
+                for (int rr = r0; rr <= r1; ++rr)
+                    for (int cc = c0; cc <= c1; ++cc) {
+                        if (cc < 0 || cc > wi - 1 || rr < 0 || rr > hi - 1) continue;
+                        float2 vv = g[rr * wi + cc];
+                        if (vv.x < -cell) continue;
+                        float ddx = vv.x - px, ddy = vv.y - py;
+                        if (sqrt_(ddx * ddx + ddy * ddy) < r) cannot = true;
+                    }
+                if (!cannot) {
+                    g[y * wi + x] = make_float2(px, py);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // stable compaction in row-major cell order (valid_element, compact_sample.cuh:13-22)
+    const int chunk = (ncell + WG - 1) / WG;
+    const int b0 = threadIdx.x * chunk, b1 = min(b0 + chunk, ncell);
+    int cnt = 0;
+    for (int i = b0; i < b1; ++i) {
+        float2 p = g[i];
+        cnt += (p.x > 0.0f && p.y > 0.0f && p.x < width && p.y < height) ? 1 : 0;
+    }
+    s_scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < WG; off <<= 1) {
+        int v = (threadIdx.x >= (unsigned)off) ? s_scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int w = s_scan[threadIdx.x] - cnt;
+    float2 *dst = a.out + (size_t)h * a.cap;
+    for (int i = b0; i < b1; ++i) {
+        float2 p = g[i];
+        if (p.x > 0.0f && p.y > 0.0f && p.x < width && p.y < height) dst[w++] = p;
+    }
+    if (threadIdx.x == WG - 1) a.count[h] = s_scan[WG - 1];
+}
+
+// ---------------------------------------------------------------------------------------
+// finalize, fill, smoothing, math self-test
+// ---------------------------------------------------------------------------------------
+__global__ void fill_u32_kernel(uint32_t *p, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+__global__ void fill_u64_kernel(unsigned long long *p, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// fixed point -> float: exact u64 -> double, exact 2^-31 scale, one rounding to float
+__global__ void finalize_kernel(const unsigned long long *energy, float *raw, float *flux, float scale,
+                                int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = (float)((double)energy[i] * 4.656612873077392578125e-10);
+    if (raw) raw[i] = v;
+    if (flux) flux[i] = v * scale;
+}
+
+// trimmed_mean (image_smooth.cu:10-58).  Each thread keeps its max- and min-heap
+// (heap.cuh:9-212, 1-based arrays) in LDS; kcap = (2r+1)^2 ratio + 1 slots each.
+__global__ void smooth_kernel(const float *in, float *out, int width, int height, int radius,
+                              float ratio, int kcap, float scale)
+{
+    extern __shared__ float heaps[];
+    const int tid = threadIdx.x;
+    float *mxh = heaps + (size_t)tid * 2 * (kcap + 1);
+    float *mnh = mxh + (kcap + 1);
+    const int idx = blockIdx.x * blockDim.x + tid;
+    if (idx >= width * height) return;
+    const int x = idx % width, y = idx / width;
+    const int lx = (x - radius >= 0) ? x - radius : 0;
+    const int ly = (y - radius >= 0) ? y - radius : 0;
+    const int rx = (x + radius < width) ? x + radius : width - 1;
+    const int ry = (y + radius < height) ? y + radius : height - 1;
+    const int cnt = (rx - lx + 1) * (ry - ly + 1);
+    const int k = (int)((float)cnt * ratio);
+    int nmax = 1, nmin = 1;
+    const int full = k + 1;
+    float value = 0.0f;
+    for (int r = ly; r <= ry; ++r)
+        for (int c = lx; c <= rx; ++c) {
+            const float v = in[r * width + c];
+            value += v;
+            if (k == 0) continue;
+            if (nmax != full) {
+                // insert into both heaps (Heap::insert)
+                int i = nmax++;
+                for (; i != 1 && v > mxh[i / 2]; i /= 2) mxh[i] = mxh[i / 2];
+                mxh[i] = v;
+                i = nmin++;
+                for (; i != 1 && v < mnh[i / 2]; i /= 2) mnh[i] = mnh[i / 2];
+                mnh[i] = v;
+            } else if (v < mxh[1]) {
+                // Heap::delete_and_insert on the max-heap
+                int i = 1;
+                while (2 * i < nmax) {
+                    int cid = (2 * i + 1 >= nmax || mxh[2 * i] > mxh[2 * i + 1]) ? 2 * i : 2 * i + 1;
+                    if (mxh[cid] > v) {
+                        mxh[i] = mxh[cid];
+                        i = cid;
+                    } else {
+                        break;
+                    }
+                }
+                mxh[i] = v;
+            } else if (v > mnh[1]) {
+                int i = 1;
+                while (2 * i < nmin) {
+                    int cid = (2 * i + 1 >= nmin || mnh[2 * i] < mnh[2 * i + 1]) ? 2 * i : 2 * i + 1;
+                    if (mnh[cid] < v) {
+                        mnh[i] = mnh[cid];
+                        i = cid;
+                    } else {
+                        break;
+                    }
+                }
+                mnh[i] = v;
+            }
+        }
+    float smax = 0.0f, smin = 0.0f;
+    for (int i = 1; i < nmax; ++i) smax += mxh[i];
+    for (int i = 1; i < nmin; ++i) smin += mnh[i];
+    out[idx] = ((value - smax - smin) / (float)(cnt - 2 * k)) * scale;
+}
+
+__global__ void math_kernel(int fn, const float *in, float *out, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = in[i], s, c;
+    switch (fn) {
+    case 0: out[i] = u01(f2u(x)); break;
+    case 1: sincos2pi(x, s, c); out[i] = s; break;
+    case 2: sincos2pi(x, s, c); out[i] = c; break;
+    case 3: sincos_small(x, s, c); out[i] = s; break;
+    case 4: sincos_small(x, s, c); out[i] = c; break;
+    case 5: out[i] = log_(x); break;
+    case 6: out[i] = exp_(x); break;
+    case 7: out[i] = erfinv_(x); break;
+    case 8: out[i] = gauss_(x); break;
+    default: out[i] = sqrt_(x); break;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------------------
+om_status launch_poisson(om_scene *s, hipStream_t st)
+{
+    if (s->H <= 0) return OM_OK;
+    size_t lds = sizeof(float2) * (size_t)s->poisson_cap;
+    if (lds > 96 * 1024) {
+        set_error("Poisson cell grid exceeds LDS (pitch too small for the heliostat)");
+        return OM_ERR_UNSUPPORTED;
+    }
+    PoissonArgs a;
+    a.helio = s->d_helio;
+    a.r = s->prm.helio_pixel_length / sqrt_(1.5f);
+    a.cell = a.r / sqrt_(2.0f);
+    a.k = s->prm.poisson_k;
+    a.cap = s->poisson_cap;
+    a.key0 = (uint32_t)s->prm.seed;
+    a.key1 = (uint32_t)(s->prm.seed >> 32);
+    a.w3 = (s->prm.scene_index << 4) | PURPOSE_POISSON;
+    a.out = s->d_poisson;
+    a.count = s->d_mcount;
+    hipLaunchKernelGGL(poisson_kernel, dim3(s->H), dim3(WG), lds, st, a);
+    OM_HIP(hipGetLastError());
+    OM_HIP(hipStreamSynchronize(st));
+    return OM_OK;
+}
+
+om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st)
+{
+    if (a.ntiles <= 0) return OM_OK;
+    size_t lds = (s->prm.sunshape == OM_SUN_BUIE) ? sizeof(float) * (3 * (size_t)a.nbins + 2) : 0;
+    dim3 grid(a.ntiles), block(WG);
+    bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
+#define OM_TRACE_LAUNCH(SUN, PO, REC) hipLaunchKernelGGL((trace_kernel<SUN, PO, REC>), grid, block, lds, st, a)
+    if (s->prm.sunshape == OM_SUN_BUIE) {
+        if (poisson) {
+            if (records) OM_TRACE_LAUNCH(OM_SUN_BUIE, true, true);
+            else OM_TRACE_LAUNCH(OM_SUN_BUIE, true, false);
+        } else {
+            if (records) OM_TRACE_LAUNCH(OM_SUN_BUIE, false, true);
+            else OM_TRACE_LAUNCH(OM_SUN_BUIE, false, false);
+        }
+    } else {
+        if (poisson) {
+            if (records) OM_TRACE_LAUNCH(OM_SUN_PILLBOX, true, true);
+            else OM_TRACE_LAUNCH(OM_SUN_PILLBOX, true, false);
+        } else {
+            if (records) OM_TRACE_LAUNCH(OM_SUN_PILLBOX, false, true);
+            else OM_TRACE_LAUNCH(OM_SUN_PILLBOX, false, false);
+        }
+    }
+#undef OM_TRACE_LAUNCH
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+om_status launch_finalize(const unsigned long long *energy, float *raw, float *flux, float scale, int n,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(finalize_kernel, dim3((n + WG - 1) / WG), dim3(WG), 0, st, energy, raw, flux,
+                       scale, n);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+om_status launch_smooth(const float *in, float *out, int w, int h, int radius, float ratio, float scale,
+                        hipStream_t st)
+{
+    int kcap = (int)((float)((2 * radius + 1) * (2 * radius + 1)) * ratio + 1);
+    if (kcap < 1) kcap = 1;
+    int threads = WG;
+    while (threads > 64 && sizeof(float) * 2 * (size_t)(kcap + 1) * threads > 64 * 1024) threads /= 2;
+    size_t lds = sizeof(float) * 2 * (size_t)(kcap + 1) * threads;
+    if (lds > 64 * 1024) {
+        set_error("trimmed-mean window too large");
+        return OM_ERR_UNSUPPORTED;
+    }
+    int n = w * h;
+    hipLaunchKernelGGL(smooth_kernel, dim3((n + threads - 1) / threads), dim3(threads), lds, st, in, out,
+                       w, h, radius, ratio, kcap, scale);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(math_kernel, dim3((n + WG - 1) / WG), dim3(WG), 0, st, fn, in, out, n);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+static om_status clear_maps(const om_flux *out, long long n, hipStream_t st)
+{
+    dim3 g((unsigned)((n + WG - 1) / WG)), b(WG);
+    hipLaunchKernelGGL(fill_u32_kernel, g, b, 0, st, out->counts, n);
+    hipLaunchKernelGGL(fill_u64_kernel, g, b, 0, st, out->energy, n);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+static om_status check_flux(const om_scene *s, const om_flux *out)
+{
+    if (!out || !out->counts || !out->energy) {
+        set_error("flux output needs counts and energy");
+        return OM_ERR_INVALID;
+    }
+    if (out->capacity < (int64_t)s->res_x * s->res_y) {
+        set_error("flux buffers smaller than the receiver map");
+        return OM_ERR_SIZE;
+    }
+    return OM_OK;
+}
+
+// tiles of heliostats [h0, h1) are contiguous in the tile table
+static void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1)
+{
+    auto lo = std::lower_bound(s->tiles.begin(), s->tiles.end(), h0,
+                               [](const int2 &t, int h) { return t.x < h; });
+    auto hi = std::lower_bound(s->tiles.begin(), s->tiles.end(), h1,
+                               [](const int2 &t, int h) { return t.x < h; });
+    t0 = (int)(lo - s->tiles.begin());
+    t1 = (int)(hi - s->tiles.begin());
+}
+
+} // namespace om
+
+using namespace om;
+
+extern "C" om_status om_trace_heliostats(om_scene *s, int h0, int h1, const om_flux *out, void *stream)
+{
+    if (!s) return OM_ERR_INVALID;
+    om_status st = check_flux(s, out);
+    if (st != OM_OK) return st;
+    if (h0 < 0 || h1 > s->H || h0 > h1) {
+        set_error("heliostat range out of bounds");
+        return OM_ERR_INVALID;
+    }
+    OM_HIP(hipSetDevice(s->ctx->device));
+    TraceArgs a = make_trace_args(s);
+    int t0, t1;
+    tile_range(s, h0, h1, t0, t1);
+    a.tile_base = t0;
+    a.ntiles = t1 - t0;
+    a.counts = out->counts;
+    a.energy = (unsigned long long *)out->energy;
+    return launch_trace(s, a, false, (hipStream_t)stream);
+}
+
+extern "C" om_status om_flux_finalize(om_scene *s, const om_flux *out, void *stream)
+{
+    if (!s) return OM_ERR_INVALID;
+    om_status st = check_flux(s, out);
+    if (st != OM_OK) return st;
+    OM_HIP(hipSetDevice(s->ctx->device));
+    int n = s->res_x * s->res_y;
+    hipStream_t hs = (hipStream_t)stream;
+    float *raw = nullptr;
+    if (out->flux_smooth) {
+        if ((st = ensure_buf(s->ctx->scratch, sizeof(float) * (size_t)n)) != OM_OK) return st;
+        raw = (float *)s->ctx->scratch.p;
+    }
+    if ((st = launch_finalize((const unsigned long long *)out->energy, raw, out->flux, s->scale, n, hs)) != OM_OK)
+        return st;
+    if (out->flux_smooth)
+        return launch_smooth(raw, out->flux_smooth, s->res_x, s->res_y, s->prm.smooth_radius,
+                             s->prm.smooth_ratio, s->scale, hs);
+    return OM_OK;
+}
+
+extern "C" om_status om_trace_plain(om_scene *s, const om_flux *out, void *stream)
+{
+    if (!s) return OM_ERR_INVALID;
+    om_status st = check_flux(s, out);
+    if (st != OM_OK) return st;
+    OM_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t hs = (hipStream_t)stream;
+    if ((st = clear_maps(out, (long long)s->res_x * s->res_y, hs)) != OM_OK) return st;
+    TraceArgs a = make_trace_args(s);
+    a.tile_base = 0;
+    a.ntiles = (int)s->tiles.size();
+    a.counts = out->counts;
+    a.energy = (unsigned long long *)out->energy;
+    if ((st = launch_trace(s, a, false, hs)) != OM_OK) return st;
+    if (out->flux || out->flux_smooth) return om_flux_finalize(s, out, stream);
+    return OM_OK;
+}
+
+extern "C" om_status om_trace_records(om_scene *s, int h0, int h1, int32_t *shade, int32_t *block,
+                                      int32_t *pixel, int64_t capacity, void *stream)
+{
+    if (!s || !shade || !block || !pixel) return OM_ERR_INVALID;
+    if (h0 < 0 || h1 > s->H || h0 > h1) {
+        set_error("heliostat range out of bounds");
+        return OM_ERR_INVALID;
+    }
+    long long need = s->ray_offset[h1] - s->ray_offset[h0];
+    if (capacity < need) {
+        set_error("record buffers too small");
+        return OM_ERR_SIZE;
+    }
+    OM_HIP(hipSetDevice(s->ctx->device));
+    TraceArgs a = make_trace_args(s);
+    int t0, t1;
+    tile_range(s, h0, h1, t0, t1);
+    a.tile_base = t0;
+    a.ntiles = t1 - t0;
+    // records never touch the maps; any valid pointers do
+    if (om_status st = ensure_buf(s->ctx->scratch2, 16 * (size_t)s->res_x * s->res_y); st != OM_OK) return st;
+    a.counts = (uint32_t *)s->ctx->scratch2.p;
+    a.energy = (unsigned long long *)((char *)s->ctx->scratch2.p + 8 * (size_t)s->res_x * s->res_y);
+    a.rec_shade = shade;
+    a.rec_block = block;
+    a.rec_pixel = pixel;
+    a.rec_base = s->ray_offset[h0];
+    return launch_trace(s, a, true, (hipStream_t)stream);
+}
+
+extern "C" om_status om_smooth_trimmed(om_ctx *ctx, const float *d_in, float *d_out, int width, int height,
+                                       int kernel_radius, float trimmed_ratio, void *stream)
+{
+    if (!ctx || !d_in || !d_out || width <= 0 || height <= 0 || kernel_radius < 0 ||
+        !(trimmed_ratio >= 0.0f) || trimmed_ratio >= 0.5f)
+        return OM_ERR_INVALID;
+    OM_HIP(hipSetDevice(ctx->device));
+    hipStream_t hs = (hipStream_t)stream;
+    const float *src = d_in;
+    if (d_in == d_out) {
+        // the reference reads a texture copy and writes the array in place
+        om_status st = ensure_buf(ctx->scratch2, sizeof(float) * (size_t)width * height);
+        if (st != OM_OK) return st;
+        OM_HIP(hipMemcpyAsync(ctx->scratch2.p, d_in, sizeof(float) * (size_t)width * height,
+                              hipMemcpyDeviceToDevice, hs));
+        src = (const float *)ctx->scratch2.p;
+    }
+    return launch_smooth(src, d_out, width, height, kernel_radius, trimmed_ratio, 1.0f, hs);
+}
+
+extern "C" om_status om_math_eval(om_ctx *ctx, int fn, const float *d_in, float *d_out, int n, void *stream)
+{
+    if (!ctx || !d_in || !d_out || n < 0) return OM_ERR_INVALID;
+    if (n == 0) return OM_OK;
+    OM_HIP(hipSetDevice(ctx->device));
+    return launch_math(fn, d_in, d_out, n, (hipStream_t)stream);
+}
+
+extern "C" om_status om_results_with_given_normal(om_scene *s, om_result *res)
+{
+    if (!s || !res) return OM_ERR_INVALID;
+    OM_HIP(hipSetDevice(s->ctx->device));
+    size_t n = (size_t)s->res_x * s->res_y;
+    uint32_t *cnt = nullptr;
+    uint64_t *en = nullptr;
+    float *fl = nullptr, *fs = nullptr;
+    om_status st = OM_OK;
+    hipError_t e = hipMalloc((void **)&cnt, 4 * n);
+    if (e == hipSuccess) e = hipMalloc((void **)&en, 8 * n);
+    if (e == hipSuccess) e = hipMalloc((void **)&fl, 4 * n);
+    if (e == hipSuccess) e = hipMalloc((void **)&fs, 4 * n);
+    if (e != hipSuccess) st = hip_fail(e, "hipMalloc");
+    std::vector<float> hf(n), hs(n);
+    if (st == OM_OK) {
+        om_flux f{cnt, en, fl, fs, (int64_t)n};
+        st = om_trace_plain(s, &f, nullptr);
+    }
+    if (st == OM_OK) {
+        e = hipMemcpy(hf.data(), fl, 4 * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hs.data(), fs, 4 * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "hipMemcpy");
+    }
+    if (cnt) (void)hipFree(cnt);
+    if (en) (void)hipFree(en);
+    if (fl) (void)hipFree(fl);
+    if (fs) (void)hipFree(fs);
+    if (st != OM_OK) return st;
+    res->ratio = om_intercept_ratio(hf.data(), s->res_x, s->res_y, s->total_energy);
+    res->ratio_smooth = om_intercept_ratio(hs.data(), s->res_x, s->res_y, s->total_energy);
+    res->total_energy = s->total_energy;
+    res->rays = s->total_rays;
+    if (res->flux) std::memcpy(res->flux, hf.data(), 4 * n);
+    if (res->flux_smooth) std::memcpy(res->flux_smooth, hs.data(), 4 * n);
+    return OM_OK;
+}

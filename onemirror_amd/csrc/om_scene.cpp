@@ -1,0 +1,978 @@
+// om_scene.cpp -- host side of the tracer library: .scn parsing, scene construction
+// (SolarScene::InitContent), upload of the scene to HBM, and the C ABI.
+//
+// Reference: Hu_smooth/ClassedSunRay (file:line cited per function).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+
+#include "om_internal.h"
+
+using om::v3;
+
+namespace {
+thread_local std::string g_last_error;
+
+v3 ld3(const float *p) { return om::mk(p[0], p[1], p[2]); }
+
+template <class T>
+om_status upload(T *&dptr, const T *host, size_t n)
+{
+    if (dptr) {
+        (void)hipFree(dptr);
+        dptr = nullptr;
+    }
+    size_t bytes = sizeof(T) * (n > 0 ? n : 1);
+    OM_HIP(hipMalloc((void **)&dptr, bytes));
+    if (n > 0) OM_HIP(hipMemcpy(dptr, host, sizeof(T) * n, hipMemcpyHostToDevice));
+    return OM_OK;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+} // namespace
+
+namespace om {
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+om_status hip_fail(hipError_t e, const char *what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return (e == hipErrorOutOfMemory) ? OM_ERR_NOMEM : OM_ERR_HIP;
+}
+
+om_status ensure_buf(DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.p) return OM_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    OM_HIP(hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+    return OM_OK;
+}
+} // namespace om
+
+extern "C" const char *om_status_string(om_status s)
+{
+    switch (s) {
+    case OM_OK: return "ok";
+    case OM_ERR_INVALID: return "invalid argument";
+    case OM_ERR_IO: return "i/o error";
+    case OM_ERR_PARSE: return "scene file parse error";
+    case OM_ERR_HIP: return "HIP runtime error";
+    case OM_ERR_NOMEM: return "out of memory";
+    case OM_ERR_UNSUPPORTED: return "unsupported configuration";
+    case OM_ERR_SIZE: return "buffer too small";
+    }
+    return "unknown status";
+}
+
+extern "C" const char *om_last_error(void) { return g_last_error.c_str(); }
+
+// Defaults of common_var.cpp:9-27 with the reference's lattice microsamples and the
+// Buie sun (Nc = 2048 lights per group, CSR 0.1, slope error 1 mrad, pitch 0.01 m).
+extern "C" void om_default_params(om_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->seed = 0x5EEDULL;
+    p->scene_index = 0;
+    p->sunshape = OM_SUN_BUIE;
+    p->pillbox_half_angle = 4.65e-3f;
+    p->csr = 0.1f;
+    p->buie_bins = 1024;
+    p->slope_sigma = 0.001f;
+    p->micro_mode = OM_MICRO_LATTICE;
+    p->helio_pixel_length = 0.01f;
+    p->poisson_k = 8;
+    p->rays_per_sample = 2048;
+    p->reflectivity = 0.88f;
+    p->smooth_radius = 5;
+    p->smooth_ratio = 0.02f;
+}
+
+extern "C" om_status om_ctx_create(int device, om_ctx **out)
+{
+    if (!out) return OM_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    OM_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        om::set_error("no such device");
+        return OM_ERR_INVALID;
+    }
+    OM_HIP(hipSetDevice(device));
+    om_ctx *c = new om_ctx();
+    c->device = device;
+    *out = c;
+    return OM_OK;
+}
+
+extern "C" om_status om_ctx_destroy(om_ctx *ctx)
+{
+    if (!ctx) return OM_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->scratch.p) (void)hipFree(ctx->scratch.p);
+    if (ctx->scratch2.p) (void)hipFree(ctx->scratch2.p);
+    delete ctx;
+    return OM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// .scn parser: SceneFileProc::SceneFileRead(SolarScene*, std::string)
+// (scene_file_proc.cpp:41-286) with its grammar and its quirks:
+//  * lines whose first character is '#' and empty lines are skipped; a line holding only
+//    blanks is not empty and reads as the key "" ;
+//  * keys unknown to the block map to "pos" (std::map::operator[] default, :155,187,232);
+//  * a failed numeric read stores 0 (std::istream semantics);
+//  * the size of a heliostat is read from the line after its "helio" line (:3-11).
+// One receiver and one grid are supported (the reference traces receivers[0] and grid0s[0]).
+// ---------------------------------------------------------------------------------------
+namespace {
+enum Key { K_POS = 0, K_SIZE, K_NORM, K_FACE, K_END, K_GAP, K_MATRIX, K_HELIO, K_INTER, K_N, K_TYPE };
+enum Mode { M_NONE, M_GROUND, M_RECV, M_GRID, M_HELIO };
+
+struct Parsed {
+    std::vector<float> pos, size, gap;
+    std::vector<int> matrix;
+    int recv_type = 0;
+    float recv_pos[3] = {0, 0, 0}, recv_size[3] = {0, 0, 0}, recv_norm[3] = {0, 0, 1};
+    int recv_face = 0;
+    int n_recv = 0;
+    float grid_pos[3] = {0, 0, 0}, grid_size[3] = {0, 0, 0}, grid_inter[3] = {1, 1, 1};
+    int n_grid = 0, grid_n = 0;
+};
+
+om_status parse_scn(const char *path, Parsed &P)
+{
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        om::set_error(std::string("cannot open scene file ") + path);
+        return OM_ERR_IO;
+    }
+    std::stringstream scene;
+    scene << f.rdbuf();
+    std::map<std::string, int> keys = {{"pos", K_POS}, {"size", K_SIZE}, {"norm", K_NORM},
+                                       {"face", K_FACE}, {"end", K_END}, {"gap", K_GAP},
+                                       {"matrix", K_MATRIX}, {"helio", K_HELIO},
+                                       {"inter", K_INTER}, {"n", K_N}, {"type", K_TYPE}};
+    Mode mode = M_NONE;
+    bool gap_set = false, matrix_set = false, in_recv = false, in_grid = false;
+    float gap[2] = {0, 0};
+    int mat[2] = {1, 1};
+    int grid_type = 0;
+    std::string line;
+    while (std::getline(scene, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::stringstream ls;
+        ls << line;
+        std::string head;
+        ls >> head;
+        if (head == "ground") {
+            mode = M_GROUND;
+            continue;
+        } else if (head == "Recv") {
+            mode = M_RECV;
+            int t = 0;
+            ls >> t;
+            if (P.n_recv == 0) {
+                P.recv_type = t;
+                in_recv = true;
+            } else {
+                in_recv = false;
+            }
+            continue;
+        } else if (head == "Grid") {
+            mode = M_GRID;
+            ls >> grid_type;
+            in_grid = (P.n_grid == 0);
+            gap_set = matrix_set = false;
+            if (P.n_grid >= 1) {
+                om::set_error("scene files with more than one grid are not supported");
+                return OM_ERR_UNSUPPORTED;
+            }
+            continue;
+        }
+        int key = keys[head]; // unknown keys read as "pos"
+        switch (mode) {
+        case M_NONE:
+        case M_GROUND:
+            break;
+        case M_RECV:
+            if (!in_recv) {
+                if (key == K_END) ++P.n_recv;
+                break;
+            }
+            if (key == K_POS) ls >> P.recv_pos[0] >> P.recv_pos[1] >> P.recv_pos[2];
+            else if (key == K_SIZE) ls >> P.recv_size[0] >> P.recv_size[1] >> P.recv_size[2];
+            else if (key == K_NORM) ls >> P.recv_norm[0] >> P.recv_norm[1] >> P.recv_norm[2];
+            else if (key == K_FACE) ls >> P.recv_face;
+            else if (key == K_END) {
+                ++P.n_recv;
+                in_recv = false;
+            }
+            break;
+        case M_GRID:
+            if (grid_type != 0) {
+                om::set_error("grid type not supported");
+                return OM_ERR_UNSUPPORTED;
+            }
+            if (!in_grid) break;
+            if (key == K_POS) ls >> P.grid_pos[0] >> P.grid_pos[1] >> P.grid_pos[2];
+            else if (key == K_SIZE) ls >> P.grid_size[0] >> P.grid_size[1] >> P.grid_size[2];
+            else if (key == K_INTER) ls >> P.grid_inter[0] >> P.grid_inter[1] >> P.grid_inter[2];
+            else if (key == K_N) ls >> P.grid_n;
+            else if (key == K_END) {
+                ++P.n_grid;
+                mode = M_HELIO;
+            }
+            break;
+        case M_HELIO:
+            if (key == K_GAP) {
+                ls >> gap[0] >> gap[1];
+                gap_set = true;
+            } else if (key == K_MATRIX) {
+                ls >> mat[0] >> mat[1];
+                matrix_set = true;
+            } else if (key == K_HELIO) {
+                if (!gap_set || !matrix_set) break; // "did not set the gap and matrix"
+                float p[3] = {0, 0, 0};
+                ls >> p[0] >> p[1] >> p[2];
+                std::string nxt;
+                std::getline(scene, nxt);
+                std::stringstream ns;
+                ns << nxt;
+                float sz[3] = {0, 0, 0};
+                ns >> sz[0] >> sz[1] >> sz[2];
+                P.pos.insert(P.pos.end(), p, p + 3);
+                P.size.insert(P.size.end(), sz, sz + 3);
+                P.gap.insert(P.gap.end(), gap, gap + 2);
+                P.matrix.insert(P.matrix.end(), mat, mat + 2);
+            }
+            break;
+        }
+    }
+    if (P.n_grid == 0) {
+        om::set_error("scene file has no grid");
+        return OM_ERR_PARSE;
+    }
+    if (P.n_recv == 0) {
+        om::set_error("scene file has no receiver");
+        return OM_ERR_PARSE;
+    }
+    return OM_OK;
+}
+} // namespace
+
+extern "C" om_status om_scn_parse(const char *path, om_scene_desc *d)
+{
+    if (!path || !d) return OM_ERR_INVALID;
+    std::memset(d, 0, sizeof(*d));
+    Parsed P;
+    om_status st = parse_scn(path, P);
+    if (st != OM_OK) return st;
+    int H = (int)(P.pos.size() / 3);
+    float *pos = new float[3 * H + 3], *size = new float[3 * H + 3], *gap = new float[2 * H + 2];
+    int *mat = new int[2 * H + 2];
+    std::copy(P.pos.begin(), P.pos.end(), pos);
+    std::copy(P.size.begin(), P.size.end(), size);
+    std::copy(P.gap.begin(), P.gap.end(), gap);
+    std::copy(P.matrix.begin(), P.matrix.end(), mat);
+    d->n_helio = H;
+    d->helio_pos = pos;
+    d->helio_size = size;
+    d->helio_gap = gap;
+    d->helio_matrix = mat;
+    d->recv_type = P.recv_type;
+    std::memcpy(d->recv_pos, P.recv_pos, 12);
+    std::memcpy(d->recv_size, P.recv_size, 12);
+    std::memcpy(d->recv_normal, P.recv_norm, 12);
+    d->recv_face = P.recv_face;
+    d->recv_pixel_length = 0.01f; // solarenergy::receiver_pixel_length (common_var.cpp:12)
+    std::memcpy(d->grid_pos, P.grid_pos, 12);
+    std::memcpy(d->grid_size, P.grid_size, 12);
+    std::memcpy(d->grid_interval, P.grid_inter, 12);
+    d->dni = 1000.0f;
+    return OM_OK;
+}
+
+extern "C" void om_scene_desc_free(om_scene_desc *d)
+{
+    if (!d) return;
+    delete[] d->helio_pos;
+    delete[] d->helio_size;
+    delete[] d->helio_gap;
+    delete[] d->helio_matrix;
+    std::memset(d, 0, sizeof(*d));
+}
+
+// ---------------------------------------------------------------------------------------
+// scene construction
+// ---------------------------------------------------------------------------------------
+namespace {
+
+// set_samplelights host part (scene_instance_process.cu:116-156, :223-260)
+void build_buie(om_scene *s)
+{
+    int num_group = s->prm.buie_bins > 0 ? s->prm.buie_bins : 1024;
+    float csr = s->prm.csr;
+    float upper_lm = 4.65f;
+    float k = 0.9f * logf(13.5f * csr) * powf(csr, -0.3f);
+    float gamma = 2.2f * logf(0.52f * csr) * powf(csr, 0.43f) - 0.1f;
+    float i93 = expf(k) / (gamma + 1) * powf(9.3f, gamma + 1);
+    float i465 = expf(k) / (gamma + 1) * powf(upper_lm, gamma + 1);
+    s->nbins = num_group;
+    s->cdf_x.assign(num_group + 1, 0.0f);
+    s->cdf_y.assign(num_group + 1, 0.0f);
+    s->slope.assign(num_group, 0.0f);
+    float li = upper_lm / (float)num_group;
+    float x = 0.0f;
+    float hp = cosf((float)(0.326 * x)) / cosf((float)(0.308 * x));
+    for (int i = 1; i <= num_group; ++i) {
+        x += li;
+        float hc = cosf((float)(0.326 * x)) / cosf((float)(0.308 * x));
+        float sub = (hp + hc) * li / 2.0f;
+        s->cdf_x[i] = x;
+        s->cdf_y[i] = sub + s->cdf_y[i - 1];
+        s->slope[i - 1] = li / sub;
+        hp = hc;
+    }
+    s->buie_gamma = gamma;
+    s->buie_int465 = s->cdf_y[num_group];
+    s->buie_A = s->buie_int465 + (i93 - i465);
+    s->buie_B = (gamma + 1) / expf(k);
+    s->buie_C = powf(4.65f, gamma + 1);
+    s->buie_inv_gp1 = 1 / (gamma + 1);
+}
+
+// RectGrid::Cinit + CGridHelioMatch (grid.cpp:3-82); binning clamped to the grid.
+om_status build_grid(om_scene *s)
+{
+    v3 gp = ld3(s->grid_pos), gs = ld3(s->grid_size), gi = ld3(s->grid_interval);
+    s->gn[0] = (int)ceilf(gs.x / gi.x);
+    s->gn[1] = (int)ceilf(gs.y / gi.y);
+    s->gn[2] = (int)ceilf(gs.z / gi.z);
+    if (s->gn[0] <= 0 || s->gn[1] <= 0 || s->gn[2] <= 0) {
+        om::set_error("grid has no cells");
+        return OM_ERR_INVALID;
+    }
+    s->cells = (long long)s->gn[0] * s->gn[1] * s->gn[2];
+    if (s->cells >= (1LL << 31) - 1) {
+        om::set_error("grid too large");
+        return OM_ERR_UNSUPPORTED;
+    }
+    std::vector<int> cnt((size_t)s->cells, 0);
+    std::vector<int> rng((size_t)s->H * 6);
+    for (int a = 0; a < 3; ++a) {
+        s->occ_lo[a] = 1 << 30;
+        s->occ_hi[a] = -1;
+    }
+    long long nnz = 0;
+    for (int i = 0; i < s->H; ++i) {
+        v3 sz = ld3(&s->size[3 * i]), p = ld3(&s->pos[3 * i]);
+        float diag = om::sqrt_(om::dot(sz, sz));
+        float radius = om::sqrt_(diag) / 2;
+        v3 mn = om::mk(p.x - radius, p.y - radius, p.z - radius);
+        v3 mx = om::mk(p.x + radius, p.y + radius, p.z + radius);
+        v3 a0 = mn - gp, a1 = mx - gp;
+        int lo[3] = {om::trunc_i(a0.x / gi.x), om::trunc_i(a0.y / gi.y), om::trunc_i(a0.z / gi.z)};
+        int hi[3] = {om::trunc_i(a1.x / gi.x), om::trunc_i(a1.y / gi.y), om::trunc_i(a1.z / gi.z)};
+        for (int a = 0; a < 3; ++a) {
+            if (lo[a] < 0) lo[a] = 0;
+            if (hi[a] > s->gn[a] - 1) hi[a] = s->gn[a] - 1;
+            rng[6 * i + a] = lo[a];
+            rng[6 * i + 3 + a] = hi[a];
+        }
+        bool any = lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+        if (!any) continue;
+        for (int a = 0; a < 3; ++a) {
+            s->occ_lo[a] = std::min(s->occ_lo[a], lo[a]);
+            s->occ_hi[a] = std::max(s->occ_hi[a], hi[a]);
+        }
+        for (int x = lo[0]; x <= hi[0]; ++x)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int z = lo[2]; z <= hi[2]; ++z) {
+                    cnt[(size_t)x * s->gn[1] * s->gn[2] + (size_t)y * s->gn[2] + z]++;
+                    ++nnz;
+                }
+    }
+    if (nnz >= (1LL << 31) - 1) {
+        om::set_error("grid holds too many entries");
+        return OM_ERR_UNSUPPORTED;
+    }
+    s->nnz = nnz;
+    s->cell_start.assign((size_t)s->cells + 1, 0);
+    for (long long c = 0; c < s->cells; ++c) s->cell_start[c + 1] = s->cell_start[c] + cnt[c];
+    s->cell_items.assign((size_t)(nnz > 0 ? nnz : 1), 0);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int i = 0; i < s->H; ++i)
+        for (int x = rng[6 * i]; x <= rng[6 * i + 3]; ++x)
+            for (int y = rng[6 * i + 1]; y <= rng[6 * i + 4]; ++y)
+                for (int z = rng[6 * i + 2]; z <= rng[6 * i + 5]; ++z) {
+                    size_t c = (size_t)x * s->gn[1] * s->gn[2] + (size_t)y * s->gn[2] + z;
+                    s->cell_items[s->cell_start[c] + cnt[c]++] = i;
+                }
+    return OM_OK;
+}
+
+// RectangleReceiver::CInit (receiver.cpp:24-117)
+om_status build_receiver(om_scene *s)
+{
+    if (s->recv_type != 0) {
+        om::set_error("only rectangle receivers (type 0) are supported");
+        return OM_ERR_UNSUPPORTED;
+    }
+    if (!(s->recv_pixel_length > 0.0f)) {
+        om::set_error("receiver pixel length must be positive");
+        return OM_ERR_INVALID;
+    }
+    int geo = (int)(1.0f / s->recv_pixel_length);
+    if (geo <= 0) {
+        om::set_error("receiver pixel length above 1 m");
+        return OM_ERR_UNSUPPORTED;
+    }
+    s->geo_px = geo;
+    s->recv_pixel = 1.0f / (float)geo;
+    v3 sz = ld3(s->recv_size), p = ld3(s->recv_pos);
+    v3 ln, v[4];
+    switch (s->recv_face) {
+    case 0:
+        ln = om::mk(0.0f, 0.0f, 1.0f);
+        v[0] = om::mk(-sz.x / 2, -sz.y / 2, sz.z / 2);
+        v[1] = om::mk(-sz.x / 2, sz.y / 2, sz.z / 2);
+        v[2] = om::mk(sz.x / 2, sz.y / 2, sz.z / 2);
+        v[3] = om::mk(sz.x / 2, -sz.y / 2, sz.z / 2);
+        break;
+    case 1:
+        ln = om::mk(1.0f, 0.0f, 0.0f);
+        v[0] = om::mk(sz.x / 2, -sz.y / 2, sz.z / 2);
+        v[1] = om::mk(sz.x / 2, sz.y / 2, sz.z / 2);
+        v[2] = om::mk(sz.x / 2, sz.y / 2, -sz.z / 2);
+        v[3] = om::mk(sz.x / 2, -sz.y / 2, -sz.z / 2);
+        break;
+    case 2:
+        ln = om::mk(0.0f, 0.0f, -1.0f);
+        v[0] = om::mk(sz.x / 2, -sz.y / 2, -sz.z / 2);
+        v[1] = om::mk(sz.x / 2, sz.y / 2, -sz.z / 2);
+        v[2] = om::mk(-sz.x / 2, sz.y / 2, -sz.z / 2);
+        v[3] = om::mk(-sz.x / 2, -sz.y / 2, -sz.z / 2);
+        break;
+    case 3:
+        ln = om::mk(-1.0f, 0.0f, 0.0f);
+        v[0] = om::mk(-sz.x / 2, -sz.y / 2, -sz.z / 2);
+        v[1] = om::mk(-sz.x / 2, sz.y / 2, -sz.z / 2);
+        v[2] = om::mk(-sz.x / 2, sz.y / 2, sz.z / 2);
+        v[3] = om::mk(-sz.x / 2, -sz.y / 2, sz.z / 2);
+        break;
+    default:
+        om::set_error("receiver face must be 0..3");
+        return OM_ERR_INVALID;
+    }
+    v3 n = om::normalize(ld3(s->recv_normal_in));
+    s->recv_normal = n;
+    for (int i = 0; i < 4; ++i) s->rv[i] = om::rotate_y(v[i], ln, n) + p;
+    s->focus = (s->rv[0] + s->rv[2]) * (1.0f / 2);
+    s->res_x = (int)(sz.x * (float)geo);
+    s->res_y = (int)(sz.y * (float)geo);
+    if (s->res_x <= 0 || s->res_y <= 0) {
+        om::set_error("receiver has no pixels");
+        return OM_ERR_INVALID;
+    }
+    return OM_OK;
+}
+
+// RectangleHelio::CRotate(focus, sun) (heliostat.cpp:16-62)
+void aim_heliostats(om_scene *s)
+{
+    s->normal.resize(s->H);
+    s->v0.resize(s->H);
+    s->v1.resize(s->H);
+    s->v2.resize(s->H);
+    s->v3.resize(s->H);
+    for (int i = 0; i < s->H; ++i) {
+        v3 p = ld3(&s->pos[3 * i]), sz = ld3(&s->size[3 * i]);
+        v3 rd = om::normalize(s->focus - p);
+        v3 n = om::normalize(rd - s->sun);
+        s->normal[i] = n;
+        om::frame f = om::make_frame(n);
+        v3 l0 = om::mk(-sz.x / 2, sz.y / 2, -sz.z / 2);
+        v3 l1 = om::mk(l0.x, l0.y, l0.z + sz.z);
+        v3 l2 = om::mk(l1.x + sz.x, l1.y, l1.z);
+        v3 l3 = om::mk(l0.x + sz.x, l0.y, l0.z);
+        s->v0[i] = om::to_world(l0, f) + p;
+        s->v1[i] = om::to_world(l1, f) + p;
+        s->v2[i] = om::to_world(l2, f) + p;
+        s->v3[i] = om::to_world(l3, f) + p;
+    }
+}
+
+// microsample layout (steps_for_raytracing.cu:67-82, :105-125)
+void micro_layout(om_scene *s)
+{
+    int H = s->H;
+    float pitch = s->prm.helio_pixel_length;
+    s->subrow_len.resize(H);
+    s->subcol_len.resize(H);
+    s->sub_rows.resize(H);
+    s->sub_cols.resize(H);
+    s->mcount.assign(H, 0);
+    s->poisson_cap = 0;
+    float r = pitch / om::sqrt_(1.5f);
+    float cell = r / om::sqrt_(2.0f);
+    for (int i = 0; i < H; ++i) {
+        int rows = s->matrix[2 * i], cols = s->matrix[2 * i + 1];
+        float gx = s->gap[2 * i], gy = s->gap[2 * i + 1];
+        s->subrow_len[i] = (s->size[3 * i + 2] - gy * (float)(rows - 1)) / (float)rows;
+        s->subcol_len[i] = (s->size[3 * i + 0] - gx * (float)(cols - 1)) / (float)cols;
+        s->sub_rows[i] = om::trunc_i(s->subrow_len[i] / pitch);
+        s->sub_cols[i] = om::trunc_i(s->subcol_len[i] / pitch);
+        if (s->prm.micro_mode == OM_MICRO_POISSON) {
+            float width = s->subcol_len[i] * (float)cols;
+            float height = s->subrow_len[i] * (float)rows;
+            int wi = (int)(ceilf(width / cell) + 2), hi = (int)(ceilf(height / cell) + 2);
+            s->poisson_cap = std::max(s->poisson_cap, wi * hi);
+        } else {
+            s->mcount[i] = s->sub_rows[i] * s->sub_cols[i] * rows * cols;
+        }
+    }
+}
+
+void finish_micro(om_scene *s)
+{
+    int H = s->H;
+    int L = s->prm.rays_per_sample;
+    s->ray_offset.assign(H + 1, 0);
+    s->total_micro = 0;
+    s->max_micro = 0;
+    for (int i = 0; i < H; ++i) {
+        s->ray_offset[i + 1] = s->ray_offset[i] + (long long)s->mcount[i] * L;
+        s->total_micro += s->mcount[i];
+        s->max_micro = std::max(s->max_micro, s->mcount[i]);
+    }
+    s->total_rays = s->ray_offset[H];
+}
+
+// results_with_given_normal normalisation (Gtest.cu:14,29,37-46)
+void flux_scale(om_scene *s)
+{
+    float Id = s->dni;
+    float total = 0.0f;
+    for (int i = 0; i < s->H; ++i)
+        total += om::fabs_(om::dot(s->normal[i], s->sun) * Id * s->size[3 * i] * s->size[3 * i + 2]);
+    float Ssub = s->prm.helio_pixel_length * s->prm.helio_pixel_length;
+    float rou = s->prm.reflectivity;
+    int Nc = s->prm.rays_per_sample;
+    float Srec = s->recv_pixel * s->recv_pixel;
+    total *= rou / Srec;
+    s->total_energy = total;
+    s->scale = Id * Ssub * rou / (float)Nc / Srec;
+}
+
+// HelioRec array and blocker geometry (A, E1 = B - A, E2 = C - A of
+// rayParallelogramIntersect with A = v0, B = v1, C = v3: rectgrid_raytracing.cu:29-31)
+void pack_heliostats(om_scene *s)
+{
+    int H = s->H;
+    s->recs.assign(H, om::HelioRec{});
+    for (int i = 0; i < H; ++i) {
+        om::HelioRec &r = s->recs[i];
+        std::memset(&r, 0, sizeof(r));
+        om::frame f = om::make_frame(s->normal[i]);
+        for (int a = 0; a < 3; ++a) r.pos[a] = s->pos[3 * i + a];
+        r.n[0] = f.n.x; r.n[1] = f.n.y; r.n[2] = f.n.z;
+        r.fu[0] = f.u.x; r.fu[1] = f.u.y; r.fu[2] = f.u.z;
+        r.fv[0] = f.v.x; r.fv[1] = f.v.y; r.fv[2] = f.v.z;
+        r.ident = f.identity;
+        r.sx2 = s->size[3 * i] / 2;
+        r.sy2 = s->size[3 * i + 1] / 2;
+        r.sz2 = s->size[3 * i + 2] / 2;
+        r.gx = s->gap[2 * i];
+        r.gy = s->gap[2 * i + 1];
+        r.subrow_len = s->subrow_len[i];
+        r.subcol_len = s->subcol_len[i];
+        r.sub_rows = s->sub_rows[i];
+        r.sub_cols = s->sub_cols[i];
+        r.cols_all = s->matrix[2 * i + 1] * s->sub_cols[i];
+        r.rows = s->matrix[2 * i];
+        r.cols = s->matrix[2 * i + 1];
+        r.mcount = s->mcount[i];
+        long long off = s->ray_offset.empty() ? 0 : s->ray_offset[i];
+        r.ray_off_lo = (int)(off & 0xffffffffLL);
+        r.ray_off_hi = (int)(off >> 32);
+    }
+}
+
+std::vector<float4> pack_geo(const om_scene *s)
+{
+    std::vector<float4> g((size_t)s->H * 3 + 3);
+    for (int i = 0; i < s->H; ++i) {
+        v3 A = s->v0[i], E1 = s->v1[i] - s->v0[i], E2 = s->v3[i] - s->v0[i];
+        g[3 * i + 0] = make_float4(A.x, A.y, A.z, 0.0f);
+        g[3 * i + 1] = make_float4(E1.x, E1.y, E1.z, 0.0f);
+        g[3 * i + 2] = make_float4(E2.x, E2.y, E2.z, 0.0f);
+    }
+    return g;
+}
+
+// Work tiles: (heliostat, first ray) covering rays_per_thread * 256 rays each.
+void build_tiles(om_scene *s)
+{
+    long long per_tile = 256LL * s->rays_per_thread;
+    s->tiles.clear();
+    for (int i = 0; i < s->H; ++i) {
+        long long nr = (long long)s->mcount[i] * s->prm.rays_per_sample;
+        for (long long r = 0; r < nr; r += per_tile) s->tiles.push_back(make_int2(i, (int)r));
+    }
+}
+
+om_status upload_geometry(om_scene *s)
+{
+    pack_heliostats(s);
+    om_status st = upload(s->d_helio, s->recs.data(), s->recs.size());
+    if (st != OM_OK) return st;
+    std::vector<float4> g = pack_geo(s);
+    return upload(s->d_geo, g.data(), g.size());
+}
+
+om_status choose_tiles(om_scene *s)
+{
+    // enough workgroups to fill 256 CUs several times over, tiles of at least 256 rays
+    long long total = s->total_rays;
+    int rpt = 16;
+    while (rpt > 1 && total / (256LL * rpt) < 8192) rpt /= 2;
+    s->rays_per_thread = rpt;
+    build_tiles(s);
+    return upload(s->d_tiles, s->tiles.data(), s->tiles.size());
+}
+
+void free_scene_dev(om_scene *s)
+{
+    dfree(s->d_helio);
+    dfree(s->d_geo);
+    dfree(s->d_cell_start);
+    dfree(s->d_cell_items);
+    dfree(s->d_cdf_x);
+    dfree(s->d_cdf_y);
+    dfree(s->d_slope);
+    dfree(s->d_poisson);
+    dfree(s->d_mcount);
+    dfree(s->d_tiles);
+}
+
+om_status validate(const om_scene_desc *d, const om_params *p)
+{
+    if (!d || !p) return OM_ERR_INVALID;
+    if (d->n_helio < 0 || (d->n_helio > 0 && (!d->helio_pos || !d->helio_size || !d->helio_gap ||
+                                              !d->helio_matrix))) {
+        om::set_error("heliostat arrays missing");
+        return OM_ERR_INVALID;
+    }
+    if (p->rays_per_sample <= 0 || !(p->helio_pixel_length > 0.0f)) {
+        om::set_error("rays_per_sample and helio_pixel_length must be positive");
+        return OM_ERR_INVALID;
+    }
+    if (p->sunshape != OM_SUN_PILLBOX && p->sunshape != OM_SUN_BUIE) {
+        om::set_error("unknown sunshape");
+        return OM_ERR_INVALID;
+    }
+    if (p->micro_mode != OM_MICRO_LATTICE && p->micro_mode != OM_MICRO_POISSON) {
+        om::set_error("unknown microsample mode");
+        return OM_ERR_INVALID;
+    }
+    if (p->sunshape == OM_SUN_BUIE && (p->buie_bins <= 0 || p->buie_bins > 8192 || !(p->csr > 0.0f))) {
+        om::set_error("buie_bins must be in 1..8192 and csr positive");
+        return OM_ERR_INVALID;
+    }
+    if (p->micro_mode == OM_MICRO_POISSON && p->poisson_k <= 0) {
+        om::set_error("poisson_k must be positive");
+        return OM_ERR_INVALID;
+    }
+    for (int i = 0; i < d->n_helio; ++i)
+        if (d->helio_matrix[2 * i] <= 0 || d->helio_matrix[2 * i + 1] <= 0) {
+            om::set_error("heliostat facet matrix must be positive");
+            return OM_ERR_INVALID;
+        }
+    for (int a = 0; a < 3; ++a)
+        if (!(d->grid_interval[a] > 0.0f)) {
+            om::set_error("grid interval must be positive");
+            return OM_ERR_INVALID;
+        }
+    return OM_OK;
+}
+
+om_status build_all(om_scene *s)
+{
+    om_status st;
+    // InitContent order (solar_scene.cpp:83-99): sun, grid, receiver, heliostats
+    s->sun = om::normalize(ld3(s->sun_in));
+    build_buie(s);
+    if ((st = build_grid(s)) != OM_OK) return st;
+    if ((st = build_receiver(s)) != OM_OK) return st;
+    aim_heliostats(s);
+    micro_layout(s);
+    if (s->H > 0 && s->H >= (1 << 28)) {
+        om::set_error("too many heliostats");
+        return OM_ERR_UNSUPPORTED;
+    }
+    // device copies
+    OM_HIP(hipSetDevice(s->ctx->device));
+    if ((st = upload(s->d_cell_start, s->cell_start.data(), s->cell_start.size())) != OM_OK) return st;
+    if ((st = upload(s->d_cell_items, s->cell_items.data(), s->cell_items.size())) != OM_OK) return st;
+    if ((st = upload(s->d_cdf_x, s->cdf_x.data(), s->cdf_x.size())) != OM_OK) return st;
+    if ((st = upload(s->d_cdf_y, s->cdf_y.data(), s->cdf_y.size())) != OM_OK) return st;
+    if ((st = upload(s->d_slope, s->slope.data(), s->slope.size())) != OM_OK) return st;
+    if (s->prm.micro_mode == OM_MICRO_POISSON) {
+        // Poisson points are drawn on the GPU (one workgroup per heliostat); their
+        // counts come back to size the ray index space.
+        s->ray_offset.assign(s->H + 1, 0);
+        if ((st = upload_geometry(s)) != OM_OK) return st;
+        dfree(s->d_poisson);
+        dfree(s->d_mcount);
+        if (s->H > 0) {
+            OM_HIP(hipMalloc((void **)&s->d_poisson, sizeof(float2) * (size_t)s->poisson_cap * s->H));
+            OM_HIP(hipMalloc((void **)&s->d_mcount, sizeof(int) * s->H));
+            if ((st = om::launch_poisson(s, nullptr)) != OM_OK) return st;
+            OM_HIP(hipMemcpy(s->mcount.data(), s->d_mcount, sizeof(int) * s->H, hipMemcpyDeviceToHost));
+        }
+    }
+    finish_micro(s);
+    if (s->total_rays >= (1LL << 40)) {
+        om::set_error("more than 2^40 rays");
+        return OM_ERR_UNSUPPORTED;
+    }
+    flux_scale(s);
+    if ((st = upload_geometry(s)) != OM_OK) return st;
+    return choose_tiles(s);
+}
+
+} // namespace
+
+namespace om {
+TraceArgs make_trace_args(const om_scene *s)
+{
+    TraceArgs a;
+    std::memset(&a, 0, sizeof(a));
+    frame sf = make_frame(s->sun);
+    a.sun_u = sf.u;
+    a.sun_n = sf.n;
+    a.sun_v = sf.v;
+    a.sun_ident = sf.identity;
+    a.pillbox = s->prm.pillbox_half_angle;
+    a.buie_int465 = s->buie_int465;
+    a.buie_A = s->buie_A;
+    a.buie_B = s->buie_B;
+    a.buie_C = s->buie_C;
+    a.buie_inv_gp1 = s->buie_inv_gp1;
+    a.nbins = (s->prm.sunshape == OM_SUN_BUIE) ? s->nbins : 0;
+    a.cdf_x = s->d_cdf_x;
+    a.cdf_y = s->d_cdf_y;
+    a.slope = s->d_slope;
+    a.slope_sigma = s->prm.slope_sigma;
+    a.key0 = (uint32_t)s->prm.seed;
+    a.key1 = (uint32_t)(s->prm.seed >> 32);
+    a.w3 = (s->prm.scene_index << 4) | PURPOSE_RAY;
+    a.L = s->prm.rays_per_sample;
+    a.pitch = s->prm.helio_pixel_length;
+    a.pitch2 = s->prm.helio_pixel_length / 2;
+    a.gpx = s->grid_pos[0];
+    a.gpy = s->grid_pos[1];
+    a.gpz = s->grid_pos[2];
+    a.gix = s->grid_interval[0];
+    a.giy = s->grid_interval[1];
+    a.giz = s->grid_interval[2];
+    a.gnx = s->gn[0];
+    a.gny = s->gn[1];
+    a.gnz = s->gn[2];
+    for (int i = 0; i < 3; ++i) {
+        a.occ_lo[i] = s->occ_lo[i];
+        a.occ_hi[i] = s->occ_hi[i];
+    }
+    a.cell_start = s->d_cell_start;
+    a.cell_items = s->d_cell_items;
+    a.geo = s->d_geo;
+    a.rA = s->rv[0];
+    a.rE1 = s->rv[1] - s->rv[0];
+    a.rE2 = s->rv[3] - s->rv[0];
+    a.res_x = s->res_x;
+    a.res_y = s->res_y;
+    a.helio = s->d_helio;
+    a.poisson = s->d_poisson;
+    a.poisson_cap = s->poisson_cap;
+    a.tiles = s->d_tiles;
+    a.ntiles = (int)s->tiles.size();
+    a.rays_per_thread = s->rays_per_thread;
+    return a;
+}
+} // namespace om
+
+static om_status scene_from_desc(om_ctx *ctx, const om_scene_desc *d, const om_params *p,
+                                 om_scene **out)
+{
+    om_status st = validate(d, p);
+    if (st != OM_OK) return st;
+    om_scene *s = new om_scene();
+    s->ctx = ctx;
+    s->prm = *p;
+    s->H = d->n_helio;
+    s->pos.assign(d->helio_pos, d->helio_pos + 3 * (size_t)s->H);
+    s->size.assign(d->helio_size, d->helio_size + 3 * (size_t)s->H);
+    s->gap.assign(d->helio_gap, d->helio_gap + 2 * (size_t)s->H);
+    s->matrix.assign(d->helio_matrix, d->helio_matrix + 2 * (size_t)s->H);
+    s->recv_type = d->recv_type;
+    std::memcpy(s->recv_pos, d->recv_pos, 12);
+    std::memcpy(s->recv_size, d->recv_size, 12);
+    std::memcpy(s->recv_normal_in, d->recv_normal, 12);
+    s->recv_face = d->recv_face;
+    s->recv_pixel_length = d->recv_pixel_length;
+    std::memcpy(s->grid_pos, d->grid_pos, 12);
+    std::memcpy(s->grid_size, d->grid_size, 12);
+    std::memcpy(s->grid_interval, d->grid_interval, 12);
+    std::memcpy(s->sun_in, d->sun_dir, 12);
+    s->dni = d->dni;
+    st = build_all(s);
+    if (st != OM_OK) {
+        free_scene_dev(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return OM_OK;
+}
+
+extern "C" om_status om_scene_create(om_ctx *ctx, const om_scene_desc *desc, const om_params *params,
+                                     om_scene **out)
+{
+    if (!ctx || !desc || !params || !out) return OM_ERR_INVALID;
+    *out = nullptr;
+    return scene_from_desc(ctx, desc, params, out);
+}
+
+extern "C" om_status om_scene_from_scn(om_ctx *ctx, const char *path, const float sun_dir[3], float dni,
+                                       const om_params *params, om_scene **out)
+{
+    if (!ctx || !path || !sun_dir || !params || !out) return OM_ERR_INVALID;
+    *out = nullptr;
+    om_scene_desc d;
+    om_status st = om_scn_parse(path, &d);
+    if (st != OM_OK) return st;
+    std::memcpy(d.sun_dir, sun_dir, 12);
+    d.dni = dni;
+    st = scene_from_desc(ctx, &d, params, out);
+    om_scene_desc_free(&d);
+    return st;
+}
+
+extern "C" om_status om_scene_destroy(om_scene *s)
+{
+    if (!s) return OM_ERR_INVALID;
+    (void)hipSetDevice(s->ctx->device);
+    free_scene_dev(s);
+    delete s;
+    return OM_OK;
+}
+
+extern "C" om_status om_scene_get_info(const om_scene *s, om_scene_info *info)
+{
+    if (!s || !info) return OM_ERR_INVALID;
+    info->n_helio = s->H;
+    info->res_x = s->res_x;
+    info->res_y = s->res_y;
+    for (int a = 0; a < 3; ++a) info->grid_num[a] = s->gn[a];
+    info->grid_cells = s->cells;
+    info->grid_nnz = s->nnz;
+    info->total_micro = s->total_micro;
+    info->total_rays = s->total_rays;
+    info->max_micro = s->max_micro;
+    info->pixel_length = s->recv_pixel;
+    info->flux_scale = s->scale;
+    info->total_energy = s->total_energy;
+    return OM_OK;
+}
+
+extern "C" om_status om_scene_set_sun(om_scene *s, const float sun_dir[3], float dni)
+{
+    if (!s || !sun_dir) return OM_ERR_INVALID;
+    std::memcpy(s->sun_in, sun_dir, 12);
+    s->dni = dni;
+    s->sun = om::normalize(ld3(s->sun_in));
+    aim_heliostats(s);
+    flux_scale(s);
+    OM_HIP(hipSetDevice(s->ctx->device));
+    // the Poisson points are drawn in each heliostat's local frame, so they survive re-aiming
+    return upload_geometry(s);
+}
+
+extern "C" om_status om_scene_export(const om_scene *s, const char *what, void *host, size_t bytes)
+{
+    if (!s || !what || !host) return OM_ERR_INVALID;
+    std::string w(what);
+    std::vector<char> buf;
+    auto put = [&](const void *p, size_t n) {
+        const char *c = (const char *)p;
+        buf.insert(buf.end(), c, c + n);
+    };
+    if (w == "helio_normal") put(s->normal.data(), 12 * (size_t)s->H);
+    else if (w == "helio_v0") put(s->v0.data(), 12 * (size_t)s->H);
+    else if (w == "helio_v1") put(s->v1.data(), 12 * (size_t)s->H);
+    else if (w == "helio_v3") put(s->v3.data(), 12 * (size_t)s->H);
+    else if (w == "grid_start") put(s->cell_start.data(), 4 * s->cell_start.size());
+    else if (w == "grid_items") put(s->cell_items.data(), 4 * (size_t)s->nnz);
+    else if (w == "receiver") {
+        put(s->rv, 48);
+        put(&s->focus, 12);
+    } else if (w == "buie") {
+        put(s->cdf_x.data(), 4 * s->cdf_x.size());
+        put(s->cdf_y.data(), 4 * s->cdf_y.size());
+        put(s->slope.data(), 4 * s->slope.size());
+    } else if (w == "micro_count") put(s->mcount.data(), 4 * (size_t)s->H);
+    else if (w == "ray_offset") put(s->ray_offset.data(), 8 * s->ray_offset.size());
+    else if (w == "sun") {
+        put(&s->sun, 12);
+        float c[5] = {s->buie_int465, s->buie_A, s->buie_B, s->buie_C, s->buie_inv_gp1};
+        put(c, 20);
+    } else if (w == "poisson") {
+        if (s->prm.micro_mode != OM_MICRO_POISSON) {
+            om::set_error("scene has no Poisson microsamples");
+            return OM_ERR_INVALID;
+        }
+        std::vector<float2> all((size_t)s->poisson_cap * s->H);
+        if (!all.empty())
+            OM_HIP(hipMemcpy(all.data(), s->d_poisson, sizeof(float2) * all.size(), hipMemcpyDeviceToHost));
+        std::vector<float2> out((size_t)s->max_micro * s->H, make_float2(0.0f, 0.0f));
+        for (int h = 0; h < s->H; ++h)
+            for (int m = 0; m < s->mcount[h]; ++m)
+                out[(size_t)h * s->max_micro + m] = all[(size_t)h * s->poisson_cap + m];
+        put(out.data(), sizeof(float2) * out.size());
+    } else {
+        om::set_error("unknown export " + w);
+        return OM_ERR_INVALID;
+    }
+    if (bytes < buf.size()) {
+        om::set_error("export buffer too small");
+        return OM_ERR_SIZE;
+    }
+    std::memcpy(host, buf.data(), buf.size());
+    return OM_OK;
+}
+
+// ImageSaver::savetxt(..., float totalEnergy) (image_save.cpp:26-47): rows are summed from
+// the last to the first, pixels below Epsilon skipped.
+extern "C" float om_intercept_ratio(const float *flux, int width, int height, float total_energy)
+{
+    float in = 0.0f;
+    for (int r = 0; r < height; ++r)
+        for (int c = 0; c < width; ++c) {
+            float v = flux[(size_t)(height - 1 - r) * width + c];
+            if (v < om::EPS_LT) continue;
+            in += v;
+        }
+    return in / total_energy;
+}

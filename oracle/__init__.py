@@ -1,0 +1,414 @@
+"""CPU oracle for the heliostat-field ray tracer -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+may import this package, and only as the checker or as the host baseline.  The product
+(``onemirror_amd``) never imports it and has no CPU fallback.
+
+The compiled part (``liboracle.so``, built by ``make -C oracle``) restates the reference
+routines of ``Hu_smooth/ClassedSunRay`` in plain C; see ``om_oracle.h`` for the function
+list and citations.  ``scn.py`` restates the ``.scn`` parser.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class SceneDescC(ctypes.Structure):
+    _fields_ = [
+        ("n_helio", ctypes.c_int),
+        ("helio_pos", _f32p),
+        ("helio_size", _f32p),
+        ("helio_gap", _f32p),
+        ("helio_matrix", _i32p),
+        ("recv_type", ctypes.c_int),
+        ("recv_pos", ctypes.c_float * 3),
+        ("recv_size", ctypes.c_float * 3),
+        ("recv_normal", ctypes.c_float * 3),
+        ("recv_face", ctypes.c_int),
+        ("recv_pixel_length", ctypes.c_float),
+        ("grid_pos", ctypes.c_float * 3),
+        ("grid_size", ctypes.c_float * 3),
+        ("grid_interval", ctypes.c_float * 3),
+        ("sun_dir", ctypes.c_float * 3),
+        ("dni", ctypes.c_float),
+    ]
+
+
+class ParamsC(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("scene_index", ctypes.c_uint32),
+        ("sunshape", ctypes.c_int),
+        ("pillbox_half_angle", ctypes.c_float),
+        ("csr", ctypes.c_float),
+        ("buie_bins", ctypes.c_int),
+        ("slope_sigma", ctypes.c_float),
+        ("micro_mode", ctypes.c_int),
+        ("helio_pixel_length", ctypes.c_float),
+        ("poisson_k", ctypes.c_int),
+        ("rays_per_sample", ctypes.c_int),
+        ("reflectivity", ctypes.c_float),
+    ]
+
+
+class StatsC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                ("rays", "shaded", "blocked", "hit", "dda_steps", "dda_steps_eff",
+                 "cand_tests", "self_skips")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle``); raises if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+    L = ctypes.CDLL(path)
+    L.oc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+    L.oc_philox_batch.argtypes = [_u32p, _u32p, _u32p, ctypes.c_int]
+    L.oc_math_batch.argtypes = [ctypes.c_int, _f32p, _f32p, ctypes.c_int]
+    L.oc_scene_build.argtypes = [ctypes.POINTER(SceneDescC), ctypes.POINTER(ParamsC),
+                                 ctypes.POINTER(ctypes.c_void_p)]
+    L.oc_scene_build.restype = ctypes.c_int
+    L.oc_scene_free.argtypes = [ctypes.c_void_p]
+    L.oc_scene_counts.argtypes = [ctypes.c_void_p, _i64p]
+    L.oc_scene_helio.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]
+    L.oc_scene_grid.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i32p]
+    L.oc_scene_receiver.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p]
+    L.oc_scene_sun.argtypes = [ctypes.c_void_p, _f32p, _f32p]
+    L.oc_scene_buie_table.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p]
+    L.oc_scene_buie_table.restype = ctypes.c_int
+    L.oc_scene_micro.argtypes = [ctypes.c_void_p, _i32p, _i64p]
+    L.oc_scene_poisson.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_int]
+    L.oc_scene_poisson.restype = ctypes.c_int
+    L.oc_scene_flux_scale.argtypes = [ctypes.c_void_p, _f32p, _f32p]
+    L.oc_trace_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _u32p, _u64p,
+                               _i32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(StatsC)]
+    L.oc_trace_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _u32p, _f64p,
+                               _i32p, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(StatsC)]
+    L.oc_trace_pooled.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_uint64, _f64p, ctypes.c_int]
+    L.oc_energy_to_float.argtypes = [_u64p, ctypes.c_int, _f32p]
+    L.oc_trimmed_mean.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_float]
+    L.oc_intercept_ratio.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    L.oc_intercept_ratio.restype = ctypes.c_float
+    L.oc_dda_probe.argtypes = [ctypes.c_void_p, _f32p, _f32p, ctypes.c_int, _i32p,
+                               ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.oc_dda_probe.restype = ctypes.c_int
+    L.oc_local2world.argtypes = [_f32p, _f32p, _f32p]
+    L.oc_rotate_y.argtypes = [_f32p, _f32p, _f32p, _f32p]
+    L.oc_ray_parallelogram.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.oc_ray_parallelogram.restype = ctypes.c_int
+    L.oc_sample_ray.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                _f32p, _f32p]
+    _LIB = L
+    return L
+
+
+def _p(a: Optional[np.ndarray], t):
+    if a is None:
+        return ctypes.cast(None, t)
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(t)
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in np.asarray(v, np.float32)])
+
+
+# ---------------------------------------------------------------------------------------
+# math
+# ---------------------------------------------------------------------------------------
+_MATH = {"u01": 0, "sin2pi": 1, "cos2pi": 2, "sin_small": 3, "cos_small": 4, "log": 5,
+         "exp": 6, "erfinv": 7, "gauss": 8, "sqrt": 9}
+
+
+def math(fn: str, x) -> np.ndarray:
+    """Evaluate one of the project-written float functions element-wise."""
+    if fn == "u01":
+        xi = np.ascontiguousarray(x, dtype=np.uint32)
+        xf = xi.view(np.float32)
+    else:
+        xf = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(xf)
+    lib().oc_math_batch(_MATH[fn], _p(xf, _f32p), _p(out, _f32p), int(xf.size))
+    return out
+
+
+def philox(ctr, key) -> np.ndarray:
+    """Philox4x32-10 of counters ``ctr`` ([n,4] uint32) under ``key`` (2 uint32)."""
+    c = np.ascontiguousarray(np.atleast_2d(ctr), dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.empty_like(c)
+    lib().oc_philox_batch(_p(c, _u32p), _p(k, _u32p), _p(out, _u32p), int(c.shape[0]))
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# scene
+# ---------------------------------------------------------------------------------------
+def _desc_struct(desc: dict, keep: list):
+    H = int(desc["helio_pos"].shape[0])
+    arrs = {
+        "helio_pos": np.ascontiguousarray(desc["helio_pos"], np.float32),
+        "helio_size": np.ascontiguousarray(desc["helio_size"], np.float32),
+        "helio_gap": np.ascontiguousarray(desc["helio_gap"], np.float32),
+        "helio_matrix": np.ascontiguousarray(desc["helio_matrix"], np.int32),
+    }
+    keep.extend(arrs.values())
+    d = SceneDescC()
+    d.n_helio = H
+    d.helio_pos = _p(arrs["helio_pos"], _f32p)
+    d.helio_size = _p(arrs["helio_size"], _f32p)
+    d.helio_gap = _p(arrs["helio_gap"], _f32p)
+    d.helio_matrix = _p(arrs["helio_matrix"], _i32p)
+    d.recv_type = int(desc.get("recv_type", 0))
+    d.recv_pos = _f3(desc["recv_pos"])
+    d.recv_size = _f3(desc["recv_size"])
+    d.recv_normal = _f3(desc["recv_normal"])
+    d.recv_face = int(desc.get("recv_face", 0))
+    d.recv_pixel_length = float(desc["recv_pixel_length"])
+    d.grid_pos = _f3(desc["grid_pos"])
+    d.grid_size = _f3(desc["grid_size"])
+    d.grid_interval = _f3(desc["grid_interval"])
+    d.sun_dir = _f3(desc["sun_dir"])
+    d.dni = float(desc["dni"])
+    return d
+
+
+def _params_struct(prm: dict):
+    p = ParamsC()
+    p.seed = int(prm["seed"])
+    p.scene_index = int(prm.get("scene_index", 0))
+    p.sunshape = int(prm["sunshape"])
+    p.pillbox_half_angle = float(prm.get("pillbox_half_angle", 4.65e-3))
+    p.csr = float(prm.get("csr", 0.1))
+    p.buie_bins = int(prm.get("buie_bins", 1024))
+    p.slope_sigma = float(prm["slope_sigma"])
+    p.micro_mode = int(prm["micro_mode"])
+    p.helio_pixel_length = float(prm["helio_pixel_length"])
+    p.poisson_k = int(prm.get("poisson_k", 8))
+    p.rays_per_sample = int(prm["rays_per_sample"])
+    p.reflectivity = float(prm.get("reflectivity", 0.88))
+    return p
+
+
+class OracleScene:
+    """An oracle scene built from a scene description and tracing parameters (the same
+    dictionaries the product takes)."""
+
+    def __init__(self, desc: dict, prm: dict):
+        L = lib()
+        keep: list = []
+        d = _desc_struct(desc, keep)
+        p = _params_struct(prm)
+        h = ctypes.c_void_p()
+        rc = L.oc_scene_build(ctypes.byref(d), ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise ValueError(f"oc_scene_build failed: {rc}")
+        self._h = h
+        self.desc, self.prm = desc, prm
+        c = np.zeros(8, np.int64)
+        L.oc_scene_counts(self._h, _p(c, _i64p))
+        (self.n_helio, self.cells, self.nnz, self.res_x, self.res_y, self.total_micro,
+         self.total_rays, self.max_micro) = [int(v) for v in c]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.oc_scene_free(self._h)
+            self._h = None
+
+    # --- accessors ---------------------------------------------------------------------
+    def helio(self):
+        H = self.n_helio
+        n, v0, v1, v3 = (np.zeros((H, 3), np.float32) for _ in range(4))
+        lib().oc_scene_helio(self._h, _p(n, _f32p), _p(v0, _f32p), _p(v1, _f32p), _p(v3, _f32p))
+        return {"normal": n, "v0": v0, "v1": v1, "v3": v3}
+
+    def grid(self):
+        cs = np.zeros(self.cells + 1, np.int32)
+        ci = np.zeros(max(self.nnz, 1), np.int32)
+        gn = np.zeros(3, np.int32)
+        lib().oc_scene_grid(self._h, _p(cs, _i32p), _p(ci, _i32p), _p(gn, _i32p))
+        return {"cell_start": cs, "cell_items": ci[: self.nnz], "grid_num": gn}
+
+    def receiver(self):
+        v = np.zeros((4, 3), np.float32)
+        f = np.zeros(3, np.float32)
+        pl = np.zeros(1, np.float32)
+        lib().oc_scene_receiver(self._h, _p(v, _f32p), _p(f, _f32p), _p(pl, _f32p))
+        return {"vertices": v, "focus": f, "pixel_length": float(pl[0]),
+                "res": (self.res_x, self.res_y)}
+
+    def sun(self):
+        s = np.zeros(3, np.float32)
+        c = np.zeros(6, np.float32)
+        lib().oc_scene_sun(self._h, _p(s, _f32p), _p(c, _f32p))
+        return {"sun_dir": s, "int465": c[0], "A": c[1], "B": c[2], "C": c[3],
+                "inv_gp1": c[4], "gamma": c[5]}
+
+    def buie_table(self):
+        n = int(self.prm.get("buie_bins", 1024))
+        x = np.zeros(n + 1, np.float32)
+        y = np.zeros(n + 1, np.float32)
+        k = np.zeros(n, np.float32)
+        lib().oc_scene_buie_table(self._h, _p(x, _f32p), _p(y, _f32p), _p(k, _f32p))
+        return {"cdf_x": x, "cdf_y": y, "slope": k}
+
+    def micro(self):
+        c = np.zeros(self.n_helio, np.int32)
+        o = np.zeros(self.n_helio + 1, np.int64)
+        lib().oc_scene_micro(self._h, _p(c, _i32p), _p(o, _i64p))
+        return {"count": c, "ray_offset": o}
+
+    def poisson(self, h: int) -> np.ndarray:
+        cap = self.max_micro
+        pts = np.zeros((max(cap, 1), 2), np.float32)
+        n = lib().oc_scene_poisson(self._h, int(h), _p(pts, _f32p), cap)
+        if n < 0:
+            raise ValueError("scene has no Poisson microsamples")
+        return pts[:n].copy()
+
+    def flux_scale(self):
+        s = np.zeros(1, np.float32)
+        t = np.zeros(1, np.float32)
+        lib().oc_scene_flux_scale(self._h, _p(s, _f32p), _p(t, _f32p))
+        return float(s[0]), float(t[0])
+
+    def sample_ray(self, h, m, l):
+        a = np.zeros(3, np.float32)
+        b = np.zeros(3, np.float32)
+        lib().oc_sample_ray(self._h, int(h), int(m), int(l), _p(a, _f32p), _p(b, _f32p))
+        return a, b
+
+    def dda_probe(self, orig, direction, self_index=-1, cap=4096):
+        o = np.ascontiguousarray(orig, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        cells = np.zeros(cap, np.int32)
+        hit = ctypes.c_int(-1)
+        n = lib().oc_dda_probe(self._h, _p(o, _f32p), _p(d, _f32p), int(self_index),
+                               _p(cells, _i32p), cap, ctypes.byref(hit))
+        return cells[: min(n, cap)].copy(), int(hit.value)
+
+    # --- tracing -----------------------------------------------------------------------
+    def _range(self, h0, h1):
+        h1 = self.n_helio if h1 is None else h1
+        off = self.micro()["ray_offset"]
+        return h0, h1, off
+
+    def trace_f32(self, h0: int = 0, h1: Optional[int] = None, records: bool = False,
+                  early_exit: bool = True, nthreads: int = 0) -> dict:
+        h0, h1, off = self._range(h0, h1)
+        npx = self.res_x * self.res_y
+        counts = np.zeros(npx, np.uint32)
+        energy = np.zeros(npx, np.uint64)
+        recs = [None, None, None]
+        if records:
+            nr = int(self.total_rays)
+            recs = [np.full(nr, -1, np.int32) for _ in range(3)]
+        st = StatsC()
+        lib().oc_trace_f32(self._h, int(h0), int(h1), _p(counts, _u32p), _p(energy, _u64p),
+                           _p(recs[0], _i32p), _p(recs[1], _i32p), _p(recs[2], _i32p),
+                           int(bool(early_exit)), int(nthreads), ctypes.byref(st))
+        out = {"counts": counts.reshape(self.res_y, self.res_x),
+               "energy": energy.reshape(self.res_y, self.res_x), "stats": st.as_dict()}
+        if records:
+            a, b = int(off[h0]), int(off[h1])
+            out.update(shade=recs[0][a:b], block=recs[1][a:b], pixel=recs[2][a:b])
+        return out
+
+    def trace_f64(self, h0: int = 0, h1: Optional[int] = None, records: bool = False,
+                  nthreads: int = 0) -> dict:
+        h0, h1, off = self._range(h0, h1)
+        npx = self.res_x * self.res_y
+        counts = np.zeros(npx, np.uint32)
+        energy = np.zeros(npx, np.float64)
+        recs = [None, None, None]
+        if records:
+            nr = int(self.total_rays)
+            recs = [np.full(nr, -1, np.int32) for _ in range(3)]
+        st = StatsC()
+        lib().oc_trace_f64(self._h, int(h0), int(h1), _p(counts, _u32p), _p(energy, _f64p),
+                           _p(recs[0], _i32p), _p(recs[1], _i32p), _p(recs[2], _i32p),
+                           int(nthreads), ctypes.byref(st))
+        out = {"counts": counts.reshape(self.res_y, self.res_x),
+               "energy": energy.reshape(self.res_y, self.res_x), "stats": st.as_dict()}
+        if records:
+            a, b = int(off[h0]), int(off[h1])
+            out.update(shade=recs[0][a:b], block=recs[1][a:b], pixel=recs[2][a:b])
+        return out
+
+    def trace_pooled(self, h0=0, h1=None, groups=128, lights=None, pool_seed=12345,
+                     nthreads=0) -> np.ndarray:
+        h1 = self.n_helio if h1 is None else h1
+        lights = int(self.prm["rays_per_sample"]) if lights is None else int(lights)
+        e = np.zeros(h1 - h0, np.float64)
+        rc = lib().oc_trace_pooled(self._h, int(h0), int(h1), int(groups), lights,
+                                   int(pool_seed), _p(e, _f64p), int(nthreads))
+        if rc != 0:
+            raise ValueError("pool smaller than rays per sample")
+        return e
+
+
+def energy_to_float(energy: np.ndarray) -> np.ndarray:
+    e = np.ascontiguousarray(energy, np.uint64)
+    out = np.empty(e.shape, np.float32)
+    lib().oc_energy_to_float(_p(e, _u64p), int(e.size), _p(out, _f32p))
+    return out
+
+
+def trimmed_mean(img: np.ndarray, kernel_radius: int, trimmed_ratio: float) -> np.ndarray:
+    a = np.ascontiguousarray(img, np.float32)
+    h, w = a.shape
+    out = np.empty_like(a)
+    lib().oc_trimmed_mean(_p(a, _f32p), _p(out, _f32p), w, h, int(kernel_radius),
+                          float(trimmed_ratio))
+    return out
+
+
+def intercept_ratio(flux: np.ndarray, total_energy: float) -> float:
+    a = np.ascontiguousarray(flux, np.float32)
+    h, w = a.shape
+    return float(lib().oc_intercept_ratio(_p(a, _f32p), w, h, float(np.float32(total_energy))))
+
+
+def local2world(d, n):
+    out = np.zeros(3, np.float32)
+    lib().oc_local2world(_p(np.ascontiguousarray(d, np.float32), _f32p),
+                         _p(np.ascontiguousarray(n, np.float32), _f32p), _p(out, _f32p))
+    return out
+
+
+def rotate_y(o, old, new):
+    out = np.zeros(3, np.float32)
+    lib().oc_rotate_y(*[_p(np.ascontiguousarray(a, np.float32), _f32p) for a in (o, old, new)],
+                      _p(out, _f32p))
+    return out
+
+
+def ray_parallelogram(orig, d, A, B, C):
+    tuv = np.zeros(3, np.float32)
+    r = lib().oc_ray_parallelogram(*[_p(np.ascontiguousarray(a, np.float32), _f32p)
+                                     for a in (orig, d, A, B, C)], _p(tuv, _f32p))
+    return bool(r), tuv

@@ -1,0 +1,148 @@
+/*
+ * om_oracle_math.h -- internal helpers of the CPU oracle (test infrastructure).
+ *
+ * float3 algebra with the host semantics of the reference's vector header
+ * (vector_arithmetic.cuh:51-54 rsqrtf = 1/sqrtf; :345-349 float3/float multiplies by the
+ * reciprocal; :383-413 dot, cross, normalize, reflect; :571-574 float3 -> int3 truncates),
+ * the geometry of global_function.cuh:14-105, and the project-written transcendental
+ * functions that the HIP kernels evaluate with the same operation sequence.
+ *
+ * Build with -ffp-contract=off: every a*b+c below must round twice, as on the GPU.
+ */
+#ifndef OM_ORACLE_MATH_H
+#define OM_ORACLE_MATH_H
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+typedef struct { float x, y, z; } f3;
+typedef struct { double x, y, z; } d3;
+
+static inline f3 mk3(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+static inline f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline f3 neg3(f3 a) { return mk3(-a.x, -a.y, -a.z); }
+static inline f3 scl3(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+static inline f3 sclf3(float s, f3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+static inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline f3 cross3(f3 a, f3 b)
+{
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+/* vector_arithmetic.cuh:397-401 with the host rsqrtf of :51-54 */
+static inline f3 normalize3(f3 v)
+{
+    float inv = 1.0f / sqrtf(dot3(v, v));
+    return scl3(v, inv);
+}
+/* vector_arithmetic.cuh:410-413: i - 2*n*dot(n,i), evaluated as (2*n)*d */
+static inline f3 reflect3(f3 i, f3 n)
+{
+    float d = dot3(n, i);
+    f3 tn = sclf3(2.0f, n);
+    return sub3(i, scl3(tn, d));
+}
+
+static inline d3 mkd(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static inline d3 addd(d3 a, d3 b) { return mkd(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 subd(d3 a, d3 b) { return mkd(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 negd(d3 a) { return mkd(-a.x, -a.y, -a.z); }
+static inline d3 scld(d3 a, double s) { return mkd(a.x * s, a.y * s, a.z * s); }
+static inline double dotd(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 crossd(d3 a, d3 b)
+{
+    return mkd(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline d3 normalized(d3 v) { return scld(v, 1.0 / sqrt(dotd(v, v))); }
+static inline d3 reflectd(d3 i, d3 n) { return subd(i, scld(n, 2.0 * dotd(n, i))); }
+static inline d3 f2d(f3 a) { return mkd(a.x, a.y, a.z); }
+
+/* The reference's Epsilon is the double literal 1e-6 (global_constant.h:6); every test
+ * against it compares a float promoted to double, which C reproduces as written. */
+#define OC_EPS 1e-6
+
+/* global_function.cuh:14-52 */
+static inline f3 local2world3(f3 d, f3 n)
+{
+    if (fabsf(n.x) < OC_EPS && fabsf(n.z) < OC_EPS)
+        return d;
+    f3 u = normalize3(cross3(mk3(0.0f, 1.0f, 0.0f), n));
+    f3 v = normalize3(cross3(u, n));
+    return mk3(d.x * u.x + d.y * n.x + d.z * v.x,
+               d.x * u.y + d.y * n.y + d.z * v.y,
+               d.x * u.z + d.y * n.z + d.z * v.z);
+}
+static inline d3 local2worldd(d3 d, d3 n)
+{
+    if (fabs(n.x) < OC_EPS && fabs(n.z) < OC_EPS)
+        return d;
+    d3 u = normalized(crossd(mkd(0.0, 1.0, 0.0), n));
+    d3 v = normalized(crossd(u, n));
+    return mkd(d.x * u.x + d.y * n.x + d.z * v.x,
+               d.x * u.y + d.y * n.y + d.z * v.y,
+               d.x * u.z + d.y * n.z + d.z * v.z);
+}
+
+/* global_function.cuh:61-74 */
+static inline f3 rotate_y3(f3 o, f3 old_dir, f3 new_dir)
+{
+    int dir = (cross3(old_dir, new_dir).y > 0) ? 1 : -1;
+    float c = dot3(old_dir, new_dir);
+    float s = (float)dir * sqrtf(1 - c * c);
+    return mk3(c * o.x + s * o.z, o.y, -s * o.x + c * o.z);
+}
+
+/* global_function.cuh:78-105 */
+static inline int ray_parallelogram3(f3 orig, f3 dir, f3 A, f3 B, f3 C,
+                                     float *t, float *u, float *v)
+{
+    f3 E1 = sub3(B, A);
+    f3 E2 = sub3(C, A);
+    f3 pvec = cross3(dir, E2);
+    float det = dot3(E1, pvec);
+    if (fabsf(det) < OC_EPS) return 0;
+    float invDet = 1 / det;
+    f3 T = sub3(orig, A);
+    *u = dot3(T, pvec) * invDet;
+    if (*u < 0 || *u > 1) return 0;
+    f3 qvec = cross3(T, E1);
+    *v = dot3(dir, qvec) * invDet;
+    if (*v < 0 || *v > 1) return 0;
+    *t = dot3(E2, qvec) * invDet;
+    if (*t < OC_EPS) return 0;
+    return 1;
+}
+static inline int ray_parallelogramd(d3 orig, d3 dir, d3 A, d3 B, d3 C,
+                                     double *t, double *u, double *v)
+{
+    d3 E1 = subd(B, A);
+    d3 E2 = subd(C, A);
+    d3 pvec = crossd(dir, E2);
+    double det = dotd(E1, pvec);
+    if (fabs(det) < OC_EPS) return 0;
+    double invDet = 1 / det;
+    d3 T = subd(orig, A);
+    *u = dotd(T, pvec) * invDet;
+    if (*u < 0 || *u > 1) return 0;
+    d3 qvec = crossd(T, E1);
+    *v = dotd(dir, qvec) * invDet;
+    if (*v < 0 || *v > 1) return 0;
+    *t = dotd(E2, qvec) * invDet;
+    if (*t < OC_EPS) return 0;
+    return 1;
+}
+
+/* float -> int truncation of make_int3(float3) (vector_arithmetic.cuh:571-574); values are
+ * clamped to +-2^30 first so that the conversion is defined on every platform. */
+static inline int trunc_i(float x)
+{
+    if (!(x > -1073741824.0f)) x = -1073741824.0f;
+    if (x > 1073741824.0f) x = 1073741824.0f;
+    return (int)x;
+}
+
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+#endif
