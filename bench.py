@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput of the heliostat-field ray tracer on one MI355X.
+
+One step = one full trace of the field (sunshape + slope-error sampling, shading DDA,
+reflection, blocking DDA, receiver hit, fixed-point accumulation) followed by the flux
+scaling and the trimmed-mean smoothing, all on the GPU with the scene resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mid|small]
+
+Prints one JSON line: rays/s, the roofline of the trace kernel (VALU lane-ops per ray from
+the cost model below times rays/s, against 256 CU x 64 lanes x 2.4 GHz), and the CPU
+oracle's rays/s on a stated sample of the same field on the host cores.
+
+VALU cost model per ray (lane operations, counted from the kernel source):
+    F0 = 820 fixed (Philox 80, Buie lookup 75, two sincos pairs 74, Gaussian 40,
+         frames/normalize/reflect 120, two DDA set-ups 300, microsample centre 50,
+         receiver + pixel + energy + atomics 75, indexing 6)
+    + 13 per DDA cell step + 58 per candidate parallelogram test + 3 per own-heliostat skip
+with the step/test counts measured by the oracle on the same sample of the field.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rays/s (w/ DDA shading+blocking) at ≤1e-5 flux err vs FP64; batched & fwd+bwd/s"
+P_VALU = 256 * 64 * 2.4e9          # lane-op/s
+F0, C_STEP, C_TEST, C_SKIP = 820.0, 13.0, 58.0, 3.0
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mid", choices=["mid", "small"])
+    ap.add_argument("--cpu-every", type=int, default=5,
+                    help="CPU baseline traces every n-th heliostat of the field")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every GPU box here has one GPU)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import onemirror_amd as om
+    from onemirror_amd import fields as F
+
+    if args.workload == "mid":
+        desc, prm = F.config_mid()
+        wl = ("mid: 5000 heliostats 10x10 m, radii 100-800 m, flat 25.6 m receiver, "
+              "Buie CSR 0.1 (2048-bin CDF), slope 2.5 mrad, 32x32 Poisson x 40 rays, "
+              "128^3 grid, 256x256 map, trimmed mean 5x5/20%")
+    else:
+        desc, prm = F.config_small()
+        wl = ("small: 100 heliostats 2x2 m, pillbox 4.65 mrad, slope 2 mrad, 16x16 Poisson "
+              "x 64 rays, 64^3 grid, 128x128 map")
+    t0 = time.time()
+    scene = om.SolarScene(desc, prm)
+    info = scene.info
+    H, R = info["n_helio"], info["total_rays"]
+    print(f"# scene built in {time.time() - t0:.1f}s: {H} heliostats, {R} rays, "
+          f"grid {info['grid_num']} nnz {info['grid_nnz']}", file=sys.stderr, flush=True)
+    maps = scene.new_maps(smooth=True)
+    stream = torch.cuda.current_stream()
+
+    def step(ev0=None, ev1=None):
+        maps.counts.zero_()
+        maps.energy.zero_()
+        if ev0 is not None:
+            ev0.record(stream)
+        scene.trace_heliostats(0, H, maps)
+        if ev1 is not None:
+            ev1.record(stream)
+        scene.finalize(maps)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        step(e0, e1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    trace_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
+    rays_s = R / (dt / args.steps)
+    hits = int(maps.counts.sum().item())
+    print(f"# {args.steps} steps: {ms_step:.3f} ms/step, trace kernel {trace_ms:.3f} ms, "
+          f"{rays_s:.4g} rays/s, receiver hits {hits}", file=sys.stderr, flush=True)
+
+    # oracle sample: step/test counts for the cost model and the CPU baseline
+    import oracle
+    osc = oracle.OracleScene(desc, prm)
+    every = max(1, args.cpu_every)
+    nthreads = host_threads()
+    blk = 40  # heliostats per block: blocks of neighbours keep the threads busy
+    blocks = [(h, min(H, h + blk)) for h in range(0, H, blk * every)]
+    if args.no_cpu:
+        blocks = blocks[:: max(1, len(blocks) // 8)]
+    st_tot = {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0}
+    osc.trace_f32(0, min(H, 4), nthreads=nthreads)  # warm-up
+    t1 = time.perf_counter()
+    for h0, h1 in blocks:
+        r = osc.trace_f32(h0, h1, nthreads=nthreads)
+        for k in st_tot:
+            st_tot[k] += r["stats"][k]
+    cpu_t = time.perf_counter() - t1
+    cpu_rays = st_tot["rays"]
+    nsample = sum(h1 - h0 for h0, h1 in blocks)
+    S = st_tot["dda_steps_eff"] / max(1, st_tot["rays"])
+    T = st_tot["cand_tests"] / max(1, st_tot["rays"])
+    K = st_tot["self_skips"] / max(1, st_tot["rays"])
+    ops_ray = F0 + C_STEP * S + C_TEST * T + C_SKIP * K
+    achieved = R * ops_ray / (trace_ms * 1e-3) / 1e12
+    peak = P_VALU / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": rays_s,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl, "heliostats": H, "rays": R, "grid": list(info["grid_num"]),
+                   "map": [info["res_y"], info["res_x"]], "parallelism": "single GPU",
+                   "trace_kernel_ms": trace_ms, "receiver_hits": hits},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "ops_per_ray": ops_ray, "dda_steps_per_ray": S, "tests_per_ray": T},
+        "cpu_baseline": None if args.no_cpu else {
+            "value": cpu_rays / cpu_t, "unit": "rays/s", "cores": nthreads, "kind": "port",
+            "sample": f"{len(blocks)} blocks of {blk} neighbouring heliostats, one in every "
+                      f"{every} of the same field ({nsample} heliostats, {cpu_rays} rays); "
+                      f"oracle FP32 tracer with the same early-exit walk",
+            "seconds": cpu_t},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -132,7 +132,10 @@ om_status om_scene_from_scn(om_ctx *ctx, const char *path, const float sun_dir[3
 om_status om_scn_parse(const char *path, om_scene_desc *desc);
 void      om_scene_desc_free(om_scene_desc *desc);
 
-/* Scene from arrays (generated fields) + InitContent (solar_scene.cpp:83-99). */
+/* Scene from arrays (generated fields) + InitContent (solar_scene.cpp:83-99).  With
+ * ctx == NULL the scene is built on the host only (aiming, grid, receiver, sun tables) for
+ * inspection through om_scene_export/om_scene_get_info; it cannot be traced, and its
+ * Poisson microsample counts are 0 because those points are drawn on the GPU. */
 om_status om_scene_create(om_ctx *ctx, const om_scene_desc *desc, const om_params *params,
                           om_scene **out);
 om_status om_scene_destroy(om_scene *scene);

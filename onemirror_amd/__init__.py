@@ -1,0 +1,431 @@
+"""onemirror_amd -- heliostat-field Monte-Carlo ray tracer on AMD MI355X (gfx950).
+
+Python binding of the C ABI in include/onemirror.h (libonemirror_amd.so, built in-tree by
+``python -m onemirror_amd.build``).  Device memory comes from PyTorch-ROCm tensors; every
+computation runs in the HIP kernels of the library.  There is no CPU fallback: if the
+library is missing or no GPU is present, the calls raise.
+
+Names mirror the reference's interface (Hu_smooth/ClassedSunRay):
+  SolarScene                    SolarScene + SceneFileProc::SceneFileRead + InitContent
+  results_with_given_normal     Gtest.cu:10-63
+  recthelio_ray_tracing         recthelio_tracing.cpp:46-79 (per-heliostat accumulate)
+  ImageSmoother.image_smooth    image_smooth.cu:60-115
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libonemirror_amd.so")
+_LIB = None
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+class OMError(RuntimeError):
+    pass
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_helio", ctypes.c_int),
+        ("helio_pos", _f32p),
+        ("helio_size", _f32p),
+        ("helio_gap", _f32p),
+        ("helio_matrix", _i32p),
+        ("recv_type", ctypes.c_int),
+        ("recv_pos", ctypes.c_float * 3),
+        ("recv_size", ctypes.c_float * 3),
+        ("recv_normal", ctypes.c_float * 3),
+        ("recv_face", ctypes.c_int),
+        ("recv_pixel_length", ctypes.c_float),
+        ("grid_pos", ctypes.c_float * 3),
+        ("grid_size", ctypes.c_float * 3),
+        ("grid_interval", ctypes.c_float * 3),
+        ("sun_dir", ctypes.c_float * 3),
+        ("dni", ctypes.c_float),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("scene_index", ctypes.c_uint32),
+        ("sunshape", ctypes.c_int),
+        ("pillbox_half_angle", ctypes.c_float),
+        ("csr", ctypes.c_float),
+        ("buie_bins", ctypes.c_int),
+        ("slope_sigma", ctypes.c_float),
+        ("micro_mode", ctypes.c_int),
+        ("helio_pixel_length", ctypes.c_float),
+        ("poisson_k", ctypes.c_int),
+        ("rays_per_sample", ctypes.c_int),
+        ("reflectivity", ctypes.c_float),
+        ("smooth_radius", ctypes.c_int),
+        ("smooth_ratio", ctypes.c_float),
+    ]
+
+
+class Flux(ctypes.Structure):
+    _fields_ = [("counts", _vp), ("energy", _vp), ("flux", _vp), ("flux_smooth", _vp),
+                ("capacity", ctypes.c_int64)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_helio", ctypes.c_int),
+        ("res_x", ctypes.c_int),
+        ("res_y", ctypes.c_int),
+        ("grid_num", ctypes.c_int * 3),
+        ("grid_cells", ctypes.c_int64),
+        ("grid_nnz", ctypes.c_int64),
+        ("total_micro", ctypes.c_int64),
+        ("total_rays", ctypes.c_int64),
+        ("max_micro", ctypes.c_int),
+        ("pixel_length", ctypes.c_float),
+        ("flux_scale", ctypes.c_float),
+        ("total_energy", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["grid_num"] = tuple(self.grid_num)
+        return d
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("flux", _f32p), ("flux_smooth", _f32p), ("ratio", ctypes.c_float),
+                ("ratio_smooth", ctypes.c_float), ("total_energy", ctypes.c_float),
+                ("rays", ctypes.c_int64)]
+
+
+_EXPORTS = [
+    "om_status_string", "om_last_error", "om_default_params", "om_ctx_create", "om_ctx_destroy",
+    "om_scene_from_scn", "om_scn_parse", "om_scene_desc_free", "om_scene_create",
+    "om_scene_destroy", "om_scene_get_info", "om_scene_set_sun", "om_scene_export",
+    "om_trace_plain", "om_trace_heliostats", "om_trace_records", "om_flux_finalize",
+    "om_smooth_trimmed", "om_results_with_given_normal", "om_intercept_ratio", "om_math_eval",
+]
+
+
+def lib():
+    """Load libonemirror_amd.so; raises OMError if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise OMError(f"{LIB_PATH} is missing: build it with `python -m onemirror_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.om_status_string.restype = ctypes.c_char_p
+    L.om_status_string.argtypes = [ctypes.c_int]
+    L.om_last_error.restype = ctypes.c_char_p
+    L.om_default_params.argtypes = [ctypes.POINTER(Params)]
+    L.om_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.om_ctx_destroy.argtypes = [_vp]
+    L.om_scene_from_scn.argtypes = [_vp, ctypes.c_char_p, _f32p, ctypes.c_float,
+                                    ctypes.POINTER(Params), ctypes.POINTER(_vp)]
+    L.om_scn_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
+    L.om_scene_desc_free.argtypes = [ctypes.POINTER(SceneDesc)]
+    L.om_scene_desc_free.restype = None
+    L.om_scene_create.argtypes = [_vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(Params),
+                                  ctypes.POINTER(_vp)]
+    L.om_scene_destroy.argtypes = [_vp]
+    L.om_scene_get_info.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
+    L.om_scene_set_sun.argtypes = [_vp, _f32p, ctypes.c_float]
+    L.om_scene_export.argtypes = [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]
+    L.om_trace_plain.argtypes = [_vp, ctypes.POINTER(Flux), _vp]
+    L.om_trace_heliostats.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Flux), _vp]
+    L.om_trace_records.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                   ctypes.c_int64, _vp]
+    L.om_flux_finalize.argtypes = [_vp, ctypes.POINTER(Flux), _vp]
+    L.om_smooth_trimmed.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_float, _vp]
+    L.om_results_with_given_normal.argtypes = [_vp, ctypes.POINTER(Result)]
+    L.om_intercept_ratio.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    L.om_intercept_ratio.restype = ctypes.c_float
+    L.om_math_eval.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]
+    for name in _EXPORTS:
+        if name not in ("om_status_string", "om_last_error", "om_default_params",
+                        "om_scene_desc_free", "om_intercept_ratio"):
+            getattr(L, name).restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        L = lib()
+        raise OMError(f"{what}: {L.om_status_string(rc).decode()} ({L.om_last_error().decode()})")
+
+
+def _stream_ptr(stream=None):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def default_params() -> dict:
+    p = Params()
+    lib().om_default_params(ctypes.byref(p))
+    return {n: getattr(p, n) for n, _ in Params._fields_}
+
+
+def _params_struct(prm: dict) -> Params:
+    base = default_params()
+    base.update(prm)
+    p = Params()
+    for n, t in Params._fields_:
+        setattr(p, n, float(base[n]) if t is ctypes.c_float else int(base[n]))
+    return p
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in np.asarray(v, np.float32)])
+
+
+def _desc_struct(desc: dict, keep: list) -> SceneDesc:
+    arrs = [np.ascontiguousarray(desc[k], t) for k, t in
+            (("helio_pos", np.float32), ("helio_size", np.float32), ("helio_gap", np.float32),
+             ("helio_matrix", np.int32))]
+    keep.extend(arrs)
+    d = SceneDesc()
+    d.n_helio = int(arrs[0].shape[0])
+    d.helio_pos = arrs[0].ctypes.data_as(_f32p)
+    d.helio_size = arrs[1].ctypes.data_as(_f32p)
+    d.helio_gap = arrs[2].ctypes.data_as(_f32p)
+    d.helio_matrix = arrs[3].ctypes.data_as(_i32p)
+    d.recv_type = int(desc.get("recv_type", 0))
+    d.recv_pos = _f3(desc["recv_pos"])
+    d.recv_size = _f3(desc["recv_size"])
+    d.recv_normal = _f3(desc["recv_normal"])
+    d.recv_face = int(desc.get("recv_face", 0))
+    d.recv_pixel_length = float(desc["recv_pixel_length"])
+    d.grid_pos = _f3(desc["grid_pos"])
+    d.grid_size = _f3(desc["grid_size"])
+    d.grid_interval = _f3(desc["grid_interval"])
+    d.sun_dir = _f3(desc["sun_dir"])
+    d.dni = float(desc["dni"])
+    return d
+
+
+def scn_parse(path: str) -> dict:
+    """Parse a .scn file (SceneFileProc::SceneFileRead) into a description dict."""
+    d = SceneDesc()
+    _check(lib().om_scn_parse(path.encode(), ctypes.byref(d)), "om_scn_parse")
+    H = d.n_helio
+    try:
+        out = {
+            "helio_pos": np.ctypeslib.as_array(d.helio_pos, (max(H, 1), 3))[:H].copy(),
+            "helio_size": np.ctypeslib.as_array(d.helio_size, (max(H, 1), 3))[:H].copy(),
+            "helio_gap": np.ctypeslib.as_array(d.helio_gap, (max(H, 1), 2))[:H].copy(),
+            "helio_matrix": np.ctypeslib.as_array(d.helio_matrix, (max(H, 1), 2))[:H].copy(),
+            "recv_type": d.recv_type, "recv_pos": np.array(d.recv_pos[:], np.float32),
+            "recv_size": np.array(d.recv_size[:], np.float32),
+            "recv_normal": np.array(d.recv_normal[:], np.float32), "recv_face": d.recv_face,
+            "recv_pixel_length": np.float32(d.recv_pixel_length),
+            "grid_pos": np.array(d.grid_pos[:], np.float32),
+            "grid_size": np.array(d.grid_size[:], np.float32),
+            "grid_interval": np.array(d.grid_interval[:], np.float32),
+            "sun_dir": np.array(d.sun_dir[:], np.float32), "dni": np.float32(d.dni),
+        }
+    finally:
+        lib().om_scene_desc_free(ctypes.byref(d))
+    return out
+
+
+class Context:
+    """A context on one GPU (om_ctx)."""
+
+    def __init__(self, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise OMError("no GPU: the tracer runs only on the HIP device")
+        torch.cuda.set_device(device)
+        h = _vp()
+        _check(lib().om_ctx_create(int(device), ctypes.byref(h)), "om_ctx_create")
+        self._h = h
+        self.device = device
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.om_ctx_destroy(self._h)
+            self._h = None
+
+    def math_eval(self, fn: int, x):
+        """Evaluate a device float function (parity self-test of the kernel arithmetic)."""
+        import torch
+        xin = torch.as_tensor(np.ascontiguousarray(x, np.float32)).cuda(self.device)
+        out = torch.empty_like(xin)
+        _check(lib().om_math_eval(self._h, int(fn), _vp(xin.data_ptr()), _vp(out.data_ptr()),
+                                  int(xin.numel()), _stream_ptr()), "om_math_eval")
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+
+_DEFAULT_CTX: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+class FluxMaps:
+    """Device maps of one receiver (torch tensors, row-major [res_y, res_x])."""
+
+    def __init__(self, res_x: int, res_y: int, device=None, smooth: bool = True):
+        import torch
+        dev = device if device is not None else torch.device("cuda")
+        self.counts = torch.zeros((res_y, res_x), dtype=torch.int32, device=dev)
+        self.energy = torch.zeros((res_y, res_x), dtype=torch.int64, device=dev)
+        self.flux = torch.zeros((res_y, res_x), dtype=torch.float32, device=dev)
+        self.flux_smooth = (torch.zeros((res_y, res_x), dtype=torch.float32, device=dev)
+                            if smooth else None)
+
+    def struct(self, with_flux: bool = True) -> Flux:
+        f = Flux()
+        f.counts = self.counts.data_ptr()
+        f.energy = self.energy.data_ptr()
+        f.flux = self.flux.data_ptr() if with_flux else None
+        f.flux_smooth = (self.flux_smooth.data_ptr()
+                         if (with_flux and self.flux_smooth is not None) else None)
+        f.capacity = self.counts.numel()
+        return f
+
+
+class SolarScene:
+    """A traced scene: description + parameters, built on the GPU (om_scene)."""
+
+    def __init__(self, desc: dict, params: dict, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.desc, self.params = desc, dict(params)
+        keep: list = []
+        d = _desc_struct(desc, keep)
+        p = _params_struct(params)
+        h = _vp()
+        _check(lib().om_scene_create(self.ctx._h, ctypes.byref(d), ctypes.byref(p),
+                                     ctypes.byref(h)), "om_scene_create")
+        self._h = h
+        self._refresh()
+
+    @classmethod
+    def from_scn(cls, path: str, sun_dir, dni: float, params: dict,
+                 ctx: Optional[Context] = None) -> "SolarScene":
+        desc = scn_parse(path)
+        desc["sun_dir"] = np.asarray(sun_dir, np.float32)
+        desc["dni"] = np.float32(dni)
+        return cls(desc, params, ctx)
+
+    def _refresh(self):
+        info = SceneInfo()
+        _check(lib().om_scene_get_info(self._h, ctypes.byref(info)), "om_scene_get_info")
+        self.info = info.as_dict()
+        self.res_x, self.res_y = info.res_x, info.res_y
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.om_scene_destroy(self._h)
+            self._h = None
+
+    def set_sun(self, sun_dir, dni: float):
+        _check(lib().om_scene_set_sun(self._h, _f3(sun_dir), float(dni)), "om_scene_set_sun")
+        self._refresh()
+
+    def export(self, what: str, nbytes: int, dtype) -> np.ndarray:
+        buf = np.zeros(max(nbytes, 1), np.uint8)
+        _check(lib().om_scene_export(self._h, what.encode(), _vp(buf.ctypes.data), buf.nbytes),
+               f"om_scene_export({what})")
+        return buf[:nbytes].view(dtype)
+
+    def new_maps(self, smooth: bool = True) -> FluxMaps:
+        return FluxMaps(self.res_x, self.res_y, smooth=smooth)
+
+    def trace_plain(self, maps: Optional[FluxMaps] = None, stream=None,
+                    finalize: bool = True) -> FluxMaps:
+        maps = maps or self.new_maps()
+        f = maps.struct(with_flux=finalize)
+        _check(lib().om_trace_plain(self._h, ctypes.byref(f), _stream_ptr(stream)),
+               "om_trace_plain")
+        return maps
+
+    def trace_heliostats(self, h0: int, h1: int, maps: FluxMaps, stream=None) -> FluxMaps:
+        f = maps.struct(with_flux=False)
+        _check(lib().om_trace_heliostats(self._h, int(h0), int(h1), ctypes.byref(f),
+                                         _stream_ptr(stream)), "om_trace_heliostats")
+        return maps
+
+    def finalize(self, maps: FluxMaps, stream=None) -> FluxMaps:
+        f = maps.struct(with_flux=True)
+        _check(lib().om_flux_finalize(self._h, ctypes.byref(f), _stream_ptr(stream)),
+               "om_flux_finalize")
+        return maps
+
+    def ray_offset(self) -> np.ndarray:
+        n = self.info["n_helio"]
+        return self.export("ray_offset", 8 * (n + 1), np.int64)
+
+    def trace_records(self, h0: int = 0, h1: Optional[int] = None, stream=None) -> dict:
+        import torch
+        h1 = self.info["n_helio"] if h1 is None else h1
+        off = self.ray_offset()
+        n = int(off[h1] - off[h0])
+        t = [torch.full((max(n, 1),), -7, dtype=torch.int32, device="cuda") for _ in range(3)]
+        _check(lib().om_trace_records(self._h, int(h0), int(h1), _vp(t[0].data_ptr()),
+                                      _vp(t[1].data_ptr()), _vp(t[2].data_ptr()), n,
+                                      _stream_ptr(stream)), "om_trace_records")
+        torch.cuda.synchronize()
+        return {"shade": t[0][:n].cpu().numpy(), "block": t[1][:n].cpu().numpy(),
+                "pixel": t[2][:n].cpu().numpy()}
+
+
+def results_with_given_normal(scene: SolarScene) -> dict:
+    """Gtest.cu:10-63: trace, scale, intercept ratio, smooth, ratio (synchronous)."""
+    n = scene.res_x * scene.res_y
+    fl = np.zeros(n, np.float32)
+    fs = np.zeros(n, np.float32)
+    r = Result()
+    r.flux = fl.ctypes.data_as(_f32p)
+    r.flux_smooth = fs.ctypes.data_as(_f32p)
+    _check(lib().om_results_with_given_normal(scene._h, ctypes.byref(r)),
+           "om_results_with_given_normal")
+    return {"flux": fl.reshape(scene.res_y, scene.res_x),
+            "flux_smooth": fs.reshape(scene.res_y, scene.res_x), "ratio": r.ratio,
+            "ratio_smooth": r.ratio_smooth, "total_energy": r.total_energy, "rays": r.rays}
+
+
+def recthelio_ray_tracing(scene: SolarScene, helio: int, maps: FluxMaps, stream=None) -> int:
+    """recthelio_ray_tracing (recthelio_tracing.cpp:46-79): accumulate one heliostat into
+    the maps; returns its microsample count."""
+    scene.trace_heliostats(helio, helio + 1, maps, stream)
+    cnt = scene.export("micro_count", 4 * scene.info["n_helio"], np.int32)
+    return int(cnt[helio])
+
+
+class ImageSmoother:
+    """ImageSmoother (image_smooth.cuh:17-20)."""
+
+    @staticmethod
+    def image_smooth(d_array, kernel_radius: int, trimmed_ratio: float, width: int, height: int,
+                     out=None, ctx: Optional[Context] = None, stream=None):
+        """Trimmed mean of a device float map; in place when ``out`` is None."""
+        ctx = ctx or default_context()
+        dst = d_array if out is None else out
+        _check(lib().om_smooth_trimmed(ctx._h, _vp(d_array.data_ptr()), _vp(dst.data_ptr()),
+                                       int(width), int(height), int(kernel_radius),
+                                       float(trimmed_ratio), _stream_ptr(stream)),
+               "om_smooth_trimmed")
+        return dst
+
+
+def intercept_ratio(flux: np.ndarray, total_energy: float) -> float:
+    a = np.ascontiguousarray(flux, np.float32)
+    h, w = a.shape
+    return float(lib().om_intercept_ratio(a.ctypes.data_as(_f32p), w, h, float(total_energy)))

@@ -512,7 +512,7 @@ static om_status clear_maps(const om_flux *out, long long n, hipStream_t st)
 {
     dim3 g((unsigned)((n + WG - 1) / WG)), b(WG);
     hipLaunchKernelGGL(fill_u32_kernel, g, b, 0, st, out->counts, n);
-    hipLaunchKernelGGL(fill_u64_kernel, g, b, 0, st, out->energy, n);
+    hipLaunchKernelGGL(fill_u64_kernel, g, b, 0, st, (unsigned long long *)out->energy, n);
     OM_HIP(hipGetLastError());
     return OM_OK;
 }
@@ -554,6 +554,10 @@ extern "C" om_status om_trace_heliostats(om_scene *s, int h0, int h1, const om_f
         set_error("heliostat range out of bounds");
         return OM_ERR_INVALID;
     }
+    if (!s->ctx) {
+        set_error("host-only scene (created without a context) cannot be traced");
+        return OM_ERR_INVALID;
+    }
     OM_HIP(hipSetDevice(s->ctx->device));
     TraceArgs a = make_trace_args(s);
     int t0, t1;
@@ -570,6 +574,10 @@ extern "C" om_status om_flux_finalize(om_scene *s, const om_flux *out, void *str
     if (!s) return OM_ERR_INVALID;
     om_status st = check_flux(s, out);
     if (st != OM_OK) return st;
+    if (!s->ctx) {
+        set_error("host-only scene (created without a context) cannot be traced");
+        return OM_ERR_INVALID;
+    }
     OM_HIP(hipSetDevice(s->ctx->device));
     int n = s->res_x * s->res_y;
     hipStream_t hs = (hipStream_t)stream;
@@ -591,6 +599,10 @@ extern "C" om_status om_trace_plain(om_scene *s, const om_flux *out, void *strea
     if (!s) return OM_ERR_INVALID;
     om_status st = check_flux(s, out);
     if (st != OM_OK) return st;
+    if (!s->ctx) {
+        set_error("host-only scene (created without a context) cannot be traced");
+        return OM_ERR_INVALID;
+    }
     OM_HIP(hipSetDevice(s->ctx->device));
     hipStream_t hs = (hipStream_t)stream;
     if ((st = clear_maps(out, (long long)s->res_x * s->res_y, hs)) != OM_OK) return st;
@@ -616,6 +628,10 @@ extern "C" om_status om_trace_records(om_scene *s, int h0, int h1, int32_t *shad
     if (capacity < need) {
         set_error("record buffers too small");
         return OM_ERR_SIZE;
+    }
+    if (!s->ctx) {
+        set_error("host-only scene (created without a context) cannot be traced");
+        return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
     TraceArgs a = make_trace_args(s);
@@ -665,6 +681,10 @@ extern "C" om_status om_math_eval(om_ctx *ctx, int fn, const float *d_in, float 
 extern "C" om_status om_results_with_given_normal(om_scene *s, om_result *res)
 {
     if (!s || !res) return OM_ERR_INVALID;
+    if (!s->ctx) {
+        set_error("host-only scene (created without a context) cannot be traced");
+        return OM_ERR_INVALID;
+    }
     OM_HIP(hipSetDevice(s->ctx->device));
     size_t n = (size_t)s->res_x * s->res_y;
     uint32_t *cnt = nullptr;

@@ -722,6 +722,14 @@ om_status build_all(om_scene *s)
         om::set_error("too many heliostats");
         return OM_ERR_UNSUPPORTED;
     }
+    if (!s->ctx) {
+        // host-only scene: geometry, grid, receiver and sun, no device state; Poisson
+        // microsamples are drawn on the GPU, so their counts stay 0 here
+        finish_micro(s);
+        flux_scale(s);
+        pack_heliostats(s);
+        return OM_OK;
+    }
     // device copies
     OM_HIP(hipSetDevice(s->ctx->device));
     if ((st = upload(s->d_cell_start, s->cell_start.data(), s->cell_start.size())) != OM_OK) return st;
@@ -850,7 +858,7 @@ static om_status scene_from_desc(om_ctx *ctx, const om_scene_desc *d, const om_p
 extern "C" om_status om_scene_create(om_ctx *ctx, const om_scene_desc *desc, const om_params *params,
                                      om_scene **out)
 {
-    if (!ctx || !desc || !params || !out) return OM_ERR_INVALID;
+    if (!desc || !params || !out) return OM_ERR_INVALID;
     *out = nullptr;
     return scene_from_desc(ctx, desc, params, out);
 }
@@ -873,8 +881,10 @@ extern "C" om_status om_scene_from_scn(om_ctx *ctx, const char *path, const floa
 extern "C" om_status om_scene_destroy(om_scene *s)
 {
     if (!s) return OM_ERR_INVALID;
-    (void)hipSetDevice(s->ctx->device);
-    free_scene_dev(s);
+    if (s->ctx) {
+        (void)hipSetDevice(s->ctx->device);
+        free_scene_dev(s);
+    }
     delete s;
     return OM_OK;
 }
@@ -905,6 +915,10 @@ extern "C" om_status om_scene_set_sun(om_scene *s, const float sun_dir[3], float
     s->sun = om::normalize(ld3(s->sun_in));
     aim_heliostats(s);
     flux_scale(s);
+    if (!s->ctx) {
+        pack_heliostats(s);
+        return OM_OK;
+    }
     OM_HIP(hipSetDevice(s->ctx->device));
     // the Poisson points are drawn in each heliostat's local frame, so they survive re-aiming
     return upload_geometry(s);
@@ -944,7 +958,7 @@ extern "C" om_status om_scene_export(const om_scene *s, const char *what, void *
             return OM_ERR_INVALID;
         }
         std::vector<float2> all((size_t)s->poisson_cap * s->H);
-        if (!all.empty())
+        if (!all.empty() && s->d_poisson)
             OM_HIP(hipMemcpy(all.data(), s->d_poisson, sizeof(float2) * all.size(), hipMemcpyDeviceToHost));
         std::vector<float2> out((size_t)s->max_micro * s->H, make_float2(0.0f, 0.0f));
         for (int h = 0; h < s->H; ++h)

@@ -1,0 +1,186 @@
+"""Synthetic heliostat fields and the named workloads of the benchmark.
+
+A scene description is a plain dict with the fields of ``om_scene_desc``
+(include/onemirror.h); tracing parameters are a dict with the fields of ``om_params``.
+
+World frame (main.cpp:71-73 of the reference): x = east, y = up, z = south.  The sun
+direction is the direction the light travels, (-sin Z sin A, -cos Z, sin Z cos A) for
+zenith Z and azimuth A measured from north through east.
+
+Workloads (heliostat counts, sizes, sunshape, microsamples, grid and map sizes):
+  small    100 heliostats 2x2 m, radial stagger over radii 40-80 m north of the tower,
+           flat 8x8 m receiver at 60 m facing north, pillbox 4.65 mrad, slope 2 mrad,
+           sun elevation 45 deg azimuth 180 deg, 16x16 Poisson microsamples x 64 rays,
+           64^3 grid, 128x128 map, trimmed mean 5x5 (20 %).
+  mid      5,000 heliostats 10x10 m, radii 100-800 m, Buie CSR 0.1 with a 2,048-bin CDF,
+           slope 2.5 mrad, 32x32 Poisson microsamples x 40 rays, 128^3 grid, 256x256 map,
+           seeded sun position.
+  face2face  the single-heliostat scene of userData/face2face_shadow.scn.
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional
+
+import numpy as np
+
+PILLBOX, BUIE = 0, 1
+LATTICE, POISSON = 0, 1
+
+
+def sun_direction(elevation_deg: float, azimuth_deg: float) -> np.ndarray:
+    """Direction the light travels for a sun at (elevation, azimuth)."""
+    z = math.radians(90.0 - elevation_deg)
+    a = math.radians(azimuth_deg)
+    return np.array([-math.sin(z) * math.sin(a), -math.cos(z), math.sin(z) * math.cos(a)],
+                    dtype=np.float32)
+
+
+def radial_stagger(n: int, r_min: float, r_max: float, spacing: float,
+                   sector_deg: float = 180.0, centre_deg: float = 0.0) -> np.ndarray:
+    """Heliostat ground positions (x, z) in rows of constant radius.
+
+    Rows are ``spacing * sqrt(3)/2`` apart; heliostats in a row are ``spacing`` apart along
+    the arc, alternate rows offset by half a spacing.  ``centre_deg`` is the azimuth of the
+    sector centre measured from north through east (0 = north of the tower).  The inner
+    rows are filled first; the last row is filled from the sector centre outwards.
+    Returns an (n, 2) float64 array, or fewer rows if the annulus is too small."""
+    dr = spacing * math.sqrt(3.0) / 2.0
+    out = []
+    k = 0
+    r = r_min
+    half = math.radians(sector_deg) / 2.0
+    c0 = math.radians(centre_deg)
+    while r <= r_max + 1e-9 and len(out) < n:
+        arc = 2.0 * half * r
+        cnt = max(1, int(arc // spacing))
+        step = 2.0 * half / cnt if sector_deg < 360.0 else 2.0 * math.pi / cnt
+        shift = 0.5 * step if (k % 2) else 0.0
+        if sector_deg < 360.0:
+            angs = -half + (np.arange(cnt) + 0.5) * step
+            angs = angs + (shift if k % 2 else 0.0)
+            angs = angs[angs < half]
+        else:
+            angs = np.arange(cnt) * step + shift
+        need = n - len(out)
+        if len(angs) > need:
+            order = np.argsort(np.abs(angs - (0.0 if sector_deg < 360.0 else 0.0)), kind="stable")
+            angs = np.sort(angs[order[:need]])
+        for a in angs:
+            az = c0 + a
+            # azimuth from north (-z) through east (+x)
+            out.append((r * math.sin(az), -r * math.cos(az)))
+        k += 1
+        r += dr
+    return np.array(out, dtype=np.float64)
+
+
+def fit_spacing(n: int, r_min: float, r_max: float, sector_deg: float, s_min: float) -> float:
+    """Largest spacing (>= s_min) that still fits n heliostats in the annulus sector."""
+    lo, hi = s_min, max(s_min * 8.0, r_max)
+    if len(radial_stagger(n, r_min, r_max, lo, sector_deg)) < n:
+        return lo
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if len(radial_stagger(n, r_min, r_max, mid, sector_deg)) >= n:
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
+def _pow2_ceil(x: float) -> float:
+    return float(2.0 ** math.ceil(math.log2(x)))
+
+
+def grid_for(pos: np.ndarray, size: np.ndarray, top: float, n: int) -> dict:
+    """Uniform n^3 grid over the heliostat boxes and up to ``top`` metres.
+
+    Cell edges are powers of two, so (p - grid_pos) / interval is exact; the grid is
+    centred on the field in x and z and starts at the ground (y = 0)."""
+    rad = np.sqrt(np.sqrt((size.astype(np.float64) ** 2).sum(1))) / 2.0
+    lo = (pos - rad[:, None]).min(0)
+    hi = (pos + rad[:, None]).max(0)
+    lo[1] = min(0.0, lo[1])
+    hi[1] = max(top, hi[1])
+    ext = (hi - lo) * 1.02 + 1.0
+    inter = np.array([_pow2_ceil(e / n) for e in ext])
+    span = inter * n
+    centre = (hi + lo) / 2.0
+    gpos = np.round((centre - span / 2.0) / inter) * inter  # multiple of the cell edge
+    gpos[1] = np.floor(lo[1] / inter[1]) * inter[1]
+    return {"grid_pos": gpos.astype(np.float32), "grid_size": span.astype(np.float32),
+            "grid_interval": inter.astype(np.float32)}
+
+
+def make_field(n: int, width: float, height: float, pivot: float, r_min: float, r_max: float,
+               sector_deg: float, jitter: float, seed: int, s_min: float) -> dict:
+    spacing = fit_spacing(n, r_min, r_max, sector_deg, s_min)
+    xz = radial_stagger(n, r_min, r_max, spacing, sector_deg)[:n]
+    if len(xz) < n:
+        raise ValueError("field annulus too small for the heliostat count")
+    rng = np.random.Generator(np.random.Philox(seed))
+    xz = xz + rng.uniform(-jitter, jitter, size=xz.shape)
+    pos = np.zeros((n, 3), np.float32)
+    pos[:, 0] = xz[:, 0]
+    pos[:, 1] = pivot
+    pos[:, 2] = xz[:, 1]
+    size = np.tile(np.array([width, 0.1, height], np.float32), (n, 1))
+    return {"helio_pos": pos, "helio_size": size,
+            "helio_gap": np.zeros((n, 2), np.float32),
+            "helio_matrix": np.ones((n, 2), np.int32)}
+
+
+def config_small(seed: int = 1) -> tuple:
+    """The 100-heliostat field (1,638,400 rays with a full 16x16 Poisson fill)."""
+    f = make_field(100, 2.0, 2.0, 1.2, 40.0, 80.0, 180.0, 0.1, seed, s_min=3.0)
+    desc = dict(f)
+    desc.update(recv_type=0, recv_pos=np.array([0.0, 60.0, 0.0], np.float32),
+                recv_size=np.array([8.0, 8.0, 0.0], np.float32),
+                recv_normal=np.array([0.0, 0.0, -1.0], np.float32), recv_face=0,
+                recv_pixel_length=np.float32(8.0 / 128.0),
+                sun_dir=sun_direction(45.0, 180.0), dni=np.float32(1000.0))
+    desc.update(grid_for(f["helio_pos"], f["helio_size"], 64.0, 64))
+    prm = dict(seed=0x0123456789ABCDEF, scene_index=0, sunshape=PILLBOX,
+               pillbox_half_angle=4.65e-3, csr=0.1, buie_bins=1024, slope_sigma=2e-3,
+               micro_mode=POISSON, helio_pixel_length=2.0 / 16.0, poisson_k=8,
+               rays_per_sample=64, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
+    return desc, prm
+
+
+def mid_sun(seed: int = 2) -> np.ndarray:
+    rng = np.random.Generator(np.random.Philox(seed))
+    elev = float(rng.uniform(30.0, 70.0))
+    azim = float(rng.uniform(120.0, 240.0))
+    return sun_direction(elev, azim)
+
+
+def config_mid(seed: int = 2, n_helio: int = 5000) -> tuple:
+    """The 5,000-heliostat field (about 2.05e8 rays)."""
+    f = make_field(n_helio, 10.0, 10.0, 6.0, 100.0, 800.0, 180.0, 0.1, seed, s_min=14.5)
+    desc = dict(f)
+    desc.update(recv_type=0, recv_pos=np.array([0.0, 150.0, 0.0], np.float32),
+                recv_size=np.array([25.6, 25.6, 0.0], np.float32),
+                recv_normal=np.array([0.0, 0.0, -1.0], np.float32), recv_face=0,
+                recv_pixel_length=np.float32(0.1),
+                sun_dir=mid_sun(seed), dni=np.float32(1000.0))
+    desc.update(grid_for(f["helio_pos"], f["helio_size"], 170.0, 128))
+    prm = dict(seed=0x00C0FFEE12345678, scene_index=0, sunshape=BUIE,
+               pillbox_half_angle=4.65e-3, csr=0.1, buie_bins=2048, slope_sigma=2.5e-3,
+               micro_mode=POISSON, helio_pixel_length=10.0 / 32.0, poisson_k=8,
+               rays_per_sample=40, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
+    return desc, prm
+
+
+def subset(desc: dict, idx) -> dict:
+    """Description restricted to heliostats ``idx`` (same grid, receiver and sun)."""
+    d = dict(desc)
+    for k in ("helio_pos", "helio_size", "helio_gap", "helio_matrix"):
+        d[k] = np.ascontiguousarray(desc[k][idx])
+    return d
+
+
+def with_lattice(prm: dict) -> dict:
+    p = dict(prm)
+    p["micro_mode"] = LATTICE
+    return p

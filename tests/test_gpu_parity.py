@@ -1,0 +1,204 @@
+"""GPU parity: the HIP path against the CPU oracle on the same Philox streams.
+
+Bar: bit-exact per-ray shade/block/pixel indices, integer pixel counts and fixed-point
+energies; FP32 flux maps equal to the oracle's conversion of the same integers.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from onemirror_amd import fields as F
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def om():
+    import onemirror_amd
+    return onemirror_amd
+
+
+def _records_equal(g, o, label):
+    for k in ("shade", "block", "pixel"):
+        a, b = g[k], o[k]
+        assert a.shape == b.shape, (label, k, a.shape, b.shape)
+        bad = np.nonzero(a != b)[0]
+        assert bad.size == 0, f"{label}: {k} differs at {bad.size} rays, first {bad[:5]}: " \
+                              f"gpu {a[bad[:5]]} oracle {b[bad[:5]]}"
+
+
+@pytest.mark.parametrize("fn", range(10))
+def test_device_math_bitexact(om, fn):
+    rng = np.random.default_rng(fn)
+    if fn == 0:
+        x = rng.integers(0, 2**32, 100000, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    elif fn in (1, 2, 8):
+        x = oracle.math("u01", rng.integers(0, 2**32, 100000, dtype=np.uint64).astype(np.uint32))
+    elif fn in (3, 4):
+        x = np.concatenate([rng.normal(0, 0.01, 50000), rng.uniform(-4, 4, 50000)]).astype(np.float32)
+    elif fn == 5:
+        x = np.exp(rng.uniform(-40, 40, 100000)).astype(np.float32)
+    elif fn == 6:
+        x = rng.uniform(-30, 30, 100000).astype(np.float32)
+    elif fn == 7:
+        x = rng.uniform(-0.9999999, 0.9999999, 100000).astype(np.float32)
+    else:
+        x = np.abs(rng.normal(0, 100, 100000)).astype(np.float32)
+    names = ["u01", "sin2pi", "cos2pi", "sin_small", "cos_small", "log", "exp", "erfinv",
+             "gauss", "sqrt"]
+    want = oracle.math(names[fn], x.view(np.uint32) if fn == 0 else x)
+    got = om.default_context().math_eval(fn, x)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), \
+        f"{names[fn]}: {(got != want).sum()} mismatches"
+
+
+@pytest.fixture(scope="module")
+def small(om):
+    d, p = F.config_small()
+    return d, p, om.SolarScene(d, p), oracle.OracleScene(d, p)
+
+
+def test_small_scene_counts(small):
+    d, p, g, o = small
+    mc = g.export("micro_count", 4 * o.n_helio, np.int32)
+    assert np.array_equal(mc, o.micro()["count"])
+    assert g.info["total_rays"] == o.total_rays
+
+
+def test_small_poisson_points_bitexact(small):
+    d, p, g, o = small
+    cap = g.info["max_micro"]
+    pts = g.export("poisson", 8 * cap * o.n_helio, np.float32).reshape(o.n_helio, cap, 2)
+    mc = o.micro()["count"]
+    for h in range(o.n_helio):
+        want = o.poisson(h)
+        assert np.array_equal(pts[h, : mc[h]].view(np.uint32), want.view(np.uint32)), h
+
+
+def test_small_records_bitexact(small):
+    d, p, g, o = small
+    gr = g.trace_records()
+    orc = o.trace_f32(records=True)
+    _records_equal(gr, orc, "small/poisson")
+
+
+def test_small_lattice_records_bitexact(om):
+    d, p = F.config_small()
+    p = F.with_lattice(p)
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    _records_equal(g.trace_records(), o.trace_f32(records=True), "small/lattice")
+
+
+def test_small_maps_bitexact(small):
+    import torch
+    d, p, g, o = small
+    maps = g.trace_plain()
+    torch.cuda.synchronize()
+    orc = o.trace_f32()
+    counts = maps.counts.cpu().numpy().view(np.uint32)
+    energy = maps.energy.cpu().numpy().view(np.uint64)
+    assert np.array_equal(counts, orc["counts"])
+    assert np.array_equal(energy, orc["energy"])
+    scale, total = o.flux_scale()
+    raw = oracle.energy_to_float(orc["energy"])
+    flux = (raw * np.float32(scale)).astype(np.float32)
+    assert np.array_equal(maps.flux.cpu().numpy().view(np.uint32), flux.view(np.uint32))
+    sm = oracle.trimmed_mean(raw, p["smooth_radius"], p["smooth_ratio"]) * np.float32(scale)
+    assert np.array_equal(maps.flux_smooth.cpu().numpy().view(np.uint32),
+                          sm.astype(np.float32).view(np.uint32))
+
+
+def test_trace_is_deterministic(small):
+    import torch
+    d, p, g, o = small
+    a = g.trace_plain()
+    b = g.trace_plain()
+    torch.cuda.synchronize()
+    assert torch.equal(a.energy, b.energy) and torch.equal(a.counts, b.counts)
+
+
+def test_heliostat_accumulation_equals_plain(om, small):
+    import torch
+    d, p, g, o = small
+    full = g.trace_plain()
+    acc = g.new_maps(smooth=False)
+    for h0 in range(0, o.n_helio, 7):
+        g.trace_heliostats(h0, min(h0 + 7, o.n_helio), acc)
+    torch.cuda.synchronize()
+    assert torch.equal(full.energy, acc.energy) and torch.equal(full.counts, acc.counts)
+
+
+def test_results_with_given_normal(om, small):
+    d, p, g, o = small
+    res = om.results_with_given_normal(g)
+    orc = o.trace_f32()
+    scale, total = o.flux_scale()
+    raw = oracle.energy_to_float(orc["energy"])
+    flux = (raw * np.float32(scale)).astype(np.float32)
+    sm = (oracle.trimmed_mean(raw, p["smooth_radius"], p["smooth_ratio"]) * np.float32(scale)).astype(np.float32)
+    assert np.array_equal(res["flux"], flux)
+    assert res["ratio"] == oracle.intercept_ratio(flux, total)
+    assert res["ratio_smooth"] == oracle.intercept_ratio(sm, total)
+    assert 0.5 < res["ratio"] <= 1.0
+
+
+def test_smoother_matches_oracle(om):
+    import torch
+    rng = np.random.default_rng(3)
+    img = rng.gamma(2.0, 1.0, (61, 77)).astype(np.float32)
+    img[rng.random(img.shape) < 0.3] = 0.0
+    for r, ratio in ((2, 0.2), (5, 0.02), (1, 0.0), (3, 0.3)):
+        t = torch.as_tensor(img).cuda()
+        om.ImageSmoother.image_smooth(t, r, ratio, 77, 61)
+        torch.cuda.synchronize()
+        want = oracle.trimmed_mean(img, r, ratio)
+        assert np.array_equal(t.cpu().numpy().view(np.uint32), want.view(np.uint32)), (r, ratio)
+
+
+def test_face2face_scene(om):
+    path = os.path.join(HERE, "golden", "face2face_shadow.scn")
+    sun = F.sun_direction(40.0, 160.0)
+    p = dict(om.default_params())
+    p.update(rays_per_sample=256, helio_pixel_length=0.05, slope_sigma=1e-3, seed=7)
+    g = om.SolarScene.from_scn(path, sun, 1000.0, p)
+    d = om.scn_parse(path)
+    d["sun_dir"] = sun
+    o = oracle.OracleScene(d, p)
+    assert g.info["res_x"] == 400 and g.info["res_y"] == 400
+    _records_equal(g.trace_records(), o.trace_f32(records=True), "face2face")
+
+
+def test_set_sun_reaims(om, small):
+    d, p, _, _ = small
+    g = om.SolarScene(d, F.with_lattice(p))
+    sun = F.sun_direction(30.0, 200.0)
+    g.set_sun(sun, 900.0)
+    d2 = dict(d)
+    d2["sun_dir"] = sun
+    d2["dni"] = np.float32(900.0)
+    o = oracle.OracleScene(d2, F.with_lattice(p))
+    _records_equal(g.trace_records(), o.trace_f32(records=True), "set_sun")
+    assert g.info["flux_scale"] == o.flux_scale()[0]
+
+
+@pytest.fixture(scope="module")
+def mid(om):
+    d, p = F.config_mid()
+    return d, p, om.SolarScene(d, p), oracle.OracleScene(d, p)
+
+
+def test_mid_poisson_counts(mid):
+    d, p, g, o = mid
+    mc = g.export("micro_count", 4 * o.n_helio, np.int32)
+    assert np.array_equal(mc, o.micro()["count"])
+
+
+@pytest.mark.parametrize("h0", [0, 2400, 4900])
+def test_mid_records_bitexact(mid, h0):
+    d, p, g, o = mid
+    h1 = h0 + 40
+    _records_equal(g.trace_records(h0, h1), o.trace_f32(h0, h1, records=True), f"mid[{h0}:{h1}]")

@@ -1,0 +1,116 @@
+"""Pins of the oracle's arithmetic: Philox known answers, project-written transcendental
+functions against libm / scipy in double precision, and the sampling distributions
+against their closed-form CDFs (Kolmogorov-Smirnov)."""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy import special, stats
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_philox_known_answers():
+    # Random123 known-answer vectors for philox4x32-10 (committed fixture)
+    with open(os.path.join(HERE, "golden", "philox4x32_10_kat.json")) as f:
+        kat = json.load(f)
+    for case in kat:
+        ctr = [int(x, 16) for x in case["ctr"]]
+        key = [int(x, 16) for x in case["key"]]
+        want = [int(x, 16) for x in case["out"]]
+        got = oracle.philox([ctr], key)[0]
+        assert [int(v) for v in got] == want, case
+
+
+def test_philox_counter_sensitivity():
+    # every counter and key word changes every output word (avalanche of a bijection)
+    base = oracle.philox([[1, 2, 3, 4]], [5, 6])[0]
+    for w in range(4):
+        c = [1, 2, 3, 4]
+        c[w] ^= 1
+        o = oracle.philox([c], [5, 6])[0]
+        assert np.all(o != base)
+    o = oracle.philox([[1, 2, 3, 4]], [5, 7])[0]
+    assert np.all(o != base)
+
+
+def test_u01_open_interval_and_exact():
+    x = np.array([0, 0x1FF, 0x200, 0xFFFFFFFF, 0x80000000], np.uint32)
+    u = oracle.math("u01", x)
+    assert u[0] == np.float32(2.0 ** -24) and u[1] == u[0]
+    assert u[2] == np.float32(3 * 2.0 ** -24)
+    assert u[3] == np.float32(1 - 2.0 ** -24)
+    assert u[4] == np.float32(0.5 + 2.0 ** -24)
+    rng = np.random.default_rng(0)
+    xs = rng.integers(0, 2**32, 200000, dtype=np.uint64).astype(np.uint32)
+    us = oracle.math("u01", xs)
+    assert us.min() > 0 and us.max() < 1
+    # exact: (2 (x >> 9) + 1) 2^-24
+    assert np.array_equal(us.astype(np.float64), ((xs >> 9).astype(np.float64) * 2 + 1) * 2.0 ** -24)
+
+
+def test_sincos2pi_accuracy():
+    u = np.random.default_rng(1).random(300000).astype(np.float32)
+    u = np.concatenate([u, np.array([0.25, 0.5, 0.75, 0.125, 1 - 2**-24], np.float32)])
+    ang = 2 * np.pi * u.astype(np.float64)
+    s, c = oracle.math("sin2pi", u), oracle.math("cos2pi", u)
+    assert np.abs(s - np.sin(ang)).max() < 1.5e-7
+    assert np.abs(c - np.cos(ang)).max() < 1.5e-7
+    # exact quadrant points
+    assert oracle.math("sin2pi", np.float32([0.25]))[0] == 1.0
+    assert oracle.math("cos2pi", np.float32([0.5]))[0] == -1.0
+
+
+def test_sincos_small_accuracy():
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.normal(0, 0.005, 200000), rng.uniform(-3, 3, 50000)]).astype(np.float32)
+    xd = x.astype(np.float64)
+    s, c = oracle.math("sin_small", x), oracle.math("cos_small", x)
+    small = np.abs(x) < 0.25
+    assert np.all(np.abs(s[small] - np.sin(xd[small])) <= 1.2e-7 * np.maximum(np.abs(np.sin(xd[small])), 1e-30) + 1e-45)
+    assert np.abs(c[small] - np.cos(xd[small])).max() < 1.2e-7
+    assert np.abs(s[~small] - np.sin(xd[~small])).max() < 2e-6
+    assert np.abs(c[~small] - np.cos(xd[~small])).max() < 2e-6
+
+
+def test_log_exp_pow_accuracy():
+    rng = np.random.default_rng(3)
+    x = np.exp(rng.uniform(-80, 80, 200000)).astype(np.float32)
+    x = x[np.isfinite(x) & (x > 1.2e-38)]
+    lg = oracle.math("log", x).astype(np.float64)
+    ref = np.log(x.astype(np.float64))
+    assert np.all(np.abs(lg - ref) <= 2.5e-7 * np.abs(ref) + 1.2e-7)
+    y = rng.uniform(-80, 80, 200000).astype(np.float32)
+    ex = oracle.math("exp", y).astype(np.float64)
+    assert np.max(np.abs(ex / np.exp(y.astype(np.float64)) - 1)) < 2.5e-7
+
+
+def test_erfinv_against_scipy():
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(-1, 1, 200000),
+                        1 - 2.0 ** -np.arange(2, 24), -1 + 2.0 ** -np.arange(2, 24)]).astype(np.float32)
+    x = x[np.abs(x) < 1]
+    got = oracle.math("erfinv", x).astype(np.float64)
+    ref = special.erfinv(x.astype(np.float64))
+    assert np.all(np.abs(got - ref) <= 6e-7 * np.abs(ref) + 1e-12)
+
+
+def _u01_stream(n, seed):
+    ctr = np.zeros((n, 4), np.uint32)
+    ctr[:, 0] = np.arange(n)
+    ctr[:, 3] = 0xABC
+    return oracle.math("u01", oracle.philox(ctr, [seed, 0])[:, 0])
+
+
+def test_gauss_ks():
+    z = oracle.math("gauss", _u01_stream(200000, 11)).astype(np.float64)
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+def test_uniform_ks():
+    u = _u01_stream(200000, 12).astype(np.float64)
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
