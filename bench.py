@@ -51,6 +51,9 @@ def main() -> int:
     ap.add_argument("--cpu-every", type=int, default=5,
                     help="CPU baseline traces every n-th heliostat of the field")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run an elementwise copy of MIB MiB (reads and writes MIB MiB "
+                         "each): the reference kernel that calibrates HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -80,6 +83,16 @@ def main() -> int:
           f"grid {info['grid_num']} nnz {info['grid_nnz']}", file=sys.stderr, flush=True)
     maps = scene.new_maps(smooth=True)
     stream = torch.cuda.current_stream()
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 4
+        xs = torch.rand(n, device="cuda")
+        ys = torch.empty_like(xs)
+        for _ in range(3):
+            torch.mul(xs, 1.0, out=ys)
+        torch.cuda.synchronize()
+        del xs, ys
+        print(f"# copy probe: 3 x elementwise copy of {args.copy_probe} MiB", file=sys.stderr,
+              flush=True)
 
     def step(ev0=None, ev1=None):
         maps.counts.zero_()

@@ -79,7 +79,13 @@ typedef struct {
     float reflectivity;        /* reflected_rate                                          */
     int   smooth_radius;       /* trimmed-mean window radius (Gtest.cu:50)                */
     float smooth_ratio;        /* trimmed fraction (Gtest.cu:51)                          */
+    int   accumulate;          /* OM_ACC_*: how hits reach the maps (results identical)   */
 } om_params;
+
+/* OM_ACC_XCD: every XCD adds into its own copy of the maps with atomics that resolve in
+ * its L2, and a reduction kernel sums the copies; OM_ACC_DEVICE: device-scope atomics
+ * straight into the output maps. */
+enum { OM_ACC_XCD = 0, OM_ACC_DEVICE = 1 };
 
 /* Output maps of one receiver, row-major [res_y][res_x], all device memory. */
 typedef struct {

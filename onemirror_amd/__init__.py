@@ -69,7 +69,11 @@ class Params(ctypes.Structure):
         ("reflectivity", ctypes.c_float),
         ("smooth_radius", ctypes.c_int),
         ("smooth_ratio", ctypes.c_float),
+        ("accumulate", ctypes.c_int),
     ]
+
+
+ACC_XCD, ACC_DEVICE = 0, 1
 
 
 class Flux(ctypes.Structure):

@@ -40,13 +40,17 @@ struct TraceArgs {
     float pillbox;
     float buie_int465, buie_A, buie_B, buie_C, buie_inv_gp1;
     int nbins;
-    const float *cdf_x, *cdf_y, *slope;
+    const float4 *buie_bins;    // [nbins]: (cdf_y[i], cdf_y[i+1], slope[i], cdf_x[i])
+    const int *buie_guide;      // [nbins]: a bin index at or below the bin of j / gscale
+    float buie_gscale;          // nbins / int465
     float slope_sigma;
     uint32_t key0, key1, w3;
     int L;
     float pitch, pitch2;
     // grid (grid.h:25-49) + occupied index range
     float gpx, gpy, gpz, gix, giy, giz;
+    float inv_gix, inv_giy, inv_giz; // exact reciprocals when grid_pow2
+    int grid_pow2;              // all cell edges are powers of two
     int gnx, gny, gnz;
     int occ_lo[3], occ_hi[3];
     const int *cell_start;
@@ -68,6 +72,11 @@ struct TraceArgs {
     unsigned long long *energy;
     int32_t *rec_shade, *rec_block, *rec_pixel;
     long long rec_base;
+    // per-XCD partial maps (OM_ACC_XCD): [16][npx]
+    uint32_t *part_counts;
+    unsigned long long *part_energy;
+    int npx;
+    int acc_xcd;
 };
 
 struct DevBuf {
@@ -82,6 +91,7 @@ struct om_ctx {
     hipStream_t stream = nullptr;
     om::DevBuf scratch;     // raw maps for finalize/smooth
     om::DevBuf scratch2;
+    om::DevBuf parts;       // per-XCD partial maps
 };
 
 struct om_scene {
@@ -123,7 +133,8 @@ struct om_scene {
     om::HelioRec *d_helio = nullptr;
     float4 *d_geo = nullptr;
     int *d_cell_start = nullptr, *d_cell_items = nullptr;
-    float *d_cdf_x = nullptr, *d_cdf_y = nullptr, *d_slope = nullptr;
+    float4 *d_buie_bins = nullptr;
+    int *d_buie_guide = nullptr;
     float2 *d_poisson = nullptr;
     int *d_mcount = nullptr;
     std::vector<int2> tiles;
@@ -144,6 +155,7 @@ om_status launch_smooth(const float *in, float *out, int w, int h, int radius, f
                         float scale, hipStream_t st);
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st);
 om_status ensure_buf(DevBuf &b, size_t bytes);
+om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st);
 } // namespace om
 
 #define OM_HIP(call)                                                                       \

@@ -29,18 +29,18 @@ __device__ __forceinline__ int xcd_chunk(int b, int n)
     return x * q + min(x, rmd) + i;
 }
 
-// samplelights::max_less_index (scene_instance_process.cu:158-173)
-__device__ __forceinline__ int max_less_index(const float *cdf_y, float value, int n)
+// Buie bin of value v: the largest i with cdf_y[i] <= v, which is what
+// samplelights::max_less_index (scene_instance_process.cu:158-173) returns for a strictly
+// increasing CDF.  A guide entry starts the walk next to it.
+__device__ __forceinline__ float4 buie_bin(const float4 *bins, const int *guide, float gscale, float v,
+                                           int n)
 {
-    int left = 0, right = n;
-    while (left <= right) {
-        int mid = (left + right) >> 1;
-        float c = cdf_y[mid];
-        if (value > c) left = mid + 1;
-        else if (value < c) right = mid - 1;
-        else return mid;
-    }
-    return right;
+    int j = min(max(trunc_i(v * gscale), 0), n - 1);
+    int i = guide[j];
+    float4 b = bins[i];
+    while (b.x > v) b = bins[--i];
+    while (b.y <= v) b = bins[++i];
+    return b;
 }
 
 // NotCollision + Intersect (rectgrid_raytracing.cu:21-117): Amanatides-Woo walk from the
@@ -48,16 +48,22 @@ __device__ __forceinline__ int max_less_index(const float *cdf_y, float value, i
 // parallelogram the ray hits.  The ray's own heliostat is skipped (the reference rejects it
 // through t < 1e-6, which holds in exact arithmetic).  The walk also ends, with the same
 // result, once the cell index on an axis has passed the occupied index range in the
-// direction of travel: every later cell is empty.
+// direction of travel: every later cell is empty.  With power-of-two cell edges the start
+// cell uses the exact reciprocal (x / 2^k == x * 2^-k).
 __device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
 {
     float rx = o.x - a.gpx, ry = o.y - a.gpy, rz = o.z - a.gpz;
-    int gx = trunc_i(rx / a.gix), gy = trunc_i(ry / a.giy), gz = trunc_i(rz / a.giz);
+    int gx, gy, gz;
+    if (a.grid_pow2) {
+        gx = trunc_i(rx * a.inv_gix);
+        gy = trunc_i(ry * a.inv_giy);
+        gz = trunc_i(rz * a.inv_giz);
+    } else {
+        gx = trunc_i(rx / a.gix);
+        gy = trunc_i(ry / a.giy);
+        gz = trunc_i(rz / a.giz);
+    }
     int sx = (d.x >= 0.0f) ? 1 : -1, sy = (d.y >= 0.0f) ? 1 : -1, sz = (d.z >= 0.0f) ? 1 : -1;
-    float mx = abs_divide(cal_tmax(d.x, a.gix, gx, rx), d.x);
-    float my = abs_divide(cal_tmax(d.y, a.giy, gy, ry), d.y);
-    float mz = abs_divide(cal_tmax(d.z, a.giz, gz, rz), d.z);
-    float dx = abs_divide(a.gix, d.x), dy = abs_divide(a.giy, d.y), dz = abs_divide(a.giz, d.z);
     // admissible index window per axis: inside the grid and not past the occupied range
     int lox = sx > 0 ? 0 : a.occ_lo[0], hix = sx > 0 ? a.occ_hi[0] : a.gnx - 1;
     int loy = sy > 0 ? 0 : a.occ_lo[1], hiy = sy > 0 ? a.occ_hi[1] : a.gny - 1;
@@ -65,6 +71,10 @@ __device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
     if ((sx > 0 ? gx > hix : gx < lox) || (sy > 0 ? gy > hiy : gy < loy) ||
         (sz > 0 ? gz > hiz : gz < loz))
         return -1;
+    float mx = abs_divide(cal_tmax(d.x, a.gix, gx, rx), d.x);
+    float my = abs_divide(cal_tmax(d.y, a.giy, gy, ry), d.y);
+    float mz = abs_divide(cal_tmax(d.z, a.giz, gz, rz), d.z);
+    float dx = abs_divide(a.gix, d.x), dy = abs_divide(a.giy, d.y), dz = abs_divide(a.giz, d.z);
     for (;;) {
         if (mx < my) {
             if (mx < mz) {
@@ -101,18 +111,24 @@ __device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
     }
 }
 
+// XCD (accelerator complex die) that runs the calling wave
+__device__ __forceinline__ uint32_t xcc_id()
+{
+    uint32_t x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return x & 15u;
+}
+
 template <int SUN, bool POISSON, bool RECORD>
 __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
 {
-    extern __shared__ float lds[];
-    float *s_cdf_y = lds;
-    float *s_slope = lds + (a.nbins + 1);
-    float *s_cdf_x = lds + (2 * a.nbins + 1);
+    extern __shared__ float4 lds4[];
+    float4 *s_bins = lds4;
+    int *s_guide = (int *)(lds4 + a.nbins);
     if (SUN == OM_SUN_BUIE) {
-        for (int i = threadIdx.x; i <= a.nbins; i += WG) {
-            s_cdf_y[i] = a.cdf_y[i];
-            s_cdf_x[i] = a.cdf_x[i];
-            if (i < a.nbins) s_slope[i] = a.slope[i];
+        for (int i = threadIdx.x; i < a.nbins; i += WG) {
+            s_bins[i] = a.buie_bins[i];
+            s_guide[i] = a.buie_guide[i];
         }
         __syncthreads();
     }
@@ -133,17 +149,36 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
     const v3 hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
     const int L = a.L;
     const int nr = hr->mcount * L;
-    for (int k = 0; k < a.rays_per_thread; ++k) {
-        const int r = tl.y + k * WG + (int)threadIdx.x;
+    const bool one_row = hr->rows == 1, one_col = hr->cols == 1;
+    uint32_t *cnt = a.counts;
+    unsigned long long *eng = a.energy;
+    if (a.acc_xcd) {
+        const uint32_t x = xcc_id();
+        cnt = a.part_counts + (size_t)x * a.npx;
+        eng = a.part_energy + (size_t)x * a.npx;
+    }
+    // ray index r = m * L + l, stepped by WG without division
+    int r = tl.y + (int)threadIdx.x;
+    int m = r / L;
+    int l = r - m * L;
+    const int dm = WG / L, dl = WG - (WG / L) * L;
+    for (int k = 0; k < a.rays_per_thread; ++k, r += WG) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= L) {
+                l -= L;
+                ++m;
+            }
+        }
         if (r >= nr) break;
-        const int m = r / L;
-        const int l = r - m * L;
         // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
         v3 loc;
         if (POISSON) {
             float2 pt = a.poisson[(size_t)h * a.poisson_cap + m];
-            int br = trunc_i(pt.y / hr->subrow_len);
-            int bc = trunc_i(pt.x / hr->subcol_len);
+            // valid points lie inside (0, width) x (0, height): one facet -> block 0
+            int br = one_row ? 0 : trunc_i(pt.y / hr->subrow_len);
+            int bc = one_col ? 0 : trunc_i(pt.x / hr->subcol_len);
             loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
         } else {
             int row = m / hr->cols_all, col = m - row * hr->cols_all;
@@ -160,8 +195,8 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         } else {
             float v = u01(x.x) * a.buie_A;
             if (v < a.buie_int465) {
-                int id = max_less_index(s_cdf_y, v, a.nbins);
-                th = (v - s_cdf_y[id]) * s_slope[id] + s_cdf_x[id];
+                float4 b = buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
+                th = (v - b.x) * b.z + b.w;
             } else {
                 th = pow_((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
             }
@@ -206,10 +241,34 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         if (px >= 0) {
             float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
             uint32_t q = (uint32_t)(e * 2147483648.0f);
-            atomicAdd(a.counts + px, 1u);
-            atomicAdd(a.energy + px, (unsigned long long)q);
+            if (a.acc_xcd) {
+                // this XCD's copy: the atomics resolve in its L2
+                __hip_atomic_fetch_add(cnt + px, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(eng + px, (unsigned long long)q, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                atomicAdd(cnt + px, 1u);
+                atomicAdd(eng + px, (unsigned long long)q);
+            }
         }
     }
+}
+
+// sum of the per-XCD partial maps into the output maps (which it adds to)
+__global__ void reduce_parts_kernel(const uint32_t *pc, const unsigned long long *pe, uint32_t *counts,
+                                    unsigned long long *energy, int npx)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    uint32_t c = counts[i];
+    unsigned long long e = energy[i];
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+        c += pc[(size_t)x * npx + i];
+        e += pe[(size_t)x * npx + i];
+    }
+    counts[i] = c;
+    energy[i] = e;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -447,7 +506,7 @@ om_status launch_poisson(om_scene *s, hipStream_t st)
 om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st)
 {
     if (a.ntiles <= 0) return OM_OK;
-    size_t lds = (s->prm.sunshape == OM_SUN_BUIE) ? sizeof(float) * (3 * (size_t)a.nbins + 2) : 0;
+    size_t lds = (s->prm.sunshape == OM_SUN_BUIE) ? (sizeof(float4) + sizeof(int)) * (size_t)a.nbins : 0;
     dim3 grid(a.ntiles), block(WG);
     bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
 #define OM_TRACE_LAUNCH(SUN, PO, REC) hipLaunchKernelGGL((trace_kernel<SUN, PO, REC>), grid, block, lds, st, a)
@@ -545,6 +604,42 @@ static void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1)
 
 using namespace om;
 
+namespace om {
+// trace heliostats [h0, h1) into out (cleared first when clear is set)
+om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st)
+{
+    TraceArgs a = make_trace_args(s);
+    int t0, t1;
+    tile_range(s, h0, h1, t0, t1);
+    a.tile_base = t0;
+    a.ntiles = t1 - t0;
+    const int npx = s->res_x * s->res_y;
+    om_status rc;
+    if (clear && (rc = clear_maps(out, npx, st)) != OM_OK) return rc;
+    if (s->prm.accumulate == OM_ACC_XCD) {
+        const size_t n16 = 16 * (size_t)npx;
+        if ((rc = ensure_buf(s->ctx->parts, n16 * (sizeof(uint32_t) + sizeof(unsigned long long)))) != OM_OK)
+            return rc;
+        a.part_counts = (uint32_t *)s->ctx->parts.p;
+        a.part_energy = (unsigned long long *)((char *)s->ctx->parts.p + n16 * sizeof(uint32_t));
+        a.npx = npx;
+        a.acc_xcd = 1;
+        dim3 g((unsigned)((n16 + WG - 1) / WG)), b(WG);
+        hipLaunchKernelGGL(fill_u32_kernel, g, b, 0, st, a.part_counts, (long long)n16);
+        hipLaunchKernelGGL(fill_u64_kernel, g, b, 0, st, a.part_energy, (long long)n16);
+        OM_HIP(hipGetLastError());
+        if ((rc = launch_trace(s, a, false, st)) != OM_OK) return rc;
+        hipLaunchKernelGGL(reduce_parts_kernel, dim3((npx + WG - 1) / WG), dim3(WG), 0, st, a.part_counts,
+                           a.part_energy, out->counts, (unsigned long long *)out->energy, npx);
+        OM_HIP(hipGetLastError());
+        return OM_OK;
+    }
+    a.counts = out->counts;
+    a.energy = (unsigned long long *)out->energy;
+    return launch_trace(s, a, false, st);
+}
+} // namespace om
+
 extern "C" om_status om_trace_heliostats(om_scene *s, int h0, int h1, const om_flux *out, void *stream)
 {
     if (!s) return OM_ERR_INVALID;
@@ -559,14 +654,7 @@ extern "C" om_status om_trace_heliostats(om_scene *s, int h0, int h1, const om_f
         return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
-    TraceArgs a = make_trace_args(s);
-    int t0, t1;
-    tile_range(s, h0, h1, t0, t1);
-    a.tile_base = t0;
-    a.ntiles = t1 - t0;
-    a.counts = out->counts;
-    a.energy = (unsigned long long *)out->energy;
-    return launch_trace(s, a, false, (hipStream_t)stream);
+    return trace_into(s, h0, h1, out, false, (hipStream_t)stream);
 }
 
 extern "C" om_status om_flux_finalize(om_scene *s, const om_flux *out, void *stream)
@@ -604,14 +692,7 @@ extern "C" om_status om_trace_plain(om_scene *s, const om_flux *out, void *strea
         return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
-    hipStream_t hs = (hipStream_t)stream;
-    if ((st = clear_maps(out, (long long)s->res_x * s->res_y, hs)) != OM_OK) return st;
-    TraceArgs a = make_trace_args(s);
-    a.tile_base = 0;
-    a.ntiles = (int)s->tiles.size();
-    a.counts = out->counts;
-    a.energy = (unsigned long long *)out->energy;
-    if ((st = launch_trace(s, a, false, hs)) != OM_OK) return st;
+    if ((st = trace_into(s, 0, s->H, out, true, (hipStream_t)stream)) != OM_OK) return st;
     if (out->flux || out->flux_smooth) return om_flux_finalize(s, out, stream);
     return OM_OK;
 }

@@ -97,6 +97,7 @@ extern "C" void om_default_params(om_params *p)
     p->reflectivity = 0.88f;
     p->smooth_radius = 5;
     p->smooth_ratio = 0.02f;
+    p->accumulate = OM_ACC_XCD;
 }
 
 extern "C" om_status om_ctx_create(int device, om_ctx **out)
@@ -122,6 +123,7 @@ extern "C" om_status om_ctx_destroy(om_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->scratch.p) (void)hipFree(ctx->scratch.p);
     if (ctx->scratch2.p) (void)hipFree(ctx->scratch2.p);
+    if (ctx->parts.p) (void)hipFree(ctx->parts.p);
     delete ctx;
     return OM_OK;
 }
@@ -659,9 +661,8 @@ void free_scene_dev(om_scene *s)
     dfree(s->d_geo);
     dfree(s->d_cell_start);
     dfree(s->d_cell_items);
-    dfree(s->d_cdf_x);
-    dfree(s->d_cdf_y);
-    dfree(s->d_slope);
+    dfree(s->d_buie_bins);
+    dfree(s->d_buie_guide);
     dfree(s->d_poisson);
     dfree(s->d_mcount);
     dfree(s->d_tiles);
@@ -689,6 +690,10 @@ om_status validate(const om_scene_desc *d, const om_params *p)
     }
     if (p->sunshape == OM_SUN_BUIE && (p->buie_bins <= 0 || p->buie_bins > 8192 || !(p->csr > 0.0f))) {
         om::set_error("buie_bins must be in 1..8192 and csr positive");
+        return OM_ERR_INVALID;
+    }
+    if (p->accumulate != OM_ACC_XCD && p->accumulate != OM_ACC_DEVICE) {
+        om::set_error("unknown accumulation mode");
         return OM_ERR_INVALID;
     }
     if (p->micro_mode == OM_MICRO_POISSON && p->poisson_k <= 0) {
@@ -734,9 +739,23 @@ om_status build_all(om_scene *s)
     OM_HIP(hipSetDevice(s->ctx->device));
     if ((st = upload(s->d_cell_start, s->cell_start.data(), s->cell_start.size())) != OM_OK) return st;
     if ((st = upload(s->d_cell_items, s->cell_items.data(), s->cell_items.size())) != OM_OK) return st;
-    if ((st = upload(s->d_cdf_x, s->cdf_x.data(), s->cdf_x.size())) != OM_OK) return st;
-    if ((st = upload(s->d_cdf_y, s->cdf_y.data(), s->cdf_y.size())) != OM_OK) return st;
-    if ((st = upload(s->d_slope, s->slope.data(), s->slope.size())) != OM_OK) return st;
+    {
+        // Buie lookup tables: packed bins and a guide from the value scale to a bin
+        int n = s->nbins;
+        std::vector<float4> bins(n);
+        for (int i = 0; i < n; ++i)
+            bins[i] = make_float4(s->cdf_y[i], s->cdf_y[i + 1], s->slope[i], s->cdf_x[i]);
+        std::vector<int> guide(n);
+        float gscale = (float)n / s->buie_int465;
+        int i = 0;
+        for (int j = 0; j < n; ++j) {
+            float vj = (float)j / gscale;
+            while (i < n - 1 && s->cdf_y[i + 1] <= vj) ++i;
+            guide[j] = i;
+        }
+        if ((st = upload(s->d_buie_bins, bins.data(), bins.size())) != OM_OK) return st;
+        if ((st = upload(s->d_buie_guide, guide.data(), guide.size())) != OM_OK) return st;
+    }
     if (s->prm.micro_mode == OM_MICRO_POISSON) {
         // Poisson points are drawn on the GPU (one workgroup per heliostat); their
         // counts come back to size the ray index space.
@@ -780,9 +799,9 @@ TraceArgs make_trace_args(const om_scene *s)
     a.buie_C = s->buie_C;
     a.buie_inv_gp1 = s->buie_inv_gp1;
     a.nbins = (s->prm.sunshape == OM_SUN_BUIE) ? s->nbins : 0;
-    a.cdf_x = s->d_cdf_x;
-    a.cdf_y = s->d_cdf_y;
-    a.slope = s->d_slope;
+    a.buie_bins = s->d_buie_bins;
+    a.buie_guide = s->d_buie_guide;
+    a.buie_gscale = (float)s->nbins / s->buie_int465;
     a.slope_sigma = s->prm.slope_sigma;
     a.key0 = (uint32_t)s->prm.seed;
     a.key1 = (uint32_t)(s->prm.seed >> 32);
@@ -796,6 +815,14 @@ TraceArgs make_trace_args(const om_scene *s)
     a.gix = s->grid_interval[0];
     a.giy = s->grid_interval[1];
     a.giz = s->grid_interval[2];
+    auto pow2 = [](float x) {
+        int e;
+        return x > 0.0f && std::frexp(x, &e) == 0.5f && e > -100 && e < 100;
+    };
+    a.grid_pow2 = pow2(a.gix) && pow2(a.giy) && pow2(a.giz);
+    a.inv_gix = 1.0f / a.gix;
+    a.inv_giy = 1.0f / a.giy;
+    a.inv_giz = 1.0f / a.giz;
     a.gnx = s->gn[0];
     a.gny = s->gn[1];
     a.gnz = s->gn[2];

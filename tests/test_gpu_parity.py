@@ -202,3 +202,20 @@ def test_mid_records_bitexact(mid, h0):
     d, p, g, o = mid
     h1 = h0 + 40
     _records_equal(g.trace_records(h0, h1), o.trace_f32(h0, h1, records=True), f"mid[{h0}:{h1}]")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_accumulation_modes_bitexact(om, mode):
+    """Per-XCD partial maps (L2-local atomics + reduction) and device-scope atomics both
+    reproduce the oracle's integer maps."""
+    import torch
+    d, p = F.config_mid()
+    d = F.subset(d, slice(1000, 1400))
+    p = dict(p, accumulate=mode)
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    maps = g.trace_plain()
+    torch.cuda.synchronize()
+    orc = o.trace_f32()
+    assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32), orc["counts"])
+    assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64), orc["energy"])
+    assert int(orc["counts"].sum()) > 1_000_000
