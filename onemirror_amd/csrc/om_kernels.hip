@@ -19,6 +19,7 @@
 namespace om {
 
 constexpr int WG = 256;
+constexpr int RPT_MAX = 8; // rays per thread of one trace workgroup (hit slots in LDS)
 
 // Workgroup b of an n-workgroup launch runs on XCD b % 8.  Give each XCD a contiguous
 // range of tiles (neighbouring heliostats share blockers and grid cells in its L2).
@@ -119,18 +120,27 @@ __device__ __forceinline__ uint32_t xcc_id()
     return x & 15u;
 }
 
-template <int SUN, bool POISSON, bool RECORD>
+// Hits of a thread's rays are kept in LDS and added to the maps after the ray loop: vector
+// memory operations complete in issue order, so an atomic issued inside the loop would hold
+// back every later load of the wave until it has been carried out in L2.
+template <int SUN, bool POISSON, bool RECORD, bool BUIE_LDS>
 __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
 {
     extern __shared__ float4 lds4[];
-    float4 *s_bins = lds4;
-    int *s_guide = (int *)(lds4 + a.nbins);
-    if (SUN == OM_SUN_BUIE) {
+    int *s_hpx = (int *)lds4;                          // [RPT_MAX][WG] pixel or -1
+    uint32_t *s_hq = (uint32_t *)(s_hpx + RPT_MAX * WG); // [RPT_MAX][WG] fixed-point energy
+    const float4 *s_bins = a.buie_bins;
+    const int *s_guide = a.buie_guide;
+    if (SUN == OM_SUN_BUIE && BUIE_LDS) {
+        float4 *lb = (float4 *)(s_hq + RPT_MAX * WG);
+        int *lg = (int *)(lb + a.nbins);
         for (int i = threadIdx.x; i < a.nbins; i += WG) {
-            s_bins[i] = a.buie_bins[i];
-            s_guide[i] = a.buie_guide[i];
+            lb[i] = a.buie_bins[i];
+            lg[i] = a.buie_guide[i];
         }
         __syncthreads();
+        s_bins = lb;
+        s_guide = lg;
     }
     const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
@@ -162,6 +172,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
     int m = r / L;
     int l = r - m * L;
     const int dm = WG / L, dl = WG - (WG / L) * L;
+    int nk = 0; // rays of this thread, hit slots used
     for (int k = 0; k < a.rays_per_thread; ++k, r += WG) {
         if (k > 0) {
             m += dm;
@@ -172,6 +183,8 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
             }
         }
         if (r >= nr) break;
+        nk = k + 1;
+        if (!RECORD) s_hpx[k * WG + threadIdx.x] = -1;
         // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
         v3 loc;
         if (POISSON) {
@@ -238,18 +251,26 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
             a.rec_block[rid] = bl;
             a.rec_pixel[rid] = px;
         }
-        if (px >= 0) {
+        if (!RECORD && px >= 0) {
             float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
-            uint32_t q = (uint32_t)(e * 2147483648.0f);
-            if (a.acc_xcd) {
-                // this XCD's copy: the atomics resolve in its L2
-                __hip_atomic_fetch_add(cnt + px, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(eng + px, (unsigned long long)q, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-                atomicAdd(cnt + px, 1u);
-                atomicAdd(eng + px, (unsigned long long)q);
-            }
+            s_hpx[k * WG + threadIdx.x] = px;
+            s_hq[k * WG + threadIdx.x] = (uint32_t)(e * 2147483648.0f);
+        }
+    }
+    if (RECORD) return;
+    // each thread adds its own hits: no loads follow, so nothing waits on the atomics
+    for (int k = 0; k < nk; ++k) {
+        const int px = s_hpx[k * WG + threadIdx.x];
+        if (px < 0) continue;
+        const uint32_t q = s_hq[k * WG + threadIdx.x];
+        if (a.acc_xcd) {
+            // this XCD's copy: the atomics resolve in its L2
+            __hip_atomic_fetch_add(cnt + px, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(eng + px, (unsigned long long)q, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            atomicAdd(cnt + px, 1u);
+            atomicAdd(eng + px, (unsigned long long)q);
         }
     }
 }
@@ -503,29 +524,37 @@ om_status launch_poisson(om_scene *s, hipStream_t st)
     return OM_OK;
 }
 
+// the Buie tables go to LDS when they fit next to the hit slots
+static bool buie_in_lds(int nbins) { return (size_t)nbins * (sizeof(float4) + sizeof(int)) <= 24 * 1024; }
+
 om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st)
 {
     if (a.ntiles <= 0) return OM_OK;
-    size_t lds = (s->prm.sunshape == OM_SUN_BUIE) ? (sizeof(float4) + sizeof(int)) * (size_t)a.nbins : 0;
+    if (a.rays_per_thread > RPT_MAX) {
+        set_error("rays per thread above the hit-slot count");
+        return OM_ERR_INVALID;
+    }
+    const bool buie = s->prm.sunshape == OM_SUN_BUIE;
+    const bool blds = buie && buie_in_lds(a.nbins);
+    size_t lds = (size_t)RPT_MAX * WG * (sizeof(int) + sizeof(uint32_t));
+    if (blds) lds += (sizeof(float4) + sizeof(int)) * (size_t)a.nbins;
     dim3 grid(a.ntiles), block(WG);
-    bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
-#define OM_TRACE_LAUNCH(SUN, PO, REC) hipLaunchKernelGGL((trace_kernel<SUN, PO, REC>), grid, block, lds, st, a)
-    if (s->prm.sunshape == OM_SUN_BUIE) {
-        if (poisson) {
-            if (records) OM_TRACE_LAUNCH(OM_SUN_BUIE, true, true);
-            else OM_TRACE_LAUNCH(OM_SUN_BUIE, true, false);
-        } else {
-            if (records) OM_TRACE_LAUNCH(OM_SUN_BUIE, false, true);
-            else OM_TRACE_LAUNCH(OM_SUN_BUIE, false, false);
-        }
+    const bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
+#define OM_TRACE_LAUNCH(SUN, BL)                                                                       \
+    do {                                                                                               \
+        if (poisson) {                                                                                 \
+            if (records) hipLaunchKernelGGL((trace_kernel<SUN, true, true, BL>), grid, block, lds, st, a);  \
+            else hipLaunchKernelGGL((trace_kernel<SUN, true, false, BL>), grid, block, lds, st, a);         \
+        } else {                                                                                       \
+            if (records) hipLaunchKernelGGL((trace_kernel<SUN, false, true, BL>), grid, block, lds, st, a); \
+            else hipLaunchKernelGGL((trace_kernel<SUN, false, false, BL>), grid, block, lds, st, a);        \
+        }                                                                                              \
+    } while (0)
+    if (buie) {
+        if (blds) OM_TRACE_LAUNCH(OM_SUN_BUIE, true);
+        else OM_TRACE_LAUNCH(OM_SUN_BUIE, false);
     } else {
-        if (poisson) {
-            if (records) OM_TRACE_LAUNCH(OM_SUN_PILLBOX, true, true);
-            else OM_TRACE_LAUNCH(OM_SUN_PILLBOX, true, false);
-        } else {
-            if (records) OM_TRACE_LAUNCH(OM_SUN_PILLBOX, false, true);
-            else OM_TRACE_LAUNCH(OM_SUN_PILLBOX, false, false);
-        }
+        OM_TRACE_LAUNCH(OM_SUN_PILLBOX, false);
     }
 #undef OM_TRACE_LAUNCH
     OM_HIP(hipGetLastError());

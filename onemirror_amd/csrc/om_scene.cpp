@@ -648,7 +648,7 @@ om_status choose_tiles(om_scene *s)
 {
     // enough workgroups to fill 256 CUs several times over, tiles of at least 256 rays
     long long total = s->total_rays;
-    int rpt = 16;
+    int rpt = 8; // at most the hit slots of a trace workgroup
     while (rpt > 1 && total / (256LL * rpt) < 8192) rpt /= 2;
     s->rays_per_thread = rpt;
     build_tiles(s);
