@@ -80,12 +80,15 @@ typedef struct {
     int   smooth_radius;       /* trimmed-mean window radius (Gtest.cu:50)                */
     float smooth_ratio;        /* trimmed fraction (Gtest.cu:51)                          */
     int   accumulate;          /* OM_ACC_*: how hits reach the maps (results identical)   */
+    int   pack_shift;          /* OM_ACC_PACKED: energy-field bits E (32..62, 0 = 44)     */
 } om_params;
 
-/* OM_ACC_XCD: every XCD adds into its own copy of the maps with atomics that resolve in
- * its L2, and a reduction kernel sums the copies; OM_ACC_DEVICE: device-scope atomics
+/* OM_ACC_PACKED: one 64-bit atomic per hit into a per-XCD word holding (count << E) +
+ * energy; carries out of the energy field and wraps of the word are detected from the
+ * returned old value and counted on the side, and a reduction kernel decodes the exact
+ * counts and energies.  OM_ACC_DEVICE: two device-scope atomics (count, energy) per hit
  * straight into the output maps. */
-enum { OM_ACC_XCD = 0, OM_ACC_DEVICE = 1 };
+enum { OM_ACC_PACKED = 0, OM_ACC_DEVICE = 1 };
 
 /* Output maps of one receiver, row-major [res_y][res_x], all device memory. */
 typedef struct {

@@ -70,10 +70,11 @@ class Params(ctypes.Structure):
         ("smooth_radius", ctypes.c_int),
         ("smooth_ratio", ctypes.c_float),
         ("accumulate", ctypes.c_int),
+        ("pack_shift", ctypes.c_int),
     ]
 
 
-ACC_XCD, ACC_DEVICE = 0, 1
+ACC_PACKED, ACC_DEVICE = 0, 1
 
 
 class Flux(ctypes.Structure):

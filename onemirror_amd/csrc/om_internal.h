@@ -72,11 +72,13 @@ struct TraceArgs {
     unsigned long long *energy;
     int32_t *rec_shade, *rec_block, *rec_pixel;
     long long rec_base;
-    // per-XCD partial maps (OM_ACC_XCD): [16][npx]
-    uint32_t *part_counts;
-    unsigned long long *part_energy;
+    // packed accumulation (OM_ACC_PACKED)
+    unsigned long long *part; // [16][npx] packed (count << pack_shift) + energy per XCD
+    uint32_t *carry;          // [npx] carries out of the energy field
+    uint32_t *wrap;           // [npx] wraps of the packed word
     int npx;
-    int acc_xcd;
+    int acc_packed;
+    int pack_shift;
 };
 
 struct DevBuf {

@@ -112,7 +112,9 @@ __device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
     }
 }
 
-// XCD (accelerator complex die) that runs the calling wave
+// XCD (accelerator complex die) that runs the calling wave.  The packed map words are kept
+// one copy per XCD to spread same-pixel atomics; the atomics themselves are agent-scope, so
+// the choice of copy affects contention only, never the result.
 __device__ __forceinline__ uint32_t xcc_id()
 {
     uint32_t x;
@@ -122,7 +124,7 @@ __device__ __forceinline__ uint32_t xcc_id()
 
 // Hits of a thread's rays are kept in LDS and added to the maps after the ray loop: vector
 // memory operations complete in issue order, so an atomic issued inside the loop would hold
-// back every later load of the wave until it has been carried out in L2.
+// back every later load of the wave until the memory side has carried it out.
 template <int SUN, bool POISSON, bool RECORD, bool BUIE_LDS>
 __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
 {
@@ -160,13 +162,6 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
     const int L = a.L;
     const int nr = hr->mcount * L;
     const bool one_row = hr->rows == 1, one_col = hr->cols == 1;
-    uint32_t *cnt = a.counts;
-    unsigned long long *eng = a.energy;
-    if (a.acc_xcd) {
-        const uint32_t x = xcc_id();
-        cnt = a.part_counts + (size_t)x * a.npx;
-        eng = a.part_energy + (size_t)x * a.npx;
-    }
     // ray index r = m * L + l, stepped by WG without division
     int r = tl.y + (int)threadIdx.x;
     int m = r / L;
@@ -258,38 +253,72 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         }
     }
     if (RECORD) return;
-    // each thread adds its own hits: no loads follow, so nothing waits on the atomics
-    for (int k = 0; k < nk; ++k) {
-        const int px = s_hpx[k * WG + threadIdx.x];
-        if (px < 0) continue;
-        const uint32_t q = s_hq[k * WG + threadIdx.x];
-        if (a.acc_xcd) {
-            // this XCD's copy: the atomics resolve in its L2
-            __hip_atomic_fetch_add(cnt + px, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(eng + px, (unsigned long long)q, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-            atomicAdd(cnt + px, 1u);
-            atomicAdd(eng + px, (unsigned long long)q);
+    // Each thread adds its own hits after its last load.
+    if (!a.acc_packed) {
+        for (int k = 0; k < nk; ++k) {
+            const int px = s_hpx[k * WG + threadIdx.x];
+            if (px < 0) continue;
+            atomicAdd(a.counts + px, 1u);
+            atomicAdd(a.energy + px, (unsigned long long)s_hq[k * WG + threadIdx.x]);
+        }
+        return;
+    }
+    // One 64-bit atomic per hit into this XCD's packed word: v = 2^E + q.  The old value
+    // tells whether the add carried out of the energy field (bits 0..E-1) or wrapped the
+    // word; both events are counted on the side so that the reduction decodes exactly.
+    unsigned long long *part = a.part + (size_t)xcc_id() * a.npx;
+    const int E = a.pack_shift;
+    const unsigned long long lowmask = (1ull << E) - 1ull;
+    unsigned long long old[RPT_MAX];
+#pragma unroll
+    for (int k = 0; k < RPT_MAX; ++k) {
+        old[k] = 0;
+        if (k < nk) {
+            const int px = s_hpx[k * WG + threadIdx.x];
+            if (px >= 0) {
+                const unsigned long long v = (1ull << E) + s_hq[k * WG + threadIdx.x];
+                old[k] = __hip_atomic_fetch_add(part + px, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPT_MAX; ++k) {
+        if (k < nk) {
+            const int px = s_hpx[k * WG + threadIdx.x];
+            if (px >= 0) {
+                const unsigned long long q = s_hq[k * WG + threadIdx.x];
+                const unsigned long long v = (1ull << E) + q;
+                if ((old[k] & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
+                if (old[k] + v < old[k]) atomicAdd(a.wrap + px, 1u);
+            }
         }
     }
 }
 
-// sum of the per-XCD partial maps into the output maps (which it adds to)
-__global__ void reduce_parts_kernel(const uint32_t *pc, const unsigned long long *pe, uint32_t *counts,
-                                    unsigned long long *energy, int npx)
+// Decode the packed per-XCD words into the output maps (which it adds to) and clear them.
+// Over all copies, with A carries out of the energy field and W wraps of the word:
+//   count = sum(word >> E) + W * 2^(64-E) - A,    energy = sum(word & (2^E - 1)) + A * 2^E.
+__global__ void reduce_packed_kernel(unsigned long long *part, uint32_t *carry, uint32_t *wrap, int E,
+                                     uint32_t *counts, unsigned long long *energy, int npx)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npx) return;
-    uint32_t c = counts[i];
-    unsigned long long e = energy[i];
+    const unsigned long long lowmask = (1ull << E) - 1ull;
+    unsigned long long n = 0, e = 0;
 #pragma unroll
     for (int x = 0; x < 16; ++x) {
-        c += pc[(size_t)x * npx + i];
-        e += pe[(size_t)x * npx + i];
+        unsigned long long w = part[(size_t)x * npx + i];
+        n += w >> E;
+        e += w & lowmask;
+        part[(size_t)x * npx + i] = 0ull;
     }
-    counts[i] = c;
-    energy[i] = e;
+    const unsigned long long A = carry[i], W = wrap[i];
+    n = n + (W << (64 - E)) - A;
+    e = e + (A << E);
+    carry[i] = 0u;
+    wrap[i] = 0u;
+    counts[i] += (uint32_t)n;
+    energy[i] += e;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -645,21 +674,26 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
     const int npx = s->res_x * s->res_y;
     om_status rc;
     if (clear && (rc = clear_maps(out, npx, st)) != OM_OK) return rc;
-    if (s->prm.accumulate == OM_ACC_XCD) {
+    if (s->prm.accumulate == OM_ACC_PACKED) {
         const size_t n16 = 16 * (size_t)npx;
-        if ((rc = ensure_buf(s->ctx->parts, n16 * (sizeof(uint32_t) + sizeof(unsigned long long)))) != OM_OK)
-            return rc;
-        a.part_counts = (uint32_t *)s->ctx->parts.p;
-        a.part_energy = (unsigned long long *)((char *)s->ctx->parts.p + n16 * sizeof(uint32_t));
+        const size_t bytes = n16 * sizeof(unsigned long long) + 2 * (size_t)npx * sizeof(uint32_t);
+        const bool fresh = s->ctx->parts.bytes < bytes;
+        if ((rc = ensure_buf(s->ctx->parts, bytes)) != OM_OK) return rc;
+        a.part = (unsigned long long *)s->ctx->parts.p;
+        a.carry = (uint32_t *)(a.part + n16);
+        a.wrap = a.carry + npx;
         a.npx = npx;
-        a.acc_xcd = 1;
-        dim3 g((unsigned)((n16 + WG - 1) / WG)), b(WG);
-        hipLaunchKernelGGL(fill_u32_kernel, g, b, 0, st, a.part_counts, (long long)n16);
-        hipLaunchKernelGGL(fill_u64_kernel, g, b, 0, st, a.part_energy, (long long)n16);
-        OM_HIP(hipGetLastError());
+        a.acc_packed = 1;
+        a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
+        if (fresh) {
+            // the reduction leaves the words cleared; a new buffer is cleared once here
+            dim3 g((unsigned)((bytes / 4 + WG - 1) / WG)), b(WG);
+            hipLaunchKernelGGL(fill_u32_kernel, g, b, 0, st, (uint32_t *)s->ctx->parts.p, (long long)(bytes / 4));
+            OM_HIP(hipGetLastError());
+        }
         if ((rc = launch_trace(s, a, false, st)) != OM_OK) return rc;
-        hipLaunchKernelGGL(reduce_parts_kernel, dim3((npx + WG - 1) / WG), dim3(WG), 0, st, a.part_counts,
-                           a.part_energy, out->counts, (unsigned long long *)out->energy, npx);
+        hipLaunchKernelGGL(reduce_packed_kernel, dim3((npx + WG - 1) / WG), dim3(WG), 0, st, a.part, a.carry,
+                           a.wrap, a.pack_shift, out->counts, (unsigned long long *)out->energy, npx);
         OM_HIP(hipGetLastError());
         return OM_OK;
     }

@@ -97,7 +97,8 @@ extern "C" void om_default_params(om_params *p)
     p->reflectivity = 0.88f;
     p->smooth_radius = 5;
     p->smooth_ratio = 0.02f;
-    p->accumulate = OM_ACC_XCD;
+    p->accumulate = OM_ACC_PACKED;
+    p->pack_shift = 0;
 }
 
 extern "C" om_status om_ctx_create(int device, om_ctx **out)
@@ -692,8 +693,12 @@ om_status validate(const om_scene_desc *d, const om_params *p)
         om::set_error("buie_bins must be in 1..8192 and csr positive");
         return OM_ERR_INVALID;
     }
-    if (p->accumulate != OM_ACC_XCD && p->accumulate != OM_ACC_DEVICE) {
+    if (p->accumulate != OM_ACC_PACKED && p->accumulate != OM_ACC_DEVICE) {
         om::set_error("unknown accumulation mode");
+        return OM_ERR_INVALID;
+    }
+    if (p->pack_shift != 0 && (p->pack_shift < 32 || p->pack_shift > 62)) {
+        om::set_error("pack_shift must be 0 or in 32..62");
         return OM_ERR_INVALID;
     }
     if (p->micro_mode == OM_MICRO_POISSON && p->poisson_k <= 0) {

@@ -204,14 +204,16 @@ def test_mid_records_bitexact(mid, h0):
     _records_equal(g.trace_records(h0, h1), o.trace_f32(h0, h1, records=True), f"mid[{h0}:{h1}]")
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_accumulation_modes_bitexact(om, mode):
-    """Per-XCD partial maps (L2-local atomics + reduction) and device-scope atomics both
-    reproduce the oracle's integer maps."""
+@pytest.mark.parametrize("mode,shift", [(0, 0), (1, 0), (0, 32), (0, 33), (0, 62), (0, 61)])
+def test_accumulation_modes_bitexact(om, mode, shift):
+    """Packed single-atomic accumulation (with its carry and wrap bookkeeping driven hard by
+    narrow fields: E = 32/33 carries out of the energy field every hit or two, E = 61/62
+    wraps the count field every few hits) and two device-scope atomics per hit all reproduce
+    the oracle's integer maps."""
     import torch
     d, p = F.config_mid()
     d = F.subset(d, slice(1000, 1400))
-    p = dict(p, accumulate=mode)
+    p = dict(p, accumulate=mode, pack_shift=shift)
     g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
     maps = g.trace_plain()
     torch.cuda.synchronize()
