@@ -112,6 +112,29 @@ __device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
     }
 }
 
+// perturbed mirror normal from the slope-error draws (map_turbulance,
+// scene_instance_process.cu:14-23; rectgrid_raytracing.cu:182-186)
+__device__ __forceinline__ v3 perturbed_normal(const u4 &x, float sigma, const frame &hf)
+{
+    float st, ct, sp, cp;
+    const float ths = gauss_(u01(x.z)) * sigma;
+    sincos_small(ths, st, ct);
+    sincos2pi(u01(x.w), sp, cp);
+    return normalize(to_world(angle2xyz(st, ct, sp, cp), hf));
+}
+
+// receiver_drawing (rectgrid_raytracing.cu:135-153): pixel of the hit, -1 for a miss; u or
+// v equal to 1 lands on the last row or column
+__device__ __forceinline__ int receiver_pixel(const TraceArgs &a, v3 orig, v3 refl)
+{
+    float t, u, v;
+    if (!parallelogram(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) return -1;
+    int row = trunc_i(u * (float)a.res_y), col = trunc_i(v * (float)a.res_x);
+    row = min(row, a.res_y - 1);
+    col = min(col, a.res_x - 1);
+    return row * a.res_x + col;
+}
+
 // XCD (accelerator complex die) that runs the calling wave.  The packed map words are kept
 // one copy per XCD to spread same-pixel atomics; the atomics themselves are agent-scope, so
 // the choice of copy affects contention only, never the result.
@@ -214,43 +237,37 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         sincos_small(th, st, ct);
         sincos2pi(u01(x.y), sp, cp);
         const v3 to_sun = -to_world(angle2xyz(st, ct, sp, cp), sf);
-        long long rid = 0;
-        if (RECORD) rid = (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
-        int sh = dda(a, orig, to_sun, h);
-        if (sh >= 0) {
-            if (RECORD) {
+        if (RECORD) {
+            // the reference order, every index recorded (rectgrid_raytracing.cu:176-195)
+            const long long rid =
+                (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
+            int sh = dda(a, orig, to_sun, h);
+            if (sh >= 0) {
                 a.rec_shade[rid] = sh;
                 a.rec_block[rid] = -2;
                 a.rec_pixel[rid] = -1;
+                continue;
             }
-            continue;
-        }
-        float ths = gauss_(u01(x.z)) * a.slope_sigma;
-        sincos_small(ths, st, ct);
-        sincos2pi(u01(x.w), sp, cp);
-        const v3 nrm = normalize(to_world(angle2xyz(st, ct, sp, cp), hf));
-        const v3 refl = normalize(reflect(-to_sun, nrm));
-        int bl = dda(a, orig, refl, h);
-        int px = -1;
-        if (bl < 0) {
-            float t, u, v;
-            if (parallelogram(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) {
-                int row = trunc_i(u * (float)a.res_y), col = trunc_i(v * (float)a.res_x);
-                row = min(row, a.res_y - 1);
-                col = min(col, a.res_x - 1);
-                px = row * a.res_x + col;
-            }
-        }
-        if (RECORD) {
+            const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
+            const v3 refl = normalize(reflect(-to_sun, nrm));
+            int bl = dda(a, orig, refl, h);
+            int px = bl < 0 ? receiver_pixel(a, orig, refl) : -1;
             a.rec_shade[rid] = -1;
             a.rec_block[rid] = bl;
             a.rec_pixel[rid] = px;
+            continue;
         }
-        if (!RECORD && px >= 0) {
-            float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
-            s_hpx[k * WG + threadIdx.x] = px;
-            s_hq[k * WG + threadIdx.x] = (uint32_t)(e * 2147483648.0f);
-        }
+        // A ray adds to the map only when it is neither shaded nor blocked and hits the
+        // receiver, so the cheap receiver test runs first and the walks only for hits.
+        const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
+        const v3 refl = normalize(reflect(-to_sun, nrm));
+        const int px = receiver_pixel(a, orig, refl);
+        if (px < 0) continue;
+        if (dda(a, orig, to_sun, h) >= 0) continue;
+        if (dda(a, orig, refl, h) >= 0) continue;
+        const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
+        s_hpx[k * WG + threadIdx.x] = px;
+        s_hq[k * WG + threadIdx.x] = (uint32_t)(e * 2147483648.0f);
     }
     if (RECORD) return;
     // Each thread adds its own hits after its last load.
