@@ -257,14 +257,12 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
             a.rec_pixel[rid] = px;
             continue;
         }
-        // A ray adds to the map only when it is neither shaded nor blocked and hits the
-        // receiver, so the cheap receiver test runs first and the walks only for hits.
+        if (dda(a, orig, to_sun, h) >= 0) continue;
         const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
         const v3 refl = normalize(reflect(-to_sun, nrm));
+        if (dda(a, orig, refl, h) >= 0) continue;
         const int px = receiver_pixel(a, orig, refl);
         if (px < 0) continue;
-        if (dda(a, orig, to_sun, h) >= 0) continue;
-        if (dda(a, orig, refl, h) >= 0) continue;
         const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
         s_hpx[k * WG + threadIdx.x] = px;
         s_hq[k * WG + threadIdx.x] = (uint32_t)(e * 2147483648.0f);
