@@ -11,11 +11,13 @@ Prints one JSON line: rays/s, the roofline of the trace kernel (VALU lane-ops pe
 the cost model below times rays/s, against 256 CU x 64 lanes x 2.4 GHz), and the CPU
 oracle's rays/s on a stated sample of the same field on the host cores.
 
-VALU cost model per ray (lane operations, counted from the kernel source):
-    F0 = 820 fixed (Philox 80, Buie lookup 75, two sincos pairs 74, Gaussian 40,
-         frames/normalize/reflect 120, two DDA set-ups 300, microsample centre 50,
-         receiver + pixel + energy + atomics 75, indexing 6)
-    + 13 per DDA cell step + 58 per candidate parallelogram test + 3 per own-heliostat skip
+VALU cost model per ray (lane operations; add/mul/fma/cmp/cvt/int = 1, div/sqrt/rcp = 1,
+sincos = 30, log = 25, Philox4x32-10 = 100):
+    F0 = 569 fixed: Philox 100, four uniforms 12, sunshape lookup 15, sun-ray sincos pair 60
+         and direction 21, two walk set-ups 114, perturbed normal 128 (Gaussian 40, sincos
+         pair 60, frame 18, normalize 10), reflect + normalize 21, microsample centre 24,
+         receiver hit + pixel 64, energy + accumulation 10
+    + 12 per DDA cell step + 58 per candidate parallelogram test + 3 per own-heliostat skip,
 with the step/test counts measured by the oracle on the same sample of the field.
 """
 from __future__ import annotations
@@ -31,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "rays/s (w/ DDA shading+blocking) at ≤1e-5 flux err vs FP64; batched & fwd+bwd/s"
 P_VALU = 256 * 64 * 2.4e9          # lane-op/s
-F0, C_STEP, C_TEST, C_SKIP = 820.0, 13.0, 58.0, 3.0
+F0, C_STEP, C_TEST, C_SKIP = 569.0, 12.0, 58.0, 3.0
 
 
 def host_threads() -> int:

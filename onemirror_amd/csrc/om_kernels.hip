@@ -51,19 +51,32 @@ __device__ __forceinline__ float4 buie_bin(const float4 *bins, const int *guide,
 // result, once the cell index on an axis has passed the occupied index range in the
 // direction of travel: every later cell is empty.  With power-of-two cell edges the start
 // cell uses the exact reciprocal (x / 2^k == x * 2^-k).
-__device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
-{
-    float rx = o.x - a.gpx, ry = o.y - a.gpy, rz = o.z - a.gpz;
+// start cell of a walk from o: make_int3((o - pos) / interval) (rectgrid_raytracing.cu:43)
+struct cell3 {
+    float rx, ry, rz; // o - grid pos
     int gx, gy, gz;
+};
+__device__ __forceinline__ cell3 start_cell(const TraceArgs &a, v3 o)
+{
+    cell3 c;
+    c.rx = o.x - a.gpx;
+    c.ry = o.y - a.gpy;
+    c.rz = o.z - a.gpz;
     if (a.grid_pow2) {
-        gx = trunc_i(rx * a.inv_gix);
-        gy = trunc_i(ry * a.inv_giy);
-        gz = trunc_i(rz * a.inv_giz);
+        c.gx = trunc_i(c.rx * a.inv_gix);
+        c.gy = trunc_i(c.ry * a.inv_giy);
+        c.gz = trunc_i(c.rz * a.inv_giz);
     } else {
-        gx = trunc_i(rx / a.gix);
-        gy = trunc_i(ry / a.giy);
-        gz = trunc_i(rz / a.giz);
+        c.gx = trunc_i(c.rx / a.gix);
+        c.gy = trunc_i(c.ry / a.giy);
+        c.gz = trunc_i(c.rz / a.giz);
     }
+    return c;
+}
+
+__device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
+{
+    int gx = c.gx, gy = c.gy, gz = c.gz;
     int sx = (d.x >= 0.0f) ? 1 : -1, sy = (d.y >= 0.0f) ? 1 : -1, sz = (d.z >= 0.0f) ? 1 : -1;
     // admissible index window per axis: inside the grid and not past the occupied range
     int lox = sx > 0 ? 0 : a.occ_lo[0], hix = sx > 0 ? a.occ_hi[0] : a.gnx - 1;
@@ -72,31 +85,24 @@ __device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
     if ((sx > 0 ? gx > hix : gx < lox) || (sy > 0 ? gy > hiy : gy < loy) ||
         (sz > 0 ? gz > hiz : gz < loz))
         return -1;
-    float mx = abs_divide(cal_tmax(d.x, a.gix, gx, rx), d.x);
-    float my = abs_divide(cal_tmax(d.y, a.giy, gy, ry), d.y);
-    float mz = abs_divide(cal_tmax(d.z, a.giz, gz, rz), d.z);
-    float dx = abs_divide(a.gix, d.x), dy = abs_divide(a.giy, d.y), dz = abs_divide(a.giz, d.z);
+    float mx = abs_divide(cal_tmax(d.x, a.gix, gx, c.rx), d.x);
+    float my = abs_divide(cal_tmax(d.y, a.giy, gy, c.ry), d.y);
+    float mz = abs_divide(cal_tmax(d.z, a.giz, gz, c.rz), d.z);
+    // tDelta of an axis is needed only once that axis has stepped and the walk goes on
+    float dx = -1.0f, dy = -1.0f, dz = -1.0f;
     for (;;) {
-        if (mx < my) {
-            if (mx < mz) {
-                gx += sx;
-                if (gx < lox || gx > hix) return -1;
-                mx += dx;
-            } else {
-                gz += sz;
-                if (gz < loz || gz > hiz) return -1;
-                mz += dz;
-            }
+        int axis;
+        if (mx < my) axis = (mx < mz) ? 0 : 2;
+        else axis = (my < mz) ? 1 : 2;
+        if (axis == 0) {
+            gx += sx;
+            if (gx < lox || gx > hix) return -1;
+        } else if (axis == 1) {
+            gy += sy;
+            if (gy < loy || gy > hiy) return -1;
         } else {
-            if (my < mz) {
-                gy += sy;
-                if (gy < loy || gy > hiy) return -1;
-                my += dy;
-            } else {
-                gz += sz;
-                if (gz < loz || gz > hiz) return -1;
-                mz += dz;
-            }
+            gz += sz;
+            if (gz < loz || gz > hiz) return -1;
         }
         int addr = (gx * a.gny + gy) * a.gnz + gz;
         int i0 = a.cell_start[addr], i1 = a.cell_start[addr + 1];
@@ -108,6 +114,16 @@ __device__ int dda(const TraceArgs &a, v3 o, v3 d, int self)
             if (parallelogram(o, d, mk(A.x, A.y, A.z), mk(E1.x, E1.y, E1.z), mk(E2.x, E2.y, E2.z),
                               t, u, v))
                 return h;
+        }
+        if (axis == 0) {
+            if (dx < 0.0f) dx = abs_divide(a.gix, d.x);
+            mx += dx;
+        } else if (axis == 1) {
+            if (dy < 0.0f) dy = abs_divide(a.giy, d.y);
+            my += dy;
+        } else {
+            if (dz < 0.0f) dz = abs_divide(a.giz, d.z);
+            mz += dz;
         }
     }
 }
@@ -218,6 +234,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
                      (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
         }
         const v3 orig = to_world(loc, hf) + hpos;
+        const cell3 c0 = start_cell(a, orig);
         // draws: sunshape theta, sunshape phi, slope-error normal deviate, slope-error phi
         u4 x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3}, a.key0, a.key1);
         float th;
@@ -241,7 +258,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
             // the reference order, every index recorded (rectgrid_raytracing.cu:176-195)
             const long long rid =
                 (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
-            int sh = dda(a, orig, to_sun, h);
+            int sh = dda(a, orig, c0, to_sun, h);
             if (sh >= 0) {
                 a.rec_shade[rid] = sh;
                 a.rec_block[rid] = -2;
@@ -250,17 +267,17 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
             }
             const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
             const v3 refl = normalize(reflect(-to_sun, nrm));
-            int bl = dda(a, orig, refl, h);
+            int bl = dda(a, orig, c0, refl, h);
             int px = bl < 0 ? receiver_pixel(a, orig, refl) : -1;
             a.rec_shade[rid] = -1;
             a.rec_block[rid] = bl;
             a.rec_pixel[rid] = px;
             continue;
         }
-        if (dda(a, orig, to_sun, h) >= 0) continue;
+        if (dda(a, orig, c0, to_sun, h) >= 0) continue;
         const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
         const v3 refl = normalize(reflect(-to_sun, nrm));
-        if (dda(a, orig, refl, h) >= 0) continue;
+        if (dda(a, orig, c0, refl, h) >= 0) continue;
         const int px = receiver_pixel(a, orig, refl);
         if (px < 0) continue;
         const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
