@@ -196,8 +196,12 @@ om_status om_results_with_given_normal(om_scene *scene, om_result *result);
 /* Intercept ratio of a host flux map (ImageSaver::savetxt, image_save.cpp:26-47). */
 float om_intercept_ratio(const float *flux, int width, int height, float total_energy);
 
-/* Device self-test of the float math (parity of the kernels' arithmetic with the host):
- * fn as in the oracle's om_math_batch; in/out device arrays of n floats. */
+/* Device self-test of the float math as the trace kernel evaluates it (parity of the
+ * kernels' arithmetic with the host).  fn: 0 u01 (input bits as uint32), 1/2 sin/cos(2 pi x),
+ * 3/4 sin/cos of a small angle, 5 log, 6 exp, 7 erfinv, 8 Gaussian deviate of a uniform,
+ * 9 sqrt, 10 division of pairs (out[i] = in[2i] / in[2i+1], i < n/2).  9 and 10 use the
+ * kernel's short exact sequences, which hold for x in [2^-90, 2^90] and for numerators 0 or
+ * in [2^-90, 2^60] over denominators in [2^-60, 2^60].  in/out device arrays of n floats. */
 om_status om_math_eval(om_ctx *ctx, int fn, const float *d_in, float *d_out, int n, void *stream);
 
 #ifdef __cplusplus

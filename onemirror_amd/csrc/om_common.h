@@ -7,7 +7,6 @@
 // Reference semantics (Hu_smooth/ClassedSunRay):
 //   vector algebra            vector_arithmetic.cuh:341-413 (host rsqrtf = 1/sqrtf, :51-54)
 //   frames and intersection   global_function.cuh:14-105, :175-180
-//   DDA helpers               rectgrid_raytracing.cu:3-17
 #pragma once
 
 #include <stdint.h>
@@ -50,10 +49,19 @@ OM_HD v3 cross(v3 a, v3 b)
 {
     return v3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+// Division and square root of the functions below come from a policy M.  ieee is the plain
+// correctly rounded operation; the trace kernel passes a policy with shorter sequences that
+// give the same bits for the operand ranges it uses.
+struct ieee {
+    OM_HD static float div(float a, float b) { return a / b; }
+    OM_HD static float sqrt(float x) { return sqrt_(x); }
+};
+
 // normalize with the host rsqrtf: v * (1 / sqrtf(v.v))
+template <class M = ieee>
 OM_HD v3 normalize(v3 v)
 {
-    float inv = 1.0f / sqrt_(dot(v, v));
+    float inv = M::div(1.0f, M::sqrt(dot(v, v)));
     return v * inv;
 }
 // reflect(i, n) = i - (2 n) dot(n, i)
@@ -110,12 +118,13 @@ OM_HD v3 rotate_y(v3 o, v3 old_dir, v3 new_dir)
 
 // ---- rayParallelogramIntersect (global_function.cuh:78-105) with E1 = B - A and
 // E2 = C - A precomputed by the caller.
+template <class M = ieee>
 OM_HD bool parallelogram(v3 orig, v3 dir, v3 A, v3 E1, v3 E2, float &t, float &u, float &v)
 {
     v3 pvec = cross(dir, E2);
     float det = dot(E1, pvec);
     if (fabs_(det) < EPS_LT) return false;
-    float invDet = 1.0f / det;
+    float invDet = M::div(1.0f, det);
     v3 T = orig - A;
     u = dot(T, pvec) * invDet;
     if (u < 0.0f || u > 1.0f) return false;
@@ -125,18 +134,6 @@ OM_HD bool parallelogram(v3 orig, v3 dir, v3 A, v3 E1, v3 E2, float &t, float &u
     t = dot(E2, qvec) * invDet;
     if (t < EPS_LT) return false;
     return true;
-}
-
-// ---- DDA helpers: calTMax and absDivide (rectgrid_raytracing.cu:3-17)
-OM_HD float cal_tmax(float dir, float interval, int idx, float cur)
-{
-    if (dir >= 0.0f) return (float)(idx + 1) * interval - cur;
-    return cur - (float)idx * interval;
-}
-OM_HD float abs_divide(float den, float num)
-{
-    if (num <= EPS_LE && num >= -EPS_LE) return INT_MAX_F;
-    return fabs_(den / num);
 }
 
 // ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
@@ -209,6 +206,7 @@ OM_HD void sincos_small(float x, float &s, float &c)
 
 // natural log of a positive normal float: m in [sqrt(1/2), sqrt(2)), 2 atanh((m-1)/(m+1))
 // as an odd series to s^11, plus e ln2 in two parts.
+template <class M = ieee>
 OM_HD float log_(float x)
 {
     uint32_t ix = f2u(x);
@@ -219,7 +217,7 @@ OM_HD float log_(float x)
         e = e + 1;
     }
     float f = m - 1.0f;
-    float s = f / (m + 1.0f);
+    float s = M::div(f, m + 1.0f);
     float s2 = s * s;
     float p = fma_(s2, 0.0909090909f, 0.111111111f);
     p = fma_(s2, p, 0.142857143f);
@@ -249,13 +247,15 @@ OM_HD float exp_(float x)
     return p * u2f((uint32_t)(ki + 127) << 23);
 }
 
-OM_HD float pow_(float x, float y) { return exp_(y * log_(x)); }
+template <class M = ieee>
+OM_HD float pow_(float x, float y) { return exp_(y * log_<M>(x)); }
 
 // erfinv in single precision: M. Giles, "Approximating the erfinv function",
 // GPU Computing Gems Jade Edition (2011): polynomial in w = -log(1 - x^2), two branches.
+template <class M = ieee>
 OM_HD float erfinv_(float x)
 {
-    float w = -log_((1.0f - x) * (1.0f + x));
+    float w = -log_<M>((1.0f - x) * (1.0f + x));
     float p;
     if (w < 5.0f) {
         w = w - 2.5f;
@@ -269,7 +269,7 @@ OM_HD float erfinv_(float x)
         p = fma_(p, w, 0.246640727f);
         p = fma_(p, w, 1.50140941f);
     } else {
-        w = sqrt_(w) - 3.0f;
+        w = M::sqrt(w) - 3.0f;
         p = -0.000200214257f;
         p = fma_(p, w, 0.000100950558f);
         p = fma_(p, w, 0.00134934322f);
@@ -284,7 +284,8 @@ OM_HD float erfinv_(float x)
 }
 
 // standard normal deviate: sqrt(2) erfinv(2u - 1)
-OM_HD float gauss_(float u) { return erfinv_(2.0f * u - 1.0f) * 1.41421356f; }
+template <class M = ieee>
+OM_HD float gauss_(float u) { return erfinv_<M>(2.0f * u - 1.0f) * 1.41421356f; }
 
 // angle2xyz (global_function.cuh:175-180) from the sines and cosines
 OM_HD v3 angle2xyz(float st, float ct, float sp, float cp) { return v3{st * cp, ct, st * sp}; }

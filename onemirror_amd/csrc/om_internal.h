@@ -73,10 +73,13 @@ struct TraceArgs {
     int32_t *rec_shade, *rec_block, *rec_pixel;
     long long rec_base;
     // packed accumulation (OM_ACC_PACKED)
-    unsigned long long *part; // [16][npx] packed (count << pack_shift) + energy per XCD
+    unsigned long long *part; // [8 << copy_shift][npx] packed (count << pack_shift) + energy
     uint32_t *carry;          // [npx] carries out of the energy field
     uint32_t *wrap;           // [npx] wraps of the packed word
     int npx;
+    int copy_shift;           // 2^copy_shift map copies per XCD
+    int copy_stride;          // words between copies of a pixel
+    int px_stride;            // words between pixels of a copy
     int acc_packed;
     int pack_shift;
 };

@@ -20,6 +20,8 @@ namespace om {
 
 constexpr int WG = 256;
 constexpr int RPT_MAX = 8; // rays per thread of one trace workgroup (hit slots in LDS)
+constexpr int COPY_SHIFT_MAX = 3;             // up to 64 copies of the packed map
+constexpr size_t PART_BYTES = 64u << 20;      // memory budget of the copies
 
 // Workgroup b of an n-workgroup launch runs on XCD b % 8.  Give each XCD a contiguous
 // range of tiles (neighbouring heliostats share blockers and grid cells in its L2).
@@ -44,17 +46,71 @@ __device__ __forceinline__ float4 buie_bin(const float4 *bins, const int *guide,
     return b;
 }
 
-// NotCollision + Intersect (rectgrid_raytracing.cu:21-117): Amanatides-Woo walk from the
-// cell of the origin (which is never tested), first candidate in CSR order whose
-// parallelogram the ray hits.  The ray's own heliostat is skipped (the reference rejects it
-// through t < 1e-6, which holds in exact arithmetic).  The walk also ends, with the same
-// result, once the cell index on an axis has passed the occupied index range in the
-// direction of travel: every later cell is empty.  With power-of-two cell edges the start
-// cell uses the exact reciprocal (x / 2^k == x * 2^-k).
-// start cell of a walk from o: make_int3((o - pos) / interval) (rectgrid_raytracing.cu:43)
+// ---------------------------------------------------------------------------------------
+// Short forms of the correctly rounded division and square root
+// ---------------------------------------------------------------------------------------
+// The correctly rounded f32 division on gfx9 is
+//   div_scale(den), div_scale(num), rcp, fma, fma, mul, fma, fma, fma, div_fmas, div_fixup.
+// div_scale rescales an operand only when the denominator, its reciprocal or the quotient is
+// denormal, the numerator is within 2^23 ulp of the denormal range or the exponents differ by
+// 96 or more; div_fmas is then a plain fma, and div_fixup changes nothing unless an operand
+// is zero, infinite or NaN.  For 2^-60 <= |b| <= 2^60 and a == 0 or 2^-90 <= |a| <= 2^60 the
+// steps below therefore give the same bits (a zero numerator gives a zero of either sign).
+__device__ __forceinline__ float div_rn(float a, float b)
+{
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const float y = __builtin_fmaf(e, y0, y0);
+    float q = a * y;
+    float r = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+
+// The correctly rounded f32 square root on gfx9 takes v_sqrt (within 1 ulp) and keeps it or
+// one of its two neighbours by the signs of the residuals; inputs below 2^-96 are scaled
+// first and zero and infinity pass through.  For 2^-90 <= x <= 2^90 these steps are the same.
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = u2f(f2u(s) - 1u), hi = u2f(f2u(s) + 1u);
+    const float rlo = __builtin_fmaf(-lo, s, x), rhi = __builtin_fmaf(-hi, s, x);
+    const float r = (rlo <= 0.0f) ? lo : s;
+    return (rhi > 0.0f) ? hi : r;
+}
+
+// Math policy of the trace kernel.  Its divisions and square roots have operands inside the
+// ranges above: scenes keep coordinates and sizes within 2^20 m, cell edges within
+// [2^-20, 2^20] m and every ray origin strictly inside the grid (checked when the scene is
+// built), so
+//   walk set-up    |k*edge - cur| / |dir|: numerator 0 or in [2^-44, 2^22], |dir| in (1e-6, 1]
+//                  edge / |dir|
+//   intersection   1 / det with 1e-6 <= |det| <= |E1| |E2| < 2^42
+//   normalize      1 / sqrt(v.v) for |v| = 1 up to rounding
+//   log            (m - 1) / (m + 1) with m in [0.7, 1.42)
+//   erfinv tail    sqrt(w), w in [5, 90]
+//   Buie angle     theta[mrad] / 1000 with theta > 2^-60
+struct gfx {
+    __device__ static float div(float a, float b) { return div_rn(a, b); }
+    __device__ static float sqrt(float x) { return sqrt_rn(x); }
+};
+
+// ---------------------------------------------------------------------------------------
+// Grid walk: NotCollision + Intersect (rectgrid_raytracing.cu:21-117)
+// ---------------------------------------------------------------------------------------
+// Amanatides-Woo walk from the cell of the origin (which is never tested); the result is the
+// first candidate in CSR order whose parallelogram the ray hits.  The ray's own heliostat is
+// skipped (the reference rejects it through t < 1e-6, which holds in exact arithmetic).  The
+// walk also ends, with the same result, once the cell index on an axis has passed the
+// occupied index range in the direction of travel: every later cell is empty.
+
+// Cell of a ray origin, shared by the two walks of a ray: make_int3((o - pos) / interval)
+// (rectgrid_raytracing.cu:43).  With power-of-two cell edges x / 2^k == x * 2^-k.
 struct cell3 {
     float rx, ry, rz; // o - grid pos
     int gx, gy, gz;
+    int addr;         // (gx * gny + gy) * gnz + gz
 };
 __device__ __forceinline__ cell3 start_cell(const TraceArgs &a, v3 o)
 {
@@ -67,63 +123,72 @@ __device__ __forceinline__ cell3 start_cell(const TraceArgs &a, v3 o)
         c.gy = trunc_i(c.ry * a.inv_giy);
         c.gz = trunc_i(c.rz * a.inv_giz);
     } else {
-        c.gx = trunc_i(c.rx / a.gix);
-        c.gy = trunc_i(c.ry / a.giy);
-        c.gz = trunc_i(c.rz / a.giz);
+        c.gx = trunc_i(div_rn(c.rx, a.gix));
+        c.gy = trunc_i(div_rn(c.ry, a.giy));
+        c.gz = trunc_i(div_rn(c.rz, a.giz));
     }
+    c.addr = (c.gx * a.gny + c.gy) * a.gnz + c.gz;
     return c;
+}
+
+// tMax of one axis, absDivide(calTMax(...), dir) (rectgrid_raytracing.cu:3-17): calTMax is
+// (idx + 1) * edge - cur for dir >= 0 and cur - idx * edge otherwise, i.e. +-(k * edge - cur)
+// with k = idx + (dir >= 0); IEEE subtraction and division are odd in each operand, so the
+// absolute quotient is the same value.  absDivide returns float(INT_MAX) for |dir| <= 1e-6.
+__device__ __forceinline__ float first_crossing(float dir, bool fwd, float edge, int idx, float cur)
+{
+    const float t = (float)(idx + (fwd ? 1 : 0)) * edge - cur;
+    const float q = fabs_(div_rn(t, dir));
+    return fabs_(dir) <= EPS_LE ? INT_MAX_F : q;
+}
+// tDelta of one axis, absDivide(interval, dir)
+__device__ __forceinline__ float cell_crossing(float dir, float edge)
+{
+    const float q = fabs_(div_rn(edge, dir));
+    return fabs_(dir) <= EPS_LE ? INT_MAX_F : q;
 }
 
 __device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
 {
-    int gx = c.gx, gy = c.gy, gz = c.gz;
-    int sx = (d.x >= 0.0f) ? 1 : -1, sy = (d.y >= 0.0f) ? 1 : -1, sz = (d.z >= 0.0f) ? 1 : -1;
-    // admissible index window per axis: inside the grid and not past the occupied range
-    int lox = sx > 0 ? 0 : a.occ_lo[0], hix = sx > 0 ? a.occ_hi[0] : a.gnx - 1;
-    int loy = sy > 0 ? 0 : a.occ_lo[1], hiy = sy > 0 ? a.occ_hi[1] : a.gny - 1;
-    int loz = sz > 0 ? 0 : a.occ_lo[2], hiz = sz > 0 ? a.occ_hi[2] : a.gnz - 1;
-    if ((sx > 0 ? gx > hix : gx < lox) || (sy > 0 ? gy > hiy : gy < loy) ||
-        (sz > 0 ? gz > hiz : gz < loz))
-        return -1;
-    float mx = abs_divide(cal_tmax(d.x, a.gix, gx, c.rx), d.x);
-    float my = abs_divide(cal_tmax(d.y, a.giy, gy, c.ry), d.y);
-    float mz = abs_divide(cal_tmax(d.z, a.giz, gz, c.rz), d.z);
-    // tDelta of an axis is needed only once that axis has stepped and the walk goes on
-    float dx = -1.0f, dy = -1.0f, dz = -1.0f;
+    const bool fx = d.x >= 0.0f, fy = d.y >= 0.0f, fz = d.z >= 0.0f;
+    // Steps left on each axis before its index leaves the admissible window: the grid,
+    // cut at the occupied range in the direction of travel.  The origin lies inside the
+    // grid, so only that end can be crossed.
+    int nx = fx ? a.occ_hi[0] - c.gx : c.gx - a.occ_lo[0];
+    int ny = fy ? a.occ_hi[1] - c.gy : c.gy - a.occ_lo[1];
+    int nz = fz ? a.occ_hi[2] - c.gz : c.gz - a.occ_lo[2];
+    if ((nx | ny | nz) < 0) return -1;
+    float mx = first_crossing(d.x, fx, a.gix, c.gx, c.rx);
+    float my = first_crossing(d.y, fy, a.giy, c.gy, c.ry);
+    float mz = first_crossing(d.z, fz, a.giz, c.gz, c.rz);
+    const float dx = cell_crossing(d.x, a.gix), dy = cell_crossing(d.y, a.giy),
+                dz = cell_crossing(d.z, a.giz);
+    // cell index strides of one step on each axis
+    const int yz = a.gny * a.gnz;
+    const int ax = fx ? yz : -yz, ay = fy ? a.gnz : -a.gnz, az = fz ? 1 : -1;
+    int addr = c.addr;
     for (;;) {
-        int axis;
-        if (mx < my) axis = (mx < mz) ? 0 : 2;
-        else axis = (my < mz) ? 1 : 2;
-        if (axis == 0) {
-            gx += sx;
-            if (gx < lox || gx > hix) return -1;
-        } else if (axis == 1) {
-            gy += sy;
-            if (gy < loy || gy > hiy) return -1;
-        } else {
-            gz += sz;
-            if (gz < loz || gz > hiz) return -1;
-        }
-        int addr = (gx * a.gny + gy) * a.gnz + gz;
-        int i0 = a.cell_start[addr], i1 = a.cell_start[addr + 1];
+        // axis of the next crossing, ties as in the reference: x only if strictly first
+        const bool sx = mx < my && mx < mz;
+        const bool sy = !(mx < my) && my < mz;
+        const bool sz = !sx && !sy;
+        nx -= sx ? 1 : 0;
+        ny -= sy ? 1 : 0;
+        nz -= sz ? 1 : 0;
+        if ((nx | ny | nz) < 0) return -1;
+        addr += sx ? ax : (sy ? ay : az);
+        mx = sx ? mx + dx : mx;
+        my = sy ? my + dy : my;
+        mz = sz ? mz + dz : mz;
+        const int i0 = a.cell_start[(uint32_t)addr], i1 = a.cell_start[(uint32_t)addr + 1u];
         for (int i = i0; i < i1; ++i) {
-            int h = a.cell_items[i];
+            const int h = a.cell_items[i];
             if (h == self) continue;
-            float4 A = a.geo[3 * h], E1 = a.geo[3 * h + 1], E2 = a.geo[3 * h + 2];
+            const float4 A = a.geo[3 * h], E1 = a.geo[3 * h + 1], E2 = a.geo[3 * h + 2];
             float t, u, v;
-            if (parallelogram(o, d, mk(A.x, A.y, A.z), mk(E1.x, E1.y, E1.z), mk(E2.x, E2.y, E2.z),
-                              t, u, v))
+            if (parallelogram<gfx>(o, d, mk(A.x, A.y, A.z), mk(E1.x, E1.y, E1.z),
+                                   mk(E2.x, E2.y, E2.z), t, u, v))
                 return h;
-        }
-        if (axis == 0) {
-            if (dx < 0.0f) dx = abs_divide(a.gix, d.x);
-            mx += dx;
-        } else if (axis == 1) {
-            if (dy < 0.0f) dy = abs_divide(a.giy, d.y);
-            my += dy;
-        } else {
-            if (dz < 0.0f) dz = abs_divide(a.giz, d.z);
-            mz += dz;
         }
     }
 }
@@ -133,10 +198,10 @@ __device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
 __device__ __forceinline__ v3 perturbed_normal(const u4 &x, float sigma, const frame &hf)
 {
     float st, ct, sp, cp;
-    const float ths = gauss_(u01(x.z)) * sigma;
+    const float ths = gauss_<gfx>(u01(x.z)) * sigma;
     sincos_small(ths, st, ct);
     sincos2pi(u01(x.w), sp, cp);
-    return normalize(to_world(angle2xyz(st, ct, sp, cp), hf));
+    return normalize<gfx>(to_world(angle2xyz(st, ct, sp, cp), hf));
 }
 
 // receiver_drawing (rectgrid_raytracing.cu:135-153): pixel of the hit, -1 for a miss; u or
@@ -144,7 +209,7 @@ __device__ __forceinline__ v3 perturbed_normal(const u4 &x, float sigma, const f
 __device__ __forceinline__ int receiver_pixel(const TraceArgs &a, v3 orig, v3 refl)
 {
     float t, u, v;
-    if (!parallelogram(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) return -1;
+    if (!parallelogram<gfx>(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) return -1;
     int row = trunc_i(u * (float)a.res_y), col = trunc_i(v * (float)a.res_x);
     row = min(row, a.res_y - 1);
     col = min(col, a.res_x - 1);
@@ -239,16 +304,16 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         u4 x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3}, a.key0, a.key1);
         float th;
         if (SUN == OM_SUN_PILLBOX) {
-            th = a.pillbox * sqrt_(u01(x.x));
+            th = a.pillbox * sqrt_rn(u01(x.x));
         } else {
             float v = u01(x.x) * a.buie_A;
             if (v < a.buie_int465) {
                 float4 b = buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
                 th = (v - b.x) * b.z + b.w;
             } else {
-                th = pow_((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
+                th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
             }
-            th = th / 1000.0f;
+            th = div_rn(th, 1000.0f);
         }
         float st, ct, sp, cp;
         sincos_small(th, st, ct);
@@ -266,7 +331,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
                 continue;
             }
             const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
-            const v3 refl = normalize(reflect(-to_sun, nrm));
+            const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
             int bl = dda(a, orig, c0, refl, h);
             int px = bl < 0 ? receiver_pixel(a, orig, refl) : -1;
             a.rec_shade[rid] = -1;
@@ -276,7 +341,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         }
         if (dda(a, orig, c0, to_sun, h) >= 0) continue;
         const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
-        const v3 refl = normalize(reflect(-to_sun, nrm));
+        const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
         if (dda(a, orig, c0, refl, h) >= 0) continue;
         const int px = receiver_pixel(a, orig, refl);
         if (px < 0) continue;
@@ -298,7 +363,10 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
     // One 64-bit atomic per hit into this XCD's packed word: v = 2^E + q.  The old value
     // tells whether the add carried out of the energy field (bits 0..E-1) or wrapped the
     // word; both events are counted on the side so that the reduction decodes exactly.
-    unsigned long long *part = a.part + (size_t)xcc_id() * a.npx;
+    // copy: the XCD's own block of copies, one of them per workgroup (workgroup b runs on
+    // XCD b % 8, so b / 8 counts the workgroups of one XCD)
+    const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
+    unsigned long long *part = a.part + (size_t)copy * a.copy_stride;
     const int E = a.pack_shift;
     const unsigned long long lowmask = (1ull << E) - 1ull;
     unsigned long long old[RPT_MAX];
@@ -309,7 +377,8 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
             const int px = s_hpx[k * WG + threadIdx.x];
             if (px >= 0) {
                 const unsigned long long v = (1ull << E) + s_hq[k * WG + threadIdx.x];
-                old[k] = __hip_atomic_fetch_add(part + px, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                old[k] = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, v, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
@@ -331,18 +400,20 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
 // Over all copies, with A carries out of the energy field and W wraps of the word:
 //   count = sum(word >> E) + W * 2^(64-E) - A,    energy = sum(word & (2^E - 1)) + A * 2^E.
 __global__ void reduce_packed_kernel(unsigned long long *part, uint32_t *carry, uint32_t *wrap, int E,
-                                     uint32_t *counts, unsigned long long *energy, int npx)
+                                     uint32_t *counts, unsigned long long *energy, int npx, int ncopy,
+                                     int copy_stride, int px_stride)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npx) return;
     const unsigned long long lowmask = (1ull << E) - 1ull;
     unsigned long long n = 0, e = 0;
-#pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        unsigned long long w = part[(size_t)x * npx + i];
-        n += w >> E;
-        e += w & lowmask;
-        part[(size_t)x * npx + i] = 0ull;
+#pragma unroll 8
+    for (int x = 0; x < ncopy; ++x) {
+        unsigned long long &w = part[(size_t)x * copy_stride + (size_t)i * px_stride];
+        const unsigned long long wv = w;
+        n += wv >> E;
+        e += wv & lowmask;
+        w = 0ull;
     }
     const unsigned long long A = carry[i], W = wrap[i];
     n = n + (W << (64 - E)) - A;
@@ -538,9 +609,15 @@ __global__ void smooth_kernel(const float *in, float *out, int width, int height
     out[idx] = ((value - smax - smin) / (float)(cnt - 2 * k)) * scale;
 }
 
+// The functions as the trace kernel evaluates them (with its math policy).  fn 10 divides
+// pairs: out[i] = in[2i] / in[2i+1] for i < n/2.
 __global__ void math_kernel(int fn, const float *in, float *out, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fn == 10) {
+        if (i < n / 2) out[i] = div_rn(in[2 * i], in[2 * i + 1]);
+        return;
+    }
     if (i >= n) return;
     float x = in[i], s, c;
     switch (fn) {
@@ -549,11 +626,11 @@ __global__ void math_kernel(int fn, const float *in, float *out, int n)
     case 2: sincos2pi(x, s, c); out[i] = c; break;
     case 3: sincos_small(x, s, c); out[i] = s; break;
     case 4: sincos_small(x, s, c); out[i] = c; break;
-    case 5: out[i] = log_(x); break;
+    case 5: out[i] = log_<gfx>(x); break;
     case 6: out[i] = exp_(x); break;
-    case 7: out[i] = erfinv_(x); break;
-    case 8: out[i] = gauss_(x); break;
-    default: out[i] = sqrt_(x); break;
+    case 7: out[i] = erfinv_<gfx>(x); break;
+    case 8: out[i] = gauss_<gfx>(x); break;
+    default: out[i] = sqrt_rn(x); break;
     }
 }
 
@@ -652,6 +729,10 @@ om_status launch_smooth(const float *in, float *out, int w, int h, int radius, f
 
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st)
 {
+    if (fn < 0 || fn > 10) {
+        set_error("math function index must be in 0..10");
+        return OM_ERR_INVALID;
+    }
     hipLaunchKernelGGL(math_kernel, dim3((n + WG - 1) / WG), dim3(WG), 0, st, fn, in, out, n);
     OM_HIP(hipGetLastError());
     return OM_OK;
@@ -707,7 +788,15 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
     om_status rc;
     if (clear && (rc = clear_maps(out, npx, st)) != OM_OK) return rc;
     if (s->prm.accumulate == OM_ACC_PACKED) {
-        const size_t n16 = 16 * (size_t)npx;
+        // 8 XCDs x 2^shift copies of the packed map, as many as fit in PART_BYTES (at least
+        // one per XCD): same-pixel atomics of concurrent workgroups then mostly land on
+        // different words.
+        int shift = 0;
+        while (shift < COPY_SHIFT_MAX && ((size_t)16 << shift) * npx * sizeof(unsigned long long) <= PART_BYTES)
+            ++shift;
+        const int ncopy = 8 << shift;
+        const int copy_stride = npx, px_stride = 1;
+        const size_t n16 = (size_t)(ncopy - 1) * copy_stride + (size_t)(npx - 1) * px_stride + 1;
         const size_t bytes = n16 * sizeof(unsigned long long) + 2 * (size_t)npx * sizeof(uint32_t);
         const bool fresh = s->ctx->parts.bytes < bytes;
         if ((rc = ensure_buf(s->ctx->parts, bytes)) != OM_OK) return rc;
@@ -715,6 +804,9 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
         a.carry = (uint32_t *)(a.part + n16);
         a.wrap = a.carry + npx;
         a.npx = npx;
+        a.copy_shift = shift;
+        a.copy_stride = copy_stride;
+        a.px_stride = px_stride;
         a.acc_packed = 1;
         a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
         if (fresh) {
@@ -725,7 +817,8 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
         }
         if ((rc = launch_trace(s, a, false, st)) != OM_OK) return rc;
         hipLaunchKernelGGL(reduce_packed_kernel, dim3((npx + WG - 1) / WG), dim3(WG), 0, st, a.part, a.carry,
-                           a.wrap, a.pack_shift, out->counts, (unsigned long long *)out->energy, npx);
+                           a.wrap, a.pack_shift, out->counts, (unsigned long long *)out->energy, npx,
+                           ncopy, a.copy_stride, a.px_stride);
         OM_HIP(hipGetLastError());
         return OM_OK;
     }

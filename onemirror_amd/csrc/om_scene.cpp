@@ -2,6 +2,7 @@
 // (SolarScene::InitContent), upload of the scene to HBM, and the C ABI.
 //
 // Reference: Hu_smooth/ClassedSunRay (file:line cited per function).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -669,6 +670,50 @@ void free_scene_dev(om_scene *s)
     dfree(s->d_tiles);
 }
 
+// Coordinates and sizes the tracer accepts: finite and within 2^20 m; cell edges within
+// [2^-20, 2^20] m.  Inside these bounds every division and square root of the trace kernel
+// has operands in the range of its short exact sequences.
+constexpr float COORD_MAX = 1048576.0f;
+constexpr float EDGE_MIN = 1.0f / 1048576.0f;
+
+bool bounded(const float *x, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!(std::fabs(x[i]) <= COORD_MAX)) return false;
+    return true;
+}
+
+// Every ray origin must lie strictly inside the grid: a walk indexes the cell arrays from
+// the origin's cell and checks only the axes it steps (rectgrid_raytracing.cu:43-48; an
+// origin outside the grid reads outside the arrays there).  Origins lie in the mirror
+// rectangle v0 v1 v2 v3; the slack covers the rounding of an origin and of its cell index.
+om_status check_in_grid(const om_scene *s)
+{
+    for (int i = 0; i < s->H; ++i) {
+        const v3 c[4] = {s->v0[i], s->v1[i], s->v2[i], s->v3[i]};
+        const double sx = s->size[3 * i], sy = s->size[3 * i + 1], sz = s->size[3 * i + 2];
+        const double R = 0.5 * std::sqrt(sx * sx + sy * sy + sz * sz);
+        for (int a = 0; a < 3; ++a) {
+            double lo = HUGE_VAL, hi = -HUGE_VAL;
+            for (int k = 0; k < 4; ++k) {
+                const double x = a == 0 ? c[k].x : (a == 1 ? c[k].y : c[k].z);
+                lo = std::min(lo, x);
+                hi = std::max(hi, x);
+            }
+            const double gp = s->grid_pos[a], ext = (double)s->gn[a] * s->grid_interval[a];
+            const double slack = 1e-5 * (std::fabs((double)s->pos[3 * i + a]) + R + std::fabs(gp) + ext);
+            if (!(lo - slack > gp && hi + slack < gp + ext)) {
+                char msg[160];
+                std::snprintf(msg, sizeof msg, "heliostat %d reaches outside the grid on axis %c "
+                              "(mirror %.6g..%.6g, grid %.6g..%.6g)", i, "xyz"[a], lo, hi, gp, gp + ext);
+                om::set_error(msg);
+                return OM_ERR_UNSUPPORTED;
+            }
+        }
+    }
+    return OM_OK;
+}
+
 om_status validate(const om_scene_desc *d, const om_params *p)
 {
     if (!d || !p) return OM_ERR_INVALID;
@@ -715,6 +760,22 @@ om_status validate(const om_scene_desc *d, const om_params *p)
             om::set_error("grid interval must be positive");
             return OM_ERR_INVALID;
         }
+    const size_t H = (size_t)d->n_helio;
+    if (!bounded(d->helio_pos, 3 * H) || !bounded(d->helio_size, 3 * H) || !bounded(d->helio_gap, 2 * H) ||
+        !bounded(d->grid_pos, 3) || !bounded(d->grid_size, 3) || !bounded(d->recv_pos, 3) ||
+        !bounded(d->recv_size, 3)) {
+        om::set_error("coordinates and sizes must be finite and within 2^20 m");
+        return OM_ERR_UNSUPPORTED;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (!(d->grid_interval[a] >= EDGE_MIN && d->grid_interval[a] <= COORD_MAX)) {
+            om::set_error("grid interval must be within [2^-20, 2^20] m");
+            return OM_ERR_UNSUPPORTED;
+        }
+    if (!bounded(d->sun_dir, 3) || (d->sun_dir[0] == 0.0f && d->sun_dir[1] == 0.0f && d->sun_dir[2] == 0.0f)) {
+        om::set_error("sun direction must be finite and nonzero");
+        return OM_ERR_INVALID;
+    }
     return OM_OK;
 }
 
@@ -727,6 +788,7 @@ om_status build_all(om_scene *s)
     if ((st = build_grid(s)) != OM_OK) return st;
     if ((st = build_receiver(s)) != OM_OK) return st;
     aim_heliostats(s);
+    if ((st = check_in_grid(s)) != OM_OK) return st;
     micro_layout(s);
     if (s->H > 0 && s->H >= (1 << 28)) {
         om::set_error("too many heliostats");
@@ -942,10 +1004,24 @@ extern "C" om_status om_scene_get_info(const om_scene *s, om_scene_info *info)
 extern "C" om_status om_scene_set_sun(om_scene *s, const float sun_dir[3], float dni)
 {
     if (!s || !sun_dir) return OM_ERR_INVALID;
+    if (!bounded(sun_dir, 3) || (sun_dir[0] == 0.0f && sun_dir[1] == 0.0f && sun_dir[2] == 0.0f)) {
+        om::set_error("sun direction must be finite and nonzero");
+        return OM_ERR_INVALID;
+    }
+    float old_sun[3];
+    std::memcpy(old_sun, s->sun_in, 12);
     std::memcpy(s->sun_in, sun_dir, 12);
-    s->dni = dni;
     s->sun = om::normalize(ld3(s->sun_in));
     aim_heliostats(s);
+    om_status st = check_in_grid(s);
+    if (st != OM_OK) {
+        // keep the scene as it was
+        std::memcpy(s->sun_in, old_sun, 12);
+        s->sun = om::normalize(ld3(s->sun_in));
+        aim_heliostats(s);
+        return st;
+    }
+    s->dni = dni;
     flux_scale(s);
     if (!s->ctx) {
         pack_heliostats(s);

@@ -56,6 +56,52 @@ def test_device_math_bitexact(om, fn):
         f"{names[fn]}: {(got != want).sum()} mismatches"
 
 
+def test_device_sqrt_exhaustive(om):
+    """The kernel's short square root equals IEEE sqrt on every float in [1, 4); an input
+    scaled by 4^k (no denormal or overflow on either side) scales its root by 2^k exactly,
+    so this covers the whole range the kernel uses it on."""
+    x = np.arange(0x3F800000, 0x40800000, dtype=np.uint32).view(np.float32)
+    got = om.default_context().math_eval(9, x)
+    want = np.sqrt(x)
+    bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first x {x[bad[:4]]}"
+
+
+def test_device_division_matches_ieee(om):
+    """The kernel's short division equals IEEE division: every reciprocal 1/b for b in
+    [1, 2) (the mantissa decides the result bits, exponents only scale them), and random
+    quotients over the operand ranges the kernel divides in (zero and 2^-90..2^60
+    numerators, 2^-60..2^60 denominators, and the walk's |dir| in (1e-6, 1])."""
+    ctx = om.default_context()
+    b = np.arange(0x3F800000, 0x40000000, dtype=np.uint32).view(np.float32)
+    pairs = np.empty(2 * b.size, np.float32)
+    pairs[0::2] = 1.0
+    pairs[1::2] = b
+    got = ctx.math_eval(10, pairs)[: b.size]
+    want = np.float32(1.0) / b
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), \
+        f"{(got != want).sum()} reciprocal mismatches"
+    rng = np.random.default_rng(12)
+    n = 1 << 22
+    sgn = lambda k: rng.choice(np.array([-1.0, 1.0]), k)
+    num = np.concatenate([2.0 ** rng.uniform(-90, 60, n) * sgn(n),
+                          2.0 ** rng.uniform(-44, 22, n) * sgn(n),
+                          np.zeros(4096)]).astype(np.float32)
+    den = np.concatenate([2.0 ** rng.uniform(-60, 60, n) * sgn(n),
+                          rng.uniform(1e-6, 1.0, n) * sgn(n),
+                          rng.uniform(1e-6, 1.0, 4096)]).astype(np.float32)
+    pairs = np.empty(2 * num.size, np.float32)
+    pairs[0::2] = num
+    pairs[1::2] = den
+    got = ctx.math_eval(10, pairs)[: num.size]
+    want = num / den
+    # a zero numerator may give a zero of either sign: the walk takes |q|, and log's result
+    # does not depend on the sign of (m - 1) / (m + 1) when that is zero
+    ok = (got.view(np.uint32) == want.view(np.uint32)) | ((num == 0) & (got == 0))
+    bad = np.nonzero(~ok)[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first {num[bad[:3]]} / {den[bad[:3]]}"
+
+
 @pytest.fixture(scope="module")
 def small(om):
     d, p = F.config_small()

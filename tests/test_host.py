@@ -186,6 +186,29 @@ def test_abi_error_paths(tmp_path):
     assert L.om_status_string(7) == b"buffer too small"
 
 
+def test_scene_extent_checks():
+    """Scenes the walk cannot index safely are refused: a mirror reaching outside the grid
+    (its rays would start outside the cell arrays), coordinates beyond 2^20 m, cell edges
+    outside [2^-20, 2^20] m, a zero sun vector."""
+    d, p = F.config_small()
+    assert HostScene(d, p).rc == 0
+    dd = dict(d)
+    pos = np.array(d["helio_pos"], np.float32)
+    pos[3, 1] = 0.2  # a 2 m mirror centred 0.2 m above the grid floor (y = 0)
+    dd["helio_pos"] = pos
+    g = HostScene(dd, p)
+    assert g.rc == 6 and b"heliostat 3 reaches outside the grid on axis y" in om.lib().om_last_error()
+    dd = dict(d)
+    dd["grid_pos"] = np.float32([-128.0, -2.0e6, -106.0])
+    assert HostScene(dd, p).rc == 6
+    dd = dict(d)
+    dd["grid_interval"] = np.float32([4.0, 2.0, 1e-7])
+    assert HostScene(dd, p).rc == 6
+    dd = dict(d)
+    dd["sun_dir"] = np.float32([0, 0, 0])
+    assert HostScene(dd, p).rc == 1
+
+
 def test_intercept_ratio_matches_oracle():
     rng = np.random.default_rng(0)
     flux = rng.gamma(1.0, 5.0, (64, 48)).astype(np.float32)
