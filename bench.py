@@ -50,8 +50,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mid", choices=["mid", "small"])
-    ap.add_argument("--cpu-every", type=int, default=5,
-                    help="CPU baseline traces every n-th heliostat of the field")
+    ap.add_argument("--cpu-every", type=int, default=1,
+                    help="CPU baseline traces one block of 40 heliostats in every n")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run an elementwise copy of MIB MiB (reads and writes MIB MiB "
@@ -149,6 +149,14 @@ def main() -> int:
     ops_ray = F0 + C_STEP * S + C_TEST * T + C_SKIP * K
     achieved = R * ops_ray / (trace_ms * 1e-3) / 1e12
     peak = P_VALU / 1e12
+    # HBM bytes per trace launch from the committed rocprof FETCH_SIZE / WRITE_SIZE runs
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"trace_kernel_{args.workload}.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            pj = json.load(f)
+        if pj.get("rays") == R:
+            traffic = pj["hbm_bytes_per_launch"]
 
     out = {
         "metric": METRIC,
@@ -167,13 +175,14 @@ def main() -> int:
                    "map": [info["res_y"], info["res_x"]], "parallelism": "single GPU",
                    "trace_kernel_ms": trace_ms, "receiver_hits": hits},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
                      "ops_per_ray": ops_ray, "dda_steps_per_ray": S, "tests_per_ray": T},
         "cpu_baseline": None if args.no_cpu else {
             "value": cpu_rays / cpu_t, "unit": "rays/s", "cores": nthreads, "kind": "port",
-            "sample": f"{len(blocks)} blocks of {blk} neighbouring heliostats, one in every "
-                      f"{every} of the same field ({nsample} heliostats, {cpu_rays} rays); "
-                      f"oracle FP32 tracer with the same early-exit walk",
+            "sample": (f"the whole field ({nsample} heliostats, {cpu_rays} rays)" if every == 1 else
+                       f"{len(blocks)} blocks of {blk} neighbouring heliostats, one in every "
+                       f"{every} of the same field ({nsample} heliostats, {cpu_rays} rays)")
+                      + "; oracle FP32 tracer with the same early-exit walk, OpenMP",
             "seconds": cpu_t},
     }
     print(json.dumps(out), flush=True)
