@@ -49,9 +49,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mid", choices=["mid", "small"])
-    ap.add_argument("--cpu-every", type=int, default=1,
-                    help="CPU baseline traces one block of 40 heliostats in every n")
+    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large"])
+    ap.add_argument("--cpu-every", type=int, default=0,
+                    help="CPU baseline traces one block of 40 heliostats in every n "
+                         "(default: the whole field, one block in 5 for large)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run an elementwise copy of MIB MiB (reads and writes MIB MiB "
@@ -73,6 +74,11 @@ def main() -> int:
         wl = ("mid: 5000 heliostats 10x10 m, radii 100-800 m, flat 25.6 m receiver, "
               "Buie CSR 0.1 (2048-bin CDF), slope 2.5 mrad, 32x32 Poisson x 40 rays, "
               "128^3 grid, 256x256 map, trimmed mean 5x5/20%")
+    elif args.workload == "large":
+        desc, prm = F.config_large()
+        wl = ("large: 25000 heliostats 12x10 m on a surround field to 1.5 km, cylinder "
+              "receiver r 8 m h 20 m at 200 m (512x256 unrolled map), Buie CSR 0.05, "
+              "slope 2 mrad, ~1000 Poisson microsamples x 40 rays, 256^3 grid")
     else:
         desc, prm = F.config_small()
         wl = ("small: 100 heliostats 2x2 m, pillbox 4.65 mrad, slope 2 mrad, 16x16 Poisson "
@@ -127,7 +133,7 @@ def main() -> int:
     # oracle sample: step/test counts for the cost model and the CPU baseline
     import oracle
     osc = oracle.OracleScene(desc, prm)
-    every = max(1, args.cpu_every)
+    every = args.cpu_every if args.cpu_every > 0 else (5 if args.workload == "large" else 1)
     nthreads = host_threads()
     blk = 40  # heliostats per block: blocks of neighbours keep the threads busy
     blocks = [(h, min(H, h + blk)) for h in range(0, H, blk * every)]

@@ -46,9 +46,9 @@ typedef struct {
     const float *helio_size;   /* [n][3] width (x), thickness (y), height (z)             */
     const float *helio_gap;    /* [n][2] gap between facets                               */
     const int   *helio_matrix; /* [n][2] facet rows, facet columns                        */
-    int   recv_type;           /* 0 rectangle (receiver.cuh:66-101)                       */
-    float recv_pos[3];
-    float recv_size[3];
+    int   recv_type;           /* 0 rectangle (receiver.cuh:66-101), 1 cylinder (new)     */
+    float recv_pos[3];         /* rectangle centre; cylinder: axis point at mid-height    */
+    float recv_size[3];        /* rectangle box size; cylinder (radius, height, unused)    */
     float recv_normal[3];
     int   recv_face;           /* 0..3 (receiver.cpp:30-48)                               */
     float recv_pixel_length;   /* solarenergy::receiver_pixel_length (common_var.cpp:12)  */
@@ -57,6 +57,8 @@ typedef struct {
     float grid_interval[3];
     float sun_dir[3];          /* direction the light travels                             */
     float dni;                 /* W/m^2                                                   */
+    int   recv_res_x, recv_res_y; /* cylinder map columns (azimuth) and rows (height);
+                                     0 = from recv_pixel_length                           */
 } om_scene_desc;
 
 enum { OM_SUN_PILLBOX = 0, OM_SUN_BUIE = 1 };
@@ -199,7 +201,8 @@ float om_intercept_ratio(const float *flux, int width, int height, float total_e
 /* Device self-test of the float math as the trace kernel evaluates it (parity of the
  * kernels' arithmetic with the host).  fn: 0 u01 (input bits as uint32), 1/2 sin/cos(2 pi x),
  * 3/4 sin/cos of a small angle, 5 log, 6 exp, 7 erfinv, 8 Gaussian deviate of a uniform,
- * 9 sqrt, 10 division of pairs (out[i] = in[2i] / in[2i+1], i < n/2).  9 and 10 use the
+ * 9 sqrt, 10 division of pairs (out[i] = in[2i] / in[2i+1], i < n/2), 11 atan2 of pairs
+ * (out[i] = atan2(in[2i], in[2i+1]), the cylinder receiver's azimuth).  9 and 10 use the
  * kernel's short exact sequences, which hold for x in [2^-90, 2^90] and for numerators 0 or
  * in [2^-90, 2^60] over denominators in [2^-60, 2^60].  in/out device arrays of n floats. */
 om_status om_math_eval(om_ctx *ctx, int fn, const float *d_in, float *d_out, int n, void *stream);

@@ -50,6 +50,8 @@ class SceneDesc(ctypes.Structure):
         ("grid_interval", ctypes.c_float * 3),
         ("sun_dir", ctypes.c_float * 3),
         ("dni", ctypes.c_float),
+        ("recv_res_x", ctypes.c_int),
+        ("recv_res_y", ctypes.c_int),
     ]
 
 
@@ -216,6 +218,8 @@ def _desc_struct(desc: dict, keep: list) -> SceneDesc:
     d.grid_interval = _f3(desc["grid_interval"])
     d.sun_dir = _f3(desc["sun_dir"])
     d.dni = float(desc["dni"])
+    d.recv_res_x = int(desc.get("recv_res_x", 0))
+    d.recv_res_y = int(desc.get("recv_res_y", 0))
     return d
 
 
@@ -238,6 +242,7 @@ def scn_parse(path: str) -> dict:
             "grid_size": np.array(d.grid_size[:], np.float32),
             "grid_interval": np.array(d.grid_interval[:], np.float32),
             "sun_dir": np.array(d.sun_dir[:], np.float32), "dni": np.float32(d.dni),
+            "recv_res_x": d.recv_res_x, "recv_res_y": d.recv_res_y,
         }
     finally:
         lib().om_scene_desc_free(ctypes.byref(d))

@@ -290,4 +290,62 @@ OM_HD float gauss_(float u) { return erfinv_<M>(2.0f * u - 1.0f) * 1.41421356f; 
 // angle2xyz (global_function.cuh:175-180) from the sines and cosines
 OM_HD v3 angle2xyz(float st, float ct, float sp, float cp) { return v3{st * cp, ct, st * sp}; }
 
+// atan2(y, x) in (-pi, pi]: t = min/max of |x|, |y| in [0, 1]; above tan(pi/8) the
+// identity atan(t) = pi/4 + atan((t - 1)/(t + 1)) brings the argument to |s| <= 0.4143,
+// where the alternating series s - s^3/3 + ... - s^23/23 is within 2e-11; then the octant.
+OM_HD float atan2_(float y, float x)
+{
+    const float ax = fabs_(x), ay = fabs_(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    const float t = mn / mx;
+    const bool big = t > 0.414213562f;
+    const float s = big ? (t - 1.0f) / (t + 1.0f) : t;
+    const float z = s * s;
+    float p = -0.0434782609f;             // -1/23
+    p = fma_(p, z, 0.0476190476f);        //  1/21
+    p = fma_(p, z, -0.0526315789f);       // -1/19
+    p = fma_(p, z, 0.0588235294f);        //  1/17
+    p = fma_(p, z, -0.0666666667f);       // -1/15
+    p = fma_(p, z, 0.0769230769f);        //  1/13
+    p = fma_(p, z, -0.0909090909f);       // -1/11
+    p = fma_(p, z, 0.111111111f);         //  1/9
+    p = fma_(p, z, -0.142857143f);        // -1/7
+    p = fma_(p, z, 0.2f);                 //  1/5
+    p = fma_(p, z, -0.333333333f);        // -1/3
+    float r = fma_(p * z, s, s);
+    if (big) r = r + 0.785398163f;
+    if (ay > ax) r = 1.57079633f - r;
+    if (x < 0.0f) r = 3.14159265f - r;
+    return y < 0.0f ? -r : r;
+}
+
+// Cylinder receiver (new: the reference's CylinderReceiver::GIntersect is an empty stub,
+// receiver.cuh:103-122).  Vertical axis through base = (cx, y0, cz), radius^2 r2, height h;
+// only the outer surface receives: the entering root of |(o + t d - base)_xz| = r taken
+// from the closest approach tc (well conditioned far from the axis), 0 <= y - y0 <= h.
+// Pixel: column = azimuth atan2(z, x) over [0, 2 pi) x res_x, row = (y - y0) / h x res_y.
+OM_HD int cylinder_pixel(v3 o, v3 d, v3 base, float r2, float h, int res_x, int res_y)
+{
+    const float ox = o.x - base.x, oz = o.z - base.z;
+    const float a = d.x * d.x + d.z * d.z;
+    if (!(a >= EPS_LT)) return -1;
+    const float tc = -(ox * d.x + oz * d.z) / a;
+    const float px = ox + tc * d.x, pz = oz + tc * d.z;
+    const float d2 = px * px + pz * pz;
+    if (!(d2 < r2)) return -1;
+    const float t = tc - sqrt_((r2 - d2) / a);
+    if (t < EPS_LT) return -1;
+    const float hx = ox + t * d.x, hz = oz + t * d.z;
+    const float hy = (o.y + t * d.y) - base.y;
+    if (hy < 0.0f || hy > h) return -1;
+    float th = atan2_(hz, hx);
+    if (th < 0.0f) th = th + 6.28318531f;
+    int col = trunc_i(th * 0.159154943f * (float)res_x);
+    int row = trunc_i(hy / h * (float)res_y);
+    col = col < res_x - 1 ? col : res_x - 1;
+    row = row < res_y - 1 ? row : res_y - 1;
+    return row * res_x + col;
+}
+
 } // namespace om

@@ -56,8 +56,11 @@ struct TraceArgs {
     const int *cell_start;
     const int *cell_items;
     const float4 *geo;      // [3H]: (v0, 0), (v1 - v0, 0), (v3 - v0, 0)
-    // receiver (receiver.cuh:79-82)
+    // receiver (receiver.cuh:79-82); cylinder: base of the axis, radius^2, height
+    int recv_type;
     v3 rA, rE1, rE2;
+    v3 cyl_base;
+    float cyl_r2, cyl_h;
     int res_x, res_y;
     // heliostats and work
     const HelioRec *helio;
@@ -110,6 +113,7 @@ struct om_scene {
     float recv_pos[3], recv_size[3], recv_normal_in[3];
     int recv_face = 0;
     float recv_pixel_length = 0.01f;
+    int recv_res_in[2] = {0, 0};
     float grid_pos[3], grid_size[3], grid_interval[3];
     float sun_in[3];
     float dni = 1000.0f;
@@ -123,6 +127,9 @@ struct om_scene {
     om::v3 rv[4], recv_normal, focus;
     int geo_px = 0, res_x = 0, res_y = 0;
     float recv_pixel = 0.0f;
+    float srec = 0.0f;            // pixel area
+    om::v3 cyl_base{0, 0, 0};
+    float cyl_r2 = 0.0f, cyl_h = 0.0f;
     int nbins = 0;
     std::vector<float> cdf_x, cdf_y, slope;
     float buie_int465 = 0, buie_A = 0, buie_B = 0, buie_C = 0, buie_inv_gp1 = 0, buie_gamma = 0;

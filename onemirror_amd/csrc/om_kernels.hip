@@ -208,6 +208,7 @@ __device__ __forceinline__ v3 perturbed_normal(const u4 &x, float sigma, const f
 // v equal to 1 lands on the last row or column
 __device__ __forceinline__ int receiver_pixel(const TraceArgs &a, v3 orig, v3 refl)
 {
+    if (a.recv_type == 1) return cylinder_pixel(orig, refl, a.cyl_base, a.cyl_r2, a.cyl_h, a.res_x, a.res_y);
     float t, u, v;
     if (!parallelogram<gfx>(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) return -1;
     int row = trunc_i(u * (float)a.res_y), col = trunc_i(v * (float)a.res_x);
@@ -609,13 +610,13 @@ __global__ void smooth_kernel(const float *in, float *out, int width, int height
     out[idx] = ((value - smax - smin) / (float)(cnt - 2 * k)) * scale;
 }
 
-// The functions as the trace kernel evaluates them (with its math policy).  fn 10 divides
-// pairs: out[i] = in[2i] / in[2i+1] for i < n/2.
+// The functions as the trace kernel evaluates them (with its math policy).  fn 10 and 11
+// take pairs: out[i] = in[2i] / in[2i+1] and atan2(in[2i], in[2i+1]) for i < n/2.
 __global__ void math_kernel(int fn, const float *in, float *out, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (fn == 10) {
-        if (i < n / 2) out[i] = div_rn(in[2 * i], in[2 * i + 1]);
+    if (fn == 10 || fn == 11) {
+        if (i < n / 2) out[i] = fn == 10 ? div_rn(in[2 * i], in[2 * i + 1]) : atan2_(in[2 * i], in[2 * i + 1]);
         return;
     }
     if (i >= n) return;
@@ -729,8 +730,8 @@ om_status launch_smooth(const float *in, float *out, int w, int h, int radius, f
 
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st)
 {
-    if (fn < 0 || fn > 10) {
-        set_error("math function index must be in 0..10");
+    if (fn < 0 || fn > 11) {
+        set_error("math function index must be in 0..11");
         return OM_ERR_INVALID;
     }
     hipLaunchKernelGGL(math_kernel, dim3((n + WG - 1) / WG), dim3(WG), 0, st, fn, in, out, n);

@@ -427,11 +427,44 @@ om_status build_grid(om_scene *s)
     return OM_OK;
 }
 
+// Cylinder receiver (new; CylinderReceiver is an empty stub in receiver.cuh:103-122): axis
+// through recv_pos (mid-height), radius size.x, height size.y.  The map unrolls the outer
+// surface, res_x columns over the azimuth and res_y rows over the height, given or from
+// geo = int(1 / pixel length) as for rectangles.  Heliostats aim at the axis point.
+om_status build_cylinder(om_scene *s, int geo)
+{
+    const float r = s->recv_size[0], h = s->recv_size[1];
+    if (!(r > 0.0f && h > 0.0f)) {
+        om::set_error("cylinder receiver needs a positive radius and height");
+        return OM_ERR_INVALID;
+    }
+    int rx = s->recv_res_in[0], ry = s->recv_res_in[1];
+    if (rx <= 0 || ry <= 0) {
+        rx = (int)(6.28318531f * r * (float)geo);
+        ry = (int)(h * (float)geo);
+    }
+    if (rx <= 0 || ry <= 0 || (long long)rx * ry >= (1LL << 28)) {
+        om::set_error("cylinder map must have 1 .. 2^28 pixels");
+        return OM_ERR_INVALID;
+    }
+    s->res_x = rx;
+    s->res_y = ry;
+    s->cyl_base = om::mk(s->recv_pos[0], s->recv_pos[1] - h / 2, s->recv_pos[2]);
+    s->cyl_r2 = r * r;
+    s->cyl_h = h;
+    s->recv_pixel = h / (float)ry;
+    s->srec = (6.28318531f * r / (float)rx) * (h / (float)ry);
+    s->focus = ld3(s->recv_pos);
+    s->recv_normal = om::mk(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < 4; ++i) s->rv[i] = om::mk(0.0f, 0.0f, 0.0f);
+    return OM_OK;
+}
+
 // RectangleReceiver::CInit (receiver.cpp:24-117)
 om_status build_receiver(om_scene *s)
 {
-    if (s->recv_type != 0) {
-        om::set_error("only rectangle receivers (type 0) are supported");
+    if (s->recv_type != 0 && s->recv_type != 1) {
+        om::set_error("only rectangle (type 0) and cylinder (type 1) receivers are supported");
         return OM_ERR_UNSUPPORTED;
     }
     if (!(s->recv_pixel_length > 0.0f)) {
@@ -439,12 +472,14 @@ om_status build_receiver(om_scene *s)
         return OM_ERR_INVALID;
     }
     int geo = (int)(1.0f / s->recv_pixel_length);
-    if (geo <= 0) {
+    if (geo <= 0 && !(s->recv_type == 1 && s->recv_res_in[0] > 0 && s->recv_res_in[1] > 0)) {
         om::set_error("receiver pixel length above 1 m");
         return OM_ERR_UNSUPPORTED;
     }
     s->geo_px = geo;
+    if (s->recv_type == 1) return build_cylinder(s, geo);
     s->recv_pixel = 1.0f / (float)geo;
+    s->srec = s->recv_pixel * s->recv_pixel;
     v3 sz = ld3(s->recv_size), p = ld3(s->recv_pos);
     v3 ln, v[4];
     switch (s->recv_face) {
@@ -574,7 +609,7 @@ void flux_scale(om_scene *s)
     float Ssub = s->prm.helio_pixel_length * s->prm.helio_pixel_length;
     float rou = s->prm.reflectivity;
     int Nc = s->prm.rays_per_sample;
-    float Srec = s->recv_pixel * s->recv_pixel;
+    float Srec = s->srec;
     total *= rou / Srec;
     s->total_energy = total;
     s->scale = Id * Ssub * rou / (float)Nc / Srec;
@@ -900,6 +935,10 @@ TraceArgs make_trace_args(const om_scene *s)
     a.cell_start = s->d_cell_start;
     a.cell_items = s->d_cell_items;
     a.geo = s->d_geo;
+    a.recv_type = s->recv_type;
+    a.cyl_base = s->cyl_base;
+    a.cyl_r2 = s->cyl_r2;
+    a.cyl_h = s->cyl_h;
     a.rA = s->rv[0];
     a.rE1 = s->rv[1] - s->rv[0];
     a.rE2 = s->rv[3] - s->rv[0];
@@ -934,6 +973,8 @@ static om_status scene_from_desc(om_ctx *ctx, const om_scene_desc *d, const om_p
     std::memcpy(s->recv_normal_in, d->recv_normal, 12);
     s->recv_face = d->recv_face;
     s->recv_pixel_length = d->recv_pixel_length;
+    s->recv_res_in[0] = d->recv_res_x;
+    s->recv_res_in[1] = d->recv_res_y;
     std::memcpy(s->grid_pos, d->grid_pos, 12);
     std::memcpy(s->grid_size, d->grid_size, 12);
     std::memcpy(s->grid_interval, d->grid_interval, 12);

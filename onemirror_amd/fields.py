@@ -15,6 +15,12 @@ Workloads (heliostat counts, sizes, sunshape, microsamples, grid and map sizes):
   mid      5,000 heliostats 10x10 m, radii 100-800 m, Buie CSR 0.1 with a 2,048-bin CDF,
            slope 2.5 mrad, 32x32 Poisson microsamples x 40 rays, 128^3 grid, 256x256 map,
            seeded sun position.
+  large    25,000 heliostats 12x10 m on a surround field (radii 100-1,500 m), cylinder
+           receiver r = 8 m, h = 20 m centred 200 m up, 512x256 map over the unrolled
+           surface, Buie CSR 0.05 (2,048-bin CDF), slope 2 mrad, ~1,000 Poisson
+           microsamples x 40 rays (about 1e9 rays), 256^3 grid, seeded sun position.
+  small_cylinder  100 heliostats 2x2 m around a cylinder r = 2 m, h = 4 m at 30 m
+           (128x64 map): the cylinder path at test size.
   face2face  the single-heliostat scene of userData/face2face_shadow.scn.
 """
 from __future__ import annotations
@@ -169,6 +175,47 @@ def config_mid(seed: int = 2, n_helio: int = 5000) -> tuple:
                pillbox_half_angle=4.65e-3, csr=0.1, buie_bins=2048, slope_sigma=2.5e-3,
                micro_mode=POISSON, helio_pixel_length=10.0 / 32.0, poisson_k=8,
                rays_per_sample=40, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
+    return desc, prm
+
+
+def seeded_sun(seed: int, elev=(30.0, 70.0), azim=(120.0, 240.0)) -> np.ndarray:
+    rng = np.random.Generator(np.random.Philox(seed))
+    e = float(rng.uniform(*elev))
+    a = float(rng.uniform(*azim))
+    return sun_direction(e, a)
+
+
+def config_large(seed: int = 5, n_helio: int = 25000) -> tuple:
+    """The 25,000-heliostat surround field with a cylinder receiver (about 1e9 rays)."""
+    f = make_field(n_helio, 12.0, 10.0, 6.0, 100.0, 1500.0, 360.0, 0.1, seed, s_min=16.0)
+    desc = dict(f)
+    desc.update(recv_type=1, recv_pos=np.array([0.0, 200.0, 0.0], np.float32),
+                recv_size=np.array([8.0, 20.0, 0.0], np.float32),
+                recv_normal=np.array([0.0, 0.0, -1.0], np.float32), recv_face=0,
+                recv_pixel_length=np.float32(0.1), recv_res_x=512, recv_res_y=256,
+                sun_dir=seeded_sun(seed), dni=np.float32(1000.0))
+    desc.update(grid_for(f["helio_pos"], f["helio_size"], 220.0, 256))
+    prm = dict(seed=0x0BADC0DE5EED0001, scene_index=0, sunshape=BUIE,
+               pillbox_half_angle=4.65e-3, csr=0.05, buie_bins=2048, slope_sigma=2e-3,
+               micro_mode=POISSON, helio_pixel_length=0.343, poisson_k=8,
+               rays_per_sample=40, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
+    return desc, prm
+
+
+def config_small_cylinder(seed: int = 6) -> tuple:
+    """100 heliostats around a small cylinder: the cylinder receiver at test size."""
+    f = make_field(100, 2.0, 2.0, 1.2, 30.0, 60.0, 360.0, 0.1, seed, s_min=3.0)
+    desc = dict(f)
+    desc.update(recv_type=1, recv_pos=np.array([0.0, 30.0, 0.0], np.float32),
+                recv_size=np.array([2.0, 4.0, 0.0], np.float32),
+                recv_normal=np.array([0.0, 0.0, -1.0], np.float32), recv_face=0,
+                recv_pixel_length=np.float32(0.0625), recv_res_x=128, recv_res_y=64,
+                sun_dir=sun_direction(60.0, 180.0), dni=np.float32(1000.0))
+    desc.update(grid_for(f["helio_pos"], f["helio_size"], 40.0, 64))
+    prm = dict(seed=0x00000000CAFEF00D, scene_index=0, sunshape=PILLBOX,
+               pillbox_half_angle=4.65e-3, csr=0.1, buie_bins=1024, slope_sigma=2e-3,
+               micro_mode=POISSON, helio_pixel_length=2.0 / 16.0, poisson_k=8,
+               rays_per_sample=64, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
     return desc, prm
 
 

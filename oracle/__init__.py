@@ -45,6 +45,8 @@ class SceneDescC(ctypes.Structure):
         ("grid_interval", ctypes.c_float * 3),
         ("sun_dir", ctypes.c_float * 3),
         ("dni", ctypes.c_float),
+        ("recv_res_x", ctypes.c_int),
+        ("recv_res_y", ctypes.c_int),
     ]
 
 
@@ -120,6 +122,9 @@ def lib():
     L.oc_rotate_y.argtypes = [_f32p, _f32p, _f32p, _f32p]
     L.oc_ray_parallelogram.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
     L.oc_ray_parallelogram.restype = ctypes.c_int
+    L.oc_cylinder_pixel_batch.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, ctypes.c_float,
+                                          ctypes.c_float, ctypes.c_int, ctypes.c_int, _i32p]
+    L.oc_cylinder_pixel_batch.restype = ctypes.c_int
     L.oc_sample_ray.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 _f32p, _f32p]
     _LIB = L
@@ -141,7 +146,7 @@ def _f3(v):
 # math
 # ---------------------------------------------------------------------------------------
 _MATH = {"u01": 0, "sin2pi": 1, "cos2pi": 2, "sin_small": 3, "cos_small": 4, "log": 5,
-         "exp": 6, "erfinv": 7, "gauss": 8, "sqrt": 9}
+         "exp": 6, "erfinv": 7, "gauss": 8, "sqrt": 9, "div": 10, "atan2": 11}
 
 
 def math(fn: str, x) -> np.ndarray:
@@ -153,6 +158,8 @@ def math(fn: str, x) -> np.ndarray:
         xf = np.ascontiguousarray(x, dtype=np.float32)
     out = np.empty_like(xf)
     lib().oc_math_batch(_MATH[fn], _p(xf, _f32p), _p(out, _f32p), int(xf.size))
+    if fn in ("div", "atan2"):  # pairs (in[2i], in[2i+1])
+        return out[: xf.size // 2].copy()
     return out
 
 
@@ -194,6 +201,8 @@ def _desc_struct(desc: dict, keep: list):
     d.grid_interval = _f3(desc["grid_interval"])
     d.sun_dir = _f3(desc["sun_dir"])
     d.dni = float(desc["dni"])
+    d.recv_res_x = int(desc.get("recv_res_x", 0))
+    d.recv_res_y = int(desc.get("recv_res_y", 0))
     return d
 
 
@@ -325,7 +334,7 @@ class OracleScene:
         energy = np.zeros(npx, np.uint64)
         recs = [None, None, None]
         if records:
-            nr = int(self.total_rays)
+            nr = int(off[h1] - off[h0])
             recs = [np.full(nr, -1, np.int32) for _ in range(3)]
         st = StatsC()
         lib().oc_trace_f32(self._h, int(h0), int(h1), _p(counts, _u32p), _p(energy, _u64p),
@@ -334,8 +343,7 @@ class OracleScene:
         out = {"counts": counts.reshape(self.res_y, self.res_x),
                "energy": energy.reshape(self.res_y, self.res_x), "stats": st.as_dict()}
         if records:
-            a, b = int(off[h0]), int(off[h1])
-            out.update(shade=recs[0][a:b], block=recs[1][a:b], pixel=recs[2][a:b])
+            out.update(shade=recs[0], block=recs[1], pixel=recs[2])
         return out
 
     def trace_f64(self, h0: int = 0, h1: Optional[int] = None, records: bool = False,
@@ -346,7 +354,7 @@ class OracleScene:
         energy = np.zeros(npx, np.float64)
         recs = [None, None, None]
         if records:
-            nr = int(self.total_rays)
+            nr = int(off[h1] - off[h0])
             recs = [np.full(nr, -1, np.int32) for _ in range(3)]
         st = StatsC()
         lib().oc_trace_f64(self._h, int(h0), int(h1), _p(counts, _u32p), _p(energy, _f64p),
@@ -355,8 +363,7 @@ class OracleScene:
         out = {"counts": counts.reshape(self.res_y, self.res_x),
                "energy": energy.reshape(self.res_y, self.res_x), "stats": st.as_dict()}
         if records:
-            a, b = int(off[h0]), int(off[h1])
-            out.update(shade=recs[0][a:b], block=recs[1][a:b], pixel=recs[2][a:b])
+            out.update(shade=recs[0], block=recs[1], pixel=recs[2])
         return out
 
     def trace_pooled(self, h0=0, h1=None, groups=128, lights=None, pool_seed=12345,
@@ -412,3 +419,14 @@ def ray_parallelogram(orig, d, A, B, C):
     r = lib().oc_ray_parallelogram(*[_p(np.ascontiguousarray(a, np.float32), _f32p)
                                      for a in (orig, d, A, B, C)], _p(tuv, _f32p))
     return bool(r), tuv
+
+
+def cylinder_pixel(orig, d, base, r2, h, res_x, res_y) -> np.ndarray:
+    """Cylinder receiver pixel (or -1) of each ray: orig, d are [n, 3]."""
+    o = np.ascontiguousarray(orig, np.float32).reshape(-1, 3)
+    dd = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(base, np.float32)
+    out = np.empty(o.shape[0], np.int32)
+    lib().oc_cylinder_pixel_batch(_p(o, _f32p), _p(dd, _f32p), int(o.shape[0]), _p(b, _f32p),
+                                  float(r2), float(h), int(res_x), int(res_y), _p(out, _i32p))
+    return out

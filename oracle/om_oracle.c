@@ -174,6 +174,12 @@ float oc_gauss_from_u(float u) { return oc_erfinvf(2.0f * u - 1.0f) * 1.41421356
 
 void oc_math_batch(int fn, const float *in, float *out, int n)
 {
+    if (fn == 10 || fn == 11) {
+        /* pairs: division and atan2 of (in[2i], in[2i+1]) */
+        for (int i = 0; i < n / 2; ++i)
+            out[i] = fn == 10 ? in[2 * i] / in[2 * i + 1] : oc_atan2f(in[2 * i], in[2 * i + 1]);
+        return;
+    }
     for (int i = 0; i < n; ++i) {
         float s, c, x = in[i];
         switch (fn) {
@@ -217,6 +223,9 @@ struct oc_scene {
     f3 rv[4], recv_normal, focus;
     float recv_pixel;
     int geo, res_x, res_y;
+    float srec;               /* pixel area */
+    f3 cyl_base;              /* cylinder: base of the axis, radius^2, height */
+    float cyl_r2, cyl_h;
     /* sun (scene_instance_process.cu:6-11, :163-226) */
     f3 sun;
     float buie_k, buie_gamma, buie_int465, buie_A, buie_B, buie_C, buie_inv_gp1;
@@ -354,13 +363,40 @@ static int build_grid(oc_scene *s)
     return 0;
 }
 
+/* Cylinder receiver (new; CylinderReceiver is an empty stub, receiver.cuh:103-122): axis
+ * through recv_pos at mid-height, radius size.x, height size.y; the map unrolls the outer
+ * surface (res_x columns over the azimuth, res_y rows over the height), by default from
+ * geo = int(1 / pixel length).  Heliostats aim at the axis point. */
+static int build_cylinder(oc_scene *s, int geo)
+{
+    float r = s->d.recv_size[0], h = s->d.recv_size[1];
+    if (!(r > 0.0f && h > 0.0f)) return -5;
+    int rx = s->d.recv_res_x, ry = s->d.recv_res_y;
+    if (rx <= 0 || ry <= 0) {
+        rx = (int)(6.28318531f * r * (float)geo);
+        ry = (int)(h * (float)geo);
+    }
+    if (rx <= 0 || ry <= 0) return -4;
+    s->res_x = rx;
+    s->res_y = ry;
+    s->cyl_base = mk3(s->d.recv_pos[0], s->d.recv_pos[1] - h / 2, s->d.recv_pos[2]);
+    s->cyl_r2 = r * r;
+    s->cyl_h = h;
+    s->recv_pixel = h / (float)ry;
+    s->srec = (6.28318531f * r / (float)rx) * (h / (float)ry);
+    s->focus = ld3(s->d.recv_pos);
+    return 0;
+}
+
 /* RectangleReceiver::CInit (receiver.cpp:24-117) */
 static int build_receiver(oc_scene *s)
 {
-    if (s->d.recv_type != 0) return -2;  /* cylinder: not in the oracle's FP32 path yet */
+    if (s->d.recv_type != 0 && s->d.recv_type != 1) return -2;
     int geo = (int)(1.0f / s->d.recv_pixel_length);
     s->geo = geo;
+    if (s->d.recv_type == 1) return build_cylinder(s, geo);
     s->recv_pixel = 1.0f / (float)geo;
+    s->srec = s->recv_pixel * s->recv_pixel;
     f3 size = ld3(s->d.recv_size), pos = ld3(s->d.recv_pos);
     f3 ln, v[4];
     switch (s->d.recv_face) {
@@ -579,7 +615,7 @@ static void flux_scale(oc_scene *s)
     float Ssub = s->p.helio_pixel_length * s->p.helio_pixel_length;
     float rou = s->p.reflectivity;
     int Nc = s->p.rays_per_sample;
-    float Srec = s->recv_pixel * s->recv_pixel;
+    float Srec = s->srec;
     total *= rou / Srec;
     s->total_energy = total;
     s->scale = Id * Ssub * rou / (float)Nc / Srec;
@@ -864,6 +900,8 @@ int oc_dda_probe(const oc_scene *s, const float orig[3], const float dir[3], int
  * is clamped to the last pixel (the reference writes past the image there). */
 static int receiver_pixel(const oc_scene *s, f3 o, f3 d)
 {
+    if (s->d.recv_type == 1)
+        return cylinder_pixel3(o, d, s->cyl_base, s->cyl_r2, s->cyl_h, s->res_x, s->res_y);
     float t, u, v;
     if (!ray_parallelogram3(o, d, s->rv[0], s->rv[1], s->rv[3], &t, &u, &v)) return -1;
     int row = trunc_i(u * (float)s->res_y);
@@ -901,7 +939,7 @@ int oc_trace_f32(const oc_scene *s, int h0, int h1,
             for (int m = 0; m < s->mcount[h]; ++m) {
                 f3 orig = add3(local2world3(micro_local(s, h, m), n), s->pos[h]);
                 for (int l = 0; l < L; ++l) {
-                    int64_t rid = s->ray_offset[h] + (int64_t)m * L + l;
+                    int64_t rid = s->ray_offset[h] - s->ray_offset[h0] + (int64_t)m * L + l;
                     st.rays++;
                     f3 to_sun, nrm;
                     sample_ray(s, h, m, l, &to_sun, &nrm);
@@ -975,7 +1013,33 @@ typedef struct {
     d3 *normal, *v0, *v1, *v3, *pos;
     double *cdf_x, *cdf_y, *slope;
     double int465, A, B, C, inv_gp1;
+    d3 cyl_base;
+    double cyl_r, cyl_h;
 } oc_scene64;
+
+/* the cylinder receiver in double: the entering root of the quadratic in t */
+static int cylinder_pixeld(d3 o, d3 d, const oc_scene64 *q, int res_x, int res_y)
+{
+    double ox = o.x - q->cyl_base.x, oz = o.z - q->cyl_base.z;
+    double a = d.x * d.x + d.z * d.z, b = 2.0 * (ox * d.x + oz * d.z);
+    double c = ox * ox + oz * oz - q->cyl_r * q->cyl_r;
+    double disc = b * b - 4.0 * a * c;
+    if (a < 1e-6 || disc <= 0.0) return -1;
+    double sq = sqrt(disc);
+    /* stable roots: q0 = -(b + sign(b) sq) / 2, t = q0 / a or c / q0 */
+    double q0 = -0.5 * (b + (b >= 0 ? sq : -sq));
+    double t1 = q0 / a, t2 = c / q0;
+    double t = t1 < t2 ? t1 : t2;
+    if (t < 1e-6) return -1;
+    double hx = ox + t * d.x, hz = oz + t * d.z, hy = o.y + t * d.y - q->cyl_base.y;
+    if (hy < 0.0 || hy > q->cyl_h) return -1;
+    double th = atan2(hz, hx);
+    if (th < 0.0) th += 6.283185307179586;
+    int col = (int)(th / 6.283185307179586 * res_x), row = (int)(hy / q->cyl_h * res_y);
+    if (col > res_x - 1) col = res_x - 1;
+    if (row > res_y - 1) row = res_y - 1;
+    return row * res_x + col;
+}
 
 static void build64(const oc_scene *s, oc_scene64 *q)
 {
@@ -1010,6 +1074,13 @@ static void build64(const oc_scene *s, oc_scene64 *q)
         q->rv[i] = addd(mkd(c * o.x + sn * o.z, o.y, -sn * o.x + c * o.z), pos);
     }
     d3 focus = scld(addd(q->rv[0], q->rv[2]), 0.5);
+    if (s->d.recv_type == 1) {
+        double r = s->d.recv_size[0], h = s->d.recv_size[1];
+        focus = pos;
+        q->cyl_base = mkd(pos.x, pos.y - h / 2, pos.z);
+        q->cyl_r = r;
+        q->cyl_h = h;
+    }
     q->normal = malloc(sizeof(d3) * (H + 1));
     q->v0 = malloc(sizeof(d3) * (H + 1));
     q->v1 = malloc(sizeof(d3) * (H + 1));
@@ -1147,7 +1218,7 @@ int oc_trace_f64(const oc_scene *s, int h0, int h1,
             for (int m = 0; m < s->mcount[h]; ++m) {
                 d3 orig = addd(local2worldd(micro_local_d(s, h, m), n), q.pos[h]);
                 for (int l = 0; l < L; ++l) {
-                    int64_t rid = s->ray_offset[h] + (int64_t)m * L + l;
+                    int64_t rid = s->ray_offset[h] - s->ray_offset[h0] + (int64_t)m * L + l;
                     st.rays++;
                     uint32_t c[4] = { (uint32_t)l, (uint32_t)m, (uint32_t)h,
                                       (s->p.scene_index << 4) | OC_PURPOSE_RAY }, o[4];
@@ -1192,7 +1263,9 @@ int oc_trace_f64(const oc_scene *s, int h0, int h1,
                     }
                     double t, u, v;
                     int px = -1;
-                    if (ray_parallelogramd(orig, refl, q.rv[0], q.rv[1], q.rv[3], &t, &u, &v)) {
+                    if (s->d.recv_type == 1) {
+                        px = cylinder_pixeld(orig, refl, &q, s->res_x, s->res_y);
+                    } else if (ray_parallelogramd(orig, refl, q.rv[0], q.rv[1], q.rv[3], &t, &u, &v)) {
                         int row = (int)(u * s->res_y), col = (int)(v * s->res_x);
                         if (row > s->res_y - 1) row = s->res_y - 1;
                         if (col > s->res_x - 1) col = s->res_x - 1;
@@ -1398,4 +1471,12 @@ int oc_ray_parallelogram(const float orig[3], const float dir[3], const float A[
     int r = ray_parallelogram3(ld3(orig), ld3(dir), ld3(A), ld3(B), ld3(C), &t, &u, &v);
     tuv[0] = t; tuv[1] = u; tuv[2] = v;
     return r;
+}
+
+int oc_cylinder_pixel_batch(const float *orig, const float *dir, int n, const float base[3], float r2,
+                            float h, int res_x, int res_y, int32_t *out)
+{
+    for (int i = 0; i < n; ++i)
+        out[i] = cylinder_pixel3(ld3(orig + 3 * i), ld3(dir + 3 * i), ld3(base), r2, h, res_x, res_y);
+    return 0;
 }

@@ -35,6 +35,7 @@ typedef struct {
     int   recv_type;           /* 0 = rectangle (receiver.cpp), 1 = cylinder (new)      */
     float recv_pos[3];
     float recv_size[3];        /* rectangle: box size; cylinder: (radius, height, -)    */
+                               /* cylinder recv_pos: axis point at mid-height           */
     float recv_normal[3];
     int   recv_face;           /* rectangle face 0..3 (receiver.cpp:30-48)              */
     float recv_pixel_length;   /* solarenergy::receiver_pixel_length (common_var.cpp:12)*/
@@ -43,6 +44,7 @@ typedef struct {
     float grid_interval[3];
     float sun_dir[3];          /* direction light travels (main.cpp:71-73)              */
     float dni;
+    int   recv_res_x, recv_res_y; /* cylinder columns (azimuth), rows; 0 = from pixel length */
 } oc_scene_desc;
 
 /* ---- tracing parameters: the solarenergy:: globals (common_var.cpp:9-27) plus the
@@ -81,6 +83,7 @@ float oc_erfinvf(float x);
 float oc_gauss_from_u(float u);
 /* batch evaluation for tests: fn 0 u01(bits), 1 sin2pi, 2 cos2pi, 3 sin_small, 4 cos_small,
    5 log, 6 exp, 7 erfinv, 8 gauss_from_u, 9 sqrt */
+/* fn 10 and 11 take pairs (in[2i], in[2i+1]): division and atan2, n/2 outputs */
 void  oc_math_batch(int fn, const float *in, float *out, int n);
 void  oc_philox_batch(const uint32_t *ctr /*[n][4]*/, const uint32_t key[2], uint32_t *out, int n);
 
@@ -115,8 +118,9 @@ typedef struct {
 
 /* FP32 reference-semantics trace of heliostats [h0, h1).
  * counts[res_y*res_x] (u32) and energy[res_y*res_x] (u64 fixed point, 2^-31 units) are
- * accumulated into (not cleared).  rec_* (may be NULL) are indexed by the global ray id
- * ray_offset[h] + m*Nc + l and hold -1 when empty: shade = blocker of the sun ray, block =
+ * accumulated into (not cleared).  rec_* (may be NULL) are indexed by the ray id
+ * ray_offset[h] - ray_offset[h0] + m*Nc + l and hold -1 when empty: shade = blocker of the
+ * sun ray, block =
  * blocker of the reflected ray (-2 when not evaluated), pixel = receiver pixel (-1 miss).
  * early_exit selects the occupied-band early exit (results are identical either way).
  * nthreads <= 0 uses OpenMP's default. */
@@ -162,6 +166,9 @@ void oc_rotate_y(const float o[3], const float old_dir[3], const float new_dir[3
 int  oc_ray_parallelogram(const float orig[3], const float dir[3], const float A[3],
                           const float B[3], const float C[3], float tuv[3]);
 void oc_sample_ray(const oc_scene *s, int h, int m, int l, float sun_to[3], float normal[3]);
+/* cylinder receiver pixel of n rays (new component; no reference counterpart) */
+int  oc_cylinder_pixel_batch(const float *orig, const float *dir, int n, const float base[3], float r2,
+                             float h, int res_x, int res_y, int32_t *out);
 
 #ifdef __cplusplus
 }

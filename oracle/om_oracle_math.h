@@ -145,4 +145,61 @@ static inline int trunc_i(float x)
 static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
+/* atan2(y, x) in (-pi, pi] (the project's float function, no reference counterpart):
+ * t = min/max of |x|, |y|; above tan(pi/8) atan(t) = pi/4 + atan((t - 1)/(t + 1)), so the
+ * alternating series to s^23 runs on |s| <= 0.4143; then the octant. */
+static inline float oc_atan2f(float y, float x)
+{
+    float ax = fabsf(x), ay = fabsf(y);
+    float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    float t = mn / mx;
+    int big = t > 0.414213562f;
+    float s = big ? (t - 1.0f) / (t + 1.0f) : t;
+    float z = s * s;
+    float p = -0.0434782609f;
+    p = fmaf(p, z, 0.0476190476f);
+    p = fmaf(p, z, -0.0526315789f);
+    p = fmaf(p, z, 0.0588235294f);
+    p = fmaf(p, z, -0.0666666667f);
+    p = fmaf(p, z, 0.0769230769f);
+    p = fmaf(p, z, -0.0909090909f);
+    p = fmaf(p, z, 0.111111111f);
+    p = fmaf(p, z, -0.142857143f);
+    p = fmaf(p, z, 0.2f);
+    p = fmaf(p, z, -0.333333333f);
+    float r = fmaf(p * z, s, s);
+    if (big) r = r + 0.785398163f;
+    if (ay > ax) r = 1.57079633f - r;
+    if (x < 0.0f) r = 3.14159265f - r;
+    return y < 0.0f ? -r : r;
+}
+
+/* Cylinder receiver pixel (new; CylinderReceiver::GIntersect is an empty stub,
+ * receiver.cuh:103-122): vertical axis through base, radius^2 r2, height h, outer surface
+ * only.  Entering root from the closest approach tc: t = tc - sqrt((r2 - d2) / a); column
+ * = azimuth atan2(z, x) in [0, 2 pi) x res_x, row = (y - base.y) / h x res_y. */
+static inline int cylinder_pixel3(f3 o, f3 d, f3 base, float r2, float h, int res_x, int res_y)
+{
+    float ox = o.x - base.x, oz = o.z - base.z;
+    float a = d.x * d.x + d.z * d.z;
+    if (!(a >= OC_EPS)) return -1;
+    float tc = -(ox * d.x + oz * d.z) / a;
+    float px = ox + tc * d.x, pz = oz + tc * d.z;
+    float d2 = px * px + pz * pz;
+    if (!(d2 < r2)) return -1;
+    float t = tc - sqrtf((r2 - d2) / a);
+    if (t < OC_EPS) return -1;
+    float hx = ox + t * d.x, hz = oz + t * d.z;
+    float hy = (o.y + t * d.y) - base.y;
+    if (hy < 0.0f || hy > h) return -1;
+    float th = oc_atan2f(hz, hx);
+    if (th < 0.0f) th = th + 6.28318531f;
+    int col = trunc_i(th * 0.159154943f * (float)res_x);
+    int row = trunc_i(hy / h * (float)res_y);
+    if (col > res_x - 1) col = res_x - 1;
+    if (row > res_y - 1) row = res_y - 1;
+    return row * res_x + col;
+}
+
 #endif

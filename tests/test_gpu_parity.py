@@ -267,3 +267,53 @@ def test_accumulation_modes_bitexact(om, mode, shift):
     assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32), orc["counts"])
     assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64), orc["energy"])
     assert int(orc["counts"].sum()) > 1_000_000
+
+
+def test_device_atan2_bitexact(om):
+    rng = np.random.default_rng(13)
+    y = rng.normal(size=200000).astype(np.float32)
+    x = rng.normal(size=200000).astype(np.float32)
+    pairs = np.empty(2 * y.size, np.float32)
+    pairs[0::2], pairs[1::2] = y, x
+    got = om.default_context().math_eval(11, pairs)[: y.size]
+    want = oracle.math("atan2", pairs)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("lattice", [False, True])
+def test_cylinder_small_records_and_maps_bitexact(om, lattice):
+    import torch
+    d, p = F.config_small_cylinder()
+    if lattice:
+        p = F.with_lattice(p)
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    orc = o.trace_f32(records=True)
+    _records_equal(g.trace_records(), orc, f"cylinder/{'lattice' if lattice else 'poisson'}")
+    assert (orc["pixel"] >= 0).mean() > 0.2
+    maps = g.trace_plain()
+    torch.cuda.synchronize()
+    assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32).ravel(), orc["counts"].ravel())
+    assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64).ravel(), orc["energy"].ravel())
+    scale, _ = o.flux_scale()
+    flux = (oracle.energy_to_float(orc["energy"]) * np.float32(scale)).astype(np.float32)
+    assert np.array_equal(maps.flux.cpu().numpy().view(np.uint32).ravel(), flux.view(np.uint32).ravel())
+
+
+@pytest.fixture(scope="module")
+def large(om):
+    d, p = F.config_large()
+    return d, p, om.SolarScene(d, p), oracle.OracleScene(d, p)
+
+
+def test_large_poisson_counts(large):
+    d, p, g, o = large
+    mc = g.export("micro_count", 4 * o.n_helio, np.int32)
+    assert np.array_equal(mc, o.micro()["count"])
+    assert g.info["total_rays"] == o.total_rays > 9.9e8
+
+
+@pytest.mark.parametrize("h0", [0, 12000, 24980])
+def test_large_records_bitexact(large, h0):
+    d, p, g, o = large
+    _records_equal(g.trace_records(h0, h0 + 20), o.trace_f32(h0, h0 + 20, records=True),
+                   f"large[{h0}:{h0 + 20}]")

@@ -103,6 +103,22 @@ def test_scene_mid_matches_oracle():
     _compare_scene(d, F.with_lattice(p))
 
 
+def test_scene_cylinder_matches_oracle():
+    d, p = F.config_small_cylinder()
+    _compare_scene(d, F.with_lattice(p))
+    _compare_scene(d, p)
+    g = HostScene(d, p)
+    assert (g.info["res_x"], g.info["res_y"]) == (128, 64)
+    # unrolled pixel area (2 pi r / 128) x (h / 64)
+    assert abs(g.info["flux_scale"] * (2 * np.pi * 2.0 / 128) * (4.0 / 64) /
+               (1000.0 * 0.125 ** 2 * 0.88 / 64) - 1.0) < 1e-5
+
+
+def test_scene_large_matches_oracle():
+    d, p = F.config_large()
+    _compare_scene(d, F.with_lattice(p))
+
+
 def test_scene_faceted_heliostats_match_oracle():
     d, p = F.config_small()
     d = dict(d)

@@ -122,6 +122,73 @@ def _exact_cells(o, d, gpos, ginter, gn):
         tnext[a] += ginter[a] / abs(d[a])
 
 
+def test_atan2_against_libm():
+    """The project's atan2 (cylinder azimuth) against double-precision atan2, over all
+    octants, the axes and tiny/huge ratios."""
+    rng = np.random.default_rng(7)
+    y = np.concatenate([rng.normal(size=200000), [0.0, 1.0, -1.0, 0.0, 1e-30, -1e-30, 3.0]])
+    x = np.concatenate([rng.normal(size=200000), [1.0, 0.0, 0.0, -1.0, -2.0, 5.0, 1e-30]])
+    y, x = y.astype(np.float32), x.astype(np.float32)
+    pairs = np.empty(2 * y.size, np.float32)
+    pairs[0::2], pairs[1::2] = y, x
+    got = oracle.math("atan2", pairs).astype(np.float64)
+    want = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    assert np.max(np.abs(got - want)) < 5e-7
+    assert got[-4] == np.float32(np.pi) and got[-6] == np.float32(np.pi / 2) and got[-7] == 0.0
+
+
+def _cylinder_pixel_f64(o, d, base, r, h, res_x, res_y):
+    """Entering root of the ray-cylinder quadratic in double, and the unrolled pixel; also
+    returns how close the hit is to a pixel edge (in pixels) or to the silhouette."""
+    o = o.astype(np.float64) - base.astype(np.float64)
+    d = d.astype(np.float64)
+    a = d[:, 0] ** 2 + d[:, 2] ** 2
+    b = 2 * (o[:, 0] * d[:, 0] + o[:, 2] * d[:, 2])
+    c = o[:, 0] ** 2 + o[:, 2] ** 2 - r * r
+    disc = b * b - 4 * a * c
+    ok = (a >= 1e-6) & (disc > 0)
+    t = np.where(ok, (-b - np.sqrt(np.maximum(disc, 0))) / (2 * a), -1.0)
+    ok &= t >= 1e-6
+    hp = o + t[:, None] * d
+    ok &= (hp[:, 1] >= 0) & (hp[:, 1] <= h)
+    th = np.mod(np.arctan2(hp[:, 2], hp[:, 0]), 2 * np.pi)
+    cu, rv = th / (2 * np.pi) * res_x, hp[:, 1] / h * res_y
+    px = np.where(ok, np.minimum(rv.astype(np.int64), res_y - 1) * res_x +
+                  np.minimum(cu.astype(np.int64), res_x - 1), -1)
+    edge = np.minimum(np.minimum(np.abs(cu - np.round(cu)), np.abs(rv - np.round(rv))),
+                      np.abs(disc) / np.maximum(b * b, 1e-30) * 1e3)
+    edge = np.minimum(edge, np.minimum(np.abs(hp[:, 1]), np.abs(hp[:, 1] - h)) / h * res_y)
+    return px, edge
+
+
+def test_cylinder_pixel_against_quadratic():
+    """Cylinder receiver pixels of the FP32 restatement agree with the FP64 quadratic,
+    except for hits within FP32 rounding of a pixel edge, the silhouette or a rim."""
+    rng = np.random.default_rng(8)
+    n = 200000
+    base, r, h, rx, ry = np.float32([3.0, 190.0, -2.0]), 8.0, 20.0, 512, 256
+    ang = rng.uniform(0, 2 * np.pi, n)
+    rad = rng.uniform(50, 1500, n)
+    o = np.stack([base[0] + rad * np.cos(ang), rng.uniform(0, 10, n), base[2] + rad * np.sin(ang)], 1)
+    aim = np.stack([base[0] + rng.uniform(-10, 10, n), base[1] + rng.uniform(-5, 25, n),
+                    base[2] + rng.uniform(-10, 10, n)], 1)
+    d = aim - o
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    o, d = o.astype(np.float32), d.astype(np.float32)
+    got = oracle.cylinder_pixel(o, d, base, np.float32(r) * np.float32(r), h, rx, ry)
+    want, edge = _cylinder_pixel_f64(o, d, base, r, h, rx, ry)
+    assert (want >= 0).mean() > 0.3 and (want < 0).mean() > 0.1
+    bad = got != want
+    assert bad.mean() < 1e-3
+    assert np.all(edge[bad] < 1e-2), edge[bad].max()
+    # entering side: the hit faces the ray's origin
+    hit = got >= 0
+    col = (got[hit] % rx + 0.5) / rx * 2 * np.pi
+    side = np.stack([np.cos(col), np.sin(col)], 1)
+    toward = np.stack([o[hit, 0] - base[0], o[hit, 2] - base[2]], 1)
+    assert np.all((side * toward).sum(1) > 0)
+
+
 def test_dda_matches_exact_cell_sequence():
     gpos = np.float32([-10.0, -3.0, 5.0])
     ginter = np.float32([3.0, 1.75, 2.5])
