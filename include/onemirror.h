@@ -198,6 +198,46 @@ om_status om_results_with_given_normal(om_scene *scene, om_result *result);
 /* Intercept ratio of a host flux map (ImageSaver::savetxt, image_save.cpp:26-47). */
 float om_intercept_ratio(const float *flux, int width, int height, float total_energy);
 
+/* ---- Batched mode: no reference counterpart (the reference traces one scene at a time
+ * from a host loop over heliostats, recthelio_tracing.cpp:46-79).  n scenes (fields, sun
+ * positions: annual time steps) are traced into n sets of maps by one persistent launch that
+ * takes (scene, heliostat, sample block) items from a work queue; the decode, flux scaling
+ * and smoothing of every map follow.  Each map equals what om_trace_plain gives for that
+ * scene alone.  Scenes of a batch share the context, the sunshape and the microsample kind. */
+typedef struct om_batch om_batch;
+
+typedef struct {
+    int workgroup;             /* threads per workgroup: 128, 256 or 512 (0 = 256)        */
+    int rays_per_thread;       /* 1..8, rays per work item = workgroup x this (0 = auto)  */
+} om_launch;
+
+/* Batch over scenes[0..n) with outputs outs[0..n) (counts and energy required, flux and
+ * flux_smooth optional; all overwritten by each run).  The scenes and outputs must outlive
+ * the batch; re-aiming a scene (om_scene_set_sun) needs a new batch. */
+om_status om_batch_create(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs,
+                          const om_launch *launch, om_batch **out);
+/* Run the batch on a stream: the first run captures the launch sequence in a hipGraph,
+ * every run replays it.  Calls on one batch must not overlap. */
+om_status om_batch_run(om_batch *batch, void *stream);
+om_status om_batch_info(const om_batch *batch, om_launch *launch, int64_t *items, int64_t *rays);
+om_status om_batch_destroy(om_batch *batch);
+/* One-shot batch without a graph; synchronous (waits for the stream). */
+om_status om_trace_batched(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs,
+                           const om_launch *launch, void *stream);
+
+/* Autotuner of the batch launch: times every workgroup size in {128, 256, 512} x rays per
+ * thread in {2, 4, 8} on the given batch (best of reps graph replays after a warm-up) and
+ * returns the fastest with the whole table.  Results do not depend on these parameters. */
+typedef struct {
+    om_launch best;
+    float     best_ms;
+    double    rays_per_s;
+    int       n_trials;
+    struct { int workgroup, rays_per_thread; float ms; } trials[9];
+} om_tune;
+om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs, int reps,
+                      om_tune *result);
+
 /* Device self-test of the float math as the trace kernel evaluates it (parity of the
  * kernels' arithmetic with the host).  fn: 0 u01 (input bits as uint32), 1/2 sin/cos(2 pi x),
  * 3/4 sin/cos of a small angle, 5 log, 6 exp, 7 erfinv, 8 Gaussian deviate of a uniform,

@@ -112,12 +112,27 @@ class Result(ctypes.Structure):
                 ("rays", ctypes.c_int64)]
 
 
+class Launch(ctypes.Structure):
+    _fields_ = [("workgroup", ctypes.c_int), ("rays_per_thread", ctypes.c_int)]
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("workgroup", ctypes.c_int), ("rays_per_thread", ctypes.c_int), ("ms", ctypes.c_float)]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("best", Launch), ("best_ms", ctypes.c_float), ("rays_per_s", ctypes.c_double),
+                ("n_trials", ctypes.c_int), ("trials", TuneTrial * 9)]
+
+
 _EXPORTS = [
     "om_status_string", "om_last_error", "om_default_params", "om_ctx_create", "om_ctx_destroy",
     "om_scene_from_scn", "om_scn_parse", "om_scene_desc_free", "om_scene_create",
     "om_scene_destroy", "om_scene_get_info", "om_scene_set_sun", "om_scene_export",
     "om_trace_plain", "om_trace_heliostats", "om_trace_records", "om_flux_finalize",
     "om_smooth_trimmed", "om_results_with_given_normal", "om_intercept_ratio", "om_math_eval",
+    "om_batch_create", "om_batch_run", "om_batch_info", "om_batch_destroy", "om_trace_batched",
+    "om_autotune",
 ]
 
 
@@ -157,6 +172,16 @@ def lib():
     L.om_intercept_ratio.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float]
     L.om_intercept_ratio.restype = ctypes.c_float
     L.om_math_eval.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]
+    L.om_batch_create.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(Flux),
+                                  ctypes.POINTER(Launch), ctypes.POINTER(_vp)]
+    L.om_batch_run.argtypes = [_vp, _vp]
+    L.om_batch_info.argtypes = [_vp, ctypes.POINTER(Launch), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int64)]
+    L.om_batch_destroy.argtypes = [_vp]
+    L.om_trace_batched.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(Flux),
+                                   ctypes.POINTER(Launch), _vp]
+    L.om_autotune.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(Flux),
+                              ctypes.c_int, ctypes.POINTER(Tune)]
     for name in _EXPORTS:
         if name not in ("om_status_string", "om_last_error", "om_default_params",
                         "om_scene_desc_free", "om_intercept_ratio"):
@@ -439,3 +464,96 @@ def intercept_ratio(flux: np.ndarray, total_energy: float) -> float:
     a = np.ascontiguousarray(flux, np.float32)
     h, w = a.shape
     return float(lib().om_intercept_ratio(a.ctypes.data_as(_f32p), w, h, float(total_energy)))
+
+
+# ---------------------------------------------------------------------------------------
+# batched mode (no reference counterpart: one persistent launch over many scenes)
+# ---------------------------------------------------------------------------------------
+def _batch_arrays(scenes, maps):
+    n = len(scenes)
+    if maps is None:
+        maps = [sc.new_maps(smooth=True) for sc in scenes]
+    hs = (_vp * n)(*[sc._h for sc in scenes])
+    fl = (Flux * n)(*[m.struct(with_flux=m.flux is not None) for m in maps])
+    return maps, hs, fl
+
+
+class Batch:
+    """Many scenes traced by one persistent launch, replayed from a hipGraph (om_batch_*).
+
+    ``maps[i]`` receives the maps of ``scenes[i]``, equal to what ``scenes[i].trace_plain``
+    gives; they are overwritten by every ``run``."""
+
+    def __init__(self, scenes, maps=None, workgroup: int = 0, rays_per_thread: int = 0):
+        self.scenes = list(scenes)
+        self.maps, self._hs, self._fl = _batch_arrays(self.scenes, maps)
+        lp = Launch(int(workgroup), int(rays_per_thread))
+        h = _vp()
+        ctx = self.scenes[0].ctx
+        _check(lib().om_batch_create(ctx._h, self._hs, len(self.scenes), self._fl, ctypes.byref(lp),
+                                     ctypes.byref(h)), "om_batch_create")
+        self._h = h
+        L, it, ry = Launch(), ctypes.c_int64(), ctypes.c_int64()
+        lib().om_batch_info(h, ctypes.byref(L), ctypes.byref(it), ctypes.byref(ry))
+        self.workgroup, self.rays_per_thread = L.workgroup, L.rays_per_thread
+        self.items, self.rays = it.value, ry.value
+
+    def run(self, stream=None):
+        _check(lib().om_batch_run(self._h, _stream_ptr(stream)), "om_batch_run")
+        return self.maps
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.om_batch_destroy(self._h)
+            self._h = None
+
+
+def trace_batched(scenes, maps=None, workgroup: int = 0, rays_per_thread: int = 0, stream=None):
+    """One-shot batch (no graph); returns the maps.  Synchronous."""
+    scenes = list(scenes)
+    maps, hs, fl = _batch_arrays(scenes, maps)
+    lp = Launch(int(workgroup), int(rays_per_thread))
+    _check(lib().om_trace_batched(scenes[0].ctx._h, hs, len(scenes), fl, ctypes.byref(lp),
+                                  _stream_ptr(stream)), "om_trace_batched")
+    return maps
+
+
+def autotune_launch(scenes, maps=None, reps: int = 2) -> dict:
+    """Time every (workgroup, rays per thread) candidate on the batch (om_autotune)."""
+    scenes = list(scenes)
+    maps, hs, fl = _batch_arrays(scenes, maps)
+    t = Tune()
+    _check(lib().om_autotune(scenes[0].ctx._h, hs, len(scenes), fl, int(reps), ctypes.byref(t)),
+           "om_autotune")
+    return {"workgroup": t.best.workgroup, "rays_per_thread": t.best.rays_per_thread,
+            "ms": t.best_ms, "rays_per_s": t.rays_per_s,
+            "trials": [(t.trials[i].workgroup, t.trials[i].rays_per_thread, t.trials[i].ms)
+                       for i in range(t.n_trials)]}
+
+
+def autotune(descs, prm: dict, grid_cells=(64, 128, 256), grid_top: Optional[float] = None,
+             sample: int = 4, reps: int = 2) -> dict:
+    """Autotuner of batched mode: grid resolution (cells per axis), workgroup size and rays
+    per work item.  The grid is part of a scene, so it is chosen first, on up to ``sample``
+    of the scenes rebuilt at each resolution; the launch parameters are timed with
+    ``om_autotune`` for each.  Returns the fastest and every trial in rays/s."""
+    from . import fields as F
+    descs = list(descs)
+    idx = np.linspace(0, len(descs) - 1, min(sample, len(descs))).round().astype(int)
+    rows = []
+    best = None
+    for n in grid_cells:
+        scenes = []
+        for i in sorted(set(idx.tolist())):
+            d = dict(descs[i])
+            top = grid_top if grid_top is not None else float(d["grid_pos"][1] + d["grid_size"][1])
+            d.update(F.grid_for(d["helio_pos"], d["helio_size"], top, int(n)))
+            scenes.append(SolarScene(d, prm))
+        r = autotune_launch(scenes, reps=reps)
+        rows.append({"grid_cells": int(n), **r})
+        if best is None or r["rays_per_s"] > best["rays_per_s"]:
+            best = rows[-1]
+        del scenes
+    return {"grid_cells": best["grid_cells"], "workgroup": best["workgroup"],
+            "rays_per_thread": best["rays_per_thread"], "rays_per_s": best["rays_per_s"],
+            "trials": rows}

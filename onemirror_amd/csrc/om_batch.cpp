@@ -1,0 +1,360 @@
+// om_batch.cpp -- batched mode: many scenes (fields, sun positions) traced by one persistent
+// launch from a (scene, heliostat, sample block) work queue, the decode / flux / smoothing of
+// every map behind it, the whole sequence captured once in a hipGraph; and the autotuner of
+// the launch parameters.
+//
+// No reference counterpart: the reference traces one scene at a time from a host loop over
+// heliostats (recthelio_tracing.cpp:46-79).  Each map of a batch equals the map om_trace_plain
+// produces for that scene alone (same Philox streams, integer accumulation).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "om_internal.h"
+
+namespace {
+
+constexpr size_t BATCH_PART_BYTES = 256u << 20; // budget of the packed map copies of a batch
+
+template <class T>
+om_status dalloc(T *&p, size_t n)
+{
+    p = nullptr;
+    if (n == 0) n = 1;
+    OM_HIP(hipMalloc((void **)&p, sizeof(T) * n));
+    return OM_OK;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+} // namespace
+
+struct om_batch {
+    om_ctx *ctx = nullptr;
+    int n = 0;
+    int sunshape = 0;
+    bool poisson = false;
+    int wgs = 256, rpt = 8;
+    int n_items = 0, nblocks = 0, max_npx = 0;
+    long long rays = 0;
+    std::vector<om_scene *> scenes;
+    std::vector<om_flux> outs;
+    std::vector<float *> raw;                   // per scene, for smoothing
+    om::TraceArgs *d_table = nullptr;
+    int4 *d_items = nullptr;
+    unsigned int *d_queue = nullptr;
+    om::ReduceSlot *d_slots = nullptr;
+    unsigned long long *d_parts = nullptr;      // copies + carry + wrap of every scene
+    float *d_raw = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void batch_free(om_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    dfree(b->d_table);
+    dfree(b->d_items);
+    dfree(b->d_queue);
+    dfree(b->d_slots);
+    dfree(b->d_parts);
+    dfree(b->d_raw);
+    delete b;
+}
+
+// The launch sequence of a batch: queue reset (a fill kernel, so that it can sit in a
+// graph), trace, decode + flux, smoothing of each map that asks for it.
+om_status batch_enqueue(om_batch *b, hipStream_t st)
+{
+    om_status rc;
+    if ((rc = om::launch_fill_u32(b->d_queue, 1, st)) != OM_OK) return rc;
+    if (b->n_items > 0 &&
+        (rc = om::launch_trace_batch(b->sunshape, b->poisson, b->wgs, b->d_table, b->d_items, b->n_items,
+                                     b->d_queue, b->nblocks, st)) != OM_OK)
+        return rc;
+    if ((rc = om::launch_reduce_batch(b->d_slots, b->n, b->max_npx, st)) != OM_OK) return rc;
+    for (int i = 0; i < b->n; ++i) {
+        const om_scene *s = b->scenes[i];
+        if (!b->outs[i].flux_smooth) continue;
+        if ((rc = om::launch_smooth(b->raw[i], b->outs[i].flux_smooth, s->res_x, s->res_y, s->prm.smooth_radius,
+                                    s->prm.smooth_ratio, s->scale, st)) != OM_OK)
+            return rc;
+    }
+    return OM_OK;
+}
+
+om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs, const om_launch *lp,
+                      om_batch **out)
+{
+    *out = nullptr;
+    if (!ctx || !scenes || !outs || n <= 0) {
+        om::set_error("batch needs a context, scenes and outputs");
+        return OM_ERR_INVALID;
+    }
+    om_batch *b = new om_batch();
+    b->ctx = ctx;
+    b->n = n;
+    b->wgs = (lp && lp->workgroup) ? lp->workgroup : 256;
+    b->rpt = (lp && lp->rays_per_thread) ? lp->rays_per_thread : 0;
+    if (b->wgs != 128 && b->wgs != 256 && b->wgs != 512) {
+        om::set_error("batch workgroup size must be 128, 256 or 512");
+        delete b;
+        return OM_ERR_INVALID;
+    }
+    if (b->rpt < 0 || b->rpt > om::RPT_LIMIT) {
+        om::set_error("rays per thread must be 1..8 (0 = automatic)");
+        delete b;
+        return OM_ERR_INVALID;
+    }
+    for (int i = 0; i < n; ++i) {
+        const om_scene *s = scenes[i];
+        const om_flux &f = outs[i];
+        if (!s || s->ctx != ctx) {
+            om::set_error("every scene of a batch must be built on the batch's context");
+            delete b;
+            return OM_ERR_INVALID;
+        }
+        if (!f.counts || !f.energy || f.capacity < (int64_t)s->res_x * s->res_y) {
+            om::set_error("batch outputs need counts and energy of at least the map size");
+            delete b;
+            return OM_ERR_SIZE;
+        }
+        if (i > 0 && (s->prm.sunshape != scenes[0]->prm.sunshape ||
+                      s->prm.micro_mode != scenes[0]->prm.micro_mode)) {
+            om::set_error("scenes of a batch must share the sunshape and microsample kind");
+            delete b;
+            return OM_ERR_UNSUPPORTED;
+        }
+        b->scenes.push_back(scenes[i]);
+        b->outs.push_back(f);
+        b->rays += s->total_rays;
+        b->max_npx = std::max(b->max_npx, s->res_x * s->res_y);
+    }
+    b->sunshape = scenes[0]->prm.sunshape;
+    b->poisson = scenes[0]->prm.micro_mode == OM_MICRO_POISSON;
+    if (b->rpt == 0) {
+        // enough items to fill 256 CUs many times over
+        b->rpt = om::RPT_LIMIT;
+        while (b->rpt > 1 && b->rays / ((long long)b->wgs * b->rpt) < 16384) b->rpt /= 2;
+    }
+    om_status rc = OM_OK;
+    auto fail = [&](om_status e) {
+        batch_free(b);
+        return e;
+    };
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(OM_ERR_HIP);
+    // work items: (scene, heliostat, first ray) blocks of wgs * rpt rays, scene by scene
+    std::vector<int4> items;
+    const int block = b->wgs * b->rpt;
+    for (int i = 0; i < n; ++i) {
+        const om_scene *s = scenes[i];
+        for (int h = 0; h < s->H; ++h) {
+            const long long nr = (long long)s->mcount[h] * s->prm.rays_per_sample;
+            for (long long r = 0; r < nr; r += block) items.push_back(make_int4(i, h, (int)r, 0));
+        }
+    }
+    if (items.size() >= (size_t)1 << 31) return fail((om::set_error("batch has too many work items"), OM_ERR_UNSUPPORTED));
+    b->n_items = (int)items.size();
+    // packed map copies: 8 << shift per scene within the budget
+    long long sum_px = 0;
+    for (int i = 0; i < n; ++i) sum_px += (long long)scenes[i]->res_x * scenes[i]->res_y;
+    int shift = 0;
+    while (shift < 3 && ((size_t)16 << shift) * sum_px * sizeof(unsigned long long) <= BATCH_PART_BYTES) ++shift;
+    const int ncopy = 8 << shift;
+    const size_t words = (size_t)sum_px * ncopy + (size_t)sum_px; // copies + (carry, wrap) as u32 pairs
+    if ((rc = dalloc(b->d_parts, words)) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_raw, (size_t)sum_px)) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_table, (size_t)n)) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_items, items.size())) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_queue, 1)) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_slots, (size_t)n)) != OM_OK) return fail(rc);
+    std::vector<om::TraceArgs> table(n);
+    std::vector<om::ReduceSlot> slots(n);
+    unsigned long long *copies = b->d_parts;
+    uint32_t *side = (uint32_t *)(b->d_parts + (size_t)sum_px * ncopy);
+    size_t px_off = 0;
+    for (int i = 0; i < n; ++i) {
+        om_scene *s = scenes[i];
+        const int npx = s->res_x * s->res_y;
+        om::TraceArgs a = om::make_trace_args(s);
+        a.rays_per_thread = b->rpt;
+        a.acc_packed = 1;
+        a.part = copies + px_off * ncopy;
+        a.carry = side + 2 * px_off;
+        a.wrap = a.carry + npx;
+        a.npx = npx;
+        a.copy_shift = shift;
+        a.copy_stride = npx;
+        a.px_stride = 1;
+        a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
+        table[i] = a;
+        om::ReduceSlot r;
+        r.part = a.part;
+        r.carry = a.carry;
+        r.wrap = a.wrap;
+        r.E = a.pack_shift;
+        r.npx = npx;
+        r.ncopy = ncopy;
+        r.copy_stride = npx;
+        r.px_stride = 1;
+        r.counts = outs[i].counts;
+        r.energy = (unsigned long long *)outs[i].energy;
+        r.raw = b->d_raw + px_off;
+        r.flux = outs[i].flux;
+        r.scale = s->scale;
+        slots[i] = r;
+        b->raw.push_back(r.raw);
+        px_off += (size_t)npx;
+    }
+    if (hipMemcpy(b->d_table, table.data(), sizeof(om::TraceArgs) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_slots, slots.data(), sizeof(om::ReduceSlot) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        (!items.empty() && hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(),
+                                     hipMemcpyHostToDevice) != hipSuccess))
+        return fail(om::hip_fail(hipGetLastError(), "hipMemcpy (batch tables)"));
+    // the decode leaves every word cleared; clear the new buffer once
+    if ((rc = om::launch_fill_u32((uint32_t *)b->d_parts, (long long)(words * 2), nullptr)) != OM_OK) return fail(rc);
+    if (hipDeviceSynchronize() != hipSuccess) return fail(om::hip_fail(hipGetLastError(), "batch set-up"));
+    // persistent workgroups: as many as are resident at once
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return fail(OM_ERR_HIP);
+    int per_cu = om::trace_batch_blocks_per_cu(b->sunshape, b->poisson, b->wgs);
+    if (per_cu < 1) per_cu = 1;
+    b->nblocks = std::max(1, std::min(b->n_items, per_cu * prop.multiProcessorCount));
+    *out = b;
+    return OM_OK;
+}
+
+} // namespace
+
+extern "C" om_status om_batch_create(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs,
+                                     const om_launch *launch, om_batch **out)
+{
+    if (!out) return OM_ERR_INVALID;
+    return batch_build(ctx, scenes, n, outs, launch, out);
+}
+
+extern "C" om_status om_batch_run(om_batch *b, void *stream)
+{
+    if (!b) return OM_ERR_INVALID;
+    OM_HIP(hipSetDevice(b->ctx->device));
+    if (!b->exec) {
+        // capture once on a stream of the batch (the null stream cannot be captured)
+        OM_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        OM_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        om_status rc = batch_enqueue(b, b->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+        if (rc != OM_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return om::hip_fail(e, "hipStreamEndCapture");
+        b->graph = g;
+        OM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    OM_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return OM_OK;
+}
+
+extern "C" om_status om_batch_info(const om_batch *b, om_launch *launch, int64_t *items, int64_t *rays)
+{
+    if (!b) return OM_ERR_INVALID;
+    if (launch) {
+        launch->workgroup = b->wgs;
+        launch->rays_per_thread = b->rpt;
+    }
+    if (items) *items = b->n_items;
+    if (rays) *rays = b->rays;
+    return OM_OK;
+}
+
+extern "C" om_status om_batch_destroy(om_batch *b)
+{
+    if (!b) return OM_ERR_INVALID;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    batch_free(b);
+    return OM_OK;
+}
+
+extern "C" om_status om_trace_batched(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs,
+                                      const om_launch *launch, void *stream)
+{
+    om_batch *b = nullptr;
+    om_status rc = batch_build(ctx, scenes, n, outs, launch, &b);
+    if (rc != OM_OK) return rc;
+    rc = batch_enqueue(b, (hipStream_t)stream);
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && rc == OM_OK)
+        rc = om::hip_fail(hipGetLastError(), "om_trace_batched");
+    batch_free(b);
+    return rc;
+}
+
+// Autotuner: every (workgroup size, rays per thread) candidate is timed on the batch itself
+// (one warm-up run, then the best of `reps` graph replays, HIP events on a private stream);
+// the fastest is returned with the whole table.  Maps are identical for every candidate,
+// so only the time differs.
+extern "C" om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs, int reps,
+                                 om_tune *result)
+{
+    if (!ctx || !result) return OM_ERR_INVALID;
+    std::memset(result, 0, sizeof(*result));
+    if (reps <= 0) reps = 2;
+    static const int wgs_list[] = {128, 256, 512};
+    static const int rpt_list[] = {2, 4, 8};
+    OM_HIP(hipSetDevice(ctx->device));
+    hipStream_t st;
+    OM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    OM_HIP(hipEventCreate(&e0));
+    OM_HIP(hipEventCreate(&e1));
+    om_status rc = OM_OK;
+    float best = 0.0f;
+    for (int w : wgs_list)
+        for (int r : rpt_list) {
+            om_launch lp{w, r};
+            om_batch *b = nullptr;
+            if ((rc = batch_build(ctx, scenes, n, outs, &lp, &b)) != OM_OK) break;
+            float ms = 0.0f;
+            rc = om_batch_run(b, st);
+            for (int k = 0; k < reps && rc == OM_OK; ++k) {
+                if (hipEventRecord(e0, st) != hipSuccess) rc = OM_ERR_HIP;
+                if (rc == OM_OK) rc = om_batch_run(b, st);
+                if (rc == OM_OK && hipEventRecord(e1, st) != hipSuccess) rc = OM_ERR_HIP;
+                if (rc == OM_OK && hipEventSynchronize(e1) != hipSuccess) rc = OM_ERR_HIP;
+                float t = 0.0f;
+                if (rc == OM_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = OM_ERR_HIP;
+                if (rc == OM_OK && (k == 0 || t < ms)) ms = t;
+            }
+            (void)hipStreamSynchronize(st);
+            const long long rays = b->rays;
+            batch_free(b);
+            if (rc != OM_OK) break;
+            int k = result->n_trials++;
+            result->trials[k].workgroup = w;
+            result->trials[k].rays_per_thread = r;
+            result->trials[k].ms = ms;
+            if (k == 0 || ms < best) {
+                best = ms;
+                result->best.workgroup = w;
+                result->best.rays_per_thread = r;
+                result->best_ms = ms;
+                result->rays_per_s = ms > 0.0f ? (double)rays / (ms * 1e-3) : 0.0;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return rc;
+}

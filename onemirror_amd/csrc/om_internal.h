@@ -87,6 +87,19 @@ struct TraceArgs {
     int pack_shift;
 };
 
+// One map of a batch for the decode (reduce_batch_kernel): packed copies and side maps,
+// outputs, and the flux scale.
+struct ReduceSlot {
+    unsigned long long *part;
+    uint32_t *carry, *wrap;
+    int E, npx, ncopy, copy_stride, px_stride;
+    uint32_t *counts;
+    unsigned long long *energy;
+    float *raw;   // float(energy * 2^-31), the smoothing input
+    float *flux;  // may be null
+    float scale;
+};
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -167,6 +180,13 @@ om_status launch_smooth(const float *in, float *out, int w, int h, int radius, f
                         float scale, hipStream_t st);
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st);
 om_status ensure_buf(DevBuf &b, size_t bytes);
+// batched mode (om_kernels.hip)
+om_status launch_trace_batch(int sunshape, bool poisson, int wgs, const TraceArgs *table, const int4 *items,
+                             int n_items, unsigned int *queue, int nblocks, hipStream_t st);
+int trace_batch_blocks_per_cu(int sunshape, bool poisson, int wgs);
+om_status launch_reduce_batch(const ReduceSlot *slots, int nslot, int max_npx, hipStream_t st);
+om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st);
+constexpr int RPT_LIMIT = 8; // rays per thread a trace workgroup holds hit slots for
 om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st);
 } // namespace om
 

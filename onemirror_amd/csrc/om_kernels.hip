@@ -19,7 +19,7 @@
 namespace om {
 
 constexpr int WG = 256;
-constexpr int RPT_MAX = 8; // rays per thread of one trace workgroup (hit slots in LDS)
+constexpr int RPT_MAX = RPT_LIMIT; // rays per thread of one trace workgroup (hit slots in LDS)
 constexpr int COPY_SHIFT_MAX = 3;             // up to 64 copies of the packed map
 constexpr size_t PART_BYTES = 64u << 20;      // memory budget of the copies
 
@@ -227,30 +227,15 @@ __device__ __forceinline__ uint32_t xcc_id()
     return x & 15u;
 }
 
-// Hits of a thread's rays are kept in LDS and added to the maps after the ray loop: vector
-// memory operations complete in issue order, so an atomic issued inside the loop would hold
-// back every later load of the wave until the memory side has carried it out.
-template <int SUN, bool POISSON, bool RECORD, bool BUIE_LDS>
-__global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
+// One tile: up to rays_per_thread rays per thread of heliostat tl.x, rays tl.y + k*WGS +
+// thread.  Hits of a thread's rays are kept in LDS and added to the maps after the ray loop:
+// vector memory operations complete in issue order, so an atomic issued inside the loop
+// would hold back every later load of the wave until the memory side has carried it out.
+// s_hpx / s_hq: this workgroup's [RPT_MAX][WGS] hit slots (each thread uses only its own).
+template <int SUN, bool POISSON, bool RECORD, int WGS>
+__device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, int *s_hpx, uint32_t *s_hq,
+                                           const float4 *s_bins, const int *s_guide)
 {
-    extern __shared__ float4 lds4[];
-    int *s_hpx = (int *)lds4;                          // [RPT_MAX][WG] pixel or -1
-    uint32_t *s_hq = (uint32_t *)(s_hpx + RPT_MAX * WG); // [RPT_MAX][WG] fixed-point energy
-    const float4 *s_bins = a.buie_bins;
-    const int *s_guide = a.buie_guide;
-    if (SUN == OM_SUN_BUIE && BUIE_LDS) {
-        float4 *lb = (float4 *)(s_hq + RPT_MAX * WG);
-        int *lg = (int *)(lb + a.nbins);
-        for (int i = threadIdx.x; i < a.nbins; i += WG) {
-            lb[i] = a.buie_bins[i];
-            lg[i] = a.buie_guide[i];
-        }
-        __syncthreads();
-        s_bins = lb;
-        s_guide = lg;
-    }
-    const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
-    const int2 tl = a.tiles[tile];
     const int h = tl.x;
     const HelioRec *hr = a.helio + h;
     frame hf;
@@ -267,13 +252,13 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
     const int L = a.L;
     const int nr = hr->mcount * L;
     const bool one_row = hr->rows == 1, one_col = hr->cols == 1;
-    // ray index r = m * L + l, stepped by WG without division
+    // ray index r = m * L + l, stepped by WGS without division
     int r = tl.y + (int)threadIdx.x;
     int m = r / L;
     int l = r - m * L;
-    const int dm = WG / L, dl = WG - (WG / L) * L;
+    const int dm = WGS / L, dl = WGS - (WGS / L) * L;
     int nk = 0; // rays of this thread, hit slots used
-    for (int k = 0; k < a.rays_per_thread; ++k, r += WG) {
+    for (int k = 0; k < a.rays_per_thread; ++k, r += WGS) {
         if (k > 0) {
             m += dm;
             l += dl;
@@ -284,7 +269,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         }
         if (r >= nr) break;
         nk = k + 1;
-        if (!RECORD) s_hpx[k * WG + threadIdx.x] = -1;
+        if (!RECORD) s_hpx[k * WGS + threadIdx.x] = -1;
         // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
         v3 loc;
         if (POISSON) {
@@ -347,23 +332,23 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         const int px = receiver_pixel(a, orig, refl);
         if (px < 0) continue;
         const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
-        s_hpx[k * WG + threadIdx.x] = px;
-        s_hq[k * WG + threadIdx.x] = (uint32_t)(e * 2147483648.0f);
+        s_hpx[k * WGS + threadIdx.x] = px;
+        s_hq[k * WGS + threadIdx.x] = (uint32_t)(e * 2147483648.0f);
     }
     if (RECORD) return;
     // Each thread adds its own hits after its last load.
     if (!a.acc_packed) {
         for (int k = 0; k < nk; ++k) {
-            const int px = s_hpx[k * WG + threadIdx.x];
+            const int px = s_hpx[k * WGS + threadIdx.x];
             if (px < 0) continue;
             atomicAdd(a.counts + px, 1u);
-            atomicAdd(a.energy + px, (unsigned long long)s_hq[k * WG + threadIdx.x]);
+            atomicAdd(a.energy + px, (unsigned long long)s_hq[k * WGS + threadIdx.x]);
         }
         return;
     }
-    // One 64-bit atomic per hit into this XCD's packed word: v = 2^E + q.  The old value
-    // tells whether the add carried out of the energy field (bits 0..E-1) or wrapped the
-    // word; both events are counted on the side so that the reduction decodes exactly.
+    // One 64-bit atomic per hit into a packed word: v = 2^E + q.  The old value tells
+    // whether the add carried out of the energy field (bits 0..E-1) or wrapped the word;
+    // both events are counted on the side so that the reduction decodes exactly.
     // copy: the XCD's own block of copies, one of them per workgroup (workgroup b runs on
     // XCD b % 8, so b / 8 counts the workgroups of one XCD)
     const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
@@ -375,9 +360,9 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
     for (int k = 0; k < RPT_MAX; ++k) {
         old[k] = 0;
         if (k < nk) {
-            const int px = s_hpx[k * WG + threadIdx.x];
+            const int px = s_hpx[k * WGS + threadIdx.x];
             if (px >= 0) {
-                const unsigned long long v = (1ull << E) + s_hq[k * WG + threadIdx.x];
+                const unsigned long long v = (1ull << E) + s_hq[k * WGS + threadIdx.x];
                 old[k] = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, v, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -386,15 +371,97 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
 #pragma unroll
     for (int k = 0; k < RPT_MAX; ++k) {
         if (k < nk) {
-            const int px = s_hpx[k * WG + threadIdx.x];
+            const int px = s_hpx[k * WGS + threadIdx.x];
             if (px >= 0) {
-                const unsigned long long q = s_hq[k * WG + threadIdx.x];
+                const unsigned long long q = s_hq[k * WGS + threadIdx.x];
                 const unsigned long long v = (1ull << E) + q;
                 if ((old[k] & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
                 if (old[k] + v < old[k]) atomicAdd(a.wrap + px, 1u);
             }
         }
     }
+}
+
+// One workgroup per tile of the scene's tile list (plain mode).
+template <int SUN, bool POISSON, bool RECORD, bool BUIE_LDS>
+__global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
+{
+    extern __shared__ float4 lds4[];
+    int *s_hpx = (int *)lds4;                          // [RPT_MAX][WG] pixel or -1
+    uint32_t *s_hq = (uint32_t *)(s_hpx + RPT_MAX * WG); // [RPT_MAX][WG] fixed-point energy
+    const float4 *s_bins = a.buie_bins;
+    const int *s_guide = a.buie_guide;
+    if (SUN == OM_SUN_BUIE && BUIE_LDS) {
+        float4 *lb = (float4 *)(s_hq + RPT_MAX * WG);
+        int *lg = (int *)(lb + a.nbins);
+        for (int i = threadIdx.x; i < a.nbins; i += WG) {
+            lb[i] = a.buie_bins[i];
+            lg[i] = a.buie_guide[i];
+        }
+        __syncthreads();
+        s_bins = lb;
+        s_guide = lg;
+    }
+    const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
+    trace_tile<SUN, POISSON, RECORD, WG>(a, a.tiles[tile], s_hpx, s_hq, s_bins, s_guide);
+}
+
+// Batched mode: persistent workgroups take (scene, heliostat, first ray) items from a
+// queue until it is empty.  Items of one scene are consecutive, so the workgroups running
+// at any time mostly share one scene's grid and heliostats in their L2.  Every workgroup
+// reads the counter until it passes n_items, so the grid drains.
+template <int SUN, bool POISSON, int WGS>
+__global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__restrict__ table,
+                                                          const int4 *__restrict__ items, int n_items,
+                                                          unsigned int *queue)
+{
+    extern __shared__ float4 lds4[];
+    int *s_hpx = (int *)lds4;
+    uint32_t *s_hq = (uint32_t *)(s_hpx + RPT_MAX * WGS);
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads(); // every thread has read s_item before thread 0 overwrites it
+        if (it >= n_items) break;
+        const int4 w = items[it];
+        const int sc = __builtin_amdgcn_readfirstlane(w.x);
+        const TraceArgs &a = table[sc];
+        trace_tile<SUN, POISSON, false, WGS>(a, make_int2(__builtin_amdgcn_readfirstlane(w.y),
+                                                          __builtin_amdgcn_readfirstlane(w.z)),
+                                             s_hpx, s_hq, a.buie_bins, a.buie_guide);
+    }
+}
+
+// Batched decode: blockIdx.y selects the map; writes (not adds) counts and energy, the raw
+// float map for smoothing and the scaled flux, with the operations of reduce_packed_kernel
+// and finalize_kernel.
+__global__ void reduce_batch_kernel(const ReduceSlot *__restrict__ slots)
+{
+    const ReduceSlot &s = slots[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.npx) return;
+    const int E = s.E;
+    const unsigned long long lowmask = (1ull << E) - 1ull;
+    unsigned long long n = 0, e = 0;
+    for (int x = 0; x < s.ncopy; ++x) {
+        unsigned long long &w = s.part[(size_t)x * s.copy_stride + (size_t)i * s.px_stride];
+        const unsigned long long wv = w;
+        n += wv >> E;
+        e += wv & lowmask;
+        w = 0ull;
+    }
+    const unsigned long long A = s.carry[i], W = s.wrap[i];
+    n = n + (W << (64 - E)) - A;
+    e = e + (A << E);
+    s.carry[i] = 0u;
+    s.wrap[i] = 0u;
+    s.counts[i] = (uint32_t)n;
+    s.energy[i] = e;
+    const float v = (float)((double)e * 4.656612873077392578125e-10);
+    s.raw[i] = v;
+    if (s.flux) s.flux[i] = v * s.scale;
 }
 
 // Decode the packed per-XCD words into the output maps (which it adds to) and clear them.
@@ -724,6 +791,81 @@ om_status launch_smooth(const float *in, float *out, int w, int h, int radius, f
     int n = w * h;
     hipLaunchKernelGGL(smooth_kernel, dim3((n + threads - 1) / threads), dim3(threads), lds, st, in, out,
                        w, h, radius, ratio, kcap, scale);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+template <int SUN, bool POISSON>
+static om_status launch_batch_wgs(int wgs, const TraceArgs *table, const int4 *items, int n_items,
+                                  unsigned int *queue, int nblocks, hipStream_t st)
+{
+    const size_t lds = (size_t)RPT_MAX * wgs * (sizeof(int) + sizeof(uint32_t));
+    switch (wgs) {
+    case 128:
+        hipLaunchKernelGGL((trace_batch_kernel<SUN, POISSON, 128>), dim3(nblocks), dim3(128), lds, st, table,
+                           items, n_items, queue);
+        break;
+    case 256:
+        hipLaunchKernelGGL((trace_batch_kernel<SUN, POISSON, 256>), dim3(nblocks), dim3(256), lds, st, table,
+                           items, n_items, queue);
+        break;
+    case 512:
+        hipLaunchKernelGGL((trace_batch_kernel<SUN, POISSON, 512>), dim3(nblocks), dim3(512), lds, st, table,
+                           items, n_items, queue);
+        break;
+    default:
+        set_error("batch workgroup size must be 128, 256 or 512");
+        return OM_ERR_INVALID;
+    }
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+om_status launch_trace_batch(int sunshape, bool poisson, int wgs, const TraceArgs *table, const int4 *items,
+                             int n_items, unsigned int *queue, int nblocks, hipStream_t st)
+{
+    if (sunshape == OM_SUN_BUIE)
+        return poisson ? launch_batch_wgs<OM_SUN_BUIE, true>(wgs, table, items, n_items, queue, nblocks, st)
+                       : launch_batch_wgs<OM_SUN_BUIE, false>(wgs, table, items, n_items, queue, nblocks, st);
+    return poisson ? launch_batch_wgs<OM_SUN_PILLBOX, true>(wgs, table, items, n_items, queue, nblocks, st)
+                   : launch_batch_wgs<OM_SUN_PILLBOX, false>(wgs, table, items, n_items, queue, nblocks, st);
+}
+
+template <int SUN, bool POISSON, int WGS>
+static int batch_occupancy()
+{
+    int n = 0;
+    const size_t lds = (size_t)RPT_MAX * WGS * (sizeof(int) + sizeof(uint32_t));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_batch_kernel<SUN, POISSON, WGS>, WGS, lds) !=
+        hipSuccess)
+        return 0;
+    return n;
+}
+
+template <int SUN, bool POISSON>
+static int batch_occupancy_wgs(int wgs)
+{
+    return wgs == 128 ? batch_occupancy<SUN, POISSON, 128>()
+                      : (wgs == 256 ? batch_occupancy<SUN, POISSON, 256>() : batch_occupancy<SUN, POISSON, 512>());
+}
+
+int trace_batch_blocks_per_cu(int sunshape, bool poisson, int wgs)
+{
+    if (sunshape == OM_SUN_BUIE)
+        return poisson ? batch_occupancy_wgs<OM_SUN_BUIE, true>(wgs) : batch_occupancy_wgs<OM_SUN_BUIE, false>(wgs);
+    return poisson ? batch_occupancy_wgs<OM_SUN_PILLBOX, true>(wgs) : batch_occupancy_wgs<OM_SUN_PILLBOX, false>(wgs);
+}
+
+om_status launch_reduce_batch(const ReduceSlot *slots, int nslot, int max_npx, hipStream_t st)
+{
+    hipLaunchKernelGGL(reduce_batch_kernel, dim3((max_npx + WG - 1) / WG, nslot), dim3(WG), 0, st, slots);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st, p, n);
     OM_HIP(hipGetLastError());
     return OM_OK;
 }

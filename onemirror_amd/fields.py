@@ -21,6 +21,7 @@ Workloads (heliostat counts, sizes, sunshape, microsamples, grid and map sizes):
            microsamples x 40 rays (about 1e9 rays), 256^3 grid, seeded sun position.
   small_cylinder  100 heliostats 2x2 m around a cylinder r = 2 m, h = 4 m at 30 m
            (128x64 map): the cylinder path at test size.
+  batched  64 scenes of 500-20,000 heliostats under seeded suns (config_batched).
   face2face  the single-heliostat scene of userData/face2face_shadow.scn.
 """
 from __future__ import annotations
@@ -120,8 +121,8 @@ def grid_for(pos: np.ndarray, size: np.ndarray, top: float, n: int) -> dict:
 
 
 def make_field(n: int, width: float, height: float, pivot: float, r_min: float, r_max: float,
-               sector_deg: float, jitter: float, seed: int, s_min: float) -> dict:
-    spacing = fit_spacing(n, r_min, r_max, sector_deg, s_min)
+               sector_deg: float, jitter: float, seed: int, s_min: float, fit: bool = True) -> dict:
+    spacing = fit_spacing(n, r_min, r_max, sector_deg, s_min) if fit else s_min
     xz = radial_stagger(n, r_min, r_max, spacing, sector_deg)[:n]
     if len(xz) < n:
         raise ValueError("field annulus too small for the heliostat count")
@@ -217,6 +218,41 @@ def config_small_cylinder(seed: int = 6) -> tuple:
                micro_mode=POISSON, helio_pixel_length=2.0 / 16.0, poisson_k=8,
                rays_per_sample=64, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
     return desc, prm
+
+
+def config_batched(seed: int = 3, n_scenes: int = 64, h_range=(500, 20000),
+                   rays_per_helio: float = 20000.0, ray_clip=(1e7, 2e8)) -> list:
+    """Batched workload: scenes with heliostat counts log-uniform in h_range, sun elevation
+    U[10, 80] deg and azimuth U[90, 270] deg, rays per scene proportional to the field size
+    and clipped to ray_clip (the microsample pitch follows), Buie CSR 0.1, slope 2.5 mrad,
+    40 rays per microsample, 128^3 grids, flat 25.6 m receivers (256x256 maps).  Returns a
+    list of (desc, params); scene i draws Philox stream i."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    out = []
+    s_min = 14.5
+    for i in range(n_scenes):
+        H = int(round(math.exp(rng.uniform(math.log(h_range[0]), math.log(h_range[1])))))
+        elev = float(rng.uniform(10.0, 80.0))
+        azim = float(rng.uniform(90.0, 270.0))
+        R = min(max(H * rays_per_helio, ray_clip[0]), ray_clip[1])
+        L = 40
+        pitch = 10.0 / math.sqrt(R / (H * L))
+        r_max = math.sqrt(100.0 ** 2 + H * s_min ** 2 * 0.866 / (math.pi / 2.0)) * 1.15 + 2 * s_min
+        f = make_field(H, 10.0, 10.0, 6.0, 100.0, r_max, 180.0, 0.1, seed * 1000 + i, s_min, fit=False)
+        tower = max(150.0, 0.2 * r_max)
+        desc = dict(f)
+        desc.update(recv_type=0, recv_pos=np.array([0.0, tower, 0.0], np.float32),
+                    recv_size=np.array([25.6, 25.6, 0.0], np.float32),
+                    recv_normal=np.array([0.0, 0.0, -1.0], np.float32), recv_face=0,
+                    recv_pixel_length=np.float32(0.1), sun_dir=sun_direction(elev, azim),
+                    dni=np.float32(1000.0))
+        desc.update(grid_for(f["helio_pos"], f["helio_size"], tower + 20.0, 128))
+        prm = dict(seed=0x00C0FFEE12345678, scene_index=i, sunshape=BUIE,
+                   pillbox_half_angle=4.65e-3, csr=0.1, buie_bins=2048, slope_sigma=2.5e-3,
+                   micro_mode=POISSON, helio_pixel_length=pitch, poisson_k=8,
+                   rays_per_sample=L, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
+        out.append((desc, prm))
+    return out
 
 
 def subset(desc: dict, idx) -> dict:

@@ -1,0 +1,109 @@
+"""Batched mode on the GPU: one persistent launch over many scenes, replayed from a hipGraph.
+
+Pin: every map of a batch equals the map the plain path gives for that scene alone (same
+Philox streams, integer accumulation), bit for bit, for every launch configuration and on
+every graph replay (replays are interleaved with other GPU work and synchronisations).
+"""
+import numpy as np
+import pytest
+
+from onemirror_amd import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def om():
+    import onemirror_amd
+    return onemirror_amd
+
+
+def _plain(scene):
+    import torch
+    m = scene.trace_plain()
+    torch.cuda.synchronize()
+    return {k: getattr(m, k).cpu().numpy().copy() for k in ("counts", "energy", "flux", "flux_smooth")}
+
+
+def _same(maps, want, label):
+    for k, v in want.items():
+        got = getattr(maps, k).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32) if got.dtype == np.float32 else got,
+                              v.view(np.uint32) if v.dtype == np.float32 else v), (label, k)
+
+
+@pytest.fixture(scope="module")
+def pillbox_batch(om):
+    """Annual steps of one field (three sun positions), the cylinder field, a second seed."""
+    scenes = []
+    d, p = F.config_small()
+    for i, (el, az) in enumerate(((45.0, 180.0), (25.0, 120.0), (70.0, 230.0))):
+        dd = dict(d)
+        dd["sun_dir"] = F.sun_direction(el, az)
+        scenes.append(om.SolarScene(dd, dict(p, scene_index=i)))
+    d, p = F.config_small_cylinder()
+    scenes.append(om.SolarScene(d, dict(p, scene_index=3)))
+    d, p = F.config_small(seed=9)
+    scenes.append(om.SolarScene(d, dict(p, scene_index=4)))
+    return scenes, [_plain(s) for s in scenes]
+
+
+@pytest.mark.parametrize("wg,rpt", [(0, 0), (128, 2), (512, 8), (256, 1)])
+def test_batch_graph_replays_equal_plain(om, pillbox_batch, wg, rpt):
+    import torch
+    scenes, want = pillbox_batch
+    b = om.Batch(scenes, workgroup=wg, rays_per_thread=rpt)
+    assert b.items > 0 and b.rays == sum(s.info["total_rays"] for s in scenes)
+    for rep in range(3):
+        maps = b.run()
+        torch.cuda.synchronize()
+        for i, m in enumerate(maps):
+            _same(m, want[i], f"scene {i} replay {rep}")
+        # unrelated GPU work between replays: another trace and a torch reduction
+        scenes[0].trace_plain()
+        torch.cuda.synchronize()
+        assert float(torch.ones(1 << 20, device="cuda").sum()) == float(1 << 20)
+
+
+def test_trace_batched_one_shot(om, pillbox_batch):
+    scenes, want = pillbox_batch
+    maps = om.trace_batched(scenes, rays_per_thread=4)
+    for i, m in enumerate(maps):
+        _same(m, want[i], f"scene {i}")
+
+
+def test_batch_buie_fields_equal_plain(om):
+    import torch
+    d, p = F.config_mid()
+    scenes = [om.SolarScene(F.subset(d, slice(0, 300)), dict(p, scene_index=0))]
+    dd = F.subset(d, slice(2000, 2400))
+    dd["sun_dir"] = F.sun_direction(20.0, 100.0)
+    scenes.append(om.SolarScene(dd, dict(p, scene_index=1)))
+    bd, bp = F.config_batched(n_scenes=2, h_range=(500, 800))[0]
+    scenes.append(om.SolarScene(bd, bp))
+    want = [_plain(s) for s in scenes]
+    b = om.Batch(scenes)
+    for rep in range(2):
+        maps = b.run()
+        torch.cuda.synchronize()
+        for i, m in enumerate(maps):
+            _same(m, want[i], f"buie scene {i} replay {rep}")
+
+
+def test_autotune_launch_table(om, pillbox_batch):
+    scenes, _ = pillbox_batch
+    r = om.autotune_launch(scenes, reps=1)
+    assert len(r["trials"]) == 9
+    assert all(ms > 0 for _, _, ms in r["trials"])
+    assert (r["workgroup"], r["rays_per_thread"], r["ms"]) in r["trials"]
+    assert r["rays_per_s"] > 0
+
+
+def test_batch_rejects_mixed_sunshapes(om):
+    d, p = F.config_small()
+    a = om.SolarScene(d, p)
+    b = om.SolarScene(d, dict(p, sunshape=F.BUIE))
+    with pytest.raises(om.OMError, match="sunshape"):
+        om.Batch([a, b])
+    with pytest.raises(om.OMError, match="workgroup"):
+        om.Batch([a], workgroup=96)
