@@ -49,7 +49,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large"])
+    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large", "batched"])
+    ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
+    ap.add_argument("--no-grid-tune", action="store_true",
+                    help="batched: keep 128^3 grids instead of tuning the resolution")
     ap.add_argument("--cpu-every", type=int, default=0,
                     help="CPU baseline traces one block of 40 heliostats in every n "
                          "(default: the whole field, one block in 5 for large)")
@@ -62,6 +65,11 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(every GPU box here has one GPU)", file=sys.stderr)
         return 2
+
+    if args.workload == "batched":
+        out = bench_batched(args)
+        print(json.dumps(out), flush=True)
+        return 0
 
     import numpy as np
     import torch
@@ -193,6 +201,125 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def oracle_sample(configs, blocks_per_scene: int, blk: int = 40):
+    """Oracle FP32 trace of the first ``blk`` heliostats of each scene (one block per scene):
+    step/test counts for the cost model and the CPU baseline."""
+    import oracle
+    nthreads = host_threads()
+    tot = {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0}
+    secs = 0.0
+    nhel = 0
+    for d, p in configs:
+        o = oracle.OracleScene(d, p)
+        H = o.n_helio
+        for b in range(blocks_per_scene):
+            h0 = (b * H) // max(1, blocks_per_scene)
+            h1 = min(H, h0 + blk)
+            t = time.perf_counter()
+            r = o.trace_f32(h0, h1, nthreads=nthreads)
+            secs += time.perf_counter() - t
+            nhel += h1 - h0
+            for k in tot:
+                tot[k] += r["stats"][k]
+    return tot, secs, nhel, nthreads
+
+
+def bench_batched(args) -> dict:
+    """Batched mode: the scenes of config_batched in one persistent launch replayed from a
+    hipGraph, the grid resolution and launch parameters from the autotuner; the plain path
+    (one om_trace_plain per scene, back to back on the stream) timed on the same scenes."""
+    import torch
+
+    import onemirror_amd as om
+    from onemirror_amd import fields as F
+
+    cfg = F.config_batched(n_scenes=args.scenes)
+    t0 = time.time()
+    grid_cells = 128
+    tune_grid = None
+    if not args.no_grid_tune:
+        tune_grid = om.autotune(cfg, grid_cells=(64, 128, 256), sample=4, reps=1)
+        grid_cells = tune_grid["grid_cells"]
+        print(f"# grid autotune ({time.time() - t0:.1f}s): "
+              + ", ".join(f"{r['grid_cells']}^3 {r['rays_per_s']:.3g} rays/s" for r in tune_grid["trials"]),
+              file=sys.stderr, flush=True)
+    t1 = time.time()
+    configs = []
+    for d, p in cfg:
+        d = dict(d)
+        d.update(F.grid_for(d["helio_pos"], d["helio_size"], float(d["grid_pos"][1] + d["grid_size"][1]),
+                            grid_cells))
+        configs.append((d, p))
+    scenes = [om.SolarScene(d, p) for d, p in configs]
+    torch.cuda.synchronize()
+    R = sum(s.info["total_rays"] for s in scenes)
+    Hs = [s.info["n_helio"] for s in scenes]
+    print(f"# {len(scenes)} scenes built in {time.time() - t1:.1f}s: {sum(Hs)} heliostats "
+          f"({min(Hs)}..{max(Hs)}), {R} rays, grids {grid_cells}^3", file=sys.stderr, flush=True)
+    maps = [s.new_maps(smooth=True) for s in scenes]
+    tl = om.autotune_launch(scenes, maps, reps=2)
+    print(f"# launch autotune: workgroup {tl['workgroup']} x {tl['rays_per_thread']} rays/thread",
+          file=sys.stderr, flush=True)
+    b = om.Batch(scenes, maps, workgroup=tl["workgroup"], rays_per_thread=tl["rays_per_thread"])
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, args.warmup)):
+        b.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        b.run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    batch_ms = e0.elapsed_time(e1) / args.steps
+    # the plain path on the same scenes: one trace + finalize + smoothing per scene
+    for s, m in zip(scenes, maps):
+        s.trace_plain(m)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        for s, m in zip(scenes, maps):
+            s.trace_plain(m)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plain_ms = e0.elapsed_time(e1) / args.steps
+    print(f"# batched {batch_ms:.2f} ms ({R / (batch_ms * 1e-3):.4g} rays/s), plain per scene "
+          f"{plain_ms:.2f} ms ({R / (plain_ms * 1e-3):.4g} rays/s)", file=sys.stderr, flush=True)
+    sample_cfg = configs[:: max(1, len(configs) // 8)]
+    st, cpu_t, nhel, nthreads = oracle_sample(sample_cfg, 1)
+    S = st["dda_steps_eff"] / max(1, st["rays"])
+    T = st["cand_tests"] / max(1, st["rays"])
+    K = st["self_skips"] / max(1, st["rays"])
+    ops_ray = F0 + C_STEP * S + C_TEST * T + C_SKIP * K
+    achieved = R * ops_ray / (batch_ms * 1e-3) / 1e12
+    peak = P_VALU / 1e12
+    return {
+        "metric": METRIC, "value": R / dt, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched: {len(scenes)} scenes, heliostats log-uniform 500-20000, "
+                               "sun elevation 10-80 deg azimuth 90-270 deg, rays per scene "
+                               "proportional to heliostats in [1e7, 2e8], Buie CSR 0.1, 256x256 maps",
+                   "scenes": len(scenes), "heliostats": sum(Hs), "rays": R,
+                   "grid_cells": grid_cells, "workgroup": tl["workgroup"],
+                   "rays_per_thread": tl["rays_per_thread"], "batch_ms": batch_ms,
+                   "plain_ms": plain_ms, "plain_rays_per_s": R / (plain_ms * 1e-3),
+                   "batched_vs_plain": plain_ms / batch_ms,
+                   "grid_tuning": tune_grid["trials"] if tune_grid else None,
+                   "launch_tuning": tl["trials"], "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
+                     "frac": achieved / peak, "traffic": None, "ops_per_ray": ops_ray,
+                     "dda_steps_per_ray": S, "tests_per_ray": T},
+        "cpu_baseline": {"value": st["rays"] / cpu_t, "unit": "rays/s", "cores": nthreads,
+                         "kind": "port",
+                         "sample": f"the first 40 heliostats of {len(sample_cfg)} of the scenes "
+                                   f"({nhel} heliostats, {st['rays']} rays); oracle FP32 tracer, OpenMP",
+                         "seconds": cpu_t},
+    }
 
 
 if __name__ == "__main__":

@@ -531,21 +531,22 @@ def autotune_launch(scenes, maps=None, reps: int = 2) -> dict:
                        for i in range(t.n_trials)]}
 
 
-def autotune(descs, prm: dict, grid_cells=(64, 128, 256), grid_top: Optional[float] = None,
+def autotune(configs, grid_cells=(64, 128, 256), grid_top: Optional[float] = None,
              sample: int = 4, reps: int = 2) -> dict:
     """Autotuner of batched mode: grid resolution (cells per axis), workgroup size and rays
-    per work item.  The grid is part of a scene, so it is chosen first, on up to ``sample``
-    of the scenes rebuilt at each resolution; the launch parameters are timed with
-    ``om_autotune`` for each.  Returns the fastest and every trial in rays/s."""
+    per work item.  ``configs`` is a list of (desc, params).  The grid is part of a scene, so
+    it is chosen first, on up to ``sample`` of the scenes rebuilt at each resolution; the
+    launch parameters are timed with ``om_autotune`` for each.  Returns the fastest and every
+    trial in rays/s."""
     from . import fields as F
-    descs = list(descs)
-    idx = np.linspace(0, len(descs) - 1, min(sample, len(descs))).round().astype(int)
+    configs = list(configs)
+    idx = np.linspace(0, len(configs) - 1, min(sample, len(configs))).round().astype(int)
     rows = []
     best = None
     for n in grid_cells:
         scenes = []
         for i in sorted(set(idx.tolist())):
-            d = dict(descs[i])
+            d, prm = dict(configs[i][0]), configs[i][1]
             top = grid_top if grid_top is not None else float(d["grid_pos"][1] + d["grid_size"][1])
             d.update(F.grid_for(d["helio_pos"], d["helio_size"], top, int(n)))
             scenes.append(SolarScene(d, prm))
