@@ -238,6 +238,31 @@ typedef struct {
 om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs, int reps,
                       om_tune *result);
 
+/* ---- Differentiable mode: no reference counterpart (the reference traces fixed aims).
+ * Parameters per heliostat, float[H][4]: aim-point offset (m) along the receiver's two unit
+ * edges e1 = (v1-v0)/|v1-v0| and e2 = (v3-v0)/|v3-v0|, and tracking deltas (mrad) that tilt
+ * the aimed normal toward the heliostat frame axes fu, fv:
+ *   n = normalize(normalize(normalize(focus + a_u e1 + a_v e2 - pos) - sun)
+ *                 + 1e-3 (t_u fu + t_v fv)).
+ * Per ray the slope-error deviation of the plain path is added to n and renormalised; the
+ * reflected ray's receiver-plane point, in pixel coordinates, is splatted with energy
+ * |sun.n'| by a Gaussian of sigma = 1 px tapered to zero value and slope at 3 px.  Rectangle
+ * receivers only. */
+/* Words (uint64) of the visibility ballots a forward pass writes. */
+om_status om_diff_mask_words(const om_scene *scene, int64_t *words);
+/* Forward: d_splat double[res_y*res_x] and d_energy double[1] (energy of the visible rays)
+ * at d_params; d_mask receives one ballot bit per ray (visible = not shaded, not blocked,
+ * receiver plane hit ahead), which the backward pass holds fixed. */
+om_status om_diff_forward(om_scene *scene, const float *d_params, uint64_t *d_mask, double *d_splat,
+                          double *d_energy, void *stream);
+/* Backward (the adjoint kernel): d_grad float[H][4] = dL/dparams from d_dsplat = dL/dS and
+ * d_denergy = dL/dE (device), recomputing the visible rays from their Philox counters. */
+om_status om_diff_backward(om_scene *scene, const float *d_params, const uint64_t *d_mask,
+                           const double *d_dsplat, const double *d_denergy, float *d_grad, void *stream);
+/* Visibility of the rays of heliostats [h0, h1) from the ballots, one byte per ray in ray order. */
+om_status om_diff_visibility(om_scene *scene, const uint64_t *d_mask, int h0, int h1, uint8_t *d_vis,
+                             int64_t capacity, void *stream);
+
 /* Device self-test of the float math as the trace kernel evaluates it (parity of the
  * kernels' arithmetic with the host).  fn: 0 u01 (input bits as uint32), 1/2 sin/cos(2 pi x),
  * 3/4 sin/cos of a small angle, 5 log, 6 exp, 7 erfinv, 8 Gaussian deviate of a uniform,

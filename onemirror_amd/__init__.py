@@ -132,7 +132,7 @@ _EXPORTS = [
     "om_trace_plain", "om_trace_heliostats", "om_trace_records", "om_flux_finalize",
     "om_smooth_trimmed", "om_results_with_given_normal", "om_intercept_ratio", "om_math_eval",
     "om_batch_create", "om_batch_run", "om_batch_info", "om_batch_destroy", "om_trace_batched",
-    "om_autotune",
+    "om_autotune", "om_diff_mask_words", "om_diff_forward", "om_diff_backward", "om_diff_visibility",
 ]
 
 
@@ -182,6 +182,10 @@ def lib():
                                    ctypes.POINTER(Launch), _vp]
     L.om_autotune.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(Flux),
                               ctypes.c_int, ctypes.POINTER(Tune)]
+    L.om_diff_mask_words.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
+    L.om_diff_forward.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.om_diff_backward.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.om_diff_visibility.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp]
     for name in _EXPORTS:
         if name not in ("om_status_string", "om_last_error", "om_default_params",
                         "om_scene_desc_free", "om_intercept_ratio"):

@@ -1,0 +1,578 @@
+// om_diff.hip -- differentiable mode: a Gaussian-kernel flux splat of the field's rays as a
+// function of per-heliostat aim-point offsets and tracking-angle deltas, and its hand-written
+// adjoint.  No reference counterpart (the reference only traces fixed aims).
+//
+// Model, per heliostat h with parameters theta = (a_u, a_v, t_u, t_v):
+//   aim   = focus + a_u e1 + a_v e2         (e1, e2: unit receiver edges; metres)
+//   n_a   = normalize(normalize(aim - pos) - sun)          (the aiming of heliostat.cpp:51-57)
+//   n     = normalize(n_a + 1e-3 (t_u fu + t_v fv))        (fu, fv: frame of the nominal
+//                                                            normal; t in milliradians)
+// per ray (the plain path's Philox draws, microsample centre and sun ray):
+//   nrm   = normalize(n + (nrm0 - n0))      (nrm0 = slope-error normal about the nominal n0:
+//                                            the deviation is held fixed in world space)
+//   r     = normalize(reflect(-to_sun, nrm)),  e = |sun . nrm|
+//   p     = o + t r on the receiver plane, (Y, X) = (u res_y - 1/2, v res_x - 1/2) pixels
+//   S[y][x] += e K(x - X) K(y - Y),  K(d) = (exp(-d^2/2) - eR (1 + (R^2 - d^2)/2)) / Z on
+//             |d| < R = 3 (a Gaussian of sigma = 1 px tapered to zero value and slope at R)
+//   E      = sum of e over the visible rays
+// Visibility (not shaded, not blocked at theta, plane hit in front) is decided in the
+// forward pass, stored as one ballot bit per ray and held fixed in the backward pass, which
+// recomputes every ray from its Philox counter instead of storing rays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "om_device.h"
+
+namespace om {
+
+struct DiffArgs {
+    const float4 *theta;        // [H] (a_u, a_v, t_u, t_v)
+    float4 *n_theta;            // [H] parametrised normals
+    unsigned long long *mask;   // [tiles][rays per thread][WG/64] visibility ballots
+    unsigned long long *S;      // [8][npx] splat, fixed point 2^-32
+    unsigned long long *E;      // [8] energy, fixed point 2^-32
+    const double *g;            // [npx] dL/dS
+    const double *gE;           // dL/dE
+    double *tile_grad;          // [tiles][4] per-tile sums of dL/dn (3 used)
+    const int *tile_first;      // [H+1] first tile of each heliostat
+    v3 A, E1, E2, N;            // receiver plane: corner, edges, normal
+    float iE1, iE2;             // 1 / |E1|^2, 1 / |E2|^2
+    v3 focus, e1, e2, sun;
+    float kZ, kR, kEr;          // splat kernel 1/Z, radius, exp(-R^2/2)
+    int res_x, res_y, npx, H;
+};
+
+constexpr int DIFF_COPIES = 8;
+constexpr float DIFF_R = 3.0f;
+
+__device__ __forceinline__ float kern(float d, const DiffArgs &p)
+{
+    return (exp_(-0.5f * d * d) - p.kEr * (1.0f + 0.5f * (p.kR * p.kR - d * d))) * p.kZ;
+}
+__device__ __forceinline__ float dkern(float d, const DiffArgs &p)
+{
+    return -d * (exp_(-0.5f * d * d) - p.kEr) * p.kZ;
+}
+
+// n(theta) of every heliostat
+__global__ void diff_normals_kernel(const HelioRec *helio, DiffArgs p)
+{
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= p.H) return;
+    const HelioRec &hr = helio[h];
+    const float4 th = p.theta[h];
+    const v3 pos = mk(hr.pos[0], hr.pos[1], hr.pos[2]);
+    const v3 fu = mk(hr.fu[0], hr.fu[1], hr.fu[2]), fv = mk(hr.fv[0], hr.fv[1], hr.fv[2]);
+    const v3 aim = p.focus + p.e1 * th.x + p.e2 * th.y;
+    const v3 na = normalize(normalize(aim - pos) - p.sun);
+    const v3 n = normalize(na + (fu * th.z + fv * th.w) * 1e-3f);
+    p.n_theta[h] = make_float4(n.x, n.y, n.z, 0.0f);
+}
+
+// One ray of the model up to the splat position; false when the plane is not hit in front.
+struct DiffRay {
+    v3 i, nrm, r, w, q;
+    float t, den, X, Y, e;
+};
+__device__ __forceinline__ bool diff_ray(const DiffArgs &p, v3 orig, v3 to_sun, v3 nrm0, v3 n0, v3 nth, DiffRay &o)
+{
+    o.q = nth + (nrm0 - n0);
+    o.nrm = normalize<gfx>(o.q);
+    o.i = -to_sun;
+    o.w = reflect(o.i, o.nrm);
+    o.r = normalize<gfx>(o.w);
+    o.den = dot(o.r, p.N);
+    if (!(fabs_(o.den) >= EPS_LT)) return false;
+    o.t = dot(p.A - orig, p.N) / o.den;
+    if (!(o.t >= EPS_LT)) return false;
+    const v3 pp = orig + o.r * o.t;
+    const float u = dot(pp - p.A, p.E1) * p.iE1, v = dot(pp - p.A, p.E2) * p.iE2;
+    o.Y = u * (float)p.res_y - 0.5f;
+    o.X = v * (float)p.res_x - 0.5f;
+    o.e = fabs_(dot(p.sun, o.nrm));
+    return true;
+}
+
+// Per-tile ray set-up shared by the forward and backward kernels (the plain path's sampling).
+struct DiffTile {
+    const HelioRec *hr;
+    frame hf, sf;
+    v3 hpos, nth;
+    int h, L, nr;
+    bool one_row, one_col;
+};
+
+template <int SUN, bool POISSON>
+__device__ __forceinline__ void diff_sample(const TraceArgs &a, const DiffTile &T, int m, int l, v3 &orig,
+                                            v3 &to_sun, u4 &x)
+{
+    const HelioRec *hr = T.hr;
+    v3 loc;
+    if (POISSON) {
+        float2 pt = a.poisson[(size_t)T.h * a.poisson_cap + m];
+        int br = T.one_row ? 0 : trunc_i(pt.y / hr->subrow_len);
+        int bc = T.one_col ? 0 : trunc_i(pt.x / hr->subcol_len);
+        loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
+    } else {
+        int row = m / hr->cols_all, col = m - row * hr->cols_all;
+        int br = row / hr->sub_rows, bc = col / hr->sub_cols;
+        loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
+                 (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
+    }
+    orig = to_world(loc, T.hf) + T.hpos;
+    x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)T.h, a.w3}, a.key0, a.key1);
+    float th;
+    if (SUN == OM_SUN_PILLBOX) {
+        th = a.pillbox * sqrt_rn(u01(x.x));
+    } else {
+        float v = u01(x.x) * a.buie_A;
+        if (v < a.buie_int465) {
+            float4 b = buie_bin(a.buie_bins, a.buie_guide, a.buie_gscale, v, a.nbins);
+            th = (v - b.x) * b.z + b.w;
+        } else {
+            th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
+        }
+        th = div_rn(th, 1000.0f);
+    }
+    float st, ct, sp, cp;
+    sincos_small(th, st, ct);
+    sincos2pi(u01(x.y), sp, cp);
+    to_sun = -to_world(angle2xyz(st, ct, sp, cp), T.sf);
+}
+
+__device__ __forceinline__ DiffTile diff_tile(const TraceArgs &a, const DiffArgs &p, int h)
+{
+    DiffTile T;
+    T.h = h;
+    T.hr = a.helio + h;
+    const HelioRec *hr = T.hr;
+    T.hf.n = mk(hr->n[0], hr->n[1], hr->n[2]);
+    T.hf.u = mk(hr->fu[0], hr->fu[1], hr->fu[2]);
+    T.hf.v = mk(hr->fv[0], hr->fv[1], hr->fv[2]);
+    T.hf.identity = hr->ident;
+    T.sf.n = a.sun_n;
+    T.sf.u = a.sun_u;
+    T.sf.v = a.sun_v;
+    T.sf.identity = a.sun_ident;
+    T.hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
+    const float4 n4 = p.n_theta[h];
+    T.nth = mk(n4.x, n4.y, n4.z);
+    T.L = a.L;
+    T.nr = hr->mcount * a.L;
+    T.one_row = hr->rows == 1;
+    T.one_col = hr->cols == 1;
+    return T;
+}
+
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs p)
+{
+    const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+    const int2 tl = a.tiles[tile];
+    const DiffTile T = diff_tile(a, p, tl.x);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long *S = p.S + (size_t)(xcc_id() & 7u) * p.npx;
+    unsigned long long esum = 0;
+    int r = tl.y + (int)threadIdx.x;
+    int m = r / T.L;
+    int l = r - m * T.L;
+    const int dm = WG / T.L, dl = WG - (WG / T.L) * T.L;
+    for (int k = 0; k < a.rays_per_thread; ++k, r += WG) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= T.L) {
+                l -= T.L;
+                ++m;
+            }
+        }
+        bool vis = false;
+        if (r < T.nr) {
+            v3 orig, to_sun;
+            u4 x;
+            diff_sample<SUN, POISSON>(a, T, m, l, orig, to_sun, x);
+            const cell3 c0 = start_cell(a, orig);
+            if (dda(a, orig, c0, to_sun, T.h) < 0) {
+                const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
+                DiffRay o;
+                if (diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o) && dda(a, orig, c0, o.r, T.h) < 0) {
+                    vis = true;
+                    esum += (unsigned long long)((double)o.e * 4294967296.0);
+                    const int x0 = max(0, trunc_i(__builtin_ceilf(o.X - DIFF_R)));
+                    const int x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(o.X + DIFF_R)));
+                    const int y0 = max(0, trunc_i(__builtin_ceilf(o.Y - DIFF_R)));
+                    const int y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(o.Y + DIFF_R)));
+                    for (int yy = y0; yy <= y1; ++yy) {
+                        const float ky = o.e * kern((float)yy - o.Y, p);
+                        for (int xx = x0; xx <= x1; ++xx) {
+                            const float w = ky * kern((float)xx - o.X, p);
+                            if (w > 0.0f)
+                                atomicAdd(S + (size_t)yy * p.res_x + xx,
+                                          (unsigned long long)((double)w * 4294967296.0));
+                        }
+                    }
+                }
+            }
+        }
+        const unsigned long long bits = __ballot(vis);
+        if (lane == 0) p.mask[((size_t)tile * a.rays_per_thread + k) * (WG / 64) + wave] = bits;
+    }
+    if (esum) atomicAdd(p.E + (xcc_id() & 7u), esum);
+}
+
+// copies -> double map (and energy); clears the copies
+__global__ void diff_decode_kernel(DiffArgs p, double *S_out, double *E_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        unsigned long long e = 0;
+        for (int c = 0; c < DIFF_COPIES; ++c) {
+            e += p.E[c];
+            p.E[c] = 0ull;
+        }
+        *E_out = (double)e * 2.3283064365386963e-10;
+    }
+    if (i >= p.npx) return;
+    unsigned long long s = 0;
+    for (int c = 0; c < DIFF_COPIES; ++c) {
+        unsigned long long &w = p.S[(size_t)c * p.npx + i];
+        s += w;
+        w = 0ull;
+    }
+    S_out[i] = (double)s * 2.3283064365386963e-10;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Adjoint: every visible ray (per the forward ballots) is recomputed from its Philox counter
+// without the walks, dL/dX, dL/dY, dL/de are gathered from dL/dS over its kernel window and
+// carried back through the plane intersection, normalisation, reflection, energy and the
+// normal's normalisation to dL/dn(theta); a tile's sum goes to tile_grad.
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs a, DiffArgs p)
+{
+    __shared__ double s_red[WG / 64][3];
+    const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+    const int2 tl = a.tiles[tile];
+    const DiffTile T = diff_tile(a, p, tl.x);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double gE = *p.gE;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    int r = tl.y + (int)threadIdx.x;
+    int m = r / T.L;
+    int l = r - m * T.L;
+    const int dm = WG / T.L, dl = WG - (WG / T.L) * T.L;
+    for (int k = 0; k < a.rays_per_thread; ++k, r += WG) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= T.L) {
+                l -= T.L;
+                ++m;
+            }
+        }
+        if (r >= T.nr) break;
+        const unsigned long long bits = p.mask[((size_t)tile * a.rays_per_thread + k) * (WG / 64) + wave];
+        if (!((bits >> lane) & 1ull)) continue;
+        v3 orig, to_sun;
+        u4 x;
+        diff_sample<SUN, POISSON>(a, T, m, l, orig, to_sun, x);
+        const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
+        DiffRay o;
+        if (!diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o)) continue; // (not reached)
+        // gather over the kernel window
+        const int x0 = max(0, trunc_i(__builtin_ceilf(o.X - DIFF_R)));
+        const int x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(o.X + DIFF_R)));
+        const int y0 = max(0, trunc_i(__builtin_ceilf(o.Y - DIFF_R)));
+        const int y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(o.Y + DIFF_R)));
+        float sA = 0.0f, sX = 0.0f, sY = 0.0f;
+        for (int yy = y0; yy <= y1; ++yy) {
+            const float dy = (float)yy - o.Y;
+            const float ky = kern(dy, p), kdy = dkern(dy, p);
+            for (int xx = x0; xx <= x1; ++xx) {
+                const float dx = (float)xx - o.X;
+                const float g = (float)p.g[(size_t)yy * p.res_x + xx];
+                const float kx = kern(dx, p);
+                sA += g * kx * ky;
+                sX += g * dkern(dx, p) * ky;
+                sY += g * kx * kdy;
+            }
+        }
+        const float LX = -o.e * sX, LY = -o.e * sY, Le = sA + (float)gE;
+        // (Y, X) = (u res_y, v res_x) - 1/2;  u = (p - A).E1 / |E1|^2, v likewise
+        const float Lu = LY * (float)p.res_y, Lv = LX * (float)p.res_x;
+        const v3 gp = p.E1 * (Lu * p.iE1) + p.E2 * (Lv * p.iE2);
+        // p = o + t r, t = (A - o).N / (r.N)
+        const v3 gr = (gp - p.N * (dot(gp, o.r) / o.den)) * o.t;
+        // r = w / |w|
+        const float wl = sqrt_rn(dot(o.w, o.w));
+        const v3 gw = (gr - o.r * dot(gr, o.r)) * (1.0f / wl);
+        // w = i - 2 (nrm.i) nrm
+        const float di = dot(o.nrm, o.i);
+        v3 gn = (gw * di + o.i * dot(gw, o.nrm)) * -2.0f;
+        // e = |sun.nrm|
+        gn = gn + p.sun * (dot(p.sun, o.nrm) >= 0.0f ? Le : -Le);
+        // nrm = q / |q|, q = n(theta) + deviation
+        const float ql = sqrt_rn(dot(o.q, o.q));
+        const v3 gq = (gn - o.nrm * dot(gn, o.nrm)) * (1.0f / ql);
+        gx += (double)gq.x;
+        gy += (double)gq.y;
+        gz += (double)gq.z;
+    }
+    gx = wave_sum(gx);
+    gy = wave_sum(gy);
+    gz = wave_sum(gz);
+    if (lane == 0) {
+        s_red[wave][0] = gx;
+        s_red[wave][1] = gy;
+        s_red[wave][2] = gz;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < WG / 64; ++w) t += s_red[w][threadIdx.x];
+        p.tile_grad[(size_t)tile * 4 + threadIdx.x] = t;
+    }
+}
+
+struct d3v {
+    double x, y, z;
+};
+__device__ __forceinline__ d3v D(v3 a) { return d3v{a.x, a.y, a.z}; }
+__device__ __forceinline__ double ddot(d3v a, d3v b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3v dadd(d3v a, d3v b) { return d3v{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3v dsub(d3v a, d3v b) { return d3v{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3v dscl(d3v a, double s) { return d3v{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double dlen(d3v a) { return sqrt(ddot(a, a)); }
+// adjoint of v -> v / |v| at v: g -> (g - u (g.u)) / |v|
+__device__ __forceinline__ d3v dnorm_adj(d3v v, d3v g)
+{
+    const double l = dlen(v);
+    const d3v u = dscl(v, 1.0 / l);
+    return dscl(dsub(g, dscl(u, ddot(g, u))), 1.0 / l);
+}
+
+// dL/dtheta of every heliostat: its tiles' dL/dn summed in order, then through
+// n = normalize(n_a + 1e-3 (t_u fu + t_v fv)), n_a = normalize(normalize(aim - pos) - sun),
+// aim = focus + a_u e1 + a_v e2, in double.
+__global__ void diff_params_kernel(const HelioRec *helio, DiffArgs p, float4 *grad)
+{
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= p.H) return;
+    d3v gn{0.0, 0.0, 0.0};
+    for (int t = p.tile_first[h]; t < p.tile_first[h + 1]; ++t)
+        gn = dadd(gn, d3v{p.tile_grad[(size_t)t * 4], p.tile_grad[(size_t)t * 4 + 1], p.tile_grad[(size_t)t * 4 + 2]});
+    const HelioRec &hr = helio[h];
+    const float4 th = p.theta[h];
+    const d3v pos{hr.pos[0], hr.pos[1], hr.pos[2]};
+    const d3v fu{hr.fu[0], hr.fu[1], hr.fu[2]}, fv{hr.fv[0], hr.fv[1], hr.fv[2]};
+    const d3v aim = dadd(dadd(D(p.focus), dscl(D(p.e1), th.x)), dscl(D(p.e2), th.y));
+    const d3v wv = dsub(aim, pos);
+    const d3v rd = dscl(wv, 1.0 / dlen(wv));
+    const d3v ya = dsub(rd, D(p.sun));
+    const d3v na = dscl(ya, 1.0 / dlen(ya));
+    const d3v z = dadd(na, dscl(dadd(dscl(fu, th.z), dscl(fv, th.w)), 1e-3));
+    const d3v gz = dnorm_adj(z, gn);
+    const d3v gya = dnorm_adj(ya, gz);
+    const d3v gw = dnorm_adj(wv, gya);
+    grad[h] = make_float4((float)ddot(gw, D(p.e1)), (float)ddot(gw, D(p.e2)), (float)(1e-3 * ddot(gz, fu)),
+                          (float)(1e-3 * ddot(gz, fv)));
+}
+
+// visibility of the rays of heliostats [h0, h1), one byte per ray in ray order
+__global__ void diff_vis_kernel(TraceArgs a, const unsigned long long *mask, int t0, int t1, long long base,
+                                uint8_t *out)
+{
+    const int tile = t0 + blockIdx.x;
+    if (tile >= t1) return;
+    const int2 tl = a.tiles[tile];
+    const HelioRec *hr = a.helio + tl.x;
+    const long long off = (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) - base;
+    const int nr = hr->mcount * a.L;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int r = tl.y + (int)threadIdx.x;
+    for (int k = 0; k < a.rays_per_thread && r < nr; ++k, r += WG) {
+        const unsigned long long bits = mask[((size_t)tile * a.rays_per_thread + k) * (WG / 64) + wave];
+        out[off + r] = (uint8_t)((bits >> lane) & 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------
+struct DiffLayout {
+    size_t S, E, n_theta, tile_first, tile_grad, bytes;
+};
+
+static DiffLayout diff_layout(const om_scene *s)
+{
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    DiffLayout L;
+    const size_t npx = (size_t)s->res_x * s->res_y;
+    L.S = 0;
+    L.E = al(L.S + sizeof(unsigned long long) * DIFF_COPIES * npx);
+    L.n_theta = al(L.E + sizeof(unsigned long long) * DIFF_COPIES);
+    L.tile_first = al(L.n_theta + sizeof(float4) * (size_t)(s->H + 1));
+    L.tile_grad = al(L.tile_first + sizeof(int) * (size_t)(s->H + 2));
+    L.bytes = al(L.tile_grad + sizeof(double) * 4 * (s->tiles.size() + 1));
+    return L;
+}
+
+static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
+{
+    if (!s || !s->ctx) {
+        set_error("differentiable mode needs a scene built on a context");
+        return OM_ERR_INVALID;
+    }
+    if (s->recv_type != 0) {
+        set_error("differentiable mode supports rectangle receivers");
+        return OM_ERR_UNSUPPORTED;
+    }
+    if (s->rays_per_thread > RPT_MAX) return OM_ERR_UNSUPPORTED;
+    OM_HIP(hipSetDevice(s->ctx->device));
+    const DiffLayout L = diff_layout(s);
+    if (s->diff_buf.bytes < L.bytes) {
+        om_status rc = ensure_buf(s->diff_buf, L.bytes);
+        if (rc != OM_OK) return rc;
+        char *b = (char *)s->diff_buf.p;
+        // the decode leaves the copies cleared; clear once here
+        if ((rc = launch_fill_u32((uint32_t *)b, (long long)(L.n_theta / 4), st)) != OM_OK) return rc;
+        std::vector<int> first(s->H + 1, 0);
+        size_t t = 0;
+        for (int h = 0; h <= s->H; ++h) {
+            while (t < s->tiles.size() && s->tiles[t].x < h) ++t;
+            first[h] = (int)t;
+        }
+        OM_HIP(hipMemcpyAsync(b + L.tile_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, st));
+    }
+    char *b = (char *)s->diff_buf.p;
+    std::memset(&p, 0, sizeof(p));
+    p.S = (unsigned long long *)(b + L.S);
+    p.E = (unsigned long long *)(b + L.E);
+    p.n_theta = (float4 *)(b + L.n_theta);
+    p.tile_first = (const int *)(b + L.tile_first);
+    p.tile_grad = (double *)(b + L.tile_grad);
+    p.A = s->rv[0];
+    p.E1 = s->rv[1] - s->rv[0];
+    p.E2 = s->rv[3] - s->rv[0];
+    p.N = normalize(cross(p.E1, p.E2));
+    p.iE1 = 1.0f / dot(p.E1, p.E1);
+    p.iE2 = 1.0f / dot(p.E2, p.E2);
+    p.focus = s->focus;
+    p.e1 = normalize(p.E1);
+    p.e2 = normalize(p.E2);
+    p.sun = s->sun;
+    const double R = DIFF_R, eR = std::exp(-0.5 * R * R);
+    const double Z = std::sqrt(2.0 * 3.14159265358979323846) * std::erf(R / std::sqrt(2.0)) - eR * (2.0 * R + 2.0 * R * R * R / 3.0);
+    p.kZ = (float)(1.0 / Z);
+    p.kR = (float)R;
+    p.kEr = (float)eR;
+    p.res_x = s->res_x;
+    p.res_y = s->res_y;
+    p.npx = s->res_x * s->res_y;
+    p.H = s->H;
+    return OM_OK;
+}
+
+template <bool FWD>
+static om_status launch_diff(om_scene *s, const TraceArgs &a, const DiffArgs &p, hipStream_t st)
+{
+    const dim3 grid((unsigned)s->tiles.size()), block(WG);
+    if (s->tiles.empty()) return OM_OK;
+    const bool po = s->prm.micro_mode == OM_MICRO_POISSON;
+    if (s->prm.sunshape == OM_SUN_BUIE) {
+        if (po) hipLaunchKernelGGL((FWD ? diff_forward_kernel<OM_SUN_BUIE, true> : diff_backward_kernel<OM_SUN_BUIE, true>), grid, block, 0, st, a, p);
+        else hipLaunchKernelGGL((FWD ? diff_forward_kernel<OM_SUN_BUIE, false> : diff_backward_kernel<OM_SUN_BUIE, false>), grid, block, 0, st, a, p);
+    } else {
+        if (po) hipLaunchKernelGGL((FWD ? diff_forward_kernel<OM_SUN_PILLBOX, true> : diff_backward_kernel<OM_SUN_PILLBOX, true>), grid, block, 0, st, a, p);
+        else hipLaunchKernelGGL((FWD ? diff_forward_kernel<OM_SUN_PILLBOX, false> : diff_backward_kernel<OM_SUN_PILLBOX, false>), grid, block, 0, st, a, p);
+    }
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+} // namespace om
+
+using namespace om;
+
+extern "C" om_status om_diff_mask_words(const om_scene *s, int64_t *words)
+{
+    if (!s || !words) return OM_ERR_INVALID;
+    *words = (int64_t)s->tiles.size() * s->rays_per_thread * (WG / 64);
+    return OM_OK;
+}
+
+extern "C" om_status om_diff_forward(om_scene *s, const float *d_params, uint64_t *d_mask, double *d_splat,
+                                     double *d_energy, void *stream)
+{
+    if (!s || !d_params || !d_mask || !d_splat || !d_energy) return OM_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    DiffArgs p;
+    om_status rc = diff_prepare(s, p, st);
+    if (rc != OM_OK) return rc;
+    p.theta = (const float4 *)d_params;
+    p.mask = (unsigned long long *)d_mask;
+    TraceArgs a = make_trace_args(s);
+    a.tile_base = 0;
+    a.ntiles = (int)s->tiles.size();
+    hipLaunchKernelGGL(diff_normals_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p);
+    OM_HIP(hipGetLastError());
+    if ((rc = launch_diff<true>(s, a, p, st)) != OM_OK) return rc;
+    hipLaunchKernelGGL(diff_decode_kernel, dim3((p.npx + WG - 1) / WG), dim3(WG), 0, st, p, d_splat, d_energy);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+extern "C" om_status om_diff_backward(om_scene *s, const float *d_params, const uint64_t *d_mask,
+                                      const double *d_dsplat, const double *d_denergy, float *d_grad, void *stream)
+{
+    if (!s || !d_params || !d_mask || !d_dsplat || !d_denergy || !d_grad) return OM_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    DiffArgs p;
+    om_status rc = diff_prepare(s, p, st);
+    if (rc != OM_OK) return rc;
+    p.theta = (const float4 *)d_params;
+    p.mask = (unsigned long long *)d_mask;
+    p.g = d_dsplat;
+    p.gE = d_denergy;
+    TraceArgs a = make_trace_args(s);
+    a.tile_base = 0;
+    a.ntiles = (int)s->tiles.size();
+    hipLaunchKernelGGL(diff_normals_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p);
+    OM_HIP(hipGetLastError());
+    if ((rc = launch_diff<false>(s, a, p, st)) != OM_OK) return rc;
+    hipLaunchKernelGGL(diff_params_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p,
+                       (float4 *)d_grad);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+extern "C" om_status om_diff_visibility(om_scene *s, const uint64_t *d_mask, int h0, int h1, uint8_t *d_vis,
+                                        int64_t capacity, void *stream)
+{
+    if (!s || !d_mask || !d_vis || h0 < 0 || h1 > s->H || h0 > h1) return OM_ERR_INVALID;
+    if (!s->ctx) return OM_ERR_INVALID;
+    const long long n = s->ray_offset[h1] - s->ray_offset[h0];
+    if (capacity < n) {
+        set_error("visibility buffer smaller than the rays of the heliostat range");
+        return OM_ERR_SIZE;
+    }
+    OM_HIP(hipSetDevice(s->ctx->device));
+    auto lo = std::lower_bound(s->tiles.begin(), s->tiles.end(), h0, [](const int2 &t, int h) { return t.x < h; });
+    auto hi = std::lower_bound(s->tiles.begin(), s->tiles.end(), h1, [](const int2 &t, int h) { return t.x < h; });
+    const int t0 = (int)(lo - s->tiles.begin()), t1 = (int)(hi - s->tiles.begin());
+    if (t1 <= t0) return OM_OK;
+    TraceArgs a = make_trace_args(s);
+    hipLaunchKernelGGL(diff_vis_kernel, dim3(t1 - t0), dim3(WG), 0, (hipStream_t)stream, a,
+                       (const unsigned long long *)d_mask, t0, t1, (long long)s->ray_offset[h0], d_vis);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}

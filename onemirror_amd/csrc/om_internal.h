@@ -164,6 +164,9 @@ struct om_scene {
     int *d_mcount = nullptr;
     std::vector<int2> tiles;
     int2 *d_tiles = nullptr;
+    // differentiable mode (om_diff.hip): copies of the splat map, energy sums, per-heliostat
+    // normals and tile ranges, per-tile adjoint partial sums
+    om::DevBuf diff_buf;
     int rays_per_thread = 8;
 };
 

@@ -703,6 +703,9 @@ void free_scene_dev(om_scene *s)
     dfree(s->d_poisson);
     dfree(s->d_mcount);
     dfree(s->d_tiles);
+    if (s->diff_buf.p) (void)hipFree(s->diff_buf.p);
+    s->diff_buf.p = nullptr;
+    s->diff_buf.bytes = 0;
 }
 
 // Coordinates and sizes the tracer accepts: finite and within 2^20 m; cell edges within

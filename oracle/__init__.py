@@ -125,6 +125,9 @@ def lib():
     L.oc_cylinder_pixel_batch.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, ctypes.c_float,
                                           ctypes.c_float, ctypes.c_int, ctypes.c_int, _i32p]
     L.oc_cylinder_pixel_batch.restype = ctypes.c_int
+    L.oc_diff_splat64.argtypes = [ctypes.c_void_p, _f64p, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_uint8), _f64p, _f64p, ctypes.c_int]
+    L.oc_diff_splat64.restype = ctypes.c_int
     L.oc_sample_ray.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 _f32p, _f32p]
     _LIB = L
@@ -365,6 +368,23 @@ class OracleScene:
         if records:
             out.update(shade=recs[0], block=recs[1], pixel=recs[2])
         return out
+
+    def diff_splat64(self, theta, vis, h0: int = 0, h1: Optional[int] = None, nthreads: int = 0):
+        """FP64 splat map and energy of the differentiable model for heliostats [h0, h1) at
+        parameters theta [H, 4], visibility vis (one byte per ray of the range)."""
+        h1 = self.n_helio if h1 is None else h1
+        th = np.ascontiguousarray(theta, np.float64).reshape(-1, 4)
+        v = np.ascontiguousarray(vis, np.uint8)
+        off = self.micro()["ray_offset"]
+        assert v.size == int(off[h1] - off[h0])
+        S = np.zeros(self.res_x * self.res_y, np.float64)
+        E = np.zeros(1, np.float64)
+        rc = lib().oc_diff_splat64(self._h, _p(th, _f64p), int(h0), int(h1),
+                                   v.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                   _p(S, _f64p), _p(E, _f64p), int(nthreads))
+        if rc != 0:
+            raise ValueError("differentiable model needs a rectangle receiver")
+        return S.reshape(self.res_y, self.res_x), float(E[0])
 
     def trace_pooled(self, h0=0, h1=None, groups=128, lights=None, pool_seed=12345,
                      nthreads=0) -> np.ndarray:

@@ -1015,6 +1015,7 @@ typedef struct {
     double int465, A, B, C, inv_gp1;
     d3 cyl_base;
     double cyl_r, cyl_h;
+    d3 focus;
 } oc_scene64;
 
 /* the cylinder receiver in double: the entering root of the quadratic in t */
@@ -1081,6 +1082,7 @@ static void build64(const oc_scene *s, oc_scene64 *q)
         q->cyl_r = r;
         q->cyl_h = h;
     }
+    q->focus = focus;
     q->normal = malloc(sizeof(d3) * (H + 1));
     q->v0 = malloc(sizeof(d3) * (H + 1));
     q->v1 = malloc(sizeof(d3) * (H + 1));
@@ -1478,5 +1480,129 @@ int oc_cylinder_pixel_batch(const float *orig, const float *dir, int n, const fl
 {
     for (int i = 0; i < n; ++i)
         out[i] = cylinder_pixel3(ld3(orig + 3 * i), ld3(dir + 3 * i), ld3(base), r2, h, res_x, res_y);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* differentiable model in FP64 (no reference counterpart): the same Philox draws and     */
+/* microsample centres as oc_trace_f64; per heliostat theta = (a_u, a_v, t_u, t_v):       */
+/*   aim = focus + a_u e1 + a_v e2,  n_a = normalize(normalize(aim - pos) - sun),          */
+/*   n = normalize(n_a + 1e-3 (t_u fu + t_v fv))  (fu, fv: frame of the nominal normal);  */
+/* per visible ray: nrm = normalize(n + (nrm0 - n0)), r = normalize(reflect(-to_sun, nrm)),*/
+/* receiver-plane hit -> pixel coordinates (Y, X) = (u res_y, v res_x) - 1/2, energy       */
+/* e = |sun.nrm|, splat e K(x - X) K(y - Y) with the tapered Gaussian K of radius 3 px.   */
+/* Visibility is given per ray (vis, in ray order of [h0, h1)).                           */
+/* ------------------------------------------------------------------------------------ */
+static double diff_kern(double d, double eR, double R, double iZ)
+{
+    return (exp(-0.5 * d * d) - eR * (1.0 + 0.5 * (R * R - d * d))) * iZ;
+}
+
+int oc_diff_splat64(const oc_scene *s, const double *theta, int h0, int h1, const uint8_t *vis,
+                    double *S, double *E, int nthreads)
+{
+    if (s->d.recv_type != 0) return -1;
+    if (h0 < 0) h0 = 0;
+    if (h1 > s->H) h1 = s->H;
+    oc_scene64 q;
+    build64(s, &q);
+    const int npx = s->res_x * s->res_y, L = s->p.rays_per_sample;
+    const d3 A = q.rv[0], E1 = subd(q.rv[1], q.rv[0]), E2 = subd(q.rv[3], q.rv[0]);
+    const d3 N = normalized(crossd(E1, E2)), e1 = normalized(E1), e2 = normalized(E2);
+    const double iE1 = 1.0 / dotd(E1, E1), iE2 = 1.0 / dotd(E2, E2);
+    const double R = 3.0, eR = exp(-0.5 * R * R);
+    const double iZ = 1.0 / (sqrt(2.0 * 3.14159265358979323846) * erf(R / sqrt(2.0)) - eR * (2.0 * R + 2.0 * R * R * R / 3.0));
+    int nt = 1;
+#ifdef _OPENMP
+    nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+    double *maps = (double *)calloc((size_t)nt * npx, sizeof(double));
+    double *esum = (double *)calloc((size_t)nt, sizeof(double));
+    uint32_t key[2] = { (uint32_t)s->p.seed, (uint32_t)(s->p.seed >> 32) };
+    #pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *Sm = maps + (size_t)tid * npx;
+        #pragma omp for schedule(static)
+        for (int h = h0; h < h1; ++h) {
+            const d3 n0 = q.normal[h], pos = q.pos[h];
+            d3 fu = mkd(1, 0, 0), fv = mkd(0, 0, 1);
+            if (!(fabs(n0.x) < OC_EPS && fabs(n0.z) < OC_EPS)) {
+                fu = normalized(crossd(mkd(0.0, 1.0, 0.0), n0));
+                fv = normalized(crossd(fu, n0));
+            }
+            const double *th = theta + 4 * (size_t)h;
+            d3 aim = addd(addd(q.focus, scld(e1, th[0])), scld(e2, th[1]));
+            d3 na = normalized(subd(normalized(subd(aim, pos)), q.sun));
+            d3 nt_ = normalized(addd(na, scld(addd(scld(fu, th[2]), scld(fv, th[3])), 1e-3)));
+            for (int m = 0; m < s->mcount[h]; ++m) {
+                d3 orig = addd(local2worldd(micro_local_d(s, h, m), n0), pos);
+                for (int l = 0; l < L; ++l) {
+                    int64_t rid = s->ray_offset[h] - s->ray_offset[h0] + (int64_t)m * L + l;
+                    if (!vis[rid]) continue;
+                    uint32_t c[4] = { (uint32_t)l, (uint32_t)m, (uint32_t)h,
+                                      (s->p.scene_index << 4) | OC_PURPOSE_RAY }, o[4];
+                    oc_philox4x32_10(c, key, o);
+                    double u0 = oc_u01(o[0]), u1 = oc_u01(o[1]), u2 = oc_u01(o[2]), u3 = oc_u01(o[3]);
+                    double thv;
+                    if (s->p.sunshape == OC_SUN_PILLBOX) {
+                        thv = (double)s->p.pillbox_half_angle * sqrt(u0);
+                    } else {
+                        double v = u0 * q.A;
+                        if (v < q.int465) {
+                            int lo = 0, hi = s->nbins;
+                            while (hi - lo > 1) { int md = (lo + hi) / 2; if (q.cdf_y[md] <= v) lo = md; else hi = md; }
+                            thv = (v - q.cdf_y[lo]) * q.slope[lo] + q.cdf_x[lo];
+                        } else {
+                            thv = pow((v - q.int465) * q.B + q.C, q.inv_gp1);
+                        }
+                        thv /= 1000.0;
+                    }
+                    double ph = 6.283185307179586 * u1;
+                    d3 smp = mkd(sin(thv) * cos(ph), cos(thv), sin(thv) * sin(ph));
+                    d3 to_sun = negd(local2worldd(smp, q.sun));
+                    double ths = erfinv_d(2.0 * u2 - 1.0) * sqrt(2.0) * (double)s->p.slope_sigma;
+                    double phs = 6.283185307179586 * u3;
+                    d3 turb = mkd(sin(ths) * cos(phs), cos(ths), sin(ths) * sin(phs));
+                    d3 nrm0 = normalized(local2worldd(turb, n0));
+                    d3 nrm = normalized(addd(nt_, subd(nrm0, n0)));
+                    d3 r = normalized(reflectd(negd(to_sun), nrm));
+                    double den = dotd(r, N);
+                    if (fabs(den) < OC_EPS) continue;
+                    double t = dotd(subd(A, orig), N) / den;
+                    if (t < OC_EPS) continue;
+                    d3 pp = addd(orig, scld(r, t));
+                    double u = dotd(subd(pp, A), E1) * iE1, v = dotd(subd(pp, A), E2) * iE2;
+                    double Y = u * s->res_y - 0.5, X = v * s->res_x - 0.5;
+                    double e = fabs(dotd(q.sun, nrm));
+                    esum[tid] += e;
+                    int x0 = (int)ceil(X - R), x1 = (int)floor(X + R), y0 = (int)ceil(Y - R), y1 = (int)floor(Y + R);
+                    if (x0 < 0) x0 = 0;
+                    if (y0 < 0) y0 = 0;
+                    if (x1 > s->res_x - 1) x1 = s->res_x - 1;
+                    if (y1 > s->res_y - 1) y1 = s->res_y - 1;
+                    double kx[8];
+                    for (int xx = x0; xx <= x1 && xx - x0 < 8; ++xx) kx[xx - x0] = diff_kern(xx - X, eR, R, iZ);
+                    for (int yy = y0; yy <= y1; ++yy) {
+                        double ky = e * diff_kern(yy - Y, eR, R, iZ);
+                        for (int xx = x0; xx <= x1 && xx - x0 < 8; ++xx) {
+                            double w = ky * kx[xx - x0];
+                            if (w > 0.0) Sm[(size_t)yy * s->res_x + xx] += w;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (int t = 0; t < nt; ++t) {
+        for (int i = 0; i < npx; ++i) S[i] += maps[(size_t)t * npx + i];
+        *E += esum[t];
+    }
+    free(maps);
+    free(esum);
+    free64(&q);
     return 0;
 }

@@ -166,6 +166,12 @@ void oc_rotate_y(const float o[3], const float old_dir[3], const float new_dir[3
 int  oc_ray_parallelogram(const float orig[3], const float dir[3], const float A[3],
                           const float B[3], const float C[3], float tuv[3]);
 void oc_sample_ray(const oc_scene *s, int h, int m, int l, float sun_to[3], float normal[3]);
+/* Differentiable model in FP64 (no reference counterpart): splat map S[res_y*res_x] and
+ * energy E of the visible rays of heliostats [h0, h1) at parameters theta[H][4] =
+ * (aim offset along e1, e2 in m; tracking deltas along fu, fv in mrad), accumulated into
+ * S and *E.  vis: one byte per ray of [h0, h1) in ray order (1 = visible). */
+int  oc_diff_splat64(const oc_scene *s, const double *theta, int h0, int h1, const uint8_t *vis,
+                     double *S, double *E, int nthreads);
 /* cylinder receiver pixel of n rays (new component; no reference counterpart) */
 int  oc_cylinder_pixel_batch(const float *orig, const float *dir, int n, const float base[3], float r2,
                              float h, int res_x, int res_y, int32_t *out);
