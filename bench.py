@@ -49,7 +49,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large", "batched"])
+    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large", "batched", "diff"])
     ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
     ap.add_argument("--no-grid-tune", action="store_true",
                     help="batched: keep 128^3 grids instead of tuning the resolution")
@@ -66,8 +66,8 @@ def main() -> int:
               "(every GPU box here has one GPU)", file=sys.stderr)
         return 2
 
-    if args.workload == "batched":
-        out = bench_batched(args)
+    if args.workload in ("batched", "diff"):
+        out = bench_batched(args) if args.workload == "batched" else bench_diff(args)
         print(json.dumps(out), flush=True)
         return 0
 
@@ -319,6 +319,82 @@ def bench_batched(args) -> dict:
                          "sample": f"the first 40 heliostats of {len(sample_cfg)} of the scenes "
                                    f"({nhel} heliostats, {st['rays']} rays); oracle FP32 tracer, OpenMP",
                          "seconds": cpu_t},
+    }
+
+
+def bench_diff(args) -> dict:
+    """Differentiable mode on the Mid field: forward splat (trace + Gaussian splat of the
+    parametrised aims), the spillage + softmax-peak loss, and the adjoint pass, timed together
+    per step; the plain trace of the same field timed beside it."""
+    import numpy as np
+    import torch
+
+    import onemirror_amd as om
+    import onemirror_amd.diff as D
+    from onemirror_amd import fields as F
+
+    desc, prm = F.config_mid()
+    scene = om.SolarScene(desc, prm)
+    H, R = scene.info["n_helio"], scene.info["total_rays"]
+    rng = np.random.default_rng(4)
+    th = np.zeros((H, 4), np.float32)
+    th[:, :2] = rng.normal(0.0, 0.3, (H, 2))
+    th[:, 2:] = rng.normal(0.0, 0.3, (H, 2))
+    p = torch.tensor(th, device="cuda", requires_grad=True)
+    with torch.no_grad():
+        S0, _, _ = D.forward(scene, p)
+    tau = float(0.05 * S0.max())
+
+    def step():
+        p.grad = None
+        S, E = D.splat(scene, p)
+        L = D.spill_peak_loss(S, E, tau, 1.0)
+        L.backward()
+        return L
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    t = time.perf_counter()
+    e[0].record(stream)
+    for _ in range(args.steps):
+        L = step()
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    fb_ms = e[0].elapsed_time(e[1]) / args.steps
+    # forward alone and the plain trace of the same field
+    e[0].record(stream)
+    for _ in range(args.steps):
+        with torch.no_grad():
+            D.forward(scene, p)
+    e[1].record(stream)
+    maps = scene.new_maps(smooth=False)
+    scene.trace_heliostats(0, H, maps)
+    e[2].record(stream)
+    for _ in range(args.steps):
+        scene.trace_heliostats(0, H, maps)
+    e[3].record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = e[0].elapsed_time(e[1]) / args.steps
+    plain_ms = e[2].elapsed_time(e[3]) / args.steps
+    print(f"# fwd+bwd {fb_ms:.2f} ms ({1e3 / fb_ms:.2f}/s), forward {fwd_ms:.2f} ms, plain trace "
+          f"{plain_ms:.2f} ms, loss {float(L):.6f}", file=sys.stderr, flush=True)
+    return {
+        "metric": METRIC, "value": R / dt, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "diff: the mid field (5000 heliostats, 2.01e8 rays), 4 FP32 parameters "
+                               "per heliostat (aim offsets, tracking deltas), Gaussian splat sigma 1 px "
+                               "on 256x256, loss spillage + softmax peak; one step = forward + loss + "
+                               "adjoint",
+                   "heliostats": H, "rays": R, "fwd_bwd_per_s": 1e3 / fb_ms, "fwd_bwd_ms": fb_ms,
+                   "forward_ms": fwd_ms, "backward_ms": fb_ms - fwd_ms, "plain_trace_ms": plain_ms,
+                   "fwd_bwd_vs_plain": plain_ms / fb_ms, "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": None,
     }
 
 

@@ -44,10 +44,43 @@ struct DiffArgs {
     v3 focus, e1, e2, sun;
     float kZ, kR, kEr;          // splat kernel 1/Z, radius, exp(-R^2/2)
     int res_x, res_y, npx, H;
+    // binned splat: records of the visible rays whose kernel window meets the map, then the
+    // same records copied once per map bin (BIN x BIN pixels) the window meets
+    float *recX, *recY, *recE;  // [rec_cap]
+    float *bX, *bY, *bE;        // [4 rec_cap]
+    unsigned int *n_rec;        // records written
+    unsigned int *bin_count;    // [NB]
+    unsigned int *bin_cursor;   // [NB]
+    unsigned int *bin_off;      // [NB+1] first entry of each bin
+    unsigned int *chunk_first;  // [NB+1] first chunk of each bin
+    int bin_edge, nbx, nby, NB;
 };
 
 constexpr int DIFF_COPIES = 8;
 constexpr float DIFF_R = 3.0f;
+constexpr int BIN = 16;        // map bin edge (pixels); a kernel window (<= 7 px) meets <= 2x2 bins
+constexpr int MAX_BINS = 256;  // bins of a map (BIN grows for larger maps)
+constexpr int CHUNK = 2048;    // records of a bin one splat workgroup takes at a time
+
+// pixels of the kernel window of (X, Y) inside the map; false when it misses the map
+__device__ __forceinline__ bool rec_window(const DiffArgs &p, float X, float Y, int &x0, int &x1, int &y0, int &y1)
+{
+    x0 = max(0, trunc_i(__builtin_ceilf(X - DIFF_R)));
+    x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(X + DIFF_R)));
+    y0 = max(0, trunc_i(__builtin_ceilf(Y - DIFF_R)));
+    y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(Y + DIFF_R)));
+    return x0 <= x1 && y0 <= y1;
+}
+
+// the map bins (<= 4) a window meets; returns how many
+__device__ __forceinline__ int rec_bins(const DiffArgs &p, int bin_edge, int x0, int x1, int y0, int y1, int b[4])
+{
+    const int bx0 = x0 / bin_edge, bx1 = x1 / bin_edge, by0 = y0 / bin_edge, by1 = y1 / bin_edge;
+    int n = 0;
+    for (int by = by0; by <= by1; ++by)
+        for (int bx = bx0; bx <= bx1; ++bx) b[n++] = by * p.nbx + bx;
+    return n;
+}
 
 __device__ __forceinline__ float kern(float d, const DiffArgs &p)
 {
@@ -171,11 +204,13 @@ __device__ __forceinline__ DiffTile diff_tile(const TraceArgs &a, const DiffArgs
 template <int SUN, bool POISSON>
 __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs p)
 {
+    __shared__ unsigned int s_hist[MAX_BINS];
+    for (int i = threadIdx.x; i < p.NB; i += WG) s_hist[i] = 0u;
+    __syncthreads();
     const int tile = xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
     const DiffTile T = diff_tile(a, p, tl.x);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    unsigned long long *S = p.S + (size_t)(xcc_id() & 7u) * p.npx;
     unsigned long long esum = 0;
     int r = tl.y + (int)threadIdx.x;
     int m = r / T.L;
@@ -190,7 +225,9 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs 
                 ++m;
             }
         }
-        bool vis = false;
+        bool vis = false, rec = false;
+        DiffRay o;
+        int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
         if (r < T.nr) {
             v3 orig, to_sun;
             u4 x;
@@ -198,30 +235,138 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs 
             const cell3 c0 = start_cell(a, orig);
             if (dda(a, orig, c0, to_sun, T.h) < 0) {
                 const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
-                DiffRay o;
                 if (diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o) && dda(a, orig, c0, o.r, T.h) < 0) {
                     vis = true;
                     esum += (unsigned long long)((double)o.e * 4294967296.0);
-                    const int x0 = max(0, trunc_i(__builtin_ceilf(o.X - DIFF_R)));
-                    const int x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(o.X + DIFF_R)));
-                    const int y0 = max(0, trunc_i(__builtin_ceilf(o.Y - DIFF_R)));
-                    const int y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(o.Y + DIFF_R)));
-                    for (int yy = y0; yy <= y1; ++yy) {
-                        const float ky = o.e * kern((float)yy - o.Y, p);
-                        for (int xx = x0; xx <= x1; ++xx) {
-                            const float w = ky * kern((float)xx - o.X, p);
-                            if (w > 0.0f)
-                                atomicAdd(S + (size_t)yy * p.res_x + xx,
-                                          (unsigned long long)((double)w * 4294967296.0));
-                        }
-                    }
+                    rec = rec_window(p, o.X, o.Y, x0, x1, y0, y1);
                 }
             }
         }
         const unsigned long long bits = __ballot(vis);
         if (lane == 0) p.mask[((size_t)tile * a.rays_per_thread + k) * (WG / 64) + wave] = bits;
+        // records of the rays whose window meets the map: one counter atomic per wave
+        const unsigned long long rbits = __ballot(rec);
+        if (rbits) {
+            unsigned int base = 0;
+            if (lane == 0) base = atomicAdd(p.n_rec, (unsigned int)__popcll(rbits));
+            base = __shfl(base, 0, 64);
+            if (rec) {
+                const unsigned int idx = base + (unsigned int)__popcll(rbits & ((1ull << lane) - 1ull));
+                p.recX[idx] = o.X;
+                p.recY[idx] = o.Y;
+                p.recE[idx] = o.e;
+                int b[4];
+                const int nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
+                for (int i = 0; i < nb; ++i) atomicAdd(&s_hist[b[i]], 1u);
+            }
+        }
     }
     if (esum) atomicAdd(p.E + (xcc_id() & 7u), esum);
+    __syncthreads();
+    for (int i = threadIdx.x; i < p.NB; i += WG)
+        if (s_hist[i]) atomicAdd(&p.bin_count[i], s_hist[i]);
+}
+
+// bin offsets and chunk offsets from the counts; resets the counts and cursors
+__global__ void diff_scan_kernel(DiffArgs p)
+{
+    if (threadIdx.x != 0) return;
+    unsigned int off = 0, ch = 0;
+    for (int b = 0; b < p.NB; ++b) {
+        const unsigned int c = p.bin_count[b];
+        p.bin_off[b] = off;
+        p.chunk_first[b] = ch;
+        off += c;
+        ch += (c + CHUNK - 1) / CHUNK;
+        p.bin_count[b] = 0u;
+        p.bin_cursor[b] = 0u;
+    }
+    p.bin_off[p.NB] = off;
+    p.chunk_first[p.NB] = ch;
+}
+
+// records -> per-bin copies; positions within a bin come from one cursor atomic per bin per
+// workgroup pass plus LDS counters (the order inside a bin does not matter: integer sums)
+__global__ __launch_bounds__(WG) void diff_scatter_kernel(DiffArgs p)
+{
+    __shared__ unsigned int s_cnt[MAX_BINS], s_base[MAX_BINS];
+    const unsigned int n = *p.n_rec;
+    for (unsigned int base = blockIdx.x * WG; base < n; base += gridDim.x * WG) {
+        for (int i = threadIdx.x; i < p.NB; i += WG) s_cnt[i] = 0u;
+        __syncthreads();
+        const unsigned int j = base + threadIdx.x;
+        int b[4], nb = 0;
+        unsigned int loc[4];
+        float X = 0.0f, Y = 0.0f, e = 0.0f;
+        if (j < n) {
+            X = p.recX[j];
+            Y = p.recY[j];
+            e = p.recE[j];
+            int x0, x1, y0, y1;
+            rec_window(p, X, Y, x0, x1, y0, y1);
+            nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
+            for (int i = 0; i < nb; ++i) loc[i] = atomicAdd(&s_cnt[b[i]], 1u);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < p.NB; i += WG) s_base[i] = s_cnt[i] ? atomicAdd(&p.bin_cursor[i], s_cnt[i]) : 0u;
+        __syncthreads();
+        for (int i = 0; i < nb; ++i) {
+            const unsigned int q = p.bin_off[b[i]] + s_base[b[i]] + loc[i];
+            p.bX[q] = X;
+            p.bY[q] = Y;
+            p.bE[q] = e;
+        }
+        __syncthreads();
+    }
+}
+
+// one chunk of a bin at a time: the window pixels inside the bin accumulate in LDS (fixed
+// point), then each nonzero pixel is added once to this XCD's map copy
+__global__ __launch_bounds__(WG) void diff_splat_kernel(DiffArgs p)
+{
+    extern __shared__ unsigned long long s_tile[];
+    const int B = p.bin_edge;
+    const unsigned int nch = p.chunk_first[p.NB];
+    unsigned long long *S = p.S + (size_t)(xcc_id() & 7u) * p.npx;
+    for (unsigned int c = blockIdx.x; c < nch; c += gridDim.x) {
+        // bin of chunk c: the last bin whose first chunk is <= c
+        int lo = 0, hi = p.NB;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (p.chunk_first[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const int bin = lo;
+        const unsigned int start = p.bin_off[bin] + (c - p.chunk_first[bin]) * CHUNK;
+        const unsigned int end = min(start + CHUNK, p.bin_off[bin + 1]);
+        const int tx0 = (bin % p.nbx) * B, ty0 = (bin / p.nbx) * B;
+        for (int i = threadIdx.x; i < B * B; i += WG) s_tile[i] = 0ull;
+        __syncthreads();
+        for (unsigned int j = start + threadIdx.x; j < end; j += WG) {
+            const float X = p.bX[j], Y = p.bY[j], e = p.bE[j];
+            int x0, x1, y0, y1;
+            rec_window(p, X, Y, x0, x1, y0, y1);
+            x0 = max(x0, tx0);
+            x1 = min(x1, tx0 + B - 1);
+            y0 = max(y0, ty0);
+            y1 = min(y1, ty0 + B - 1);
+            for (int yy = y0; yy <= y1; ++yy) {
+                const float ky = e * kern((float)yy - Y, p);
+                for (int xx = x0; xx <= x1; ++xx) {
+                    const float w = ky * kern((float)xx - X, p);
+                    if (w > 0.0f)
+                        atomicAdd(&s_tile[(yy - ty0) * B + (xx - tx0)], (unsigned long long)((double)w * 4294967296.0));
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < B * B; i += WG) {
+            const unsigned long long v = s_tile[i];
+            const int yy = ty0 + i / B, xx = tx0 + i % B;
+            if (v && yy < p.res_y && xx < p.res_x) atomicAdd(S + (size_t)yy * p.res_x + xx, v);
+        }
+        __syncthreads();
+    }
 }
 
 // copies -> double map (and energy); clears the copies
@@ -409,20 +554,28 @@ __global__ void diff_vis_kernel(TraceArgs a, const unsigned long long *mask, int
 // host side
 // ---------------------------------------------------------------------------------------
 struct DiffLayout {
-    size_t S, E, n_theta, tile_first, tile_grad, bytes;
+    size_t S, E, n_rec, bins, n_theta, tile_first, tile_grad, rec, bin_rec, bytes;
+    size_t rec_cap;
 };
 
+// one allocation per scene: the cleared part (map copies, energy, counters, bin tables)
+// first, then the per-heliostat and per-tile arrays, the records and their bin copies
 static DiffLayout diff_layout(const om_scene *s)
 {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     DiffLayout L;
     const size_t npx = (size_t)s->res_x * s->res_y;
+    L.rec_cap = (size_t)s->total_rays;
     L.S = 0;
     L.E = al(L.S + sizeof(unsigned long long) * DIFF_COPIES * npx);
-    L.n_theta = al(L.E + sizeof(unsigned long long) * DIFF_COPIES);
+    L.n_rec = al(L.E + sizeof(unsigned long long) * DIFF_COPIES);
+    L.bins = al(L.n_rec + sizeof(unsigned int));
+    L.n_theta = al(L.bins + sizeof(unsigned int) * 4 * (MAX_BINS + 1));
     L.tile_first = al(L.n_theta + sizeof(float4) * (size_t)(s->H + 1));
     L.tile_grad = al(L.tile_first + sizeof(int) * (size_t)(s->H + 2));
-    L.bytes = al(L.tile_grad + sizeof(double) * 4 * (s->tiles.size() + 1));
+    L.rec = al(L.tile_grad + sizeof(double) * 4 * (s->tiles.size() + 1));
+    L.bin_rec = al(L.rec + sizeof(float) * 3 * L.rec_cap);
+    L.bytes = al(L.bin_rec + sizeof(float) * 3 * 4 * L.rec_cap);
     return L;
 }
 
@@ -437,6 +590,10 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
         return OM_ERR_UNSUPPORTED;
     }
     if (s->rays_per_thread > RPT_MAX) return OM_ERR_UNSUPPORTED;
+    if (s->total_rays >= (1LL << 30) || s->res_x > 1024 || s->res_y > 1024) {
+        set_error("differentiable mode supports up to 2^30 rays and maps up to 1024 pixels a side");
+        return OM_ERR_UNSUPPORTED;
+    }
     OM_HIP(hipSetDevice(s->ctx->device));
     const DiffLayout L = diff_layout(s);
     if (s->diff_buf.bytes < L.bytes) {
@@ -460,6 +617,26 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
     p.n_theta = (float4 *)(b + L.n_theta);
     p.tile_first = (const int *)(b + L.tile_first);
     p.tile_grad = (double *)(b + L.tile_grad);
+    p.n_rec = (unsigned int *)(b + L.n_rec);
+    unsigned int *bins = (unsigned int *)(b + L.bins);
+    p.bin_count = bins;
+    p.bin_cursor = bins + (MAX_BINS + 1);
+    p.bin_off = bins + 2 * (MAX_BINS + 1);
+    p.chunk_first = bins + 3 * (MAX_BINS + 1);
+    float *rec = (float *)(b + L.rec);
+    p.recX = rec;
+    p.recY = rec + L.rec_cap;
+    p.recE = rec + 2 * L.rec_cap;
+    float *brec = (float *)(b + L.bin_rec);
+    p.bX = brec;
+    p.bY = brec + 4 * L.rec_cap;
+    p.bE = brec + 8 * L.rec_cap;
+    // bins of bin_edge pixels (a multiple of 16), at most MAX_BINS of them
+    const int mres = std::max(s->res_x, s->res_y);
+    p.bin_edge = BIN * ((mres + BIN * 16 - 1) / (BIN * 16));
+    p.nbx = (s->res_x + p.bin_edge - 1) / p.bin_edge;
+    p.nby = (s->res_y + p.bin_edge - 1) / p.bin_edge;
+    p.NB = p.nbx * p.nby;
     p.A = s->rv[0];
     p.E1 = s->rv[1] - s->rv[0];
     p.E2 = s->rv[3] - s->rv[0];
@@ -525,7 +702,13 @@ extern "C" om_status om_diff_forward(om_scene *s, const float *d_params, uint64_
     a.ntiles = (int)s->tiles.size();
     hipLaunchKernelGGL(diff_normals_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p);
     OM_HIP(hipGetLastError());
+    if ((rc = launch_fill_u32(p.n_rec, 1, st)) != OM_OK) return rc;
     if ((rc = launch_diff<true>(s, a, p, st)) != OM_OK) return rc;
+    hipLaunchKernelGGL(diff_scan_kernel, dim3(1), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(diff_scatter_kernel, dim3(2048), dim3(WG), 0, st, p);
+    hipLaunchKernelGGL(diff_splat_kernel, dim3(4096), dim3(WG), sizeof(unsigned long long) * p.bin_edge * p.bin_edge,
+                       st, p);
+    OM_HIP(hipGetLastError());
     hipLaunchKernelGGL(diff_decode_kernel, dim3((p.npx + WG - 1) / WG), dim3(WG), 0, st, p, d_splat, d_energy);
     OM_HIP(hipGetLastError());
     return OM_OK;
