@@ -36,6 +36,7 @@ struct DiffArgs {
     unsigned long long *S;      // [8][npx] splat, fixed point 2^-32
     unsigned long long *E;      // [8] energy, fixed point 2^-32
     const double *g;            // [npx] dL/dS
+    float *gf;                  // [npx + 16] dL/dS in float (read 8 pixels at a time)
     const double *gE;           // dL/dE
     double *tile_grad;          // [tiles][4] per-tile sums of dL/dn (3 used)
     const int *tile_first;      // [H+1] first tile of each heliostat
@@ -89,6 +90,14 @@ __device__ __forceinline__ float kern(float d, const DiffArgs &p)
 __device__ __forceinline__ float dkern(float d, const DiffArgs &p)
 {
     return -d * (exp_(-0.5f * d * d) - p.kEr) * p.kZ;
+}
+
+// K and K' at d from one exponential
+__device__ __forceinline__ void kern2(float d, const DiffArgs &p, float &k, float &dk)
+{
+    const float ex = exp_(-0.5f * d * d);
+    k = (ex - p.kEr * (1.0f + 0.5f * (p.kR * p.kR - d * d))) * p.kZ;
+    dk = -d * (ex - p.kEr) * p.kZ;
 }
 
 // n(theta) of every heliostat
@@ -205,8 +214,11 @@ template <int SUN, bool POISSON>
 __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs p)
 {
     __shared__ unsigned int s_hist[MAX_BINS];
+    __shared__ float s_rx[RPT_MAX * WG], s_ry[RPT_MAX * WG], s_re[RPT_MAX * WG];
+    __shared__ unsigned int s_wt[WG / 64], s_base;
     for (int i = threadIdx.x; i < p.NB; i += WG) s_hist[i] = 0u;
     __syncthreads();
+    int nrec = 0; // records of this thread, staged in LDS slots [nrec][thread]
     const int tile = xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
     const DiffTile T = diff_tile(a, p, tl.x);
@@ -244,25 +256,39 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs 
         }
         const unsigned long long bits = __ballot(vis);
         if (lane == 0) p.mask[((size_t)tile * a.rays_per_thread + k) * (WG / 64) + wave] = bits;
-        // records of the rays whose window meets the map: one counter atomic per wave
-        const unsigned long long rbits = __ballot(rec);
-        if (rbits) {
-            unsigned int base = 0;
-            if (lane == 0) base = atomicAdd(p.n_rec, (unsigned int)__popcll(rbits));
-            base = __shfl(base, 0, 64);
-            if (rec) {
-                const unsigned int idx = base + (unsigned int)__popcll(rbits & ((1ull << lane) - 1ull));
-                p.recX[idx] = o.X;
-                p.recY[idx] = o.Y;
-                p.recE[idx] = o.e;
-                int b[4];
-                const int nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
-                for (int i = 0; i < nb; ++i) atomicAdd(&s_hist[b[i]], 1u);
-            }
+        // a record for each ray whose window meets the map
+        if (rec) {
+            s_rx[nrec * WG + threadIdx.x] = o.X;
+            s_ry[nrec * WG + threadIdx.x] = o.Y;
+            s_re[nrec * WG + threadIdx.x] = o.e;
+            ++nrec;
+            int b[4];
+            const int nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
+            for (int i = 0; i < nb; ++i) atomicAdd(&s_hist[b[i]], 1u);
         }
     }
     if (esum) atomicAdd(p.E + (xcc_id() & 7u), esum);
+    // workgroup prefix of the record counts, one counter atomic per workgroup
+    unsigned int incl = (unsigned int)nrec;
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_wt[wave] = incl;
     __syncthreads();
+    unsigned int woff = 0, total = 0;
+    for (int w = 0; w < WG / 64; ++w) {
+        if (w < wave) woff += s_wt[w];
+        total += s_wt[w];
+    }
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(p.n_rec, total) : 0u;
+    __syncthreads();
+    const unsigned int pos = s_base + woff + incl - (unsigned int)nrec;
+    for (int i = 0; i < nrec; ++i) {
+        p.recX[pos + i] = s_rx[i * WG + threadIdx.x];
+        p.recY[pos + i] = s_ry[i * WG + threadIdx.x];
+        p.recE[pos + i] = s_re[i * WG + threadIdx.x];
+    }
     for (int i = threadIdx.x; i < p.NB; i += WG)
         if (s_hist[i]) atomicAdd(&p.bin_count[i], s_hist[i]);
 }
@@ -286,35 +312,52 @@ __global__ void diff_scan_kernel(DiffArgs p)
 }
 
 // records -> per-bin copies; positions within a bin come from one cursor atomic per bin per
-// workgroup pass plus LDS counters (the order inside a bin does not matter: integer sums)
+// workgroup pass (SCATTER_K records per thread) plus LDS counters (the order inside a bin
+// does not matter: the sums are integers)
+constexpr int SCATTER_K = 8;
 __global__ __launch_bounds__(WG) void diff_scatter_kernel(DiffArgs p)
 {
     __shared__ unsigned int s_cnt[MAX_BINS], s_base[MAX_BINS];
+    __shared__ unsigned int s_loc[SCATTER_K * 4 * WG];  // index of the copy within its bin
+    __shared__ unsigned short s_bin[SCATTER_K * 4 * WG]; // bin of the copy, 0xffff = none
     const unsigned int n = *p.n_rec;
-    for (unsigned int base = blockIdx.x * WG; base < n; base += gridDim.x * WG) {
+    for (unsigned int base = blockIdx.x * WG * SCATTER_K; base < n; base += gridDim.x * WG * SCATTER_K) {
         for (int i = threadIdx.x; i < p.NB; i += WG) s_cnt[i] = 0u;
         __syncthreads();
-        const unsigned int j = base + threadIdx.x;
-        int b[4], nb = 0;
-        unsigned int loc[4];
-        float X = 0.0f, Y = 0.0f, e = 0.0f;
-        if (j < n) {
-            X = p.recX[j];
-            Y = p.recY[j];
-            e = p.recE[j];
-            int x0, x1, y0, y1;
-            rec_window(p, X, Y, x0, x1, y0, y1);
-            nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
-            for (int i = 0; i < nb; ++i) loc[i] = atomicAdd(&s_cnt[b[i]], 1u);
+        for (int k = 0; k < SCATTER_K; ++k) {
+            const unsigned int j = base + k * WG + threadIdx.x;
+            int nb = 0, b[4];
+            if (j < n) {
+                int x0, x1, y0, y1;
+                rec_window(p, p.recX[j], p.recY[j], x0, x1, y0, y1);
+                nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
+            }
+            for (int i = 0; i < 4; ++i) {
+                const int slot = (k * 4 + i) * WG + threadIdx.x;
+                if (i < nb) {
+                    s_bin[slot] = (unsigned short)b[i];
+                    s_loc[slot] = atomicAdd(&s_cnt[b[i]], 1u);
+                } else {
+                    s_bin[slot] = 0xffff;
+                }
+            }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < p.NB; i += WG) s_base[i] = s_cnt[i] ? atomicAdd(&p.bin_cursor[i], s_cnt[i]) : 0u;
         __syncthreads();
-        for (int i = 0; i < nb; ++i) {
-            const unsigned int q = p.bin_off[b[i]] + s_base[b[i]] + loc[i];
-            p.bX[q] = X;
-            p.bY[q] = Y;
-            p.bE[q] = e;
+        for (int k = 0; k < SCATTER_K; ++k) {
+            const unsigned int j = base + k * WG + threadIdx.x;
+            if (j >= n) continue;
+            const float X = p.recX[j], Y = p.recY[j], e = p.recE[j];
+            for (int i = 0; i < 4; ++i) {
+                const int slot = (k * 4 + i) * WG + threadIdx.x;
+                const unsigned int bb = s_bin[slot];
+                if (bb == 0xffff) break;
+                const unsigned int q = p.bin_off[bb] + s_base[bb] + s_loc[slot];
+                p.bX[q] = X;
+                p.bY[q] = Y;
+                p.bE[q] = e;
+            }
         }
         __syncthreads();
     }
@@ -350,12 +393,16 @@ __global__ __launch_bounds__(WG) void diff_splat_kernel(DiffArgs p)
             x1 = min(x1, tx0 + B - 1);
             y0 = max(y0, ty0);
             y1 = min(y1, ty0 + B - 1);
+            float kx[7];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) kx[i] = kern((float)(x0 + i) - X, p);
             for (int yy = y0; yy <= y1; ++yy) {
                 const float ky = e * kern((float)yy - Y, p);
-                for (int xx = x0; xx <= x1; ++xx) {
-                    const float w = ky * kern((float)xx - X, p);
-                    if (w > 0.0f)
-                        atomicAdd(&s_tile[(yy - ty0) * B + (xx - tx0)], (unsigned long long)((double)w * 4294967296.0));
+#pragma unroll
+                for (int i = 0; i < 7; ++i) {
+                    const float w = ky * kx[i];
+                    if (x0 + i <= x1 && w > 0.0f)
+                        atomicAdd(&s_tile[(yy - ty0) * B + (x0 + i - tx0)], (unsigned long long)((double)w * 4294967296.0));
                 }
             }
         }
@@ -367,6 +414,13 @@ __global__ __launch_bounds__(WG) void diff_splat_kernel(DiffArgs p)
         }
         __syncthreads();
     }
+}
+
+__global__ void diff_gconv_kernel(DiffArgs p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < p.npx) p.gf[i] = (float)p.g[i];
+    else if (i < p.npx + 16) p.gf[i] = 0.0f;
 }
 
 // copies -> double map (and energy); clears the copies
@@ -438,18 +492,30 @@ __global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs a, DiffArgs
         const int x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(o.X + DIFF_R)));
         const int y0 = max(0, trunc_i(__builtin_ceilf(o.Y - DIFF_R)));
         const int y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(o.Y + DIFF_R)));
+        float kx[7], dkx[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) kern2((float)(x0 + i) - o.X, p, kx[i], dkx[i]);
+        // window pixels outside [x0, x1] carry zero weight
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+            if (x0 + i > x1) kx[i] = dkx[i] = 0.0f;
         float sA = 0.0f, sX = 0.0f, sY = 0.0f;
         for (int yy = y0; yy <= y1; ++yy) {
-            const float dy = (float)yy - o.Y;
-            const float ky = kern(dy, p), kdy = dkern(dy, p);
-            for (int xx = x0; xx <= x1; ++xx) {
-                const float dx = (float)xx - o.X;
-                const float g = (float)p.g[(size_t)yy * p.res_x + xx];
-                const float kx = kern(dx, p);
-                sA += g * kx * ky;
-                sX += g * dkern(dx, p) * ky;
-                sY += g * kx * kdy;
+            float ky, kdy;
+            kern2((float)yy - o.Y, p, ky, kdy);
+            // eight consecutive pixels from x0 (the buffer is padded past the map's end)
+            const float *grow = p.gf + (size_t)yy * p.res_x + x0;
+            float gv[8];
+            __builtin_memcpy(gv, grow, sizeof(gv));
+            float rA = 0.0f, rX = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {
+                rA += gv[i] * kx[i];
+                rX += gv[i] * dkx[i];
             }
+            sA += ky * rA;
+            sX += ky * rX;
+            sY += kdy * rA;
         }
         const float LX = -o.e * sX, LY = -o.e * sY, Le = sA + (float)gE;
         // (Y, X) = (u res_y, v res_x) - 1/2;  u = (p - A).E1 / |E1|^2, v likewise
@@ -554,7 +620,7 @@ __global__ void diff_vis_kernel(TraceArgs a, const unsigned long long *mask, int
 // host side
 // ---------------------------------------------------------------------------------------
 struct DiffLayout {
-    size_t S, E, n_rec, bins, n_theta, tile_first, tile_grad, rec, bin_rec, bytes;
+    size_t S, E, n_rec, bins, n_theta, tile_first, tile_grad, gf, rec, bin_rec, bytes;
     size_t rec_cap;
 };
 
@@ -573,7 +639,8 @@ static DiffLayout diff_layout(const om_scene *s)
     L.n_theta = al(L.bins + sizeof(unsigned int) * 4 * (MAX_BINS + 1));
     L.tile_first = al(L.n_theta + sizeof(float4) * (size_t)(s->H + 1));
     L.tile_grad = al(L.tile_first + sizeof(int) * (size_t)(s->H + 2));
-    L.rec = al(L.tile_grad + sizeof(double) * 4 * (s->tiles.size() + 1));
+    L.gf = al(L.tile_grad + sizeof(double) * 4 * (s->tiles.size() + 1));
+    L.rec = al(L.gf + sizeof(float) * (npx + 16));
     L.bin_rec = al(L.rec + sizeof(float) * 3 * L.rec_cap);
     L.bytes = al(L.bin_rec + sizeof(float) * 3 * 4 * L.rec_cap);
     return L;
@@ -618,6 +685,7 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
     p.tile_first = (const int *)(b + L.tile_first);
     p.tile_grad = (double *)(b + L.tile_grad);
     p.n_rec = (unsigned int *)(b + L.n_rec);
+    p.gf = (float *)(b + L.gf);
     unsigned int *bins = (unsigned int *)(b + L.bins);
     p.bin_count = bins;
     p.bin_cursor = bins + (MAX_BINS + 1);
@@ -730,6 +798,8 @@ extern "C" om_status om_diff_backward(om_scene *s, const float *d_params, const 
     a.tile_base = 0;
     a.ntiles = (int)s->tiles.size();
     hipLaunchKernelGGL(diff_normals_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p);
+    OM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(diff_gconv_kernel, dim3((p.npx + 16 + WG - 1) / WG), dim3(WG), 0, st, p);
     OM_HIP(hipGetLastError());
     if ((rc = launch_diff<false>(s, a, p, st)) != OM_OK) return rc;
     hipLaunchKernelGGL(diff_params_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p,
