@@ -53,6 +53,8 @@ def main() -> int:
     ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
     ap.add_argument("--no-grid-tune", action="store_true",
                     help="batched: keep 128^3 grids instead of tuning the resolution")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="mid: skip the batched and fwd+bwd summaries added to the line")
     ap.add_argument("--cpu-every", type=int, default=0,
                     help="CPU baseline traces one block of 40 heliostats in every n "
                          "(default: the whole field, one block in 5 for large)")
@@ -199,6 +201,22 @@ def main() -> int:
                       + "; oracle FP32 tracer with the same early-exit walk, OpenMP",
             "seconds": cpu_t},
     }
+    if args.workload == "mid" and not args.no_modes:
+        # the other two forms of the metric, on their own workloads (3 timed steps each)
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = 3, 1
+        b = bench_batched(sub, with_cpu=False)
+        d = bench_diff(sub)
+        out["modes"] = {
+            "batched": {"value": b["value"], "unit": "rays/s", "workload": b["config"]["workload"],
+                        **{k: b["config"][k] for k in ("scenes", "rays", "batch_ms", "plain_ms",
+                                                       "plain_rays_per_s", "batched_vs_plain",
+                                                       "grid_cells", "workgroup", "rays_per_thread")}},
+            "fwd_bwd": {"value": d["config"]["fwd_bwd_per_s"], "unit": "fwd+bwd/s",
+                        "rays_per_s": d["value"], "workload": d["config"]["workload"],
+                        **{k: d["config"][k] for k in ("fwd_bwd_ms", "forward_ms", "backward_ms",
+                                                       "plain_trace_ms", "fwd_bwd_vs_plain")}},
+        }
     print(json.dumps(out), flush=True)
     return 0
 
@@ -226,7 +244,7 @@ def oracle_sample(configs, blocks_per_scene: int, blk: int = 40):
     return tot, secs, nhel, nthreads
 
 
-def bench_batched(args) -> dict:
+def bench_batched(args, with_cpu: bool = True) -> dict:
     """Batched mode: the scenes of config_batched in one persistent launch replayed from a
     hipGraph, the grid resolution and launch parameters from the autotuner; the plain path
     (one om_trace_plain per scene, back to back on the stream) timed on the same scenes."""
@@ -290,7 +308,10 @@ def bench_batched(args) -> dict:
     print(f"# batched {batch_ms:.2f} ms ({R / (batch_ms * 1e-3):.4g} rays/s), plain per scene "
           f"{plain_ms:.2f} ms ({R / (plain_ms * 1e-3):.4g} rays/s)", file=sys.stderr, flush=True)
     sample_cfg = configs[:: max(1, len(configs) // 8)]
-    st, cpu_t, nhel, nthreads = oracle_sample(sample_cfg, 1)
+    if not with_cpu:
+        sample_cfg = []
+    st, cpu_t, nhel, nthreads = oracle_sample(sample_cfg, 1) if sample_cfg else (
+        {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0}, 0.0, 0, 0)
     S = st["dda_steps_eff"] / max(1, st["rays"])
     T = st["cand_tests"] / max(1, st["rays"])
     K = st["self_skips"] / max(1, st["rays"])
@@ -314,7 +335,7 @@ def bench_batched(args) -> dict:
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
                      "frac": achieved / peak, "traffic": None, "ops_per_ray": ops_ray,
                      "dda_steps_per_ray": S, "tests_per_ray": T},
-        "cpu_baseline": {"value": st["rays"] / cpu_t, "unit": "rays/s", "cores": nthreads,
+        "cpu_baseline": None if not sample_cfg else {"value": st["rays"] / cpu_t, "unit": "rays/s", "cores": nthreads,
                          "kind": "port",
                          "sample": f"the first 40 heliostats of {len(sample_cfg)} of the scenes "
                                    f"({nhel} heliostats, {st['rays']} rays); oracle FP32 tracer, OpenMP",
