@@ -89,8 +89,10 @@ typedef struct {
  * energy; carries out of the energy field and wraps of the word are detected from the
  * returned old value and counted on the side, and a reduction kernel decodes the exact
  * counts and energies.  OM_ACC_DEVICE: two device-scope atomics (count, energy) per hit
- * straight into the output maps. */
-enum { OM_ACC_PACKED = 0, OM_ACC_DEVICE = 1 };
+ * straight into the output maps.  OM_ACC_WINDOW: packed words as OM_ACC_PACKED, but one
+ * persistent workgroup per CU keeps the central block of the map (up to 18,400 pixels) in
+ * LDS for all the tiles it traces, so only hits outside it reach device memory. */
+enum { OM_ACC_PACKED = 0, OM_ACC_DEVICE = 1, OM_ACC_WINDOW = 2 };
 
 /* Output maps of one receiver, row-major [res_y][res_x], all device memory. */
 typedef struct {

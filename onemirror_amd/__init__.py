@@ -76,7 +76,7 @@ class Params(ctypes.Structure):
     ]
 
 
-ACC_PACKED, ACC_DEVICE = 0, 1
+ACC_PACKED, ACC_DEVICE, ACC_WINDOW = 0, 1, 2
 
 
 class Flux(ctypes.Structure):

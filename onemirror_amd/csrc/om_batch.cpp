@@ -189,6 +189,7 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
         const int npx = s->res_x * s->res_y;
         om::TraceArgs a = om::make_trace_args(s);
         a.rays_per_thread = b->rpt;
+        a.tile_rays = b->wgs * b->rpt;
         a.acc_packed = 1;
         a.part = copies + px_off * ncopy;
         a.carry = side + 2 * px_off;

@@ -325,27 +325,35 @@ OM_HD float atan2_(float y, float x)
 // only the outer surface receives: the entering root of |(o + t d - base)_xz| = r taken
 // from the closest approach tc (well conditioned far from the axis), 0 <= y - y0 <= h.
 // Pixel: column = azimuth atan2(z, x) over [0, 2 pi) x res_x, row = (y - y0) / h x res_y.
-OM_HD int cylinder_pixel(v3 o, v3 d, v3 base, float r2, float h, int res_x, int res_y)
+// Pixel (row, column) of the cylinder hit; row < 0 for a miss.
+OM_HD int2 cylinder_rc(v3 o, v3 d, v3 base, float r2, float h, int res_x, int res_y)
 {
+    const int2 miss = make_int2(-1, -1);
     const float ox = o.x - base.x, oz = o.z - base.z;
     const float a = d.x * d.x + d.z * d.z;
-    if (!(a >= EPS_LT)) return -1;
+    if (!(a >= EPS_LT)) return miss;
     const float tc = -(ox * d.x + oz * d.z) / a;
     const float px = ox + tc * d.x, pz = oz + tc * d.z;
     const float d2 = px * px + pz * pz;
-    if (!(d2 < r2)) return -1;
+    if (!(d2 < r2)) return miss;
     const float t = tc - sqrt_((r2 - d2) / a);
-    if (t < EPS_LT) return -1;
+    if (t < EPS_LT) return miss;
     const float hx = ox + t * d.x, hz = oz + t * d.z;
     const float hy = (o.y + t * d.y) - base.y;
-    if (hy < 0.0f || hy > h) return -1;
+    if (hy < 0.0f || hy > h) return miss;
     float th = atan2_(hz, hx);
     if (th < 0.0f) th = th + 6.28318531f;
     int col = trunc_i(th * 0.159154943f * (float)res_x);
     int row = trunc_i(hy / h * (float)res_y);
     col = col < res_x - 1 ? col : res_x - 1;
     row = row < res_y - 1 ? row : res_y - 1;
-    return row * res_x + col;
+    return make_int2(row, col);
+}
+
+OM_HD int cylinder_pixel(v3 o, v3 d, v3 base, float r2, float h, int res_x, int res_y)
+{
+    const int2 rc = cylinder_rc(o, d, base, r2, h, res_x, res_y);
+    return rc.x >= 0 ? rc.x * res_x + rc.y : -1;
 }
 
 } // namespace om

@@ -195,17 +195,19 @@ __device__ __forceinline__ v3 perturbed_normal(const u4 &x, float sigma, const f
     return normalize<gfx>(to_world(angle2xyz(st, ct, sp, cp), hf));
 }
 
-// receiver_drawing (rectgrid_raytracing.cu:135-153): pixel of the hit, -1 for a miss; u or
-// v equal to 1 lands on the last row or column
+// receiver_drawing (rectgrid_raytracing.cu:135-153): (row, column) of the hit, row -1 for a
+// miss; u or v equal to 1 lands on the last row or column
+__device__ __forceinline__ int2 receiver_rc(const TraceArgs &a, v3 orig, v3 refl)
+{
+    if (a.recv_type == 1) return cylinder_rc(orig, refl, a.cyl_base, a.cyl_r2, a.cyl_h, a.res_x, a.res_y);
+    float t, u, v;
+    if (!parallelogram<gfx>(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) return make_int2(-1, -1);
+    return make_int2(min(trunc_i(u * (float)a.res_y), a.res_y - 1), min(trunc_i(v * (float)a.res_x), a.res_x - 1));
+}
 __device__ __forceinline__ int receiver_pixel(const TraceArgs &a, v3 orig, v3 refl)
 {
-    if (a.recv_type == 1) return cylinder_pixel(orig, refl, a.cyl_base, a.cyl_r2, a.cyl_h, a.res_x, a.res_y);
-    float t, u, v;
-    if (!parallelogram<gfx>(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) return -1;
-    int row = trunc_i(u * (float)a.res_y), col = trunc_i(v * (float)a.res_x);
-    row = min(row, a.res_y - 1);
-    col = min(col, a.res_x - 1);
-    return row * a.res_x + col;
+    const int2 rc = receiver_rc(a, orig, refl);
+    return rc.x >= 0 ? rc.x * a.res_x + rc.y : -1;
 }
 
 // XCD (accelerator complex die) that runs the calling wave.  The packed map words are kept

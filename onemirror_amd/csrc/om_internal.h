@@ -69,6 +69,7 @@ struct TraceArgs {
     const int2 *tiles;      // (heliostat, first ray of the tile)
     int tile_base;          // first tile of this launch
     int ntiles;             // tiles of this launch
+    int tile_rays;          // rays of a tile (the last tile of a heliostat may hold fewer)
     int rays_per_thread;
     // outputs
     uint32_t *counts;
@@ -85,6 +86,12 @@ struct TraceArgs {
     int px_stride;            // words between pixels of a copy
     int acc_packed;
     int pack_shift;
+    // OM_ACC_WINDOW: rows [win_y0, win_y0 + win_h) x columns [win_x0, win_x0 + win_w) of the
+    // map are accumulated in LDS by persistent workgroups, which store their windows to
+    // win_part[workgroup][win_h * win_w] at the end; chunks of tiles come from 8 queues
+    unsigned long long *win_part;
+    uint32_t *win_queue;      // [8] chunks taken from each queue, zero before the launch
+    int win_x0, win_y0, win_w, win_h;
 };
 
 // One map of a batch for the decode (reduce_batch_kernel): packed copies and side maps,
@@ -113,6 +120,7 @@ struct om_ctx {
     om::DevBuf scratch;     // raw maps for finalize/smooth
     om::DevBuf scratch2;
     om::DevBuf parts;       // per-XCD partial maps
+    int win_wgs = -1;       // persistent workgroups of window accumulation (-1: not looked up)
 };
 
 struct om_scene {

@@ -18,33 +18,183 @@
 
 namespace om {
 
-// One tile: up to rays_per_thread rays per thread of heliostat tl.x, rays tl.y + k*WGS +
-// thread.  Hits of a thread's rays are kept in LDS and added to the maps after the ray loop:
-// vector memory operations complete in issue order, so an atomic issued inside the loop
-// would hold back every later load of the wave until the memory side has carried it out.
-// s_hpx / s_hq: this workgroup's [RPT_MAX][WGS] hit slots (each thread uses only its own).
-template <int SUN, bool POISSON, bool RECORD, int WGS>
-__device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, int *s_hpx, uint32_t *s_hq,
-                                           const float4 *s_bins, const int *s_guide)
+// Add one hit (v = 2^E + q) to a packed word; the old value tells whether the add carried
+// out of the energy field (bits 0..E-1) or wrapped the word, and both events are counted in
+// the side maps of pixel px, which the decode applies.
+__device__ __forceinline__ void packed_add_lds(const TraceArgs &a, unsigned long long *w, int px, uint32_t q)
 {
-    const int h = tl.x;
-    const HelioRec *hr = a.helio + h;
-    frame hf;
-    hf.n = mk(hr->n[0], hr->n[1], hr->n[2]);
-    hf.u = mk(hr->fu[0], hr->fu[1], hr->fu[2]);
-    hf.v = mk(hr->fv[0], hr->fv[1], hr->fv[2]);
-    hf.identity = hr->ident;
-    frame sf;
-    sf.n = a.sun_n;
-    sf.u = a.sun_u;
-    sf.v = a.sun_v;
-    sf.identity = a.sun_ident;
-    const v3 hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
+    const int E = a.pack_shift;
+    const unsigned long long lowmask = (1ull << E) - 1ull;
+    const unsigned long long v = (1ull << E) + q;
+    const unsigned long long old = __hip_atomic_fetch_add(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if ((old & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
+    if (old + v < old) atomicAdd(a.wrap + px, 1u);
+}
+
+// Per-tile state of one heliostat: its record, frame and position, and the sun frame.
+struct HelioTile {
+    const HelioRec *hr;
+    int h;
+    frame hf, sf;
+    v3 hpos;
+    bool one_row, one_col;
+};
+
+__device__ __forceinline__ HelioTile helio_tile(const TraceArgs &a, int h)
+{
+    HelioTile t;
+    t.h = h;
+    t.hr = a.helio + h;
+    const HelioRec *hr = t.hr;
+    t.hf.n = mk(hr->n[0], hr->n[1], hr->n[2]);
+    t.hf.u = mk(hr->fu[0], hr->fu[1], hr->fu[2]);
+    t.hf.v = mk(hr->fv[0], hr->fv[1], hr->fv[2]);
+    t.hf.identity = hr->ident;
+    t.sf.n = a.sun_n;
+    t.sf.u = a.sun_u;
+    t.sf.v = a.sun_v;
+    t.sf.identity = a.sun_ident;
+    t.hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
+    t.one_row = hr->rows == 1;
+    t.one_col = hr->cols == 1;
+    return t;
+}
+
+// Outcome of one ray: the shading heliostat (-1: none), the blocking heliostat (-1: none,
+// -2: not looked for, the ray is shaded), the receiver pixel (row, column; row -1 for a miss
+// or an obstructed ray) and the energy |sun.n'| in 2^-31 units.
+struct RayOut {
+    int shade, block;
+    int2 rc;
+    uint32_t q;
+};
+
+// Ray r = m * L + l of the tile's heliostat: map_tracing (rectgrid_raytracing.cu:155-196).
+template <int SUN, bool POISSON>
+__device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile &t, int m, int l,
+                                            const float4 *s_bins, const int *s_guide)
+{
+    RayOut o;
+    o.shade = -1;
+    o.block = -2;
+    o.rc = make_int2(-1, -1);
+    o.q = 0u;
+    const HelioRec *hr = t.hr;
+    // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
+    v3 loc;
+    if (POISSON) {
+        float2 pt = a.poisson[(size_t)t.h * a.poisson_cap + m];
+        // valid points lie inside (0, width) x (0, height): one facet -> block 0
+        int br = t.one_row ? 0 : trunc_i(pt.y / hr->subrow_len);
+        int bc = t.one_col ? 0 : trunc_i(pt.x / hr->subcol_len);
+        loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
+    } else {
+        int row = m / hr->cols_all, col = m - row * hr->cols_all;
+        int br = row / hr->sub_rows, bc = col / hr->sub_cols;
+        loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
+                 (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
+    }
+    const v3 orig = to_world(loc, t.hf) + t.hpos;
+    const cell3 c0 = start_cell(a, orig);
+    // draws: sunshape theta, sunshape phi, slope-error normal deviate, slope-error phi
+    u4 x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)t.h, a.w3}, a.key0, a.key1);
+    float th;
+    if (SUN == OM_SUN_PILLBOX) {
+        th = a.pillbox * sqrt_rn(u01(x.x));
+    } else {
+        float v = u01(x.x) * a.buie_A;
+        if (v < a.buie_int465) {
+            float4 b = buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
+            th = (v - b.x) * b.z + b.w;
+        } else {
+            th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
+        }
+        th = div_rn(th, 1000.0f);
+    }
+    float st, ct, sp, cp;
+    sincos_small(th, st, ct);
+    sincos2pi(u01(x.y), sp, cp);
+    const v3 to_sun = -to_world(angle2xyz(st, ct, sp, cp), t.sf);
+    // the reference order: shading, perturbed normal, reflection, blocking, receiver
+    o.shade = dda(a, orig, c0, to_sun, t.h);
+    if (o.shade >= 0) return o;
+    const v3 nrm = perturbed_normal(x, a.slope_sigma, t.hf);
+    const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
+    o.block = dda(a, orig, c0, refl, t.h);
+    if (o.block >= 0) return o;
+    o.rc = receiver_rc(a, orig, refl);
+    const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
+    o.q = (uint32_t)(e * 2147483648.0f);
+    return o;
+}
+
+// Adds the hits in the first nk of a thread's hit slots (pixel, or -1 for none) to the maps.
+template <int WGS, int RPTS>
+__device__ __forceinline__ void flush_slots(const TraceArgs &a, const int *s_hpx, const uint32_t *s_hq,
+                                            const int tid, const int nk)
+{
+    if (!a.acc_packed) {
+        for (int k = 0; k < nk; ++k) {
+            const int px = s_hpx[k * WGS + tid];
+            if (px < 0) continue;
+            atomicAdd(a.counts + px, 1u);
+            atomicAdd(a.energy + px, (unsigned long long)s_hq[k * WGS + tid]);
+        }
+        return;
+    }
+    // One 64-bit atomic per hit into a packed word: v = 2^E + q.  The old value tells
+    // whether the add carried out of the energy field (bits 0..E-1) or wrapped the word;
+    // both events are counted on the side so that the reduction decodes exactly.
+    // copy: the XCD's own block of copies, one of them per workgroup (workgroup b runs on
+    // XCD b % 8, so b / 8 counts the workgroups of one XCD)
+    const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
+    unsigned long long *part = a.part + (size_t)copy * a.copy_stride;
+    const int E = a.pack_shift;
+    const unsigned long long lowmask = (1ull << E) - 1ull;
+    unsigned long long old[RPTS];
+#pragma unroll
+    for (int k = 0; k < RPTS; ++k) {
+        old[k] = 0;
+        if (k < nk) {
+            const int px = s_hpx[k * WGS + tid];
+            if (px >= 0) {
+                const unsigned long long v = (1ull << E) + s_hq[k * WGS + tid];
+                old[k] = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, v, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPTS; ++k) {
+        if (k < nk) {
+            const int px = s_hpx[k * WGS + tid];
+            if (px >= 0) {
+                const unsigned long long q = s_hq[k * WGS + tid];
+                const unsigned long long v = (1ull << E) + q;
+                if ((old[k] & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
+                if (old[k] + v < old[k]) atomicAdd(a.wrap + px, 1u);
+            }
+        }
+    }
+}
+
+// One tile: up to rays_per_thread rays per thread of heliostat tl.x, rays tl.y + k*WGS +
+// tid among the nrays rays from tl.y.  Hits of a thread's rays are kept in LDS and added to
+// the maps after the ray loop: vector memory operations complete in issue order, so an
+// atomic issued inside the loop would hold back every later load of the wave until the
+// memory side has carried it out.  s_hpx / s_hq: the [RPTS][WGS] hit slots of the WGS
+// threads (each thread uses only its own).  RECORD: every ray's shading, blocking and pixel
+// indices go to the record arrays instead (rays in the reference order).
+template <int SUN, bool POISSON, bool RECORD, int WGS, int RPTS = RPT_MAX>
+__device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, const int tid, const int nrays,
+                                           int *s_hpx, uint32_t *s_hq, const float4 *s_bins,
+                                           const int *s_guide)
+{
+    const HelioTile t = helio_tile(a, tl.x);
     const int L = a.L;
-    const int nr = hr->mcount * L;
-    const bool one_row = hr->rows == 1, one_col = hr->cols == 1;
+    const int nr = min(t.hr->mcount * L, tl.y + nrays); // end of this tile's rays
     // ray index r = m * L + l, stepped by WGS without division
-    int r = tl.y + (int)threadIdx.x;
+    int r = tl.y + tid;
     int m = r / L;
     int l = r - m * L;
     const int dm = WGS / L, dl = WGS - (WGS / L) * L;
@@ -60,117 +210,22 @@ __device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, int *s_h
         }
         if (r >= nr) break;
         nk = k + 1;
-        if (!RECORD) s_hpx[k * WGS + threadIdx.x] = -1;
-        // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
-        v3 loc;
-        if (POISSON) {
-            float2 pt = a.poisson[(size_t)h * a.poisson_cap + m];
-            // valid points lie inside (0, width) x (0, height): one facet -> block 0
-            int br = one_row ? 0 : trunc_i(pt.y / hr->subrow_len);
-            int bc = one_col ? 0 : trunc_i(pt.x / hr->subcol_len);
-            loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
-        } else {
-            int row = m / hr->cols_all, col = m - row * hr->cols_all;
-            int br = row / hr->sub_rows, bc = col / hr->sub_cols;
-            loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
-                     (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
-        }
-        const v3 orig = to_world(loc, hf) + hpos;
-        const cell3 c0 = start_cell(a, orig);
-        // draws: sunshape theta, sunshape phi, slope-error normal deviate, slope-error phi
-        u4 x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3}, a.key0, a.key1);
-        float th;
-        if (SUN == OM_SUN_PILLBOX) {
-            th = a.pillbox * sqrt_rn(u01(x.x));
-        } else {
-            float v = u01(x.x) * a.buie_A;
-            if (v < a.buie_int465) {
-                float4 b = buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
-                th = (v - b.x) * b.z + b.w;
-            } else {
-                th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
-            }
-            th = div_rn(th, 1000.0f);
-        }
-        float st, ct, sp, cp;
-        sincos_small(th, st, ct);
-        sincos2pi(u01(x.y), sp, cp);
-        const v3 to_sun = -to_world(angle2xyz(st, ct, sp, cp), sf);
+        const RayOut o = trace_ray<SUN, POISSON>(a, t, m, l, s_bins, s_guide);
+        const int px = o.rc.x >= 0 ? o.rc.x * a.res_x + o.rc.y : -1;
         if (RECORD) {
-            // the reference order, every index recorded (rectgrid_raytracing.cu:176-195)
             const long long rid =
-                (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
-            int sh = dda(a, orig, c0, to_sun, h);
-            if (sh >= 0) {
-                a.rec_shade[rid] = sh;
-                a.rec_block[rid] = -2;
-                a.rec_pixel[rid] = -1;
-                continue;
-            }
-            const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
-            const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
-            int bl = dda(a, orig, c0, refl, h);
-            int px = bl < 0 ? receiver_pixel(a, orig, refl) : -1;
-            a.rec_shade[rid] = -1;
-            a.rec_block[rid] = bl;
+                (((long long)t.hr->ray_off_hi << 32) | (unsigned)t.hr->ray_off_lo) + r - a.rec_base;
+            a.rec_shade[rid] = o.shade;
+            a.rec_block[rid] = o.block;
             a.rec_pixel[rid] = px;
             continue;
         }
-        if (dda(a, orig, c0, to_sun, h) >= 0) continue;
-        const v3 nrm = perturbed_normal(x, a.slope_sigma, hf);
-        const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
-        if (dda(a, orig, c0, refl, h) >= 0) continue;
-        const int px = receiver_pixel(a, orig, refl);
-        if (px < 0) continue;
-        const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
-        s_hpx[k * WGS + threadIdx.x] = px;
-        s_hq[k * WGS + threadIdx.x] = (uint32_t)(e * 2147483648.0f);
+        s_hpx[k * WGS + tid] = px;
+        s_hq[k * WGS + tid] = o.q;
     }
     if (RECORD) return;
     // Each thread adds its own hits after its last load.
-    if (!a.acc_packed) {
-        for (int k = 0; k < nk; ++k) {
-            const int px = s_hpx[k * WGS + threadIdx.x];
-            if (px < 0) continue;
-            atomicAdd(a.counts + px, 1u);
-            atomicAdd(a.energy + px, (unsigned long long)s_hq[k * WGS + threadIdx.x]);
-        }
-        return;
-    }
-    // One 64-bit atomic per hit into a packed word: v = 2^E + q.  The old value tells
-    // whether the add carried out of the energy field (bits 0..E-1) or wrapped the word;
-    // both events are counted on the side so that the reduction decodes exactly.
-    // copy: the XCD's own block of copies, one of them per workgroup (workgroup b runs on
-    // XCD b % 8, so b / 8 counts the workgroups of one XCD)
-    const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
-    unsigned long long *part = a.part + (size_t)copy * a.copy_stride;
-    const int E = a.pack_shift;
-    const unsigned long long lowmask = (1ull << E) - 1ull;
-    unsigned long long old[RPT_MAX];
-#pragma unroll
-    for (int k = 0; k < RPT_MAX; ++k) {
-        old[k] = 0;
-        if (k < nk) {
-            const int px = s_hpx[k * WGS + threadIdx.x];
-            if (px >= 0) {
-                const unsigned long long v = (1ull << E) + s_hq[k * WGS + threadIdx.x];
-                old[k] = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, v, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < RPT_MAX; ++k) {
-        if (k < nk) {
-            const int px = s_hpx[k * WGS + threadIdx.x];
-            if (px >= 0) {
-                const unsigned long long q = s_hq[k * WGS + threadIdx.x];
-                const unsigned long long v = (1ull << E) + q;
-                if ((old[k] & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
-                if (old[k] + v < old[k]) atomicAdd(a.wrap + px, 1u);
-            }
-        }
-    }
+    flush_slots<WGS, RPTS>(a, s_hpx, s_hq, tid, nk);
 }
 
 // One workgroup per tile of the scene's tile list (plain mode).
@@ -194,7 +249,210 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
         s_guide = lg;
     }
     const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
-    trace_tile<SUN, POISSON, RECORD, WG>(a, a.tiles[tile], s_hpx, s_hq, s_bins, s_guide);
+    trace_tile<SUN, POISSON, RECORD, WG>(a, a.tiles[tile], (int)threadIdx.x, a.tile_rays, s_hpx, s_hq, s_bins,
+                                         s_guide);
+}
+
+// ---------------------------------------------------------------------------------------
+// OM_ACC_WINDOW: window of the packed map in LDS, persistent workgroups
+// ---------------------------------------------------------------------------------------
+// One workgroup per CU keeps the central block of the packed map in LDS (WIN_PX_MAX words
+// with the ring below: the whole 160 KiB) for every ray it traces.  WIN_COMPUTE of its waves
+// trace rays: hits inside the window are LDS atomics, hits outside go to a ring buffer in
+// LDS.  The last wave drains the ring into the packed per-XCD copies in device memory, so
+// the tracing waves never hold their loads behind a memory-side atomic.  The window is
+// stored (not added) to this workgroup's slice of win_part at the end.
+constexpr int WIN_WGS = 1024;
+constexpr int WIN_COMPUTE = WIN_WGS / 64 - 1; // tracing waves; the last wave drains the ring
+constexpr int WIN_RING = 2048;                // ring entries (px << 32 | q), a power of two
+constexpr int WIN_DRAIN = 16;                 // ring entries per lane of the draining wave per round
+static_assert(64 * WIN_DRAIN + 64 <= WIN_RING, "the ring holds a drain round and a pass beyond it");
+constexpr int WIN_CHUNK = 2048;               // rays a tracing wave takes from a queue at a time
+constexpr int WIN_PX_MAX = (160 * 1024 - WIN_RING * 8 - 256) / 8;
+constexpr unsigned long long RING_EMPTY = ~0ull;
+
+struct WinRing {
+    unsigned long long *ring; // [WIN_RING]
+    uint32_t *tail;           // entries reserved by the tracing waves
+    uint32_t *head;           // entries drained (written by the draining wave only)
+    uint32_t *done;           // tracing waves that have finished
+};
+
+// Adds this pass's hits outside the window (out) to the ring, in lane order.  Lane 0
+// reserves the wave's entries with one LDS atomic and, before they are written, waits until
+// the ring has room for them; head_seen caches the last head read.  The draining wave only
+// waits for entries within 64 * WIN_DRAIN of the head, whose writers need no more room than
+// the ring has at that head (64 * WIN_DRAIN + 64 <= WIN_RING), so neither waits for the other.
+__device__ __forceinline__ void ring_push(const WinRing &R, bool out, int px, uint32_t q, int lane,
+                                          uint32_t &head_seen)
+{
+    const unsigned long long m = __ballot(out);
+    if (m == 0ull) return;
+    const uint32_t n = (uint32_t)__popcll(m);
+    uint32_t base = 0;
+    if (lane == 0) base = __hip_atomic_fetch_add(R.tail, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    base = __builtin_amdgcn_readlane(base, 0);
+    while (base + n - head_seen > (uint32_t)WIN_RING) {
+        head_seen = __hip_atomic_load(R.head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (base + n - head_seen > (uint32_t)WIN_RING) __builtin_amdgcn_s_sleep(2);
+    }
+    if (out) {
+        const uint32_t k = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        __hip_atomic_store(R.ring + ((base + k) & (uint32_t)(WIN_RING - 1)), ((unsigned long long)px << 32) | q,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// The draining wave: takes up to 64 * WIN_DRAIN entries at a time from the ring head, adds
+// them to the packed per-XCD copies (one returning 64-bit atomic each, carries and wraps
+// counted as in flush_slots), clears them and publishes the new head; ends when every
+// tracing wave has finished and the ring is empty.  Rounds are large because each waits
+// once for the memory side.
+__device__ void ring_drain(const TraceArgs &a, const WinRing &R, int lane)
+{
+    const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
+    unsigned long long *part = a.part + (size_t)copy * a.copy_stride;
+    const int E = a.pack_shift;
+    const unsigned long long lowmask = (1ull << E) - 1ull;
+    uint32_t head = 0;
+    for (;;) {
+        const uint32_t done = __hip_atomic_load(R.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t tail = __hip_atomic_load(R.tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t avail = tail - head;
+        if (avail == 0u) {
+            if (done == (uint32_t)WIN_COMPUTE) break;
+            __builtin_amdgcn_s_sleep(4);
+            continue;
+        }
+        const uint32_t nb = min(avail, (uint32_t)(64 * WIN_DRAIN));
+        unsigned long long e[WIN_DRAIN], old[WIN_DRAIN];
+#pragma unroll
+        for (int b = 0; b < WIN_DRAIN; ++b) {
+            e[b] = RING_EMPTY;
+            const uint32_t i = (uint32_t)(b * 64 + lane);
+            if (i < nb) {
+                // reserved entries are written right after their reservation
+                unsigned long long *slot = R.ring + ((head + i) & (uint32_t)(WIN_RING - 1));
+                do {
+                    e[b] = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } while (e[b] == RING_EMPTY);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < WIN_DRAIN; ++b) {
+            old[b] = 0ull;
+            if (e[b] != RING_EMPTY) {
+                const uint32_t px = (uint32_t)(e[b] >> 32);
+                const unsigned long long v = (1ull << E) + (e[b] & 0xffffffffull);
+                old[b] = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, v, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < WIN_DRAIN; ++b) {
+            if (e[b] != RING_EMPTY) {
+                const uint32_t px = (uint32_t)(e[b] >> 32);
+                const unsigned long long q = e[b] & 0xffffffffull;
+                const unsigned long long v = (1ull << E) + q;
+                if ((old[b] & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
+                if (old[b] + v < old[b]) atomicAdd(a.wrap + px, 1u);
+                __hip_atomic_store(R.ring + ((head + (uint32_t)(b * 64 + lane)) & (uint32_t)(WIN_RING - 1)),
+                                   RING_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        head += nb;
+        if (lane == 0) __hip_atomic_store(R.head, head, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// nrays (a multiple of 64) rays of heliostat tl.x from tl.y, traced by one wave, one ray per
+// lane per pass.
+template <int SUN, bool POISSON>
+__device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, const int lane, const int nrays,
+                                                   const WinRing &R, unsigned long long *s_win)
+{
+    const HelioTile t = helio_tile(a, tl.x);
+    const int L = a.L;
+    const int nr = min(t.hr->mcount * L, tl.y + nrays);
+    int r = tl.y + lane;
+    int m = r / L;
+    int l = r - m * L;
+    const int dm = 64 / L, dl = 64 - (64 / L) * L;
+    uint32_t head_seen = 0u; // a head read earlier: the true head is at least this
+    for (int k = 0; k < nrays; k += 64, r += 64) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= L) {
+                l -= L;
+                ++m;
+            }
+        }
+        bool out = false;
+        int px = 0;
+        uint32_t q = 0u;
+        if (r < nr) {
+            const RayOut o = trace_ray<SUN, POISSON>(a, t, m, l, a.buie_bins, a.buie_guide);
+            if (o.rc.x >= 0) {
+                px = o.rc.x * a.res_x + o.rc.y;
+                q = o.q;
+                const uint32_t wr = (uint32_t)(o.rc.x - a.win_y0), wc = (uint32_t)(o.rc.y - a.win_x0);
+                if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w)
+                    packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+                else
+                    out = true;
+            }
+        }
+        ring_push(R, out, px, q, lane, head_seen); // the whole wave takes part: no lane has left the loop
+    }
+}
+
+// Tracing waves take chunks of up to WIN_CHUNK rays of one tile from 8 queues, one per XCD,
+// each over a contiguous eighth of the tile list: a wave drains its own XCD's queue first,
+// then helps the others, so no wave waits for another until all queues are empty.  The
+// waves of an XCD take consecutive chunks, so at any time they trace a few neighbouring
+// heliostats whose grid cells and blockers stay in that XCD's L2 (spreading an XCD over
+// several ranges of the list is slower).
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WIN_WGS) void trace_window_kernel(TraceArgs a)
+{
+    __shared__ unsigned long long s_win[WIN_PX_MAX];
+    __shared__ unsigned long long s_ring[WIN_RING];
+    __shared__ uint32_t s_ctl[3];
+    const int wpx = a.win_w * a.win_h; // <= WIN_PX_MAX, checked on the host
+    for (int i = threadIdx.x; i < wpx; i += WIN_WGS) s_win[i] = 0ull;
+    for (int i = threadIdx.x; i < WIN_RING; i += WIN_WGS) s_ring[i] = RING_EMPTY;
+    if (threadIdx.x < 3) s_ctl[threadIdx.x] = 0u;
+    __syncthreads();
+    const WinRing R{s_ring, s_ctl, s_ctl + 1, s_ctl + 2};
+    const int lane = (int)threadIdx.x & 63;
+    if ((int)(threadIdx.x >> 6) == WIN_COMPUTE) {
+        ring_drain(a, R, lane);
+    } else {
+        // tiles hold 64..2048 rays (a power of two): chunks of min(tile, WIN_CHUNK) rays
+        const int chunk = min(a.tile_rays, WIN_CHUNK);
+        const int parts = a.tile_rays / chunk;
+        const int n = a.ntiles * parts, q = n >> 3, rmd = n & 7;
+        const uint32_t xcd = xcc_id() & 7u;
+        for (uint32_t j = 0; j < 8u; ++j) {
+            const int x = (int)((xcd + j) & 7u);
+            const int first = x * q + min(x, rmd), cnt = q + (x < rmd ? 1 : 0);
+            for (;;) {
+                uint32_t t = 0;
+                if (lane == 0) t = atomicAdd(a.win_queue + x, 1u);
+                t = __builtin_amdgcn_readfirstlane(t); // the whole wave is active here
+                if (t >= (uint32_t)cnt) break;
+                const int c = first + (int)t;
+                const int tile = c / parts, part = c - tile * parts;
+                const int2 tl = a.tiles[a.tile_base + tile];
+                trace_chunk_window<SUN, POISSON>(a, make_int2(tl.x, tl.y + part * chunk), lane, chunk, R, s_win);
+            }
+        }
+        if (lane == 0) __hip_atomic_fetch_add(R.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    unsigned long long *out = a.win_part + (size_t)blockIdx.x * wpx;
+    for (int i = threadIdx.x; i < wpx; i += WIN_WGS) out[i] = s_win[i];
 }
 
 // Batched mode: persistent workgroups take (scene, heliostat, first ray) items from a
@@ -221,7 +479,8 @@ __global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__res
         const TraceArgs &a = table[sc];
         trace_tile<SUN, POISSON, false, WGS>(a, make_int2(__builtin_amdgcn_readfirstlane(w.y),
                                                           __builtin_amdgcn_readfirstlane(w.z)),
-                                             s_hpx, s_hq, a.buie_bins, a.buie_guide);
+                                             (int)threadIdx.x, a.tile_rays, s_hpx, s_hq, a.buie_bins,
+                                             a.buie_guide);
     }
 }
 
@@ -258,11 +517,21 @@ __global__ void reduce_batch_kernel(const ReduceSlot *__restrict__ slots)
 // Decode the packed per-XCD words into the output maps (which it adds to) and clear them.
 // Over all copies, with A carries out of the energy field and W wraps of the word:
 //   count = sum(word >> E) + W * 2^(64-E) - A,    energy = sum(word & (2^E - 1)) + A * 2^E.
+// With a window (nwin workgroup slices of win_part), its pixels add the stored LDS words too.
+// Everything read is cleared, the queue counters too, so the whole buffer is zero again for
+// the next trace.
+struct WindowRef {
+    unsigned long long *part; // [nwin][w * h] window words of each workgroup
+    uint32_t *queue;          // [8] queue counters
+    int nwin, x0, y0, w, h, res_x;
+};
+
 __global__ void reduce_packed_kernel(unsigned long long *part, uint32_t *carry, uint32_t *wrap, int E,
                                      uint32_t *counts, unsigned long long *energy, int npx, int ncopy,
-                                     int copy_stride, int px_stride)
+                                     int copy_stride, int px_stride, WindowRef win)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (win.nwin > 0 && i < 8) win.queue[i] = 0u;
     if (i >= npx) return;
     const unsigned long long lowmask = (1ull << E) - 1ull;
     unsigned long long n = 0, e = 0;
@@ -273,6 +542,21 @@ __global__ void reduce_packed_kernel(unsigned long long *part, uint32_t *carry, 
         n += wv >> E;
         e += wv & lowmask;
         w = 0ull;
+    }
+    if (win.nwin > 0) {
+        const int row = i / win.res_x, col = i - row * win.res_x;
+        const uint32_t wr = (uint32_t)(row - win.y0), wc = (uint32_t)(col - win.x0);
+        if (wr < (uint32_t)win.h && wc < (uint32_t)win.w) {
+            const size_t wpx = (size_t)win.w * win.h;
+            unsigned long long *p = win.part + wr * (size_t)win.w + wc;
+#pragma unroll 8
+            for (int g = 0; g < win.nwin; ++g) {
+                const unsigned long long wv = p[(size_t)g * wpx];
+                n += wv >> E;
+                e += wv & lowmask;
+                p[(size_t)g * wpx] = 0ull;
+            }
+        }
     }
     const unsigned long long A = carry[i], W = wrap[i];
     n = n + (W << (64 - E)) - A;
@@ -694,6 +978,75 @@ static om_status check_flux(const om_scene *s, const om_flux *out)
     return OM_OK;
 }
 
+// Persistent workgroups of trace_window_kernel: one per CU (its LDS takes the whole CU)
+template <int SUN, bool POISSON>
+static int window_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_window_kernel<SUN, POISSON>, WIN_WGS, 0) !=
+        hipSuccess)
+        return 0;
+    return n;
+}
+
+static om_status launch_trace_window(om_scene *s, const TraceArgs &a, int nwg, hipStream_t st)
+{
+    const bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
+    dim3 grid(nwg), block(WIN_WGS);
+    if (s->prm.sunshape == OM_SUN_BUIE) {
+        if (poisson) hipLaunchKernelGGL((trace_window_kernel<OM_SUN_BUIE, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((trace_window_kernel<OM_SUN_BUIE, false>), grid, block, 0, st, a);
+    } else {
+        if (poisson) hipLaunchKernelGGL((trace_window_kernel<OM_SUN_PILLBOX, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((trace_window_kernel<OM_SUN_PILLBOX, false>), grid, block, 0, st, a);
+    }
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+// Window of OM_ACC_WINDOW: the whole map when it fits in WIN_PX_MAX words, else the largest
+// near-square block of rows and columns about the map centre (every heliostat aims at the
+// receiver centre); nwg = CUs x resident workgroups per CU.  The workgroup count of a device
+// is looked up once.
+static om_status window_plan(om_scene *s, WindowRef &win, int &nwg)
+{
+    const int tile_rays = s->rays_per_thread * WG;
+    if (tile_rays < 64 || (tile_rays & (tile_rays - 1)) != 0 || (tile_rays > WIN_CHUNK && tile_rays % WIN_CHUNK != 0)) {
+        set_error("window accumulation needs tiles of a power of two of at least 64 rays");
+        return OM_ERR_UNSUPPORTED;
+    }
+    if (s->ctx->win_wgs < 0) {
+        int cus = 0;
+        OM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->ctx->device));
+        const bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
+        int per = s->prm.sunshape == OM_SUN_BUIE
+                      ? (poisson ? window_blocks_per_cu<OM_SUN_BUIE, true>() : window_blocks_per_cu<OM_SUN_BUIE, false>())
+                      : (poisson ? window_blocks_per_cu<OM_SUN_PILLBOX, true>()
+                                 : window_blocks_per_cu<OM_SUN_PILLBOX, false>());
+        if (per < 1) {
+            set_error("trace_window_kernel cannot be resident on this device");
+            return OM_ERR_UNSUPPORTED;
+        }
+        s->ctx->win_wgs = cus * per;
+    }
+    nwg = s->ctx->win_wgs;
+    const int rx = s->res_x, ry = s->res_y;
+    if ((long long)rx * ry <= WIN_PX_MAX) {
+        win.w = rx;
+        win.h = ry;
+    } else {
+        int side = 1;
+        while ((side + 1) * (side + 1) <= WIN_PX_MAX) ++side;
+        win.w = std::min(rx, side);
+        win.h = std::min(ry, WIN_PX_MAX / win.w);
+        win.w = std::min(rx, WIN_PX_MAX / win.h);
+    }
+    win.x0 = (rx - win.w) / 2;
+    win.y0 = (ry - win.h) / 2;
+    win.res_x = rx;
+    return OM_OK;
+}
+
 // tiles of heliostats [h0, h1) are contiguous in the tile table
 static void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1)
 {
@@ -721,7 +1074,7 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
     const int npx = s->res_x * s->res_y;
     om_status rc;
     if (clear && (rc = clear_maps(out, npx, st)) != OM_OK) return rc;
-    if (s->prm.accumulate == OM_ACC_PACKED) {
+    if (s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW) {
         // 8 XCDs x 2^shift copies of the packed map, as many as fit in PART_BYTES (at least
         // one per XCD): same-pixel atomics of concurrent workgroups then mostly land on
         // different words.
@@ -731,10 +1084,19 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
         const int ncopy = 8 << shift;
         const int copy_stride = npx, px_stride = 1;
         const size_t n16 = (size_t)(ncopy - 1) * copy_stride + (size_t)(npx - 1) * px_stride + 1;
-        const size_t bytes = n16 * sizeof(unsigned long long) + 2 * (size_t)npx * sizeof(uint32_t);
+        size_t bytes = n16 * sizeof(unsigned long long) + 2 * (size_t)npx * sizeof(uint32_t);
+        WindowRef win{nullptr, nullptr, 0, 0, 0, 0, 0, s->res_x};
+        int nwg = 0;
+        if (s->prm.accumulate == OM_ACC_WINDOW && a.ntiles > 0 &&
+            (rc = window_plan(s, win, nwg)) != OM_OK)
+            return rc;
+        // window mode: queue counters | window slices
+        const size_t off_queue = bytes, off_win = bytes + 64;
+        if (nwg > 0) bytes = off_win + (size_t)nwg * win.w * win.h * sizeof(unsigned long long);
         const bool fresh = s->ctx->parts.bytes < bytes;
         if ((rc = ensure_buf(s->ctx->parts, bytes)) != OM_OK) return rc;
-        a.part = (unsigned long long *)s->ctx->parts.p;
+        char *base = (char *)s->ctx->parts.p;
+        a.part = (unsigned long long *)base;
         a.carry = (uint32_t *)(a.part + n16);
         a.wrap = a.carry + npx;
         a.npx = npx;
@@ -749,10 +1111,24 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
             hipLaunchKernelGGL(fill_u32_kernel, g, b, 0, st, (uint32_t *)s->ctx->parts.p, (long long)(bytes / 4));
             OM_HIP(hipGetLastError());
         }
-        if ((rc = launch_trace(s, a, false, st)) != OM_OK) return rc;
+        if (nwg > 0) {
+            a.win_queue = (uint32_t *)(base + off_queue);
+            a.win_part = (unsigned long long *)(base + off_win);
+            win.queue = a.win_queue;
+            a.win_x0 = win.x0;
+            a.win_y0 = win.y0;
+            a.win_w = win.w;
+            a.win_h = win.h;
+            win.part = a.win_part;
+            win.nwin = nwg;
+            if ((rc = launch_fill_u32(a.win_queue, 8, st)) != OM_OK) return rc;
+            if ((rc = launch_trace_window(s, a, nwg, st)) != OM_OK) return rc;
+        } else if ((rc = launch_trace(s, a, false, st)) != OM_OK) {
+            return rc;
+        }
         hipLaunchKernelGGL(reduce_packed_kernel, dim3((npx + WG - 1) / WG), dim3(WG), 0, st, a.part, a.carry,
                            a.wrap, a.pack_shift, out->counts, (unsigned long long *)out->energy, npx,
-                           ncopy, a.copy_stride, a.px_stride);
+                           ncopy, a.copy_stride, a.px_stride, win);
         OM_HIP(hipGetLastError());
         return OM_OK;
     }

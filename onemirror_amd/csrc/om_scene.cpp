@@ -776,7 +776,7 @@ om_status validate(const om_scene_desc *d, const om_params *p)
         om::set_error("buie_bins must be in 1..8192 and csr positive");
         return OM_ERR_INVALID;
     }
-    if (p->accumulate != OM_ACC_PACKED && p->accumulate != OM_ACC_DEVICE) {
+    if (p->accumulate != OM_ACC_PACKED && p->accumulate != OM_ACC_DEVICE && p->accumulate != OM_ACC_WINDOW) {
         om::set_error("unknown accumulation mode");
         return OM_ERR_INVALID;
     }
@@ -953,6 +953,7 @@ TraceArgs make_trace_args(const om_scene *s)
     a.tiles = s->d_tiles;
     a.ntiles = (int)s->tiles.size();
     a.rays_per_thread = s->rays_per_thread;
+    a.tile_rays = 256 * s->rays_per_thread;
     return a;
 }
 } // namespace om

@@ -250,12 +250,15 @@ def test_mid_records_bitexact(mid, h0):
     _records_equal(g.trace_records(h0, h1), o.trace_f32(h0, h1, records=True), f"mid[{h0}:{h1}]")
 
 
-@pytest.mark.parametrize("mode,shift", [(0, 0), (1, 0), (0, 32), (0, 33), (0, 62), (0, 61)])
+@pytest.mark.parametrize("mode,shift", [(0, 0), (1, 0), (0, 32), (0, 33), (0, 62), (0, 61),
+                                        (2, 0), (2, 32), (2, 33), (2, 62), (2, 61)])
 def test_accumulation_modes_bitexact(om, mode, shift):
     """Packed single-atomic accumulation (with its carry and wrap bookkeeping driven hard by
     narrow fields: E = 32/33 carries out of the energy field every hit or two, E = 61/62
-    wraps the count field every few hits) and two device-scope atomics per hit all reproduce
-    the oracle's integer maps."""
+    wraps the count field every few hits), the same words kept in an LDS window of the map
+    by persistent workgroups (mode 2: hits inside and outside the window, LDS words that
+    carry and wrap) and two device-scope atomics per hit all reproduce the oracle's integer
+    maps."""
     import torch
     d, p = F.config_mid()
     d = F.subset(d, slice(1000, 1400))
