@@ -101,6 +101,11 @@ def main() -> int:
           f"grid {info['grid_num']} nnz {info['grid_nnz']}", file=sys.stderr, flush=True)
     maps = scene.new_maps(smooth=True)
     stream = torch.cuda.current_stream()
+    # accumulation autotuner: per-tile packed atomics or the LDS window (identical maps)
+    acc = scene.autotune_accumulate(maps, reps=3)
+    kernel = "trace_window_kernel" if acc["accumulate"] == "window" else "trace_kernel"
+    print(f"# accumulation autotune: packed {acc['packed_ms']:.3f} ms, window {acc['window_ms']:.3f} ms "
+          f"-> {acc['accumulate']}", file=sys.stderr, flush=True)
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 4
         xs = torch.rand(n, device="cuda")
@@ -167,7 +172,7 @@ def main() -> int:
     peak = P_VALU / 1e12
     # HBM bytes per trace launch from the committed rocprof FETCH_SIZE / WRITE_SIZE runs
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"trace_kernel_{args.workload}.json")
+    prof = os.path.join(ROOT, "profiles", f"{kernel}_{args.workload}.json")
     if os.path.exists(prof):
         with open(prof) as f:
             pj = json.load(f)
@@ -189,7 +194,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl, "heliostats": H, "rays": R, "grid": list(info["grid_num"]),
                    "map": [info["res_y"], info["res_x"]], "parallelism": "single GPU",
-                   "trace_kernel_ms": trace_ms, "receiver_hits": hits},
+                   "trace_kernel_ms": trace_ms, "receiver_hits": hits, "kernel": kernel,
+                   "accumulate": acc},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "ops_per_ray": ops_ray, "dda_steps_per_ray": S, "tests_per_ray": T},

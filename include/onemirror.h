@@ -197,6 +197,13 @@ om_status om_smooth_trimmed(om_ctx *ctx, const float *d_in, float *d_out, int wi
  * smooth, scale, ratio.  Synchronous; host outputs. */
 om_status om_results_with_given_normal(om_scene *scene, om_result *result);
 
+/* Accumulation autotuner, no reference counterpart: traces the whole scene into maps with
+ * OM_ACC_PACKED and with OM_ACC_WINDOW (identical maps; a mode the scene cannot use is
+ * skipped, time -1), best of reps timed traces each after a warm-up, and keeps the faster
+ * mode for the scene's later traces.  ms[2] (may be NULL) receives the two times, best the
+ * mode kept.  Synchronous, on the null stream; the maps are overwritten. */
+om_status om_autotune_accumulate(om_scene *scene, const om_flux *maps, int reps, float *ms, int *best);
+
 /* Intercept ratio of a host flux map (ImageSaver::savetxt, image_save.cpp:26-47). */
 float om_intercept_ratio(const float *flux, int width, int height, float total_energy);
 

@@ -132,7 +132,8 @@ _EXPORTS = [
     "om_trace_plain", "om_trace_heliostats", "om_trace_records", "om_flux_finalize",
     "om_smooth_trimmed", "om_results_with_given_normal", "om_intercept_ratio", "om_math_eval",
     "om_batch_create", "om_batch_run", "om_batch_info", "om_batch_destroy", "om_trace_batched",
-    "om_autotune", "om_diff_mask_words", "om_diff_forward", "om_diff_backward", "om_diff_visibility",
+    "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
+    "om_diff_visibility",
 ]
 
 
@@ -162,6 +163,8 @@ def lib():
     L.om_scene_set_sun.argtypes = [_vp, _f32p, ctypes.c_float]
     L.om_scene_export.argtypes = [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]
     L.om_trace_plain.argtypes = [_vp, ctypes.POINTER(Flux), _vp]
+    L.om_autotune_accumulate.argtypes = [_vp, ctypes.POINTER(Flux), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]
     L.om_trace_heliostats.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Flux), _vp]
     L.om_trace_records.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                    ctypes.c_int64, _vp]
@@ -406,6 +409,20 @@ class SolarScene:
         _check(lib().om_flux_finalize(self._h, ctypes.byref(f), _stream_ptr(stream)),
                "om_flux_finalize")
         return maps
+
+    def autotune_accumulate(self, maps: Optional[FluxMaps] = None, reps: int = 3) -> dict:
+        """Time OM_ACC_PACKED and OM_ACC_WINDOW on this scene (identical maps) and keep the
+        faster for later traces (om_autotune_accumulate).  Times in ms, -1 for a mode the
+        scene cannot use."""
+        maps = maps or self.new_maps(smooth=False)
+        f = maps.struct(with_flux=False)
+        ms = (ctypes.c_float * 2)()
+        best = ctypes.c_int()
+        _check(lib().om_autotune_accumulate(self._h, ctypes.byref(f), int(reps), ms, ctypes.byref(best)),
+               "om_autotune_accumulate")
+        self.params["accumulate"] = best.value
+        return {"accumulate": {ACC_PACKED: "packed", ACC_WINDOW: "window"}[best.value],
+                "packed_ms": float(ms[0]), "window_ms": float(ms[1])}
 
     def ray_offset(self) -> np.ndarray:
         n = self.info["n_helio"]

@@ -1195,6 +1195,57 @@ extern "C" om_status om_trace_plain(om_scene *s, const om_flux *out, void *strea
     return OM_OK;
 }
 
+extern "C" om_status om_autotune_accumulate(om_scene *s, const om_flux *maps, int reps, float *ms, int *best)
+{
+    if (!s || reps <= 0) return OM_ERR_INVALID;
+    om_status st = check_flux(s, maps);
+    if (st != OM_OK) return st;
+    if (!s->ctx) {
+        set_error("host-only scene (created without a context) cannot be traced");
+        return OM_ERR_INVALID;
+    }
+    if (s->prm.accumulate == OM_ACC_DEVICE) {
+        set_error("the scene accumulates with OM_ACC_DEVICE");
+        return OM_ERR_INVALID;
+    }
+    OM_HIP(hipSetDevice(s->ctx->device));
+    const int modes[2] = {OM_ACC_PACKED, OM_ACC_WINDOW};
+    const int keep = s->prm.accumulate;
+    float t[2] = {-1.0f, -1.0f};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he != hipSuccess) st = hip_fail(he, "hipEventCreate");
+    for (int i = 0; i < 2 && st == OM_OK; ++i) {
+        s->prm.accumulate = modes[i];
+        // warm-up (buffers, first launch); a mode that cannot run on this scene is skipped
+        if (trace_into(s, 0, s->H, maps, true, nullptr) != OM_OK) continue;
+        for (int r = 0; r < reps && st == OM_OK; ++r) {
+            float ms1 = 0.0f;
+            if ((he = hipEventRecord(e0, nullptr)) != hipSuccess) break;
+            if ((st = trace_into(s, 0, s->H, maps, true, nullptr)) != OM_OK) break;
+            if ((he = hipEventRecord(e1, nullptr)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess ||
+                (he = hipEventElapsedTime(&ms1, e0, e1)) != hipSuccess)
+                break;
+            if (t[i] < 0.0f || ms1 < t[i]) t[i] = ms1;
+        }
+        if (st == OM_OK && he != hipSuccess) st = hip_fail(he, "accumulation autotune timing");
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st != OM_OK || t[0] < 0.0f) {
+        s->prm.accumulate = keep;
+        return st != OM_OK ? st : OM_ERR_HIP;
+    }
+    s->prm.accumulate = (t[1] >= 0.0f && t[1] < t[0]) ? OM_ACC_WINDOW : OM_ACC_PACKED;
+    if (ms) {
+        ms[0] = t[0];
+        ms[1] = t[1];
+    }
+    if (best) *best = s->prm.accumulate;
+    return OM_OK;
+}
+
 extern "C" om_status om_trace_records(om_scene *s, int h0, int h1, int32_t *shade, int32_t *block,
                                       int32_t *pixel, int64_t capacity, void *stream)
 {

@@ -272,6 +272,23 @@ def test_accumulation_modes_bitexact(om, mode, shift):
     assert int(orc["counts"].sum()) > 1_000_000
 
 
+def test_accumulation_autotune_keeps_the_maps(om):
+    """The accumulation autotuner times both atomic strategies on the scene and keeps the
+    faster; the maps after it are the oracle's, whichever it kept."""
+    import torch
+    d, p = F.config_mid()
+    d = F.subset(d, slice(2000, 2300))
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    r = g.autotune_accumulate(reps=2)
+    assert r["accumulate"] in ("packed", "window")
+    assert r["packed_ms"] > 0 and r["window_ms"] > 0
+    maps = g.trace_plain()
+    torch.cuda.synchronize()
+    orc = o.trace_f32()
+    assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32), orc["counts"])
+    assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64), orc["energy"])
+
+
 def test_device_atan2_bitexact(om):
     rng = np.random.default_rng(13)
     y = rng.normal(size=200000).astype(np.float32)
