@@ -33,7 +33,7 @@ struct DiffArgs {
     const float4 *theta;        // [H] (a_u, a_v, t_u, t_v)
     float4 *n_theta;            // [H] parametrised normals
     unsigned long long *mask;   // [tiles][rays per thread][WG/64] visibility ballots
-    unsigned long long *S;      // [8][npx] splat, fixed point 2^-32
+    unsigned long long *slices; // [G][Q][SPLAT_T^2] splat tiles of the workgroups, fixed point 2^-32
     unsigned long long *E;      // [8] energy, fixed point 2^-32
     const double *g;            // [npx] dL/dS
     float *gf;                  // [npx + 16] dL/dS in float (read 8 pixels at a time)
@@ -45,23 +45,17 @@ struct DiffArgs {
     v3 focus, e1, e2, sun;
     float kZ, kR, kEr;          // splat kernel 1/Z, radius, exp(-R^2/2)
     int res_x, res_y, npx, H;
-    // binned splat: records of the visible rays whose kernel window meets the map, then the
-    // same records copied once per map bin (BIN x BIN pixels) the window meets
+    // tiled splat: records (X, Y, e) of the visible rays whose kernel window meets the map;
+    // the map in ntx x nty tiles of SPLAT_T pixels, G workgroups per tile
     float *recX, *recY, *recE;  // [rec_cap]
-    float *bX, *bY, *bE;        // [4 rec_cap]
     unsigned int *n_rec;        // records written
-    unsigned int *bin_count;    // [NB]
-    unsigned int *bin_cursor;   // [NB]
-    unsigned int *bin_off;      // [NB+1] first entry of each bin
-    unsigned int *chunk_first;  // [NB+1] first chunk of each bin
-    int bin_edge, nbx, nby, NB;
+    int ntx, nty, Q, G;         // Q = ntx * nty
 };
 
 constexpr int DIFF_COPIES = 8;
 constexpr float DIFF_R = 3.0f;
-constexpr int BIN = 16;        // map bin edge (pixels); a kernel window (<= 7 px) meets <= 2x2 bins
-constexpr int MAX_BINS = 256;  // bins of a map (BIN grows for larger maps)
-constexpr int CHUNK = 2048;    // records of a bin one splat workgroup takes at a time
+constexpr int SPLAT_T = 128;     // splat tile edge: SPLAT_T^2 u64 words are 128 KiB of LDS
+constexpr int SPLAT_WGS = 1024;  // threads of a splat workgroup (one workgroup per CU)
 
 // pixels of the kernel window of (X, Y) inside the map; false when it misses the map
 __device__ __forceinline__ bool rec_window(const DiffArgs &p, float X, float Y, int &x0, int &x1, int &y0, int &y1)
@@ -73,16 +67,6 @@ __device__ __forceinline__ bool rec_window(const DiffArgs &p, float X, float Y, 
     return x0 <= x1 && y0 <= y1;
 }
 
-// the map bins (<= 4) a window meets; returns how many
-__device__ __forceinline__ int rec_bins(const DiffArgs &p, int bin_edge, int x0, int x1, int y0, int y1, int b[4])
-{
-    const int bx0 = x0 / bin_edge, bx1 = x1 / bin_edge, by0 = y0 / bin_edge, by1 = y1 / bin_edge;
-    int n = 0;
-    for (int by = by0; by <= by1; ++by)
-        for (int bx = bx0; bx <= bx1; ++bx) b[n++] = by * p.nbx + bx;
-    return n;
-}
-
 __device__ __forceinline__ float kern(float d, const DiffArgs &p)
 {
     return (exp_(-0.5f * d * d) - p.kEr * (1.0f + 0.5f * (p.kR * p.kR - d * d))) * p.kZ;
@@ -90,6 +74,14 @@ __device__ __forceinline__ float kern(float d, const DiffArgs &p)
 __device__ __forceinline__ float dkern(float d, const DiffArgs &p)
 {
     return -d * (exp_(-0.5f * d * d) - p.kEr) * p.kZ;
+}
+
+// K for the forward splat from the hardware base-2 exponential (within a few ulp of kern;
+// the splat is compared with the FP64 model to a tolerance, and its integer sums keep it
+// deterministic)
+__device__ __forceinline__ float kern_fast(float d, const DiffArgs &p)
+{
+    return (__builtin_amdgcn_exp2f(-0.72134752f * d * d) - p.kEr * (1.0f + 0.5f * (p.kR * p.kR - d * d))) * p.kZ;
 }
 
 // K and K' at d from one exponential
@@ -213,11 +205,8 @@ __device__ __forceinline__ DiffTile diff_tile(const TraceArgs &a, const DiffArgs
 template <int SUN, bool POISSON>
 __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs p)
 {
-    __shared__ unsigned int s_hist[MAX_BINS];
     __shared__ float s_rx[RPT_MAX * WG], s_ry[RPT_MAX * WG], s_re[RPT_MAX * WG];
     __shared__ unsigned int s_wt[WG / 64], s_base;
-    for (int i = threadIdx.x; i < p.NB; i += WG) s_hist[i] = 0u;
-    __syncthreads();
     int nrec = 0; // records of this thread, staged in LDS slots [nrec][thread]
     const int tile = xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
@@ -262,9 +251,6 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs 
             s_ry[nrec * WG + threadIdx.x] = o.Y;
             s_re[nrec * WG + threadIdx.x] = o.e;
             ++nrec;
-            int b[4];
-            const int nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
-            for (int i = 0; i < nb; ++i) atomicAdd(&s_hist[b[i]], 1u);
         }
     }
     if (esum) atomicAdd(p.E + (xcc_id() & 7u), esum);
@@ -289,131 +275,95 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs 
         p.recY[pos + i] = s_ry[i * WG + threadIdx.x];
         p.recE[pos + i] = s_re[i * WG + threadIdx.x];
     }
-    for (int i = threadIdx.x; i < p.NB; i += WG)
-        if (s_hist[i]) atomicAdd(&p.bin_count[i], s_hist[i]);
 }
 
-// bin offsets and chunk offsets from the counts; resets the counts and cursors
-__global__ void diff_scan_kernel(DiffArgs p)
+// e K(x - X) K(y - Y) of one record over the pixels [x0, x1] x [y0, y1] of its kernel
+// window inside the tile at (tx0, ty0), as 2^-32 fixed point into the LDS tile.  w = e K K
+// < 1 (K(0)^2 < 1, checked on the host), so w * 2^32 fits in 32 bits and its truncation is
+// the fixed-point value (the scaling by 2^32 is exact).
+__device__ __forceinline__ void splat_record(const DiffArgs &p, unsigned long long *s_tile, int tx0, int ty0,
+                                             float X, float Y, float e, int x0, int x1, int y0, int y1)
 {
-    if (threadIdx.x != 0) return;
-    unsigned int off = 0, ch = 0;
-    for (int b = 0; b < p.NB; ++b) {
-        const unsigned int c = p.bin_count[b];
-        p.bin_off[b] = off;
-        p.chunk_first[b] = ch;
-        off += c;
-        ch += (c + CHUNK - 1) / CHUNK;
-        p.bin_count[b] = 0u;
-        p.bin_cursor[b] = 0u;
+    float kx[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) kx[i] = kern_fast((float)(x0 + i) - X, p) * 4294967296.0f;
+    for (int yy = y0; yy <= y1; ++yy) {
+        const float ky = e * kern_fast((float)yy - Y, p);
+        unsigned long long *row = s_tile + (yy - ty0) * SPLAT_T + (x0 - tx0);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            const float w = ky * kx[i];
+            if (x0 + i <= x1 && w >= 1.0f) atomicAdd(row + i, (unsigned long long)(uint32_t)w);
+        }
     }
-    p.bin_off[p.NB] = off;
-    p.chunk_first[p.NB] = ch;
 }
 
-// records -> per-bin copies; positions within a bin come from one cursor atomic per bin per
-// workgroup pass (SCATTER_K records per thread) plus LDS counters (the order inside a bin
-// does not matter: the sums are integers)
-constexpr int SCATTER_K = 8;
-__global__ __launch_bounds__(WG) void diff_scatter_kernel(DiffArgs p)
+// Splat of the records: workgroup b takes tile b % Q of the map and every G-th block of
+// SPLAT_WGS records from block b / Q on (records come in heliostat order, and neighbouring
+// heliostats light the same part of the map: strided blocks give every workgroup of a tile
+// a like share of its work).  A wave queues the records of its lanes whose kernel window
+// meets the tile in LDS and splats them 64 at a time, one per lane, so that lanes do not idle
+// on the records that miss the tile.  Sums are 2^-32 fixed point in an LDS copy of the tile
+// (integer sums: any order gives the same map), stored to this workgroup's slice at the end.
+// The Q workgroups of one block run side by side, so a block is read from HBM about once.
+constexpr int SPLAT_QUEUE = 128; // queued records per wave (a power of two, >= 2 x 64)
+
+__global__ __launch_bounds__(SPLAT_WGS) void diff_splat_kernel(DiffArgs p)
 {
-    __shared__ unsigned int s_cnt[MAX_BINS], s_base[MAX_BINS];
-    __shared__ unsigned int s_loc[SCATTER_K * 4 * WG];  // index of the copy within its bin
-    __shared__ unsigned short s_bin[SCATTER_K * 4 * WG]; // bin of the copy, 0xffff = none
+    __shared__ unsigned long long s_tile[SPLAT_T * SPLAT_T];
+    __shared__ float s_q[SPLAT_WGS / 64][3][SPLAT_QUEUE];
+    const int t = (int)blockIdx.x % p.Q, g = (int)blockIdx.x / p.Q;
+    const int tx0 = (t % p.ntx) * SPLAT_T, ty0 = (t / p.ntx) * SPLAT_T;
+    const int tx1 = min(tx0 + SPLAT_T, p.res_x) - 1, ty1 = min(ty0 + SPLAT_T, p.res_y) - 1;
+    const int lane = threadIdx.x & 63;
+    float *qx = s_q[threadIdx.x >> 6][0], *qy = s_q[threadIdx.x >> 6][1], *qe = s_q[threadIdx.x >> 6][2];
+    for (int i = threadIdx.x; i < SPLAT_T * SPLAT_T; i += SPLAT_WGS) s_tile[i] = 0ull;
+    __syncthreads();
     const unsigned int n = *p.n_rec;
-    for (unsigned int base = blockIdx.x * WG * SCATTER_K; base < n; base += gridDim.x * WG * SCATTER_K) {
-        for (int i = threadIdx.x; i < p.NB; i += WG) s_cnt[i] = 0u;
-        __syncthreads();
-        for (int k = 0; k < SCATTER_K; ++k) {
-            const unsigned int j = base + k * WG + threadIdx.x;
-            int nb = 0, b[4];
-            if (j < n) {
-                int x0, x1, y0, y1;
-                rec_window(p, p.recX[j], p.recY[j], x0, x1, y0, y1);
-                nb = rec_bins(p, p.bin_edge, x0, x1, y0, y1, b);
-            }
-            for (int i = 0; i < 4; ++i) {
-                const int slot = (k * 4 + i) * WG + threadIdx.x;
-                if (i < nb) {
-                    s_bin[slot] = (unsigned short)b[i];
-                    s_loc[slot] = atomicAdd(&s_cnt[b[i]], 1u);
-                } else {
-                    s_bin[slot] = 0xffff;
-                }
-            }
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < p.NB; i += WG) s_base[i] = s_cnt[i] ? atomicAdd(&p.bin_cursor[i], s_cnt[i]) : 0u;
-        __syncthreads();
-        for (int k = 0; k < SCATTER_K; ++k) {
-            const unsigned int j = base + k * WG + threadIdx.x;
-            if (j >= n) continue;
-            const float X = p.recX[j], Y = p.recY[j], e = p.recE[j];
-            for (int i = 0; i < 4; ++i) {
-                const int slot = (k * 4 + i) * WG + threadIdx.x;
-                const unsigned int bb = s_bin[slot];
-                if (bb == 0xffff) break;
-                const unsigned int q = p.bin_off[bb] + s_base[bb] + s_loc[slot];
-                p.bX[q] = X;
-                p.bY[q] = Y;
-                p.bE[q] = e;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// one chunk of a bin at a time: the window pixels inside the bin accumulate in LDS (fixed
-// point), then each nonzero pixel is added once to this XCD's map copy
-__global__ __launch_bounds__(WG) void diff_splat_kernel(DiffArgs p)
-{
-    extern __shared__ unsigned long long s_tile[];
-    const int B = p.bin_edge;
-    const unsigned int nch = p.chunk_first[p.NB];
-    unsigned long long *S = p.S + (size_t)(xcc_id() & 7u) * p.npx;
-    for (unsigned int c = blockIdx.x; c < nch; c += gridDim.x) {
-        // bin of chunk c: the last bin whose first chunk is <= c
-        int lo = 0, hi = p.NB;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (p.chunk_first[mid] <= c) lo = mid;
-            else hi = mid;
-        }
-        const int bin = lo;
-        const unsigned int start = p.bin_off[bin] + (c - p.chunk_first[bin]) * CHUNK;
-        const unsigned int end = min(start + CHUNK, p.bin_off[bin + 1]);
-        const int tx0 = (bin % p.nbx) * B, ty0 = (bin / p.nbx) * B;
-        for (int i = threadIdx.x; i < B * B; i += WG) s_tile[i] = 0ull;
-        __syncthreads();
-        for (unsigned int j = start + threadIdx.x; j < end; j += WG) {
-            const float X = p.bX[j], Y = p.bY[j], e = p.bE[j];
+    unsigned int head = 0, tail = 0; // this wave's queue: entries [head, tail)
+    const unsigned int step = (unsigned)p.G * SPLAT_WGS;
+    for (unsigned int j0 = (unsigned)g * SPLAT_WGS; j0 < n; j0 += step) {
+        // the whole wave runs this loop: j0 is uniform
+        const unsigned int j = j0 + threadIdx.x;
+        bool in = false;
+        float X = 0.0f, Y = 0.0f, e = 0.0f;
+        if (j < n) {
+            X = p.recX[j];
+            Y = p.recY[j];
+            e = p.recE[j];
             int x0, x1, y0, y1;
             rec_window(p, X, Y, x0, x1, y0, y1);
-            x0 = max(x0, tx0);
-            x1 = min(x1, tx0 + B - 1);
-            y0 = max(y0, ty0);
-            y1 = min(y1, ty0 + B - 1);
-            float kx[7];
-#pragma unroll
-            for (int i = 0; i < 7; ++i) kx[i] = kern((float)(x0 + i) - X, p);
-            for (int yy = y0; yy <= y1; ++yy) {
-                const float ky = e * kern((float)yy - Y, p);
-#pragma unroll
-                for (int i = 0; i < 7; ++i) {
-                    const float w = ky * kx[i];
-                    if (x0 + i <= x1 && w > 0.0f)
-                        atomicAdd(&s_tile[(yy - ty0) * B + (x0 + i - tx0)], (unsigned long long)((double)w * 4294967296.0));
-                }
-            }
+            in = max(x0, tx0) <= min(x1, tx1) && max(y0, ty0) <= min(y1, ty1);
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < B * B; i += WG) {
-            const unsigned long long v = s_tile[i];
-            const int yy = ty0 + i / B, xx = tx0 + i % B;
-            if (v && yy < p.res_y && xx < p.res_x) atomicAdd(S + (size_t)yy * p.res_x + xx, v);
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const unsigned int k = (tail + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))) &
+                                   (SPLAT_QUEUE - 1);
+            qx[k] = X;
+            qy[k] = Y;
+            qe[k] = e;
         }
-        __syncthreads();
+        tail += (unsigned)__popcll(m);
+        if (tail - head >= 64u) {
+            const unsigned int k = (head + lane) & (SPLAT_QUEUE - 1);
+            const float Xq = qx[k], Yq = qy[k], eq = qe[k];
+            int x0, x1, y0, y1;
+            rec_window(p, Xq, Yq, x0, x1, y0, y1);
+            splat_record(p, s_tile, tx0, ty0, Xq, Yq, eq, max(x0, tx0), min(x1, tx1), max(y0, ty0), min(y1, ty1));
+            head += 64u;
+        }
     }
+    if (lane < (int)(tail - head)) {
+        const unsigned int k = (head + lane) & (SPLAT_QUEUE - 1);
+        const float Xq = qx[k], Yq = qy[k], eq = qe[k];
+        int x0, x1, y0, y1;
+        rec_window(p, Xq, Yq, x0, x1, y0, y1);
+        splat_record(p, s_tile, tx0, ty0, Xq, Yq, eq, max(x0, tx0), min(x1, tx1), max(y0, ty0), min(y1, ty1));
+    }
+    __syncthreads();
+    unsigned long long *out = p.slices + (size_t)blockIdx.x * (SPLAT_T * SPLAT_T);
+    for (int i = threadIdx.x; i < SPLAT_T * SPLAT_T; i += SPLAT_WGS) out[i] = s_tile[i];
 }
 
 __global__ void diff_gconv_kernel(DiffArgs p)
@@ -423,7 +373,7 @@ __global__ void diff_gconv_kernel(DiffArgs p)
     else if (i < p.npx + 16) p.gf[i] = 0.0f;
 }
 
-// copies -> double map (and energy); clears the copies
+// slices -> double map; energy copies -> double (and cleared)
 __global__ void diff_decode_kernel(DiffArgs p, double *S_out, double *E_out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,12 +386,11 @@ __global__ void diff_decode_kernel(DiffArgs p, double *S_out, double *E_out)
         *E_out = (double)e * 2.3283064365386963e-10;
     }
     if (i >= p.npx) return;
+    const int y = i / p.res_x, x = i - y * p.res_x;
+    const int t = (y / SPLAT_T) * p.ntx + x / SPLAT_T;
+    const unsigned long long *w = p.slices + (size_t)t * SPLAT_T * SPLAT_T + (y % SPLAT_T) * SPLAT_T + (x % SPLAT_T);
     unsigned long long s = 0;
-    for (int c = 0; c < DIFF_COPIES; ++c) {
-        unsigned long long &w = p.S[(size_t)c * p.npx + i];
-        s += w;
-        w = 0ull;
-    }
+    for (int g = 0; g < p.G; ++g) s += w[(size_t)g * p.Q * SPLAT_T * SPLAT_T];
     S_out[i] = (double)s * 2.3283064365386963e-10;
 }
 
@@ -620,29 +569,32 @@ __global__ void diff_vis_kernel(TraceArgs a, const unsigned long long *mask, int
 // host side
 // ---------------------------------------------------------------------------------------
 struct DiffLayout {
-    size_t S, E, n_rec, bins, n_theta, tile_first, tile_grad, gf, rec, bin_rec, bytes;
+    size_t E, n_rec, n_theta, tile_first, tile_grad, gf, slices, rec, bytes;
     size_t rec_cap;
+    int ntx, nty, Q, G;
 };
 
-// one allocation per scene: the cleared part (map copies, energy, counters, bin tables)
-// first, then the per-heliostat and per-tile arrays, the records and their bin copies
-static DiffLayout diff_layout(const om_scene *s)
+// one allocation per scene: the cleared part (energy copies, record counter) first, then the
+// per-heliostat and per-tile arrays, the splat slices and the records
+static DiffLayout diff_layout(const om_scene *s, int cus)
 {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     DiffLayout L;
     const size_t npx = (size_t)s->res_x * s->res_y;
     L.rec_cap = (size_t)s->total_rays;
-    L.S = 0;
-    L.E = al(L.S + sizeof(unsigned long long) * DIFF_COPIES * npx);
+    L.ntx = (s->res_x + SPLAT_T - 1) / SPLAT_T;
+    L.nty = (s->res_y + SPLAT_T - 1) / SPLAT_T;
+    L.Q = L.ntx * L.nty;
+    L.G = std::max(1, cus / L.Q); // one splat workgroup per CU
+    L.E = 0;
     L.n_rec = al(L.E + sizeof(unsigned long long) * DIFF_COPIES);
-    L.bins = al(L.n_rec + sizeof(unsigned int));
-    L.n_theta = al(L.bins + sizeof(unsigned int) * 4 * (MAX_BINS + 1));
+    L.n_theta = al(L.n_rec + sizeof(unsigned int));
     L.tile_first = al(L.n_theta + sizeof(float4) * (size_t)(s->H + 1));
     L.tile_grad = al(L.tile_first + sizeof(int) * (size_t)(s->H + 2));
     L.gf = al(L.tile_grad + sizeof(double) * 4 * (s->tiles.size() + 1));
-    L.rec = al(L.gf + sizeof(float) * (npx + 16));
-    L.bin_rec = al(L.rec + sizeof(float) * 3 * L.rec_cap);
-    L.bytes = al(L.bin_rec + sizeof(float) * 3 * 4 * L.rec_cap);
+    L.slices = al(L.gf + sizeof(float) * (npx + 16));
+    L.rec = al(L.slices + sizeof(unsigned long long) * SPLAT_T * SPLAT_T * (size_t)L.Q * L.G);
+    L.bytes = al(L.rec + sizeof(float) * 3 * L.rec_cap);
     return L;
 }
 
@@ -662,12 +614,14 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
         return OM_ERR_UNSUPPORTED;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
-    const DiffLayout L = diff_layout(s);
+    int cus = 0;
+    OM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->ctx->device));
+    const DiffLayout L = diff_layout(s, cus);
     if (s->diff_buf.bytes < L.bytes) {
         om_status rc = ensure_buf(s->diff_buf, L.bytes);
         if (rc != OM_OK) return rc;
         char *b = (char *)s->diff_buf.p;
-        // the decode leaves the copies cleared; clear once here
+        // the decode leaves the energy copies cleared; clear once here
         if ((rc = launch_fill_u32((uint32_t *)b, (long long)(L.n_theta / 4), st)) != OM_OK) return rc;
         std::vector<int> first(s->H + 1, 0);
         size_t t = 0;
@@ -679,32 +633,21 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
     }
     char *b = (char *)s->diff_buf.p;
     std::memset(&p, 0, sizeof(p));
-    p.S = (unsigned long long *)(b + L.S);
     p.E = (unsigned long long *)(b + L.E);
+    p.slices = (unsigned long long *)(b + L.slices);
     p.n_theta = (float4 *)(b + L.n_theta);
     p.tile_first = (const int *)(b + L.tile_first);
     p.tile_grad = (double *)(b + L.tile_grad);
     p.n_rec = (unsigned int *)(b + L.n_rec);
     p.gf = (float *)(b + L.gf);
-    unsigned int *bins = (unsigned int *)(b + L.bins);
-    p.bin_count = bins;
-    p.bin_cursor = bins + (MAX_BINS + 1);
-    p.bin_off = bins + 2 * (MAX_BINS + 1);
-    p.chunk_first = bins + 3 * (MAX_BINS + 1);
     float *rec = (float *)(b + L.rec);
     p.recX = rec;
     p.recY = rec + L.rec_cap;
     p.recE = rec + 2 * L.rec_cap;
-    float *brec = (float *)(b + L.bin_rec);
-    p.bX = brec;
-    p.bY = brec + 4 * L.rec_cap;
-    p.bE = brec + 8 * L.rec_cap;
-    // bins of bin_edge pixels (a multiple of 16), at most MAX_BINS of them
-    const int mres = std::max(s->res_x, s->res_y);
-    p.bin_edge = BIN * ((mres + BIN * 16 - 1) / (BIN * 16));
-    p.nbx = (s->res_x + p.bin_edge - 1) / p.bin_edge;
-    p.nby = (s->res_y + p.bin_edge - 1) / p.bin_edge;
-    p.NB = p.nbx * p.nby;
+    p.ntx = L.ntx;
+    p.nty = L.nty;
+    p.Q = L.Q;
+    p.G = L.G;
     p.A = s->rv[0];
     p.E1 = s->rv[1] - s->rv[0];
     p.E2 = s->rv[3] - s->rv[0];
@@ -718,6 +661,12 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
     const double R = DIFF_R, eR = std::exp(-0.5 * R * R);
     const double Z = std::sqrt(2.0 * 3.14159265358979323846) * std::erf(R / std::sqrt(2.0)) - eR * (2.0 * R + 2.0 * R * R * R / 3.0);
     p.kZ = (float)(1.0 / Z);
+    // the splat's 32-bit fixed-point conversion needs e K(0)^2 < 1 (e <= 1)
+    const double K0 = (1.0 - eR * (1.0 + 0.5 * R * R)) / Z;
+    if (!(K0 * K0 < 0.5)) {
+        set_error("splat kernel peak too large for the fixed-point splat");
+        return OM_ERR_UNSUPPORTED;
+    }
     p.kR = (float)R;
     p.kEr = (float)eR;
     p.res_x = s->res_x;
@@ -772,10 +721,7 @@ extern "C" om_status om_diff_forward(om_scene *s, const float *d_params, uint64_
     OM_HIP(hipGetLastError());
     if ((rc = launch_fill_u32(p.n_rec, 1, st)) != OM_OK) return rc;
     if ((rc = launch_diff<true>(s, a, p, st)) != OM_OK) return rc;
-    hipLaunchKernelGGL(diff_scan_kernel, dim3(1), dim3(64), 0, st, p);
-    hipLaunchKernelGGL(diff_scatter_kernel, dim3(2048), dim3(WG), 0, st, p);
-    hipLaunchKernelGGL(diff_splat_kernel, dim3(4096), dim3(WG), sizeof(unsigned long long) * p.bin_edge * p.bin_edge,
-                       st, p);
+    hipLaunchKernelGGL(diff_splat_kernel, dim3((unsigned)(p.Q * p.G)), dim3(SPLAT_WGS), 0, st, p);
     OM_HIP(hipGetLastError());
     hipLaunchKernelGGL(diff_decode_kernel, dim3((p.npx + WG - 1) / WG), dim3(WG), 0, st, p, d_splat, d_energy);
     OM_HIP(hipGetLastError());
