@@ -57,37 +57,41 @@ constexpr float DIFF_R = 3.0f;
 constexpr int SPLAT_T = 128;     // splat tile edge: SPLAT_T^2 u64 words are 128 KiB of LDS
 constexpr int SPLAT_WGS = 1024;  // threads of a splat workgroup (one workgroup per CU)
 
-// pixels of the kernel window of (X, Y) inside the map; false when it misses the map
+// pixels of the kernel window of (X, Y) inside the map; false when it misses the map, and
+// then the window is empty on both axes (x1 < x0 and y1 < y0), so that a loop over its rows
+// reads nothing.  A point 10^6 pixels or more off (maps are at most 1024 pixels a side) or
+// not finite misses it, which also keeps the integer conversions in range.
 __device__ __forceinline__ bool rec_window(const DiffArgs &p, float X, float Y, int &x0, int &x1, int &y0, int &y1)
 {
-    x0 = max(0, trunc_i(__builtin_ceilf(X - DIFF_R)));
-    x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(X + DIFF_R)));
-    y0 = max(0, trunc_i(__builtin_ceilf(Y - DIFF_R)));
-    y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(Y + DIFF_R)));
-    return x0 <= x1 && y0 <= y1;
+    bool in = fabs_(X) < 1.0e6f && fabs_(Y) < 1.0e6f;
+    if (in) {
+        x0 = max(0, trunc_i(__builtin_ceilf(X - DIFF_R)));
+        x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(X + DIFF_R)));
+        y0 = max(0, trunc_i(__builtin_ceilf(Y - DIFF_R)));
+        y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(Y + DIFF_R)));
+        in = x0 <= x1 && y0 <= y1;
+    }
+    if (!in) {
+        x0 = y0 = 0;
+        x1 = y1 = -1;
+    }
+    return in;
 }
 
-__device__ __forceinline__ float kern(float d, const DiffArgs &p)
-{
-    return (exp_(-0.5f * d * d) - p.kEr * (1.0f + 0.5f * (p.kR * p.kR - d * d))) * p.kZ;
-}
-__device__ __forceinline__ float dkern(float d, const DiffArgs &p)
-{
-    return -d * (exp_(-0.5f * d * d) - p.kEr) * p.kZ;
-}
-
-// K for the forward splat from the hardware base-2 exponential (within a few ulp of kern;
-// the splat is compared with the FP64 model to a tolerance, and its integer sums keep it
+// K(d) = (exp(-d^2/2) - eR (1 + (R^2 - d^2)/2)) / Z for the forward splat, from the
+// hardware base-2 exponential (exp(-d^2/2) = 2^(-d^2 / (2 ln 2)), within a few ulp; the
+// splat is compared with the FP64 model to a tolerance, and its integer sums keep it
 // deterministic)
 __device__ __forceinline__ float kern_fast(float d, const DiffArgs &p)
 {
     return (__builtin_amdgcn_exp2f(-0.72134752f * d * d) - p.kEr * (1.0f + 0.5f * (p.kR * p.kR - d * d))) * p.kZ;
 }
 
-// K and K' at d from one exponential
+// K and K' at d from one exponential (the hardware base-2 exponential: the adjoint is
+// compared with FP64 central differences to a tolerance)
 __device__ __forceinline__ void kern2(float d, const DiffArgs &p, float &k, float &dk)
 {
-    const float ex = exp_(-0.5f * d * d);
+    const float ex = __builtin_amdgcn_exp2f(-0.72134752f * d * d);
     k = (ex - p.kEr * (1.0f + 0.5f * (p.kR * p.kR - d * d))) * p.kZ;
     dk = -d * (ex - p.kEr) * p.kZ;
 }
@@ -436,18 +440,17 @@ __global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs a, DiffArgs
         const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
         DiffRay o;
         if (!diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o)) continue; // (not reached)
-        // gather over the kernel window
-        const int x0 = max(0, trunc_i(__builtin_ceilf(o.X - DIFF_R)));
-        const int x1 = min(p.res_x - 1, trunc_i(__builtin_floorf(o.X + DIFF_R)));
-        const int y0 = max(0, trunc_i(__builtin_ceilf(o.Y - DIFF_R)));
-        const int y1 = min(p.res_y - 1, trunc_i(__builtin_floorf(o.Y + DIFF_R)));
+        // gather over the kernel window (none when the window misses the map: then only the
+        // energy term remains)
+        int x0, x1, y0, y1;
+        const bool win = rec_window(p, o.X, o.Y, x0, x1, y0, y1);
         float kx[7], dkx[7];
 #pragma unroll
         for (int i = 0; i < 7; ++i) kern2((float)(x0 + i) - o.X, p, kx[i], dkx[i]);
         // window pixels outside [x0, x1] carry zero weight
 #pragma unroll
         for (int i = 0; i < 7; ++i)
-            if (x0 + i > x1) kx[i] = dkx[i] = 0.0f;
+            if (!win || x0 + i > x1) kx[i] = dkx[i] = 0.0f;
         float sA = 0.0f, sX = 0.0f, sY = 0.0f;
         for (int yy = y0; yy <= y1; ++yy) {
             float ky, kdy;
@@ -629,7 +632,9 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
             while (t < s->tiles.size() && s->tiles[t].x < h) ++t;
             first[h] = (int)t;
         }
+        // the host vector ends with this block: wait for the copy to have read it
         OM_HIP(hipMemcpyAsync(b + L.tile_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, st));
+        OM_HIP(hipStreamSynchronize(st));
     }
     char *b = (char *)s->diff_buf.p;
     std::memset(&p, 0, sizeof(p));

@@ -42,6 +42,7 @@ def _fd_check(o, D, scene, th, heliostats):
     L = D.spill_peak_loss(S, E, tau, LAM)
     L.backward()
     g = p.grad.cpu().numpy().astype(np.float64)
+    assert np.isfinite(g).all(), np.where(~np.isfinite(g).all(1))[0][:8]
     _, _, mask = D.forward(scene, p)
     vis = D.visibility(scene, mask)
     off = o.micro()["ray_offset"]
