@@ -49,7 +49,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large", "batched", "diff"])
+    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large", "batched", "diff", "annual"])
     ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
     ap.add_argument("--no-grid-tune", action="store_true",
                     help="batched: keep 128^3 grids instead of tuning the resolution")
@@ -68,8 +68,8 @@ def main() -> int:
               "(every GPU box here has one GPU)", file=sys.stderr)
         return 2
 
-    if args.workload in ("batched", "diff"):
-        out = bench_batched(args) if args.workload == "batched" else bench_diff(args)
+    if args.workload in ("batched", "diff", "annual"):
+        out = {"batched": bench_batched, "diff": bench_diff, "annual": bench_annual}[args.workload](args)
         print(json.dumps(out), flush=True)
         return 0
 
@@ -213,11 +213,17 @@ def main() -> int:
         sub.steps, sub.warmup = 3, 1
         b = bench_batched(sub, with_cpu=False)
         d = bench_diff(sub)
+        sub_a = argparse.Namespace(**vars(args))
+        sub_a.steps, sub_a.warmup = 1, 1
+        an = bench_annual(sub_a)
         out["modes"] = {
             "batched": {"value": b["value"], "unit": "rays/s", "workload": b["config"]["workload"],
                         **{k: b["config"][k] for k in ("scenes", "rays", "batch_ms", "plain_ms",
                                                        "plain_rays_per_s", "batched_vs_plain",
                                                        "grid_cells", "workgroup", "rays_per_thread")}},
+            "annual": {"value": an["value"], "unit": "rays/s", "workload": an["config"]["workload"],
+                       **{k: an["config"][k] for k in ("time_steps", "rays", "mode", "batch_ms", "plain_ms",
+                                                        "time_steps_per_s", "annual_ratio")}},
             "fwd_bwd": {"value": d["config"]["fwd_bwd_per_s"], "unit": "fwd+bwd/s",
                         "rays_per_s": d["value"], "workload": d["config"]["workload"],
                         **{k: d["config"][k] for k in ("fwd_bwd_ms", "forward_ms", "backward_ms",
@@ -225,6 +231,61 @@ def main() -> int:
         }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_annual(args) -> dict:
+    """Annual driver (main.cpp:56-116): the mid field at every time step of the reference's
+    schedule with the sun above 15 degrees, each step a fork of the field, traced by the
+    batched launch and by one plain (accumulation-autotuned) trace per step; the faster is
+    the value."""
+    import torch
+
+    import onemirror_amd as om
+    from onemirror_amd import annual as A
+    from onemirror_amd import fields as F
+
+    d, p = F.config_mid()
+    t0 = time.time()
+    root = om.SolarScene(d, p)
+    acc = root.autotune_accumulate(reps=2)
+    steps = A.annual_steps()
+    runs = {m: A.AnnualRun(root, steps, batch_size=64, mode=m) for m in ("batch", "plain")}
+    R = runs["batch"].rays
+    print(f"# annual: {len(steps)} time steps (zenith <= {A.ZENITH_MAX}), {R} rays, forks built in "
+          f"{time.time() - t0:.1f}s; accumulation {acc['accumulate']}", file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream()
+    ms = {}
+    for m, run in runs.items():
+        for _ in range(max(1, args.warmup)):
+            run.trace()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            run.trace()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms[m] = e0.elapsed_time(e1) / args.steps
+        print(f"# annual {m}: {ms[m]:.1f} ms per year ({R / (ms[m] * 1e-3):.4g} rays/s)", file=sys.stderr, flush=True)
+    best = min(ms, key=ms.get)
+    res = runs[best].results()
+    print(f"# annual intercept ratio {res['annual_ratio']:.4f}, intercepted {res['intercepted_w_sum'] / 1e6:.1f} "
+          f"MW summed over the steps", file=sys.stderr, flush=True)
+    return {
+        "metric": METRIC, "value": R / (ms[best] * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms[best], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": ("annual: the mid field at the reference schedule (months 1-12, days 1-28 "
+                                "every 3, hours 6-18 every 2) at 30.3N 120.2E UTC+8, 40 m, zenith <= 75 deg "
+                                "(approximate solar positions), clear-sky DNI of getDNI; one step = the "
+                                "whole year"),
+                   "time_steps": len(steps), "rays": R, "mode": best,
+                   "batch_ms": ms["batch"], "plain_ms": ms["plain"],
+                   "time_steps_per_s": len(steps) / (ms[best] * 1e-3),
+                   "accumulate": acc["accumulate"], "annual_ratio": res["annual_ratio"],
+                   "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
 
 
 def oracle_sample(configs, blocks_per_scene: int, blk: int = 40):

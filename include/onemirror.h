@@ -158,6 +158,24 @@ om_status om_scene_get_info(const om_scene *scene, om_scene_info *info);
  * (set_sunray_content + set_helio_content, scene_instance_process.cpp:21-28). */
 om_status om_scene_set_sun(om_scene *scene, const float sun_dir[3], float dni);
 
+/* ---- Annual time steps (main.cpp:56-116 builds a new scene for every sun position of the
+ * year on the same field).  A fork is that scene for another sun direction, DNI and Philox
+ * stream (scene_index): it shares the source's grid, microsamples, tiles and Buie tables on
+ * the device and owns its re-aimed heliostats, so that hundreds of time steps of one field
+ * fit side by side (for a batch).  The source (or the root it was forked from) must outlive
+ * its forks.  The microsamples are the root's (Poisson points drawn with the root's scene
+ * index); the rays use the fork's scene_index.  With the root's scene index, tracing a fork
+ * gives what a scene built from scratch with that sun gives. */
+om_status om_scene_fork(const om_scene *src, const float sun_dir[3], float dni, uint32_t scene_index,
+                        om_scene **out);
+/* Light travel direction of the sun at zenith and azimuth (degrees, azimuth clockwise from
+ * north) as main.cpp:71-73 sets sun_dir: (-sin z sin az, -cos z, sin z cos az). */
+om_status om_sun_direction(float zenith_deg, float azimuth_deg, float out[3]);
+/* Clear-sky DNI of getDNI (SPA.cpp:1174-1189), W/m^2: (1353 + 45 cos(2 pi (n + 10) / 365))
+ * (a0 + a1 exp(-k / cos z)), n the day of a 365-day year from month (1-12) and day, and
+ * Hottel's a0, a1, k for a site at elevation_m.  NaN for a month out of range. */
+float om_dni_hottel(float zenith_deg, int month, int day, float elevation_m);
+
 /* Copy scene arrays to host buffers for inspection (parity tests).  what:
  *  "helio_normal" [n][3] f32   "helio_v0" / "helio_v1" / "helio_v3" [n][3] f32
  *  "grid_start" [cells+1] i32  "grid_items" [nnz] i32   "receiver" [4][3] f32 + focus[3]

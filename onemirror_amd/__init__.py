@@ -133,7 +133,7 @@ _EXPORTS = [
     "om_smooth_trimmed", "om_results_with_given_normal", "om_intercept_ratio", "om_math_eval",
     "om_batch_create", "om_batch_run", "om_batch_info", "om_batch_destroy", "om_trace_batched",
     "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
-    "om_diff_visibility",
+    "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel",
 ]
 
 
@@ -161,6 +161,10 @@ def lib():
     L.om_scene_destroy.argtypes = [_vp]
     L.om_scene_get_info.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
     L.om_scene_set_sun.argtypes = [_vp, _f32p, ctypes.c_float]
+    L.om_scene_fork.argtypes = [_vp, _f32p, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(_vp)]
+    L.om_sun_direction.argtypes = [ctypes.c_float, ctypes.c_float, _f32p]
+    L.om_dni_hottel.argtypes = [ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    L.om_dni_hottel.restype = ctypes.c_float
     L.om_scene_export.argtypes = [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]
     L.om_trace_plain.argtypes = [_vp, ctypes.POINTER(Flux), _vp]
     L.om_autotune_accumulate.argtypes = [_vp, ctypes.POINTER(Flux), ctypes.c_int,
@@ -191,7 +195,7 @@ def lib():
     L.om_diff_visibility.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp]
     for name in _EXPORTS:
         if name not in ("om_status_string", "om_last_error", "om_default_params",
-                        "om_scene_desc_free", "om_intercept_ratio"):
+                        "om_scene_desc_free", "om_intercept_ratio", "om_dni_hottel"):
             getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -381,6 +385,21 @@ class SolarScene:
         _check(lib().om_scene_set_sun(self._h, _f3(sun_dir), float(dni)), "om_scene_set_sun")
         self._refresh()
 
+    def fork(self, sun_dir, dni: float, scene_index: Optional[int] = None) -> "SolarScene":
+        """This field under another sun (om_scene_fork): shares the grid, microsamples and
+        tiles on the device, re-aims the heliostats.  The fork keeps its root alive."""
+        h = _vp()
+        idx = self.params.get("scene_index", 0) if scene_index is None else int(scene_index)
+        _check(lib().om_scene_fork(self._h, _f3(sun_dir), float(dni), idx, ctypes.byref(h)), "om_scene_fork")
+        f = SolarScene.__new__(SolarScene)
+        f.ctx = self.ctx
+        f.desc = dict(self.desc, sun_dir=np.asarray(sun_dir, np.float32), dni=np.float32(dni))
+        f.params = dict(self.params, scene_index=idx)
+        f.root = getattr(self, "root", None) or self
+        f._h = h
+        f._refresh()
+        return f
+
     def export(self, what: str, nbytes: int, dtype) -> np.ndarray:
         buf = np.zeros(max(nbytes, 1), np.uint8)
         _check(lib().om_scene_export(self._h, what.encode(), _vp(buf.ctypes.data), buf.nbytes),
@@ -479,6 +498,19 @@ class ImageSmoother:
                                        float(trimmed_ratio), _stream_ptr(stream)),
                "om_smooth_trimmed")
         return dst
+
+
+def sun_direction(zenith_deg: float, azimuth_deg: float) -> np.ndarray:
+    """Light travel direction of the sun (main.cpp:71-73), om_sun_direction."""
+    out = np.zeros(3, np.float32)
+    _check(lib().om_sun_direction(float(zenith_deg), float(azimuth_deg), out.ctypes.data_as(_f32p)),
+           "om_sun_direction")
+    return out
+
+
+def dni_hottel(zenith_deg: float, month: int, day: int, elevation_m: float) -> float:
+    """Clear-sky DNI of getDNI (SPA.cpp:1174-1189), om_dni_hottel, W/m^2."""
+    return float(lib().om_dni_hottel(float(zenith_deg), int(month), int(day), float(elevation_m)))
 
 
 def intercept_ratio(flux: np.ndarray, total_energy: float) -> float:

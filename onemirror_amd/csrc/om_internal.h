@@ -176,6 +176,9 @@ struct om_scene {
     // normals and tile ranges, per-tile adjoint partial sums
     om::DevBuf diff_buf;
     int rays_per_thread = 8;
+    // forks (om_scene_fork): the root scene whose grid, microsamples, tiles and Buie tables
+    // this scene's device pointers share, and whose host CSR it reads; null for a root
+    const om_scene *grid_src = nullptr;
 };
 
 namespace om {

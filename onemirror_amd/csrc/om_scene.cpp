@@ -696,6 +696,10 @@ void free_scene_dev(om_scene *s)
 {
     dfree(s->d_helio);
     dfree(s->d_geo);
+    if (s->diff_buf.p) (void)hipFree(s->diff_buf.p);
+    s->diff_buf.p = nullptr;
+    s->diff_buf.bytes = 0;
+    if (s->grid_src) return; // the rest belongs to the root scene
     dfree(s->d_cell_start);
     dfree(s->d_cell_items);
     dfree(s->d_buie_bins);
@@ -703,9 +707,6 @@ void free_scene_dev(om_scene *s)
     dfree(s->d_poisson);
     dfree(s->d_mcount);
     dfree(s->d_tiles);
-    if (s->diff_buf.p) (void)hipFree(s->diff_buf.p);
-    s->diff_buf.p = nullptr;
-    s->diff_buf.bytes = 0;
 }
 
 // Coordinates and sizes the tracer accepts: finite and within 2^20 m; cell edges within
@@ -1077,6 +1078,54 @@ extern "C" om_status om_scene_set_sun(om_scene *s, const float sun_dir[3], float
     return upload_geometry(s);
 }
 
+// One time step of an annual run on the same field (main.cpp:56-116 builds a new scene per
+// step): the source's device grid, microsamples, tiles and Buie tables are shared, the
+// heliostats are re-aimed for the new sun and get their own records and vertices.
+extern "C" om_status om_scene_fork(const om_scene *src, const float sun_dir[3], float dni, uint32_t scene_index,
+                                   om_scene **out)
+{
+    if (!src || !sun_dir || !out) return OM_ERR_INVALID;
+    *out = nullptr;
+    if (!src->ctx) {
+        om::set_error("host-only scene (created without a context) cannot be forked");
+        return OM_ERR_INVALID;
+    }
+    if (!bounded(sun_dir, 3) || (sun_dir[0] == 0.0f && sun_dir[1] == 0.0f && sun_dir[2] == 0.0f)) {
+        om::set_error("sun direction must be finite and nonzero");
+        return OM_ERR_INVALID;
+    }
+    // copy everything but the host CSR, which stays with the root
+    om_scene &m = const_cast<om_scene &>(*src);
+    std::vector<int> cs, ci;
+    cs.swap(m.cell_start);
+    ci.swap(m.cell_items);
+    om_scene *s = new om_scene(*src);
+    m.cell_start.swap(cs);
+    m.cell_items.swap(ci);
+    s->grid_src = src->grid_src ? src->grid_src : src;
+    s->d_helio = nullptr;
+    s->d_geo = nullptr;
+    s->diff_buf = om::DevBuf{};
+    s->prm.scene_index = scene_index;
+    std::memcpy(s->sun_in, sun_dir, 12);
+    s->sun = om::normalize(ld3(s->sun_in));
+    aim_heliostats(s);
+    om_status st = check_in_grid(s);
+    if (st == OM_OK) {
+        s->dni = dni;
+        flux_scale(s);
+        if (hipSetDevice(s->ctx->device) != hipSuccess) st = om::hip_fail(hipGetLastError(), "hipSetDevice");
+        else st = upload_geometry(s);
+    }
+    if (st != OM_OK) {
+        free_scene_dev(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return OM_OK;
+}
+
 extern "C" om_status om_scene_export(const om_scene *s, const char *what, void *host, size_t bytes)
 {
     if (!s || !what || !host) return OM_ERR_INVALID;
@@ -1090,8 +1139,13 @@ extern "C" om_status om_scene_export(const om_scene *s, const char *what, void *
     else if (w == "helio_v0") put(s->v0.data(), 12 * (size_t)s->H);
     else if (w == "helio_v1") put(s->v1.data(), 12 * (size_t)s->H);
     else if (w == "helio_v3") put(s->v3.data(), 12 * (size_t)s->H);
-    else if (w == "grid_start") put(s->cell_start.data(), 4 * s->cell_start.size());
-    else if (w == "grid_items") put(s->cell_items.data(), 4 * (size_t)s->nnz);
+    else if (w == "grid_start") {
+        const om_scene *g = s->grid_src ? s->grid_src : s;
+        put(g->cell_start.data(), 4 * g->cell_start.size());
+    } else if (w == "grid_items") {
+        const om_scene *g = s->grid_src ? s->grid_src : s;
+        put(g->cell_items.data(), 4 * (size_t)g->nnz);
+    }
     else if (w == "receiver") {
         put(s->rv, 48);
         put(&s->focus, 12);
