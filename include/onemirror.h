@@ -38,15 +38,32 @@ typedef enum {
 typedef struct om_ctx om_ctx;
 typedef struct om_scene om_scene;
 
+/* One grid of a field (RectGrid pos_/size_/interval_, grid.h:25-49; the Grid block of a
+ * .scn file, scene_file_proc.cpp:110-138,185-219) and the heliostats it holds,
+ * [first_helio, first_helio + n_helio) of the scene's list (start_helio_pos_, num_helios_). */
+typedef struct {
+    float pos[3];
+    float size[3];
+    float interval[3];
+    int   first_helio;
+    int   n_helio;
+} om_grid_desc;
+
 /* Scene description: what a .scn file carries (scene_file_proc.cpp:41-286) plus the sun
- * of SunRay (sunray.h:4-60).  World frame x = east, y = up, z = south (main.cpp:71-73). */
+ * of SunRay (sunray.h:4-60).  World frame x = east, y = up, z = south (main.cpp:71-73).
+ * Grids: with n_grids == 0, grid_pos/size/interval describe the one grid, which holds
+ * every heliostat; with n_grids > 0, grids[k] hold consecutive heliostat ranges that cover
+ * [0, n_helio) in order (a .scn file lists each grid's heliostats after its block) and
+ * grid_pos/size/interval are not read.  Which grids a scene traces: om_params.grid_select. */
 typedef struct {
     int          n_helio;
     const float *helio_pos;    /* [n][3] heliostat centre                                 */
     const float *helio_size;   /* [n][3] width (x), thickness (y), height (z)             */
     const float *helio_gap;    /* [n][2] gap between facets                               */
     const int   *helio_matrix; /* [n][2] facet rows, facet columns                        */
-    int   recv_type;           /* 0 rectangle (receiver.cuh:66-101), 1 cylinder (new)     */
+    int   recv_type;           /* 0 rectangle (receiver.cuh:66-101), 1 cylinder (new);
+                                  2 (the truncated cone stub, receiver.cuh:125-140) parses
+                                  but cannot be traced                                    */
     float recv_pos[3];         /* rectangle centre; cylinder: axis point at mid-height    */
     float recv_size[3];        /* rectangle box size; cylinder (radius, height, unused)    */
     float recv_normal[3];
@@ -59,6 +76,10 @@ typedef struct {
     float dni;                 /* W/m^2                                                   */
     int   recv_res_x, recv_res_y; /* cylinder map columns (azimuth) and rows (height);
                                      0 = from recv_pixel_length                           */
+    int   n_grids;             /* 0: the grid_* fields above; else the grids array        */
+    const om_grid_desc *grids; /* [n_grids]                                               */
+    float ground_length, ground_width; /* "ground" line (SolarScene ground_length_/width_;
+                                          not used in tracing)                            */
 } om_scene_desc;
 
 enum { OM_SUN_PILLBOX = 0, OM_SUN_BUIE = 1 };
@@ -83,7 +104,13 @@ typedef struct {
     float smooth_ratio;        /* trimmed fraction (Gtest.cu:51)                          */
     int   accumulate;          /* OM_ACC_*: how hits reach the maps (results identical)   */
     int   pack_shift;          /* OM_ACC_PACKED: energy-field bits E (32..62, 0 = 44)     */
+    int   grid_select;         /* k >= 0: the scene is grid k and its heliostats alone, as
+                                  the reference traces grid 0 (Gtest.cu:12,21-23; default
+                                  0); OM_GRIDS_ALL: every grid, each heliostat's rays
+                                  walking the grid that holds it                          */
 } om_params;
+
+enum { OM_GRIDS_ALL = -1 };
 
 /* OM_ACC_PACKED: one 64-bit atomic per hit into a per-XCD word holding (count << E) +
  * energy; carries out of the energy field and wraps of the word are detected from the
@@ -114,6 +141,7 @@ typedef struct {
     float   pixel_length;      /* receiver pixel edge                                     */
     float   flux_scale;        /* DNI * Ssub * rho / Nc / Srec (Gtest.cu:37-46)           */
     float   total_energy;      /* sum |n.s| DNI A rho / Srec (Gtest.cu:29,44)             */
+    int     n_grids;           /* grids traced; grid_num is grid 0's, cells and nnz sums  */
 } om_scene_info;
 
 /* Host-side result of the full driver. */
@@ -141,9 +169,21 @@ om_status om_ctx_destroy(om_ctx *ctx);
 om_status om_scene_from_scn(om_ctx *ctx, const char *path, const float sun_dir[3], float dni,
                             const om_params *params, om_scene **out);
 
-/* Parse a .scn file into a description whose arrays live until om_scene_desc_free. */
+/* Parse a .scn file into a description whose arrays live until om_scene_desc_free.  Every
+ * Grid block becomes an entry of desc->grids (grid 0 is also copied to grid_pos/size/
+ * interval); the first receiver is kept (the reference traces receivers[0], Gtest.cu:13).
+ * Where the reference only prints a warning and goes on into undefined behaviour, parsing
+ * fails with OM_ERR_PARSE: a grid whose "n" differs from the heliostats listed under it
+ * (scene_file_proc.cpp:112-115,250-252), heliostat type other than 0 (:253), a receiver type
+ * other than 0..2 (:96-101) or a grid type other than 0 (:123-126, :220-222). */
 om_status om_scn_parse(const char *path, om_scene_desc *desc);
 void      om_scene_desc_free(om_scene_desc *desc);
+
+/* Write a description as a .scn file that SceneFileProc::SceneFileRead reads back to the
+ * same description (numbers with 9 significant digits, which round-trip every float): the
+ * ground, receiver and grid blocks, and each grid's heliostats with "gap" and "matrix"
+ * lines where they change.  No reference counterpart (the reference only reads scenes). */
+om_status om_scn_write(const char *path, const om_scene_desc *desc);
 
 /* Scene from arrays (generated fields) + InitContent (solar_scene.cpp:83-99).  With
  * ctx == NULL the scene is built on the host only (aiming, grid, receiver, sun tables) for
@@ -178,7 +218,9 @@ float om_dni_hottel(float zenith_deg, int month, int day, float elevation_m);
 
 /* Copy scene arrays to host buffers for inspection (parity tests).  what:
  *  "helio_normal" [n][3] f32   "helio_v0" / "helio_v1" / "helio_v3" [n][3] f32
- *  "grid_start" [cells+1] i32  "grid_items" [nnz] i32   "receiver" [4][3] f32 + focus[3]
+ *  "grid_start" [cells+1] i32  "grid_items" [nnz] i32 (grid 0; "grid_start.K",
+ *  "grid_items.K" for grid K, "grid_num.K" i32[3] its cells per axis, "grid_helio.K" i32[2]
+ *  its first heliostat and count)                        "receiver" [4][3] f32 + focus[3]
  *  "buie" cdf_x[b+1], cdf_y[b+1], slope[b] f32         "micro_count" [n] i32
  *  "ray_offset" [n+1] i64      "poisson" [n][max_micro][2] f32 (zero padded)
  *  "sun" sun[3] + buie constants int465, A, B, C, 1/(gamma+1)            */

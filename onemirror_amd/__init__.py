@@ -32,6 +32,16 @@ class OMError(RuntimeError):
     pass
 
 
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("pos", ctypes.c_float * 3),
+        ("size", ctypes.c_float * 3),
+        ("interval", ctypes.c_float * 3),
+        ("first_helio", ctypes.c_int),
+        ("n_helio", ctypes.c_int),
+    ]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [
         ("n_helio", ctypes.c_int),
@@ -52,6 +62,10 @@ class SceneDesc(ctypes.Structure):
         ("dni", ctypes.c_float),
         ("recv_res_x", ctypes.c_int),
         ("recv_res_y", ctypes.c_int),
+        ("n_grids", ctypes.c_int),
+        ("grids", ctypes.POINTER(GridDesc)),
+        ("ground_length", ctypes.c_float),
+        ("ground_width", ctypes.c_float),
     ]
 
 
@@ -73,10 +87,12 @@ class Params(ctypes.Structure):
         ("smooth_ratio", ctypes.c_float),
         ("accumulate", ctypes.c_int),
         ("pack_shift", ctypes.c_int),
+        ("grid_select", ctypes.c_int),
     ]
 
 
 ACC_PACKED, ACC_DEVICE, ACC_WINDOW = 0, 1, 2
+GRIDS_ALL = -1
 
 
 class Flux(ctypes.Structure):
@@ -98,6 +114,7 @@ class SceneInfo(ctypes.Structure):
         ("pixel_length", ctypes.c_float),
         ("flux_scale", ctypes.c_float),
         ("total_energy", ctypes.c_float),
+        ("n_grids", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -133,7 +150,7 @@ _EXPORTS = [
     "om_smooth_trimmed", "om_results_with_given_normal", "om_intercept_ratio", "om_math_eval",
     "om_batch_create", "om_batch_run", "om_batch_info", "om_batch_destroy", "om_trace_batched",
     "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
-    "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel",
+    "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel", "om_scn_write",
 ]
 
 
@@ -154,6 +171,7 @@ def lib():
     L.om_scene_from_scn.argtypes = [_vp, ctypes.c_char_p, _f32p, ctypes.c_float,
                                     ctypes.POINTER(Params), ctypes.POINTER(_vp)]
     L.om_scn_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
+    L.om_scn_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
     L.om_scene_desc_free.argtypes = [ctypes.POINTER(SceneDesc)]
     L.om_scene_desc_free.restype = None
     L.om_scene_create.argtypes = [_vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(Params),
@@ -256,15 +274,55 @@ def _desc_struct(desc: dict, keep: list) -> SceneDesc:
     d.dni = float(desc["dni"])
     d.recv_res_x = int(desc.get("recv_res_x", 0))
     d.recv_res_y = int(desc.get("recv_res_y", 0))
+    grids = desc.get("grids") or []
+    if grids:
+        arr = (GridDesc * len(grids))()
+        for i, g in enumerate(grids):
+            arr[i].pos = _f3(g["pos"])
+            arr[i].size = _f3(g["size"])
+            arr[i].interval = _f3(g["interval"])
+            arr[i].first_helio = int(g["first_helio"])
+            arr[i].n_helio = int(g["n_helio"])
+        keep.append(arr)
+        d.n_grids = len(grids)
+        d.grids = ctypes.cast(arr, ctypes.POINTER(GridDesc))
+    d.ground_length = float(desc.get("ground_length", 0.0))
+    d.ground_width = float(desc.get("ground_width", 0.0))
     return d
 
 
+def grids_of(desc: dict) -> list:
+    """The grids of a description: its "grids" list, or the one grid of grid_pos/size/interval
+    holding every heliostat."""
+    if desc.get("grids"):
+        return list(desc["grids"])
+    return [{"pos": np.asarray(desc["grid_pos"], np.float32), "size": np.asarray(desc["grid_size"], np.float32),
+             "interval": np.asarray(desc["grid_interval"], np.float32), "first_helio": 0,
+             "n_helio": int(np.asarray(desc["helio_pos"]).shape[0])}]
+
+
+def scn_write(path: str, desc: dict):
+    """Write a description as a .scn file that SceneFileProc::SceneFileRead (and scn_parse)
+    reads back to the same description (om_scn_write)."""
+    keep: list = []
+    full = dict(desc)
+    for k, v in (("recv_pixel_length", 0.01), ("sun_dir", (0.0, -1.0, 0.0)), ("dni", 1000.0)):
+        full.setdefault(k, v)
+    d = _desc_struct(full, keep)
+    _check(lib().om_scn_write(path.encode(), ctypes.byref(d)), "om_scn_write")
+
+
 def scn_parse(path: str) -> dict:
-    """Parse a .scn file (SceneFileProc::SceneFileRead) into a description dict."""
+    """Parse a .scn file (SceneFileProc::SceneFileRead) into a description dict; "grids" lists
+    every Grid block with its heliostat range."""
     d = SceneDesc()
     _check(lib().om_scn_parse(path.encode(), ctypes.byref(d)), "om_scn_parse")
     H = d.n_helio
     try:
+        grids = [{"pos": np.array(d.grids[k].pos[:], np.float32), "size": np.array(d.grids[k].size[:], np.float32),
+                  "interval": np.array(d.grids[k].interval[:], np.float32),
+                  "first_helio": d.grids[k].first_helio, "n_helio": d.grids[k].n_helio}
+                 for k in range(d.n_grids)]
         out = {
             "helio_pos": np.ctypeslib.as_array(d.helio_pos, (max(H, 1), 3))[:H].copy(),
             "helio_size": np.ctypeslib.as_array(d.helio_size, (max(H, 1), 3))[:H].copy(),
@@ -279,6 +337,8 @@ def scn_parse(path: str) -> dict:
             "grid_interval": np.array(d.grid_interval[:], np.float32),
             "sun_dir": np.array(d.sun_dir[:], np.float32), "dni": np.float32(d.dni),
             "recv_res_x": d.recv_res_x, "recv_res_y": d.recv_res_y,
+            "grids": grids, "ground_length": np.float32(d.ground_length),
+            "ground_width": np.float32(d.ground_width),
         }
     finally:
         lib().om_scene_desc_free(ctypes.byref(d))

@@ -154,16 +154,24 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
         return e;
     };
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(OM_ERR_HIP);
-    // work items: (scene, heliostat, first ray) blocks of wgs * rpt rays, scene by scene
+    // work items: (table entry, heliostat, first ray) blocks of wgs * rpt rays, scene by
+    // scene; a table entry is one grid of one scene (its heliostats walk that grid)
     std::vector<int4> items;
+    std::vector<int> entry_scene, entry_grid;
     const int block = b->wgs * b->rpt;
     for (int i = 0; i < n; ++i) {
         const om_scene *s = scenes[i];
-        for (int h = 0; h < s->H; ++h) {
-            const long long nr = (long long)s->mcount[h] * s->prm.rays_per_sample;
-            for (long long r = 0; r < nr; r += block) items.push_back(make_int4(i, h, (int)r, 0));
+        for (size_t g = 0; g < s->grids.size(); ++g) {
+            const int e = (int)entry_scene.size();
+            entry_scene.push_back(i);
+            entry_grid.push_back((int)g);
+            for (int h = s->grids[g].h0; h < s->grids[g].h1; ++h) {
+                const long long nr = (long long)s->mcount[h] * s->prm.rays_per_sample;
+                for (long long r = 0; r < nr; r += block) items.push_back(make_int4(e, h, (int)r, 0));
+            }
         }
     }
+    const int n_entries = (int)entry_scene.size();
     if (items.size() >= (size_t)1 << 31) return fail((om::set_error("batch has too many work items"), OM_ERR_UNSUPPORTED));
     b->n_items = (int)items.size();
     // packed map copies: 8 << shift per scene within the budget
@@ -175,11 +183,11 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     const size_t words = (size_t)sum_px * ncopy + (size_t)sum_px; // copies + (carry, wrap) as u32 pairs
     if ((rc = dalloc(b->d_parts, words)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_raw, (size_t)sum_px)) != OM_OK) return fail(rc);
-    if ((rc = dalloc(b->d_table, (size_t)n)) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_table, (size_t)n_entries)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_items, items.size())) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_queue, 1)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_slots, (size_t)n)) != OM_OK) return fail(rc);
-    std::vector<om::TraceArgs> table(n);
+    std::vector<om::TraceArgs> table(n_entries);
     std::vector<om::ReduceSlot> slots(n);
     unsigned long long *copies = b->d_parts;
     uint32_t *side = (uint32_t *)(b->d_parts + (size_t)sum_px * ncopy);
@@ -187,7 +195,7 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     for (int i = 0; i < n; ++i) {
         om_scene *s = scenes[i];
         const int npx = s->res_x * s->res_y;
-        om::TraceArgs a = om::make_trace_args(s);
+        om::TraceArgs a = om::make_trace_args(s, 0);
         a.rays_per_thread = b->rpt;
         a.tile_rays = b->wgs * b->rpt;
         a.acc_packed = 1;
@@ -199,7 +207,12 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
         a.copy_stride = npx;
         a.px_stride = 1;
         a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
-        table[i] = a;
+        for (int e = 0; e < n_entries; ++e) {
+            if (entry_scene[e] != i) continue;
+            // the entry's grid; maps and launch shape as above
+            table[e] = a;
+            om::set_trace_grid(table[e], s, entry_grid[e]);
+        }
         om::ReduceSlot r;
         r.part = a.part;
         r.carry = a.carry;
@@ -218,7 +231,7 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
         b->raw.push_back(r.raw);
         px_off += (size_t)npx;
     }
-    if (hipMemcpy(b->d_table, table.data(), sizeof(om::TraceArgs) * n, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(b->d_table, table.data(), sizeof(om::TraceArgs) * n_entries, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_slots, slots.data(), sizeof(om::ReduceSlot) * n, hipMemcpyHostToDevice) != hipSuccess ||
         (!items.empty() && hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(),
                                      hipMemcpyHostToDevice) != hipSuccess))

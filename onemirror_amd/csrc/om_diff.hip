@@ -212,7 +212,7 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs 
     __shared__ float s_rx[RPT_MAX * WG], s_ry[RPT_MAX * WG], s_re[RPT_MAX * WG];
     __shared__ unsigned int s_wt[WG / 64], s_base;
     int nrec = 0; // records of this thread, staged in LDS slots [nrec][thread]
-    const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+    const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
     const DiffTile T = diff_tile(a, p, tl.x);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -412,7 +412,7 @@ template <int SUN, bool POISSON>
 __global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs a, DiffArgs p)
 {
     __shared__ double s_red[WG / 64][3];
-    const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+    const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
     const DiffTile T = diff_tile(a, p, tl.x);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -682,10 +682,26 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
 }
 
 template <bool FWD>
+static om_status launch_diff(om_scene *s, const TraceArgs &a, const DiffArgs &p, hipStream_t st);
+
+// the diff trace of every grid's heliostats over that grid
+template <bool FWD>
+static om_status launch_diff_grids(om_scene *s, const DiffArgs &p, hipStream_t st)
+{
+    for (size_t g = 0; g < s->grids.size(); ++g) {
+        TraceArgs a = make_trace_args(s, (int)g);
+        a.tile_base = s->grids[g].t0;
+        a.ntiles = s->grids[g].t1 - s->grids[g].t0;
+        if (om_status rc = launch_diff<FWD>(s, a, p, st); rc != OM_OK) return rc;
+    }
+    return OM_OK;
+}
+
+template <bool FWD>
 static om_status launch_diff(om_scene *s, const TraceArgs &a, const DiffArgs &p, hipStream_t st)
 {
-    const dim3 grid((unsigned)s->tiles.size()), block(WG);
-    if (s->tiles.empty()) return OM_OK;
+    const dim3 grid((unsigned)a.ntiles), block(WG);
+    if (a.ntiles <= 0) return OM_OK;
     const bool po = s->prm.micro_mode == OM_MICRO_POISSON;
     if (s->prm.sunshape == OM_SUN_BUIE) {
         if (po) hipLaunchKernelGGL((FWD ? diff_forward_kernel<OM_SUN_BUIE, true> : diff_backward_kernel<OM_SUN_BUIE, true>), grid, block, 0, st, a, p);
@@ -719,13 +735,10 @@ extern "C" om_status om_diff_forward(om_scene *s, const float *d_params, uint64_
     if (rc != OM_OK) return rc;
     p.theta = (const float4 *)d_params;
     p.mask = (unsigned long long *)d_mask;
-    TraceArgs a = make_trace_args(s);
-    a.tile_base = 0;
-    a.ntiles = (int)s->tiles.size();
     hipLaunchKernelGGL(diff_normals_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p);
     OM_HIP(hipGetLastError());
     if ((rc = launch_fill_u32(p.n_rec, 1, st)) != OM_OK) return rc;
-    if ((rc = launch_diff<true>(s, a, p, st)) != OM_OK) return rc;
+    if ((rc = launch_diff_grids<true>(s, p, st)) != OM_OK) return rc;
     hipLaunchKernelGGL(diff_splat_kernel, dim3((unsigned)(p.Q * p.G)), dim3(SPLAT_WGS), 0, st, p);
     OM_HIP(hipGetLastError());
     hipLaunchKernelGGL(diff_decode_kernel, dim3((p.npx + WG - 1) / WG), dim3(WG), 0, st, p, d_splat, d_energy);
@@ -745,14 +758,11 @@ extern "C" om_status om_diff_backward(om_scene *s, const float *d_params, const 
     p.mask = (unsigned long long *)d_mask;
     p.g = d_dsplat;
     p.gE = d_denergy;
-    TraceArgs a = make_trace_args(s);
-    a.tile_base = 0;
-    a.ntiles = (int)s->tiles.size();
     hipLaunchKernelGGL(diff_normals_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p);
     OM_HIP(hipGetLastError());
     hipLaunchKernelGGL(diff_gconv_kernel, dim3((p.npx + 16 + WG - 1) / WG), dim3(WG), 0, st, p);
     OM_HIP(hipGetLastError());
-    if ((rc = launch_diff<false>(s, a, p, st)) != OM_OK) return rc;
+    if ((rc = launch_diff_grids<false>(s, p, st)) != OM_OK) return rc;
     hipLaunchKernelGGL(diff_params_kernel, dim3((s->H + WG - 1) / WG), dim3(WG), 0, st, s->d_helio, p,
                        (float4 *)d_grad);
     OM_HIP(hipGetLastError());

@@ -112,6 +112,22 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// One uniform grid of a field (RectGrid, grid.h:25-49) and the heliostats [h0, h1) it holds:
+// per-cell CSR lists of global heliostat ids (CGridHelioMatch, grid.cpp:47-82), the range of
+// occupied cell indices, and the work tiles [t0, t1) of its heliostats.  The rays of a
+// heliostat walk the grid that holds it (recthelio_tracing.cpp:46-79 passes "the grid
+// heliostat belongs to"); ids are global so that the vertex array is the scene's.
+struct GridState {
+    float pos[3] = {0, 0, 0}, size[3] = {0, 0, 0}, interval[3] = {1, 1, 1};
+    int h0 = 0, h1 = 0;
+    int gn[3] = {0, 0, 0};
+    long long cells = 0, nnz = 0;
+    int occ_lo[3] = {0, 0, 0}, occ_hi[3] = {-1, -1, -1};
+    std::vector<int> cell_start, cell_items; // host CSR (root scenes only)
+    int *d_cell_start = nullptr, *d_cell_items = nullptr;
+    int t0 = 0, t1 = 0;
+};
+
 } // namespace om
 
 struct om_ctx {
@@ -135,16 +151,12 @@ struct om_scene {
     int recv_face = 0;
     float recv_pixel_length = 0.01f;
     int recv_res_in[2] = {0, 0};
-    float grid_pos[3], grid_size[3], grid_interval[3];
+    std::vector<om::GridState> grids; // in heliostat order; each traced over its own CSR
     float sun_in[3];
     float dni = 1000.0f;
     // derived host state
     om::v3 sun;
     std::vector<om::v3> normal, v0, v1, v2, v3;
-    int gn[3] = {0, 0, 0};
-    long long cells = 0, nnz = 0;
-    int occ_lo[3], occ_hi[3];
-    std::vector<int> cell_start, cell_items;
     om::v3 rv[4], recv_normal, focus;
     int geo_px = 0, res_x = 0, res_y = 0;
     float recv_pixel = 0.0f;
@@ -165,7 +177,6 @@ struct om_scene {
     // device state
     om::HelioRec *d_helio = nullptr;
     float4 *d_geo = nullptr;
-    int *d_cell_start = nullptr, *d_cell_items = nullptr;
     float4 *d_buie_bins = nullptr;
     int *d_buie_guide = nullptr;
     float2 *d_poisson = nullptr;
@@ -184,7 +195,12 @@ struct om_scene {
 namespace om {
 void set_error(const std::string &msg);
 om_status hip_fail(hipError_t e, const char *what);
-TraceArgs make_trace_args(const om_scene *s);
+// kernel arguments of a scene with the grid g (and its CSR) of the scene's grids
+TraceArgs make_trace_args(const om_scene *s, int g = 0);
+// the grid fields of a (box, cells, CSR, occupied range) set to grid g of s
+void set_trace_grid(TraceArgs &a, const om_scene *s, int g);
+// tiles [t0, t1) of heliostats [h0, h1) (tiles are in heliostat order)
+void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1);
 // kernels (om_kernels.hip)
 om_status launch_poisson(om_scene *s, hipStream_t st);
 om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st);

@@ -1048,7 +1048,7 @@ static om_status window_plan(om_scene *s, WindowRef &win, int &nwg)
 }
 
 // tiles of heliostats [h0, h1) are contiguous in the tile table
-static void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1)
+void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1)
 {
     auto lo = std::lower_bound(s->tiles.begin(), s->tiles.end(), h0,
                                [](const int2 &t, int h) { return t.x < h; });
@@ -1064,16 +1064,16 @@ using namespace om;
 
 namespace om {
 // trace heliostats [h0, h1) into out (cleared first when clear is set)
-om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st)
+// heliostats [h0, h1), all held by grid g, accumulated into out
+static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *out, hipStream_t st)
 {
-    TraceArgs a = make_trace_args(s);
+    TraceArgs a = make_trace_args(s, g);
     int t0, t1;
     tile_range(s, h0, h1, t0, t1);
     a.tile_base = t0;
     a.ntiles = t1 - t0;
     const int npx = s->res_x * s->res_y;
     om_status rc;
-    if (clear && (rc = clear_maps(out, npx, st)) != OM_OK) return rc;
     if (s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW) {
         // 8 XCDs x 2^shift copies of the packed map, as many as fit in PART_BYTES (at least
         // one per XCD): same-pixel atomics of concurrent workgroups then mostly land on
@@ -1135,6 +1135,19 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
     a.counts = out->counts;
     a.energy = (unsigned long long *)out->energy;
     return launch_trace(s, a, false, st);
+}
+
+om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st)
+{
+    om_status rc;
+    if (clear && (rc = clear_maps(out, s->res_x * s->res_y, st)) != OM_OK) return rc;
+    // one launch (and decode, which adds into out) per grid: a grid's CSR holds its own
+    // heliostats only
+    for (size_t g = 0; g < s->grids.size(); ++g) {
+        const int a0 = std::max(h0, s->grids[g].h0), a1 = std::min(h1, s->grids[g].h1);
+        if (a0 < a1 && (rc = trace_grid(s, (int)g, a0, a1, out, st)) != OM_OK) return rc;
+    }
+    return OM_OK;
 }
 } // namespace om
 
@@ -1264,20 +1277,25 @@ extern "C" om_status om_trace_records(om_scene *s, int h0, int h1, int32_t *shad
         return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
-    TraceArgs a = make_trace_args(s);
-    int t0, t1;
-    tile_range(s, h0, h1, t0, t1);
-    a.tile_base = t0;
-    a.ntiles = t1 - t0;
     // records never touch the maps; any valid pointers do
     if (om_status st = ensure_buf(s->ctx->scratch2, 16 * (size_t)s->res_x * s->res_y); st != OM_OK) return st;
-    a.counts = (uint32_t *)s->ctx->scratch2.p;
-    a.energy = (unsigned long long *)((char *)s->ctx->scratch2.p + 8 * (size_t)s->res_x * s->res_y);
-    a.rec_shade = shade;
-    a.rec_block = block;
-    a.rec_pixel = pixel;
-    a.rec_base = s->ray_offset[h0];
-    return launch_trace(s, a, true, (hipStream_t)stream);
+    for (size_t g = 0; g < s->grids.size(); ++g) {
+        const int a0 = std::max(h0, s->grids[g].h0), a1 = std::min(h1, s->grids[g].h1);
+        if (a0 >= a1) continue;
+        TraceArgs a = make_trace_args(s, (int)g);
+        int t0, t1;
+        tile_range(s, a0, a1, t0, t1);
+        a.tile_base = t0;
+        a.ntiles = t1 - t0;
+        a.counts = (uint32_t *)s->ctx->scratch2.p;
+        a.energy = (unsigned long long *)((char *)s->ctx->scratch2.p + 8 * (size_t)s->res_x * s->res_y);
+        a.rec_shade = shade;
+        a.rec_block = block;
+        a.rec_pixel = pixel;
+        a.rec_base = s->ray_offset[h0];
+        if (om_status st = launch_trace(s, a, true, (hipStream_t)stream); st != OM_OK) return st;
+    }
+    return OM_OK;
 }
 
 extern "C" om_status om_smooth_trimmed(om_ctx *ctx, const float *d_in, float *d_out, int width, int height,

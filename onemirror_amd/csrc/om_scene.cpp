@@ -137,12 +137,27 @@ extern "C" om_status om_ctx_destroy(om_ctx *ctx)
 //    blanks is not empty and reads as the key "" ;
 //  * keys unknown to the block map to "pos" (std::map::operator[] default, :155,187,232);
 //  * a failed numeric read stores 0 (std::istream semantics);
-//  * the size of a heliostat is read from the line after its "helio" line (:3-11).
-// One receiver and one grid are supported (the reference traces receivers[0] and grid0s[0]).
+//  * the size of a heliostat is read from the line after its "helio" line (:3-11);
+//  * "gap" and "matrix" hold until the next Grid block, which forgets both (:136-137), and a
+//    heliostat line before both are set is skipped (:246-249);
+//  * a grid's heliostats start at the sum of the "n" of the grids before it (:133, :199-201).
+// The first receiver is kept (the reference traces receivers[0]); every grid is kept.
+// Where the reference prints a warning and goes on into undefined behaviour, this parser
+// stops with an error (see om_scn_parse).
 // ---------------------------------------------------------------------------------------
 namespace {
 enum Key { K_POS = 0, K_SIZE, K_NORM, K_FACE, K_END, K_GAP, K_MATRIX, K_HELIO, K_INTER, K_N, K_TYPE };
 enum Mode { M_NONE, M_GROUND, M_RECV, M_GRID, M_HELIO };
+
+struct ParsedGrid {
+    int type = 0;                 // Grid <type> (only 0, RectGrid, is defined)
+    float pos[3] = {0, 0, 0}, size[3] = {0, 0, 0}, inter[3] = {1, 1, 1};
+    int n = 0;                    // declared heliostats (num_helios_)
+    int helio_type = 0;           // heliostat type of the grid (helio_type_)
+    int first = 0;                // start_helio_pos_
+    int listed = 0;               // heliostats read under it
+    int skipped = 0;              // heliostat lines skipped (type != 0, no gap / matrix)
+};
 
 struct Parsed {
     std::vector<float> pos, size, gap;
@@ -151,17 +166,21 @@ struct Parsed {
     float recv_pos[3] = {0, 0, 0}, recv_size[3] = {0, 0, 0}, recv_norm[3] = {0, 0, 1};
     int recv_face = 0;
     int n_recv = 0;
-    float grid_pos[3] = {0, 0, 0}, grid_size[3] = {0, 0, 0}, grid_inter[3] = {1, 1, 1};
-    int n_grid = 0, grid_n = 0;
+    float ground[2] = {0, 0};
+    int ngrid = 0;
+    std::vector<ParsedGrid> grids;
 };
+
+om_status parse_fail(om_status st, const std::string &msg)
+{
+    om::set_error(msg);
+    return st;
+}
 
 om_status parse_scn(const char *path, Parsed &P)
 {
     std::ifstream f(path);
-    if (!f.is_open()) {
-        om::set_error(std::string("cannot open scene file ") + path);
-        return OM_ERR_IO;
-    }
+    if (!f.is_open()) return parse_fail(OM_ERR_IO, std::string("cannot open scene file ") + path);
     std::stringstream scene;
     scene << f.rdbuf();
     std::map<std::string, int> keys = {{"pos", K_POS}, {"size", K_SIZE}, {"norm", K_NORM},
@@ -172,7 +191,7 @@ om_status parse_scn(const char *path, Parsed &P)
     bool gap_set = false, matrix_set = false, in_recv = false, in_grid = false;
     float gap[2] = {0, 0};
     int mat[2] = {1, 1};
-    int grid_type = 0;
+    int next_first = 0;
     std::string line;
     while (std::getline(scene, line)) {
         if (line.empty() || line[0] == '#') continue;
@@ -182,39 +201,43 @@ om_status parse_scn(const char *path, Parsed &P)
         ls >> head;
         if (head == "ground") {
             mode = M_GROUND;
+            ls >> P.ground[0] >> P.ground[1];
             continue;
         } else if (head == "Recv") {
             mode = M_RECV;
             int t = 0;
             ls >> t;
-            if (P.n_recv == 0) {
+            if (t < 0 || t > 2)
+                return parse_fail(OM_ERR_PARSE, "receiver type " + std::to_string(t) + " is not defined (0..2)");
+            // a new receiver object; the first one that reaches "end" is receivers[0]
+            in_recv = P.n_recv == 0;
+            if (in_recv) {
                 P.recv_type = t;
-                in_recv = true;
-            } else {
-                in_recv = false;
+                for (int a = 0; a < 3; ++a) P.recv_pos[a] = P.recv_size[a] = 0.0f;
+                P.recv_norm[0] = P.recv_norm[1] = 0.0f;
+                P.recv_norm[2] = 1.0f;
+                P.recv_face = 0;
             }
             continue;
         } else if (head == "Grid") {
             mode = M_GRID;
-            ls >> grid_type;
-            in_grid = (P.n_grid == 0);
+            ParsedGrid g;
+            ls >> g.type;
+            g.first = next_first;
+            P.grids.push_back(g);
+            in_grid = true;
             gap_set = matrix_set = false;
-            if (P.n_grid >= 1) {
-                om::set_error("scene files with more than one grid are not supported");
-                return OM_ERR_UNSUPPORTED;
-            }
             continue;
         }
         int key = keys[head]; // unknown keys read as "pos"
         switch (mode) {
         case M_NONE:
+            break;
         case M_GROUND:
+            if (head == "ngrid") ls >> P.ngrid;
             break;
         case M_RECV:
-            if (!in_recv) {
-                if (key == K_END) ++P.n_recv;
-                break;
-            }
+            if (!in_recv) break;
             if (key == K_POS) ls >> P.recv_pos[0] >> P.recv_pos[1] >> P.recv_pos[2];
             else if (key == K_SIZE) ls >> P.recv_size[0] >> P.recv_size[1] >> P.recv_size[2];
             else if (key == K_NORM) ls >> P.recv_norm[0] >> P.recv_norm[1] >> P.recv_norm[2];
@@ -224,22 +247,29 @@ om_status parse_scn(const char *path, Parsed &P)
                 in_recv = false;
             }
             break;
-        case M_GRID:
-            if (grid_type != 0) {
-                om::set_error("grid type not supported");
-                return OM_ERR_UNSUPPORTED;
-            }
+        case M_GRID: {
+            ParsedGrid &g = P.grids.back();
+            if (g.type != 0)
+                return parse_fail(OM_ERR_UNSUPPORTED, "grid type " + std::to_string(g.type) +
+                                                          " is not supported (only 0, the rectangular grid)");
             if (!in_grid) break;
-            if (key == K_POS) ls >> P.grid_pos[0] >> P.grid_pos[1] >> P.grid_pos[2];
-            else if (key == K_SIZE) ls >> P.grid_size[0] >> P.grid_size[1] >> P.grid_size[2];
-            else if (key == K_INTER) ls >> P.grid_inter[0] >> P.grid_inter[1] >> P.grid_inter[2];
-            else if (key == K_N) ls >> P.grid_n;
+            if (key == K_POS) ls >> g.pos[0] >> g.pos[1] >> g.pos[2];
+            else if (key == K_SIZE) ls >> g.size[0] >> g.size[1] >> g.size[2];
+            else if (key == K_INTER) ls >> g.inter[0] >> g.inter[1] >> g.inter[2];
+            else if (key == K_N) {
+                int n = 0;
+                ls >> n;
+                g.n = n;
+                next_first += n;
+            } else if (key == K_TYPE) ls >> g.helio_type;
             else if (key == K_END) {
-                ++P.n_grid;
+                in_grid = false;
                 mode = M_HELIO;
             }
             break;
-        case M_HELIO:
+        }
+        case M_HELIO: {
+            ParsedGrid &g = P.grids.back();
             if (key == K_GAP) {
                 ls >> gap[0] >> gap[1];
                 gap_set = true;
@@ -247,7 +277,10 @@ om_status parse_scn(const char *path, Parsed &P)
                 ls >> mat[0] >> mat[1];
                 matrix_set = true;
             } else if (key == K_HELIO) {
-                if (!gap_set || !matrix_set) break; // "did not set the gap and matrix"
+                if (!gap_set || !matrix_set || g.helio_type != 0) {
+                    ++g.skipped; // "did not set the gap and matrix", or not a rectangle
+                    break;
+                }
                 float p[3] = {0, 0, 0};
                 ls >> p[0] >> p[1] >> p[2];
                 std::string nxt;
@@ -260,19 +293,35 @@ om_status parse_scn(const char *path, Parsed &P)
                 P.size.insert(P.size.end(), sz, sz + 3);
                 P.gap.insert(P.gap.end(), gap, gap + 2);
                 P.matrix.insert(P.matrix.end(), mat, mat + 2);
+                ++g.listed;
             }
             break;
         }
+        }
     }
-    if (P.n_grid == 0) {
-        om::set_error("scene file has no grid");
-        return OM_ERR_PARSE;
-    }
-    if (P.n_recv == 0) {
-        om::set_error("scene file has no receiver");
-        return OM_ERR_PARSE;
+    if (P.grids.empty()) return parse_fail(OM_ERR_PARSE, "scene file has no grid");
+    if (P.n_recv == 0) return parse_fail(OM_ERR_PARSE, "scene file has no receiver");
+    for (size_t k = 0; k < P.grids.size(); ++k) {
+        const ParsedGrid &g = P.grids[k];
+        const std::string gk = "grid " + std::to_string(k);
+        if (g.helio_type != 0 && (g.n > 0 || g.skipped > 0))
+            return parse_fail(OM_ERR_UNSUPPORTED, gk + ": heliostat type " + std::to_string(g.helio_type) +
+                                                      " is not supported (only 0, the rectangular heliostat)");
+        if (g.n < 0 || g.listed != g.n)
+            return parse_fail(OM_ERR_PARSE, gk + " declares " + std::to_string(g.n) + " heliostats and lists " +
+                                                std::to_string(g.listed) +
+                                                (g.skipped ? " (heliostat lines before its gap and matrix are skipped)" : ""));
     }
     return OM_OK;
+}
+
+void put_grid(om_grid_desc &o, const float pos[3], const float size[3], const float inter[3], int first, int n)
+{
+    std::memcpy(o.pos, pos, 12);
+    std::memcpy(o.size, size, 12);
+    std::memcpy(o.interval, inter, 12);
+    o.first_helio = first;
+    o.n_helio = n;
 }
 } // namespace
 
@@ -290,6 +339,11 @@ extern "C" om_status om_scn_parse(const char *path, om_scene_desc *d)
     std::copy(P.size.begin(), P.size.end(), size);
     std::copy(P.gap.begin(), P.gap.end(), gap);
     std::copy(P.matrix.begin(), P.matrix.end(), mat);
+    om_grid_desc *grids = new om_grid_desc[P.grids.size()];
+    for (size_t k = 0; k < P.grids.size(); ++k) {
+        const ParsedGrid &g = P.grids[k];
+        put_grid(grids[k], g.pos, g.size, g.inter, g.first, g.n);
+    }
     d->n_helio = H;
     d->helio_pos = pos;
     d->helio_size = size;
@@ -301,9 +355,13 @@ extern "C" om_status om_scn_parse(const char *path, om_scene_desc *d)
     std::memcpy(d->recv_normal, P.recv_norm, 12);
     d->recv_face = P.recv_face;
     d->recv_pixel_length = 0.01f; // solarenergy::receiver_pixel_length (common_var.cpp:12)
-    std::memcpy(d->grid_pos, P.grid_pos, 12);
-    std::memcpy(d->grid_size, P.grid_size, 12);
-    std::memcpy(d->grid_interval, P.grid_inter, 12);
+    std::memcpy(d->grid_pos, grids[0].pos, 12);
+    std::memcpy(d->grid_size, grids[0].size, 12);
+    std::memcpy(d->grid_interval, grids[0].interval, 12);
+    d->n_grids = (int)P.grids.size();
+    d->grids = grids;
+    d->ground_length = P.ground[0];
+    d->ground_width = P.ground[1];
     d->dni = 1000.0f;
     return OM_OK;
 }
@@ -315,7 +373,96 @@ extern "C" void om_scene_desc_free(om_scene_desc *d)
     delete[] d->helio_size;
     delete[] d->helio_gap;
     delete[] d->helio_matrix;
+    delete[] d->grids;
     std::memset(d, 0, sizeof(*d));
+}
+
+// .scn writer: the grammar SceneFileRead reads (scene_file_proc.cpp:41-286).  "%.9g" prints
+// every float with enough digits that the stream extraction gives it back exactly.
+extern "C" om_status om_scn_write(const char *path, const om_scene_desc *d)
+{
+    if (!path || !d || d->n_helio < 0 || d->n_grids < 0 || (d->n_grids > 0 && !d->grids) ||
+        (d->n_helio > 0 && (!d->helio_pos || !d->helio_size || !d->helio_gap || !d->helio_matrix)))
+        return OM_ERR_INVALID;
+    std::vector<om_grid_desc> grids;
+    if (d->n_grids == 0) {
+        om_grid_desc g;
+        put_grid(g, d->grid_pos, d->grid_size, d->grid_interval, 0, d->n_helio);
+        grids.push_back(g);
+    } else {
+        grids.assign(d->grids, d->grids + d->n_grids);
+    }
+    int next = 0;
+    for (const om_grid_desc &g : grids) {
+        if (g.first_helio != next || g.n_helio < 0) {
+            om::set_error("grids must hold consecutive heliostat ranges from 0");
+            return OM_ERR_INVALID;
+        }
+        next += g.n_helio;
+    }
+    if (next != d->n_helio) {
+        om::set_error("the grids' heliostats must add up to n_helio");
+        return OM_ERR_INVALID;
+    }
+    if (d->recv_type < 0 || d->recv_type > 2) {
+        om::set_error("receiver type must be 0..2");
+        return OM_ERR_INVALID;
+    }
+    // every number must read back: finite floats only
+    auto finite = [](const float *x, size_t n) {
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(x[i])) return false;
+        return true;
+    };
+    bool ok = finite(d->recv_pos, 3) && finite(d->recv_size, 3) && finite(d->recv_normal, 3) &&
+              finite(&d->ground_length, 1) && finite(&d->ground_width, 1) &&
+              finite(d->helio_pos, 3 * (size_t)d->n_helio) && finite(d->helio_size, 3 * (size_t)d->n_helio) &&
+              finite(d->helio_gap, 2 * (size_t)d->n_helio);
+    for (const om_grid_desc &g : grids) ok = ok && finite(g.pos, 3) && finite(g.size, 3) && finite(g.interval, 3);
+    if (!ok) {
+        om::set_error("scene values must be finite to be written");
+        return OM_ERR_INVALID;
+    }
+    FILE *f = std::fopen(path, "w");
+    if (!f) {
+        om::set_error(std::string("cannot write scene file ") + path);
+        return OM_ERR_IO;
+    }
+    auto v3 = [&](const char *key, const float *x) { std::fprintf(f, "%s\t%.9g\t%.9g\t%.9g\n", key, x[0], x[1], x[2]); };
+    std::fprintf(f, "# Ground Boundary\nground\t%.9g\t%.9g\nngrid\t%d\n\n", d->ground_length, d->ground_width,
+                 (int)grids.size());
+    std::fprintf(f, "# Receiver attributes\nRecv\t%d\n", d->recv_type);
+    v3("pos", d->recv_pos);
+    v3("size", d->recv_size);
+    v3("norm", d->recv_normal);
+    std::fprintf(f, "face\t%d\nend\n", d->recv_face);
+    for (size_t k = 0; k < grids.size(); ++k) {
+        const om_grid_desc &g = grids[k];
+        std::fprintf(f, "\n# Grid%zu attributes\nGrid\t0\n", k);
+        v3("pos", g.pos);
+        v3("size", g.size);
+        v3("inter", g.interval);
+        std::fprintf(f, "n\t%d\ntype\t0\nend\n\n# Heliostats\n", g.n_helio);
+        bool first = true;
+        for (int i = g.first_helio; i < g.first_helio + g.n_helio; ++i) {
+            const float *gp = d->helio_gap + 2 * (size_t)i;
+            const int *mp = d->helio_matrix + 2 * (size_t)i;
+            if (first || gp[0] != gp[-2] || gp[1] != gp[-1] ||
+                std::signbit(gp[0]) != std::signbit(gp[-2]) || std::signbit(gp[1]) != std::signbit(gp[-1]))
+                std::fprintf(f, "gap\t%.9g\t%.9g\n", gp[0], gp[1]);
+            if (first || mp[0] != mp[-2] || mp[1] != mp[-1]) std::fprintf(f, "matrix\t%d\t%d\n", mp[0], mp[1]);
+            first = false;
+            v3("helio", d->helio_pos + 3 * (size_t)i);
+            const float *sz = d->helio_size + 3 * (size_t)i;
+            std::fprintf(f, "\t%.9g\t%.9g\t%.9g\n", sz[0], sz[1], sz[2]);
+        }
+    }
+    const bool bad = std::ferror(f) != 0;
+    if (std::fclose(f) != 0 || bad) {
+        om::set_error(std::string("error writing scene file ") + path);
+        return OM_ERR_IO;
+    }
+    return OM_OK;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -357,30 +504,32 @@ void build_buie(om_scene *s)
     s->buie_inv_gp1 = 1 / (gamma + 1);
 }
 
-// RectGrid::Cinit + CGridHelioMatch (grid.cpp:3-82); binning clamped to the grid.
-om_status build_grid(om_scene *s)
+// RectGrid::Cinit + CGridHelioMatch (grid.cpp:3-82) of grid G over its heliostats
+// [G.h0, G.h1), with global heliostat ids (grid.cpp:53); binning clamped to the grid.
+om_status build_grid(const om_scene *s, om::GridState &G)
 {
-    v3 gp = ld3(s->grid_pos), gs = ld3(s->grid_size), gi = ld3(s->grid_interval);
-    s->gn[0] = (int)ceilf(gs.x / gi.x);
-    s->gn[1] = (int)ceilf(gs.y / gi.y);
-    s->gn[2] = (int)ceilf(gs.z / gi.z);
-    if (s->gn[0] <= 0 || s->gn[1] <= 0 || s->gn[2] <= 0) {
+    v3 gp = ld3(G.pos), gs = ld3(G.size), gi = ld3(G.interval);
+    G.gn[0] = (int)ceilf(gs.x / gi.x);
+    G.gn[1] = (int)ceilf(gs.y / gi.y);
+    G.gn[2] = (int)ceilf(gs.z / gi.z);
+    if (G.gn[0] <= 0 || G.gn[1] <= 0 || G.gn[2] <= 0) {
         om::set_error("grid has no cells");
         return OM_ERR_INVALID;
     }
-    s->cells = (long long)s->gn[0] * s->gn[1] * s->gn[2];
-    if (s->cells >= (1LL << 31) - 1) {
+    G.cells = (long long)G.gn[0] * G.gn[1] * G.gn[2];
+    if (G.cells >= (1LL << 31) - 1) {
         om::set_error("grid too large");
         return OM_ERR_UNSUPPORTED;
     }
-    std::vector<int> cnt((size_t)s->cells, 0);
-    std::vector<int> rng((size_t)s->H * 6);
+    std::vector<int> cnt((size_t)G.cells, 0);
+    const int nh = G.h1 - G.h0;
+    std::vector<int> rng((size_t)nh * 6);
     for (int a = 0; a < 3; ++a) {
-        s->occ_lo[a] = 1 << 30;
-        s->occ_hi[a] = -1;
+        G.occ_lo[a] = 1 << 30;
+        G.occ_hi[a] = -1;
     }
     long long nnz = 0;
-    for (int i = 0; i < s->H; ++i) {
+    for (int i = G.h0; i < G.h1; ++i) {
         v3 sz = ld3(&s->size[3 * i]), p = ld3(&s->pos[3 * i]);
         float diag = om::sqrt_(om::dot(sz, sz));
         float radius = om::sqrt_(diag) / 2;
@@ -389,22 +538,23 @@ om_status build_grid(om_scene *s)
         v3 a0 = mn - gp, a1 = mx - gp;
         int lo[3] = {om::trunc_i(a0.x / gi.x), om::trunc_i(a0.y / gi.y), om::trunc_i(a0.z / gi.z)};
         int hi[3] = {om::trunc_i(a1.x / gi.x), om::trunc_i(a1.y / gi.y), om::trunc_i(a1.z / gi.z)};
+        int *r = &rng[6 * (size_t)(i - G.h0)];
         for (int a = 0; a < 3; ++a) {
             if (lo[a] < 0) lo[a] = 0;
-            if (hi[a] > s->gn[a] - 1) hi[a] = s->gn[a] - 1;
-            rng[6 * i + a] = lo[a];
-            rng[6 * i + 3 + a] = hi[a];
+            if (hi[a] > G.gn[a] - 1) hi[a] = G.gn[a] - 1;
+            r[a] = lo[a];
+            r[3 + a] = hi[a];
         }
         bool any = lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
         if (!any) continue;
         for (int a = 0; a < 3; ++a) {
-            s->occ_lo[a] = std::min(s->occ_lo[a], lo[a]);
-            s->occ_hi[a] = std::max(s->occ_hi[a], hi[a]);
+            G.occ_lo[a] = std::min(G.occ_lo[a], lo[a]);
+            G.occ_hi[a] = std::max(G.occ_hi[a], hi[a]);
         }
         for (int x = lo[0]; x <= hi[0]; ++x)
             for (int y = lo[1]; y <= hi[1]; ++y)
                 for (int z = lo[2]; z <= hi[2]; ++z) {
-                    cnt[(size_t)x * s->gn[1] * s->gn[2] + (size_t)y * s->gn[2] + z]++;
+                    cnt[(size_t)x * G.gn[1] * G.gn[2] + (size_t)y * G.gn[2] + z]++;
                     ++nnz;
                 }
     }
@@ -412,18 +562,20 @@ om_status build_grid(om_scene *s)
         om::set_error("grid holds too many entries");
         return OM_ERR_UNSUPPORTED;
     }
-    s->nnz = nnz;
-    s->cell_start.assign((size_t)s->cells + 1, 0);
-    for (long long c = 0; c < s->cells; ++c) s->cell_start[c + 1] = s->cell_start[c] + cnt[c];
-    s->cell_items.assign((size_t)(nnz > 0 ? nnz : 1), 0);
+    G.nnz = nnz;
+    G.cell_start.assign((size_t)G.cells + 1, 0);
+    for (long long c = 0; c < G.cells; ++c) G.cell_start[c + 1] = G.cell_start[c] + cnt[c];
+    G.cell_items.assign((size_t)(nnz > 0 ? nnz : 1), 0);
     std::fill(cnt.begin(), cnt.end(), 0);
-    for (int i = 0; i < s->H; ++i)
-        for (int x = rng[6 * i]; x <= rng[6 * i + 3]; ++x)
-            for (int y = rng[6 * i + 1]; y <= rng[6 * i + 4]; ++y)
-                for (int z = rng[6 * i + 2]; z <= rng[6 * i + 5]; ++z) {
-                    size_t c = (size_t)x * s->gn[1] * s->gn[2] + (size_t)y * s->gn[2] + z;
-                    s->cell_items[s->cell_start[c] + cnt[c]++] = i;
+    for (int i = G.h0; i < G.h1; ++i) {
+        const int *r = &rng[6 * (size_t)(i - G.h0)];
+        for (int x = r[0]; x <= r[3]; ++x)
+            for (int y = r[1]; y <= r[4]; ++y)
+                for (int z = r[2]; z <= r[5]; ++z) {
+                    size_t c = (size_t)x * G.gn[1] * G.gn[2] + (size_t)y * G.gn[2] + z;
+                    G.cell_items[G.cell_start[c] + cnt[c]++] = i;
                 }
+    }
     return OM_OK;
 }
 
@@ -670,6 +822,7 @@ void build_tiles(om_scene *s)
         long long nr = (long long)s->mcount[i] * s->prm.rays_per_sample;
         for (long long r = 0; r < nr; r += per_tile) s->tiles.push_back(make_int2(i, (int)r));
     }
+    for (om::GridState &G : s->grids) om::tile_range(s, G.h0, G.h1, G.t0, G.t1);
 }
 
 om_status upload_geometry(om_scene *s)
@@ -700,8 +853,10 @@ void free_scene_dev(om_scene *s)
     s->diff_buf.p = nullptr;
     s->diff_buf.bytes = 0;
     if (s->grid_src) return; // the rest belongs to the root scene
-    dfree(s->d_cell_start);
-    dfree(s->d_cell_items);
+    for (om::GridState &G : s->grids) {
+        dfree(G.d_cell_start);
+        dfree(G.d_cell_items);
+    }
     dfree(s->d_buie_bins);
     dfree(s->d_buie_guide);
     dfree(s->d_poisson);
@@ -728,29 +883,41 @@ bool bounded(const float *x, size_t n)
 // rectangle v0 v1 v2 v3; the slack covers the rounding of an origin and of its cell index.
 om_status check_in_grid(const om_scene *s)
 {
-    for (int i = 0; i < s->H; ++i) {
-        const v3 c[4] = {s->v0[i], s->v1[i], s->v2[i], s->v3[i]};
-        const double sx = s->size[3 * i], sy = s->size[3 * i + 1], sz = s->size[3 * i + 2];
-        const double R = 0.5 * std::sqrt(sx * sx + sy * sy + sz * sz);
-        for (int a = 0; a < 3; ++a) {
-            double lo = HUGE_VAL, hi = -HUGE_VAL;
-            for (int k = 0; k < 4; ++k) {
-                const double x = a == 0 ? c[k].x : (a == 1 ? c[k].y : c[k].z);
-                lo = std::min(lo, x);
-                hi = std::max(hi, x);
-            }
-            const double gp = s->grid_pos[a], ext = (double)s->gn[a] * s->grid_interval[a];
-            const double slack = 1e-5 * (std::fabs((double)s->pos[3 * i + a]) + R + std::fabs(gp) + ext);
-            if (!(lo - slack > gp && hi + slack < gp + ext)) {
-                char msg[160];
-                std::snprintf(msg, sizeof msg, "heliostat %d reaches outside the grid on axis %c "
-                              "(mirror %.6g..%.6g, grid %.6g..%.6g)", i, "xyz"[a], lo, hi, gp, gp + ext);
-                om::set_error(msg);
-                return OM_ERR_UNSUPPORTED;
+    for (size_t g = 0; g < s->grids.size(); ++g) {
+        const om::GridState &G = s->grids[g];
+        for (int i = G.h0; i < G.h1; ++i) {
+            const v3 c[4] = {s->v0[i], s->v1[i], s->v2[i], s->v3[i]};
+            const double sx = s->size[3 * i], sy = s->size[3 * i + 1], sz = s->size[3 * i + 2];
+            const double R = 0.5 * std::sqrt(sx * sx + sy * sy + sz * sz);
+            for (int a = 0; a < 3; ++a) {
+                double lo = HUGE_VAL, hi = -HUGE_VAL;
+                for (int k = 0; k < 4; ++k) {
+                    const double x = a == 0 ? c[k].x : (a == 1 ? c[k].y : c[k].z);
+                    lo = std::min(lo, x);
+                    hi = std::max(hi, x);
+                }
+                const double gp = G.pos[a], ext = (double)G.gn[a] * G.interval[a];
+                const double slack = 1e-5 * (std::fabs((double)s->pos[3 * i + a]) + R + std::fabs(gp) + ext);
+                if (!(lo - slack > gp && hi + slack < gp + ext)) {
+                    char msg[160];
+                    std::snprintf(msg, sizeof msg, "heliostat %d reaches outside the grid on axis %c "
+                                  "(mirror %.6g..%.6g, grid %d %.6g..%.6g)", i, "xyz"[a], lo, hi, (int)g, gp, gp + ext);
+                    om::set_error(msg);
+                    return OM_ERR_UNSUPPORTED;
+                }
             }
         }
     }
     return OM_OK;
+}
+
+// the grids of a description: the grids array, or the one grid of the grid_* fields
+std::vector<om_grid_desc> desc_grids(const om_scene_desc *d)
+{
+    if (d->n_grids > 0) return std::vector<om_grid_desc>(d->grids, d->grids + d->n_grids);
+    om_grid_desc g;
+    put_grid(g, d->grid_pos, d->grid_size, d->grid_interval, 0, d->n_helio);
+    return {g};
 }
 
 om_status validate(const om_scene_desc *d, const om_params *p)
@@ -794,23 +961,49 @@ om_status validate(const om_scene_desc *d, const om_params *p)
             om::set_error("heliostat facet matrix must be positive");
             return OM_ERR_INVALID;
         }
-    for (int a = 0; a < 3; ++a)
-        if (!(d->grid_interval[a] > 0.0f)) {
-            om::set_error("grid interval must be positive");
+    if (d->n_grids < 0 || (d->n_grids > 0 && !d->grids)) {
+        om::set_error("grid array missing");
+        return OM_ERR_INVALID;
+    }
+    const std::vector<om_grid_desc> grids = desc_grids(d);
+    if (p->grid_select != OM_GRIDS_ALL && (p->grid_select < 0 || p->grid_select >= (int)grids.size())) {
+        om::set_error("grid_select must be OM_GRIDS_ALL or the index of a grid");
+        return OM_ERR_INVALID;
+    }
+    int next = 0;
+    for (const om_grid_desc &g : grids) {
+        if (g.first_helio != next || g.n_helio < 0) {
+            om::set_error("grids must hold consecutive heliostat ranges from 0");
             return OM_ERR_INVALID;
         }
+        next += g.n_helio;
+        for (int a = 0; a < 3; ++a)
+            if (!(g.interval[a] > 0.0f)) {
+                om::set_error("grid interval must be positive");
+                return OM_ERR_INVALID;
+            }
+    }
+    if (next != d->n_helio) {
+        om::set_error("the grids' heliostats must add up to n_helio");
+        return OM_ERR_INVALID;
+    }
     const size_t H = (size_t)d->n_helio;
     if (!bounded(d->helio_pos, 3 * H) || !bounded(d->helio_size, 3 * H) || !bounded(d->helio_gap, 2 * H) ||
-        !bounded(d->grid_pos, 3) || !bounded(d->grid_size, 3) || !bounded(d->recv_pos, 3) ||
-        !bounded(d->recv_size, 3)) {
+        !bounded(d->recv_pos, 3) || !bounded(d->recv_size, 3)) {
         om::set_error("coordinates and sizes must be finite and within 2^20 m");
         return OM_ERR_UNSUPPORTED;
     }
-    for (int a = 0; a < 3; ++a)
-        if (!(d->grid_interval[a] >= EDGE_MIN && d->grid_interval[a] <= COORD_MAX)) {
-            om::set_error("grid interval must be within [2^-20, 2^20] m");
+    for (const om_grid_desc &g : grids) {
+        if (!bounded(g.pos, 3) || !bounded(g.size, 3)) {
+            om::set_error("coordinates and sizes must be finite and within 2^20 m");
             return OM_ERR_UNSUPPORTED;
         }
+        for (int a = 0; a < 3; ++a)
+            if (!(g.interval[a] >= EDGE_MIN && g.interval[a] <= COORD_MAX)) {
+                om::set_error("grid interval must be within [2^-20, 2^20] m");
+                return OM_ERR_UNSUPPORTED;
+            }
+    }
     if (!bounded(d->sun_dir, 3) || (d->sun_dir[0] == 0.0f && d->sun_dir[1] == 0.0f && d->sun_dir[2] == 0.0f)) {
         om::set_error("sun direction must be finite and nonzero");
         return OM_ERR_INVALID;
@@ -824,7 +1017,8 @@ om_status build_all(om_scene *s)
     // InitContent order (solar_scene.cpp:83-99): sun, grid, receiver, heliostats
     s->sun = om::normalize(ld3(s->sun_in));
     build_buie(s);
-    if ((st = build_grid(s)) != OM_OK) return st;
+    for (om::GridState &G : s->grids)
+        if ((st = build_grid(s, G)) != OM_OK) return st;
     if ((st = build_receiver(s)) != OM_OK) return st;
     aim_heliostats(s);
     if ((st = check_in_grid(s)) != OM_OK) return st;
@@ -843,8 +1037,10 @@ om_status build_all(om_scene *s)
     }
     // device copies
     OM_HIP(hipSetDevice(s->ctx->device));
-    if ((st = upload(s->d_cell_start, s->cell_start.data(), s->cell_start.size())) != OM_OK) return st;
-    if ((st = upload(s->d_cell_items, s->cell_items.data(), s->cell_items.size())) != OM_OK) return st;
+    for (om::GridState &G : s->grids) {
+        if ((st = upload(G.d_cell_start, G.cell_start.data(), G.cell_start.size())) != OM_OK) return st;
+        if ((st = upload(G.d_cell_items, G.cell_items.data(), G.cell_items.size())) != OM_OK) return st;
+    }
     {
         // Buie lookup tables: packed bins and a guide from the value scale to a bin
         int n = s->nbins;
@@ -889,7 +1085,36 @@ om_status build_all(om_scene *s)
 } // namespace
 
 namespace om {
-TraceArgs make_trace_args(const om_scene *s)
+void set_trace_grid(TraceArgs &a, const om_scene *s, int g)
+{
+    const GridState &G = s->grids[g];
+    const GridState &D = (s->grid_src ? s->grid_src : s)->grids[g]; // device CSR owner
+    a.gpx = G.pos[0];
+    a.gpy = G.pos[1];
+    a.gpz = G.pos[2];
+    a.gix = G.interval[0];
+    a.giy = G.interval[1];
+    a.giz = G.interval[2];
+    auto pow2 = [](float x) {
+        int e;
+        return x > 0.0f && std::frexp(x, &e) == 0.5f && e > -100 && e < 100;
+    };
+    a.grid_pow2 = pow2(a.gix) && pow2(a.giy) && pow2(a.giz);
+    a.inv_gix = 1.0f / a.gix;
+    a.inv_giy = 1.0f / a.giy;
+    a.inv_giz = 1.0f / a.giz;
+    a.gnx = G.gn[0];
+    a.gny = G.gn[1];
+    a.gnz = G.gn[2];
+    for (int i = 0; i < 3; ++i) {
+        a.occ_lo[i] = G.occ_lo[i];
+        a.occ_hi[i] = G.occ_hi[i];
+    }
+    a.cell_start = D.d_cell_start;
+    a.cell_items = D.d_cell_items;
+}
+
+TraceArgs make_trace_args(const om_scene *s, int g)
 {
     TraceArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -915,29 +1140,7 @@ TraceArgs make_trace_args(const om_scene *s)
     a.L = s->prm.rays_per_sample;
     a.pitch = s->prm.helio_pixel_length;
     a.pitch2 = s->prm.helio_pixel_length / 2;
-    a.gpx = s->grid_pos[0];
-    a.gpy = s->grid_pos[1];
-    a.gpz = s->grid_pos[2];
-    a.gix = s->grid_interval[0];
-    a.giy = s->grid_interval[1];
-    a.giz = s->grid_interval[2];
-    auto pow2 = [](float x) {
-        int e;
-        return x > 0.0f && std::frexp(x, &e) == 0.5f && e > -100 && e < 100;
-    };
-    a.grid_pow2 = pow2(a.gix) && pow2(a.giy) && pow2(a.giz);
-    a.inv_gix = 1.0f / a.gix;
-    a.inv_giy = 1.0f / a.giy;
-    a.inv_giz = 1.0f / a.giz;
-    a.gnx = s->gn[0];
-    a.gny = s->gn[1];
-    a.gnz = s->gn[2];
-    for (int i = 0; i < 3; ++i) {
-        a.occ_lo[i] = s->occ_lo[i];
-        a.occ_hi[i] = s->occ_hi[i];
-    }
-    a.cell_start = s->d_cell_start;
-    a.cell_items = s->d_cell_items;
+    set_trace_grid(a, s, g);
     a.geo = s->d_geo;
     a.recv_type = s->recv_type;
     a.cyl_base = s->cyl_base;
@@ -980,9 +1183,29 @@ static om_status scene_from_desc(om_ctx *ctx, const om_scene_desc *d, const om_p
     s->recv_pixel_length = d->recv_pixel_length;
     s->recv_res_in[0] = d->recv_res_x;
     s->recv_res_in[1] = d->recv_res_y;
-    std::memcpy(s->grid_pos, d->grid_pos, 12);
-    std::memcpy(s->grid_size, d->grid_size, 12);
-    std::memcpy(s->grid_interval, d->grid_interval, 12);
+    // grids: every grid over its heliostats, or grid k alone with its heliostats as the
+    // scene's (the reference traces grid 0's heliostats over grid 0, Gtest.cu:12,21-23)
+    std::vector<om_grid_desc> gl = desc_grids(d);
+    if (p->grid_select != OM_GRIDS_ALL) {
+        const om_grid_desc k = gl[p->grid_select];
+        const size_t f = (size_t)k.first_helio;
+        s->H = k.n_helio;
+        s->pos.assign(d->helio_pos + 3 * f, d->helio_pos + 3 * (f + s->H));
+        s->size.assign(d->helio_size + 3 * f, d->helio_size + 3 * (f + s->H));
+        s->gap.assign(d->helio_gap + 2 * f, d->helio_gap + 2 * (f + s->H));
+        s->matrix.assign(d->helio_matrix + 2 * f, d->helio_matrix + 2 * (f + s->H));
+        gl = {k};
+        gl[0].first_helio = 0;
+    }
+    for (const om_grid_desc &g : gl) {
+        om::GridState G;
+        std::memcpy(G.pos, g.pos, 12);
+        std::memcpy(G.size, g.size, 12);
+        std::memcpy(G.interval, g.interval, 12);
+        G.h0 = g.first_helio;
+        G.h1 = g.first_helio + g.n_helio;
+        s->grids.push_back(std::move(G));
+    }
     std::memcpy(s->sun_in, d->sun_dir, 12);
     s->dni = d->dni;
     st = build_all(s);
@@ -1035,9 +1258,14 @@ extern "C" om_status om_scene_get_info(const om_scene *s, om_scene_info *info)
     info->n_helio = s->H;
     info->res_x = s->res_x;
     info->res_y = s->res_y;
-    for (int a = 0; a < 3; ++a) info->grid_num[a] = s->gn[a];
-    info->grid_cells = s->cells;
-    info->grid_nnz = s->nnz;
+    for (int a = 0; a < 3; ++a) info->grid_num[a] = s->grids[0].gn[a];
+    info->grid_cells = 0;
+    info->grid_nnz = 0;
+    for (const om::GridState &G : s->grids) {
+        info->grid_cells += G.cells;
+        info->grid_nnz += G.nnz;
+    }
+    info->n_grids = (int)s->grids.size();
     info->total_micro = s->total_micro;
     info->total_rays = s->total_rays;
     info->max_micro = s->max_micro;
@@ -1096,12 +1324,17 @@ extern "C" om_status om_scene_fork(const om_scene *src, const float sun_dir[3], 
     }
     // copy everything but the host CSR, which stays with the root
     om_scene &m = const_cast<om_scene &>(*src);
-    std::vector<int> cs, ci;
-    cs.swap(m.cell_start);
-    ci.swap(m.cell_items);
+    const size_t ng = m.grids.size();
+    std::vector<std::vector<int>> cs(ng), ci(ng);
+    for (size_t g = 0; g < ng; ++g) {
+        cs[g].swap(m.grids[g].cell_start);
+        ci[g].swap(m.grids[g].cell_items);
+    }
     om_scene *s = new om_scene(*src);
-    m.cell_start.swap(cs);
-    m.cell_items.swap(ci);
+    for (size_t g = 0; g < ng; ++g) {
+        m.grids[g].cell_start.swap(cs[g]);
+        m.grids[g].cell_items.swap(ci[g]);
+    }
     s->grid_src = src->grid_src ? src->grid_src : src;
     s->d_helio = nullptr;
     s->d_geo = nullptr;
@@ -1139,14 +1372,36 @@ extern "C" om_status om_scene_export(const om_scene *s, const char *what, void *
     else if (w == "helio_v0") put(s->v0.data(), 12 * (size_t)s->H);
     else if (w == "helio_v1") put(s->v1.data(), 12 * (size_t)s->H);
     else if (w == "helio_v3") put(s->v3.data(), 12 * (size_t)s->H);
-    else if (w == "grid_start") {
-        const om_scene *g = s->grid_src ? s->grid_src : s;
-        put(g->cell_start.data(), 4 * g->cell_start.size());
-    } else if (w == "grid_items") {
-        const om_scene *g = s->grid_src ? s->grid_src : s;
-        put(g->cell_items.data(), 4 * (size_t)g->nnz);
-    }
-    else if (w == "receiver") {
+    else if (w.compare(0, 5, "grid_") == 0) {
+        // grid_start / grid_items / grid_num / grid_helio, of grid 0 or of grid K (".K")
+        std::string base = w;
+        int k = 0;
+        const size_t dot = w.find('.');
+        if (dot != std::string::npos) {
+            base = w.substr(0, dot);
+            const std::string num = w.substr(dot + 1);
+            if (num.empty() || num.size() > 6 || num.find_first_not_of("0123456789") != std::string::npos) {
+                om::set_error("unknown export " + w);
+                return OM_ERR_INVALID;
+            }
+            k = std::stoi(num);
+        }
+        if (k >= (int)s->grids.size()) {
+            om::set_error("export " + w + ": no such grid");
+            return OM_ERR_INVALID;
+        }
+        const om::GridState &G = (s->grid_src ? s->grid_src : s)->grids[k];
+        if (base == "grid_start") put(G.cell_start.data(), 4 * G.cell_start.size());
+        else if (base == "grid_items") put(G.cell_items.data(), 4 * (size_t)G.nnz);
+        else if (base == "grid_num") put(G.gn, 12);
+        else if (base == "grid_helio") {
+            const int hn[2] = {G.h0, G.h1 - G.h0};
+            put(hn, 8);
+        } else {
+            om::set_error("unknown export " + w);
+            return OM_ERR_INVALID;
+        }
+    } else if (w == "receiver") {
         put(s->rv, 48);
         put(&s->focus, 12);
     } else if (w == "buie") {

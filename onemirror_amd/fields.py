@@ -22,6 +22,9 @@ Workloads (heliostat counts, sizes, sunshape, microsamples, grid and map sizes):
   small_cylinder  100 heliostats 2x2 m around a cylinder r = 2 m, h = 4 m at 30 m
            (128x64 map): the cylinder path at test size.
   batched  64 scenes of 500-20,000 heliostats under seeded suns (config_batched).
+  two_grids  two sub-fields of 50 heliostats 2x2 m (north-west and north-east of the tower,
+           radii 40-80 m) in two 64^3 grids of their own, the receiver and sun of small:
+           the multi-grid scene of the .scn format.
   face2face  the single-heliostat scene of userData/face2face_shadow.scn.
 """
 from __future__ import annotations
@@ -152,6 +155,40 @@ def config_small(seed: int = 1) -> tuple:
                pillbox_half_angle=4.65e-3, csr=0.1, buie_bins=1024, slope_sigma=2e-3,
                micro_mode=POISSON, helio_pixel_length=2.0 / 16.0, poisson_k=8,
                rays_per_sample=64, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
+    return desc, prm
+
+
+def config_two_grids(seed: int = 7, n_each: int = 50) -> tuple:
+    """Two sub-fields, each in a grid of its own (om_grid_desc): heliostats 0..n_each-1
+    north-west of the tower (azimuths 270-330 deg) in grid 0, the rest north-east (30-90
+    deg) in grid 1.  The grids are more than a cell apart, so neither grid bins a
+    heliostat of the other."""
+    desc, prm = config_small(seed)
+    rng = np.random.Generator(np.random.Philox(seed))
+    parts = []
+    for centre in (300.0, 60.0):
+        xz = radial_stagger(n_each, 40.0, 80.0, 3.0, 60.0, centre)[:n_each]
+        if len(xz) < n_each:
+            raise ValueError("sub-field annulus too small for the heliostat count")
+        xz = xz + rng.uniform(-0.1, 0.1, size=xz.shape)
+        pos = np.zeros((n_each, 3), np.float32)
+        pos[:, 0] = xz[:, 0]
+        pos[:, 1] = 1.2
+        pos[:, 2] = xz[:, 1]
+        parts.append(pos)
+    pos = np.concatenate(parts)
+    n = pos.shape[0]
+    size = np.tile(np.array([2.0, 0.1, 2.0], np.float32), (n, 1))
+    desc.update(helio_pos=pos, helio_size=size, helio_gap=np.zeros((n, 2), np.float32),
+                helio_matrix=np.ones((n, 2), np.int32))
+    grids = []
+    for k in range(2):
+        sl = slice(k * n_each, (k + 1) * n_each)
+        g = grid_for(pos[sl], size[sl], 8.0, 64)
+        grids.append({"pos": g["grid_pos"], "size": g["grid_size"], "interval": g["grid_interval"],
+                      "first_helio": k * n_each, "n_helio": n_each})
+    desc.update(grid_pos=grids[0]["pos"], grid_size=grids[0]["size"], grid_interval=grids[0]["interval"],
+                grids=grids)
     return desc, prm
 
 

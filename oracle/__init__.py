@@ -27,6 +27,16 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _f64p = ctypes.POINTER(ctypes.c_double)
 
 
+class GridDescC(ctypes.Structure):
+    _fields_ = [
+        ("pos", ctypes.c_float * 3),
+        ("size", ctypes.c_float * 3),
+        ("interval", ctypes.c_float * 3),
+        ("first_helio", ctypes.c_int),
+        ("n_helio", ctypes.c_int),
+    ]
+
+
 class SceneDescC(ctypes.Structure):
     _fields_ = [
         ("n_helio", ctypes.c_int),
@@ -47,6 +57,8 @@ class SceneDescC(ctypes.Structure):
         ("dni", ctypes.c_float),
         ("recv_res_x", ctypes.c_int),
         ("recv_res_y", ctypes.c_int),
+        ("n_grids", ctypes.c_int),
+        ("grids", ctypes.POINTER(GridDescC)),
     ]
 
 
@@ -95,6 +107,8 @@ def lib():
     L.oc_scene_counts.argtypes = [ctypes.c_void_p, _i64p]
     L.oc_scene_helio.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]
     L.oc_scene_grid.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i32p]
+    L.oc_scene_grid_k.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, _i32p, _i64p]
+    L.oc_scene_grid_k.restype = ctypes.c_int
     L.oc_scene_receiver.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p]
     L.oc_scene_sun.argtypes = [ctypes.c_void_p, _f32p, _f32p]
     L.oc_scene_buie_table.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p]
@@ -206,6 +220,18 @@ def _desc_struct(desc: dict, keep: list):
     d.dni = float(desc["dni"])
     d.recv_res_x = int(desc.get("recv_res_x", 0))
     d.recv_res_y = int(desc.get("recv_res_y", 0))
+    grids = desc.get("grids") or []
+    if grids:
+        arr = (GridDescC * len(grids))()
+        for i, g in enumerate(grids):
+            arr[i].pos = _f3(g["pos"])
+            arr[i].size = _f3(g["size"])
+            arr[i].interval = _f3(g["interval"])
+            arr[i].first_helio = int(g["first_helio"])
+            arr[i].n_helio = int(g["n_helio"])
+        keep.append(arr)
+        d.n_grids = len(grids)
+        d.grids = ctypes.cast(arr, ctypes.POINTER(GridDescC))
     return d
 
 
@@ -258,12 +284,17 @@ class OracleScene:
         lib().oc_scene_helio(self._h, _p(n, _f32p), _p(v0, _f32p), _p(v1, _f32p), _p(v3, _f32p))
         return {"normal": n, "v0": v0, "v1": v1, "v3": v3}
 
-    def grid(self):
-        cs = np.zeros(self.cells + 1, np.int32)
-        ci = np.zeros(max(self.nnz, 1), np.int32)
+    def grid(self, k: int = 0):
+        """CSR of grid k: cell_start [cells+1], cell_items [nnz] (global heliostat ids)."""
         gn = np.zeros(3, np.int32)
-        lib().oc_scene_grid(self._h, _p(cs, _i32p), _p(ci, _i32p), _p(gn, _i32p))
-        return {"cell_start": cs, "cell_items": ci[: self.nnz], "grid_num": gn}
+        nnz = np.zeros(1, np.int64)
+        if lib().oc_scene_grid_k(self._h, int(k), None, None, _p(gn, _i32p), _p(nnz, _i64p)) != 0:
+            raise IndexError(f"no grid {k}")
+        cells = int(np.prod(gn.astype(np.int64)))
+        cs = np.zeros(cells + 1, np.int32)
+        ci = np.zeros(max(int(nnz[0]), 1), np.int32)
+        lib().oc_scene_grid_k(self._h, int(k), _p(cs, _i32p), _p(ci, _i32p), _p(gn, _i32p), None)
+        return {"cell_start": cs, "cell_items": ci[: int(nnz[0])], "grid_num": gn}
 
     def receiver(self):
         v = np.zeros((4, 3), np.float32)

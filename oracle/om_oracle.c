@@ -205,20 +205,28 @@ void oc_philox_batch(const uint32_t *ctr, const uint32_t key[2], uint32_t *out, 
 /* ------------------------------------------------------------------------------------ */
 /* scene                                                                                 */
 /* ------------------------------------------------------------------------------------ */
+/* one grid (grid.cpp:3-82) over the heliostats [h0, h1) */
+typedef struct {
+    f3 pos, interval;
+    int h0, h1;
+    int gn[3];
+    int64_t cells, nnz;
+    int32_t *cell_start, *cell_items;
+    int occ_lo[3], occ_hi[3];
+} oc_grid;
+
 struct oc_scene {
     oc_scene_desc d;
     oc_params p;
     int H;
+    /* grids (every grid of the description; a heliostat's rays walk its own) */
+    int ngrids;
+    oc_grid *grids;
+    int *helio_grid;          /* [H] index of the grid holding each heliostat */
     /* heliostats (heliostat.cpp:16-62) */
     f3 *pos, *size, *normal, *v0, *v1, *v2, *v3;
     float *gap;   /* [H][2] */
     int *matrix;  /* [H][2] */
-    /* grid (grid.cpp:3-82) */
-    int gn[3];
-    int64_t cells;
-    int32_t *cell_start, *cell_items;
-    int64_t nnz;
-    int occ_lo[3], occ_hi[3];
     /* receiver (receiver.cpp:24-117) */
     f3 rv[4], recv_normal, focus;
     float recv_pixel;
@@ -302,21 +310,25 @@ static int max_less_index(const float *cdf_y, float value, int n)
     return right;
 }
 
-/* RectGrid::Cinit + CGridHelioMatch + boxIntersect (grid.cpp:3-82).  Cell ranges are
- * clamped to the grid (out-of-grid binning is undefined behaviour in the reference). */
-static int build_grid(oc_scene *s)
+/* RectGrid::Cinit + CGridHelioMatch + boxIntersect (grid.cpp:3-82) of one grid over its
+ * heliostats [g->h0, g->h1), with global ids (grid.cpp:53).  Cell ranges are clamped to the
+ * grid (out-of-grid binning is undefined behaviour in the reference). */
+static int build_grid(oc_scene *s, oc_grid *g, f3 gpos, f3 gsize, f3 gint)
 {
-    f3 gpos = ld3(s->d.grid_pos), gsize = ld3(s->d.grid_size), gint = ld3(s->d.grid_interval);
-    s->gn[0] = (int)ceilf(gsize.x / gint.x);
-    s->gn[1] = (int)ceilf(gsize.y / gint.y);
-    s->gn[2] = (int)ceilf(gsize.z / gint.z);
-    if (s->gn[0] <= 0 || s->gn[1] <= 0 || s->gn[2] <= 0) return -1;
-    s->cells = (int64_t)s->gn[0] * s->gn[1] * s->gn[2];
-    int32_t *cnt = (int32_t *)calloc((size_t)s->cells, sizeof(int32_t));
-    int (*rng)[6] = malloc(sizeof(int[6]) * (size_t)s->H);
+    g->pos = gpos;
+    g->interval = gint;
+    g->gn[0] = (int)ceilf(gsize.x / gint.x);
+    g->gn[1] = (int)ceilf(gsize.y / gint.y);
+    g->gn[2] = (int)ceilf(gsize.z / gint.z);
+    if (g->gn[0] <= 0 || g->gn[1] <= 0 || g->gn[2] <= 0) return -1;
+    g->cells = (int64_t)g->gn[0] * g->gn[1] * g->gn[2];
+    int32_t *cnt = (int32_t *)calloc((size_t)g->cells, sizeof(int32_t));
+    int nh = g->h1 - g->h0;
+    int (*rng)[6] = malloc(sizeof(int[6]) * (size_t)(nh > 0 ? nh : 1));
     int64_t nnz = 0;
-    for (int i = 0; i < 3; ++i) { s->occ_lo[i] = 1 << 30; s->occ_hi[i] = -1; }
-    for (int i = 0; i < s->H; ++i) {
+    for (int i = 0; i < 3; ++i) { g->occ_lo[i] = 1 << 30; g->occ_hi[i] = -1; }
+    for (int i = g->h0; i < g->h1; ++i) {
+        int *r = rng[i - g->h0];
         float diag = sqrtf(dot3(s->size[i], s->size[i]));
         float radius = sqrtf(diag) / 2;
         f3 mn = mk3(s->pos[i].x - radius, s->pos[i].y - radius, s->pos[i].z - radius);
@@ -327,40 +339,68 @@ static int build_grid(oc_scene *s)
                       trunc_i(sub3(mx, gpos).z / gint.z) };
         for (int a = 0; a < 3; ++a) {
             if (lo[a] < 0) lo[a] = 0;
-            if (hi[a] > s->gn[a] - 1) hi[a] = s->gn[a] - 1;
-            rng[i][a] = lo[a];
-            rng[i][3 + a] = hi[a];
+            if (hi[a] > g->gn[a] - 1) hi[a] = g->gn[a] - 1;
+            r[a] = lo[a];
+            r[3 + a] = hi[a];
         }
         for (int x = lo[0]; x <= hi[0]; ++x)
             for (int y = lo[1]; y <= hi[1]; ++y)
                 for (int z = lo[2]; z <= hi[2]; ++z) {
-                    int64_t c = (int64_t)x * s->gn[1] * s->gn[2] + (int64_t)y * s->gn[2] + z;
+                    int64_t c = (int64_t)x * g->gn[1] * g->gn[2] + (int64_t)y * g->gn[2] + z;
                     cnt[c]++;
                     nnz++;
                 }
         if (lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2])
             for (int a = 0; a < 3; ++a) {
-                if (lo[a] < s->occ_lo[a]) s->occ_lo[a] = lo[a];
-                if (hi[a] > s->occ_hi[a]) s->occ_hi[a] = hi[a];
+                if (lo[a] < g->occ_lo[a]) g->occ_lo[a] = lo[a];
+                if (hi[a] > g->occ_hi[a]) g->occ_hi[a] = hi[a];
             }
     }
-    s->nnz = nnz;
-    s->cell_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(s->cells + 1));
-    s->cell_items = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
-    s->cell_start[0] = 0;
-    for (int64_t c = 0; c < s->cells; ++c) s->cell_start[c + 1] = s->cell_start[c] + cnt[c];
-    memset(cnt, 0, sizeof(int32_t) * (size_t)s->cells);
+    g->nnz = nnz;
+    g->cell_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(g->cells + 1));
+    g->cell_items = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    g->cell_start[0] = 0;
+    for (int64_t c = 0; c < g->cells; ++c) g->cell_start[c + 1] = g->cell_start[c] + cnt[c];
+    memset(cnt, 0, sizeof(int32_t) * (size_t)g->cells);
     /* items in push order: heliostat index ascending within each cell */
-    for (int i = 0; i < s->H; ++i)
-        for (int x = rng[i][0]; x <= rng[i][3]; ++x)
-            for (int y = rng[i][1]; y <= rng[i][4]; ++y)
-                for (int z = rng[i][2]; z <= rng[i][5]; ++z) {
-                    int64_t c = (int64_t)x * s->gn[1] * s->gn[2] + (int64_t)y * s->gn[2] + z;
-                    s->cell_items[s->cell_start[c] + cnt[c]++] = i;
+    for (int i = g->h0; i < g->h1; ++i) {
+        const int *r = rng[i - g->h0];
+        for (int x = r[0]; x <= r[3]; ++x)
+            for (int y = r[1]; y <= r[4]; ++y)
+                for (int z = r[2]; z <= r[5]; ++z) {
+                    int64_t c = (int64_t)x * g->gn[1] * g->gn[2] + (int64_t)y * g->gn[2] + z;
+                    g->cell_items[g->cell_start[c] + cnt[c]++] = i;
                 }
+    }
     free(cnt);
     free(rng);
     return 0;
+}
+
+/* every grid of the description, or the one grid of grid_* holding every heliostat */
+static int build_grids(oc_scene *s)
+{
+    const oc_scene_desc *d = &s->d;
+    int n = d->n_grids > 0 ? d->n_grids : 1;
+    s->ngrids = n;
+    s->grids = (oc_grid *)calloc((size_t)n, sizeof(oc_grid));
+    s->helio_grid = (int *)malloc(sizeof(int) * (size_t)(s->H + 1));
+    int next = 0;
+    for (int k = 0; k < n; ++k) {
+        const float *gp = d->n_grids > 0 ? d->grids[k].pos : d->grid_pos;
+        const float *gs = d->n_grids > 0 ? d->grids[k].size : d->grid_size;
+        const float *gi = d->n_grids > 0 ? d->grids[k].interval : d->grid_interval;
+        int first = d->n_grids > 0 ? d->grids[k].first_helio : 0;
+        int cnt = d->n_grids > 0 ? d->grids[k].n_helio : s->H;
+        if (first != next || cnt < 0 || first + cnt > s->H) return -6;
+        s->grids[k].h0 = first;
+        s->grids[k].h1 = first + cnt;
+        for (int i = first; i < first + cnt; ++i) s->helio_grid[i] = k;
+        next += cnt;
+        int rc = build_grid(s, &s->grids[k], ld3(gp), ld3(gs), ld3(gi));
+        if (rc) return rc;
+    }
+    return next == s->H ? 0 : -6;
 }
 
 /* Cylinder receiver (new; CylinderReceiver is an empty stub, receiver.cuh:103-122): axis
@@ -650,7 +690,7 @@ int oc_scene_build(const oc_scene_desc *desc, const oc_params *prm, oc_scene **o
     /* InitContent order (solar_scene.cpp:83-99): sun, grid, receiver, heliostats */
     s->sun = normalize3(ld3(desc->sun_dir));
     build_buie(s);
-    int rc = build_grid(s);
+    int rc = build_grids(s);
     if (!rc) rc = build_receiver(s);
     if (rc) { oc_scene_free(s); return rc; }
     build_heliostats(s);
@@ -666,7 +706,11 @@ void oc_scene_free(oc_scene *s)
     free(s->pos); free(s->size); free(s->normal);
     free(s->v0); free(s->v1); free(s->v2); free(s->v3);
     free(s->gap); free(s->matrix);
-    free(s->cell_start); free(s->cell_items);
+    for (int k = 0; k < s->ngrids && s->grids; ++k) {
+        free(s->grids[k].cell_start);
+        free(s->grids[k].cell_items);
+    }
+    free(s->grids); free(s->helio_grid);
     free(s->cdf_x); free(s->cdf_y); free(s->slope);
     free(s->mcount); free(s->ray_offset); free(s->poisson_pts);
     free(s->subrow_len); free(s->subcol_len); free(s->sub_rows); free(s->sub_cols);
@@ -675,7 +719,9 @@ void oc_scene_free(oc_scene *s)
 
 int oc_scene_counts(const oc_scene *s, int64_t out[8])
 {
-    out[0] = s->H; out[1] = s->cells; out[2] = s->nnz; out[3] = s->res_x; out[4] = s->res_y;
+    int64_t cells = 0, nnz = 0;
+    for (int k = 0; k < s->ngrids; ++k) { cells += s->grids[k].cells; nnz += s->grids[k].nnz; }
+    out[0] = s->H; out[1] = cells; out[2] = nnz; out[3] = s->res_x; out[4] = s->res_y;
     out[5] = s->total_micro; out[6] = s->total_rays; out[7] = s->max_micro;
     return 0;
 }
@@ -690,11 +736,21 @@ void oc_scene_helio(const oc_scene *s, float *normal, float *v0, float *v1, floa
     }
 }
 
+int oc_scene_grid_k(const oc_scene *s, int k, int32_t *cell_start, int32_t *cell_items, int32_t grid_num[3],
+                    int64_t *nnz)
+{
+    if (k < 0 || k >= s->ngrids) return -1;
+    const oc_grid *g = &s->grids[k];
+    if (cell_start) memcpy(cell_start, g->cell_start, sizeof(int32_t) * (size_t)(g->cells + 1));
+    if (cell_items) memcpy(cell_items, g->cell_items, sizeof(int32_t) * (size_t)g->nnz);
+    if (grid_num) { grid_num[0] = g->gn[0]; grid_num[1] = g->gn[1]; grid_num[2] = g->gn[2]; }
+    if (nnz) *nnz = g->nnz;
+    return 0;
+}
+
 void oc_scene_grid(const oc_scene *s, int32_t *cell_start, int32_t *cell_items, int32_t grid_num[3])
 {
-    if (cell_start) memcpy(cell_start, s->cell_start, sizeof(int32_t) * (size_t)(s->cells + 1));
-    if (cell_items) memcpy(cell_items, s->cell_items, sizeof(int32_t) * (size_t)s->nnz);
-    if (grid_num) { grid_num[0] = s->gn[0]; grid_num[1] = s->gn[1]; grid_num[2] = s->gn[2]; }
+    oc_scene_grid_k(s, 0, cell_start, cell_items, grid_num, NULL);
 }
 
 void oc_scene_receiver(const oc_scene *s, float *v4x3, float focus[3], float *pixel_length)
@@ -815,10 +871,10 @@ static inline float abs_divide(float den, float num)
 /* Intersect (rectgrid_raytracing.cu:21-36): first candidate of the cell's CSR list hit by
  * the ray, skipping the ray's own heliostat (exact-arithmetic reading of the t >= 1e-6
  * guard; see the module documentation). */
-static int cell_hit(const oc_scene *s, f3 o, f3 d, int64_t addr, int self, dda_ctr *k)
+static int cell_hit(const oc_scene *s, const oc_grid *g, f3 o, f3 d, int64_t addr, int self, dda_ctr *k)
 {
-    for (int32_t i = s->cell_start[addr]; i < s->cell_start[addr + 1]; ++i) {
-        int h = s->cell_items[i];
+    for (int32_t i = g->cell_start[addr]; i < g->cell_start[addr + 1]; ++i) {
+        int h = g->cell_items[i];
         if (h == self) { k->self_skips++; continue; }
         k->tests++;
         float t, u, v;
@@ -831,10 +887,17 @@ static int cell_hit(const oc_scene *s, f3 o, f3 d, int64_t addr, int self, dda_c
  * when the ray leaves the grid.  With early_exit, the walk stops as soon as the cell index
  * on some axis has passed the occupied index range in the direction of travel: every later
  * cell is then empty, so the result is unchanged. */
+static const oc_grid *grid_of(const oc_scene *s, int h)
+{
+    return &s->grids[(h >= 0 && h < s->H) ? s->helio_grid[h] : 0];
+}
+
 static int dda(const oc_scene *s, f3 o, f3 d, int self, int early_exit, dda_ctr *k,
                int32_t *trace, int cap, int *ntrace)
 {
-    f3 gpos = ld3(s->d.grid_pos), gint = ld3(s->d.grid_interval);
+    /* the grid of the ray's heliostat (grid 0 for a probe without one) */
+    const oc_grid *G = grid_of(s, self);
+    f3 gpos = G->pos, gint = G->interval;
     f3 rel = sub3(o, gpos);
     int px = trunc_i(rel.x / gint.x), py = trunc_i(rel.y / gint.y), pz = trunc_i(rel.z / gint.z);
     int sx = (d.x >= 0) ? 1 : -1, sy = (d.y >= 0) ? 1 : -1, sz = (d.z >= 0) ? 1 : -1;
@@ -852,32 +915,32 @@ static int dda(const oc_scene *s, f3 o, f3 d, int self, int early_exit, dda_ctr 
         else axis = (my < mz) ? 1 : 2;
         if (axis == 0) {
             gx += sx;
-            if (gx >= s->gn[0] || gx < 0) return -1;
+            if (gx >= G->gn[0] || gx < 0) return -1;
             mx += dx;
         } else if (axis == 1) {
             gy += sy;
-            if (gy >= s->gn[1] || gy < 0) return -1;
+            if (gy >= G->gn[1] || gy < 0) return -1;
             my += dy;
         } else {
             gz += sz;
-            if (gz >= s->gn[2] || gz < 0) return -1;
+            if (gz >= G->gn[2] || gz < 0) return -1;
             mz += dz;
         }
         k->steps++;
-        int past = (sx > 0 ? gx > s->occ_hi[0] : gx < s->occ_lo[0]) ||
-                   (sy > 0 ? gy > s->occ_hi[1] : gy < s->occ_lo[1]) ||
-                   (sz > 0 ? gz > s->occ_hi[2] : gz < s->occ_lo[2]);
+        int past = (sx > 0 ? gx > G->occ_hi[0] : gx < G->occ_lo[0]) ||
+                   (sy > 0 ? gy > G->occ_hi[1] : gy < G->occ_lo[1]) ||
+                   (sz > 0 ? gz > G->occ_hi[2] : gz < G->occ_lo[2]);
         if (past) {
             if (!eff_done) { eff_done = 1; k->steps_eff++; }
             if (early_exit) return -1;
         } else if (!eff_done) {
             k->steps_eff++;
         }
-        int64_t addr = (int64_t)gx * s->gn[1] * s->gn[2] + (int64_t)gy * s->gn[2] + gz;
+        int64_t addr = (int64_t)gx * G->gn[1] * G->gn[2] + (int64_t)gy * G->gn[2] + gz;
         if (trace && n < cap) trace[n] = (int32_t)addr;
         n++;
         if (ntrace) *ntrace = n;
-        int h = cell_hit(s, o, d, addr, self, k);
+        int h = cell_hit(s, G, o, d, addr, self, k);
         if (h >= 0) return h;
     }
 }
@@ -1136,8 +1199,9 @@ static void free64(oc_scene64 *q)
 
 static int dda64(const oc_scene *s, const oc_scene64 *q, d3 o, d3 d, int self)
 {
-    d3 gpos = mkd(s->d.grid_pos[0], s->d.grid_pos[1], s->d.grid_pos[2]);
-    d3 gint = mkd(s->d.grid_interval[0], s->d.grid_interval[1], s->d.grid_interval[2]);
+    const oc_grid *G = grid_of(s, self);
+    d3 gpos = f2d(G->pos);
+    d3 gint = f2d(G->interval);
     d3 rel = subd(o, gpos);
     int g[3] = { (int)(rel.x / gint.x), (int)(rel.y / gint.y), (int)(rel.z / gint.z) };
     double dd[3] = { d.x, d.y, d.z }, gi[3] = { gint.x, gint.y, gint.z }, cur[3] = { rel.x, rel.y, rel.z };
@@ -1154,15 +1218,15 @@ static int dda64(const oc_scene *s, const oc_scene64 *q, d3 o, d3 d, int self)
         if (tm[0] < tm[1]) a = (tm[0] < tm[2]) ? 0 : 2;
         else a = (tm[1] < tm[2]) ? 1 : 2;
         g[a] += st[a];
-        if (g[a] >= s->gn[a] || g[a] < 0) return -1;
+        if (g[a] >= G->gn[a] || g[a] < 0) return -1;
         tm[a] += td[a];
-        if ((st[0] > 0 ? g[0] > s->occ_hi[0] : g[0] < s->occ_lo[0]) ||
-            (st[1] > 0 ? g[1] > s->occ_hi[1] : g[1] < s->occ_lo[1]) ||
-            (st[2] > 0 ? g[2] > s->occ_hi[2] : g[2] < s->occ_lo[2]))
+        if ((st[0] > 0 ? g[0] > G->occ_hi[0] : g[0] < G->occ_lo[0]) ||
+            (st[1] > 0 ? g[1] > G->occ_hi[1] : g[1] < G->occ_lo[1]) ||
+            (st[2] > 0 ? g[2] > G->occ_hi[2] : g[2] < G->occ_lo[2]))
             return -1;
-        int64_t addr = (int64_t)g[0] * s->gn[1] * s->gn[2] + (int64_t)g[1] * s->gn[2] + g[2];
-        for (int32_t i = s->cell_start[addr]; i < s->cell_start[addr + 1]; ++i) {
-            int h = s->cell_items[i];
+        int64_t addr = (int64_t)g[0] * G->gn[1] * G->gn[2] + (int64_t)g[1] * G->gn[2] + g[2];
+        for (int32_t i = G->cell_start[addr]; i < G->cell_start[addr + 1]; ++i) {
+            int h = G->cell_items[i];
             if (h == self) continue;
             double t, u, v;
             if (ray_parallelogramd(o, d, q->v0[h], q->v1[h], q->v3[h], &t, &u, &v)) return h;

@@ -24,6 +24,16 @@
 extern "C" {
 #endif
 
+/* ---- one grid of a field (RectGrid, grid.h:25-49) and its heliostats
+ *      [first_helio, first_helio + n_helio) (start_helio_pos_, num_helios_) ---- */
+typedef struct {
+    float pos[3];
+    float size[3];
+    float interval[3];
+    int   first_helio;
+    int   n_helio;
+} oc_grid_desc;
+
 /* ---- scene description: the inputs a .scn file carries (scene_file_proc.cpp:41-286)
  *      plus the sun of SunRay (sunray.h:4-60). All arrays are host memory. ---- */
 typedef struct {
@@ -45,6 +55,10 @@ typedef struct {
     float sun_dir[3];          /* direction light travels (main.cpp:71-73)              */
     float dni;
     int   recv_res_x, recv_res_y; /* cylinder columns (azimuth), rows; 0 = from pixel length */
+    int   n_grids;             /* 0: one grid (grid_* above) holding every heliostat;    */
+    const oc_grid_desc *grids; /* else grids[k] hold consecutive ranges covering [0, n),
+                                  each heliostat's rays walking its own grid over global
+                                  ids (recthelio_tracing.cpp:46-79, grid.cpp:53)          */
 } oc_scene_desc;
 
 /* ---- tracing parameters: the solarenergy:: globals (common_var.cpp:9-27) plus the
@@ -97,6 +111,9 @@ int  oc_scene_counts(const oc_scene *s, int64_t out[8]);
    out[5]=total microsamples out[6]=total rays out[7]=max microsamples per heliostat */
 void oc_scene_helio(const oc_scene *s, float *normal, float *v0, float *v1, float *v3);
 void oc_scene_grid(const oc_scene *s, int32_t *cell_start, int32_t *cell_items, int32_t grid_num[3]);
+/* grid k: CSR, cells per axis, nnz; returns -1 for no such grid */
+int  oc_scene_grid_k(const oc_scene *s, int k, int32_t *cell_start, int32_t *cell_items, int32_t grid_num[3],
+                     int64_t *nnz);
 void oc_scene_receiver(const oc_scene *s, float *v4x3, float focus[3], float *pixel_length);
 void oc_scene_sun(const oc_scene *s, float sun_dir[3], float *buie_consts /*[6]*/);
 int  oc_scene_buie_table(const oc_scene *s, float *cdf_x, float *cdf_y, float *slope);

@@ -138,17 +138,29 @@ def test_scene_face2face_matches_oracle():
         _compare_scene(dd, p)
 
 
+def _same_desc(a: dict, b: dict):
+    """Descriptions equal key by key, the grid lists entry by entry."""
+    for k in a:
+        if k == "grids":
+            assert len(a[k]) == len(b[k]), k
+            for ga, gb in zip(a[k], b[k]):
+                for f in ga:
+                    assert np.array_equal(np.asarray(ga[f]), np.asarray(gb[f])), (k, f, ga[f], gb[f])
+        else:
+            assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), (k, a[k], b[k])
+
+
 def test_scn_parser_matches_restatement(tmp_path):
     a = oracle.scn.parse_file(SCN)
     b = om.scn_parse(SCN)
-    for k in a:
-        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    _same_desc(a, b)
+    assert len(b["grids"]) == 1 and b["grids"][0]["n_helio"] == 1
     # quirks: unknown key reads as "pos"; a blank-only line is the key "" (= pos);
     # heliostats before gap/matrix are skipped; failed numbers read as 0; size on next line
     txt = "\n".join([
         "# comment", "ground 10 10", "ngrid 1", "",
         "Recv 0", "pos 1 2 3", "size 4 4 0.5", "norm 0 0 1", "face 2", "end",
-        "Grid 0", "pos -50 0 -50", "size 100 20 100", "inter 5 5 5", "n 3", "type 0", "end",
+        "Grid 0", "pos -50 0 -50", "size 100 20 100", "inter 5 5 5", "n 2", "type 0", "end",
         "helio 1 1 1", "2 0.1 2",
         "gap 0.1 0.2", "matrix 2 x",
         "helio 3 1.5 -7", "\t2.5 0.1 2.5",
@@ -160,8 +172,7 @@ def test_scn_parser_matches_restatement(tmp_path):
     p.write_text(txt)
     a = oracle.scn.parse_file(str(p))
     b = om.scn_parse(str(p))
-    for k in a:
-        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), (k, a[k], b[k])
+    _same_desc(a, b)
     assert b["helio_pos"].shape == (2, 3)
     assert np.array_equal(b["helio_matrix"], np.int32([[2, 0], [2, 0]]))
     assert np.array_equal(b["helio_pos"][1], np.float32([4, 1.5, 0]))
@@ -184,12 +195,23 @@ def test_abi_error_paths(tmp_path):
     dd = dict(d)
     dd["grid_interval"] = np.float32([0, 1, 1])
     assert HostScene(dd, p).rc == 1
-    # missing file and a second grid
+    # missing file; files the reference reads into undefined behaviour
     desc = om.SceneDesc()
     assert L.om_scn_parse(b"/nonexistent/x.scn", ctypes.byref(desc)) == 2
-    two = tmp_path / "two.scn"
-    two.write_text("Recv 0\nend\nGrid 0\nend\nGrid 0\nend\n")
-    assert L.om_scn_parse(str(two).encode(), ctypes.byref(desc)) == 6
+    bad = tmp_path / "bad.scn"
+    for text, rc, msg in (
+            ("Recv 0\nend\nGrid 0\nn 2\nend\ngap 0 0\nmatrix 1 1\nhelio 0 1 0\n1 1 1\n", 3, b"declares 2"),
+            ("Recv 0\nend\nGrid 0\nn 1\ntype 1\nend\ngap 0 0\nmatrix 1 1\nhelio 0 1 0\n1 1 1\n", 6,
+             b"heliostat type 1"),
+            ("Recv 5\nend\nGrid 0\nend\n", 3, b"receiver type 5"),
+            ("Recv 0\nend\nGrid 1\npos 0 0 0\nend\n", 6, b"grid type 1"),
+            ("Grid 0\nend\n", 3, b"no receiver"),
+            ("Recv 0\nend\n", 3, b"no grid")):
+        bad.write_text(text)
+        assert L.om_scn_parse(str(bad).encode(), ctypes.byref(desc)) == rc, text
+        assert msg in L.om_last_error(), (text, L.om_last_error())
+        with pytest.raises(ValueError):
+            oracle.scn.parse(text)
     # a host-only scene cannot be traced
     g = HostScene(d, F.with_lattice(p))
     f = om.Flux()
