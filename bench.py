@@ -49,7 +49,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mid", choices=["mid", "small", "large", "batched", "diff", "annual"])
+    ap.add_argument("--workload", default="mid",
+                    choices=["mid", "small", "large", "batched", "diff", "annual", "facet_annual", "mid_grids"])
     ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
     ap.add_argument("--no-grid-tune", action="store_true",
                     help="batched: keep 128^3 grids instead of tuning the resolution")
@@ -68,8 +69,9 @@ def main() -> int:
               "(every GPU box here has one GPU)", file=sys.stderr)
         return 2
 
-    if args.workload in ("batched", "diff", "annual"):
-        out = {"batched": bench_batched, "diff": bench_diff, "annual": bench_annual}[args.workload](args)
+    if args.workload in ("batched", "diff", "annual", "facet_annual"):
+        out = {"batched": bench_batched, "diff": bench_diff, "annual": bench_annual,
+               "facet_annual": bench_facet_annual}[args.workload](args)
         print(json.dumps(out), flush=True)
         return 0
 
@@ -84,6 +86,11 @@ def main() -> int:
         wl = ("mid: 5000 heliostats 10x10 m, radii 100-800 m, flat 25.6 m receiver, "
               "Buie CSR 0.1 (2048-bin CDF), slope 2.5 mrad, 32x32 Poisson x 40 rays, "
               "128^3 grid, 256x256 map, trimmed mean 5x5/20%")
+    elif args.workload == "mid_grids":
+        desc, prm = F.config_mid_grids()
+        prm = dict(prm, grid_select=om.GRIDS_ALL)
+        wl = ("mid_grids: the mid field's heliostats west and east of the tower in two 128^3 "
+              "grids of their own (.scn multi-grid scene), every grid traced")
     elif args.workload == "large":
         desc, prm = F.config_large()
         wl = ("large: 25000 heliostats 12x10 m on a surround field to 1.5 km, cylinder "
@@ -284,6 +291,69 @@ def bench_annual(args) -> dict:
                    "time_steps_per_s": len(steps) / (ms[best] * 1e-3),
                    "accumulate": acc["accumulate"], "annual_ratio": res["annual_ratio"],
                    "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def bench_facet_annual(args) -> dict:
+    """The reference driver's own run (main.cpp:56-116): the faceted paraboloid heliostat
+    rebuilt and traced for every time step of the year with the reference's default
+    parameters, batched and one plain trace per step; the faster is the value."""
+    import torch
+
+    import onemirror_amd as om
+    from onemirror_amd import annual as A
+    from onemirror_amd import fields as F
+
+    site = F.facet_site()
+    t0 = time.time()
+    steps = A.annual_steps()
+    p = om.default_params()
+    # accumulation autotune on the middle step's scene, kept for every step's plain trace
+    mid = steps[len(steps) // 2]
+    probe = om.SolarScene(om.facet_scene(site["a"], site["b"], site["X"], site["Y"], site["Z"], mid["sun_dir"],
+                                         mid["dni"]), p)
+    acc = probe.autotune_accumulate(reps=3)
+    del probe
+    print(f"# facet accumulation autotune: packed {acc['packed_ms']:.3f} ms, window {acc['window_ms']:.3f} ms "
+          f"-> {acc['accumulate']}", file=sys.stderr, flush=True)
+    p_plain = dict(p, accumulate=om.ACC_WINDOW if acc["accumulate"] == "window" else om.ACC_PACKED)
+    runs = {m: A.AnnualRun.facets(site["a"], site["b"], site["X"], site["Y"], site["Z"], steps,
+                                  p if m == "batch" else p_plain, batch_size=len(steps), mode=m)
+            for m in ("batch", "plain")}
+    R = runs["batch"].rays
+    print(f"# facet annual: {len(steps)} time steps, {R} rays ({R // len(steps)} a step), scenes built in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream()
+    ms = {}
+    for m, run in runs.items():
+        for _ in range(max(1, args.warmup)):
+            run.trace()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            run.trace()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms[m] = e0.elapsed_time(e1) / args.steps
+        print(f"# facet annual {m}: {ms[m]:.1f} ms per year ({R / (ms[m] * 1e-3):.4g} rays/s)", file=sys.stderr,
+              flush=True)
+    best = min(ms, key=ms.get)
+    res = runs[best].results()
+    print(f"# facet annual intercept ratio {res['annual_ratio']:.4f}", file=sys.stderr, flush=True)
+    return {
+        "metric": METRIC, "value": R / (ms[best] * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms[best], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": ("facet_annual: the annual driver's faceted paraboloid heliostat (9 facets "
+                                "0.47 m, a = b = %.2f m) at (12, 1.2, 107) m turned to each step's master "
+                                "normal, 1.4 m receiver, reference defaults (Buie CSR 0.1, 1024 bins, slope "
+                                "1 mrad, lattice 0.01 m x 2048 rays); the reference schedule at 30.3N 120.2E, "
+                                "zenith <= 75 deg; one step = the whole year" % site["a"]),
+                   "time_steps": len(steps), "rays": R, "mode": best, "batch_ms": ms["batch"],
+                   "plain_ms": ms["plain"], "time_steps_per_s": len(steps) / (ms[best] * 1e-3),
+                   "accumulate": acc, "annual_ratio": res["annual_ratio"], "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
     }
 

@@ -80,6 +80,10 @@ typedef struct {
     const om_grid_desc *grids; /* [n_grids]                                               */
     float ground_length, ground_width; /* "ground" line (SolarScene ground_length_/width_;
                                           not used in tracing)                            */
+    const float *helio_normal; /* [n][3] given heliostat normals (InitContent(normals),
+                                  solar_scene.cpp:101-115; CRotate(dir), heliostat.cpp:22-27),
+                                  normalised and kept for every sun; NULL: each heliostat is
+                                  aimed at the receiver focus (CRotate(focus, sun))       */
 } om_scene_desc;
 
 enum { OM_SUN_PILLBOX = 0, OM_SUN_BUIE = 1 };
@@ -215,6 +219,27 @@ om_status om_sun_direction(float zenith_deg, float azimuth_deg, float out[3]);
  * (a0 + a1 exp(-k / cos z)), n the day of a 365-day year from month (1-12) and day, and
  * Hottel's a0, a1, k for a site at elevation_m.  NaN for a month out of range. */
 float om_dni_hottel(float zenith_deg, int month, int day, float elevation_m);
+
+/* ---- The faceted heliostat of the annual driver (main.cpp:56-116): nine flat facets on a
+ * 3 x 3 lattice of pitch sqrt(2)/3 + 0.1 m, lifted onto the paraboloid y = x^2/a^2 + z^2/b^2,
+ * with the paraboloid's normals, turned by the matrix whose column 1 is the master normal
+ * and placed at (X, Y, Z) (SceneFileProc::SceneFileRead(SolarScene*, a, b, mirrorNormal, X,
+ * Y, Z), scene_file_proc.cpp:288-339,464-487).  The matrix's horizontal column is scaled by
+ * sqrt(n0^2 + n1^2) as the reference writes it (:302), which is a unit vector only for n1 = 0.
+ * Outputs pos, normal (not normalised) and size, [9][3] each. */
+om_status om_facet_heliostat(float a, float b, const float mirror_normal[3], float X, float Y, float Z,
+                             float pos[27], float normal[27], float size[27]);
+/* Master normal of the annual driver (main.cpp:84-91): the bisector of the direction from
+ * (X, Y, Z) to the point (0, 13.01, 0) and the direction to the sun (-sun_dir). */
+om_status om_master_normal(float X, float Y, float Z, const float sun_dir[3], float out[3]);
+/* The whole faceted scene as the reference builds it (scene_file_proc.cpp:288-534): the
+ * nine facets with their normals given, gap 0, matrix 1 x 1, a 1.4 x 1.4 m receiver at
+ * (0, 13, -1.5) facing +z (face 0), one grid of 110 m cells at (-5.5, -5.5, 94.5) of size
+ * (110, 110, 220), pixel length 0.01, sun straight down and DNI 1000 (set sun_dir and dni
+ * before building).  The description points into the caller's arrays. */
+om_status om_facet_scene_desc(float a, float b, const float mirror_normal[3], float X, float Y, float Z,
+                              float pos[27], float normal[27], float size[27], float gap[18], int matrix[18],
+                              om_scene_desc *desc);
 
 /* Copy scene arrays to host buffers for inspection (parity tests).  what:
  *  "helio_normal" [n][3] f32   "helio_v0" / "helio_v1" / "helio_v3" [n][3] f32

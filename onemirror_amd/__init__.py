@@ -66,6 +66,7 @@ class SceneDesc(ctypes.Structure):
         ("grids", ctypes.POINTER(GridDesc)),
         ("ground_length", ctypes.c_float),
         ("ground_width", ctypes.c_float),
+        ("helio_normal", _f32p),
     ]
 
 
@@ -151,6 +152,7 @@ _EXPORTS = [
     "om_batch_create", "om_batch_run", "om_batch_info", "om_batch_destroy", "om_trace_batched",
     "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
     "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel", "om_scn_write",
+    "om_master_normal", "om_facet_heliostat", "om_facet_scene_desc",
 ]
 
 
@@ -172,6 +174,11 @@ def lib():
                                     ctypes.POINTER(Params), ctypes.POINTER(_vp)]
     L.om_scn_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
     L.om_scn_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
+    L.om_master_normal.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p, _f32p]
+    L.om_facet_heliostat.argtypes = [ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, _f32p, _f32p, _f32p]
+    L.om_facet_scene_desc.argtypes = [ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, _f32p, _f32p, _f32p, _f32p, _i32p, ctypes.POINTER(SceneDesc)]
     L.om_scene_desc_free.argtypes = [ctypes.POINTER(SceneDesc)]
     L.om_scene_desc_free.restype = None
     L.om_scene_create.argtypes = [_vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(Params),
@@ -288,6 +295,12 @@ def _desc_struct(desc: dict, keep: list) -> SceneDesc:
         d.grids = ctypes.cast(arr, ctypes.POINTER(GridDesc))
     d.ground_length = float(desc.get("ground_length", 0.0))
     d.ground_width = float(desc.get("ground_width", 0.0))
+    if desc.get("helio_normal") is not None:
+        n = np.ascontiguousarray(desc["helio_normal"], np.float32).reshape(-1, 3)
+        if n.shape[0] != d.n_helio:
+            raise ValueError("helio_normal needs one normal per heliostat")
+        keep.append(n)
+        d.helio_normal = n.ctypes.data_as(_f32p)
     return d
 
 
@@ -310,6 +323,54 @@ def scn_write(path: str, desc: dict):
         full.setdefault(k, v)
     d = _desc_struct(full, keep)
     _check(lib().om_scn_write(path.encode(), ctypes.byref(d)), "om_scn_write")
+
+
+def master_normal(X: float, Y: float, Z: float, sun_dir) -> np.ndarray:
+    """Master normal of the faceted heliostat at (X, Y, Z) for a sun direction: the bisector
+    of the directions to the point (0, 13.01, 0) and to the sun (main.cpp:84-91)."""
+    out = np.zeros(3, np.float32)
+    _check(lib().om_master_normal(float(X), float(Y), float(Z), _f3(sun_dir), out.ctypes.data_as(_f32p)),
+           "om_master_normal")
+    return out
+
+
+def facet_heliostat(a: float, b: float, mirror_normal, X: float, Y: float, Z: float) -> dict:
+    """The nine facets of the paraboloid heliostat y = x^2/a^2 + z^2/b^2 turned to
+    mirror_normal at (X, Y, Z) (scene_file_proc.cpp:288-339): pos, normal (not normalised),
+    size, [9, 3] each."""
+    pos, nrm, size = (np.zeros((9, 3), np.float32) for _ in range(3))
+    _check(lib().om_facet_heliostat(float(a), float(b), _f3(mirror_normal), float(X), float(Y), float(Z),
+                                    pos.ctypes.data_as(_f32p), nrm.ctypes.data_as(_f32p),
+                                    size.ctypes.data_as(_f32p)), "om_facet_heliostat")
+    return {"pos": pos, "normal": nrm, "size": size}
+
+
+def facet_scene(a: float, b: float, X: float, Y: float, Z: float, sun_dir, dni: float,
+                mirror_normal=None) -> dict:
+    """Description of the reference's faceted scene (scene_file_proc.cpp:288-534): the nine
+    facets with their normals given, its receiver and grid; the master normal defaults to
+    the driver's for this sun (main.cpp:84-91)."""
+    if mirror_normal is None:
+        mirror_normal = master_normal(X, Y, Z, sun_dir)
+    pos, nrm, size, gap = (np.zeros(n, np.float32) for n in (27, 27, 27, 18))
+    mat = np.zeros(18, np.int32)
+    d = SceneDesc()
+    _check(lib().om_facet_scene_desc(float(a), float(b), _f3(mirror_normal), float(X), float(Y), float(Z),
+                                     pos.ctypes.data_as(_f32p), nrm.ctypes.data_as(_f32p),
+                                     size.ctypes.data_as(_f32p), gap.ctypes.data_as(_f32p),
+                                     mat.ctypes.data_as(_i32p), ctypes.byref(d)), "om_facet_scene_desc")
+    return {
+        "helio_pos": pos.reshape(9, 3), "helio_size": size.reshape(9, 3), "helio_gap": gap.reshape(9, 2),
+        "helio_matrix": mat.reshape(9, 2), "helio_normal": nrm.reshape(9, 3),
+        "recv_type": d.recv_type, "recv_pos": np.array(d.recv_pos[:], np.float32),
+        "recv_size": np.array(d.recv_size[:], np.float32), "recv_normal": np.array(d.recv_normal[:], np.float32),
+        "recv_face": d.recv_face, "recv_pixel_length": np.float32(d.recv_pixel_length),
+        "grid_pos": np.array(d.grid_pos[:], np.float32), "grid_size": np.array(d.grid_size[:], np.float32),
+        "grid_interval": np.array(d.grid_interval[:], np.float32),
+        "ground_length": np.float32(d.ground_length), "ground_width": np.float32(d.ground_width),
+        "sun_dir": np.asarray(sun_dir, np.float32), "dni": np.float32(dni),
+        "mirror_normal": np.asarray(mirror_normal, np.float32),
+    }
 
 
 def scn_parse(path: str) -> dict:

@@ -17,7 +17,7 @@ from typing import Iterable, List, Optional, Sequence
 
 import numpy as np
 
-from . import Batch, SolarScene, dni_hottel, lib, sun_direction, _f32p
+from . import Batch, SolarScene, dni_hottel, facet_scene, lib, sun_direction, _f32p
 
 # main.cpp:34-44: the reference's site (Hangzhou) and year
 REFERENCE_SITE = {"latitude": 30.3, "longitude": 120.2, "timezone": 8.0, "elevation_m": 40.0}
@@ -102,19 +102,26 @@ class AnnualRun:
     Each step is a fork of ``scene`` with its own Philox stream (scene index = step number),
     so its maps are what a scene built for that sun alone gives.  ``run`` returns per-step
     intercept ratios (unsmoothed and smoothed, om_intercept_ratio as results_with_given_normal
-    computes them), total and intercepted power, and the annual sums."""
+    computes them), total and intercepted power, and the annual sums.  ``AnnualRun.facets``
+    is the reference driver's own run: the faceted heliostat rebuilt for every step."""
 
-    def __init__(self, scene: SolarScene, steps: Iterable[dict], batch_size: int = 64,
-                 workgroup: int = 0, rays_per_thread: int = 0, mode: str = "batch"):
+    def __init__(self, scene: Optional[SolarScene], steps: Iterable[dict], batch_size: int = 64,
+                 workgroup: int = 0, rays_per_thread: int = 0, mode: str = "batch", scenes=None):
         """``mode``: "batch" traces the steps with the batched launch (hipGraph replays),
-        "plain" with one om_trace_plain per step, in the accumulation mode of ``scene``
-        (see SolarScene.autotune_accumulate); the maps are the same."""
+        "plain" with one om_trace_plain per step, in the accumulation mode of each scene
+        (see SolarScene.autotune_accumulate); the maps are the same.  ``scenes`` (one per
+        step) replaces the forks of ``scene``."""
         if mode not in ("batch", "plain"):
             raise ValueError("mode is 'batch' or 'plain'")
         self.scene = scene
         self.mode = mode
         self.steps = list(steps)
-        self.forks = [scene.fork(st["sun_dir"], st["dni"], scene_index=i) for i, st in enumerate(self.steps)]
+        if scenes is not None:
+            self.forks = list(scenes)
+            if len(self.forks) != len(self.steps):
+                raise ValueError("one scene per time step")
+        else:
+            self.forks = [scene.fork(st["sun_dir"], st["dni"], scene_index=i) for i, st in enumerate(self.steps)]
         self.batch_size = int(batch_size)
         self.workgroup, self.rays_per_thread = workgroup, rays_per_thread
         self._batches = []
@@ -126,6 +133,18 @@ class AnnualRun:
         else:
             self._maps = [f.new_maps(smooth=True) for f in self.forks]
         self.rays = sum(f.info["total_rays"] for f in self.forks)
+
+    @classmethod
+    def facets(cls, a: float, b: float, X: float, Y: float, Z: float, steps: Iterable[dict], params: dict,
+               ctx=None, **kw) -> "AnnualRun":
+        """main.cpp:56-116: for every time step the faceted paraboloid heliostat
+        y = x^2/a^2 + z^2/b^2 at (X, Y, Z) turned to that step's master normal
+        (main.cpp:84-91) and built into the reference's scene (scene_file_proc.cpp:288-534),
+        with the step's sun and DNI; Philox stream = step number."""
+        steps = list(steps)
+        scenes = [SolarScene(facet_scene(a, b, X, Y, Z, st["sun_dir"], st["dni"]), dict(params, scene_index=i), ctx)
+                  for i, st in enumerate(steps)]
+        return cls(None, steps, scenes=scenes, **kw)
 
     def trace(self, stream=None):
         """Trace every step; the maps stay on the device."""

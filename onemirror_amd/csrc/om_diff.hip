@@ -611,6 +611,10 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
         set_error("differentiable mode supports rectangle receivers");
         return OM_ERR_UNSUPPORTED;
     }
+    if (!s->normal_in.empty()) {
+        set_error("differentiable mode aims the heliostats itself (scene has given normals)");
+        return OM_ERR_UNSUPPORTED;
+    }
     if (s->rays_per_thread > RPT_MAX) return OM_ERR_UNSUPPORTED;
     if (s->total_rays >= (1LL << 30) || s->res_x > 1024 || s->res_y > 1024) {
         set_error("differentiable mode supports up to 2^30 rays and maps up to 1024 pixels a side");

@@ -146,6 +146,7 @@ struct om_scene {
     int H = 0;
     std::vector<float> pos, size, gap;
     std::vector<int> matrix;
+    std::vector<float> normal_in; // given heliostat normals [H][3]; empty: aim at the focus
     int recv_type = 0;
     float recv_pos[3], recv_size[3], recv_normal_in[3];
     int recv_face = 0;

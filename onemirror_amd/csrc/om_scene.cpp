@@ -680,7 +680,8 @@ om_status build_receiver(om_scene *s)
     return OM_OK;
 }
 
-// RectangleHelio::CRotate(focus, sun) (heliostat.cpp:16-62)
+// RectangleHelio::CRotate(focus, sun) (heliostat.cpp:16-62), or CRotate(dir) with the given
+// normals (heliostat.cpp:22-27)
 void aim_heliostats(om_scene *s)
 {
     s->normal.resize(s->H);
@@ -690,8 +691,13 @@ void aim_heliostats(om_scene *s)
     s->v3.resize(s->H);
     for (int i = 0; i < s->H; ++i) {
         v3 p = ld3(&s->pos[3 * i]), sz = ld3(&s->size[3 * i]);
-        v3 rd = om::normalize(s->focus - p);
-        v3 n = om::normalize(rd - s->sun);
+        v3 n;
+        if (!s->normal_in.empty()) {
+            n = om::normalize(ld3(&s->normal_in[3 * i])); // Cset_normal(dir) (heliostat.cpp:73-76)
+        } else {
+            v3 rd = om::normalize(s->focus - p);
+            n = om::normalize(rd - s->sun);
+        }
         s->normal[i] = n;
         om::frame f = om::make_frame(n);
         v3 l0 = om::mk(-sz.x / 2, sz.y / 2, -sz.z / 2);
@@ -988,6 +994,19 @@ om_status validate(const om_scene_desc *d, const om_params *p)
         return OM_ERR_INVALID;
     }
     const size_t H = (size_t)d->n_helio;
+    if (d->helio_normal) {
+        if (!bounded(d->helio_normal, 3 * H)) {
+            om::set_error("coordinates and sizes must be finite and within 2^20 m");
+            return OM_ERR_UNSUPPORTED;
+        }
+        for (size_t i = 0; i < H; ++i) {
+            const float *n = d->helio_normal + 3 * i;
+            if (!(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] > 1e-12f)) {
+                om::set_error("given heliostat normals must be nonzero");
+                return OM_ERR_INVALID;
+            }
+        }
+    }
     if (!bounded(d->helio_pos, 3 * H) || !bounded(d->helio_size, 3 * H) || !bounded(d->helio_gap, 2 * H) ||
         !bounded(d->recv_pos, 3) || !bounded(d->recv_size, 3)) {
         om::set_error("coordinates and sizes must be finite and within 2^20 m");
@@ -1175,6 +1194,7 @@ static om_status scene_from_desc(om_ctx *ctx, const om_scene_desc *d, const om_p
     s->size.assign(d->helio_size, d->helio_size + 3 * (size_t)s->H);
     s->gap.assign(d->helio_gap, d->helio_gap + 2 * (size_t)s->H);
     s->matrix.assign(d->helio_matrix, d->helio_matrix + 2 * (size_t)s->H);
+    if (d->helio_normal) s->normal_in.assign(d->helio_normal, d->helio_normal + 3 * (size_t)s->H);
     s->recv_type = d->recv_type;
     std::memcpy(s->recv_pos, d->recv_pos, 12);
     std::memcpy(s->recv_size, d->recv_size, 12);
@@ -1194,6 +1214,7 @@ static om_status scene_from_desc(om_ctx *ctx, const om_scene_desc *d, const om_p
         s->size.assign(d->helio_size + 3 * f, d->helio_size + 3 * (f + s->H));
         s->gap.assign(d->helio_gap + 2 * f, d->helio_gap + 2 * (f + s->H));
         s->matrix.assign(d->helio_matrix + 2 * f, d->helio_matrix + 2 * (f + s->H));
+        if (d->helio_normal) s->normal_in.assign(d->helio_normal + 3 * f, d->helio_normal + 3 * (f + s->H));
         gl = {k};
         gl[0].first_helio = 0;
     }

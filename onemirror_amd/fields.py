@@ -26,6 +26,10 @@ Workloads (heliostat counts, sizes, sunshape, microsamples, grid and map sizes):
            radii 40-80 m) in two 64^3 grids of their own, the receiver and sun of small:
            the multi-grid scene of the .scn format.
   face2face  the single-heliostat scene of userData/face2face_shadow.scn.
+  facet    the annual driver's faceted paraboloid heliostat (nine 0.47 m facets) at the
+           position of the face2face heliostat, a = b = 2 sqrt(108.3 m) (focal length the
+           distance to the receiver point), with the reference's default parameters
+           (lattice microsamples every 0.01 m x 2048 rays: 40.7 million rays a step).
 """
 from __future__ import annotations
 
@@ -190,6 +194,34 @@ def config_two_grids(seed: int = 7, n_each: int = 50) -> tuple:
     desc.update(grid_pos=grids[0]["pos"], grid_size=grids[0]["size"], grid_interval=grids[0]["interval"],
                 grids=grids)
     return desc, prm
+
+
+def facet_site() -> dict:
+    """(a, b, X, Y, Z) of the faceted heliostat workload: at the face2face heliostat's
+    position, focal length a^2 / 4 = its distance to the point (0, 13.01, 0)."""
+    X, Y, Z = 12.0, 1.2, 107.0
+    f = math.sqrt(X * X + Z * Z + (13.01 - Y) ** 2)
+    a = 2.0 * math.sqrt(f)
+    return {"a": a, "b": a, "X": X, "Y": Y, "Z": Z}
+
+
+def config_mid_grids(seed: int = 2) -> tuple:
+    """The mid field as a two-grid scene: its heliostats west of the tower (x < 0) first,
+    then the east ones, each half in a 128^3 grid of its own."""
+    desc, prm = config_mid(seed)
+    west = desc["helio_pos"][:, 0] < 0.0
+    order = np.concatenate([np.nonzero(west)[0], np.nonzero(~west)[0]])
+    d = subset(desc, order)
+    n0 = int(west.sum())
+    grids = []
+    for first, n in ((0, n0), (n0, d["helio_pos"].shape[0] - n0)):
+        sl = slice(first, first + n)
+        g = grid_for(d["helio_pos"][sl], d["helio_size"][sl], 170.0, 128)
+        grids.append({"pos": g["grid_pos"], "size": g["grid_size"], "interval": g["grid_interval"],
+                      "first_helio": first, "n_helio": n})
+    d.update(grid_pos=grids[0]["pos"], grid_size=grids[0]["size"], grid_interval=grids[0]["interval"],
+             grids=grids)
+    return d, prm
 
 
 def mid_sun(seed: int = 2) -> np.ndarray:

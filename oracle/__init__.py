@@ -59,6 +59,7 @@ class SceneDescC(ctypes.Structure):
         ("recv_res_y", ctypes.c_int),
         ("n_grids", ctypes.c_int),
         ("grids", ctypes.POINTER(GridDescC)),
+        ("helio_normal", _f32p),
     ]
 
 
@@ -109,6 +110,11 @@ def lib():
     L.oc_scene_grid.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i32p]
     L.oc_scene_grid_k.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, _i32p, _i64p]
     L.oc_scene_grid_k.restype = ctypes.c_int
+    L.oc_master_normal.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p, _f32p]
+    L.oc_master_normal.restype = ctypes.c_int
+    L.oc_facet_heliostat.argtypes = [ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, _f32p, _f32p, _f32p]
+    L.oc_facet_heliostat.restype = ctypes.c_int
     L.oc_scene_receiver.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p]
     L.oc_scene_sun.argtypes = [ctypes.c_void_p, _f32p, _f32p]
     L.oc_scene_buie_table.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p]
@@ -232,6 +238,10 @@ def _desc_struct(desc: dict, keep: list):
         keep.append(arr)
         d.n_grids = len(grids)
         d.grids = ctypes.cast(arr, ctypes.POINTER(GridDescC))
+    if desc.get("helio_normal") is not None:
+        n = np.ascontiguousarray(desc["helio_normal"], np.float32).reshape(H, 3)
+        keep.append(n)
+        d.helio_normal = _p(n, _f32p)
     return d
 
 
@@ -427,6 +437,23 @@ class OracleScene:
         if rc != 0:
             raise ValueError("pool smaller than rays per sample")
         return e
+
+
+def master_normal(X, Y, Z, sun_dir) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    s = np.ascontiguousarray(sun_dir, np.float32)
+    if lib().oc_master_normal(float(X), float(Y), float(Z), _p(s, _f32p), _p(out, _f32p)) != 0:
+        raise ValueError("degenerate master normal")
+    return out
+
+
+def facet_heliostat(a, b, mirror_normal, X, Y, Z) -> dict:
+    n = np.ascontiguousarray(mirror_normal, np.float32)
+    pos, nrm, size = (np.zeros((9, 3), np.float32) for _ in range(3))
+    if lib().oc_facet_heliostat(float(a), float(b), _p(n, _f32p), float(X), float(Y), float(Z),
+                                _p(pos, _f32p), _p(nrm, _f32p), _p(size, _f32p)) != 0:
+        raise ValueError("degenerate facet heliostat")
+    return {"pos": pos, "normal": nrm, "size": size}
 
 
 def energy_to_float(energy: np.ndarray) -> np.ndarray:

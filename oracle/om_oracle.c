@@ -477,11 +477,18 @@ static int build_receiver(oc_scene *s)
 }
 
 /* RectangleHelio::CRotate(focus, sun) (heliostat.cpp:16-62) */
+/* RectangleHelio::CRotate(focus, sun) (heliostat.cpp:16-62), or CRotate(dir) on the given
+ * normals of InitContent(normals) (heliostat.cpp:22-27,73-76) */
 static void build_heliostats(oc_scene *s)
 {
     for (int i = 0; i < s->H; ++i) {
-        f3 rd = normalize3(sub3(s->focus, s->pos[i]));
-        f3 n = normalize3(sub3(rd, s->sun));
+        f3 n;
+        if (s->d.helio_normal) {
+            n = normalize3(ld3(s->d.helio_normal + 3 * i));
+        } else {
+            f3 rd = normalize3(sub3(s->focus, s->pos[i]));
+            n = normalize3(sub3(rd, s->sun));
+        }
         s->normal[i] = n;
         f3 sz = s->size[i];
         f3 l0 = mk3(-sz.x / 2, sz.y / 2, -sz.z / 2);
@@ -1668,5 +1675,85 @@ int oc_diff_splat64(const oc_scene *s, const double *theta, int h0, int h1, cons
     free(maps);
     free(esum);
     free64(&q);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* the faceted heliostat of the annual driver                                            */
+/* ------------------------------------------------------------------------------------ */
+
+/* main.cpp:84-91.  receiverH (main.cpp:8) is a double literal, so the height terms are
+ * evaluated in double; the rest in float as declared there. */
+int oc_master_normal(float X, float Y, float Z, const float sun_dir[3], float out[3])
+{
+    const double receiverH = 13.01;
+    float a = sun_dir[0], b = sun_dir[1], c = sun_dir[2];
+    float distance = (float)sqrt(X * X + Z * Z + (receiverH - Y) * (receiverH - Y));
+    if (!(distance > 0.0f)) return -1;
+    float rox = -X / distance;
+    float roz = -Z / distance;
+    float roy = (float)((receiverH - Y) / distance);
+    float NR = sqrtf((-a + rox) * (-a + rox) + (-b + roy) * (-b + roy) + (-c + roz) * (-c + roz));
+    if (!(NR > 0.0f)) return -1;
+    out[0] = (-a + rox) / NR;
+    out[1] = (-b + roy) / NR;
+    out[2] = (-c + roz) / NR;
+    return 0;
+}
+
+/* SceneFileProc::matrixMulti (scene_file_proc.cpp:536-543) */
+static void matrix_multi(float A[3][3], const float *b, float *c)
+{
+    for (int i = 0; i < 3; i++) {
+        c[i] = 0;
+        for (int j = 0; j < 3; j++) c[i] += A[i][j] * b[j];
+    }
+}
+
+/* SceneFileProc::SceneFileRead(SolarScene*, a, b, mirrorNormal, X, Y, Z), facet part
+ * (scene_file_proc.cpp:289-339) and facet sizes (:478-481). */
+int oc_facet_heliostat(float a, float b, const float mirror_normal[3], float X, float Y, float Z,
+                       float pos[27], float normal[27], float size[27])
+{
+    float centerX[9] = { -sqrt(2.0) / 3 - 0.1, 0, sqrt(2.0) / 3 + 0.1, -sqrt(2.0) / 3 - 0.1, 0,
+                         sqrt(2.0) / 3 + 0.1, -sqrt(2.0) / 3 - 0.1, 0, sqrt(2.0) / 3 + 0.1 };
+    float centerZ[9] = { -sqrt(2.0) / 3 - 0.1, -sqrt(2.0) / 3 - 0.1, -sqrt(2.0) / 3 - 0.1, 0, 0, 0,
+                         sqrt(2.0) / 3 + 0.1, sqrt(2.0) / 3 + 0.1, sqrt(2.0) / 3 + 0.1 };
+    float centerY[9];
+    float m[3][3];
+    const float *nm = mirror_normal;
+    if (a == 0.0f || b == 0.0f) return -1;
+    for (int i = 0; i < 9; i++)
+        centerY[i] = centerX[i] * centerX[i] / a / a + centerZ[i] * centerZ[i] / b / b;
+    float temp = sqrtf(nm[0] * nm[0] + nm[1] * nm[1]);
+    if (nm[2] > 0) {
+        m[0][0] = nm[2] / temp; m[2][0] = -nm[0] / temp; m[1][0] = 0;
+    } else if (nm[2] < 0) {
+        m[0][0] = -nm[2] / temp; m[2][0] = nm[0] / temp; m[1][0] = 0;
+    } else {
+        m[0][0] = 0; m[1][0] = 0; m[2][0] = 1;
+    }
+    for (int i = 0; i < 3; i++) m[i][1] = nm[i];
+    m[0][2] = m[1][0] * m[2][1] - m[2][0] * m[1][1];
+    m[1][2] = m[2][0] * m[0][1] - m[0][0] * m[2][1];
+    m[2][2] = m[0][0] * m[1][1] - m[1][0] * m[0][1];
+    for (int i = 0; i < 9; i++) {
+        float center[3] = { centerX[i], centerY[i], centerZ[i] };
+        float nrm[3] = { -center[0] * 2 / a / a, 1, -center[2] * 2 / b / b };
+        float ct[3], nt[3];
+        matrix_multi(m, center, ct);
+        matrix_multi(m, nrm, nt);
+        pos[3 * i] = ct[0] + X;
+        pos[3 * i + 1] = ct[1] + Y;
+        pos[3 * i + 2] = ct[2] + Z;
+        normal[3 * i] = nt[0];
+        normal[3 * i + 1] = nt[1];
+        normal[3 * i + 2] = nt[2];
+        size[3 * i] = (float)(1.4142136 / 3);
+        size[3 * i + 1] = 0.1f;
+        size[3 * i + 2] = (float)(1.4142136 / 3);
+    }
+    for (int i = 0; i < 27; ++i)
+        if (!isfinite(pos[i]) || !isfinite(normal[i])) return -1;
     return 0;
 }

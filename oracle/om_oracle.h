@@ -59,7 +59,15 @@ typedef struct {
     const oc_grid_desc *grids; /* else grids[k] hold consecutive ranges covering [0, n),
                                   each heliostat's rays walking its own grid over global
                                   ids (recthelio_tracing.cpp:46-79, grid.cpp:53)          */
+    const float *helio_normal; /* [n][3] given normals (InitContent(normals),
+                                  solar_scene.cpp:101-115), or NULL: aim at the focus     */
 } oc_scene_desc;
+
+/* The faceted heliostat (scene_file_proc.cpp:288-339,464-487) and the master normal of the
+ * annual driver (main.cpp:84-91): see om_facet_heliostat / om_master_normal. */
+int oc_facet_heliostat(float a, float b, const float mirror_normal[3], float X, float Y, float Z,
+                       float pos[27], float normal[27], float size[27]);
+int oc_master_normal(float X, float Y, float Z, const float sun_dir[3], float out[3]);
 
 /* ---- tracing parameters: the solarenergy:: globals (common_var.cpp:9-27) plus the
  *      per-ray sampling choices that replace the cuRAND pools. ---- */
