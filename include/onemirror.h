@@ -292,6 +292,20 @@ om_status om_autotune_accumulate(om_scene *scene, const om_flux *maps, int reps,
 /* Intercept ratio of a host flux map (ImageSaver::savetxt, image_save.cpp:26-47). */
 float om_intercept_ratio(const float *flux, int width, int height, float total_energy);
 
+/* ---- Text outputs of the reference driver (both append to the file, as it does). */
+/* ImageSaver::savetxt(filename, w, h, h_data, precision) (image_save.cpp:5-24): the host map
+ * [height][width] written last row first, one line per row, each pixel "0," below 1e-6 and
+ * "%.<precision>f," otherwise (the reference's default precision is 2). */
+om_status om_savetxt_map(const char *path, int width, int height, const float *map, int precision);
+/* ImageSaver::savetxt(filename, w, h, h_data, totalEnergy) (image_save.cpp:26-47), what
+ * results_with_given_normal writes per trace (Gtest.cu:47,59): one line holding the
+ * intercept ratio (om_intercept_ratio) in default float formatting ("%g"). */
+om_status om_savetxt_ratio(const char *path, int width, int height, const float *map, float total_energy);
+/* load_noraml (scene_normals.cpp:3-13): a count N then N normals "x y z" into out[3N] (the
+ * om_scene_desc.helio_normal of InitContent(normals)).  *n receives N; out == NULL only
+ * counts; fewer readable normals than N is OM_ERR_PARSE. */
+om_status om_load_normals(const char *path, float *out, int capacity, int *n);
+
 /* ---- Batched mode: no reference counterpart (the reference traces one scene at a time
  * from a host loop over heliostats, recthelio_tracing.cpp:46-79).  n scenes (fields, sun
  * positions: annual time steps) are traced into n sets of maps by one persistent launch that

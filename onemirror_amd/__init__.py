@@ -152,7 +152,8 @@ _EXPORTS = [
     "om_batch_create", "om_batch_run", "om_batch_info", "om_batch_destroy", "om_trace_batched",
     "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
     "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel", "om_scn_write",
-    "om_master_normal", "om_facet_heliostat", "om_facet_scene_desc",
+    "om_master_normal", "om_facet_heliostat", "om_facet_scene_desc", "om_savetxt_map", "om_savetxt_ratio",
+    "om_load_normals",
 ]
 
 
@@ -174,6 +175,9 @@ def lib():
                                     ctypes.POINTER(Params), ctypes.POINTER(_vp)]
     L.om_scn_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
     L.om_scn_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
+    L.om_savetxt_map.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int]
+    L.om_savetxt_ratio.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float]
+    L.om_load_normals.argtypes = [ctypes.c_char_p, _f32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.om_master_normal.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p, _f32p]
     L.om_facet_heliostat.argtypes = [ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, _f32p, _f32p, _f32p]
@@ -582,8 +586,11 @@ class SolarScene:
                 "pixel": t[2][:n].cpu().numpy()}
 
 
-def results_with_given_normal(scene: SolarScene) -> dict:
-    """Gtest.cu:10-63: trace, scale, intercept ratio, smooth, ratio (synchronous)."""
+def results_with_given_normal(scene: SolarScene, nonsmooth_path: Optional[str] = None,
+                              smooth_path: Optional[str] = None) -> dict:
+    """Gtest.cu:10-63: trace, scale, intercept ratio, smooth, ratio (synchronous).  With
+    paths, the ratio lines are appended to them as the reference appends to
+    solarenergy::result_save_nonsmooth_path / result_save_smooth_path (Gtest.cu:47,59)."""
     n = scene.res_x * scene.res_y
     fl = np.zeros(n, np.float32)
     fs = np.zeros(n, np.float32)
@@ -592,9 +599,52 @@ def results_with_given_normal(scene: SolarScene) -> dict:
     r.flux_smooth = fs.ctypes.data_as(_f32p)
     _check(lib().om_results_with_given_normal(scene._h, ctypes.byref(r)),
            "om_results_with_given_normal")
-    return {"flux": fl.reshape(scene.res_y, scene.res_x),
-            "flux_smooth": fs.reshape(scene.res_y, scene.res_x), "ratio": r.ratio,
-            "ratio_smooth": r.ratio_smooth, "total_energy": r.total_energy, "rays": r.rays}
+    out = {"flux": fl.reshape(scene.res_y, scene.res_x),
+           "flux_smooth": fs.reshape(scene.res_y, scene.res_x), "ratio": r.ratio,
+           "ratio_smooth": r.ratio_smooth, "total_energy": r.total_energy, "rays": r.rays}
+    for path, key in ((nonsmooth_path, "flux"), (smooth_path, "flux_smooth")):
+        if path:
+            save_ratio_txt(path, out[key], r.total_energy)
+    return out
+
+
+class ImageSaver:
+    """ImageSaver (image_save.h:8-13): appending text writers of host maps."""
+
+    @staticmethod
+    def savetxt(filename: str, w: int, h: int, h_data, precision_or_total=2):
+        """The reference's two overloads: an int selects the map writer with that precision,
+        a float the ratio line against that total energy."""
+        m = np.ascontiguousarray(h_data, np.float32).reshape(h, w)
+        if isinstance(precision_or_total, (int, np.integer)) and not isinstance(precision_or_total, bool):
+            save_map_txt(filename, m, int(precision_or_total))
+        else:
+            save_ratio_txt(filename, m, float(precision_or_total))
+
+
+def save_map_txt(path: str, flux, precision: int = 2):
+    """om_savetxt_map: append the map as the reference's CSV (last row first)."""
+    m = np.ascontiguousarray(flux, np.float32)
+    h, w = m.shape
+    _check(lib().om_savetxt_map(path.encode(), w, h, m.ctypes.data_as(_f32p), int(precision)), "om_savetxt_map")
+
+
+def save_ratio_txt(path: str, flux, total_energy: float):
+    """om_savetxt_ratio: append the intercept ratio line."""
+    m = np.ascontiguousarray(flux, np.float32)
+    h, w = m.shape
+    _check(lib().om_savetxt_ratio(path.encode(), w, h, m.ctypes.data_as(_f32p), float(total_energy)),
+           "om_savetxt_ratio")
+
+
+def load_normals(path: str) -> np.ndarray:
+    """load_noraml (scene_normals.cpp:3-13): the [N, 3] normals of a normals file."""
+    n = ctypes.c_int()
+    _check(lib().om_load_normals(path.encode(), None, 0, ctypes.byref(n)), "om_load_normals")
+    out = np.zeros((max(n.value, 1), 3), np.float32)
+    _check(lib().om_load_normals(path.encode(), out.ctypes.data_as(_f32p), n.value, ctypes.byref(n)),
+           "om_load_normals")
+    return out[: n.value]
 
 
 def recthelio_ray_tracing(scene: SolarScene, helio: int, maps: FluxMaps, stream=None) -> int:

@@ -160,8 +160,12 @@ class AnnualRun:
             return [m for b in self._batches for m in b.maps]
         return self._maps
 
-    def results(self, keep_maps: bool = False) -> dict:
-        """Per-step and annual results from the maps of the last ``trace``."""
+    def results(self, keep_maps: bool = False, nonsmooth_path: Optional[str] = None,
+                smooth_path: Optional[str] = None) -> dict:
+        """Per-step and annual results from the maps of the last ``trace``.  With paths, each
+        step appends its ratio lines to them, as the reference driver's steps do
+        (Gtest.cu:47,59 via ImageSaver::savetxt)."""
+        from . import save_ratio_txt
         import torch
         torch.cuda.synchronize()
         L = lib()
@@ -173,6 +177,10 @@ class AnnualRun:
             te = float(f.info["total_energy"])
             r = float(L.om_intercept_ratio(flux.ctypes.data_as(_f32p), f.res_x, f.res_y, te))
             rs = float(L.om_intercept_ratio(smooth.ctypes.data_as(_f32p), f.res_x, f.res_y, te))
+            if nonsmooth_path:
+                save_ratio_txt(nonsmooth_path, flux.reshape(f.res_y, f.res_x), te)
+            if smooth_path:
+                save_ratio_txt(smooth_path, smooth.reshape(f.res_y, f.res_x), te)
             area = _pixel_area(f)
             rows.append({**{k: st[k] for k in ("month", "day", "hour", "zenith", "azimuth", "dni")},
                          "ratio": r, "ratio_smooth": rs, "total_w": te * area, "intercepted_w": r * te * area})

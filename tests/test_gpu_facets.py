@@ -69,3 +69,34 @@ def test_facet_annual_batch_equals_plain(om):
     rows = res["batch"]["steps"]
     assert len(rows) == len(steps) and all(0.3 < r["ratio"] <= 1.0 for r in rows)
     assert 0.3 < res["batch"]["annual_ratio"] <= 1.0
+
+
+def test_driver_text_outputs_and_loaded_normals(om, tmp_path):
+    """The driver's outputs (Gtest.cu:47,59: one ratio line per trace appended to the
+    unsmoothed and smoothed result files) for the faceted scene with the normals of the
+    reference's userData/normal.txt (InitContent(normals) with load_noraml), against the
+    oracle's maps written by the restatement of ImageSaver::savetxt."""
+    import os
+    import oracle.io as oio
+    here = os.path.dirname(os.path.abspath(__file__))
+    normals = om.load_normals(os.path.join(here, "golden", "normal.txt"))
+    p = dict(om.default_params(), rays_per_sample=128)
+    ns, sm = str(tmp_path / "result_nonsmooth.txt"), str(tmp_path / "result_smooth.txt")
+    want_ns = want_sm = ""
+    for el, az in ((50.0, 160.0), (30.0, 200.0)):
+        d = _desc(om, F.sun_direction(el, az))
+        d["helio_normal"] = normals
+        g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+        res = om.results_with_given_normal(g, ns, sm)
+        orc = o.trace_f32()
+        scale, total = o.flux_scale()
+        raw = oracle.energy_to_float(orc["energy"])
+        flux = (raw * np.float32(scale)).astype(np.float32)
+        smooth = (oracle.trimmed_mean(raw, p["smooth_radius"], p["smooth_ratio"]) * np.float32(scale)).astype(np.float32)
+        assert np.array_equal(res["flux"].view(np.uint32), flux.view(np.uint32))
+        assert np.array_equal(res["flux_smooth"].view(np.uint32), smooth.view(np.uint32))
+        h, w = flux.shape
+        want_ns += oio.savetxt_ratio(w, h, flux, total)
+        want_sm += oio.savetxt_ratio(w, h, smooth, total)
+    assert open(ns).read() == want_ns
+    assert open(sm).read() == want_sm
