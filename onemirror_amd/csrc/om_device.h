@@ -210,6 +210,28 @@ __device__ __forceinline__ int receiver_pixel(const TraceArgs &a, v3 orig, v3 re
     return rc.x >= 0 ? rc.x * a.res_x + rc.y : -1;
 }
 
+// Philox4x32-10 of om_common.h for kernels with a uniform key (the same bits).  The key of
+// each round is bumped by a scalar add in place: as plain code the ten round keys are
+// loop-invariant for every ray of a kernel, so the compiler precomputes all twenty, runs out
+// of SGPRs and re-reads them from VGPR lanes (two v_readlane and a hazard wait per round);
+// an asm add cannot be hoisted, so each round key lives in one SGPR pair for one round.
+__device__ __forceinline__ u4 philox_uniform_key(u4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        if (r < 9) {
+            asm volatile("s_add_u32 %0, %0, 0x9e3779b9" : "+s"(k0) : : "scc");
+            asm volatile("s_add_u32 %0, %0, 0xbb67ae85" : "+s"(k1) : : "scc");
+        }
+    }
+    return c;
+}
+
 // XCD (accelerator complex die) that runs the calling wave.  The packed map words are kept
 // one copy per XCD to spread same-pixel atomics; the atomics themselves are agent-scope, so
 // the choice of copy affects contention only, never the result.

@@ -162,7 +162,7 @@ __device__ __forceinline__ void diff_sample(const TraceArgs &a, const DiffTile &
                  (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
     }
     orig = to_world(loc, T.hf) + T.hpos;
-    x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)T.h, a.w3}, a.key0, a.key1);
+    x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)T.h, a.w3}, a.key0, a.key1);
     float th;
     if (SUN == OM_SUN_PILLBOX) {
         th = a.pillbox * sqrt_rn(u01(x.x));

@@ -97,7 +97,7 @@ __device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile 
     const v3 orig = to_world(loc, t.hf) + t.hpos;
     const cell3 c0 = start_cell(a, orig);
     // draws: sunshape theta, sunshape phi, slope-error normal deviate, slope-error phi
-    u4 x = philox(u4{(uint32_t)l, (uint32_t)m, (uint32_t)t.h, a.w3}, a.key0, a.key1);
+    u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)t.h, a.w3}, a.key0, a.key1);
     float th;
     if (SUN == OM_SUN_PILLBOX) {
         th = a.pillbox * sqrt_rn(u01(x.x));
