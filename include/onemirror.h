@@ -61,11 +61,13 @@ typedef struct {
     const float *helio_size;   /* [n][3] width (x), thickness (y), height (z)             */
     const float *helio_gap;    /* [n][2] gap between facets                               */
     const int   *helio_matrix; /* [n][2] facet rows, facet columns                        */
-    int   recv_type;           /* 0 rectangle (receiver.cuh:66-101), 1 cylinder (new);
-                                  2 (the truncated cone stub, receiver.cuh:125-140) parses
-                                  but cannot be traced                                    */
-    float recv_pos[3];         /* rectangle centre; cylinder: axis point at mid-height    */
-    float recv_size[3];        /* rectangle box size; cylinder (radius, height, unused)    */
+    int   recv_type;           /* 0 rectangle (receiver.cuh:66-101); 1 cylinder and 2
+                                  truncated cone (new: both are empty stubs in the
+                                  reference, receiver.cuh:103-140)                        */
+    float recv_pos[3];         /* rectangle centre; cylinder, cone: axis point at mid-height */
+    float recv_size[3];        /* rectangle box size; cylinder (radius, height, unused);
+                                  cone (top radius, bottom radius, height), the order of
+                                  topradius_bottomradius_hight_ (receiver.cuh:138)         */
     float recv_normal[3];
     int   recv_face;           /* 0..3 (receiver.cpp:30-48)                               */
     float recv_pixel_length;   /* solarenergy::receiver_pixel_length (common_var.cpp:12)  */
@@ -74,7 +76,8 @@ typedef struct {
     float grid_interval[3];
     float sun_dir[3];          /* direction the light travels                             */
     float dni;                 /* W/m^2                                                   */
-    int   recv_res_x, recv_res_y; /* cylinder map columns (azimuth) and rows (height);
+    int   recv_res_x, recv_res_y; /* cylinder / cone map columns (azimuth) and rows
+                                     (height; the cone's rows cover equal areas);
                                      0 = from recv_pixel_length                           */
     int   n_grids;             /* 0: the grid_* fields above; else the grids array        */
     const om_grid_desc *grids; /* [n_grids]                                               */
@@ -146,6 +149,7 @@ typedef struct {
     float   flux_scale;        /* DNI * Ssub * rho / Nc / Srec (Gtest.cu:37-46)           */
     float   total_energy;      /* sum |n.s| DNI A rho / Srec (Gtest.cu:29,44)             */
     int     n_grids;           /* grids traced; grid_num is grid 0's, cells and nnz sums  */
+    float   pixel_area;        /* Srec, the area of one receiver pixel (m^2)              */
 } om_scene_info;
 
 /* Host-side result of the full driver. */

@@ -116,6 +116,7 @@ class SceneInfo(ctypes.Structure):
         ("flux_scale", ctypes.c_float),
         ("total_energy", ctypes.c_float),
         ("n_grids", ctypes.c_int),
+        ("pixel_area", ctypes.c_float),
     ]
 
     def as_dict(self):

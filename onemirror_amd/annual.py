@@ -88,12 +88,9 @@ def annual_steps(schedule: Optional[Sequence[tuple]] = None, site: Optional[dict
 
 
 def _pixel_area(scene: SolarScene) -> float:
-    """Receiver pixel area (the Srec of Gtest.cu:40): square pixels of the rectangle, or an
-    azimuth x height cell of the unrolled cylinder."""
-    if int(scene.desc.get("recv_type", 0)) == 1:
-        r, h = float(scene.desc["recv_size"][0]), float(scene.desc["recv_size"][1])
-        return (2.0 * math.pi * r / scene.res_x) * (h / scene.res_y)
-    return float(scene.info["pixel_length"]) ** 2
+    """Receiver pixel area (the Srec of Gtest.cu:40): square pixels of the rectangle, an
+    azimuth x height cell of the unrolled cylinder, an equal-area cell of the cone."""
+    return float(scene.info["pixel_area"])
 
 
 class AnnualRun:

@@ -356,4 +356,61 @@ OM_HD int cylinder_pixel(v3 o, v3 d, v3 base, float r2, float h, int res_x, int 
     return rc.x >= 0 ? rc.x * res_x + rc.y : -1;
 }
 
+// Truncated-cone receiver (new: the reference's CircularTruncatedConeReceiver::GIntersect is
+// an empty stub, receiver.cuh:125-140).  Vertical axis through base = (cx, y0, cz); radius
+// r(y) = r0 + k y at height y in [0, h] above the base (r0 the bottom radius, k = (r1 - r0)
+// / h); only the outer surface receives.  The ray is first moved to its closest approach to
+// the axis in the xz plane (tc), where the quadratic |p_xz|^2 = r(p_y)^2 has small
+// coefficients: with w0 = r(py), wd = k dy, A = dx^2 + dz^2 - wd^2, its discriminant is
+// (dx^2 + dz^2)(w0^2 - d2) + wd^2 d2 (d2 = |p_xz|^2) and its roots tc + (w0 wd -+ sqrt(disc))
+// / A.  The nearer root ahead of the ray within the height range must enter the surface
+// (the ray against the outward normal (x, -k r(y), z)); otherwise the ray misses.
+// Pixel: column = azimuth as for the cylinder; row = the lateral area below the hit as a
+// fraction of the whole, y (2 r0 + k y) / (h (r0 + r1)) = y (2 r0 + k y) inv_area, so that
+// every pixel covers the same area.
+OM_HD int2 cone_rc(v3 o, v3 d, v3 base, float r0, float k, float h, float inv_area, int res_x, int res_y)
+{
+    const int2 miss = make_int2(-1, -1);
+    const float ox = o.x - base.x, oy = o.y - base.y, oz = o.z - base.z;
+    const float axy = d.x * d.x + d.z * d.z;
+    if (!(axy >= EPS_LT)) return miss;
+    const float tc = -(ox * d.x + oz * d.z) / axy;
+    const float px = ox + tc * d.x, py = oy + tc * d.y, pz = oz + tc * d.z;
+    const float d2 = px * px + pz * pz;
+    const float w0 = r0 + k * py, wd = k * d.y;
+    const float A = axy - wd * wd;
+    const float disc = axy * (w0 * w0 - d2) + wd * wd * d2;
+    if (!(disc >= 0.0f) || A == 0.0f) return miss;
+    const float sq = sqrt_(disc);
+    const float ta = tc + (w0 * wd - sq) / A, tb = tc + (w0 * wd + sq) / A;
+    const float t0 = ta < tb ? ta : tb, t1 = ta < tb ? tb : ta;
+    float t = -1.0f, hy = 0.0f;
+    for (int i = 0; i < 2; ++i) {
+        const float ti = i == 0 ? t0 : t1;
+        if (ti < EPS_LT) continue;
+        const float yi = oy + ti * d.y;
+        if (yi < 0.0f || yi > h) continue;
+        t = ti;
+        hy = yi;
+        break;
+    }
+    if (t < 0.0f) return miss;
+    const float hx = ox + t * d.x, hz = oz + t * d.z;
+    // entering: d . (x, -k r(y), z) < 0
+    if (!(d.x * hx + d.z * hz - d.y * (k * (r0 + k * hy)) < 0.0f)) return miss;
+    float th = atan2_(hz, hx);
+    if (th < 0.0f) th = th + 6.28318531f;
+    int col = trunc_i(th * 0.159154943f * (float)res_x);
+    int row = trunc_i(hy * (2.0f * r0 + k * hy) * inv_area * (float)res_y);
+    col = col < res_x - 1 ? col : res_x - 1;
+    row = row < res_y - 1 ? row : res_y - 1;
+    return make_int2(row, col);
+}
+
+OM_HD int cone_pixel(v3 o, v3 d, v3 base, float r0, float k, float h, float inv_area, int res_x, int res_y)
+{
+    const int2 rc = cone_rc(o, d, base, r0, k, h, inv_area, res_x, res_y);
+    return rc.x >= 0 ? rc.x * res_x + rc.y : -1;
+}
+
 } // namespace om

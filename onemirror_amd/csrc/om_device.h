@@ -200,6 +200,8 @@ __device__ __forceinline__ v3 perturbed_normal(const u4 &x, float sigma, const f
 __device__ __forceinline__ int2 receiver_rc(const TraceArgs &a, v3 orig, v3 refl)
 {
     if (a.recv_type == 1) return cylinder_rc(orig, refl, a.cyl_base, a.cyl_r2, a.cyl_h, a.res_x, a.res_y);
+    if (a.recv_type == 2)
+        return cone_rc(orig, refl, a.cyl_base, a.cone_r0, a.cone_k, a.cyl_h, a.cone_inv_area, a.res_x, a.res_y);
     float t, u, v;
     if (!parallelogram<gfx>(orig, refl, a.rA, a.rE1, a.rE2, t, u, v)) return make_int2(-1, -1);
     return make_int2(min(trunc_i(u * (float)a.res_y), a.res_y - 1), min(trunc_i(v * (float)a.res_x), a.res_x - 1));

@@ -59,8 +59,9 @@ struct TraceArgs {
     // receiver (receiver.cuh:79-82); cylinder: base of the axis, radius^2, height
     int recv_type;
     v3 rA, rE1, rE2;
-    v3 cyl_base;
-    float cyl_r2, cyl_h;
+    v3 cyl_base;                // cylinder and cone: base of the axis
+    float cyl_r2, cyl_h;        // cylinder radius^2; cylinder and cone height
+    float cone_r0, cone_k, cone_inv_area; // cone: bottom radius, dr/dy, 1 / (h (r0 + r1))
     int res_x, res_y;
     // heliostats and work
     const HelioRec *helio;
@@ -164,6 +165,7 @@ struct om_scene {
     float srec = 0.0f;            // pixel area
     om::v3 cyl_base{0, 0, 0};
     float cyl_r2 = 0.0f, cyl_h = 0.0f;
+    float cone_r0 = 0.0f, cone_k = 0.0f, cone_inv_area = 0.0f;
     int nbins = 0;
     std::vector<float> cdf_x, cdf_y, slope;
     float buie_int465 = 0, buie_A = 0, buie_B = 0, buie_C = 0, buie_inv_gp1 = 0, buie_gamma = 0;

@@ -612,11 +612,49 @@ om_status build_cylinder(om_scene *s, int geo)
     return OM_OK;
 }
 
+// Truncated-cone receiver (new; CircularTruncatedConeReceiver is an empty stub in
+// receiver.cuh:125-140, whose topradius_bottomradius_hight_ gives the order of size): axis
+// through recv_pos (mid-height), top radius size.x, bottom radius size.y, height size.z.  The
+// map unrolls the lateral surface, res_x columns over the azimuth and res_y rows of equal
+// area over the height, given or from geo = int(1 / pixel length) (the mean circumference and
+// the slant height in pixels).  Heliostats aim at the axis point.
+om_status build_cone(om_scene *s, int geo)
+{
+    const float r1 = s->recv_size[0], r0 = s->recv_size[1], h = s->recv_size[2];
+    if (!(r0 >= 0.0f && r1 >= 0.0f && r0 + r1 > 0.0f && h > 0.0f)) {
+        om::set_error("cone receiver needs radii >= 0 (not both 0) and a positive height");
+        return OM_ERR_INVALID;
+    }
+    const float slant = om::sqrt_(h * h + (r1 - r0) * (r1 - r0));
+    int rx = s->recv_res_in[0], ry = s->recv_res_in[1];
+    if (rx <= 0 || ry <= 0) {
+        rx = (int)(3.14159265f * (r0 + r1) * (float)geo);
+        ry = (int)(slant * (float)geo);
+    }
+    if (rx <= 0 || ry <= 0 || (long long)rx * ry >= (1LL << 28)) {
+        om::set_error("cone map must have 1 .. 2^28 pixels");
+        return OM_ERR_INVALID;
+    }
+    s->res_x = rx;
+    s->res_y = ry;
+    s->cyl_base = om::mk(s->recv_pos[0], s->recv_pos[1] - h / 2, s->recv_pos[2]);
+    s->cyl_h = h;
+    s->cone_r0 = r0;
+    s->cone_k = (r1 - r0) / h;
+    s->cone_inv_area = 1.0f / (h * (r0 + r1));
+    s->recv_pixel = slant / (float)ry;
+    s->srec = 3.14159265f * (r0 + r1) * slant / ((float)rx * (float)ry);
+    s->focus = ld3(s->recv_pos);
+    s->recv_normal = om::mk(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < 4; ++i) s->rv[i] = om::mk(0.0f, 0.0f, 0.0f);
+    return OM_OK;
+}
+
 // RectangleReceiver::CInit (receiver.cpp:24-117)
 om_status build_receiver(om_scene *s)
 {
-    if (s->recv_type != 0 && s->recv_type != 1) {
-        om::set_error("only rectangle (type 0) and cylinder (type 1) receivers are supported");
+    if (s->recv_type < 0 || s->recv_type > 2) {
+        om::set_error("receiver type must be 0 (rectangle), 1 (cylinder) or 2 (truncated cone)");
         return OM_ERR_UNSUPPORTED;
     }
     if (!(s->recv_pixel_length > 0.0f)) {
@@ -624,12 +662,13 @@ om_status build_receiver(om_scene *s)
         return OM_ERR_INVALID;
     }
     int geo = (int)(1.0f / s->recv_pixel_length);
-    if (geo <= 0 && !(s->recv_type == 1 && s->recv_res_in[0] > 0 && s->recv_res_in[1] > 0)) {
+    if (geo <= 0 && !(s->recv_type != 0 && s->recv_res_in[0] > 0 && s->recv_res_in[1] > 0)) {
         om::set_error("receiver pixel length above 1 m");
         return OM_ERR_UNSUPPORTED;
     }
     s->geo_px = geo;
     if (s->recv_type == 1) return build_cylinder(s, geo);
+    if (s->recv_type == 2) return build_cone(s, geo);
     s->recv_pixel = 1.0f / (float)geo;
     s->srec = s->recv_pixel * s->recv_pixel;
     v3 sz = ld3(s->recv_size), p = ld3(s->recv_pos);
@@ -1165,6 +1204,9 @@ TraceArgs make_trace_args(const om_scene *s, int g)
     a.cyl_base = s->cyl_base;
     a.cyl_r2 = s->cyl_r2;
     a.cyl_h = s->cyl_h;
+    a.cone_r0 = s->cone_r0;
+    a.cone_k = s->cone_k;
+    a.cone_inv_area = s->cone_inv_area;
     a.rA = s->rv[0];
     a.rE1 = s->rv[1] - s->rv[0];
     a.rE2 = s->rv[3] - s->rv[0];
@@ -1287,6 +1329,7 @@ extern "C" om_status om_scene_get_info(const om_scene *s, om_scene_info *info)
         info->grid_nnz += G.nnz;
     }
     info->n_grids = (int)s->grids.size();
+    info->pixel_area = s->srec;
     info->total_micro = s->total_micro;
     info->total_rays = s->total_rays;
     info->max_micro = s->max_micro;

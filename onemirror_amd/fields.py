@@ -21,6 +21,8 @@ Workloads (heliostat counts, sizes, sunshape, microsamples, grid and map sizes):
            microsamples x 40 rays (about 1e9 rays), 256^3 grid, seeded sun position.
   small_cylinder  100 heliostats 2x2 m around a cylinder r = 2 m, h = 4 m at 30 m
            (128x64 map): the cylinder path at test size.
+  small_cone  the same field around a truncated cone (radii 1.5 m top, 2.5 m bottom, 4 m
+           high, 128x64 equal-area map).
   batched  64 scenes of 500-20,000 heliostats under seeded suns (config_batched).
   two_grids  two sub-fields of 50 heliostats 2x2 m (north-west and north-east of the tower,
            radii 40-80 m) in two 64^3 grids of their own, the receiver and sun of small:
@@ -286,6 +288,15 @@ def config_small_cylinder(seed: int = 6) -> tuple:
                pillbox_half_angle=4.65e-3, csr=0.1, buie_bins=1024, slope_sigma=2e-3,
                micro_mode=POISSON, helio_pixel_length=2.0 / 16.0, poisson_k=8,
                rays_per_sample=64, reflectivity=0.88, smooth_radius=2, smooth_ratio=0.2)
+    return desc, prm
+
+
+def config_small_cone(seed: int = 8) -> tuple:
+    """100 heliostats around a truncated cone (top radius 1.5 m, bottom 2.5 m, height 4 m,
+    centred 30 m up): the cone receiver at test size, 128 x 64 equal-area pixels."""
+    desc, prm = config_small_cylinder(seed)
+    desc.update(recv_type=2, recv_size=np.array([1.5, 2.5, 4.0], np.float32))
+    prm = dict(prm, seed=0x00000000C0DEC0DE)
     return desc, prm
 
 

@@ -145,6 +145,9 @@ def lib():
     L.oc_cylinder_pixel_batch.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, ctypes.c_float,
                                           ctypes.c_float, ctypes.c_int, ctypes.c_int, _i32p]
     L.oc_cylinder_pixel_batch.restype = ctypes.c_int
+    L.oc_cone_pixel_batch.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, _i32p]
+    L.oc_cone_pixel_batch.restype = ctypes.c_int
     L.oc_diff_splat64.argtypes = [ctypes.c_void_p, _f64p, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_uint8), _f64p, _f64p, ctypes.c_int]
     L.oc_diff_splat64.restype = ctypes.c_int
@@ -497,6 +500,17 @@ def ray_parallelogram(orig, d, A, B, C):
     r = lib().oc_ray_parallelogram(*[_p(np.ascontiguousarray(a, np.float32), _f32p)
                                      for a in (orig, d, A, B, C)], _p(tuv, _f32p))
     return bool(r), tuv
+
+
+def cone_pixel(orig, d, base, r0, k, h, inv_area, res_x, res_y) -> np.ndarray:
+    """Truncated-cone receiver pixel (or -1) of each ray: orig, d are [n, 3]."""
+    o = np.ascontiguousarray(orig, np.float32).reshape(-1, 3)
+    dd = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(base, np.float32)
+    out = np.empty(o.shape[0], np.int32)
+    lib().oc_cone_pixel_batch(_p(o, _f32p), _p(dd, _f32p), int(o.shape[0]), _p(b, _f32p), float(r0), float(k),
+                              float(h), float(inv_area), int(res_x), int(res_y), _p(out, _i32p))
+    return out
 
 
 def cylinder_pixel(orig, d, base, r2, h, res_x, res_y) -> np.ndarray:

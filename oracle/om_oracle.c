@@ -234,6 +234,7 @@ struct oc_scene {
     float srec;               /* pixel area */
     f3 cyl_base;              /* cylinder: base of the axis, radius^2, height */
     float cyl_r2, cyl_h;
+    float cone_r0, cone_k, cone_inv_area; /* cone: bottom radius, dr/dy, 1 / (h (r0 + r1)) */
     /* sun (scene_instance_process.cu:6-11, :163-226) */
     f3 sun;
     float buie_k, buie_gamma, buie_int465, buie_A, buie_B, buie_C, buie_inv_gp1;
@@ -428,13 +429,43 @@ static int build_cylinder(oc_scene *s, int geo)
     return 0;
 }
 
+/* Truncated-cone receiver (the project's; the reference's is an empty stub,
+ * receiver.cuh:125-140): axis through recv_pos at mid-height, top radius size.x, bottom
+ * radius size.y, height size.z (the order of topradius_bottomradius_hight_); the map unrolls
+ * the lateral surface, res_x columns over the azimuth, res_y rows of equal area, by default
+ * the mean circumference and the slant height in pixels. */
+static int build_cone(oc_scene *s, int geo)
+{
+    float r1 = s->d.recv_size[0], r0 = s->d.recv_size[1], h = s->d.recv_size[2];
+    if (!(r0 >= 0.0f && r1 >= 0.0f && r0 + r1 > 0.0f && h > 0.0f)) return -5;
+    float slant = sqrtf(h * h + (r1 - r0) * (r1 - r0));
+    int rx = s->d.recv_res_x, ry = s->d.recv_res_y;
+    if (rx <= 0 || ry <= 0) {
+        rx = (int)(3.14159265f * (r0 + r1) * (float)geo);
+        ry = (int)(slant * (float)geo);
+    }
+    if (rx <= 0 || ry <= 0) return -4;
+    s->res_x = rx;
+    s->res_y = ry;
+    s->cyl_base = mk3(s->d.recv_pos[0], s->d.recv_pos[1] - h / 2, s->d.recv_pos[2]);
+    s->cyl_h = h;
+    s->cone_r0 = r0;
+    s->cone_k = (r1 - r0) / h;
+    s->cone_inv_area = 1.0f / (h * (r0 + r1));
+    s->recv_pixel = slant / (float)ry;
+    s->srec = 3.14159265f * (r0 + r1) * slant / ((float)rx * (float)ry);
+    s->focus = ld3(s->d.recv_pos);
+    return 0;
+}
+
 /* RectangleReceiver::CInit (receiver.cpp:24-117) */
 static int build_receiver(oc_scene *s)
 {
-    if (s->d.recv_type != 0 && s->d.recv_type != 1) return -2;
+    if (s->d.recv_type < 0 || s->d.recv_type > 2) return -2;
     int geo = (int)(1.0f / s->d.recv_pixel_length);
     s->geo = geo;
     if (s->d.recv_type == 1) return build_cylinder(s, geo);
+    if (s->d.recv_type == 2) return build_cone(s, geo);
     s->recv_pixel = 1.0f / (float)geo;
     s->srec = s->recv_pixel * s->recv_pixel;
     f3 size = ld3(s->d.recv_size), pos = ld3(s->d.recv_pos);
@@ -972,6 +1003,9 @@ static int receiver_pixel(const oc_scene *s, f3 o, f3 d)
 {
     if (s->d.recv_type == 1)
         return cylinder_pixel3(o, d, s->cyl_base, s->cyl_r2, s->cyl_h, s->res_x, s->res_y);
+    if (s->d.recv_type == 2)
+        return cone_pixel3(o, d, s->cyl_base, s->cone_r0, s->cone_k, s->cyl_h, s->cone_inv_area, s->res_x,
+                           s->res_y);
     float t, u, v;
     if (!ray_parallelogram3(o, d, s->rv[0], s->rv[1], s->rv[3], &t, &u, &v)) return -1;
     int row = trunc_i(u * (float)s->res_y);
@@ -1085,8 +1119,47 @@ typedef struct {
     double int465, A, B, C, inv_gp1;
     d3 cyl_base;
     double cyl_r, cyl_h;
+    double cone_r0, cone_r1;
     d3 focus;
 } oc_scene64;
+
+/* the truncated cone in double: the roots of |p_xz|^2 = (r0 + k p_y)^2 from the original
+ * origin, the nearer one ahead within [0, h] entering the surface; rows of equal area */
+static int cone_pixeld(d3 o, d3 d, const oc_scene64 *q, int res_x, int res_y)
+{
+    double h = q->cyl_h, r0 = q->cone_r0, r1 = q->cone_r1, k = (r1 - r0) / h;
+    double ox = o.x - q->cyl_base.x, oy = o.y - q->cyl_base.y, oz = o.z - q->cyl_base.z;
+    double w0 = r0 + k * oy, wd = k * d.y;
+    double a = d.x * d.x + d.z * d.z - wd * wd;
+    double b = 2.0 * (ox * d.x + oz * d.z - w0 * wd);
+    double c = ox * ox + oz * oz - w0 * w0;
+    double disc = b * b - 4.0 * a * c;
+    if (d.x * d.x + d.z * d.z < 1e-6 || disc < 0.0 || a == 0.0) return -1;
+    double sq = sqrt(disc);
+    double q0 = -0.5 * (b + (b >= 0 ? sq : -sq));
+    double ta = q0 / a, tb = c / q0;
+    double t0 = ta < tb ? ta : tb, t1 = ta < tb ? tb : ta;
+    double t = -1.0, hy = 0.0;
+    for (int i = 0; i < 2; ++i) {
+        double ti = i == 0 ? t0 : t1;
+        if (ti < 1e-6) continue;
+        double yi = oy + ti * d.y;
+        if (yi < 0.0 || yi > h) continue;
+        t = ti;
+        hy = yi;
+        break;
+    }
+    if (t < 0.0) return -1;
+    double hx = ox + t * d.x, hz = oz + t * d.z;
+    if (!(d.x * hx + d.z * hz - d.y * k * (r0 + k * hy) < 0.0)) return -1;
+    double th = atan2(hz, hx);
+    if (th < 0.0) th += 6.283185307179586;
+    int col = (int)(th / 6.283185307179586 * res_x);
+    int row = (int)(hy * (2.0 * r0 + k * hy) / (h * (r0 + r1)) * res_y);
+    if (col > res_x - 1) col = res_x - 1;
+    if (row > res_y - 1) row = res_y - 1;
+    return row * res_x + col;
+}
 
 /* the cylinder receiver in double: the entering root of the quadratic in t */
 static int cylinder_pixeld(d3 o, d3 d, const oc_scene64 *q, int res_x, int res_y)
@@ -1151,6 +1224,13 @@ static void build64(const oc_scene *s, oc_scene64 *q)
         q->cyl_base = mkd(pos.x, pos.y - h / 2, pos.z);
         q->cyl_r = r;
         q->cyl_h = h;
+    } else if (s->d.recv_type == 2) {
+        double h = s->d.recv_size[2];
+        focus = pos;
+        q->cyl_base = mkd(pos.x, pos.y - h / 2, pos.z);
+        q->cone_r1 = s->d.recv_size[0];
+        q->cone_r0 = s->d.recv_size[1];
+        q->cyl_h = h;
     }
     q->focus = focus;
     q->normal = malloc(sizeof(d3) * (H + 1));
@@ -1161,7 +1241,9 @@ static void build64(const oc_scene *s, oc_scene64 *q)
     for (int i = 0; i < H; ++i) {
         d3 p = f2d(s->pos[i]), z = f2d(s->size[i]);
         q->pos[i] = p;
-        d3 nn = normalized(subd(normalized(subd(focus, p)), sun));
+        d3 nn = s->d.helio_normal
+                    ? normalized(mkd(s->d.helio_normal[3 * i], s->d.helio_normal[3 * i + 1], s->d.helio_normal[3 * i + 2]))
+                    : normalized(subd(normalized(subd(focus, p)), sun));
         q->normal[i] = nn;
         d3 l0 = mkd(-z.x / 2, z.y / 2, -z.z / 2);
         d3 l1 = mkd(l0.x, l0.y, l0.z + z.z);
@@ -1338,6 +1420,8 @@ int oc_trace_f64(const oc_scene *s, int h0, int h1,
                     int px = -1;
                     if (s->d.recv_type == 1) {
                         px = cylinder_pixeld(orig, refl, &q, s->res_x, s->res_y);
+                    } else if (s->d.recv_type == 2) {
+                        px = cone_pixeld(orig, refl, &q, s->res_x, s->res_y);
                     } else if (ray_parallelogramd(orig, refl, q.rv[0], q.rv[1], q.rv[3], &t, &u, &v)) {
                         int row = (int)(u * s->res_y), col = (int)(v * s->res_x);
                         if (row > s->res_y - 1) row = s->res_y - 1;
@@ -1551,6 +1635,14 @@ int oc_cylinder_pixel_batch(const float *orig, const float *dir, int n, const fl
 {
     for (int i = 0; i < n; ++i)
         out[i] = cylinder_pixel3(ld3(orig + 3 * i), ld3(dir + 3 * i), ld3(base), r2, h, res_x, res_y);
+    return 0;
+}
+
+int oc_cone_pixel_batch(const float *orig, const float *dir, int n, const float base[3], float r0, float k,
+                        float h, float inv_area, int res_x, int res_y, int32_t *out)
+{
+    for (int i = 0; i < n; ++i)
+        out[i] = cone_pixel3(ld3(orig + 3 * i), ld3(dir + 3 * i), ld3(base), r0, k, h, inv_area, res_x, res_y);
     return 0;
 }
 

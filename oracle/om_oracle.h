@@ -42,9 +42,10 @@ typedef struct {
     const float *helio_size;   /* [n][3] x = width, y = thickness, z = height           */
     const float *helio_gap;    /* [n][2] gap between facets (x, z)                      */
     const int   *helio_matrix; /* [n][2] facet rows, facet cols                         */
-    int   recv_type;           /* 0 = rectangle (receiver.cpp), 1 = cylinder (new)      */
+    int   recv_type;           /* 0 = rectangle (receiver.cpp), 1 = cylinder, 2 = cone (new) */
     float recv_pos[3];
-    float recv_size[3];        /* rectangle: box size; cylinder: (radius, height, -)    */
+    float recv_size[3];        /* rectangle: box size; cylinder: (radius, height, -);
+                                  cone: (top radius, bottom radius, height)              */
                                /* cylinder recv_pos: axis point at mid-height           */
     float recv_normal[3];
     int   recv_face;           /* rectangle face 0..3 (receiver.cpp:30-48)              */
@@ -200,6 +201,9 @@ int  oc_diff_splat64(const oc_scene *s, const double *theta, int h0, int h1, con
 /* cylinder receiver pixel of n rays (new component; no reference counterpart) */
 int  oc_cylinder_pixel_batch(const float *orig, const float *dir, int n, const float base[3], float r2,
                              float h, int res_x, int res_y, int32_t *out);
+/* truncated-cone receiver pixels (cone_pixel3) of n rays */
+int  oc_cone_pixel_batch(const float *orig, const float *dir, int n, const float base[3], float r0, float k,
+                         float h, float inv_area, int res_x, int res_y, int32_t *out);
 
 #ifdef __cplusplus
 }

@@ -202,4 +202,48 @@ static inline int cylinder_pixel3(f3 o, f3 d, f3 base, float r2, float h, int re
     return row * res_x + col;
 }
 
+/* Truncated-cone receiver (the project's; CircularTruncatedConeReceiver::GIntersect is an
+ * empty stub, receiver.cuh:125-140): vertical axis through base, radius r0 + k y at height
+ * y in [0, h]; outer surface only.  The quadratic |p_xz|^2 = r(p_y)^2 is solved from the
+ * ray's closest approach to the axis in the xz plane (small coefficients); the nearer root
+ * ahead within the height range must enter the surface.  Column = azimuth as for the
+ * cylinder, row = lateral area below the hit over the whole: y (2 r0 + k y) inv_area. */
+static inline int cone_pixel3(f3 o, f3 d, f3 base, float r0, float k, float h, float inv_area, int res_x,
+                              int res_y)
+{
+    float ox = o.x - base.x, oy = o.y - base.y, oz = o.z - base.z;
+    float axy = d.x * d.x + d.z * d.z;
+    if (!(axy >= OC_EPS)) return -1;
+    float tc = -(ox * d.x + oz * d.z) / axy;
+    float px = ox + tc * d.x, py = oy + tc * d.y, pz = oz + tc * d.z;
+    float d2 = px * px + pz * pz;
+    float w0 = r0 + k * py, wd = k * d.y;
+    float A = axy - wd * wd;
+    float disc = axy * (w0 * w0 - d2) + wd * wd * d2;
+    if (!(disc >= 0.0f) || A == 0.0f) return -1;
+    float sq = sqrtf(disc);
+    float ta = tc + (w0 * wd - sq) / A, tb = tc + (w0 * wd + sq) / A;
+    float t0 = ta < tb ? ta : tb, t1 = ta < tb ? tb : ta;
+    float t = -1.0f, hy = 0.0f;
+    for (int i = 0; i < 2; ++i) {
+        float ti = i == 0 ? t0 : t1;
+        if (ti < OC_EPS) continue;
+        float yi = oy + ti * d.y;
+        if (yi < 0.0f || yi > h) continue;
+        t = ti;
+        hy = yi;
+        break;
+    }
+    if (t < 0.0f) return -1;
+    float hx = ox + t * d.x, hz = oz + t * d.z;
+    if (!(d.x * hx + d.z * hz - d.y * (k * (r0 + k * hy)) < 0.0f)) return -1;
+    float th = oc_atan2f(hz, hx);
+    if (th < 0.0f) th = th + 6.28318531f;
+    int col = trunc_i(th * 0.159154943f * (float)res_x);
+    int row = trunc_i(hy * (2.0f * r0 + k * hy) * inv_area * (float)res_y);
+    if (col > res_x - 1) col = res_x - 1;
+    if (row > res_y - 1) row = res_y - 1;
+    return row * res_x + col;
+}
+
 #endif
