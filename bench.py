@@ -215,7 +215,8 @@ def main() -> int:
             "seconds": cpu_t},
     }
     if args.workload == "mid" and not args.no_modes:
-        # the other two forms of the metric, on their own workloads (3 timed steps each)
+        # the batched and differentiable forms of the metric and the annual runs, on their own
+        # workloads (a few timed steps each)
         sub = argparse.Namespace(**vars(args))
         sub.steps, sub.warmup = 3, 1
         b = bench_batched(sub, with_cpu=False)
@@ -223,6 +224,7 @@ def main() -> int:
         sub_a = argparse.Namespace(**vars(args))
         sub_a.steps, sub_a.warmup = 1, 1
         an = bench_annual(sub_a)
+        fa = bench_facet_annual(sub_a)
         out["modes"] = {
             "batched": {"value": b["value"], "unit": "rays/s", "workload": b["config"]["workload"],
                         **{k: b["config"][k] for k in ("scenes", "rays", "batch_ms", "plain_ms",
@@ -231,6 +233,9 @@ def main() -> int:
             "annual": {"value": an["value"], "unit": "rays/s", "workload": an["config"]["workload"],
                        **{k: an["config"][k] for k in ("time_steps", "rays", "mode", "batch_ms", "plain_ms",
                                                         "time_steps_per_s", "annual_ratio")}},
+            "facet_annual": {"value": fa["value"], "unit": "rays/s", "workload": fa["config"]["workload"],
+                             **{k: fa["config"][k] for k in ("time_steps", "rays", "mode", "batch_ms", "plain_ms",
+                                                              "time_steps_per_s", "annual_ratio")}},
             "fwd_bwd": {"value": d["config"]["fwd_bwd_per_s"], "unit": "fwd+bwd/s",
                         "rays_per_s": d["value"], "workload": d["config"]["workload"],
                         **{k: d["config"][k] for k in ("fwd_bwd_ms", "forward_ms", "backward_ms",
