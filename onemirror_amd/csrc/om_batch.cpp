@@ -15,7 +15,7 @@
 
 namespace {
 
-constexpr size_t BATCH_PART_BYTES = (size_t)2 << 30; // budget of the packed map copies of a batch
+constexpr size_t BATCH_PART_BYTES = (size_t)8 << 30; // budget of the packed map copies of a batch
 
 template <class T>
 om_status dalloc(T *&p, size_t n)
