@@ -16,6 +16,9 @@
 namespace {
 
 constexpr size_t BATCH_PART_BYTES = (size_t)8 << 30; // budget of the packed map copies of a batch
+// copies of one map at most (words): more copies spread the same-pixel atomics of small maps
+// that take many hits (64 copies of a 140 x 140 map), while a 256 x 256 map keeps 16
+constexpr size_t SCENE_COPY_WORDS = (size_t)5 << 18;
 
 template <class T>
 om_status dalloc(T *&p, size_t n)
@@ -178,7 +181,9 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     long long sum_px = 0;
     for (int i = 0; i < n; ++i) sum_px += (long long)scenes[i]->res_x * scenes[i]->res_y;
     int shift = 0;
-    while (shift < 3 && ((size_t)16 << shift) * sum_px * sizeof(unsigned long long) <= BATCH_PART_BYTES) ++shift;
+    while (shift < 3 && ((size_t)16 << shift) * sum_px * sizeof(unsigned long long) <= BATCH_PART_BYTES &&
+           ((size_t)16 << shift) * (size_t)b->max_npx <= SCENE_COPY_WORDS)
+        ++shift;
     const int ncopy = 8 << shift;
     const size_t words = (size_t)sum_px * ncopy + (size_t)sum_px; // copies + (carry, wrap) as u32 pairs
     if ((rc = dalloc(b->d_parts, words)) != OM_OK) return fail(rc);
