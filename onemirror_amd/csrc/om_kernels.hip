@@ -1063,7 +1063,6 @@ void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1)
 using namespace om;
 
 namespace om {
-// trace heliostats [h0, h1) into out (cleared first when clear is set)
 // heliostats [h0, h1), all held by grid g, accumulated into out
 static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *out, hipStream_t st)
 {
@@ -1137,6 +1136,7 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
     return launch_trace(s, a, false, st);
 }
 
+// trace heliostats [h0, h1) into out (cleared first when clear is set)
 om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st)
 {
     om_status rc;
