@@ -203,7 +203,8 @@ om_status om_scene_destroy(om_scene *scene);
 om_status om_scene_get_info(const om_scene *scene, om_scene_info *info);
 
 /* New sun position: renormalise the sun and re-aim every heliostat at the receiver focus
- * (set_sunray_content + set_helio_content, scene_instance_process.cpp:21-28). */
+ * (set_sunray_content + set_helio_content, scene_instance_process.cpp:21-28); heliostats
+ * with given normals (om_scene_desc.helio_normal) keep them. */
 om_status om_scene_set_sun(om_scene *scene, const float sun_dir[3], float dni);
 
 /* ---- Annual time steps (main.cpp:56-116 builds a new scene for every sun position of the
@@ -213,7 +214,8 @@ om_status om_scene_set_sun(om_scene *scene, const float sun_dir[3], float dni);
  * fit side by side (for a batch).  The source (or the root it was forked from) must outlive
  * its forks.  The microsamples are the root's (Poisson points drawn with the root's scene
  * index); the rays use the fork's scene_index.  With the root's scene index, tracing a fork
- * gives what a scene built from scratch with that sun gives. */
+ * gives what a scene built from scratch with that sun gives.  Given heliostat normals are
+ * kept, as by om_scene_set_sun. */
 om_status om_scene_fork(const om_scene *src, const float sun_dir[3], float dni, uint32_t scene_index,
                         om_scene **out);
 /* Light travel direction of the sun at zenith and azimuth (degrees, azimuth clockwise from
