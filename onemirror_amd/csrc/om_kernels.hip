@@ -391,6 +391,7 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
         bool out = false;
         int px = 0;
         uint32_t q = 0u;
+        asm volatile("" ::: "memory"); // see trace_window_kernel
         if (r < nr) {
             const RayOut o = trace_ray<SUN, POISSON>(a, t, m, l, a.buie_bins, a.buie_guide);
             if (o.rc.x >= 0) {
@@ -413,9 +414,14 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
 // waves of an XCD take consecutive chunks, so at any time they trace a few neighbouring
 // heliostats whose grid cells and blockers stay in that XCD's L2 (spreading an XCD over
 // several ranges of the list is slower).
+// The arguments are read from the kernel-argument segment where each use needs them (a
+// compiler barrier per ray pass keeps the loads there): held in SGPRs across the ray loop,
+// they exceed the SGPR file and about a hundred of them spill to VGPR lanes, each use then
+// costing a v_readlane in the VALU-bound loop; scalar loads from the segment cost no VALU.
 template <int SUN, bool POISSON>
-__global__ __launch_bounds__(WIN_WGS) void trace_window_kernel(TraceArgs a)
+__global__ __launch_bounds__(WIN_WGS) void trace_window_kernel(TraceArgs)
 {
+    const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ unsigned long long s_win[WIN_PX_MAX];
     __shared__ unsigned long long s_ring[WIN_RING];
     __shared__ uint32_t s_ctl[3];
