@@ -207,8 +207,11 @@ __device__ __forceinline__ DiffTile diff_tile(const TraceArgs &a, const DiffArgs
 }
 
 template <int SUN, bool POISSON>
-__global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs p)
+__global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs, DiffArgs p)
 {
+    // the trace arguments are read from the kernel-argument segment per ray pass, as in the
+    // plain trace kernels (held in SGPRs they spill to VGPR lanes)
+    const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ float s_rx[RPT_MAX * WG], s_ry[RPT_MAX * WG], s_re[RPT_MAX * WG];
     __shared__ unsigned int s_wt[WG / 64], s_base;
     int nrec = 0; // records of this thread, staged in LDS slots [nrec][thread]
@@ -233,6 +236,7 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs a, DiffArgs 
         bool vis = false, rec = false;
         DiffRay o;
         int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+        asm volatile("" ::: "memory");
         if (r < T.nr) {
             v3 orig, to_sun;
             u4 x;
@@ -409,8 +413,9 @@ __device__ __forceinline__ double wave_sum(double v)
 // carried back through the plane intersection, normalisation, reflection, energy and the
 // normal's normalisation to dL/dn(theta); a tile's sum goes to tile_grad.
 template <int SUN, bool POISSON>
-__global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs a, DiffArgs p)
+__global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs, DiffArgs p)
 {
+    const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ double s_red[WG / 64][3];
     const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
@@ -432,6 +437,7 @@ __global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs a, DiffArgs
             }
         }
         if (r >= T.nr) break;
+        asm volatile("" ::: "memory");
         const unsigned long long bits = p.mask[((size_t)tile * a.rays_per_thread + k) * (WG / 64) + wave];
         if (!((bits >> lane) & 1ull)) continue;
         v3 orig, to_sun;

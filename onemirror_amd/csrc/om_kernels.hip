@@ -210,6 +210,7 @@ __device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, const in
         }
         if (r >= nr) break;
         nk = k + 1;
+        asm volatile("" ::: "memory"); // arguments re-read per ray pass (see trace_window_kernel)
         const RayOut o = trace_ray<SUN, POISSON>(a, t, m, l, s_bins, s_guide);
         const int px = o.rc.x >= 0 ? o.rc.x * a.res_x + o.rc.y : -1;
         if (RECORD) {
@@ -228,10 +229,12 @@ __device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, const in
     flush_slots<WGS, RPTS>(a, s_hpx, s_hq, tid, nk);
 }
 
-// One workgroup per tile of the scene's tile list (plain mode).
+// One workgroup per tile of the scene's tile list (plain mode).  The arguments are read from
+// the kernel-argument segment, as in trace_window_kernel.
 template <int SUN, bool POISSON, bool RECORD, bool BUIE_LDS>
-__global__ __launch_bounds__(WG) void trace_kernel(TraceArgs a)
+__global__ __launch_bounds__(WG) void trace_kernel(TraceArgs)
 {
+    const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     extern __shared__ float4 lds4[];
     int *s_hpx = (int *)lds4;                          // [RPT_MAX][WG] pixel or -1
     uint32_t *s_hq = (uint32_t *)(s_hpx + RPT_MAX * WG); // [RPT_MAX][WG] fixed-point energy
