@@ -52,6 +52,15 @@ struct DiffArgs {
     int ntx, nty, Q, G;         // Q = ntx * nty
 };
 
+// The kernel-argument segment of diff_forward_kernel / diff_backward_kernel (TraceArgs,
+// DiffArgs): explicit arguments are laid out in order, each at its type's alignment, as the
+// members of this struct.
+struct DiffKernArgs {
+    TraceArgs a;
+    DiffArgs p;
+};
+static_assert(sizeof(TraceArgs) % alignof(DiffArgs) == 0, "DiffArgs follows TraceArgs without padding");
+
 constexpr int DIFF_COPIES = 8;
 constexpr float DIFF_R = 3.0f;
 constexpr int SPLAT_T = 128;     // splat tile edge: SPLAT_T^2 u64 words are 128 KiB of LDS
@@ -207,11 +216,13 @@ __device__ __forceinline__ DiffTile diff_tile(const TraceArgs &a, const DiffArgs
 }
 
 template <int SUN, bool POISSON>
-__global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs, DiffArgs p)
+__global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs, DiffArgs)
 {
-    // the trace arguments are read from the kernel-argument segment per ray pass, as in the
-    // plain trace kernels (held in SGPRs they spill to VGPR lanes)
-    const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    // the arguments are read from the kernel-argument segment per ray pass, as in the plain
+    // trace kernels (held in SGPRs they spill to VGPR lanes)
+    const DiffKernArgs &k_args = *(const DiffKernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    const TraceArgs &a = k_args.a;
+    const DiffArgs &p = k_args.p;
     __shared__ float s_rx[RPT_MAX * WG], s_ry[RPT_MAX * WG], s_re[RPT_MAX * WG];
     __shared__ unsigned int s_wt[WG / 64], s_base;
     int nrec = 0; // records of this thread, staged in LDS slots [nrec][thread]
@@ -413,9 +424,11 @@ __device__ __forceinline__ double wave_sum(double v)
 // carried back through the plane intersection, normalisation, reflection, energy and the
 // normal's normalisation to dL/dn(theta); a tile's sum goes to tile_grad.
 template <int SUN, bool POISSON>
-__global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs, DiffArgs p)
+__global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs, DiffArgs)
 {
-    const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    const DiffKernArgs &k_args = *(const DiffKernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    const TraceArgs &a = k_args.a;
+    const DiffArgs &p = k_args.p;
     __shared__ double s_red[WG / 64][3];
     const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
     const int2 tl = a.tiles[tile];
