@@ -171,11 +171,14 @@ __device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
         mx = sx ? mx + dx : mx;
         my = sy ? my + dy : my;
         mz = sz ? mz + dz : mz;
-        const int i0 = a.cell_start[(uint32_t)addr], i1 = a.cell_start[(uint32_t)addr + 1u];
+        // one address for both ends of the cell's range (the second load takes an offset)
+        const int *cs = a.cell_start + (uint32_t)addr;
+        const int i0 = cs[0], i1 = cs[1];
         for (int i = i0; i < i1; ++i) {
             const int h = a.cell_items[i];
             if (h == self) continue;
-            const float4 A = a.geo[3 * h], E1 = a.geo[3 * h + 1], E2 = a.geo[3 * h + 2];
+            const float4 *g = a.geo + 3u * (uint32_t)h;
+            const float4 A = g[0], E1 = g[1], E2 = g[2];
             float t, u, v;
             if (parallelogram<gfx>(o, d, mk(A.x, A.y, A.z), mk(E1.x, E1.y, E1.z),
                                    mk(E2.x, E2.y, E2.z), t, u, v))

@@ -69,16 +69,16 @@ struct RayOut {
     uint32_t q;
 };
 
-// Ray r = m * L + l of the tile's heliostat: map_tracing (rectgrid_raytracing.cu:155-196).
-template <int SUN, bool POISSON>
-__device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile &t, int m, int l,
-                                            const float4 *s_bins, const int *s_guide)
+// Origin of the rays of microsample m, shared by its L rays: the microsample centre in world
+// coordinates and the grid cell that holds it.
+struct RayStart {
+    v3 orig;
+    cell3 c0;
+};
+
+template <bool POISSON>
+__device__ __forceinline__ RayStart ray_start(const TraceArgs &a, const HelioTile &t, int m)
 {
-    RayOut o;
-    o.shade = -1;
-    o.block = -2;
-    o.rc = make_int2(-1, -1);
-    o.q = 0u;
     const HelioRec *hr = t.hr;
     // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
     v3 loc;
@@ -94,8 +94,25 @@ __device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile 
         loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
                  (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
     }
-    const v3 orig = to_world(loc, t.hf) + t.hpos;
-    const cell3 c0 = start_cell(a, orig);
+    RayStart st;
+    st.orig = to_world(loc, t.hf) + t.hpos;
+    st.c0 = start_cell(a, st.orig);
+    return st;
+}
+
+// Ray l of microsample m of the tile's heliostat from its origin rs: map_tracing
+// (rectgrid_raytracing.cu:155-196).
+template <int SUN>
+__device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile &t, const RayStart &rs, int m,
+                                             int l, const float4 *s_bins, const int *s_guide)
+{
+    RayOut o;
+    o.shade = -1;
+    o.block = -2;
+    o.rc = make_int2(-1, -1);
+    o.q = 0u;
+    const v3 orig = rs.orig;
+    const cell3 &c0 = rs.c0;
     // draws: sunshape theta, sunshape phi, slope-error normal deviate, slope-error phi
     u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)t.h, a.w3}, a.key0, a.key1);
     float th;
@@ -126,6 +143,14 @@ __device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile 
     const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
     o.q = (uint32_t)(e * 2147483648.0f);
     return o;
+}
+
+// Ray r = m * L + l of the tile's heliostat.
+template <int SUN, bool POISSON>
+__device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile &t, int m, int l,
+                                            const float4 *s_bins, const int *s_guide)
+{
+    return trace_from<SUN>(a, t, ray_start<POISSON>(a, t, m), m, l, s_bins, s_guide);
 }
 
 // Adds the hits in the first nk of a thread's hit slots (pixel, or -1 for none) to the maps.
@@ -199,6 +224,8 @@ __device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, const in
     int l = r - m * L;
     const int dm = WGS / L, dl = WGS - (WGS / L) * L;
     int nk = 0; // rays of this thread, hit slots used
+    RayStart rs{};
+    int m_rs = -1; // microsample of rs: its origin is found again only when m changes
     for (int k = 0; k < a.rays_per_thread; ++k, r += WGS) {
         if (k > 0) {
             m += dm;
@@ -211,7 +238,11 @@ __device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, const in
         if (r >= nr) break;
         nk = k + 1;
         asm volatile("" ::: "memory"); // arguments re-read per ray pass (see trace_window_kernel)
-        const RayOut o = trace_ray<SUN, POISSON>(a, t, m, l, s_bins, s_guide);
+        if (m != m_rs) {
+            rs = ray_start<POISSON>(a, t, m);
+            m_rs = m;
+        }
+        const RayOut o = trace_from<SUN>(a, t, rs, m, l, s_bins, s_guide);
         const int px = o.rc.x >= 0 ? o.rc.x * a.res_x + o.rc.y : -1;
         if (RECORD) {
             const long long rid =
@@ -369,7 +400,9 @@ __device__ void ring_drain(const TraceArgs &a, const WinRing &R, int lane)
 }
 
 // nrays (a multiple of 64) rays of heliostat tl.x from tl.y, traced by one wave, one ray per
-// lane per pass.
+// lane per pass.  A lane finds the origin of its microsample again only when the
+// microsample changes: with L >= 64 rays per microsample a pass holds at most two, and most
+// passes none that is new to the wave.
 template <int SUN, bool POISSON>
 __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, const int lane, const int nrays,
                                                    const WinRing &R, unsigned long long *s_win)
@@ -381,6 +414,8 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
     int m = r / L;
     int l = r - m * L;
     const int dm = 64 / L, dl = 64 - (64 / L) * L;
+    RayStart rs{};
+    int m_rs = -1; // microsample of rs
     uint32_t head_seen = 0u; // a head read earlier: the true head is at least this
     for (int k = 0; k < nrays; k += 64, r += 64) {
         if (k > 0) {
@@ -396,7 +431,11 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
         uint32_t q = 0u;
         asm volatile("" ::: "memory"); // see trace_window_kernel
         if (r < nr) {
-            const RayOut o = trace_ray<SUN, POISSON>(a, t, m, l, a.buie_bins, a.buie_guide);
+            if (m != m_rs) {
+                rs = ray_start<POISSON>(a, t, m);
+                m_rs = m;
+            }
+            const RayOut o = trace_from<SUN>(a, t, rs, m, l, a.buie_bins, a.buie_guide);
             if (o.rc.x >= 0) {
                 px = o.rc.x * a.res_x + o.rc.y;
                 q = o.q;

@@ -272,6 +272,28 @@ def test_accumulation_modes_bitexact(om, mode, shift):
     assert int(orc["counts"].sum()) > 1_000_000
 
 
+@pytest.mark.parametrize("L,lattice", [(1, False), (7, False), (8, True), (40, True), (64, False),
+                                       (65, True), (100, False), (2048, True)])
+def test_window_sample_blocks_bitexact(om, L, lattice):
+    """The window kernel traces sample blocks (one microsample per lane, its rays one per
+    pass): rays per microsample below, at and above one part of 64, runs of several
+    microsamples per lane (L = 1, 7) and a heliostat's last, partial run all give the
+    oracle's maps."""
+    import torch
+    d, p = F.config_mid()
+    n = max(4, min(200, 8000 // L))
+    d = F.subset(d, slice(1500, 1500 + n))
+    p = dict(F.with_lattice(p) if lattice else p, accumulate=2, rays_per_sample=L)
+    print(f"window blocks L={L} lattice={lattice} heliostats={n}", flush=True)
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    maps = g.trace_plain()
+    torch.cuda.synchronize()
+    orc = o.trace_f32()
+    assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32), orc["counts"])
+    assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64), orc["energy"])
+    assert int(orc["counts"].sum()) > 10_000
+
+
 def test_accumulation_autotune_keeps_the_maps(om):
     """The accumulation autotuner times both atomic strategies on the scene and keeps the
     faster; the maps after it are the oracle's, whichever it kept."""
