@@ -149,6 +149,9 @@ __device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
     int ny = fy ? a.occ_hi[1] - c.gy : c.gy - a.occ_lo[1];
     int nz = fz ? a.occ_hi[2] - c.gz : c.gz - a.occ_lo[2];
     if ((nx | ny | nz) < 0) return -1;
+    // no step left on any axis: the first step, whichever axis it takes, ends the walk
+    // before any cell is tested
+    if ((nx | ny | nz) == 0) return -1;
     float mx = first_crossing(d.x, fx, a.gix, c.gx, c.rx);
     float my = first_crossing(d.y, fy, a.giy, c.gy, c.ry);
     float mz = first_crossing(d.z, fz, a.giz, c.gz, c.rz);
