@@ -37,6 +37,21 @@ __device__ __forceinline__ float4 buie_bin(const float4 *bins, const int *guide,
     return b;
 }
 
+// The same bin from the guide and the upper edge of its bin, held side by side: a value
+// beyond that edge starts at the next bin, so the walk of buie_bin takes no step when the
+// guide entry's value range meets at most one bin edge (the usual case), and the bin is the
+// second load of the lookup, never a third.
+__device__ __forceinline__ float4 buie_bin_split(const TraceArgs &a, float v)
+{
+    const int j = min(max(trunc_i(v * a.buie_gscale), 0), a.nbins - 1);
+    const int2 g = a.buie_split[j];
+    int i = g.x + (v >= u2f((uint32_t)g.y) ? 1 : 0); // at most nbins - 1: v < int465
+    float4 b = a.buie_bins[i];
+    while (b.x > v) b = a.buie_bins[--i];
+    while (b.y <= v) b = a.buie_bins[++i];
+    return b;
+}
+
 // ---------------------------------------------------------------------------------------
 // Short forms of the correctly rounded division and square root
 // ---------------------------------------------------------------------------------------

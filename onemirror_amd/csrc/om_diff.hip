@@ -178,7 +178,7 @@ __device__ __forceinline__ void diff_sample(const TraceArgs &a, const DiffTile &
     } else {
         float v = u01(x.x) * a.buie_A;
         if (v < a.buie_int465) {
-            float4 b = buie_bin(a.buie_bins, a.buie_guide, a.buie_gscale, v, a.nbins);
+            const float4 b = buie_bin_split(a, v);
             th = (v - b.x) * b.z + b.w;
         } else {
             th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);

@@ -43,6 +43,7 @@ struct TraceArgs {
     const float4 *buie_bins;    // [nbins]: (cdf_y[i], cdf_y[i+1], slope[i], cdf_x[i])
     const int *buie_guide;      // [nbins]: a bin index at or below the bin of j / gscale
     float buie_gscale;          // nbins / int465
+    const int2 *buie_split;     // [nbins]: (guide[j], bits of the upper edge cdf_y[guide[j] + 1])
     float slope_sigma;
     uint32_t key0, key1, w3;
     int L;
@@ -182,6 +183,7 @@ struct om_scene {
     float4 *d_geo = nullptr;
     float4 *d_buie_bins = nullptr;
     int *d_buie_guide = nullptr;
+    int2 *d_buie_split = nullptr;
     float2 *d_poisson = nullptr;
     int *d_mcount = nullptr;
     std::vector<int2> tiles;

@@ -101,8 +101,9 @@ __device__ __forceinline__ RayStart ray_start(const TraceArgs &a, const HelioTil
 }
 
 // Ray l of microsample m of the tile's heliostat from its origin rs: map_tracing
-// (rectgrid_raytracing.cu:155-196).
-template <int SUN>
+// (rectgrid_raytracing.cu:155-196).  FINE: the Buie bin comes from the split guide in device
+// memory, else from the tables s_bins / s_guide (in LDS).
+template <int SUN, bool FINE>
 __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile &t, const RayStart &rs, int m,
                                              int l, const float4 *s_bins, const int *s_guide)
 {
@@ -121,7 +122,7 @@ __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile
     } else {
         float v = u01(x.x) * a.buie_A;
         if (v < a.buie_int465) {
-            float4 b = buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
+            const float4 b = FINE ? buie_bin_split(a, v) : buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
             th = (v - b.x) * b.z + b.w;
         } else {
             th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
@@ -146,11 +147,11 @@ __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile
 }
 
 // Ray r = m * L + l of the tile's heliostat.
-template <int SUN, bool POISSON>
+template <int SUN, bool POISSON, bool FINE>
 __device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile &t, int m, int l,
                                             const float4 *s_bins, const int *s_guide)
 {
-    return trace_from<SUN>(a, t, ray_start<POISSON>(a, t, m), m, l, s_bins, s_guide);
+    return trace_from<SUN, FINE>(a, t, ray_start<POISSON>(a, t, m), m, l, s_bins, s_guide);
 }
 
 // Adds the hits in the first nk of a thread's hit slots (pixel, or -1 for none) to the maps.
@@ -210,7 +211,7 @@ __device__ __forceinline__ void flush_slots(const TraceArgs &a, const int *s_hpx
 // memory side has carried it out.  s_hpx / s_hq: the [RPTS][WGS] hit slots of the WGS
 // threads (each thread uses only its own).  RECORD: every ray's shading, blocking and pixel
 // indices go to the record arrays instead (rays in the reference order).
-template <int SUN, bool POISSON, bool RECORD, int WGS, int RPTS = RPT_MAX>
+template <int SUN, bool POISSON, bool RECORD, int WGS, bool FINE, int RPTS = RPT_MAX>
 __device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, const int tid, const int nrays,
                                            int *s_hpx, uint32_t *s_hq, const float4 *s_bins,
                                            const int *s_guide)
@@ -242,7 +243,7 @@ __device__ __forceinline__ void trace_tile(const TraceArgs &a, int2 tl, const in
             rs = ray_start<POISSON>(a, t, m);
             m_rs = m;
         }
-        const RayOut o = trace_from<SUN>(a, t, rs, m, l, s_bins, s_guide);
+        const RayOut o = trace_from<SUN, FINE>(a, t, rs, m, l, s_bins, s_guide);
         const int px = o.rc.x >= 0 ? o.rc.x * a.res_x + o.rc.y : -1;
         if (RECORD) {
             const long long rid =
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs)
         s_guide = lg;
     }
     const int tile = a.tile_base + xcd_chunk(blockIdx.x, gridDim.x);
-    trace_tile<SUN, POISSON, RECORD, WG>(a, a.tiles[tile], (int)threadIdx.x, a.tile_rays, s_hpx, s_hq, s_bins,
+    trace_tile<SUN, POISSON, RECORD, WG, !BUIE_LDS>(a, a.tiles[tile], (int)threadIdx.x, a.tile_rays, s_hpx, s_hq, s_bins,
                                          s_guide);
 }
 
@@ -435,7 +436,7 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
                 rs = ray_start<POISSON>(a, t, m);
                 m_rs = m;
             }
-            const RayOut o = trace_from<SUN>(a, t, rs, m, l, a.buie_bins, a.buie_guide);
+            const RayOut o = trace_from<SUN, true>(a, t, rs, m, l, a.buie_bins, a.buie_guide);
             if (o.rc.x >= 0) {
                 px = o.rc.x * a.res_x + o.rc.y;
                 q = o.q;
@@ -525,7 +526,7 @@ __global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__res
         const int4 w = items[it];
         const int sc = __builtin_amdgcn_readfirstlane(w.x);
         const TraceArgs &a = table[sc];
-        trace_tile<SUN, POISSON, false, WGS>(a, make_int2(__builtin_amdgcn_readfirstlane(w.y),
+        trace_tile<SUN, POISSON, false, WGS, true>(a, make_int2(__builtin_amdgcn_readfirstlane(w.y),
                                                           __builtin_amdgcn_readfirstlane(w.z)),
                                              (int)threadIdx.x, a.tile_rays, s_hpx, s_hq, a.buie_bins,
                                              a.buie_guide);

@@ -904,6 +904,7 @@ void free_scene_dev(om_scene *s)
     }
     dfree(s->d_buie_bins);
     dfree(s->d_buie_guide);
+    dfree(s->d_buie_split);
     dfree(s->d_poisson);
     dfree(s->d_mcount);
     dfree(s->d_tiles);
@@ -1113,8 +1114,18 @@ om_status build_all(om_scene *s)
             while (i < n - 1 && s->cdf_y[i + 1] <= vj) ++i;
             guide[j] = i;
         }
+        // the guide with the upper edge of each entry's bin (the last bin's edge is int465,
+        // above every value looked up)
+        std::vector<int2> split(n);
+        for (int j = 0; j < n; ++j) {
+            float edge = s->cdf_y[guide[j] + 1];
+            int bits;
+            std::memcpy(&bits, &edge, 4);
+            split[j] = make_int2(guide[j], bits);
+        }
         if ((st = upload(s->d_buie_bins, bins.data(), bins.size())) != OM_OK) return st;
         if ((st = upload(s->d_buie_guide, guide.data(), guide.size())) != OM_OK) return st;
+        if ((st = upload(s->d_buie_split, split.data(), split.size())) != OM_OK) return st;
     }
     if (s->prm.micro_mode == OM_MICRO_POISSON) {
         // Poisson points are drawn on the GPU (one workgroup per heliostat); their
@@ -1191,6 +1202,7 @@ TraceArgs make_trace_args(const om_scene *s, int g)
     a.buie_bins = s->d_buie_bins;
     a.buie_guide = s->d_buie_guide;
     a.buie_gscale = (float)s->nbins / s->buie_int465;
+    a.buie_split = s->d_buie_split;
     a.slope_sigma = s->prm.slope_sigma;
     a.key0 = (uint32_t)s->prm.seed;
     a.key1 = (uint32_t)(s->prm.seed >> 32);
