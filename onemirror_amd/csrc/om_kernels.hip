@@ -102,7 +102,7 @@ __device__ __forceinline__ RayStart ray_start(const TraceArgs &a, const HelioTil
 
 // Ray l of microsample m of the tile's heliostat from its origin rs: map_tracing
 // (rectgrid_raytracing.cu:155-196).  FINE: the Buie bin comes from the split guide in device
-// memory, else from the tables s_bins / s_guide (in LDS).
+// memory (buie_bin_split), else from the tables s_bins / s_guide (in LDS).
 template <int SUN, bool FINE>
 __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile &t, const RayStart &rs, int m,
                                              int l, const float4 *s_bins, const int *s_guide)
@@ -116,27 +116,54 @@ __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile
     const cell3 &c0 = rs.c0;
     // draws: sunshape theta, sunshape phi, slope-error normal deviate, slope-error phi
     u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)t.h, a.w3}, a.key0, a.key1);
-    float th;
-    if (SUN == OM_SUN_PILLBOX) {
-        th = a.pillbox * sqrt_rn(u01(x.x));
-    } else {
-        float v = u01(x.x) * a.buie_A;
+    float th, st, ct, sp, cp;
+    v3 nrm;
+    if (SUN == OM_SUN_BUIE && FINE) {
+        // buie_bin_split with its two loads issued ahead of the draws that do not depend on
+        // them (the sunshape azimuth and the perturbed normal, which the reference computes
+        // after the shading walk: the same values), so that the loads are in flight meanwhile
+        const float v = u01(x.x) * a.buie_A;
+        const int j = min(max(trunc_i(v * a.buie_gscale), 0), a.nbins - 1);
+        const int2 g = a.buie_split[j];
+        __builtin_amdgcn_sched_barrier(0); // the load above is issued before this work
+        nrm = perturbed_normal(x, a.slope_sigma, t.hf);
+        asm volatile("" ::"v"(nrm.x), "v"(nrm.y), "v"(nrm.z)); // computed here
+        __builtin_amdgcn_sched_barrier(0);
+        int i = min(g.x + (v >= u2f((uint32_t)g.y) ? 1 : 0), a.nbins - 1);
+        float4 b = a.buie_bins[i];
+        __builtin_amdgcn_sched_barrier(0);
+        sincos2pi(u01(x.y), sp, cp);
+        asm volatile("" ::"v"(sp), "v"(cp));
+        __builtin_amdgcn_sched_barrier(0);
         if (v < a.buie_int465) {
-            const float4 b = FINE ? buie_bin_split(a, v) : buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
+            while (b.x > v) b = a.buie_bins[--i];
+            while (b.y <= v) b = a.buie_bins[++i];
             th = (v - b.x) * b.z + b.w;
         } else {
             th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
         }
         th = div_rn(th, 1000.0f);
+    } else {
+        if (SUN == OM_SUN_PILLBOX) {
+            th = a.pillbox * sqrt_rn(u01(x.x));
+        } else {
+            float v = u01(x.x) * a.buie_A;
+            if (v < a.buie_int465) {
+                const float4 b = buie_bin(s_bins, s_guide, a.buie_gscale, v, a.nbins);
+                th = (v - b.x) * b.z + b.w;
+            } else {
+                th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
+            }
+            th = div_rn(th, 1000.0f);
+        }
+        sincos2pi(u01(x.y), sp, cp);
     }
-    float st, ct, sp, cp;
     sincos_small(th, st, ct);
-    sincos2pi(u01(x.y), sp, cp);
     const v3 to_sun = -to_world(angle2xyz(st, ct, sp, cp), t.sf);
     // the reference order: shading, perturbed normal, reflection, blocking, receiver
     o.shade = dda(a, orig, c0, to_sun, t.h);
     if (o.shade >= 0) return o;
-    const v3 nrm = perturbed_normal(x, a.slope_sigma, t.hf);
+    if (!(SUN == OM_SUN_BUIE && FINE)) nrm = perturbed_normal(x, a.slope_sigma, t.hf);
     const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
     o.block = dda(a, orig, c0, refl, t.h);
     if (o.block >= 0) return o;
