@@ -175,23 +175,40 @@ __device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
     // cell index strides of one step on each axis
     const int yz = a.gny * a.gnz;
     const int ax = fx ? yz : -yz, ay = fy ? a.gnz : -a.gnz, az = fz ? 1 : -1;
+    // One step: the address of the next cell, false once the walk has left the window.
     int addr = c.addr;
+#define OM_WALK_STEP(more)                                                                   \
+    do {                                                                                   \
+        /* axis of the next crossing, ties as in the reference: x only if strictly first */ \
+        const bool sx = mx < my && mx < mz;                                                \
+        const bool sy = !(mx < my) && my < mz;                                             \
+        const bool sz = !sx && !sy;                                                        \
+        nx -= sx ? 1 : 0;                                                                  \
+        ny -= sy ? 1 : 0;                                                                  \
+        nz -= sz ? 1 : 0;                                                                  \
+        more = (nx | ny | nz) >= 0;                                                        \
+        addr += sx ? ax : (sy ? ay : az);                                                  \
+        mx = sx ? mx + dx : mx;                                                            \
+        my = sy ? my + dy : my;                                                            \
+        mz = sz ? mz + dz : mz;                                                            \
+    } while (0)
+    bool first;
+    OM_WALK_STEP(first);
+    if (!first) return -1;
+    // the candidate range of a cell: one address for both ends (the second load takes an offset)
+    const int *cs = a.cell_start + (uint32_t)addr;
+    int i0 = cs[0], i1 = cs[1];
     for (;;) {
-        // axis of the next crossing, ties as in the reference: x only if strictly first
-        const bool sx = mx < my && mx < mz;
-        const bool sy = !(mx < my) && my < mz;
-        const bool sz = !sx && !sy;
-        nx -= sx ? 1 : 0;
-        ny -= sy ? 1 : 0;
-        nz -= sz ? 1 : 0;
-        if ((nx | ny | nz) < 0) return -1;
-        addr += sx ? ax : (sy ? ay : az);
-        mx = sx ? mx + dx : mx;
-        my = sy ? my + dy : my;
-        mz = sz ? mz + dz : mz;
-        // one address for both ends of the cell's range (the second load takes an offset)
-        const int *cs = a.cell_start + (uint32_t)addr;
-        const int i0 = cs[0], i1 = cs[1];
+        // the next cell is found, and its range requested, before this cell's candidates are
+        // tested, so that the range load of one step is in flight during the previous one
+        bool more;
+        OM_WALK_STEP(more);
+        int n0 = 0, n1 = 0;
+        if (more) {
+            const int *ns = a.cell_start + (uint32_t)addr;
+            n0 = ns[0];
+            n1 = ns[1];
+        }
         for (int i = i0; i < i1; ++i) {
             const int h = a.cell_items[i];
             if (h == self) continue;
@@ -202,7 +219,11 @@ __device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
                                    mk(E2.x, E2.y, E2.z), t, u, v))
                 return h;
         }
+        if (!more) return -1;
+        i0 = n0;
+        i1 = n1;
     }
+#undef OM_WALK_STEP
 }
 
 // perturbed mirror normal from the slope-error draws (map_turbulance,
