@@ -210,10 +210,11 @@ __device__ int dda(const TraceArgs &a, v3 o, const cell3 &c, v3 d, int self)
             n1 = ns[1];
         }
         for (int i = i0; i < i1; ++i) {
-            const int h = a.cell_items[i];
+            const float4 *g = a.cell_geo + 3u * (uint32_t)i;
+            const float4 A = g[0];
+            const int h = (int)f2u(A.w);
             if (h == self) continue;
-            const float4 *g = a.geo + 3u * (uint32_t)h;
-            const float4 A = g[0], E1 = g[1], E2 = g[2];
+            const float4 E1 = g[1], E2 = g[2];
             float t, u, v;
             if (parallelogram<gfx>(o, d, mk(A.x, A.y, A.z), mk(E1.x, E1.y, E1.z),
                                    mk(E2.x, E2.y, E2.z), t, u, v))

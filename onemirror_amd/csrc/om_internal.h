@@ -56,6 +56,7 @@ struct TraceArgs {
     int occ_lo[3], occ_hi[3];
     const int *cell_start;
     const int *cell_items;
+    const float4 *cell_geo; // [3 nnz]: (A, bits of the id), (E1, 0), (E2, 0) of cell_items[i]
     const float4 *geo;      // [3H]: (v0, 0), (v1 - v0, 0), (v3 - v0, 0)
     // receiver (receiver.cuh:79-82); cylinder: base of the axis, radius^2, height
     int recv_type;
@@ -181,6 +182,7 @@ struct om_scene {
     // device state
     om::HelioRec *d_helio = nullptr;
     float4 *d_geo = nullptr;
+    std::vector<float4 *> d_cell_geo; // per grid: this scene's vertices of its CSR entries
     float4 *d_buie_bins = nullptr;
     int *d_buie_guide = nullptr;
     int2 *d_buie_split = nullptr;

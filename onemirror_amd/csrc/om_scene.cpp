@@ -876,7 +876,24 @@ om_status upload_geometry(om_scene *s)
     om_status st = upload(s->d_helio, s->recs.data(), s->recs.size());
     if (st != OM_OK) return st;
     std::vector<float4> g = pack_geo(s);
-    return upload(s->d_geo, g.data(), g.size());
+    if ((st = upload(s->d_geo, g.data(), g.size())) != OM_OK) return st;
+    // the blockers of every grid cell in CSR order, each with its id, so that a walk reads
+    // a candidate's vertices without first reading its id
+    const om_scene *root = s->grid_src ? s->grid_src : s;
+    s->d_cell_geo.resize(s->grids.size(), nullptr);
+    for (size_t k = 0; k < s->grids.size(); ++k) {
+        const std::vector<int> &items = root->grids[k].cell_items;
+        std::vector<float4> cg(3 * items.size());
+        for (size_t i = 0; i < items.size(); ++i) {
+            const int h = items[i];
+            cg[3 * i] = g[3 * (size_t)h];
+            std::memcpy(&cg[3 * i].w, &h, 4);
+            cg[3 * i + 1] = g[3 * (size_t)h + 1];
+            cg[3 * i + 2] = g[3 * (size_t)h + 2];
+        }
+        if ((st = upload(s->d_cell_geo[k], cg.data(), cg.size())) != OM_OK) return st;
+    }
+    return OM_OK;
 }
 
 om_status choose_tiles(om_scene *s)
@@ -894,6 +911,7 @@ void free_scene_dev(om_scene *s)
 {
     dfree(s->d_helio);
     dfree(s->d_geo);
+    for (float4 *&p : s->d_cell_geo) dfree(p);
     if (s->diff_buf.p) (void)hipFree(s->diff_buf.p);
     s->diff_buf.p = nullptr;
     s->diff_buf.bytes = 0;
@@ -1181,6 +1199,7 @@ void set_trace_grid(TraceArgs &a, const om_scene *s, int g)
     }
     a.cell_start = D.d_cell_start;
     a.cell_items = D.d_cell_items;
+    a.cell_geo = (size_t)g < s->d_cell_geo.size() ? s->d_cell_geo[g] : nullptr;
 }
 
 TraceArgs make_trace_args(const om_scene *s, int g)
@@ -1414,6 +1433,7 @@ extern "C" om_status om_scene_fork(const om_scene *src, const float sun_dir[3], 
     s->grid_src = src->grid_src ? src->grid_src : src;
     s->d_helio = nullptr;
     s->d_geo = nullptr;
+    s->d_cell_geo.assign(ng, nullptr);
     s->diff_buf = om::DevBuf{};
     s->prm.scene_index = scene_index;
     std::memcpy(s->sun_in, sun_dir, 12);
