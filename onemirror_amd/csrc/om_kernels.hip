@@ -173,14 +173,6 @@ __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile
     return o;
 }
 
-// Ray r = m * L + l of the tile's heliostat.
-template <int SUN, bool POISSON, bool FINE>
-__device__ __forceinline__ RayOut trace_ray(const TraceArgs &a, const HelioTile &t, int m, int l,
-                                            const float4 *s_bins, const int *s_guide)
-{
-    return trace_from<SUN, FINE>(a, t, ray_start<POISSON>(a, t, m), m, l, s_bins, s_guide);
-}
-
 // Adds the hits in the first nk of a thread's hit slots (pixel, or -1 for none) to the maps.
 template <int WGS, int RPTS>
 __device__ __forceinline__ void flush_slots(const TraceArgs &a, const int *s_hpx, const uint32_t *s_hq,
