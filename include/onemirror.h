@@ -254,7 +254,9 @@ om_status om_facet_scene_desc(float a, float b, const float mirror_normal[3], fl
  *  its first heliostat and count)                        "receiver" [4][3] f32 + focus[3]
  *  "buie" cdf_x[b+1], cdf_y[b+1], slope[b] f32         "micro_count" [n] i32
  *  "ray_offset" [n+1] i64      "poisson" [n][max_micro][2] f32 (zero padded)
- *  "sun" sun[3] + buie constants int465, A, B, C, 1/(gamma+1)            */
+ *  "sun" sun[3] + buie constants int465, A, B, C, 1/(gamma+1)
+ *  "helio_clear" [n] i32: 1 where no ray toward the sun from that mirror can meet another
+ *  heliostat (its shading walks are skipped; they would all return -1)   */
 om_status om_scene_export(const om_scene *scene, const char *what, void *host, size_t bytes);
 
 /* Full trace of the field into caller maps (cleared first), then flux scaling and, when

@@ -28,7 +28,8 @@ struct alignas(16) HelioRec {
     int mcount;             // microsamples of this heliostat
     int ray_off_lo, ray_off_hi; // global ray id of (m=0, l=0), for records
     int rows, cols;         // facet rows, facet columns
-    int pad[4];
+    int clear;              // 1: no ray toward the sun from this mirror can meet another heliostat
+    int pad[3];
 };
 static_assert(sizeof(HelioRec) == 128, "HelioRec is 32 words");
 

@@ -38,6 +38,7 @@ struct HelioTile {
     frame hf, sf;
     v3 hpos;
     bool one_row, one_col;
+    bool clear; // the shading walk finds nothing for any ray of this heliostat
 };
 
 __device__ __forceinline__ HelioTile helio_tile(const TraceArgs &a, int h)
@@ -57,6 +58,7 @@ __device__ __forceinline__ HelioTile helio_tile(const TraceArgs &a, int h)
     t.hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
     t.one_row = hr->rows == 1;
     t.one_col = hr->cols == 1;
+    t.clear = hr->clear != 0;
     return t;
 }
 
@@ -161,7 +163,7 @@ __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile
     sincos_small(th, st, ct);
     const v3 to_sun = -to_world(angle2xyz(st, ct, sp, cp), t.sf);
     // the reference order: shading, perturbed normal, reflection, blocking, receiver
-    o.shade = dda(a, orig, c0, to_sun, t.h);
+    o.shade = t.clear ? -1 : dda(a, orig, c0, to_sun, t.h);
     if (o.shade >= 0) return o;
     if (!(SUN == OM_SUN_BUIE && FINE)) nrm = perturbed_normal(x, a.slope_sigma, t.hf);
     const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));

@@ -814,6 +814,190 @@ void flux_scale(om_scene *s)
 
 // HelioRec array and blocker geometry (A, E1 = B - A, E2 = C - A of
 // rayParallelogramIntersect with A = v0, B = v1, C = v3: rectgrid_raytracing.cu:29-31)
+// Largest angle between a sun ray of the sunshape and the sun direction, rounded up: the
+// pillbox's half angle, or the Buie inverse's top, 9.3 mrad (the tail's inverse at the top
+// of the value range is (9.3^(g+1))^(1/(g+1)); the bins stop at 4.65 mrad).
+double sunshape_max_angle(const om_scene *s)
+{
+    const double th = s->prm.sunshape == OM_SUN_BUIE ? 9.3e-3 : (double)s->prm.pillbox_half_angle;
+    return th * 1.01 + 1e-5;
+}
+
+// Heliostats whose shading walk finds nothing whatever the ray: every other heliostat is
+// kept out of the set of rays that leave this mirror toward the sun within the sunshape's
+// largest angle alpha.  Looked along the sun direction d0, a ray from mirror point p that
+// meets heliostat k at x (t = |x - p| <= T, the largest corner distance) projects within
+// T sin(alpha) of p's projection; so k is out of reach when, on the plane normal to d0, the
+// projections of the two parallelograms are farther apart than T alpha + 0.1 m (a
+// separating edge normal of either quadrilateral), or when every point of k lies more than
+// 0.1 m behind every mirror point along every such direction.  Candidates whose plane comes
+// within 0.05 rad of the rays are never cleared.  The 0.1 m and 0.05 rad margins exceed the
+// float error of the kernel's intersection (coordinates within 2^20 m give errors of about
+// 1e-4 m, divided by the sine of the grazing angle), so a cleared heliostat's walk returns
+// -1 for every ray, as the walk itself would.  Only heliostats the rays can reach before
+// climbing above every mirror are candidates (horizontal reach from the sun's elevation
+// less alpha); with the sun within 0.05 rad of that elevation nothing is cleared.
+void mark_clear(om_scene *s)
+{
+    const int H = s->H;
+    for (om::HelioRec &r : s->recs) r.clear = 0;
+    if (H == 0) return;
+    const double alpha = sunshape_max_angle(s);
+    const double sl = std::sqrt((double)s->sun.x * s->sun.x + (double)s->sun.y * s->sun.y +
+                                (double)s->sun.z * s->sun.z);
+    const double d0[3] = {-s->sun.x / sl, -s->sun.y / sl, -s->sun.z / sl};
+    const double emin = std::asin(std::min(1.0, d0[1])) - alpha;
+    if (!(emin > 0.05)) return;
+    const double cot = std::cos(emin) / std::sin(emin);
+    // plane basis normal to d0
+    double e1[3], e2[3];
+    {
+        const double ax[3] = {std::fabs(d0[0]) < 0.9 ? 1.0 : 0.0, std::fabs(d0[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+        e1[0] = d0[1] * ax[2] - d0[2] * ax[1];
+        e1[1] = d0[2] * ax[0] - d0[0] * ax[2];
+        e1[2] = d0[0] * ax[1] - d0[1] * ax[0];
+        const double l = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        for (double &c : e1) c /= l;
+        e2[0] = d0[1] * e1[2] - d0[2] * e1[1];
+        e2[1] = d0[2] * e1[0] - d0[0] * e1[2];
+        e2[2] = d0[0] * e1[1] - d0[1] * e1[0];
+    }
+    // mirror quadrilaterals (ray origins: loc = (+-sx2, sy2, +-sz2) through the frame) and
+    // blocker quadrilaterals (v0, v1, v2, v3), in double
+    struct Quad {
+        double p[4][3];
+        double cx, cz, r; // horizontal centre and radius
+        double ylo, yhi;
+    };
+    auto finish = [](Quad &q) {
+        q.cx = q.cz = 0.0;
+        q.ylo = 1e300;
+        q.yhi = -1e300;
+        for (auto &c : q.p) {
+            q.cx += c[0] / 4;
+            q.cz += c[2] / 4;
+            q.ylo = std::min(q.ylo, c[1]);
+            q.yhi = std::max(q.yhi, c[1]);
+        }
+        q.r = 0.0;
+        for (auto &c : q.p) q.r = std::max(q.r, std::hypot(c[0] - q.cx, c[2] - q.cz));
+    };
+    std::vector<Quad> mir(H), blk(H);
+    double ytop = -1e300;
+    for (int i = 0; i < H; ++i) {
+        const om::HelioRec &r = s->recs[i];
+        om::frame f;
+        f.n = om::mk(r.n[0], r.n[1], r.n[2]);
+        f.u = om::mk(r.fu[0], r.fu[1], r.fu[2]);
+        f.v = om::mk(r.fv[0], r.fv[1], r.fv[2]);
+        f.identity = r.ident;
+        // the mirror grown by one microsample pitch on each side holds every ray origin
+        const float gx = r.sx2 + s->prm.helio_pixel_length, gz = r.sz2 + s->prm.helio_pixel_length;
+        const float sx[4] = {-gx, gx, gx, -gx}, sz[4] = {-gz, -gz, gz, gz};
+        for (int c = 0; c < 4; ++c) {
+            const v3 w = om::to_world(om::mk(sx[c], r.sy2, sz[c]), f);
+            mir[i].p[c][0] = (double)w.x + r.pos[0];
+            mir[i].p[c][1] = (double)w.y + r.pos[1];
+            mir[i].p[c][2] = (double)w.z + r.pos[2];
+        }
+        const v3 q[4] = {s->v0[i], s->v1[i], s->v1[i] + s->v3[i] - s->v0[i], s->v3[i]};
+        for (int c = 0; c < 4; ++c) {
+            blk[i].p[c][0] = q[c].x;
+            blk[i].p[c][1] = q[c].y;
+            blk[i].p[c][2] = q[c].z;
+        }
+        finish(mir[i]);
+        finish(blk[i]);
+        ytop = std::max(ytop, blk[i].yhi);
+    }
+    // horizontal buckets of the blockers
+    double rmax = 0.0, xlo = 1e300, zlo = 1e300, xhi = -1e300, zhi = -1e300;
+    for (const Quad &q : blk) {
+        rmax = std::max(rmax, q.r);
+        xlo = std::min(xlo, q.cx);
+        xhi = std::max(xhi, q.cx);
+        zlo = std::min(zlo, q.cz);
+        zhi = std::max(zhi, q.cz);
+    }
+    const double bs = std::max(8.0, 4.0 * rmax);
+    const int nbx = (int)std::min(4096.0, std::floor((xhi - xlo) / bs) + 1);
+    const int nbz = (int)std::min(4096.0, std::floor((zhi - zlo) / bs) + 1);
+    std::vector<std::vector<int>> bucket((size_t)nbx * nbz);
+    auto bx = [&](double x) { return std::min(nbx - 1, std::max(0, (int)std::floor((x - xlo) / bs))); };
+    auto bz = [&](double z) { return std::min(nbz - 1, std::max(0, (int)std::floor((z - zlo) / bs))); };
+    for (int k = 0; k < H; ++k) bucket[(size_t)bx(blk[k].cx) * nbz + bz(blk[k].cz)].push_back(k);
+    auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    for (int h = 0; h < H; ++h) {
+        const Quad &m = mir[h];
+        const double reach = (ytop - m.ylo) * cot + m.r + rmax + 1.0;
+        if (reach > 1e5) continue;
+        double P[4][2];
+        for (int c = 0; c < 4; ++c) {
+            P[c][0] = dot(m.p[c], e1);
+            P[c][1] = dot(m.p[c], e2);
+        }
+        bool clear = true;
+        for (int ix = bx(m.cx - reach); clear && ix <= bx(m.cx + reach); ++ix)
+            for (int iz = bz(m.cz - reach); clear && iz <= bz(m.cz + reach); ++iz)
+                for (int k : bucket[(size_t)ix * nbz + iz]) {
+                    if (k == h) continue;
+                    const Quad &b = blk[k];
+                    if (std::hypot(b.cx - m.cx, b.cz - m.cz) > reach) continue;
+                    // every point of k behind every mirror point along every ray direction
+                    double T = 0.0, ahead = -1e300;
+                    for (int c = 0; c < 4; ++c)
+                        for (int e = 0; e < 4; ++e) {
+                            const double w[3] = {b.p[c][0] - m.p[e][0], b.p[c][1] - m.p[e][1], b.p[c][2] - m.p[e][2]};
+                            const double l = std::sqrt(dot(w, w));
+                            T = std::max(T, l);
+                            ahead = std::max(ahead, dot(w, d0) + l * alpha);
+                        }
+                    if (ahead < -0.1) continue;
+                    // the rays cross k's plane at more than 0.05 rad
+                    const double E1[3] = {b.p[1][0] - b.p[0][0], b.p[1][1] - b.p[0][1], b.p[1][2] - b.p[0][2]};
+                    const double E2[3] = {b.p[3][0] - b.p[0][0], b.p[3][1] - b.p[0][1], b.p[3][2] - b.p[0][2]};
+                    double nk[3] = {E1[1] * E2[2] - E1[2] * E2[1], E1[2] * E2[0] - E1[0] * E2[2],
+                                    E1[0] * E2[1] - E1[1] * E2[0]};
+                    const double nl = std::sqrt(dot(nk, nk));
+                    if (!(nl > 0.0) || std::fabs(dot(nk, d0)) / nl < std::sin(alpha + 0.05)) {
+                        clear = false;
+                        break;
+                    }
+                    // separating edge normal of the projections, margin T alpha + 0.1
+                    double Q[4][2];
+                    for (int c = 0; c < 4; ++c) {
+                        Q[c][0] = dot(b.p[c], e1);
+                        Q[c][1] = dot(b.p[c], e2);
+                    }
+                    const double margin = T * alpha + 0.1;
+                    bool sep = false;
+                    for (int poly = 0; poly < 2 && !sep; ++poly) {
+                        const double(*R)[2] = poly ? Q : P;
+                        for (int e = 0; e < 4 && !sep; ++e) {
+                            const double ex = R[(e + 1) & 3][0] - R[e][0], ey = R[(e + 1) & 3][1] - R[e][1];
+                            const double l = std::hypot(ex, ey);
+                            if (!(l > 1e-9)) continue;
+                            const double nx = -ey / l, ny = ex / l;
+                            double pa = 1e300, pb = -1e300, qa = 1e300, qb = -1e300;
+                            for (int c = 0; c < 4; ++c) {
+                                const double a1 = P[c][0] * nx + P[c][1] * ny, b1 = Q[c][0] * nx + Q[c][1] * ny;
+                                pa = std::min(pa, a1);
+                                pb = std::max(pb, a1);
+                                qa = std::min(qa, b1);
+                                qb = std::max(qb, b1);
+                            }
+                            sep = pb + margin < qa || qb + margin < pa;
+                        }
+                    }
+                    if (!sep) {
+                        clear = false;
+                        break;
+                    }
+                }
+        s->recs[h].clear = clear ? 1 : 0;
+    }
+}
+
 void pack_heliostats(om_scene *s)
 {
     int H = s->H;
@@ -844,6 +1028,7 @@ void pack_heliostats(om_scene *s)
         r.ray_off_lo = (int)(off & 0xffffffffLL);
         r.ray_off_hi = (int)(off >> 32);
     }
+    mark_clear(s);
 }
 
 std::vector<float4> pack_geo(const om_scene *s)
@@ -1465,6 +1650,9 @@ extern "C" om_status om_scene_export(const om_scene *s, const char *what, void *
         buf.insert(buf.end(), c, c + n);
     };
     if (w == "helio_normal") put(s->normal.data(), 12 * (size_t)s->H);
+    else if (w == "helio_clear") {
+        for (const om::HelioRec &r : s->recs) put(&r.clear, 4);
+    }
     else if (w == "helio_v0") put(s->v0.data(), 12 * (size_t)s->H);
     else if (w == "helio_v1") put(s->v1.data(), 12 * (size_t)s->H);
     else if (w == "helio_v3") put(s->v3.data(), 12 * (size_t)s->H);
