@@ -18,7 +18,9 @@ sincos = 30, log = 25, Philox4x32-10 = 100):
          pair 60, frame 18, normalize 10), reflect + normalize 21, microsample centre 24,
          receiver hit + pixel 64, energy + accumulation 10
     + 12 per DDA cell step + 58 per candidate parallelogram test + 3 per own-heliostat skip,
-with the step/test counts measured by the oracle on the same sample of the field.
+with the step/test counts measured by the oracle on the same sample of the field; the
+shading walks the kernels skip (heliostats no sun ray can leave toward another heliostat)
+are taken out: their steps and tests, and one walk set-up (57) per ray.
 """
 from __future__ import annotations
 
@@ -34,6 +36,17 @@ sys.path.insert(0, ROOT)
 METRIC = "rays/s (w/ DDA shading+blocking) at ≤1e-5 flux err vs FP64; batched & fwd+bwd/s"
 P_VALU = 256 * 64 * 2.4e9          # lane-op/s
 F0, C_STEP, C_TEST, C_SKIP = 569.0, 12.0, 58.0, 3.0
+C_SETUP = 57.0  # one walk set-up, the half of F0's two that a skipped shading walk saves
+
+
+def model_ops(st: dict) -> tuple:
+    """Lane-ops per ray of the cost model from the oracle's counts, with the shading walks of
+    the heliostats the kernels skip (tagged in the oracle) taken out; (ops, S, T)."""
+    n = max(1, st["rays"])
+    S = (st["dda_steps_eff"] - st.get("tag_shade_steps_eff", 0)) / n
+    T = (st["cand_tests"] - st.get("tag_shade_tests", 0)) / n
+    K = (st["self_skips"] - st.get("tag_shade_self_skips", 0)) / n
+    return F0 - C_SETUP * st.get("tag_rays", 0) / n + C_STEP * S + C_TEST * T + C_SKIP * K, S, T
 
 
 def host_threads() -> int:
@@ -161,7 +174,9 @@ def main() -> int:
     blocks = [(h, min(H, h + blk)) for h in range(0, H, blk * every)]
     if args.no_cpu:
         blocks = blocks[:: max(1, len(blocks) // 8)]
-    st_tot = {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0}
+    st_tot = {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0, "tag_rays": 0,
+              "tag_shade_steps_eff": 0, "tag_shade_tests": 0, "tag_shade_self_skips": 0}
+    osc.set_tags(scene.export("helio_clear", 4 * H, np.int32))  # shading walks the kernels skip
     osc.trace_f32(0, min(H, 4), nthreads=nthreads)  # warm-up
     t1 = time.perf_counter()
     for h0, h1 in blocks:
@@ -171,10 +186,7 @@ def main() -> int:
     cpu_t = time.perf_counter() - t1
     cpu_rays = st_tot["rays"]
     nsample = sum(h1 - h0 for h0, h1 in blocks)
-    S = st_tot["dda_steps_eff"] / max(1, st_tot["rays"])
-    T = st_tot["cand_tests"] / max(1, st_tot["rays"])
-    K = st_tot["self_skips"] / max(1, st_tot["rays"])
-    ops_ray = F0 + C_STEP * S + C_TEST * T + C_SKIP * K
+    ops_ray, S, T = model_ops(st_tot)
     achieved = R * ops_ray / (trace_ms * 1e-3) / 1e12
     peak = P_VALU / 1e12
     # HBM bytes per trace launch from the committed rocprof FETCH_SIZE / WRITE_SIZE runs
@@ -368,11 +380,14 @@ def oracle_sample(configs, blocks_per_scene: int, blk: int = 40):
     step/test counts for the cost model and the CPU baseline."""
     import oracle
     nthreads = host_threads()
-    tot = {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0}
+    import onemirror_amd as om
+    tot = {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0, "tag_rays": 0,
+           "tag_shade_steps_eff": 0, "tag_shade_tests": 0, "tag_shade_self_skips": 0}
     secs = 0.0
     nhel = 0
     for d, p in configs:
         o = oracle.OracleScene(d, p)
+        o.set_tags(om.clear_heliostats(d, p))  # shading walks the kernels skip
         H = o.n_helio
         for b in range(blocks_per_scene):
             h0 = (b * H) // max(1, blocks_per_scene)
@@ -454,10 +469,7 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
         sample_cfg = []
     st, cpu_t, nhel, nthreads = oracle_sample(sample_cfg, 1) if sample_cfg else (
         {"rays": 0, "dda_steps_eff": 0, "cand_tests": 0, "self_skips": 0}, 0.0, 0, 0)
-    S = st["dda_steps_eff"] / max(1, st["rays"])
-    T = st["cand_tests"] / max(1, st["rays"])
-    K = st["self_skips"] / max(1, st["rays"])
-    ops_ray = F0 + C_STEP * S + C_TEST * T + C_SKIP * K
+    ops_ray, S, T = model_ops(st)
     achieved = R * ops_ray / (batch_ms * 1e-3) / 1e12
     peak = P_VALU / 1e12
     return {

@@ -330,6 +330,23 @@ def scn_write(path: str, desc: dict):
     _check(lib().om_scn_write(path.encode(), ctypes.byref(d)), "om_scn_write")
 
 
+def clear_heliostats(desc: dict, prm: dict) -> np.ndarray:
+    """int32 [n]: 1 for the heliostats whose shading walks the trace kernels skip (no ray
+    toward the sun from the mirror can meet another heliostat), from a host-only scene
+    (om_scene_create without a context; om_scene_export "helio_clear")."""
+    keep: list = []
+    d = _desc_struct(desc, keep)
+    p = _params_struct(prm)
+    h = _vp()
+    _check(lib().om_scene_create(None, ctypes.byref(d), ctypes.byref(p), ctypes.byref(h)), "om_scene_create")
+    try:
+        out = np.zeros(int(np.asarray(desc["helio_pos"]).shape[0]), np.int32)
+        _check(lib().om_scene_export(h, b"helio_clear", out.ctypes.data_as(_vp), out.nbytes), "om_scene_export")
+        return out
+    finally:
+        lib().om_scene_destroy(h)
+
+
 def master_normal(X: float, Y: float, Z: float, sun_dir) -> np.ndarray:
     """Master normal of the faceted heliostat at (X, Y, Z) for a sun direction: the bisector
     of the directions to the point (0, 13.01, 0) and to the sun (main.cpp:84-91)."""

@@ -83,7 +83,8 @@ class ParamsC(ctypes.Structure):
 class StatsC(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in
                 ("rays", "shaded", "blocked", "hit", "dda_steps", "dda_steps_eff",
-                 "cand_tests", "self_skips")]
+                 "cand_tests", "self_skips", "tag_rays", "tag_shade_steps_eff",
+                 "tag_shade_tests", "tag_shade_self_skips")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -123,6 +124,7 @@ def lib():
     L.oc_scene_poisson.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_int]
     L.oc_scene_poisson.restype = ctypes.c_int
     L.oc_scene_flux_scale.argtypes = [ctypes.c_void_p, _f32p, _f32p]
+    L.oc_scene_set_tags.argtypes = [ctypes.c_void_p, _i32p]
     L.oc_trace_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _u32p, _u64p,
                                _i32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int,
                                ctypes.POINTER(StatsC)]
@@ -289,6 +291,15 @@ class OracleScene:
         if getattr(self, "_h", None) and _LIB is not None:
             _LIB.oc_scene_free(self._h)
             self._h = None
+
+    def set_tags(self, tags=None):
+        """Heliostats whose rays and shading-walk work oc_stats' tag_* fields count."""
+        if tags is None:
+            lib().oc_scene_set_tags(self._h, None)
+            return
+        t = np.ascontiguousarray(tags, np.int32)
+        assert t.shape == (self.n_helio,)
+        lib().oc_scene_set_tags(self._h, _p(t, _i32p))
 
     # --- accessors ---------------------------------------------------------------------
     def helio(self):

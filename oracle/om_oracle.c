@@ -252,6 +252,7 @@ struct oc_scene {
     int *sub_rows, *sub_cols;
     /* flux scale (Gtest.cu:29,37-46) */
     float scale, total_energy;
+    int32_t *tag;             /* [H] or NULL: heliostats counted in oc_stats' tag_* fields */
 };
 
 static f3 ld3(const float *p) { return mk3(p[0], p[1], p[2]); }
@@ -752,7 +753,19 @@ void oc_scene_free(oc_scene *s)
     free(s->cdf_x); free(s->cdf_y); free(s->slope);
     free(s->mcount); free(s->ray_offset); free(s->poisson_pts);
     free(s->subrow_len); free(s->subcol_len); free(s->sub_rows); free(s->sub_cols);
+    free(s->tag);
     free(s);
+}
+
+int oc_scene_set_tags(oc_scene *s, const int32_t *tags)
+{
+    free(s->tag);
+    s->tag = NULL;
+    if (!tags) return 0;
+    s->tag = (int32_t *)malloc(sizeof(int32_t) * (size_t)(s->H > 0 ? s->H : 1));
+    if (!s->tag) return -1;
+    memcpy(s->tag, tags, sizeof(int32_t) * (size_t)s->H);
+    return 0;
 }
 
 int oc_scene_counts(const oc_scene *s, int64_t out[8])
@@ -1036,10 +1049,12 @@ int oc_trace_f32(const oc_scene *s, int h0, int h1,
         uint64_t *le = (uint64_t *)calloc(npx, sizeof(uint64_t));
         oc_stats st;
         memset(&st, 0, sizeof(st));
-        dda_ctr k = {0, 0, 0, 0};
+        dda_ctr k = {0, 0, 0, 0}, kt = {0, 0, 0, 0};
         #pragma omp for schedule(dynamic, 1)
         for (int h = h0; h < h1; ++h) {
             f3 n = s->normal[h];
+            const int tagged = s->tag && s->tag[h];
+            if (tagged) st.tag_rays += (int64_t)s->mcount[h] * L;
             for (int m = 0; m < s->mcount[h]; ++m) {
                 f3 orig = add3(local2world3(micro_local(s, h, m), n), s->pos[h]);
                 for (int l = 0; l < L; ++l) {
@@ -1047,7 +1062,7 @@ int oc_trace_f32(const oc_scene *s, int h0, int h1,
                     st.rays++;
                     f3 to_sun, nrm;
                     sample_ray(s, h, m, l, &to_sun, &nrm);
-                    int sh = dda(s, orig, to_sun, h, early_exit, &k, NULL, 0, NULL);
+                    int sh = dda(s, orig, to_sun, h, early_exit, tagged ? &kt : &k, NULL, 0, NULL);
                     if (rec_shade) rec_shade[rid] = sh;
                     if (sh >= 0) {
                         st.shaded++;
@@ -1078,8 +1093,11 @@ int oc_trace_f32(const oc_scene *s, int h0, int h1,
             if (counts) for (int i = 0; i < npx; ++i) counts[i] += lc[i];
             if (energy) for (int i = 0; i < npx; ++i) energy[i] += le[i];
             tot.rays += st.rays; tot.shaded += st.shaded; tot.blocked += st.blocked;
-            tot.hit += st.hit; tot.dda_steps += k.steps; tot.dda_steps_eff += k.steps_eff;
-            tot.cand_tests += k.tests; tot.self_skips += k.self_skips;
+            tot.hit += st.hit; tot.dda_steps += k.steps + kt.steps;
+            tot.dda_steps_eff += k.steps_eff + kt.steps_eff;
+            tot.cand_tests += k.tests + kt.tests; tot.self_skips += k.self_skips + kt.self_skips;
+            tot.tag_rays += st.tag_rays; tot.tag_shade_steps_eff += kt.steps_eff;
+            tot.tag_shade_tests += kt.tests; tot.tag_shade_self_skips += kt.self_skips;
         }
         free(lc);
         free(le);

@@ -140,7 +140,14 @@ typedef struct {
     int64_t dda_steps_eff; /* cells entered up to the occupied-band early exit           */
     int64_t cand_tests;    /* candidate parallelogram tests                              */
     int64_t self_skips;    /* candidate == own heliostat, skipped                        */
+    /* the rays of tagged heliostats (oc_scene_set_tags) and their shading walks' share of
+     * dda_steps_eff, cand_tests and self_skips (a work count only: walks and results are
+     * the same for every heliostat)                                                      */
+    int64_t tag_rays, tag_shade_steps_eff, tag_shade_tests, tag_shade_self_skips;
 } oc_stats;
+
+/* Tags heliostats for oc_stats' tag_* counts (tags[H], nonzero = tagged; NULL clears). */
+int oc_scene_set_tags(oc_scene *s, const int32_t *tags);
 
 /* FP32 reference-semantics trace of heliostats [h0, h1).
  * counts[res_y*res_x] (u32) and energy[res_y*res_x] (u64 fixed point, 2^-31 units) are

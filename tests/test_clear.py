@@ -7,8 +7,6 @@ would all return -1.  Pinned here against the oracle, which walks every ray: no 
 clear heliostat is shaded in the oracle's records, on fields and suns where shading is
 common, and nothing is marked when the sun is too low for the test.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
@@ -18,19 +16,7 @@ from onemirror_amd import fields as F
 
 
 def _clear(d, p):
-    keep = []
-    dd = om._desc_struct(d, keep)
-    pp = om._params_struct(p)
-    h = ctypes.c_void_p()
-    assert om.lib().om_scene_create(None, ctypes.byref(dd), ctypes.byref(pp), ctypes.byref(h)) == 0, \
-        om.lib().om_last_error()
-    try:
-        n = len(d["helio_pos"])
-        c = np.zeros(n, np.int32)
-        assert om.lib().om_scene_export(h, b"helio_clear", c.ctypes.data, c.nbytes) == 0
-        return c
-    finally:
-        om.lib().om_scene_destroy(h)
+    return om.clear_heliostats(d, p)
 
 
 def _cases():
