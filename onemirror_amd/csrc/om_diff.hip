@@ -151,6 +151,7 @@ struct DiffTile {
     v3 hpos, nth;
     int h, L, nr;
     bool one_row, one_col;
+    bool clear; // the shading walk finds nothing for any ray of this heliostat
 };
 
 template <int SUN, bool POISSON>
@@ -212,6 +213,7 @@ __device__ __forceinline__ DiffTile diff_tile(const TraceArgs &a, const DiffArgs
     T.nr = hr->mcount * a.L;
     T.one_row = hr->rows == 1;
     T.one_col = hr->cols == 1;
+    T.clear = hr->clear != 0;
     return T;
 }
 
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs, DiffArgs)
             u4 x;
             diff_sample<SUN, POISSON>(a, T, m, l, orig, to_sun, x);
             const cell3 c0 = start_cell(a, orig);
-            if (dda(a, orig, c0, to_sun, T.h) < 0) {
+            if (T.clear || dda(a, orig, c0, to_sun, T.h) < 0) {
                 const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
                 if (diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o) && dda(a, orig, c0, o.r, T.h) < 0) {
                     vis = true;
