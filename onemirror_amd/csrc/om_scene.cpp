@@ -10,6 +10,7 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <thread>
 
 #include "om_internal.h"
 
@@ -927,10 +928,10 @@ void mark_clear(om_scene *s)
     auto bz = [&](double z) { return std::min(nbz - 1, std::max(0, (int)std::floor((z - zlo) / bs))); };
     for (int k = 0; k < H; ++k) bucket[(size_t)bx(blk[k].cx) * nbz + bz(blk[k].cz)].push_back(k);
     auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-    for (int h = 0; h < H; ++h) {
+    auto mark = [&](int h) {
         const Quad &m = mir[h];
         const double reach = (ytop - m.ylo) * cot + m.r + rmax + 1.0;
-        if (reach > 1e5) continue;
+        if (reach > 1e5) return;
         double P[4][2];
         for (int c = 0; c < 4; ++c) {
             P[c][0] = dot(m.p[c], e1);
@@ -995,7 +996,19 @@ void mark_clear(om_scene *s)
                     }
                 }
         s->recs[h].clear = clear ? 1 : 0;
+    };
+    // heliostats are independent: a few host threads over interleaved heliostats
+    const int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (nt <= 1 || H < 512) {
+        for (int h = 0; h < H; ++h) mark(h);
+        return;
     }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int h = t; h < H; h += nt) mark(h);
+        });
+    for (std::thread &th : pool) th.join();
 }
 
 void pack_heliostats(om_scene *s)

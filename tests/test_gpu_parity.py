@@ -243,6 +243,26 @@ def test_mid_poisson_counts(mid):
     assert np.array_equal(mc, o.micro()["count"])
 
 
+@pytest.mark.parametrize("elev", [18.0, 35.0])
+def test_clear_heliostats_keep_records_and_maps(om, elev):
+    """Low suns shade much of a field: the kernels skip the shading walks of the heliostats
+    marked clear and walk the rest; records and maps stay the oracle's."""
+    import torch
+    d, p = F.config_mid()
+    d = dict(F.subset(d, slice(2000, 2250)), sun_dir=F.sun_direction(elev, 200.0))
+    clear = om.clear_heliostats(d, p)
+    print(f"clear heliostats elevation {elev}: {int(clear.sum())} of {clear.size}", flush=True)
+    assert 0 < clear.sum() < clear.size
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    orc = o.trace_f32(records=True)
+    _records_equal(g.trace_records(), orc, f"clear/{elev}")
+    assert (orc["shade"] >= 0).sum() > 0
+    maps = g.trace_plain()
+    torch.cuda.synchronize()
+    assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32), orc["counts"])
+    assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64), orc["energy"])
+
+
 @pytest.mark.parametrize("h0", [0, 2400, 4900])
 def test_mid_records_bitexact(mid, h0):
     d, p, g, o = mid
