@@ -65,8 +65,8 @@ def main() -> int:
     ap.add_argument("--workload", default="mid",
                     choices=["mid", "small", "large", "batched", "diff", "annual", "facet_annual", "mid_grids"])
     ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
-    ap.add_argument("--no-grid-tune", action="store_true",
-                    help="batched: keep 128^3 grids instead of tuning the resolution")
+    ap.add_argument("--grid-tune", action="store_true",
+                    help="batched: tune the grid resolution (64, 128, 256) instead of 128^3")
     ap.add_argument("--no-modes", action="store_true",
                     help="mid: skip the batched and fwd+bwd summaries added to the line")
     ap.add_argument("--cpu-every", type=int, default=0,
@@ -402,9 +402,11 @@ def oracle_sample(configs, blocks_per_scene: int, blk: int = 40):
 
 
 def bench_batched(args, with_cpu: bool = True) -> dict:
-    """Batched mode: the scenes of config_batched in one persistent launch replayed from a
-    hipGraph, the grid resolution and launch parameters from the autotuner; the plain path
-    (one om_trace_plain per scene, back to back on the stream) timed on the same scenes."""
+    """Batched mode: the scenes of config_batched (128^3 grids; --grid-tune picks 64, 128 or
+    256 cells per axis by timing the persistent kernel on a sample) in one om_batch replayed
+    from a hipGraph, with the launch the autotuner picks: one persistent launch over every
+    scene, or the window trace of each scene in turn; the plain path (one om_trace_plain per
+    scene, back to back on the stream) timed on the same scenes."""
     import torch
 
     import onemirror_amd as om
@@ -414,7 +416,7 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
     t0 = time.time()
     grid_cells = 128
     tune_grid = None
-    if not args.no_grid_tune:
+    if args.grid_tune:
         tune_grid = om.autotune(cfg, grid_cells=(64, 128, 256), sample=4, reps=1)
         grid_cells = tune_grid["grid_cells"]
         print(f"# grid autotune ({time.time() - t0:.1f}s): "

@@ -141,7 +141,7 @@ class TuneTrial(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("best", Launch), ("best_ms", ctypes.c_float), ("rays_per_s", ctypes.c_double),
-                ("n_trials", ctypes.c_int), ("trials", TuneTrial * 9)]
+                ("n_trials", ctypes.c_int), ("trials", TuneTrial * 10)]
 
 
 _EXPORTS = [
@@ -720,8 +720,12 @@ def _batch_arrays(scenes, maps):
     return maps, hs, fl
 
 
+BATCH_PER_SCENE = 1024  # Batch workgroup: the window trace of each scene in turn (OM_BATCH_PER_SCENE)
+
+
 class Batch:
-    """Many scenes traced by one persistent launch, replayed from a hipGraph (om_batch_*).
+    """Many scenes traced by one persistent launch, or with workgroup=BATCH_PER_SCENE by the
+    window trace of each scene in turn, replayed from a hipGraph (om_batch_*).
 
     ``maps[i]`` receives the maps of ``scenes[i]``, equal to what ``scenes[i].trace_plain``
     gives; they are overwritten by every ``run``."""

@@ -44,6 +44,7 @@ struct om_batch {
     int sunshape = 0;
     bool poisson = false;
     int wgs = 256, rpt = 8;
+    bool per_scene = false;                     // OM_BATCH_PER_SCENE: window traces scene by scene
     int n_items = 0, nblocks = 0, max_npx = 0;
     long long rays = 0;
     std::vector<om_scene *> scenes;
@@ -81,6 +82,20 @@ void batch_free(om_batch *b)
 om_status batch_enqueue(om_batch *b, hipStream_t st)
 {
     om_status rc;
+    if (b->per_scene) {
+        // om_trace_plain of each scene with the LDS-window accumulation
+        for (int i = 0; i < b->n; ++i) {
+            om_scene *s = b->scenes[i];
+            const int keep = s->prm.accumulate;
+            s->prm.accumulate = OM_ACC_WINDOW;
+            rc = om::trace_into(s, 0, s->H, &b->outs[i], true, st);
+            s->prm.accumulate = keep;
+            if (rc != OM_OK) return rc;
+            if ((b->outs[i].flux || b->outs[i].flux_smooth) && (rc = om_flux_finalize(s, &b->outs[i], st)) != OM_OK)
+                return rc;
+        }
+        return OM_OK;
+    }
     if ((rc = om::launch_fill_u32(b->d_queue, 1, st)) != OM_OK) return rc;
     if (b->n_items > 0 &&
         (rc = om::launch_trace_batch(b->sunshape, b->poisson, b->wgs, b->d_table, b->d_items, b->n_items,
@@ -110,8 +125,8 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     b->n = n;
     b->wgs = (lp && lp->workgroup) ? lp->workgroup : 256;
     b->rpt = (lp && lp->rays_per_thread) ? lp->rays_per_thread : 0;
-    if (b->wgs != 128 && b->wgs != 256 && b->wgs != 512) {
-        om::set_error("batch workgroup size must be 128, 256 or 512");
+    if (b->wgs != 128 && b->wgs != 256 && b->wgs != 512 && b->wgs != OM_BATCH_PER_SCENE) {
+        om::set_error("batch workgroup size must be 128, 256, 512 or OM_BATCH_PER_SCENE");
         delete b;
         return OM_ERR_INVALID;
     }
@@ -157,6 +172,17 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
         return e;
     };
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(OM_ERR_HIP);
+    if (b->wgs == OM_BATCH_PER_SCENE) {
+        b->per_scene = true;
+        b->n_items = n;
+        // one eager run sizes the context's shared buffers (window slices, map copies, the
+        // smoothing input), so that the graph capture of the first om_batch_run allocates
+        // nothing
+        if ((rc = batch_enqueue(b, nullptr)) != OM_OK) return fail(rc);
+        if (hipDeviceSynchronize() != hipSuccess) return fail(OM_ERR_HIP);
+        *out = b;
+        return OM_OK;
+    }
     // work items: (table entry, heliostat, first ray) blocks of wgs * rpt rays, scene by
     // scene; a table entry is one grid of one scene (its heliostats walk that grid)
     std::vector<int4> items;
@@ -330,8 +356,8 @@ extern "C" om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, co
     if (!ctx || !result) return OM_ERR_INVALID;
     std::memset(result, 0, sizeof(*result));
     if (reps <= 0) reps = 2;
-    static const int wgs_list[] = {128, 256, 512};
-    static const int rpt_list[] = {2, 4, 8};
+    static const int wgs_list[] = {128, 256, 512, OM_BATCH_PER_SCENE};
+    static const std::vector<int> rpt_list = {2, 4, 8}, rpt_none = {0};
     OM_HIP(hipSetDevice(ctx->device));
     hipStream_t st;
     OM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -341,7 +367,7 @@ extern "C" om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, co
     om_status rc = OM_OK;
     float best = 0.0f;
     for (int w : wgs_list)
-        for (int r : rpt_list) {
+        for (int r : (w == OM_BATCH_PER_SCENE ? rpt_none : rpt_list)) {
             om_launch lp{w, r};
             om_batch *b = nullptr;
             if ((rc = batch_build(ctx, scenes, n, outs, &lp, &b)) != OM_OK) break;

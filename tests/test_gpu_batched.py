@@ -1,4 +1,5 @@
-"""Batched mode on the GPU: one persistent launch over many scenes, replayed from a hipGraph.
+"""Batched mode on the GPU: one persistent launch over many scenes (or the window trace of
+each scene in turn, OM_BATCH_PER_SCENE), replayed from a hipGraph.
 
 Pin: every map of a batch equals the map the plain path gives for that scene alone (same
 Philox streams, integer accumulation), bit for bit, for every launch configuration and on
@@ -48,11 +49,12 @@ def pillbox_batch(om):
     return scenes, [_plain(s) for s in scenes]
 
 
-@pytest.mark.parametrize("wg,rpt", [(0, 0), (128, 2), (512, 8), (256, 1)])
+@pytest.mark.parametrize("wg,rpt", [(0, 0), (128, 2), (512, 8), (256, 1), (1024, 0)])
 def test_batch_graph_replays_equal_plain(om, pillbox_batch, wg, rpt):
     import torch
     scenes, want = pillbox_batch
     b = om.Batch(scenes, workgroup=wg, rays_per_thread=rpt)
+    assert b.workgroup == (wg or 256)
     assert b.items > 0 and b.rays == sum(s.info["total_rays"] for s in scenes)
     for rep in range(3):
         maps = b.run()
@@ -72,7 +74,8 @@ def test_trace_batched_one_shot(om, pillbox_batch):
         _same(m, want[i], f"scene {i}")
 
 
-def test_batch_buie_fields_equal_plain(om):
+@pytest.mark.parametrize("wg", [0, 1024])
+def test_batch_buie_fields_equal_plain(om, wg):
     import torch
     d, p = F.config_mid()
     scenes = [om.SolarScene(F.subset(d, slice(0, 300)), dict(p, scene_index=0))]
@@ -82,7 +85,7 @@ def test_batch_buie_fields_equal_plain(om):
     bd, bp = F.config_batched(n_scenes=2, h_range=(500, 800))[0]
     scenes.append(om.SolarScene(bd, bp))
     want = [_plain(s) for s in scenes]
-    b = om.Batch(scenes)
+    b = om.Batch(scenes, workgroup=wg)
     for rep in range(2):
         maps = b.run()
         torch.cuda.synchronize()
@@ -93,7 +96,7 @@ def test_batch_buie_fields_equal_plain(om):
 def test_autotune_launch_table(om, pillbox_batch):
     scenes, _ = pillbox_batch
     r = om.autotune_launch(scenes, reps=1)
-    assert len(r["trials"]) == 9
+    assert len(r["trials"]) == 10 and (om.BATCH_PER_SCENE, 0) in [t[:2] for t in r["trials"]]
     assert all(ms > 0 for _, _, ms in r["trials"])
     assert (r["workgroup"], r["rays_per_thread"], r["ms"]) in r["trials"]
     assert r["rays_per_s"] > 0
