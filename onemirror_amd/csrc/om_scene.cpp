@@ -1004,10 +1004,17 @@ void mark_clear(om_scene *s)
         return;
     }
     std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t)
-        pool.emplace_back([&, t] {
+    int started = 0;
+    try {
+        for (; started < nt; ++started)
+            pool.emplace_back([&, started] {
+                for (int h = started; h < H; h += nt) mark(h);
+            });
+    } catch (...) {
+        // no more threads: the heliostats of the threads not started are marked here
+        for (int t = started; t < nt; ++t)
             for (int h = t; h < H; h += nt) mark(h);
-        });
+    }
     for (std::thread &th : pool) th.join();
 }
 
