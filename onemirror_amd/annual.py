@@ -17,7 +17,7 @@ from typing import Iterable, List, Optional, Sequence
 
 import numpy as np
 
-from . import Batch, SolarScene, dni_hottel, facet_scene, lib, sun_direction, _f32p
+from . import BATCH_PER_SCENE, Batch, SolarScene, dni_hottel, facet_scene, lib, sun_direction, _f32p
 
 # main.cpp:34-44: the reference's site (Hangzhou) and year
 REFERENCE_SITE = {"latitude": 30.3, "longitude": 120.2, "timezone": 8.0, "elevation_m": 40.0}
@@ -106,8 +106,10 @@ class AnnualRun:
                  workgroup: int = 0, rays_per_thread: int = 0, mode: str = "batch", scenes=None):
         """``mode``: "batch" traces the steps with the batched launch (hipGraph replays),
         "plain" with one om_trace_plain per step, in the accumulation mode of each scene
-        (see SolarScene.autotune_accumulate); the maps are the same.  ``scenes`` (one per
-        step) replaces the forks of ``scene``."""
+        (see SolarScene.autotune_accumulate); the maps are the same.  With ``workgroup`` 0
+        a batch of steps that each hold 1e7 rays or more (each fills the GPU by itself)
+        traces them one window trace after another (BATCH_PER_SCENE), smaller steps share
+        the persistent launch.  ``scenes`` (one per step) replaces the forks of ``scene``."""
         if mode not in ("batch", "plain"):
             raise ValueError("mode is 'batch' or 'plain'")
         self.scene = scene
@@ -126,7 +128,10 @@ class AnnualRun:
         if mode == "batch":
             for b in range(0, len(self.forks), self.batch_size):
                 part = self.forks[b: b + self.batch_size]
-                self._batches.append(Batch(part, workgroup=workgroup, rays_per_thread=rays_per_thread))
+                wg = workgroup
+                if wg == 0 and min(f.info["total_rays"] for f in part) >= 10_000_000:
+                    wg = BATCH_PER_SCENE
+                self._batches.append(Batch(part, workgroup=wg, rays_per_thread=rays_per_thread))
         else:
             self._maps = [f.new_maps(smooth=True) for f in self.forks]
         self.rays = sum(f.info["total_rays"] for f in self.forks)
