@@ -56,6 +56,11 @@ struct om_batch {
     om::ReduceSlot *d_slots = nullptr;
     unsigned long long *d_parts = nullptr;      // copies + carry + wrap of every scene
     float *d_raw = nullptr;
+    // OM_BATCH_PER_SCENE: the batch's own trace workspace (the largest any of its scenes
+    // needs) and smoothing input, so that the captured graph holds no pointer into buffers
+    // of the context, which later calls may reallocate
+    void *d_ws = nullptr;
+    size_t ws_bytes = 0;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -74,6 +79,7 @@ void batch_free(om_batch *b)
     dfree(b->d_slots);
     dfree(b->d_parts);
     dfree(b->d_raw);
+    if (b->d_ws) (void)hipFree(b->d_ws);
     delete b;
 }
 
@@ -83,15 +89,20 @@ om_status batch_enqueue(om_batch *b, hipStream_t st)
 {
     om_status rc;
     if (b->per_scene) {
-        // om_trace_plain of each scene with the LDS-window accumulation
+        // om_trace_plain of each scene with the LDS-window accumulation, in the batch's
+        // own workspace
+        om::Workspace ws;
+        ws.p = b->d_ws;
+        ws.bytes = b->ws_bytes;
         for (int i = 0; i < b->n; ++i) {
             om_scene *s = b->scenes[i];
             const int keep = s->prm.accumulate;
             s->prm.accumulate = OM_ACC_WINDOW;
-            rc = om::trace_into(s, 0, s->H, &b->outs[i], true, st);
+            rc = om::trace_into(s, 0, s->H, &b->outs[i], true, st, ws);
             s->prm.accumulate = keep;
             if (rc != OM_OK) return rc;
-            if ((b->outs[i].flux || b->outs[i].flux_smooth) && (rc = om_flux_finalize(s, &b->outs[i], st)) != OM_OK)
+            if ((b->outs[i].flux || b->outs[i].flux_smooth) &&
+                (rc = om::flux_finalize(s, &b->outs[i], b->d_raw, st)) != OM_OK)
                 return rc;
         }
         return OM_OK;
@@ -175,11 +186,23 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     if (b->wgs == OM_BATCH_PER_SCENE) {
         b->per_scene = true;
         b->n_items = n;
-        // one eager run sizes the context's shared buffers (window slices, map copies, the
-        // smoothing input), so that the graph capture of the first om_batch_run allocates
-        // nothing
-        if ((rc = batch_enqueue(b, nullptr)) != OM_OK) return fail(rc);
-        if (hipDeviceSynchronize() != hipSuccess) return fail(OM_ERR_HIP);
+        // workspace and smoothing input of the largest scene, cleared once (each decode
+        // leaves the words it reads cleared)
+        for (int i = 0; i < n; ++i) {
+            om_scene *s = scenes[i];
+            const int keep = s->prm.accumulate;
+            s->prm.accumulate = OM_ACC_WINDOW;
+            const size_t need = om::trace_workspace_bytes(s);
+            s->prm.accumulate = keep;
+            if (need == 0) return fail(OM_ERR_UNSUPPORTED);
+            b->ws_bytes = std::max(b->ws_bytes, need);
+        }
+        if (hipMalloc(&b->d_ws, b->ws_bytes) != hipSuccess)
+            return fail(om::hip_fail(hipGetLastError(), "hipMalloc (batch workspace)"));
+        if ((rc = dalloc(b->d_raw, (size_t)b->max_npx)) != OM_OK) return fail(rc);
+        if ((rc = om::launch_fill_u32((uint32_t *)b->d_ws, (long long)(b->ws_bytes / 4), nullptr)) != OM_OK)
+            return fail(rc);
+        if (hipDeviceSynchronize() != hipSuccess) return fail(om::hip_fail(hipGetLastError(), "batch set-up"));
         *out = b;
         return OM_OK;
     }

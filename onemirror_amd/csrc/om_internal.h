@@ -116,6 +116,20 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Device memory a trace uses beyond its inputs and outputs: the packed map copies with their
+// carry and wrap maps, the window queue counters and window slices.  Either a library buffer
+// that grows on demand (grow; the context's, for calls that may allocate) or fixed memory
+// (p, bytes) that a trace never outgrows: a batch's own buffer, sized when the batch is
+// built, or the static scratch of the allocation-free entry point.  The decode leaves every
+// word it reads cleared, so fixed memory private to one user is cleared once; memory shared
+// by callers (clear_each) is cleared at the start of every trace.
+struct Workspace {
+    DevBuf *grow = nullptr;
+    void *p = nullptr;
+    size_t bytes = 0;
+    bool clear_each = false;
+};
+
 // One uniform grid of a field (RectGrid, grid.h:25-49) and the heliostats [h0, h1) it holds:
 // per-cell CSR lists of global heliostat ids (CGridHelioMatch, grid.cpp:47-82), the range of
 // occupied cell indices, and the work tiles [t0, t1) of its heliostats.  The rays of a
@@ -225,7 +239,16 @@ int trace_batch_blocks_per_cu(int sunshape, bool poisson, int wgs);
 om_status launch_reduce_batch(const ReduceSlot *slots, int nslot, int max_npx, hipStream_t st);
 om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st);
 constexpr int RPT_LIMIT = 8; // rays per thread a trace workgroup holds hit slots for
-om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st);
+// trace heliostats [h0, h1) into out (cleared first when clear is set), with the workspace ws
+om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st,
+                     const Workspace &ws);
+// the context's growing workspace (calls that may allocate)
+Workspace ctx_workspace(om_ctx *ctx);
+// bytes of workspace a trace of s needs (0 when s cannot be traced with its accumulation)
+size_t trace_workspace_bytes(om_scene *s);
+// flux (and, with a smoothing output, the smoothing input raw[res_x * res_y]) from the
+// energy map of out: the scaling loop of Gtest.cu:37-46 and image_smooth
+om_status flux_finalize(om_scene *s, const om_flux *out, float *raw, hipStream_t st);
 } // namespace om
 
 #define OM_HIP(call)                                                                       \

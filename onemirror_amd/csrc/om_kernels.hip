@@ -1133,8 +1133,61 @@ void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1)
 using namespace om;
 
 namespace om {
-// heliostats [h0, h1), all held by grid g, accumulated into out
-static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *out, hipStream_t st)
+// Layout of a trace's workspace for scene s: 8 << shift packed copies of the map (same-pixel
+// atomics of concurrent workgroups then mostly land on different words), the carry and wrap
+// maps, and in window mode the 8 queue counters and one window slice per persistent
+// workgroup.  The copies double while twice as many still fit in copy_budget bytes (at least
+// one per XCD) and, with a cap on the total, halve again until the whole layout fits.
+struct WsPlan {
+    int shift = 0;
+    size_t n16 = 0, bytes = 0, off_queue = 0, off_win = 0;
+    WindowRef win{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
+    int nwg = 0;
+};
+
+static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
+{
+    const size_t npx = (size_t)s->res_x * s->res_y;
+    p = WsPlan{};
+    p.win.res_x = s->res_x;
+    if (s->prm.accumulate == OM_ACC_WINDOW) {
+        om_status rc = window_plan(s, p.win, p.nwg);
+        if (rc != OM_OK) return rc;
+    }
+    int shift = 0;
+    while (shift < COPY_SHIFT_MAX && ((size_t)16 << shift) * npx * sizeof(unsigned long long) <= copy_budget) ++shift;
+    for (;; --shift) {
+        const size_t ncopy = (size_t)8 << shift;
+        p.shift = shift;
+        p.n16 = ncopy * npx;
+        p.off_queue = p.n16 * sizeof(unsigned long long) + 2 * npx * sizeof(uint32_t);
+        p.off_win = p.off_queue + 64;
+        p.bytes = p.nwg > 0 ? p.off_win + (size_t)p.nwg * p.win.w * p.win.h * sizeof(unsigned long long)
+                            : p.off_queue;
+        if (p.bytes <= cap) return OM_OK;
+        if (shift == 0) break;
+    }
+    set_error("the trace needs " + std::to_string(p.bytes) + " bytes of workspace, more than the " +
+              std::to_string(cap) + " available");
+    return OM_ERR_SIZE;
+}
+
+size_t trace_workspace_bytes(om_scene *s)
+{
+    WsPlan p;
+    return ws_plan(s, PART_BYTES, ~(size_t)0, p) == OM_OK ? p.bytes : 0;
+}
+
+Workspace ctx_workspace(om_ctx *ctx)
+{
+    Workspace w;
+    w.grow = &ctx->parts;
+    return w;
+}
+
+// heliostats [h0, h1), all held by grid g, accumulated into out through the workspace plan p
+static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *out, hipStream_t st,
+                            const WsPlan &p, char *base)
 {
     TraceArgs a = make_trace_args(s, g);
     int t0, t1;
@@ -1144,54 +1197,30 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
     const int npx = s->res_x * s->res_y;
     om_status rc;
     if (s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW) {
-        // 8 XCDs x 2^shift copies of the packed map, as many as fit in PART_BYTES (at least
-        // one per XCD): same-pixel atomics of concurrent workgroups then mostly land on
-        // different words.
-        int shift = 0;
-        while (shift < COPY_SHIFT_MAX && ((size_t)16 << shift) * npx * sizeof(unsigned long long) <= PART_BYTES)
-            ++shift;
-        const int ncopy = 8 << shift;
-        const int copy_stride = npx, px_stride = 1;
-        const size_t n16 = (size_t)(ncopy - 1) * copy_stride + (size_t)(npx - 1) * px_stride + 1;
-        size_t bytes = n16 * sizeof(unsigned long long) + 2 * (size_t)npx * sizeof(uint32_t);
-        WindowRef win{nullptr, nullptr, 0, 0, 0, 0, 0, s->res_x};
-        int nwg = 0;
-        if (s->prm.accumulate == OM_ACC_WINDOW && a.ntiles > 0 &&
-            (rc = window_plan(s, win, nwg)) != OM_OK)
-            return rc;
-        // window mode: queue counters | window slices
-        const size_t off_queue = bytes, off_win = bytes + 64;
-        if (nwg > 0) bytes = off_win + (size_t)nwg * win.w * win.h * sizeof(unsigned long long);
-        const bool fresh = s->ctx->parts.bytes < bytes;
-        if ((rc = ensure_buf(s->ctx->parts, bytes)) != OM_OK) return rc;
-        char *base = (char *)s->ctx->parts.p;
+        const int ncopy = 8 << p.shift;
         a.part = (unsigned long long *)base;
-        a.carry = (uint32_t *)(a.part + n16);
+        a.carry = (uint32_t *)(a.part + p.n16);
         a.wrap = a.carry + npx;
         a.npx = npx;
-        a.copy_shift = shift;
-        a.copy_stride = copy_stride;
-        a.px_stride = px_stride;
+        a.copy_shift = p.shift;
+        a.copy_stride = npx;
+        a.px_stride = 1;
         a.acc_packed = 1;
         a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
-        if (fresh) {
-            // the reduction leaves the words cleared; a new buffer is cleared once here
-            dim3 g((unsigned)((bytes / 4 + WG - 1) / WG)), b(WG);
-            hipLaunchKernelGGL(fill_u32_kernel, g, b, 0, st, (uint32_t *)s->ctx->parts.p, (long long)(bytes / 4));
-            OM_HIP(hipGetLastError());
-        }
-        if (nwg > 0) {
-            a.win_queue = (uint32_t *)(base + off_queue);
-            a.win_part = (unsigned long long *)(base + off_win);
-            win.queue = a.win_queue;
+        WindowRef win = p.win;
+        win.nwin = 0;
+        if (p.nwg > 0 && a.ntiles > 0) {
+            a.win_queue = (uint32_t *)(base + p.off_queue);
+            a.win_part = (unsigned long long *)(base + p.off_win);
             a.win_x0 = win.x0;
             a.win_y0 = win.y0;
             a.win_w = win.w;
             a.win_h = win.h;
+            win.queue = a.win_queue;
             win.part = a.win_part;
-            win.nwin = nwg;
+            win.nwin = p.nwg;
             if ((rc = launch_fill_u32(a.win_queue, 8, st)) != OM_OK) return rc;
-            if ((rc = launch_trace_window(s, a, nwg, st)) != OM_OK) return rc;
+            if ((rc = launch_trace_window(s, a, p.nwg, st)) != OM_OK) return rc;
         } else if ((rc = launch_trace(s, a, false, st)) != OM_OK) {
             return rc;
         }
@@ -1206,17 +1235,46 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
     return launch_trace(s, a, false, st);
 }
 
-// trace heliostats [h0, h1) into out (cleared first when clear is set)
-om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st)
+om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st,
+                     const Workspace &ws)
 {
     om_status rc;
     if (clear && (rc = clear_maps(out, s->res_x * s->res_y, st)) != OM_OK) return rc;
+    WsPlan p;
+    char *base = nullptr;
+    if (s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW) {
+        if ((rc = ws_plan(s, PART_BYTES, ws.grow ? ~(size_t)0 : ws.bytes, p)) != OM_OK) return rc;
+        size_t clear_bytes = 0;
+        if (ws.grow) {
+            // a new buffer is cleared once; the decode leaves the words it reads cleared
+            if (ws.grow->bytes < p.bytes) clear_bytes = p.bytes;
+            if ((rc = ensure_buf(*ws.grow, p.bytes)) != OM_OK) return rc;
+            base = (char *)ws.grow->p;
+        } else {
+            base = (char *)ws.p;
+            if (ws.clear_each) clear_bytes = p.off_queue; // copies, carry and wrap maps
+        }
+        if (clear_bytes > 0 && (rc = launch_fill_u32((uint32_t *)base, (long long)(clear_bytes / 4), st)) != OM_OK)
+            return rc;
+    }
     // one launch (and decode, which adds into out) per grid: a grid's CSR holds its own
     // heliostats only
     for (size_t g = 0; g < s->grids.size(); ++g) {
         const int a0 = std::max(h0, s->grids[g].h0), a1 = std::min(h1, s->grids[g].h1);
-        if (a0 < a1 && (rc = trace_grid(s, (int)g, a0, a1, out, st)) != OM_OK) return rc;
+        if (a0 < a1 && (rc = trace_grid(s, (int)g, a0, a1, out, st, p, base)) != OM_OK) return rc;
     }
+    return OM_OK;
+}
+
+om_status flux_finalize(om_scene *s, const om_flux *out, float *raw, hipStream_t st)
+{
+    const int n = s->res_x * s->res_y;
+    om_status rc = launch_finalize((const unsigned long long *)out->energy, out->flux_smooth ? raw : nullptr,
+                                   out->flux, s->scale, n, st);
+    if (rc != OM_OK) return rc;
+    if (out->flux_smooth)
+        return launch_smooth(raw, out->flux_smooth, s->res_x, s->res_y, s->prm.smooth_radius, s->prm.smooth_ratio,
+                             s->scale, st);
     return OM_OK;
 }
 } // namespace om
@@ -1235,7 +1293,7 @@ extern "C" om_status om_trace_heliostats(om_scene *s, int h0, int h1, const om_f
         return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
-    return trace_into(s, h0, h1, out, false, (hipStream_t)stream);
+    return trace_into(s, h0, h1, out, false, (hipStream_t)stream, ctx_workspace(s->ctx));
 }
 
 extern "C" om_status om_flux_finalize(om_scene *s, const om_flux *out, void *stream)
@@ -1248,19 +1306,12 @@ extern "C" om_status om_flux_finalize(om_scene *s, const om_flux *out, void *str
         return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
-    int n = s->res_x * s->res_y;
-    hipStream_t hs = (hipStream_t)stream;
     float *raw = nullptr;
     if (out->flux_smooth) {
-        if ((st = ensure_buf(s->ctx->scratch, sizeof(float) * (size_t)n)) != OM_OK) return st;
+        if ((st = ensure_buf(s->ctx->scratch, sizeof(float) * (size_t)s->res_x * s->res_y)) != OM_OK) return st;
         raw = (float *)s->ctx->scratch.p;
     }
-    if ((st = launch_finalize((const unsigned long long *)out->energy, raw, out->flux, s->scale, n, hs)) != OM_OK)
-        return st;
-    if (out->flux_smooth)
-        return launch_smooth(raw, out->flux_smooth, s->res_x, s->res_y, s->prm.smooth_radius,
-                             s->prm.smooth_ratio, s->scale, hs);
-    return OM_OK;
+    return flux_finalize(s, out, raw, (hipStream_t)stream);
 }
 
 extern "C" om_status om_trace_plain(om_scene *s, const om_flux *out, void *stream)
@@ -1273,7 +1324,7 @@ extern "C" om_status om_trace_plain(om_scene *s, const om_flux *out, void *strea
         return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(s->ctx->device));
-    if ((st = trace_into(s, 0, s->H, out, true, (hipStream_t)stream)) != OM_OK) return st;
+    if ((st = trace_into(s, 0, s->H, out, true, (hipStream_t)stream, ctx_workspace(s->ctx))) != OM_OK) return st;
     if (out->flux || out->flux_smooth) return om_flux_finalize(s, out, stream);
     return OM_OK;
 }
@@ -1302,11 +1353,11 @@ extern "C" om_status om_autotune_accumulate(om_scene *s, const om_flux *maps, in
     for (int i = 0; i < 2 && st == OM_OK; ++i) {
         s->prm.accumulate = modes[i];
         // warm-up (buffers, first launch); a mode that cannot run on this scene is skipped
-        if (trace_into(s, 0, s->H, maps, true, nullptr) != OM_OK) continue;
+        if (trace_into(s, 0, s->H, maps, true, nullptr, ctx_workspace(s->ctx)) != OM_OK) continue;
         for (int r = 0; r < reps && st == OM_OK; ++r) {
             float ms1 = 0.0f;
             if ((he = hipEventRecord(e0, nullptr)) != hipSuccess) break;
-            if ((st = trace_into(s, 0, s->H, maps, true, nullptr)) != OM_OK) break;
+            if ((st = trace_into(s, 0, s->H, maps, true, nullptr, ctx_workspace(s->ctx))) != OM_OK) break;
             if ((he = hipEventRecord(e1, nullptr)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess ||
                 (he = hipEventElapsedTime(&ms1, e0, e1)) != hipSuccess)
                 break;

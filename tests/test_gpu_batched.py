@@ -110,3 +110,33 @@ def test_batch_rejects_mixed_sunshapes(om):
         om.Batch([a, b])
     with pytest.raises(om.OMError, match="workgroup"):
         om.Batch([a], workgroup=96)
+
+
+def test_per_scene_batch_survives_context_buffer_growth(om):
+    """A per-scene batch (OM_BATCH_PER_SCENE) traces in its own workspace: a later plain
+    trace of a larger map on the same context, which reallocates the context's buffers
+    (map copies, window slices, smoothing input), leaves the batch's graph replays correct,
+    and so does a second per-scene batch run between them."""
+    import torch
+    ctx = om.Context(0)
+    d, p = F.config_small()
+    small = [om.SolarScene(d, dict(p, scene_index=i), ctx=ctx) for i in range(2)]
+    want = [_plain(s) for s in small]
+    b = om.Batch(small, workgroup=om.BATCH_PER_SCENE)
+    maps = b.run()
+    torch.cuda.synchronize()
+    for i, m in enumerate(maps):
+        _same(m, want[i], f"scene {i} first run")
+    # a larger map on the same context: 300 heliostats of the mid field, 256 x 256 map,
+    # window accumulation
+    dl, pl = F.config_mid()
+    big = om.SolarScene(F.subset(dl, slice(0, 300)), dict(pl, accumulate=om.ACC_WINDOW), ctx=ctx)
+    assert big.info["res_x"] * big.info["res_y"] > small[0].info["res_x"] * small[0].info["res_y"]
+    big.trace_plain()
+    other = om.Batch([big], workgroup=om.BATCH_PER_SCENE)
+    other.run()
+    for rep in range(2):
+        maps = b.run()
+        torch.cuda.synchronize()
+        for i, m in enumerate(maps):
+            _same(m, want[i], f"scene {i} after growth, replay {rep}")
