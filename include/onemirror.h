@@ -266,6 +266,21 @@ om_status om_scene_export(const om_scene *scene, const char *what, void *host, s
  * -> map_tracing (rectgrid_raytracing.cu:155-196) in one launch. */
 om_status om_trace_plain(om_scene *scene, const om_flux *out, void *stream);
 
+/* Allocation-free variant of om_trace_plain, with no counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same outputs as om_trace_plain, bit for bit; it
+ * calls no hipMalloc or hipFree.  Its workspace is a static device array of
+ * OM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: the packed map copies (8 to
+ * 64 of 8 bytes per pixel, fewer when space is short), 8 bytes per pixel of side maps, in
+ * OM_ACC_WINDOW mode one window slice of up to 147,200 bytes per persistent workgroup (one
+ * per CU), and with flux_smooth 4 bytes per pixel of smoothing input.  A scene whose
+ * workspace does not fit gets OM_ERR_SIZE before anything is launched (with 256 CUs: maps up
+ * to about 880,000 pixels in OM_ACC_PACKED mode, 400,000 in OM_ACC_WINDOW mode).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race and give wrong maps. */
+#define OM_NOALLOC_SCRATCH_BYTES (64u << 20)
+om_status om_trace_plain_noalloc(om_scene *scene, const om_flux *out, void *stream);
+
 /* Trace heliostats [h0, h1) and accumulate (no clear) into counts/energy: the per-heliostat
  * entry recthelio_ray_tracing (recthelio_tracing.cpp:46-79; recthelio_tracing.h:16). */
 om_status om_trace_heliostats(om_scene *scene, int h0, int h1, const om_flux *out, void *stream);

@@ -154,7 +154,7 @@ _EXPORTS = [
     "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
     "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel", "om_scn_write",
     "om_master_normal", "om_facet_heliostat", "om_facet_scene_desc", "om_savetxt_map", "om_savetxt_ratio",
-    "om_load_normals",
+    "om_load_normals", "om_trace_plain_noalloc",
 ]
 
 
@@ -197,6 +197,7 @@ def lib():
     L.om_dni_hottel.restype = ctypes.c_float
     L.om_scene_export.argtypes = [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]
     L.om_trace_plain.argtypes = [_vp, ctypes.POINTER(Flux), _vp]
+    L.om_trace_plain_noalloc.argtypes = [_vp, ctypes.POINTER(Flux), _vp]
     L.om_autotune_accumulate.argtypes = [_vp, ctypes.POINTER(Flux), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]
     L.om_trace_heliostats.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Flux), _vp]
@@ -558,6 +559,15 @@ class SolarScene:
         f = maps.struct(with_flux=finalize)
         _check(lib().om_trace_plain(self._h, ctypes.byref(f), _stream_ptr(stream)),
                "om_trace_plain")
+        return maps
+
+    def trace_plain_noalloc(self, maps: FluxMaps, stream=None, finalize: bool = True) -> FluxMaps:
+        """trace_plain into caller maps without allocating device memory: the workspace is
+        the library's static per-GPU array (om_trace_plain_noalloc).  Calls that overlap on
+        one GPU race."""
+        f = maps.struct(with_flux=finalize)
+        _check(lib().om_trace_plain_noalloc(self._h, ctypes.byref(f), _stream_ptr(stream)),
+               "om_trace_plain_noalloc")
         return maps
 
     def trace_heliostats(self, h0: int, h1: int, maps: FluxMaps, stream=None) -> FluxMaps:

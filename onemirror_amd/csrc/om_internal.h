@@ -155,6 +155,7 @@ struct om_ctx {
     om::DevBuf scratch2;
     om::DevBuf parts;       // per-XCD partial maps
     int win_wgs = -1;       // persistent workgroups of window accumulation (-1: not looked up)
+    void *static_scratch = nullptr; // this device's address of the allocation-free workspace
 };
 
 struct om_scene {
@@ -242,6 +243,9 @@ constexpr int RPT_LIMIT = 8; // rays per thread a trace workgroup holds hit slot
 // trace heliostats [h0, h1) into out (cleared first when clear is set), with the workspace ws
 om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st,
                      const Workspace &ws);
+// device address of the static workspace of om_trace_plain_noalloc on device (current),
+// looked up once per device
+om_status static_scratch(int device, void **addr);
 // the context's growing workspace (calls that may allocate)
 Workspace ctx_workspace(om_ctx *ctx);
 // bytes of workspace a trace of s needs (0 when s cannot be traced with its accumulation)

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "om_device.h"
@@ -1239,11 +1241,13 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
                      const Workspace &ws)
 {
     om_status rc;
-    if (clear && (rc = clear_maps(out, s->res_x * s->res_y, st)) != OM_OK) return rc;
     WsPlan p;
     char *base = nullptr;
-    if (s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW) {
-        if ((rc = ws_plan(s, PART_BYTES, ws.grow ? ~(size_t)0 : ws.bytes, p)) != OM_OK) return rc;
+    const bool packed = s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW;
+    // a workspace that is too small fails before anything is launched
+    if (packed && (rc = ws_plan(s, PART_BYTES, ws.grow ? ~(size_t)0 : ws.bytes, p)) != OM_OK) return rc;
+    if (clear && (rc = clear_maps(out, s->res_x * s->res_y, st)) != OM_OK) return rc;
+    if (packed) {
         size_t clear_bytes = 0;
         if (ws.grow) {
             // a new buffer is cleared once; the decode leaves the words it reads cleared
@@ -1326,6 +1330,68 @@ extern "C" om_status om_trace_plain(om_scene *s, const om_flux *out, void *strea
     OM_HIP(hipSetDevice(s->ctx->device));
     if ((st = trace_into(s, 0, s->H, out, true, (hipStream_t)stream, ctx_workspace(s->ctx))) != OM_OK) return st;
     if (out->flux || out->flux_smooth) return om_flux_finalize(s, out, stream);
+    return OM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free trace: the workspace is a static device array
+// ---------------------------------------------------------------------------------------
+// The map copies, side maps, window queue and slices of om_trace_plain_noalloc, and its
+// smoothing input at the end.  Each device holds its own copy for the life of the process.
+// In host code `scratch` names a host-side stand-in, not device memory: host code uses the
+// per-device address that static_scratch looks up, never the array's name.
+static __device__ __align__(256) unsigned long long scratch[OM_NOALLOC_SCRATCH_BYTES / 8];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_addr; // [device] device address of scratch, null before lookup
+} // namespace
+
+om_status om::static_scratch(int device, void **addr)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return OM_ERR_INVALID;
+    if ((int)g_scratch_addr.size() <= device) g_scratch_addr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_addr[device] = p;
+    }
+    *addr = g_scratch_addr[device];
+    return OM_OK;
+}
+
+extern "C" om_status om_trace_plain_noalloc(om_scene *s, const om_flux *out, void *stream)
+{
+    if (!s) return OM_ERR_INVALID;
+    om_status st = check_flux(s, out);
+    if (st != OM_OK) return st;
+    if (!s->ctx) {
+        set_error("host-only scene (created without a context) cannot be traced");
+        return OM_ERR_INVALID;
+    }
+    if (!s->ctx->static_scratch) {
+        set_error("the context has no static scratch");
+        return OM_ERR_INVALID;
+    }
+    const size_t npx = (size_t)s->res_x * s->res_y;
+    // smoothing input at the end of the scratch, the trace workspace before it
+    const size_t raw_bytes = out->flux_smooth ? ((npx * sizeof(float) + 255) & ~(size_t)255) : 0;
+    if (raw_bytes >= OM_NOALLOC_SCRATCH_BYTES) {
+        set_error("the smoothing input of a " + std::to_string(npx) + "-pixel map does not fit in the " +
+                  std::to_string((size_t)OM_NOALLOC_SCRATCH_BYTES) + "-byte static scratch");
+        return OM_ERR_SIZE;
+    }
+    OM_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t hs = (hipStream_t)stream;
+    Workspace ws;
+    ws.p = s->ctx->static_scratch;
+    ws.bytes = OM_NOALLOC_SCRATCH_BYTES - raw_bytes;
+    ws.clear_each = true; // every call on the device shares the array
+    if ((st = trace_into(s, 0, s->H, out, true, hs, ws)) != OM_OK) return st;
+    if (out->flux || out->flux_smooth)
+        return flux_finalize(s, out, out->flux_smooth ? (float *)((char *)ws.p + ws.bytes) : nullptr, hs);
     return OM_OK;
 }
 

@@ -114,8 +114,12 @@ extern "C" om_status om_ctx_create(int device, om_ctx **out)
         return OM_ERR_INVALID;
     }
     OM_HIP(hipSetDevice(device));
+    void *sp = nullptr;
+    const om_status rc = om::static_scratch(device, &sp);
+    if (rc != OM_OK) return rc;
     om_ctx *c = new om_ctx();
     c->device = device;
+    c->static_scratch = sp;
     *out = c;
     return OM_OK;
 }
