@@ -115,6 +115,14 @@ typedef struct {
                                   the reference traces grid 0 (Gtest.cu:12,21-23; default
                                   0); OM_GRIDS_ALL: every grid, each heliostat's rays
                                   walking the grid that holds it                          */
+    int   literal_self;        /* 0 (default): a walk skips the ray's own heliostat, the
+                                  exact-arithmetic reading of the t >= 1e-6 test of
+                                  rayParallelogramIntersect (global_function.cuh:102), since
+                                  the ray starts on that mirror's plane; 1: the own heliostat
+                                  is tested like any candidate, as the reference's Intersect
+                                  loop does (rectgrid_raytracing.cu:21-36), so that an FP32
+                                  rounding of t to >= 1e-6 stops the ray at its own mirror.
+                                  Not supported by the differentiable mode.              */
 } om_params;
 
 enum { OM_GRIDS_ALL = -1 };

@@ -89,6 +89,7 @@ class Params(ctypes.Structure):
         ("accumulate", ctypes.c_int),
         ("pack_shift", ctypes.c_int),
         ("grid_select", ctypes.c_int),
+        ("literal_self", ctypes.c_int),
     ]
 
 

@@ -632,6 +632,10 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
         set_error("differentiable mode supports rectangle receivers");
         return OM_ERR_UNSUPPORTED;
     }
+    if (s->prm.literal_self) {
+        set_error("differentiable mode skips the ray's own heliostat (literal_self is not supported)");
+        return OM_ERR_UNSUPPORTED;
+    }
     if (!s->normal_in.empty()) {
         set_error("differentiable mode aims the heliostats itself (scene has given normals)");
         return OM_ERR_UNSUPPORTED;

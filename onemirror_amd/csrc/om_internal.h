@@ -96,6 +96,7 @@ struct TraceArgs {
     unsigned long long *win_part;
     uint32_t *win_queue;      // [8] chunks taken from each queue, zero before the launch
     int win_x0, win_y0, win_w, win_h;
+    int literal_self;         // walks test the ray's own heliostat (om_params.literal_self)
 };
 
 // One map of a batch for the decode (reduce_batch_kernel): packed copies and side maps,

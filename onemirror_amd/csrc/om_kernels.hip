@@ -41,6 +41,7 @@ struct HelioTile {
     v3 hpos;
     bool one_row, one_col;
     bool clear; // the shading walk finds nothing for any ray of this heliostat
+    int self;   // the id a walk skips: the heliostat's own, or -1 (none) for the literal rule
 };
 
 __device__ __forceinline__ HelioTile helio_tile(const TraceArgs &a, int h)
@@ -60,7 +61,10 @@ __device__ __forceinline__ HelioTile helio_tile(const TraceArgs &a, int h)
     t.hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
     t.one_row = hr->rows == 1;
     t.one_col = hr->cols == 1;
-    t.clear = hr->clear != 0;
+    // under the literal rule a walk may stop at the ray's own heliostat, which the clear
+    // test (other heliostats only) does not cover
+    t.clear = hr->clear != 0 && !a.literal_self;
+    t.self = a.literal_self ? -1 : h;
     return t;
 }
 
@@ -165,11 +169,11 @@ __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile
     sincos_small(th, st, ct);
     const v3 to_sun = -to_world(angle2xyz(st, ct, sp, cp), t.sf);
     // the reference order: shading, perturbed normal, reflection, blocking, receiver
-    o.shade = t.clear ? -1 : dda(a, orig, c0, to_sun, t.h);
+    o.shade = t.clear ? -1 : dda(a, orig, c0, to_sun, t.self);
     if (o.shade >= 0) return o;
     if (!(SUN == OM_SUN_BUIE && FINE)) nrm = perturbed_normal(x, a.slope_sigma, t.hf);
     const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
-    o.block = dda(a, orig, c0, refl, t.h);
+    o.block = dda(a, orig, c0, refl, t.self);
     if (o.block >= 0) return o;
     o.rc = receiver_rc(a, orig, refl);
     const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1

@@ -1415,6 +1415,7 @@ TraceArgs make_trace_args(const om_scene *s, int g)
 {
     TraceArgs a;
     std::memset(&a, 0, sizeof(a));
+    a.literal_self = s->prm.literal_self ? 1 : 0;
     frame sf = make_frame(s->sun);
     a.sun_u = sf.u;
     a.sun_n = sf.n;

@@ -84,7 +84,7 @@ class StatsC(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in
                 ("rays", "shaded", "blocked", "hit", "dda_steps", "dda_steps_eff",
                  "cand_tests", "self_skips", "tag_rays", "tag_shade_steps_eff",
-                 "tag_shade_tests", "tag_shade_self_skips")]
+                 "tag_shade_tests", "tag_shade_self_skips", "self_hits")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -123,6 +123,9 @@ def lib():
     L.oc_scene_micro.argtypes = [ctypes.c_void_p, _i32p, _i64p]
     L.oc_scene_poisson.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_int]
     L.oc_scene_poisson.restype = ctypes.c_int
+    L.oc_scene_poisson_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_int, _i32p, _f32p]
+    L.oc_scene_poisson_grid.restype = ctypes.c_int
+    L.oc_scene_set_literal_self.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oc_scene_flux_scale.argtypes = [ctypes.c_void_p, _f32p, _f32p]
     L.oc_scene_set_tags.argtypes = [ctypes.c_void_p, _i32p]
     L.oc_trace_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _u32p, _u64p,
@@ -282,6 +285,8 @@ class OracleScene:
             raise ValueError(f"oc_scene_build failed: {rc}")
         self._h = h
         self.desc, self.prm = desc, prm
+        if prm.get("literal_self"):
+            L.oc_scene_set_literal_self(self._h, 1)
         c = np.zeros(8, np.int64)
         L.oc_scene_counts(self._h, _p(c, _i64p))
         (self.n_helio, self.cells, self.nnz, self.res_x, self.res_y, self.total_micro,
@@ -356,6 +361,25 @@ class OracleScene:
         if n < 0:
             raise ValueError("scene has no Poisson microsamples")
         return pts[:n].copy()
+
+    def poisson_grid(self, h: int) -> dict:
+        """Heliostat h's Poisson cell grid before compaction: 'grid' [rows][cols][2] (a point
+        per cell, or -2 cell for none; the padding ring included), 'r', 'cell', and 'n' the
+        accepted point count."""
+        dims = np.zeros(2, np.int32)
+        rc = np.zeros(2, np.float32)
+        ncell = lib().oc_scene_poisson_grid(self._h, int(h), None, 0, _p(dims, _i32p), _p(rc, _f32p))
+        if ncell < 0:
+            raise ValueError("no such heliostat")
+        g = np.zeros((int(dims[1]), int(dims[0]), 2), np.float32)
+        n = lib().oc_scene_poisson_grid(self._h, int(h), _p(g, _f32p), int(ncell), _p(dims, _i32p),
+                                        _p(rc, _f32p))
+        return {"grid": g, "r": float(rc[0]), "cell": float(rc[1]), "n": int(n)}
+
+    def set_literal_self(self, on: bool = True):
+        """Walks test the ray's own heliostat like any candidate (the reference's literal
+        Intersect loop) instead of skipping it."""
+        lib().oc_scene_set_literal_self(self._h, int(bool(on)))
 
     def flux_scale(self):
         s = np.zeros(1, np.float32)

@@ -253,6 +253,7 @@ struct oc_scene {
     /* flux scale (Gtest.cu:29,37-46) */
     float scale, total_energy;
     int32_t *tag;             /* [H] or NULL: heliostats counted in oc_stats' tag_* fields */
+    int literal_self;         /* 1: a walk tests the ray's own heliostat like any candidate */
 };
 
 static f3 ld3(const float *p) { return mk3(p[0], p[1], p[2]); }
@@ -539,7 +540,7 @@ static void build_heliostats(oc_scene *s)
  * compaction of generate_microhelio_origin / valid_element (compact_sample.cu:6-19,
  * compact_sample.cuh:13-22).  Returns the number of accepted points written to pts. */
 static int poisson_one(const oc_scene *s, int h, float width, float height, float r, int k,
-                       float *pts, int cap)
+                       float *pts, int cap, float *grid_out)
 {
     float cell = r / sqrtf(2.0f);
     int width_int = (int)(ceilf(width / cell) + 2);
@@ -594,6 +595,7 @@ static int poisson_one(const oc_scene *s, int h, float width, float height, floa
                 }
             }
     }
+    if (grid_out) memcpy(grid_out, g, sizeof(float) * 2 * ncell);
     int n = 0;
     for (int i = 0; i < ncell; ++i) {
         float x = g[2 * i], y = g[2 * i + 1];
@@ -646,7 +648,7 @@ static int build_micro(oc_scene *s)
             float width = s->subcol_len[i] * (float)s->matrix[2 * i + 1];
             float height = s->subrow_len[i] * (float)s->matrix[2 * i];
             s->mcount[i] = poisson_one(s, i, width, height, r, s->p.poisson_k,
-                                       s->poisson_pts + 2 * (size_t)cap * i, cap);
+                                       s->poisson_pts + 2 * (size_t)cap * i, cap, NULL);
         }
     } else {
         for (int i = 0; i < H; ++i)
@@ -834,6 +836,26 @@ void oc_scene_micro(const oc_scene *s, int32_t *count, int64_t *ray_offset)
     if (ray_offset) memcpy(ray_offset, s->ray_offset, sizeof(int64_t) * (s->H + 1));
 }
 
+int oc_scene_poisson_grid(const oc_scene *s, int h, float *grid, int cap, int dims[2], float rc[2])
+{
+    if (h < 0 || h >= s->H) return -1;
+    float pitch = s->p.helio_pixel_length;
+    float r = pitch / sqrtf(1.5f);
+    float cell = r / sqrtf(2.0f);
+    float width = s->subcol_len[h] * (float)s->matrix[2 * h + 1];
+    float height = s->subrow_len[h] * (float)s->matrix[2 * h];
+    int wi = (int)(ceilf(width / cell) + 2), hi = (int)(ceilf(height / cell) + 2);
+    dims[0] = wi;
+    dims[1] = hi;
+    rc[0] = r;
+    rc[1] = cell;
+    if (!grid || cap < wi * hi) return wi * hi;
+    float *pts = (float *)malloc(sizeof(float) * 2 * (size_t)wi * hi);
+    int n = poisson_one(s, h, width, height, r, s->p.poisson_k, pts, wi * hi, grid);
+    free(pts);
+    return n;
+}
+
 int oc_scene_poisson(const oc_scene *s, int h, float *pts, int cap)
 {
     if (s->p.micro_mode != OC_MICRO_POISSON || h < 0 || h >= s->H) return -1;
@@ -904,7 +926,7 @@ void oc_sample_ray(const oc_scene *s, int h, int m, int l, float sun_to[3], floa
 /* ------------------------------------------------------------------------------------ */
 /* 3D-DDA                                                                                */
 /* ------------------------------------------------------------------------------------ */
-typedef struct { int64_t steps, steps_eff, tests, self_skips; } dda_ctr;
+typedef struct { int64_t steps, steps_eff, tests, self_skips, self_hits; } dda_ctr;
 
 /* calTMax (rectgrid_raytracing.cu:3-9) */
 static inline float cal_tmax(float dir, float interval, int idx, float cur)
@@ -920,18 +942,29 @@ static inline float abs_divide(float den, float num)
 }
 
 /* Intersect (rectgrid_raytracing.cu:21-36): first candidate of the cell's CSR list hit by
- * the ray, skipping the ray's own heliostat (exact-arithmetic reading of the t >= 1e-6
- * guard; see the module documentation). */
+ * the ray.  By default the ray's own heliostat is skipped (exact-arithmetic reading of the
+ * t >= 1e-6 guard of rayParallelogramIntersect, global_function.cuh:102: the origin lies on
+ * the mirror plane, so t = 0); with literal_self it is tested like any candidate, as the
+ * reference's loop does, and a pass is counted in self_hits. */
 static int cell_hit(const oc_scene *s, const oc_grid *g, f3 o, f3 d, int64_t addr, int self, dda_ctr *k)
 {
     for (int32_t i = g->cell_start[addr]; i < g->cell_start[addr + 1]; ++i) {
         int h = g->cell_items[i];
-        if (h == self) { k->self_skips++; continue; }
+        if (h == self && !s->literal_self) { k->self_skips++; continue; }
         k->tests++;
         float t, u, v;
-        if (ray_parallelogram3(o, d, s->v0[h], s->v1[h], s->v3[h], &t, &u, &v)) return h;
+        if (ray_parallelogram3(o, d, s->v0[h], s->v1[h], s->v3[h], &t, &u, &v)) {
+            if (h == self) k->self_hits++;
+            return h;
+        }
     }
     return -1;
+}
+
+int oc_scene_set_literal_self(oc_scene *s, int on)
+{
+    s->literal_self = on ? 1 : 0;
+    return 0;
 }
 
 /* NotCollision (rectgrid_raytracing.cu:38-117).  Returns the blocking heliostat, or -1
@@ -999,7 +1032,7 @@ static int dda(const oc_scene *s, f3 o, f3 d, int self, int early_exit, dda_ctr 
 int oc_dda_probe(const oc_scene *s, const float orig[3], const float dir[3], int self,
                  int32_t *cells, int cap, int *hit_helio)
 {
-    dda_ctr k = {0, 0, 0, 0};
+    dda_ctr k = {0, 0, 0, 0, 0};
     int n = 0;
     int h = dda(s, ld3(orig), ld3(dir), self, 0, &k, cells, cap, &n);
     if (hit_helio) *hit_helio = h;
@@ -1049,7 +1082,7 @@ int oc_trace_f32(const oc_scene *s, int h0, int h1,
         uint64_t *le = (uint64_t *)calloc(npx, sizeof(uint64_t));
         oc_stats st;
         memset(&st, 0, sizeof(st));
-        dda_ctr k = {0, 0, 0, 0}, kt = {0, 0, 0, 0};
+        dda_ctr k = {0, 0, 0, 0, 0}, kt = {0, 0, 0, 0, 0};
         #pragma omp for schedule(dynamic, 1)
         for (int h = h0; h < h1; ++h) {
             f3 n = s->normal[h];
@@ -1096,6 +1129,7 @@ int oc_trace_f32(const oc_scene *s, int h0, int h1,
             tot.hit += st.hit; tot.dda_steps += k.steps + kt.steps;
             tot.dda_steps_eff += k.steps_eff + kt.steps_eff;
             tot.cand_tests += k.tests + kt.tests; tot.self_skips += k.self_skips + kt.self_skips;
+            tot.self_hits += k.self_hits + kt.self_hits;
             tot.tag_rays += st.tag_rays; tot.tag_shade_steps_eff += kt.steps_eff;
             tot.tag_shade_tests += kt.tests; tot.tag_shade_self_skips += kt.self_skips;
         }
@@ -1494,7 +1528,7 @@ int oc_trace_pooled(const oc_scene *s, int h0, int h1, int groups, int lights,
         f3 *light = (f3 *)malloc(sizeof(f3) * pool);
         f3 *turb = (f3 *)malloc(sizeof(f3) * pool);
         int *grp = (int *)malloc(sizeof(int) * (s->max_micro + 1));
-        dda_ctr k = {0, 0, 0, 0};
+        dda_ctr k = {0, 0, 0, 0, 0};
         #pragma omp for schedule(dynamic, 1)
         for (int h = h0; h < h1; ++h) {
             for (int i = 0; i < pool; ++i) {

@@ -128,6 +128,12 @@ void oc_scene_sun(const oc_scene *s, float sun_dir[3], float *buie_consts /*[6]*
 int  oc_scene_buie_table(const oc_scene *s, float *cdf_x, float *cdf_y, float *slope);
 void oc_scene_micro(const oc_scene *s, int32_t *count, int64_t *ray_offset);
 int  oc_scene_poisson(const oc_scene *s, int h, float *pts /*[2*cap]*/, int cap);
+/* Heliostat h's Poisson cell grid before compaction (poisson_sample's array after the last
+ * phase, poisson_disk.cu:93-201): dims = (cells per row, rows) including the padding ring,
+ * rc = (r, cell edge), grid[2 * (y * dims[0] + x)] the point of cell (x, y) or -2 cell for
+ * none.  Returns the accepted point count, or the cell count when grid is NULL or cap is
+ * smaller than it. */
+int  oc_scene_poisson_grid(const oc_scene *s, int h, float *grid, int cap, int dims[2], float rc[2]);
 void oc_scene_flux_scale(const oc_scene *s, float *scale, float *total_energy);
 
 /* ---- tracing ---- */
@@ -144,7 +150,13 @@ typedef struct {
      * dda_steps_eff, cand_tests and self_skips (a work count only: walks and results are
      * the same for every heliostat)                                                      */
     int64_t tag_rays, tag_shade_steps_eff, tag_shade_tests, tag_shade_self_skips;
+    int64_t self_hits;     /* walks stopped by the ray's own heliostat (literal_self only)   */
 } oc_stats;
+
+/* Literal self-intersection semantics of Intersect (rectgrid_raytracing.cu:21-36): on = 1
+ * tests the ray's own heliostat like every other candidate, so that an FP32 t >= 1e-6 at
+ * the ray's own mirror (global_function.cuh:102) stops the walk; on = 0 (default) skips it. */
+int oc_scene_set_literal_self(oc_scene *s, int on);
 
 /* Tags heliostats for oc_stats' tag_* counts (tags[H], nonzero = tagged; NULL clears). */
 int oc_scene_set_tags(oc_scene *s, const int32_t *tags);
