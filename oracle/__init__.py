@@ -126,6 +126,7 @@ def lib():
     L.oc_scene_poisson_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_int, _i32p, _f32p]
     L.oc_scene_poisson_grid.restype = ctypes.c_int
     L.oc_scene_set_literal_self.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oc_scene_set_ray_stream.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
     L.oc_scene_flux_scale.argtypes = [ctypes.c_void_p, _f32p, _f32p]
     L.oc_scene_set_tags.argtypes = [ctypes.c_void_p, _i32p]
     L.oc_trace_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _u32p, _u64p,
@@ -375,6 +376,11 @@ class OracleScene:
         n = lib().oc_scene_poisson_grid(self._h, int(h), _p(g, _f32p), int(ncell), _p(dims, _i32p),
                                         _p(rc, _f32p))
         return {"grid": g, "r": float(rc[0]), "cell": float(rc[1]), "n": int(n)}
+
+    def set_ray_stream(self, scene_index: int):
+        """Rays from Philox stream scene_index; the microsamples keep the build's stream (the
+        draws of a scene fork)."""
+        lib().oc_scene_set_ray_stream(self._h, int(scene_index))
 
     def set_literal_self(self, on: bool = True):
         """Walks test the ray's own heliostat like any candidate (the reference's literal

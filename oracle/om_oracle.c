@@ -961,6 +961,12 @@ static int cell_hit(const oc_scene *s, const oc_grid *g, f3 o, f3 d, int64_t add
     return -1;
 }
 
+int oc_scene_set_ray_stream(oc_scene *s, uint32_t scene_index)
+{
+    s->p.scene_index = scene_index;
+    return 0;
+}
+
 int oc_scene_set_literal_self(oc_scene *s, int on)
 {
     s->literal_self = on ? 1 : 0;

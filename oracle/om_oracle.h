@@ -158,6 +158,11 @@ typedef struct {
  * the ray's own mirror (global_function.cuh:102) stops the walk; on = 0 (default) skips it. */
 int oc_scene_set_literal_self(oc_scene *s, int on);
 
+/* Philox stream (scene_index) of the rays from now on; the microsamples keep the stream the
+ * scene was built with.  This is how a scene fork draws (om_scene_fork: the root's
+ * microsamples, the fork's rays). */
+int oc_scene_set_ray_stream(oc_scene *s, uint32_t scene_index);
+
 /* Tags heliostats for oc_stats' tag_* counts (tags[H], nonzero = tagged; NULL clears). */
 int oc_scene_set_tags(oc_scene *s, const int32_t *tags);
 

@@ -88,3 +88,54 @@ def test_annual_run_equals_plain_traces(om, mode):
         assert res["steps"][i]["ratio"] == pytest.approx(want, rel=0, abs=0)
         assert 0.0 < res["steps"][i]["ratio"] <= 1.0
     assert 0.0 < res["annual_ratio"] <= 1.0
+
+
+@pytest.mark.parametrize("cfg", ["small", "mid_slice"])
+def test_fork_with_own_stream_equals_oracle(om, cfg):
+    """A fork under another sun, on a Philox stream of its own (scene_index != 0), against
+    the oracle built for that sun with the root's microsamples and the fork's ray stream
+    (main.cpp:56-116 builds each time step's scene on the same field): per-ray records and
+    integer maps bit-exact."""
+    import oracle
+    import torch
+    if cfg == "small":
+        d, p = F.config_small()
+    else:
+        d, p = F.config_mid()
+        d = F.subset(d, slice(1500, 1650))
+    root = om.SolarScene(d, p)
+    for k, (sun, dni) in enumerate(SUNS):
+        idx = 5 + k
+        f = root.fork(sun, dni, idx)
+        o = oracle.OracleScene(dict(d, sun_dir=np.float32(sun), dni=np.float32(dni)), p)
+        o.set_ray_stream(idx)
+        want = o.trace_f32(records=True)
+        rec = f.trace_records()
+        for key in ("shade", "block", "pixel"):
+            assert np.array_equal(rec[key], want[key]), (cfg, k, key)
+        m = f.trace_plain()
+        torch.cuda.synchronize()
+        assert np.array_equal(m.counts.cpu().numpy().view(np.uint32).ravel(), want["counts"].ravel()), (cfg, k)
+        assert np.array_equal(m.energy.cpu().numpy().view(np.uint64).ravel(), want["energy"].ravel()), (cfg, k)
+        assert want["counts"].sum() > 0
+
+
+def test_annual_batch_steps_equal_oracle(om):
+    """Every time step of a batched annual run (forks i = 0.. with stream i) against the
+    oracle of that step's sun and DNI: integer maps bit-exact, and the flux scale (DNI) of
+    the step."""
+    import oracle
+    from onemirror_amd import annual as A
+    d, p = F.config_small()
+    root = om.SolarScene(d, p)
+    steps = A.annual_steps([(3, 1, 8), (6, 16, 12), (9, 10, 16), (12, 4, 10)])
+    run = A.AnnualRun(root, steps, batch_size=4, mode="batch")
+    run.run(keep_maps=True)
+    for i, st in enumerate(steps):
+        o = oracle.OracleScene(dict(d, sun_dir=np.float32(st["sun_dir"]), dni=np.float32(st["dni"])), p)
+        o.set_ray_stream(i)
+        want = o.trace_f32()
+        c, e, _ = _maps(run.maps()[i])
+        assert np.array_equal(c.ravel(), want["counts"].ravel()), i
+        assert np.array_equal(e.ravel(), want["energy"].ravel()), i
+        assert run.forks[i].info["flux_scale"] == o.flux_scale()[0], i

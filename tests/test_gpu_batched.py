@@ -3,10 +3,13 @@ each scene in turn, OM_BATCH_PER_SCENE), replayed from a hipGraph.
 
 Pin: every map of a batch equals the map the plain path gives for that scene alone (same
 Philox streams, integer accumulation), bit for bit, for every launch configuration and on
-every graph replay (replays are interleaved with other GPU work and synchronisations).
+every graph replay (replays are interleaved with other GPU work and synchronisations); and
+those plain maps equal the oracle's integer maps of each scene.
 """
 import numpy as np
 import pytest
+
+import oracle
 
 from onemirror_amd import fields as F
 
@@ -26,6 +29,14 @@ def _plain(scene):
     return {k: getattr(m, k).cpu().numpy().copy() for k in ("counts", "energy", "flux", "flux_smooth")}
 
 
+def _oracle_equal(scenes, want, descs):
+    """The plain maps of each scene equal the oracle's integer maps."""
+    for i, (d, p) in enumerate(descs):
+        o = oracle.OracleScene(d, p).trace_f32()
+        assert np.array_equal(want[i]["counts"].view(np.uint32).ravel(), o["counts"].ravel()), i
+        assert np.array_equal(want[i]["energy"].view(np.uint64).ravel(), o["energy"].ravel()), i
+
+
 def _same(maps, want, label):
     for k, v in want.items():
         got = getattr(maps, k).cpu().numpy()
@@ -36,17 +47,20 @@ def _same(maps, want, label):
 @pytest.fixture(scope="module")
 def pillbox_batch(om):
     """Annual steps of one field (three sun positions), the cylinder field, a second seed."""
-    scenes = []
+    descs = []
     d, p = F.config_small()
     for i, (el, az) in enumerate(((45.0, 180.0), (25.0, 120.0), (70.0, 230.0))):
         dd = dict(d)
         dd["sun_dir"] = F.sun_direction(el, az)
-        scenes.append(om.SolarScene(dd, dict(p, scene_index=i)))
+        descs.append((dd, dict(p, scene_index=i)))
     d, p = F.config_small_cylinder()
-    scenes.append(om.SolarScene(d, dict(p, scene_index=3)))
+    descs.append((d, dict(p, scene_index=3)))
     d, p = F.config_small(seed=9)
-    scenes.append(om.SolarScene(d, dict(p, scene_index=4)))
-    return scenes, [_plain(s) for s in scenes]
+    descs.append((d, dict(p, scene_index=4)))
+    scenes = [om.SolarScene(d, p) for d, p in descs]
+    want = [_plain(s) for s in scenes]
+    _oracle_equal(scenes, want, descs)
+    return scenes, want
 
 
 @pytest.mark.parametrize("wg,rpt", [(0, 0), (128, 2), (512, 8), (256, 1), (1024, 0)])
@@ -78,13 +92,15 @@ def test_trace_batched_one_shot(om, pillbox_batch):
 def test_batch_buie_fields_equal_plain(om, wg):
     import torch
     d, p = F.config_mid()
-    scenes = [om.SolarScene(F.subset(d, slice(0, 300)), dict(p, scene_index=0))]
+    descs = [(F.subset(d, slice(0, 300)), dict(p, scene_index=0))]
     dd = F.subset(d, slice(2000, 2400))
     dd["sun_dir"] = F.sun_direction(20.0, 100.0)
-    scenes.append(om.SolarScene(dd, dict(p, scene_index=1)))
-    bd, bp = F.config_batched(n_scenes=2, h_range=(500, 800))[0]
-    scenes.append(om.SolarScene(bd, bp))
+    descs.append((dd, dict(p, scene_index=1)))
+    descs.append(F.config_batched(n_scenes=2, h_range=(500, 800))[0])
+    scenes = [om.SolarScene(d, p) for d, p in descs]
     want = [_plain(s) for s in scenes]
+    if wg == 0:
+        _oracle_equal(scenes, want, descs)
     b = om.Batch(scenes, workgroup=wg)
     for rep in range(2):
         maps = b.run()
