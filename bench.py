@@ -49,12 +49,83 @@ def model_ops(st: dict) -> tuple:
     return F0 - C_SETUP * st.get("tag_rays", 0) / n + C_STEP * S + C_TEST * T + C_SKIP * K, S, T
 
 
-def host_threads() -> int:
+def host_cores() -> dict:
+    """Host cores this process may run on, the machine's count, the CPU model, and the
+    threads the CPU baseline uses: every core it may run on, or fewer where OMP_NUM_THREADS
+    sets this process's share of a machine shared by several GPUs (16 per GPU on the GPU
+    boxes here)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        avail = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        avail = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = avail
+    if share and share.isdigit() and int(share) > 0:
+        threads = min(avail, int(share))
+    return {"threads": max(1, threads), "available": avail, "machine": os.cpu_count(), "model": model,
+            "omp_num_threads": share}
+
+
+def host_threads() -> int:
+    return host_cores()["threads"]
+
+
+def flux_error_vs_fp64(scene, maps, osc, nthreads: int, block: int = 2000) -> dict:
+    """FP32 flux of the GPU trace (maps, the whole field) against the FP64 model of the same
+    rays (the oracle traces the same Philox draws with double geometry and double sums):
+    per-pixel relative error over the pixels the FP64 map lights (median, p99, max, share
+    above 1e-5) and over the pixels above 1 % of the peak, the total-power error, the error
+    of the accumulation alone (the GPU's FP32 map against its own exact integer sums), and
+    the share of rays whose FP32 and FP64 receiver pixel differ on one block of heliostats
+    (GPU records against FP64 records)."""
+    import numpy as np
+    import torch
+
+    torch.cuda.synchronize()
+    scale = float(np.float32(scene.info["flux_scale"]))
+    flux32 = maps.flux.cpu().numpy().astype(np.float64).ravel()
+    energy = maps.energy.cpu().numpy().view(np.uint64).ravel()
+    t = time.perf_counter()
+    f64 = osc.trace_f64(nthreads=nthreads)
+    secs = time.perf_counter() - t
+    flux64 = f64["energy"].ravel() * scale
+    lit = flux64 > 0
+    rel = np.abs(flux32[lit] - flux64[lit]) / flux64[lit]
+    big = flux64 > 0.01 * flux64.max()
+    relb = np.abs(flux32[big] - flux64[big]) / flux64[big]
+    exact = energy.astype(np.float64) * 2.0 ** -31 * scale
+    nz = exact > 0
+    acc = float(np.max(np.abs(flux32[nz] - exact[nz]) / exact[nz])) if nz.any() else 0.0
+    H = scene.info["n_helio"]
+    h0 = min(block, max(0, H - 40))
+    h1 = min(H, h0 + 40)
+    rg = scene.trace_records(h0, h1)
+    r64 = osc.trace_f64(h0, h1, records=True, nthreads=nthreads)
+    nr = rg["pixel"].size
+    return {
+        "pixels_lit": int(lit.sum()),
+        "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)), "max": float(rel.max()),
+        "share_above_1e-5": float((rel > 1e-5).mean()),
+        "bright_pixels": int(big.sum()), "bright_median": float(np.median(relb)),
+        "bright_p99": float(np.quantile(relb, 0.99)), "bright_share_above_1e-5": float((relb > 1e-5).mean()),
+        "total_power": float(abs(flux32.sum() - flux64.sum()) / flux64.sum()),
+        "accumulation_only_max": acc,
+        "pixel_flip_rate": float((rg["pixel"] != r64["pixel"]).sum() / nr),
+        "shade_flip_rate": float((rg["shade"] != r64["shade"]).sum() / nr),
+        "block_flip_rate": float((rg["block"] != r64["block"]).sum() / nr),
+        "flip_sample": f"heliostats [{h0}, {h1}), {nr} rays",
+        "per_pixel_1e-5_met": bool((rel > 1e-5).sum() == 0),
+        "fp64_seconds": secs,
+    }
 
 
 def main() -> int:
@@ -169,7 +240,8 @@ def main() -> int:
     import oracle
     osc = oracle.OracleScene(desc, prm)
     every = args.cpu_every if args.cpu_every > 0 else (5 if args.workload == "large" else 1)
-    nthreads = host_threads()
+    cores = host_cores()
+    nthreads = cores["threads"]
     blk = 40  # heliostats per block: blocks of neighbours keep the threads busy
     blocks = [(h, min(H, h + blk)) for h in range(0, H, blk * every)]
     if args.no_cpu:
@@ -187,6 +259,14 @@ def main() -> int:
     cpu_rays = st_tot["rays"]
     nsample = sum(h1 - h0 for h0, h1 in blocks)
     ops_ray, S, T = model_ops(st_tot)
+    flux_err = None
+    if args.workload in ("mid", "small", "mid_grids") and not args.no_cpu:
+        flux_err = flux_error_vs_fp64(scene, maps, osc, nthreads)
+        print(f"# FP32 flux vs FP64 (whole field): per-pixel median {flux_err['median']:.2g}, p99 "
+              f"{flux_err['p99']:.2g}, {100 * flux_err['share_above_1e-5']:.1f} % of lit pixels above 1e-5, "
+              f"total power {flux_err['total_power']:.2g}, accumulation alone {flux_err['accumulation_only_max']:.2g}, "
+              f"pixel flips {flux_err['pixel_flip_rate']:.2g} of rays ({flux_err['fp64_seconds']:.1f} s)",
+              file=sys.stderr, flush=True)
     achieved = R * ops_ray / (trace_ms * 1e-3) / 1e12
     peak = P_VALU / 1e12
     # HBM bytes per trace launch from the committed rocprof FETCH_SIZE / WRITE_SIZE runs
@@ -218,8 +298,10 @@ def main() -> int:
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "ops_per_ray": ops_ray, "dda_steps_per_ray": S, "tests_per_ray": T},
+        "flux_err_vs_fp64": flux_err,
         "cpu_baseline": None if args.no_cpu else {
             "value": cpu_rays / cpu_t, "unit": "rays/s", "cores": nthreads, "kind": "port",
+            "host_cores": cores,
             "sample": (f"the whole field ({nsample} heliostats, {cpu_rays} rays)" if every == 1 else
                        f"{len(blocks)} blocks of {blk} neighbouring heliostats, one in every "
                        f"{every} of the same field ({nsample} heliostats, {cpu_rays} rays)")
@@ -453,8 +535,10 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / args.steps
     batch_ms = e0.elapsed_time(e1) / args.steps
-    # the plain path on the same scenes: one trace + finalize + smoothing per scene
+    # the plain path on the same scenes: one trace + finalize + smoothing per scene, each scene
+    # with the accumulation its own autotuner picks (packed or LDS window)
     for s, m in zip(scenes, maps):
+        s.autotune_accumulate(m, reps=2)
         s.trace_plain(m)
     torch.cuda.synchronize()
     e0.record(stream)
@@ -549,6 +633,7 @@ def bench_diff(args) -> dict:
             D.forward(scene, p)
     e[1].record(stream)
     maps = scene.new_maps(smooth=False)
+    scene.autotune_accumulate(maps, reps=2)  # the plain path as the plain bench runs it
     scene.trace_heliostats(0, H, maps)
     e[2].record(stream)
     for _ in range(args.steps):

@@ -1,11 +1,21 @@
-"""FP32 path against the FP64 model of the same rays (oracle, CPU).
+"""FP32 path against the FP64 model of the same rays (oracle, CPU; the GPU's maps equal the
+oracle's FP32 maps bit for bit, which the GPU parity tests check).
 
 The FP64 model traces the same Philox draws with double geometry and sums energies in
 double.  Two parts of the FP32 flux error are checked separately:
   * accumulation: integer fixed-point sums converted once to float, so for the same hits
-    the FP32 map is within 2^-24 relative of the exact sum of the per-ray energies;
-  * geometry: FP32 and FP64 rays that land on neighbouring pixels (a few in 10^4) move
-    single-ray amounts between pixels; total power still agrees to ~1e-6.
+    the FP32 map is within 2^-24 relative of the exact sum of the per-ray energies: the
+    1e-5 bar holds for the flux computation itself;
+  * geometry: FP32 and FP64 rays that land on neighbouring pixels (coordinates of 50 to
+    1,500 m round to 4e-6..1.2e-4 m in FP32 against 0.06..0.1 m pixels) move single-ray
+    amounts between pixels.  The per-pixel 1e-5 bar is NOT met under the reference's FP32
+    ray geometry (rectgrid_raytracing.cu:135-153): several per cent of the lit pixels of
+    every sample exceed it, and on the whole Mid field (about 1,700 hits per pixel) most do.
+    Total power agrees to a few 1e-6.
+
+The bounds below are the figures stated for each sample, with a 10 % margin: the samples
+are deterministic (Philox streams, integer maps), so a change of the geometry that raises
+the error fails here.
 """
 import numpy as np
 import pytest
@@ -14,32 +24,63 @@ import oracle
 from onemirror_amd import fields as F
 
 
-@pytest.mark.parametrize("lattice", [False, True])
-def test_small_fp32_against_fp64(lattice):
-    d, p = F.config_small()
+def sample_stats(o, hs):
+    """FP32-vs-FP64 statistics over the heliostats hs of oracle scene o."""
+    scale, _ = o.flux_scale()
+    npx = o.res_x * o.res_y
+    e32 = np.zeros(npx, np.uint64)
+    e64 = np.zeros(npx)
+    flips = shade = block = nr = 0
+    for h in hs:
+        a = o.trace_f32(h, h + 1, records=True)
+        b = o.trace_f64(h, h + 1, records=True)
+        e32 += a["energy"].ravel()
+        e64 += b["energy"].ravel()
+        flips += int((a["pixel"] != b["pixel"]).sum())
+        shade += int((a["shade"] != b["shade"]).sum())
+        block += int((a["block"] != b["block"]).sum())
+        nr += a["pixel"].size
+    f32 = (oracle.energy_to_float(e32) * np.float32(scale)).astype(np.float64)
+    f64 = e64 * float(scale)
+    m = f64 > 0
+    rel = np.abs(f32[m] - f64[m]) / f64[m]
+    exact = e32.astype(np.float64) * 2.0 ** -31
+    raw = oracle.energy_to_float(e32).astype(np.float64)
+    nz = exact > 0
+    return {"rays": nr, "flip": flips / nr, "shade": shade / nr, "block": block / nr,
+            "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+            "share": float((rel > 1e-5).mean()), "total": float(abs(f32.sum() - f64.sum()) / f64.sum()),
+            "acc": float(np.max(np.abs(raw[nz] - exact[nz]) / exact[nz]))}
+
+
+# sample -> (field, every n-th heliostat, lattice, stated values: pixel flips, median, p99,
+#            share of lit pixels above 1e-5, total power, blocker flips)
+STATED = {
+    "S poisson": (F.config_small, 1, False, dict(flip=4.38e-4, median=1.17e-7, p99=4.87e-2, share=8.34e-2,
+                                                 total=7.23e-7, block=0.0)),
+    "S lattice": (F.config_small, 1, True, dict(flip=4.41e-4, median=1.17e-7, p99=6.41e-2, share=8.66e-2,
+                                                total=1.90e-6, block=0.0)),
+    "Mid every 50th": (F.config_mid, 50, False, dict(flip=6.32e-4, median=4.43e-8, p99=6.45e-2, share=7.77e-2,
+                                                     total=4.98e-7, block=7.45e-7)),
+    "Lg every 250th": (F.config_large, 250, False, dict(flip=1.67e-3, median=1.88e-7, p99=1.15e-1,
+                                                        share=9.57e-2, total=3.06e-6, block=2.01e-6)),
+}
+
+
+@pytest.mark.parametrize("name", list(STATED))
+def test_fp32_against_fp64_as_stated(name):
+    fn, every, lattice, want = STATED[name]
+    d, p = fn()
     if lattice:
         p = F.with_lattice(p)
     o = oracle.OracleScene(d, p)
-    scale, _ = o.flux_scale()
-    a = o.trace_f32(records=True)
-    b = o.trace_f64(records=True)
-    e_exact = a["energy"].astype(np.float64) * 2.0 ** -31     # exact sum of FP32 ray energies
-    f32 = (oracle.energy_to_float(a["energy"]) * np.float32(scale)).astype(np.float64)
-    f64 = b["energy"] * float(scale)
-    # accumulation: the float map against the exact sum of the same hits
-    raw = oracle.energy_to_float(a["energy"]).astype(np.float64)
-    m = e_exact > 0
-    assert np.max(np.abs(raw[m] - e_exact[m]) / e_exact[m]) <= 2.0 ** -24
-    # geometry: rays whose FP32 and FP64 pixel differ, blockers that differ
-    nr = a["pixel"].size
-    flips = int((a["pixel"] != b["pixel"]).sum())
-    assert 0 < flips < 1e-3 * nr
-    assert int((a["shade"] != b["shade"]).sum()) <= 1e-5 * nr
-    assert int((a["block"] != b["block"]).sum()) <= 1e-5 * nr
-    # total intercepted power
-    assert abs(f32.sum() - f64.sum()) / f64.sum() < 1e-5
-    # most pixels agree to 1e-5; the rest differ by whole rays
-    mm = f64 > 0
-    rel = np.abs(f32[mm] - f64[mm]) / f64[mm]
-    assert np.median(rel) < 1e-6
-    assert (rel > 1e-5).mean() < 0.15
+    st = sample_stats(o, range(0, o.n_helio, every))
+    # accumulation: exact sums, one rounding
+    assert st["acc"] <= 2.0 ** -24
+    # geometry, within 10 % of the stated figures
+    for k in ("flip", "median", "p99", "share", "total", "block"):
+        assert st[k] <= 1.1 * want[k] + 1e-12, (name, k, st[k], want[k])
+    assert st["flip"] >= 0.9 * want["flip"]
+    assert st["shade"] == 0.0
+    # the per-pixel bar is not met under FP32 geometry
+    assert st["share"] > 0.0
