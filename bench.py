@@ -8,8 +8,11 @@ scaling and the trimmed-mean smoothing, all on the GPU with the scene resident i
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mid|small]
 
 Prints one JSON line: rays/s, the roofline of the trace kernel (VALU lane-ops per ray from
-the cost model below times rays/s, against 256 CU x 64 lanes x 2.4 GHz), and the CPU
-oracle's rays/s on a stated sample of the same field on the host cores.
+the cost model below times rays/s, against 256 CU x 128 lanes x 2.4 GHz: a CU issues two
+64-lane FP32 VALU instructions per clock, which the VALU issue probe (--valu-probe) sustains
+and AMD's 157.3 TFLOP/s FP32 vector peak states), the counter-based VALU utilisation of the
+same kernel from its committed rocprof run, and the CPU oracle's rays/s on a stated sample
+of the same field on the host cores.
 
 VALU cost model per ray (lane operations; add/mul/fma/cmp/cvt/int = 1, div/sqrt/rcp = 1,
 sincos = 30, log = 25, Philox4x32-10 = 100):
@@ -34,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "rays/s (w/ DDA shading+blocking) at ≤1e-5 flux err vs FP64; batched & fwd+bwd/s"
-P_VALU = 256 * 64 * 2.4e9          # lane-op/s
+P_VALU = 256 * 128 * 2.4e9         # lane-op/s: two 64-lane VALU instructions per CU clock
 F0, C_STEP, C_TEST, C_SKIP = 569.0, 12.0, 58.0, 3.0
 C_SETUP = 57.0  # one walk set-up, the half of F0's two that a skipped shading walk saves
 
@@ -47,6 +50,46 @@ def model_ops(st: dict) -> tuple:
     T = (st["cand_tests"] - st.get("tag_shade_tests", 0)) / n
     K = (st["self_skips"] - st.get("tag_shade_self_skips", 0)) / n
     return F0 - C_SETUP * st.get("tag_rays", 0) / n + C_STEP * S + C_TEST * T + C_SKIP * K, S, T
+
+
+# VALU issue probes of om_math_eval (fn 12..14): VALU instructions per lane of each kernel,
+# counted in the assembly of this build (the loop's instructions times its trips, plus the
+# set-up and the final sum)
+VALU_PROBES = {12: ("fma_f32", 2048 + 27), 13: ("fma_f32+add_u32", 4096 + 42), 14: ("fma_f32+exp_f32", 2560 + 27)}
+
+
+def valu_probe(mi: int, reps: int = 5) -> dict:
+    """Runs each VALU issue probe over mi Mi threads and reports its time (HIP events on the
+    launch stream) and the VALU wave-instructions it issues per CU cycle at 2.4 GHz."""
+    import torch
+
+    import onemirror_amd as om
+
+    n = mi << 20
+    x = torch.rand(n, device="cuda")
+    y = torch.empty_like(x)
+    ctx = om.default_context()
+    stream = torch.cuda.current_stream()
+    out = {}
+    for fn, (name, per_lane) in VALU_PROBES.items():
+        def run():
+            om._check(om.lib().om_math_eval(ctx._h, fn, om._vp(x.data_ptr()), om._vp(y.data_ptr()), n,
+                                            om._stream_ptr()), "om_math_eval")
+        run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        instr = n / 64 * per_lane
+        out[name] = {"ms": ms, "valu_wave_instructions": instr, "valu_per_lane": per_lane,
+                     "per_cu_cycle_at_2.4GHz": instr / (ms * 1e-3) / (256 * 2.4e9)}
+        print(f"# VALU probe {name}: {ms:.3f} ms, {instr:.4g} wave-instructions, "
+              f"{out[name]['per_cu_cycle_at_2.4GHz']:.3f} per CU cycle at 2.4 GHz", file=sys.stderr, flush=True)
+    return out
 
 
 def host_cores() -> dict:
@@ -144,6 +187,10 @@ def main() -> int:
                     help="CPU baseline traces one block of 40 heliostats in every n "
                          "(default: the whole field, one block in 5 for large)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--valu-probe", type=int, default=0, metavar="MI",
+                    help="first run the VALU issue probes (om_math_eval 12..14) over MI Mi threads: "
+                         "kernels with a known VALU instruction count that calibrate the VALU peak "
+                         "and counters")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run an elementwise copy of MIB MiB (reads and writes MIB MiB "
                          "each): the reference kernel that calibrates HBM byte counters")
@@ -197,6 +244,7 @@ def main() -> int:
     kernel = "trace_window_kernel" if acc["accumulate"] == "window" else "trace_kernel"
     print(f"# accumulation autotune: packed {acc['packed_ms']:.3f} ms, window {acc['window_ms']:.3f} ms "
           f"-> {acc['accumulate']}", file=sys.stderr, flush=True)
+    probe = valu_probe(args.valu_probe) if args.valu_probe > 0 else None
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 4
         xs = torch.rand(n, device="cuda")
@@ -269,6 +317,21 @@ def main() -> int:
               file=sys.stderr, flush=True)
     achieved = R * ops_ray / (trace_ms * 1e-3) / 1e12
     peak = P_VALU / 1e12
+    # counter-based VALU utilisation of the trace kernel from its committed rocprof run:
+    # SQ_INSTS_VALU x 64 lanes / kernel time against the same peak, and VALU instructions
+    # per CU busy cycle against what the VALU issue probe sustains
+    valu_util = None
+    vp = os.path.join(ROOT, "profiles", "valu_probe_kernel.json")
+    if os.path.exists(vp):
+        with open(vp) as f:
+            vj = json.load(f)
+        tw = vj.get(f"{kernel}_{args.workload}")
+        if tw:
+            valu_util = {"valu_util": tw["SQ_INSTS_VALU"] * 64 / (tw["avg_ns"] * 1e-9) / P_VALU,
+                         "valu_per_cu_busy_cycle": tw["valu_per_cu_busy_cycle"],
+                         "probe_valu_per_cu_busy_cycle": vj["probes"]["probe_fma"]["valu_per_cu_busy_cycle"],
+                         "of_probe": tw["valu_per_cu_busy_cycle"] / vj["probes"]["probe_fma"]["valu_per_cu_busy_cycle"],
+                         "source": "profiles/valu_probe_kernel.json"}
     # HBM bytes per trace launch from the committed rocprof FETCH_SIZE / WRITE_SIZE runs
     traffic = None
     prof = os.path.join(ROOT, "profiles", f"{kernel}_{args.workload}.json")
@@ -297,8 +360,10 @@ def main() -> int:
                    "accumulate": acc},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "ops_per_ray": ops_ray, "dda_steps_per_ray": S, "tests_per_ray": T},
+                     "ops_per_ray": ops_ray, "dda_steps_per_ray": S, "tests_per_ray": T,
+                     "counters": valu_util},
         "flux_err_vs_fp64": flux_err,
+        "valu_probe": probe,
         "cpu_baseline": None if args.no_cpu else {
             "value": cpu_rays / cpu_t, "unit": "rays/s", "cores": nthreads, "kind": "port",
             "host_cores": cores,

@@ -853,6 +853,57 @@ __global__ void math_kernel(int fn, const float *in, float *out, int n)
     }
 }
 
+// VALU issue probes (om_math_eval fn 12..14): a known instruction mix per lane, so that the
+// kernel's time and rocprof's counters give the VALU issue rate per CU cycle.  Each thread
+// runs PROBE_ITERS iterations over 8 independent chains (no dependence stalls at 4 waves
+// per SIMD); per iteration and lane:
+//   mix 0: 8 v_fma_f32
+//   mix 1: 8 v_fma_f32 + 8 v_add_u32 (integer adds on separate chains)
+//   mix 2: 8 v_fma_f32 + 2 v_exp_f32 (transcendental)
+// The loop counter lives in scalar registers.
+constexpr int PROBE_ITERS = 256;
+
+template <int MIX>
+__global__ __launch_bounds__(WG) void valu_probe_kernel(const float *in, float *out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x0 = in[i];
+    float x[8];
+    uint32_t k[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        x[j] = x0 + (float)j;
+        k[j] = f2u(x0) + (uint32_t)j;
+    }
+    const float a = 0.99951171875f, b = 0.000244140625f;
+    for (int it = 0; it < PROBE_ITERS; ++it) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = __builtin_fmaf(x[j], a, b);
+        if (MIX == 1) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) k[j] += 0x9E3779B9u + (uint32_t)it;
+        }
+        if (MIX == 2) {
+            x[0] = __builtin_amdgcn_exp2f(x[0]);
+            x[4] = __builtin_amdgcn_exp2f(x[4]);
+        }
+        asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]),
+                     "+v"(x[6]), "+v"(x[7]));
+        if (MIX == 1)
+            asm volatile("" : "+v"(k[0]), "+v"(k[1]), "+v"(k[2]), "+v"(k[3]), "+v"(k[4]), "+v"(k[5]),
+                         "+v"(k[6]), "+v"(k[7]));
+    }
+    float s = 0.0f;
+    uint32_t t = 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        s += x[j];
+        t ^= k[j];
+    }
+    out[i] = s + (MIX == 1 ? (float)(t & 1u) : 0.0f);
+}
+
 // ---------------------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------------------
@@ -1023,11 +1074,15 @@ om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st)
 
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st)
 {
-    if (fn < 0 || fn > 11) {
-        set_error("math function index must be in 0..11");
+    if (fn < 0 || fn > 14) {
+        set_error("math function index must be in 0..14");
         return OM_ERR_INVALID;
     }
-    hipLaunchKernelGGL(math_kernel, dim3((n + WG - 1) / WG), dim3(WG), 0, st, fn, in, out, n);
+    const dim3 grid((n + WG - 1) / WG), block(WG);
+    if (fn == 12) hipLaunchKernelGGL(valu_probe_kernel<0>, grid, block, 0, st, in, out, n);
+    else if (fn == 13) hipLaunchKernelGGL(valu_probe_kernel<1>, grid, block, 0, st, in, out, n);
+    else if (fn == 14) hipLaunchKernelGGL(valu_probe_kernel<2>, grid, block, 0, st, in, out, n);
+    else hipLaunchKernelGGL(math_kernel, grid, block, 0, st, fn, in, out, n);
     OM_HIP(hipGetLastError());
     return OM_OK;
 }
