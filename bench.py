@@ -187,6 +187,9 @@ def main() -> int:
                     help="CPU baseline traces one block of 40 heliostats in every n "
                          "(default: the whole field, one block in 5 for large)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"],
+                    help="ray geometry: fp32 (the reference's, bit-exact against its host restatement) "
+                         "or fp64 (the FP64 model: per-pixel flux within 1e-5 of FP64)")
     ap.add_argument("--valu-probe", type=int, default=0, metavar="MI",
                     help="first run the VALU issue probes (om_math_eval 12..14) over MI Mi threads: "
                          "kernels with a known VALU instruction count that calibrate the VALU peak "
@@ -231,6 +234,8 @@ def main() -> int:
         desc, prm = F.config_small()
         wl = ("small: 100 heliostats 2x2 m, pillbox 4.65 mrad, slope 2 mrad, 16x16 Poisson "
               "x 64 rays, 64^3 grid, 128x128 map")
+    if args.precision == "fp64":
+        prm = dict(prm, precision=om.PREC_FP64)
     t0 = time.time()
     scene = om.SolarScene(desc, prm)
     info = scene.info
@@ -242,6 +247,8 @@ def main() -> int:
     # accumulation autotuner: per-tile packed atomics or the LDS window (identical maps)
     acc = scene.autotune_accumulate(maps, reps=3)
     kernel = "trace_window_kernel" if acc["accumulate"] == "window" else "trace_kernel"
+    if args.precision == "fp64":
+        kernel = "trace64_kernel"
     print(f"# accumulation autotune: packed {acc['packed_ms']:.3f} ms, window {acc['window_ms']:.3f} ms "
           f"-> {acc['accumulate']}", file=sys.stderr, flush=True)
     probe = valu_probe(args.valu_probe) if args.valu_probe > 0 else None
@@ -352,9 +359,10 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": wl, "heliostats": H, "rays": R, "grid": list(info["grid_num"]),
+        "config": {"workload": wl, "precision": args.precision, "heliostats": H, "rays": R,
+                   "grid": list(info["grid_num"]),
                    "map": [info["res_y"], info["res_x"]], "parallelism": "single GPU",
                    "trace_kernel_ms": trace_ms, "receiver_hits": hits, "kernel": kernel,
                    "accumulate": acc},

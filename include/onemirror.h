@@ -123,7 +123,17 @@ typedef struct {
                                   loop does (rectgrid_raytracing.cu:21-36), so that an FP32
                                   rounding of t to >= 1e-6 stops the ray at its own mirror.
                                   Not supported by the differentiable mode.              */
+    int   precision;           /* OM_PREC_FP32 (default): the reference's FP32 ray geometry,
+                                  bit-exact against the host restatement; OM_PREC_FP64: the
+                                  same Philox draws with sampling, frames, walks,
+                                  intersections and receiver in double (the FP64 model the
+                                  FP32 error is measured against), energies summed exactly
+                                  in the same fixed point.  Plain traces, per-heliostat
+                                  traces, records and per-scene batches; not the persistent
+                                  batch launch or the differentiable mode.               */
 } om_params;
+
+enum { OM_PREC_FP32 = 0, OM_PREC_FP64 = 1 };
 
 enum { OM_GRIDS_ALL = -1 };
 

@@ -90,10 +90,12 @@ class Params(ctypes.Structure):
         ("pack_shift", ctypes.c_int),
         ("grid_select", ctypes.c_int),
         ("literal_self", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
 
 
 ACC_PACKED, ACC_DEVICE, ACC_WINDOW = 0, 1, 2
+PREC_FP32, PREC_FP64 = 0, 1
 GRIDS_ALL = -1
 
 

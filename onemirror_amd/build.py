@@ -18,7 +18,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libonemirror_amd.so")
-SOURCES = ["om_scene.cpp", "om_kernels.hip", "om_batch.cpp", "om_diff.hip", "om_annual.cpp", "om_facets.cpp", "om_io.cpp"]
+SOURCES = ["om_scene.cpp", "om_kernels.hip", "om_trace64.hip", "om_batch.cpp", "om_diff.hip", "om_annual.cpp",
+           "om_facets.cpp", "om_io.cpp"]
 ARCH = os.environ.get("OM_OFFLOAD_ARCH", "gfx950")
 
 

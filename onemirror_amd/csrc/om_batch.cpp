@@ -159,6 +159,12 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
             delete b;
             return OM_ERR_SIZE;
         }
+        if (s->prm.precision != OM_PREC_FP32 && b->wgs != OM_BATCH_PER_SCENE) {
+            om::set_error("the persistent batch launch traces in FP32 geometry: FP64 scenes need "
+                          "OM_BATCH_PER_SCENE");
+            delete b;
+            return OM_ERR_UNSUPPORTED;
+        }
         if (i > 0 && (s->prm.sunshape != scenes[0]->prm.sunshape ||
                       s->prm.micro_mode != scenes[0]->prm.micro_mode)) {
             om::set_error("scenes of a batch must share the sunshape and microsample kind");
@@ -393,7 +399,14 @@ extern "C" om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, co
         for (int r : (w == OM_BATCH_PER_SCENE ? rpt_none : rpt_list)) {
             om_launch lp{w, r};
             om_batch *b = nullptr;
-            if ((rc = batch_build(ctx, scenes, n, outs, &lp, &b)) != OM_OK) break;
+            if ((rc = batch_build(ctx, scenes, n, outs, &lp, &b)) != OM_OK) {
+                // a launch form the scenes cannot use (FP64 scenes and the persistent launch)
+                if (rc == OM_ERR_UNSUPPORTED && w != OM_BATCH_PER_SCENE) {
+                    rc = OM_OK;
+                    continue;
+                }
+                break;
+            }
             float ms = 0.0f;
             rc = om_batch_run(b, st);
             for (int k = 0; k < reps && rc == OM_OK; ++k) {

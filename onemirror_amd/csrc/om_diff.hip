@@ -636,6 +636,10 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
         set_error("differentiable mode skips the ray's own heliostat (literal_self is not supported)");
         return OM_ERR_UNSUPPORTED;
     }
+    if (s->prm.precision != OM_PREC_FP32) {
+        set_error("differentiable mode traces in FP32 geometry (precision OM_PREC_FP64 is not supported)");
+        return OM_ERR_UNSUPPORTED;
+    }
     if (!s->normal_in.empty()) {
         set_error("differentiable mode aims the heliostats itself (scene has given normals)");
         return OM_ERR_UNSUPPORTED;

@@ -99,6 +99,42 @@ struct TraceArgs {
     int literal_self;         // walks test the ray's own heliostat (om_params.literal_self)
 };
 
+// ---- FP64 geometry (om_params.precision = OM_PREC_FP64) ---------------------------------
+// The scene in double as the FP64 model builds it: heliostat centres, aimed normals and
+// their local2world frames, sizes and facet layout, the vertex parallelograms of the walk's
+// candidates, the receiver and the Buie table, all from the FP32 inputs converted exactly.
+struct alignas(16) Helio64 {
+    double pos[3], n[3], u[3], v[3]; // centre, normal, frame of local2world around n
+    double sz[3];                    // size (x width, y thickness, z height)
+    double subrow, subcol;           // facet row and column lengths
+    double gx, gy;                   // facet gaps
+    int ident;                       // frame is the identity (|n.x|, |n.z| < Epsilon)
+    int pad;
+};
+struct alignas(16) Geo64 {
+    double A[3], E1[3], E2[3]; // v0, v1 - v0, v3 - v0
+    int id;                    // the heliostat (entries of a grid's CSR list)
+    int pad;
+};
+
+// Kernel arguments of the FP64 trace beside TraceArgs (which supplies the tiles, the grid's
+// CSR and occupied range, the Philox key and stream, the maps and the records).
+struct Trace64Args {
+    const Helio64 *helio;
+    const Geo64 *cell_geo;                  // the launch grid's CSR entries in CSR order
+    const double *cdf_x, *cdf_y, *slope;    // Buie table, nbins bins
+    const int *guide;                       // [nbins]: the bin of value j / gscale
+    double gscale;                          // nbins / int465
+    int nbins;
+    double sun_n[3], sun_u[3], sun_v[3];
+    int sun_ident;
+    double pillbox, sigma, pitch;
+    double int465, A, B, C, inv_gp1;
+    double gpos[3], gint[3];                // the launch's grid
+    double rA[3], rE1[3], rE2[3];           // rectangle receiver: v0, v1 - v0, v3 - v0
+    double cyl_base[3], cyl_r, cyl_h, cone_r0, cone_r1;
+};
+
 // One map of a batch for the decode (reduce_batch_kernel): packed copies and side maps,
 // outputs, and the flux scale.
 struct ReduceSlot {
@@ -211,6 +247,11 @@ struct om_scene {
     // normals and tile ranges, per-tile adjoint partial sums
     om::DevBuf diff_buf;
     int rays_per_thread = 8;
+    // FP64 geometry: device block of Helio64[H], Geo64[H] and the double Buie table, and the
+    // host-side values of the kernel arguments (om_params.precision = OM_PREC_FP64)
+    char *d_f64 = nullptr;
+    std::vector<om::Geo64 *> d_cell_geo64; // per grid: this scene's double vertices of its CSR entries
+    om::Trace64Args a64{};
     // forks (om_scene_fork): the root scene whose grid, microsamples, tiles and Buie tables
     // this scene's device pointers share, and whose host CSR it reads; null for a root
     const om_scene *grid_src = nullptr;
@@ -228,6 +269,9 @@ void tile_range(const om_scene *s, int h0, int h1, int &t0, int &t1);
 // kernels (om_kernels.hip)
 om_status launch_poisson(om_scene *s, hipStream_t st);
 om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st);
+// the FP64 geometry trace (om_trace64.hip): the kernel arguments of s for grid g, the launch
+Trace64Args trace64_args(const om_scene *s, int g);
+om_status launch_trace64(om_scene *s, const TraceArgs &a, const Trace64Args &b, bool records, hipStream_t st);
 om_status launch_finalize(const unsigned long long *energy, float *raw, float *flux, float scale,
                           int n, hipStream_t st);
 om_status launch_smooth(const float *in, float *out, int w, int h, int radius, float ratio,

@@ -1211,7 +1211,7 @@ static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
     const size_t npx = (size_t)s->res_x * s->res_y;
     p = WsPlan{};
     p.win.res_x = s->res_x;
-    if (s->prm.accumulate == OM_ACC_WINDOW) {
+    if (s->prm.accumulate == OM_ACC_WINDOW && s->prm.precision != OM_PREC_FP64) {
         om_status rc = window_plan(s, p.win, p.nwg);
         if (rc != OM_OK) return rc;
     }
@@ -1257,6 +1257,7 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
     a.ntiles = t1 - t0;
     const int npx = s->res_x * s->res_y;
     om_status rc;
+    const bool fp64 = s->prm.precision == OM_PREC_FP64;
     if (s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW) {
         const int ncopy = 8 << p.shift;
         a.part = (unsigned long long *)base;
@@ -1270,7 +1271,10 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
         a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
         WindowRef win = p.win;
         win.nwin = 0;
-        if (p.nwg > 0 && a.ntiles > 0) {
+        if (fp64) {
+            // the FP64 trace accumulates per tile into the packed copies (no LDS window)
+            if ((rc = launch_trace64(s, a, trace64_args(s, g), false, st)) != OM_OK) return rc;
+        } else if (p.nwg > 0 && a.ntiles > 0) {
             a.win_queue = (uint32_t *)(base + p.off_queue);
             a.win_part = (unsigned long long *)(base + p.off_win);
             a.win_x0 = win.x0;
@@ -1293,7 +1297,7 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
     }
     a.counts = out->counts;
     a.energy = (unsigned long long *)out->energy;
-    return launch_trace(s, a, false, st);
+    return fp64 ? launch_trace64(s, a, trace64_args(s, g), false, st) : launch_trace(s, a, false, st);
 }
 
 om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st,
@@ -1539,7 +1543,10 @@ extern "C" om_status om_trace_records(om_scene *s, int h0, int h1, int32_t *shad
         a.rec_block = block;
         a.rec_pixel = pixel;
         a.rec_base = s->ray_offset[h0];
-        if (om_status st = launch_trace(s, a, true, (hipStream_t)stream); st != OM_OK) return st;
+        const om_status st = s->prm.precision == OM_PREC_FP64
+                                 ? launch_trace64(s, a, trace64_args(s, (int)g), true, (hipStream_t)stream)
+                                 : launch_trace(s, a, true, (hipStream_t)stream);
+        if (st != OM_OK) return st;
     }
     return OM_OK;
 }

@@ -1079,8 +1079,219 @@ void build_tiles(om_scene *s)
     for (om::GridState &G : s->grids) om::tile_range(s, G.h0, G.h1, G.t0, G.t1);
 }
 
+// ---- FP64 geometry (om_params.precision = OM_PREC_FP64) ---------------------------------
+// The scene in double as the FP64 model builds it (the same operations in the same order, so
+// that the values are the same bits): sun, receiver corners by face and rotateY
+// (receiver.cpp:24-117), the focus, each heliostat's aimed normal and local2world frame
+// (heliostat.cpp:16-62, global_function.cuh:14-46), its vertices, and the Buie table
+// (scene_instance_process.cu:116-226) with libm in double.
+struct D3 {
+    double x, y, z;
+};
+D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+D3 dadd(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+D3 dsub(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+D3 dscl(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+D3 dcross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+D3 dnorm(D3 v) { return dscl(v, 1.0 / std::sqrt(ddot(v, v))); }
+D3 dload(const float *p) { return d3(p[0], p[1], p[2]); }
+void dstore(double *o, D3 v)
+{
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+struct DFrame {
+    D3 n, u, v;
+    bool ident;
+};
+DFrame dframe(D3 n)
+{
+    DFrame f;
+    f.n = n;
+    f.ident = std::fabs(n.x) < 1e-6 && std::fabs(n.z) < 1e-6;
+    f.u = f.ident ? d3(1, 0, 0) : dnorm(dcross(d3(0.0, 1.0, 0.0), n));
+    f.v = f.ident ? d3(0, 0, 1) : dnorm(dcross(f.u, n));
+    return f;
+}
+D3 dworld(D3 d, const DFrame &f)
+{
+    if (f.ident) return d;
+    return d3(d.x * f.u.x + d.y * f.n.x + d.z * f.v.x, d.x * f.u.y + d.y * f.n.y + d.z * f.v.y,
+              d.x * f.u.z + d.y * f.n.z + d.z * f.v.z);
+}
+
+om_status upload_fp64(om_scene *s)
+{
+    om::Trace64Args &q = s->a64;
+    q = om::Trace64Args{};
+    const D3 sun = dnorm(dload(s->sun_in));
+    const DFrame sf = dframe(sun);
+    dstore(q.sun_n, sf.n);
+    dstore(q.sun_u, sf.u);
+    dstore(q.sun_v, sf.v);
+    q.sun_ident = sf.ident ? 1 : 0;
+    // receiver corners (the construction of receiver.cpp:24-117 in double)
+    const D3 sz = dload(s->recv_size), pos = dload(s->recv_pos);
+    D3 ln, v[4];
+    switch (s->recv_face) {
+    case 0:
+        ln = d3(0, 0, 1);
+        v[0] = d3(-sz.x / 2, -sz.y / 2, sz.z / 2); v[1] = d3(-sz.x / 2, sz.y / 2, sz.z / 2);
+        v[2] = d3(sz.x / 2, sz.y / 2, sz.z / 2); v[3] = d3(sz.x / 2, -sz.y / 2, sz.z / 2);
+        break;
+    case 1:
+        ln = d3(1, 0, 0);
+        v[0] = d3(sz.x / 2, -sz.y / 2, sz.z / 2); v[1] = d3(sz.x / 2, sz.y / 2, sz.z / 2);
+        v[2] = d3(sz.x / 2, sz.y / 2, -sz.z / 2); v[3] = d3(sz.x / 2, -sz.y / 2, -sz.z / 2);
+        break;
+    case 2:
+        ln = d3(0, 0, -1);
+        v[0] = d3(sz.x / 2, -sz.y / 2, -sz.z / 2); v[1] = d3(sz.x / 2, sz.y / 2, -sz.z / 2);
+        v[2] = d3(-sz.x / 2, sz.y / 2, -sz.z / 2); v[3] = d3(-sz.x / 2, -sz.y / 2, -sz.z / 2);
+        break;
+    default:
+        ln = d3(-1, 0, 0);
+        v[0] = d3(-sz.x / 2, -sz.y / 2, -sz.z / 2); v[1] = d3(-sz.x / 2, sz.y / 2, -sz.z / 2);
+        v[2] = d3(-sz.x / 2, sz.y / 2, sz.z / 2); v[3] = d3(-sz.x / 2, -sz.y / 2, sz.z / 2);
+        break;
+    }
+    const D3 rn = dnorm(dload(s->recv_normal_in));
+    D3 rv[4];
+    for (int i = 0; i < 4; ++i) {
+        const int dir = (dcross(ln, rn).y > 0) ? 1 : -1;
+        const double c = ddot(ln, rn);
+        const double sn = dir * std::sqrt(std::fmax(0.0, 1 - c * c));
+        const D3 o = v[i];
+        rv[i] = dadd(d3(c * o.x + sn * o.z, o.y, -sn * o.x + c * o.z), pos);
+    }
+    D3 focus = dscl(dadd(rv[0], rv[2]), 0.5);
+    dstore(q.rA, rv[0]);
+    dstore(q.rE1, dsub(rv[1], rv[0]));
+    dstore(q.rE2, dsub(rv[3], rv[0]));
+    if (s->recv_type == 1) {
+        const double r = s->recv_size[0], h = s->recv_size[1];
+        focus = pos;
+        dstore(q.cyl_base, d3(pos.x, pos.y - h / 2, pos.z));
+        q.cyl_r = r;
+        q.cyl_h = h;
+    } else if (s->recv_type == 2) {
+        const double h = s->recv_size[2];
+        focus = pos;
+        dstore(q.cyl_base, d3(pos.x, pos.y - h / 2, pos.z));
+        q.cone_r0 = s->recv_size[1];
+        q.cone_r1 = s->recv_size[0];
+        q.cyl_h = h;
+    }
+    // heliostats
+    const int H = s->H;
+    std::vector<om::Helio64> hel((size_t)std::max(H, 1));
+    std::vector<om::Geo64> geo((size_t)std::max(H, 1), om::Geo64{});
+    for (int i = 0; i < H; ++i) {
+        const D3 p = dload(&s->pos[3 * (size_t)i]), z = dload(&s->size[3 * (size_t)i]);
+        const D3 nn = !s->normal_in.empty() ? dnorm(dload(&s->normal_in[3 * (size_t)i]))
+                                            : dnorm(dsub(dnorm(dsub(focus, p)), sun));
+        const DFrame f = dframe(nn);
+        om::Helio64 &hr = hel[i];
+        dstore(hr.pos, p);
+        dstore(hr.n, f.n);
+        dstore(hr.u, f.u);
+        dstore(hr.v, f.v);
+        dstore(hr.sz, z);
+        hr.ident = f.ident ? 1 : 0;
+        const double gx = s->gap[2 * (size_t)i], gy = s->gap[2 * (size_t)i + 1];
+        const int rows = s->matrix[2 * (size_t)i], cols = s->matrix[2 * (size_t)i + 1];
+        hr.gx = gx;
+        hr.gy = gy;
+        hr.subrow = (z.z - gy * (rows - 1)) / rows;
+        hr.subcol = (z.x - gx * (cols - 1)) / cols;
+        const D3 l0 = d3(-z.x / 2, z.y / 2, -z.z / 2);
+        const D3 l1 = d3(l0.x, l0.y, l0.z + z.z);
+        const D3 l3 = d3(l0.x + z.x, l0.y, l0.z);
+        const D3 v0 = dadd(dworld(l0, f), p), v1 = dadd(dworld(l1, f), p), v3 = dadd(dworld(l3, f), p);
+        dstore(geo[i].A, v0);
+        dstore(geo[i].E1, dsub(v1, v0));
+        dstore(geo[i].E2, dsub(v3, v0));
+        geo[i].id = i;
+    }
+    // Buie table in double (inter_larger_465_intensity, parameters_generate)
+    const int ng = s->nbins;
+    const double csr = s->prm.csr, upper = 4.65;
+    const double k = 0.9 * std::log(13.5 * csr) * std::pow(csr, -0.3);
+    const double g = 2.2 * std::log(0.52 * csr) * std::pow(csr, 0.43) - 0.1;
+    const double i93 = std::exp(k) / (g + 1) * std::pow(9.3, g + 1), i465 = std::exp(k) / (g + 1) * std::pow(upper, g + 1);
+    std::vector<double> cx((size_t)ng + 1), cy((size_t)ng + 1), sl((size_t)std::max(ng, 1));
+    const double li = upper / ng;
+    cx[0] = 0;
+    cy[0] = 0;
+    double hp = 1.0;
+    for (int i = 1; i <= ng; ++i) {
+        const double x = li * i;
+        const double hc = std::cos(0.326 * x) / std::cos(0.308 * x);
+        const double sa = (hp + hc) * li / 2;
+        cx[i] = x;
+        cy[i] = sa + cy[i - 1];
+        sl[i - 1] = li / sa;
+        hp = hc;
+    }
+    q.nbins = ng;
+    q.int465 = cy[ng];
+    q.A = q.int465 + (i93 - i465);
+    q.B = (g + 1) / std::exp(k);
+    q.C = std::pow(upper, g + 1);
+    q.inv_gp1 = 1 / (g + 1);
+    q.pillbox = s->prm.pillbox_half_angle;
+    q.sigma = s->prm.slope_sigma;
+    q.pitch = s->prm.helio_pixel_length;
+    // guide of the Buie lookup: the bin of value j / gscale (the largest i with
+    // cdf_y[i] <= j / gscale); the device walk from it ends on the bin a binary search finds
+    q.gscale = ng / q.int465;
+    std::vector<int> guide((size_t)std::max(ng, 1), 0);
+    for (int j = 0, i = 0; j < ng; ++j) {
+        const double vj = j / q.gscale;
+        while (i < ng - 1 && cy[i + 1] <= vj) ++i;
+        guide[j] = i;
+    }
+    // one device block: Helio64[H] | cdf_x | cdf_y | slope | guide
+    const size_t nh = hel.size();
+    const size_t b_hel = sizeof(om::Helio64) * nh;
+    const size_t b_tab = sizeof(double) * ((size_t)ng + 1), b_sl = sizeof(double) * sl.size();
+    std::vector<char> blk(b_hel + 2 * b_tab + b_sl + sizeof(int) * guide.size());
+    char *o = blk.data();
+    std::memcpy(o, hel.data(), b_hel);
+    std::memcpy(o + b_hel, cx.data(), b_tab);
+    std::memcpy(o + b_hel + b_tab, cy.data(), b_tab);
+    std::memcpy(o + b_hel + 2 * b_tab, sl.data(), b_sl);
+    std::memcpy(o + b_hel + 2 * b_tab + b_sl, guide.data(), sizeof(int) * guide.size());
+    dfree(s->d_f64);
+    om_status st = upload(s->d_f64, blk.data(), blk.size());
+    if (st != OM_OK) return st;
+    q.helio = (const om::Helio64 *)s->d_f64;
+    q.cdf_x = (const double *)(s->d_f64 + b_hel);
+    q.cdf_y = (const double *)(s->d_f64 + b_hel + b_tab);
+    q.slope = (const double *)(s->d_f64 + b_hel + 2 * b_tab);
+    q.guide = (const int *)(s->d_f64 + b_hel + 2 * b_tab + b_sl);
+    // the candidates of every grid cell in CSR order with their ids (a walk reads a
+    // candidate's vertices without first reading its id)
+    const om_scene *root = s->grid_src ? s->grid_src : s;
+    for (om::Geo64 *&p : s->d_cell_geo64) dfree(p);
+    s->d_cell_geo64.assign(s->grids.size(), nullptr);
+    for (size_t k = 0; k < s->grids.size(); ++k) {
+        const std::vector<int> &items = root->grids[k].cell_items;
+        std::vector<om::Geo64> cg(items.size());
+        for (size_t i = 0; i < items.size(); ++i) cg[i] = geo[(size_t)items[i]];
+        if ((st = upload(s->d_cell_geo64[k], cg.data(), cg.size())) != OM_OK) return st;
+    }
+    return OM_OK;
+}
+
 om_status upload_geometry(om_scene *s)
 {
+    if (s->prm.precision == OM_PREC_FP64) {
+        om_status st = upload_fp64(s);
+        if (st != OM_OK) return st;
+    }
     pack_heliostats(s);
     om_status st = upload(s->d_helio, s->recs.data(), s->recs.size());
     if (st != OM_OK) return st;
@@ -1118,6 +1329,8 @@ om_status choose_tiles(om_scene *s)
 
 void free_scene_dev(om_scene *s)
 {
+    dfree(s->d_f64);
+    for (om::Geo64 *&p : s->d_cell_geo64) dfree(p);
     dfree(s->d_helio);
     dfree(s->d_geo);
     for (float4 *&p : s->d_cell_geo) dfree(p);
@@ -1211,6 +1424,10 @@ om_status validate(const om_scene_desc *d, const om_params *p)
     }
     if (p->micro_mode != OM_MICRO_LATTICE && p->micro_mode != OM_MICRO_POISSON) {
         om::set_error("unknown microsample mode");
+        return OM_ERR_INVALID;
+    }
+    if (p->precision != OM_PREC_FP32 && p->precision != OM_PREC_FP64) {
+        om::set_error("precision must be OM_PREC_FP32 or OM_PREC_FP64");
         return OM_ERR_INVALID;
     }
     if (p->sunshape == OM_SUN_BUIE && (p->buie_bins <= 0 || p->buie_bins > 8192 || !(p->csr > 0.0f))) {
@@ -1643,6 +1860,8 @@ extern "C" om_status om_scene_fork(const om_scene *src, const float sun_dir[3], 
     s->grid_src = src->grid_src ? src->grid_src : src;
     s->d_helio = nullptr;
     s->d_geo = nullptr;
+    s->d_f64 = nullptr;
+    s->d_cell_geo64.assign(ng, nullptr);
     s->d_cell_geo.assign(ng, nullptr);
     s->diff_buf = om::DevBuf{};
     s->prm.scene_index = scene_index;
