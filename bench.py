@@ -55,7 +55,8 @@ def model_ops(st: dict) -> tuple:
 # VALU issue probes of om_math_eval (fn 12..14): VALU instructions per lane of each kernel,
 # counted in the assembly of this build (the loop's instructions times its trips, plus the
 # set-up and the final sum)
-VALU_PROBES = {12: ("fma_f32", 2048 + 27), 13: ("fma_f32+add_u32", 4096 + 42), 14: ("fma_f32+exp_f32", 2560 + 27)}
+VALU_PROBES = {12: ("fma_f32", 2048 + 27), 13: ("fma_f32+add_u32", 4096 + 42), 14: ("fma_f32+exp_f32", 2560 + 27),
+               15: ("fma_f64", 2048 + 27)}
 
 
 def valu_probe(mi: int, reps: int = 5) -> dict:

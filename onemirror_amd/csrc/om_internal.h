@@ -272,6 +272,8 @@ om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_
 // the FP64 geometry trace (om_trace64.hip): the kernel arguments of s for grid g, the launch
 Trace64Args trace64_args(const om_scene *s, int g);
 om_status launch_trace64(om_scene *s, const TraceArgs &a, const Trace64Args &b, bool records, hipStream_t st);
+// the FP64 trace with the LDS-window accumulation (nwg persistent workgroups)
+om_status launch_trace64_window(om_scene *s, const TraceArgs &a, const Trace64Args &b, int nwg, hipStream_t st);
 om_status launch_finalize(const unsigned long long *energy, float *raw, float *flux, float scale,
                           int n, hipStream_t st);
 om_status launch_smooth(const float *in, float *out, int w, int h, int radius, float ratio,

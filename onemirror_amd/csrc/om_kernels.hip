@@ -16,22 +16,9 @@
 #include <string>
 #include <vector>
 
-#include "om_device.h"
+#include "om_window.h"
 
 namespace om {
-
-// Add one hit (v = 2^E + q) to a packed word; the old value tells whether the add carried
-// out of the energy field (bits 0..E-1) or wrapped the word, and both events are counted in
-// the side maps of pixel px, which the decode applies.
-__device__ __forceinline__ void packed_add_lds(const TraceArgs &a, unsigned long long *w, int px, uint32_t q)
-{
-    const int E = a.pack_shift;
-    const unsigned long long lowmask = (1ull << E) - 1ull;
-    const unsigned long long v = (1ull << E) + q;
-    const unsigned long long old = __hip_atomic_fetch_add(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if ((old & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
-    if (old + v < old) atomicAdd(a.wrap + px, 1u);
-}
 
 // Per-tile state of one heliostat: its record, frame and position, and the sun frame.
 struct HelioTile {
@@ -315,118 +302,6 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs)
                                          s_guide);
 }
 
-// ---------------------------------------------------------------------------------------
-// OM_ACC_WINDOW: window of the packed map in LDS, persistent workgroups
-// ---------------------------------------------------------------------------------------
-// One workgroup per CU keeps the central block of the packed map in LDS (WIN_PX_MAX words
-// with the ring below: the whole 160 KiB) for every ray it traces.  WIN_COMPUTE of its waves
-// trace rays: hits inside the window are LDS atomics, hits outside go to a ring buffer in
-// LDS.  The last wave drains the ring into the packed per-XCD copies in device memory, so
-// the tracing waves never hold their loads behind a memory-side atomic.  The window is
-// stored (not added) to this workgroup's slice of win_part at the end.
-constexpr int WIN_WGS = 1024;
-constexpr int WIN_COMPUTE = WIN_WGS / 64 - 1; // tracing waves; the last wave drains the ring
-constexpr int WIN_RING = 2048;                // ring entries (px << 32 | q), a power of two
-constexpr int WIN_DRAIN = 16;                 // ring entries per lane of the draining wave per round
-static_assert(64 * WIN_DRAIN + 64 <= WIN_RING, "the ring holds a drain round and a pass beyond it");
-constexpr int WIN_CHUNK = 2048;               // rays a tracing wave takes from a queue at a time
-constexpr int WIN_PX_MAX = (160 * 1024 - WIN_RING * 8 - 256) / 8;
-constexpr unsigned long long RING_EMPTY = ~0ull;
-
-struct WinRing {
-    unsigned long long *ring; // [WIN_RING]
-    uint32_t *tail;           // entries reserved by the tracing waves
-    uint32_t *head;           // entries drained (written by the draining wave only)
-    uint32_t *done;           // tracing waves that have finished
-};
-
-// Adds this pass's hits outside the window (out) to the ring, in lane order.  Lane 0
-// reserves the wave's entries with one LDS atomic and, before they are written, waits until
-// the ring has room for them; head_seen caches the last head read.  The draining wave only
-// waits for entries within 64 * WIN_DRAIN of the head, whose writers need no more room than
-// the ring has at that head (64 * WIN_DRAIN + 64 <= WIN_RING), so neither waits for the other.
-__device__ __forceinline__ void ring_push(const WinRing &R, bool out, int px, uint32_t q, int lane,
-                                          uint32_t &head_seen)
-{
-    const unsigned long long m = __ballot(out);
-    if (m == 0ull) return;
-    const uint32_t n = (uint32_t)__popcll(m);
-    uint32_t base = 0;
-    if (lane == 0) base = __hip_atomic_fetch_add(R.tail, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    base = __builtin_amdgcn_readlane(base, 0);
-    while (base + n - head_seen > (uint32_t)WIN_RING) {
-        head_seen = __hip_atomic_load(R.head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (base + n - head_seen > (uint32_t)WIN_RING) __builtin_amdgcn_s_sleep(2);
-    }
-    if (out) {
-        const uint32_t k = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        __hip_atomic_store(R.ring + ((base + k) & (uint32_t)(WIN_RING - 1)), ((unsigned long long)px << 32) | q,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-}
-
-// The draining wave: takes up to 64 * WIN_DRAIN entries at a time from the ring head, adds
-// them to the packed per-XCD copies (one returning 64-bit atomic each, carries and wraps
-// counted as in flush_slots), clears them and publishes the new head; ends when every
-// tracing wave has finished and the ring is empty.  Rounds are large because each waits
-// once for the memory side.
-__device__ void ring_drain(const TraceArgs &a, const WinRing &R, int lane)
-{
-    const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
-    unsigned long long *part = a.part + (size_t)copy * a.copy_stride;
-    const int E = a.pack_shift;
-    const unsigned long long lowmask = (1ull << E) - 1ull;
-    uint32_t head = 0;
-    for (;;) {
-        const uint32_t done = __hip_atomic_load(R.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t tail = __hip_atomic_load(R.tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t avail = tail - head;
-        if (avail == 0u) {
-            if (done == (uint32_t)WIN_COMPUTE) break;
-            __builtin_amdgcn_s_sleep(4);
-            continue;
-        }
-        const uint32_t nb = min(avail, (uint32_t)(64 * WIN_DRAIN));
-        unsigned long long e[WIN_DRAIN], old[WIN_DRAIN];
-#pragma unroll
-        for (int b = 0; b < WIN_DRAIN; ++b) {
-            e[b] = RING_EMPTY;
-            const uint32_t i = (uint32_t)(b * 64 + lane);
-            if (i < nb) {
-                // reserved entries are written right after their reservation
-                unsigned long long *slot = R.ring + ((head + i) & (uint32_t)(WIN_RING - 1));
-                do {
-                    e[b] = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                } while (e[b] == RING_EMPTY);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < WIN_DRAIN; ++b) {
-            old[b] = 0ull;
-            if (e[b] != RING_EMPTY) {
-                const uint32_t px = (uint32_t)(e[b] >> 32);
-                const unsigned long long v = (1ull << E) + (e[b] & 0xffffffffull);
-                old[b] = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, v, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < WIN_DRAIN; ++b) {
-            if (e[b] != RING_EMPTY) {
-                const uint32_t px = (uint32_t)(e[b] >> 32);
-                const unsigned long long q = e[b] & 0xffffffffull;
-                const unsigned long long v = (1ull << E) + q;
-                if ((old[b] & lowmask) + q > lowmask) atomicAdd(a.carry + px, 1u);
-                if (old[b] + v < old[b]) atomicAdd(a.wrap + px, 1u);
-                __hip_atomic_store(R.ring + ((head + (uint32_t)(b * 64 + lane)) & (uint32_t)(WIN_RING - 1)),
-                                   RING_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-        head += nb;
-        if (lane == 0) __hip_atomic_store(R.head, head, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-}
-
 // nrays (a multiple of 64) rays of heliostat tl.x from tl.y, traced by one wave, one ray per
 // lane per pass.  A lane finds the origin of its microsample again only when the
 // microsample changes: with L >= 64 rays per microsample a pass holds at most two, and most
@@ -478,12 +353,7 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
     }
 }
 
-// Tracing waves take chunks of up to WIN_CHUNK rays of one tile from 8 queues, one per XCD,
-// each over a contiguous eighth of the tile list: a wave drains its own XCD's queue first,
-// then helps the others, so no wave waits for another until all queues are empty.  The
-// waves of an XCD take consecutive chunks, so at any time they trace a few neighbouring
-// heliostats whose grid cells and blockers stay in that XCD's L2 (spreading an XCD over
-// several ranges of the list is slower).
+// The window kernel of the FP32 geometry: window_body (om_window.h) over trace_chunk_window.
 // The arguments are read from the kernel-argument segment where each use needs them (a
 // compiler barrier per ray pass keeps the loads there): held in SGPRs across the ray loop,
 // they exceed the SGPR file and about a hundred of them spill to VGPR lanes, each use then
@@ -495,40 +365,10 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window_kernel(TraceArgs)
     __shared__ unsigned long long s_win[WIN_PX_MAX];
     __shared__ unsigned long long s_ring[WIN_RING];
     __shared__ uint32_t s_ctl[3];
-    const int wpx = a.win_w * a.win_h; // <= WIN_PX_MAX, checked on the host
-    for (int i = threadIdx.x; i < wpx; i += WIN_WGS) s_win[i] = 0ull;
-    for (int i = threadIdx.x; i < WIN_RING; i += WIN_WGS) s_ring[i] = RING_EMPTY;
-    if (threadIdx.x < 3) s_ctl[threadIdx.x] = 0u;
-    __syncthreads();
-    const WinRing R{s_ring, s_ctl, s_ctl + 1, s_ctl + 2};
-    const int lane = (int)threadIdx.x & 63;
-    if ((int)(threadIdx.x >> 6) == WIN_COMPUTE) {
-        ring_drain(a, R, lane);
-    } else {
-        // tiles hold 64..2048 rays (a power of two): chunks of min(tile, WIN_CHUNK) rays
-        const int chunk = min(a.tile_rays, WIN_CHUNK);
-        const int parts = a.tile_rays / chunk;
-        const int n = a.ntiles * parts, q = n >> 3, rmd = n & 7;
-        const uint32_t xcd = xcc_id() & 7u;
-        for (uint32_t j = 0; j < 8u; ++j) {
-            const int x = (int)((xcd + j) & 7u);
-            const int first = x * q + min(x, rmd), cnt = q + (x < rmd ? 1 : 0);
-            for (;;) {
-                uint32_t t = 0;
-                if (lane == 0) t = atomicAdd(a.win_queue + x, 1u);
-                t = __builtin_amdgcn_readfirstlane(t); // the whole wave is active here
-                if (t >= (uint32_t)cnt) break;
-                const int c = first + (int)t;
-                const int tile = c / parts, part = c - tile * parts;
-                const int2 tl = a.tiles[a.tile_base + tile];
-                trace_chunk_window<SUN, POISSON>(a, make_int2(tl.x, tl.y + part * chunk), lane, chunk, R, s_win);
-            }
-        }
-        if (lane == 0) __hip_atomic_fetch_add(R.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    __syncthreads();
-    unsigned long long *out = a.win_part + (size_t)blockIdx.x * wpx;
-    for (int i = threadIdx.x; i < wpx; i += WIN_WGS) out[i] = s_win[i];
+    window_body(a, s_win, s_ring, s_ctl,
+                [&](int2 tl, int lane, int nrays, const WinRing &R, unsigned long long *w) {
+                    trace_chunk_window<SUN, POISSON>(a, tl, lane, nrays, R, w);
+                });
 }
 
 // Batched mode: persistent workgroups take (scene, heliostat, first ray) items from a
@@ -860,6 +700,7 @@ __global__ void math_kernel(int fn, const float *in, float *out, int n)
 //   mix 0: 8 v_fma_f32
 //   mix 1: 8 v_fma_f32 + 8 v_add_u32 (integer adds on separate chains)
 //   mix 2: 8 v_fma_f32 + 2 v_exp_f32 (transcendental)
+//   mix 3: 8 v_fma_f64 (the chains in double)
 // The loop counter lives in scalar registers.
 constexpr int PROBE_ITERS = 256;
 
@@ -877,6 +718,22 @@ __global__ __launch_bounds__(WG) void valu_probe_kernel(const float *in, float *
         k[j] = f2u(x0) + (uint32_t)j;
     }
     const float a = 0.99951171875f, b = 0.000244140625f;
+    if (MIX == 3) {
+        double xd[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) xd[j] = (double)x0 + (double)j;
+        for (int it = 0; it < PROBE_ITERS; ++it) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) xd[j] = __builtin_fma(xd[j], (double)b, xd[j]); // x (1 + 2^-12)
+            asm volatile("" : "+v"(xd[0]), "+v"(xd[1]), "+v"(xd[2]), "+v"(xd[3]), "+v"(xd[4]), "+v"(xd[5]),
+                         "+v"(xd[6]), "+v"(xd[7]));
+        }
+        double sd = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sd += xd[j];
+        out[i] = (float)sd;
+        return;
+    }
     for (int it = 0; it < PROBE_ITERS; ++it) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) x[j] = __builtin_fmaf(x[j], a, b);
@@ -1074,14 +931,15 @@ om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st)
 
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st)
 {
-    if (fn < 0 || fn > 14) {
-        set_error("math function index must be in 0..14");
+    if (fn < 0 || fn > 15) {
+        set_error("math function index must be in 0..15");
         return OM_ERR_INVALID;
     }
     const dim3 grid((n + WG - 1) / WG), block(WG);
     if (fn == 12) hipLaunchKernelGGL(valu_probe_kernel<0>, grid, block, 0, st, in, out, n);
     else if (fn == 13) hipLaunchKernelGGL(valu_probe_kernel<1>, grid, block, 0, st, in, out, n);
     else if (fn == 14) hipLaunchKernelGGL(valu_probe_kernel<2>, grid, block, 0, st, in, out, n);
+    else if (fn == 15) hipLaunchKernelGGL(valu_probe_kernel<3>, grid, block, 0, st, in, out, n);
     else hipLaunchKernelGGL(math_kernel, grid, block, 0, st, fn, in, out, n);
     OM_HIP(hipGetLastError());
     return OM_OK;
@@ -1211,7 +1069,7 @@ static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
     const size_t npx = (size_t)s->res_x * s->res_y;
     p = WsPlan{};
     p.win.res_x = s->res_x;
-    if (s->prm.accumulate == OM_ACC_WINDOW && s->prm.precision != OM_PREC_FP64) {
+    if (s->prm.accumulate == OM_ACC_WINDOW) {
         om_status rc = window_plan(s, p.win, p.nwg);
         if (rc != OM_OK) return rc;
     }
@@ -1271,8 +1129,7 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
         a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
         WindowRef win = p.win;
         win.nwin = 0;
-        if (fp64) {
-            // the FP64 trace accumulates per tile into the packed copies (no LDS window)
+        if (fp64 && !(p.nwg > 0 && a.ntiles > 0)) {
             if ((rc = launch_trace64(s, a, trace64_args(s, g), false, st)) != OM_OK) return rc;
         } else if (p.nwg > 0 && a.ntiles > 0) {
             a.win_queue = (uint32_t *)(base + p.off_queue);
@@ -1285,7 +1142,8 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
             win.part = a.win_part;
             win.nwin = p.nwg;
             if ((rc = launch_fill_u32(a.win_queue, 8, st)) != OM_OK) return rc;
-            if ((rc = launch_trace_window(s, a, p.nwg, st)) != OM_OK) return rc;
+            rc = fp64 ? launch_trace64_window(s, a, trace64_args(s, g), p.nwg, st) : launch_trace_window(s, a, p.nwg, st);
+            if (rc != OM_OK) return rc;
         } else if ((rc = launch_trace(s, a, false, st)) != OM_OK) {
             return rc;
         }

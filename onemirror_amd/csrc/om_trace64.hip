@@ -13,7 +13,7 @@
 // scene_instance_process.cu:6-226, receiver_drawing (:135-153).
 #include <hip/hip_runtime.h>
 
-#include "om_device.h"
+#include "om_window.h"
 
 namespace om {
 namespace {
@@ -107,7 +107,9 @@ __device__ __forceinline__ bool parallelogram64(d3 orig, d3 dir, d3 A, d3 E1, d3
 // NotCollision + Intersect (rectgrid_raytracing.cu:21-117) in double: start cell by
 // truncation and never tested, absDivide's INT_MAX for |dir| <= 1e-6, the reference's
 // tie-break, first candidate in CSR order; the walk ends once an axis has passed the occupied
-// index range (every later cell is empty).  self: the id skipped, or -1.
+// index range (every later cell is empty).  The FP64 model divides by each direction
+// component once (absDivide's quotients as products with 1 / dir).  self: the id skipped,
+// or -1.
 __device__ int walk64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d, int self)
 {
     const d3 rel = sub(o, ld(b.gpos));
@@ -120,8 +122,9 @@ __device__ int walk64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d, int 
     for (int k = 0; k < 3; ++k) {
         st[k] = dd[k] >= 0 ? 1 : -1;
         const double t = dd[k] >= 0 ? (g[k] + 1) * b.gint[k] - cur[k] : cur[k] - g[k] * b.gint[k];
-        tm[k] = (fabs(dd[k]) <= 1e-6) ? 2147483647.0 : fabs(t / dd[k]);
-        td[k] = (fabs(dd[k]) <= 1e-6) ? 2147483647.0 : fabs(b.gint[k] / dd[k]);
+        const double inv = 1.0 / dd[k];
+        tm[k] = (fabs(dd[k]) <= 1e-6) ? 2147483647.0 : fabs(t * inv);
+        td[k] = (fabs(dd[k]) <= 1e-6) ? 2147483647.0 : fabs(b.gint[k] * inv);
     }
     for (;;) {
         int k;
@@ -146,8 +149,9 @@ __device__ int walk64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d, int 
     }
 }
 
-// receiver_drawing in double; cylinder and cone as the new receivers of the FP32 path
-__device__ int receiver64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d)
+// receiver_drawing in double; cylinder and cone as the new receivers of the FP32 path:
+// (row, column) of the hit, row -1 for a miss
+__device__ int2 receiver64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d)
 {
     const int rx = a.res_x, ry = a.res_y;
     if (a.recv_type == 1) {
@@ -155,19 +159,19 @@ __device__ int receiver64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d)
         const double qa = d.x * d.x + d.z * d.z, qb = 2.0 * (ox * d.x + oz * d.z);
         const double qc = ox * ox + oz * oz - b.cyl_r * b.cyl_r;
         const double disc = qb * qb - 4.0 * qa * qc;
-        if (qa < 1e-6 || disc <= 0.0) return -1;
+        if (qa < 1e-6 || disc <= 0.0) return make_int2(-1, -1);
         const double sq = sqrt(disc);
         const double q0 = -0.5 * (qb + (qb >= 0 ? sq : -sq));
         const double t1 = q0 / qa, t2 = qc / q0;
         const double t = t1 < t2 ? t1 : t2;
-        if (t < 1e-6) return -1;
+        if (t < 1e-6) return make_int2(-1, -1);
         const double hx = ox + t * d.x, hz = oz + t * d.z, hy = o.y + t * d.y - b.cyl_base[1];
-        if (hy < 0.0 || hy > b.cyl_h) return -1;
+        if (hy < 0.0 || hy > b.cyl_h) return make_int2(-1, -1);
         double th = atan2_64(hz, hx);
         if (th < 0.0) th += 6.283185307179586;
         const int col = min((int)(th / 6.283185307179586 * rx), rx - 1);
         const int row = min((int)(hy / b.cyl_h * ry), ry - 1);
-        return row * rx + col;
+        return make_int2(row, col);
     }
     if (a.recv_type == 2) {
         const double h = b.cyl_h, r0 = b.cone_r0, r1 = b.cone_r1, k = (r1 - r0) / h;
@@ -177,7 +181,7 @@ __device__ int receiver64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d)
         const double qb = 2.0 * (ox * d.x + oz * d.z - w0 * wd);
         const double qc = ox * ox + oz * oz - w0 * w0;
         const double disc = qb * qb - 4.0 * qa * qc;
-        if (d.x * d.x + d.z * d.z < 1e-6 || disc < 0.0 || qa == 0.0) return -1;
+        if (d.x * d.x + d.z * d.z < 1e-6 || disc < 0.0 || qa == 0.0) return make_int2(-1, -1);
         const double sq = sqrt(disc);
         const double q0 = -0.5 * (qb + (qb >= 0 ? sq : -sq));
         const double ta = q0 / qa, tb = qc / q0;
@@ -192,19 +196,18 @@ __device__ int receiver64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d)
             hy = yi;
             break;
         }
-        if (t < 0.0) return -1;
+        if (t < 0.0) return make_int2(-1, -1);
         const double hx = ox + t * d.x, hz = oz + t * d.z;
-        if (!(d.x * hx + d.z * hz - d.y * k * (r0 + k * hy) < 0.0)) return -1;
+        if (!(d.x * hx + d.z * hz - d.y * k * (r0 + k * hy) < 0.0)) return make_int2(-1, -1);
         double th = atan2_64(hz, hx);
         if (th < 0.0) th += 6.283185307179586;
         const int col = min((int)(th / 6.283185307179586 * rx), rx - 1);
         const int row = min((int)(hy * (2.0 * r0 + k * hy) / (h * (r0 + r1)) * ry), ry - 1);
-        return row * rx + col;
+        return make_int2(row, col);
     }
     double t, u, v;
-    if (!parallelogram64(o, d, ld(b.rA), ld(b.rE1), ld(b.rE2), t, u, v)) return -1;
-    const int row = min((int)(u * ry), ry - 1), col = min((int)(v * rx), rx - 1);
-    return row * rx + col;
+    if (!parallelogram64(o, d, ld(b.rA), ld(b.rE1), ld(b.rE2), t, u, v)) return make_int2(-1, -1);
+    return make_int2(min((int)(u * ry), ry - 1), min((int)(v * rx), rx - 1));
 }
 
 // microsample centre in the heliostat frame (steps_for_raytracing.cu:4-38, :53-65)
@@ -226,7 +229,8 @@ __device__ __forceinline__ d3 micro_local64(const TraceArgs &a, const Trace64Arg
 }
 
 struct Ray64 {
-    int shade, block, px;
+    int shade, block, px; // px: receiver pixel, -1 for none
+    int2 rc;              // its (row, column)
     uint32_t q;
 };
 
@@ -236,7 +240,7 @@ template <int SUN>
 __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dframe &hf, const dframe &sf, d3 orig,
                              int h, int m, int l, int self, bool clear)
 {
-    Ray64 r{-1, -2, -1, 0u};
+    Ray64 r{-1, -2, -1, make_int2(-1, -1), 0u};
     const u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3}, a.key0, a.key1);
     const double u0 = u01(x.x), u1 = u01(x.y), u2 = u01(x.z), u3 = u01(x.w);
     double th;
@@ -268,8 +272,11 @@ __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dfr
     const d3 refl = normalized(reflected(neg(to_sun), nrm));
     r.block = walk64(a, b, orig, refl, self);
     if (r.block >= 0) return r;
-    r.px = receiver64(a, b, orig, refl);
-    if (r.px >= 0) r.q = (uint32_t)(fabs(dot(ld(b.sun_n), nrm)) * 2147483648.0);
+    r.rc = receiver64(a, b, orig, refl);
+    if (r.rc.x >= 0) {
+        r.px = r.rc.x * a.res_x + r.rc.y;
+        r.q = (uint32_t)(fabs(dot(ld(b.sun_n), nrm)) * 2147483648.0);
+    }
     return r;
 }
 
@@ -311,13 +318,18 @@ __global__ __launch_bounds__(WG) void trace64_kernel(Trace64Launch)
     const int L = a.L;
     const int nr = min(hr->mcount * L, tl.y + a.tile_rays);
     int nk = 0;
+    int m_o = -1; // microsample of orig: its origin is found again only when m changes
+    d3 orig{0.0, 0.0, 0.0};
     for (int k = 0; k < a.rays_per_thread; ++k) {
         const int r = tl.y + tid + k * WG;
         if (r >= nr) break;
         nk = k + 1;
         asm volatile("" ::: "memory"); // arguments re-read per ray pass
         const int m = r / L, l = r - m * L;
-        const d3 orig = add(world(micro_local64<POISSON>(a, b, hr, h64, h, m), hf), hpos);
+        if (m != m_o) {
+            orig = add(world(micro_local64<POISSON>(a, b, hr, h64, h, m), hf), hpos);
+            m_o = m;
+        }
         const Ray64 o = trace_ray64<SUN>(a, b, hf, sf, orig, h, m, l, self, clear);
         if (RECORD) {
             const long long rid = (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
@@ -355,7 +367,95 @@ __global__ __launch_bounds__(WG) void trace64_kernel(Trace64Launch)
     }
 }
 
+// nrays (a multiple of 64) rays of heliostat tl.x from tl.y traced by one wave of a window
+// workgroup, one ray per lane per pass: hits inside the LDS window are LDS atomics, the
+// others go to the ring (window_body, om_window.h).
+template <int SUN, bool POISSON>
+__device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const Trace64Args &b, int2 tl, const int lane,
+                                                     const int nrays, const WinRing &R, unsigned long long *s_win)
+{
+    const int h = tl.x;
+    const HelioRec *hr = a.helio + h;
+    const Helio64 &h64 = b.helio[h];
+    dframe hf, sf;
+    hf.n = ld(h64.n);
+    hf.u = ld(h64.u);
+    hf.v = ld(h64.v);
+    hf.ident = h64.ident != 0;
+    sf.n = ld(b.sun_n);
+    sf.u = ld(b.sun_u);
+    sf.v = ld(b.sun_v);
+    sf.ident = b.sun_ident != 0;
+    const d3 hpos = ld(h64.pos);
+    const int self = a.literal_self ? -1 : h;
+    const bool clear = hr->clear != 0 && !a.literal_self;
+    const int L = a.L;
+    const int nr = min(hr->mcount * L, tl.y + nrays);
+    uint32_t head_seen = 0u;
+    int m_o = -1; // microsample of orig
+    d3 orig{0.0, 0.0, 0.0};
+    for (int k = 0; k < nrays; k += 64) {
+        const int r = tl.y + k + lane;
+        bool out = false;
+        int px = 0;
+        uint32_t q = 0u;
+        asm volatile("" ::: "memory"); // arguments re-read per ray pass
+        if (r < nr) {
+            const int m = r / L, l = r - m * L;
+            if (m != m_o) {
+                orig = add(world(micro_local64<POISSON>(a, b, hr, h64, h, m), hf), hpos);
+                m_o = m;
+            }
+            const Ray64 o = trace_ray64<SUN>(a, b, hf, sf, orig, h, m, l, self, clear);
+            if (o.px >= 0) {
+                px = o.px;
+                q = o.q;
+                const uint32_t wr = (uint32_t)(o.rc.x - a.win_y0), wc = (uint32_t)(o.rc.y - a.win_x0);
+                if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w)
+                    packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+                else
+                    out = true;
+            }
+        }
+        ring_push(R, out, px, q, lane, head_seen); // the whole wave takes part: no lane has left the loop
+    }
+}
+
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
+{
+    const Trace64Launch &args = *(const Trace64Launch *)__builtin_amdgcn_kernarg_segment_ptr();
+    __shared__ unsigned long long s_win[WIN_PX_MAX];
+    __shared__ unsigned long long s_ring[WIN_RING];
+    __shared__ uint32_t s_ctl[3];
+    window_body(args.a, s_win, s_ring, s_ctl,
+                [&](int2 tl, int lane, int nrays, const WinRing &R, unsigned long long *w) {
+                    trace_chunk_window64<SUN, POISSON>(args.a, args.b, tl, lane, nrays, R, w);
+                });
+}
+
 } // namespace
+
+om_status launch_trace64_window(om_scene *s, const TraceArgs &a, const Trace64Args &b, int nwg, hipStream_t st)
+{
+    if (a.ntiles <= 0) return OM_OK;
+    if (!b.helio || !b.cell_geo) {
+        set_error("scene has no FP64 geometry");
+        return OM_ERR_INVALID;
+    }
+    const bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
+    const dim3 grid(nwg), block(WIN_WGS);
+    const Trace64Launch ab{a, b};
+    if (s->prm.sunshape == OM_SUN_BUIE) {
+        if (poisson) hipLaunchKernelGGL((trace_window64_kernel<OM_SUN_BUIE, true>), grid, block, 0, st, ab);
+        else hipLaunchKernelGGL((trace_window64_kernel<OM_SUN_BUIE, false>), grid, block, 0, st, ab);
+    } else {
+        if (poisson) hipLaunchKernelGGL((trace_window64_kernel<OM_SUN_PILLBOX, true>), grid, block, 0, st, ab);
+        else hipLaunchKernelGGL((trace_window64_kernel<OM_SUN_PILLBOX, false>), grid, block, 0, st, ab);
+    }
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
 
 Trace64Args trace64_args(const om_scene *s, int g)
 {

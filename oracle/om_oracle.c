@@ -1354,11 +1354,14 @@ static int dda64(const oc_scene *s, const oc_scene64 *q, d3 o, d3 d, int self)
     double dd[3] = { d.x, d.y, d.z }, gi[3] = { gint.x, gint.y, gint.z }, cur[3] = { rel.x, rel.y, rel.z };
     int st[3];
     double tm[3], td[3];
+    /* absDivide's quotients as products with 1 / dir (one division per axis: the FP64
+     * model's form; either form is exact to an ulp) */
     for (int a = 0; a < 3; ++a) {
         st[a] = dd[a] >= 0 ? 1 : -1;
         double t = dd[a] >= 0 ? (g[a] + 1) * gi[a] - cur[a] : cur[a] - g[a] * gi[a];
-        tm[a] = (fabs(dd[a]) <= OC_EPS) ? 2147483647.0 : fabs(t / dd[a]);
-        td[a] = (fabs(dd[a]) <= OC_EPS) ? 2147483647.0 : fabs(gi[a] / dd[a]);
+        double inv = 1.0 / dd[a];
+        tm[a] = (fabs(dd[a]) <= OC_EPS) ? 2147483647.0 : fabs(t * inv);
+        td[a] = (fabs(dd[a]) <= OC_EPS) ? 2147483647.0 : fabs(gi[a] * inv);
     }
     for (;;) {
         int a;

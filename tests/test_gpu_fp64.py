@@ -54,9 +54,9 @@ def _flux_err(g, o, want):
     return np.abs(flux32[lit] - flux64[lit]) / flux64[lit]
 
 
-@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder", "cone"])
+@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder", "cone", "small_window", "cylinder_window"])
 def test_small_fields_records_counts_flux(om, cfg):
-    if cfg == "cylinder":
+    if cfg.startswith("cylinder"):
         d, p = F.config_small_cylinder()
     elif cfg == "cone":
         d, p = F.config_small_cone()
@@ -64,7 +64,10 @@ def test_small_fields_records_counts_flux(om, cfg):
         d, p = F.config_small()
         if cfg == "small_lattice":
             p = F.with_lattice(p)
+    # "_window": the LDS-window accumulation (the map equals the packed one)
     p = _fp64(p, om)
+    if cfg.endswith("_window"):
+        p = dict(p, accumulate=om.ACC_WINDOW)
     g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
     want = o.trace_f64(records=True)
     _records_equal(g.trace_records(), want, cfg)
@@ -80,9 +83,10 @@ def test_field_slice_records(om, name, h0, h1):
     _records_equal(g.trace_records(h0, h1), o.trace_f64(h0, h1, records=True), f"{name} [{h0}, {h1})")
 
 
-def test_mid_whole_field_flux_within_1e5_of_fp64(om):
+@pytest.mark.parametrize("acc", ["packed", "window"])
+def test_mid_whole_field_flux_within_1e5_of_fp64(om, acc):
     d, p = F.config_mid()
-    p = _fp64(p, om)
+    p = dict(_fp64(p, om), accumulate=om.ACC_WINDOW if acc == "window" else om.ACC_PACKED)
     g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
     want = o.trace_f64()
     rel = _flux_err(g, o, want)
