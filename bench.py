@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "rays/s (w/ DDA shading+blocking) at ≤1e-5 flux err vs FP64; batched & fwd+bwd/s"
 P_VALU = 256 * 128 * 2.4e9         # lane-op/s: two 64-lane VALU instructions per CU clock
+P_VALU64 = 256 * 64 * 2.4e9        # FP64 lane-op/s: one 64-lane FP64 instruction per CU clock
 F0, C_STEP, C_TEST, C_SKIP = 569.0, 12.0, 58.0, 3.0
 C_SETUP = 57.0  # one walk set-up, the half of F0's two that a skipped shading walk saves
 
@@ -123,14 +124,14 @@ def host_threads() -> int:
     return host_cores()["threads"]
 
 
-def flux_error_vs_fp64(scene, maps, osc, nthreads: int, block: int = 2000) -> dict:
-    """FP32 flux of the GPU trace (maps, the whole field) against the FP64 model of the same
-    rays (the oracle traces the same Philox draws with double geometry and double sums):
-    per-pixel relative error over the pixels the FP64 map lights (median, p99, max, share
-    above 1e-5) and over the pixels above 1 % of the peak, the total-power error, the error
-    of the accumulation alone (the GPU's FP32 map against its own exact integer sums), and
-    the share of rays whose FP32 and FP64 receiver pixel differ on one block of heliostats
-    (GPU records against FP64 records)."""
+def flux_error_vs_fp64(scene, maps, osc, f64: dict, nthreads: int, block: int = 2000) -> dict:
+    """FP32 flux map of a GPU trace (maps, the whole field) against the FP64 model of the same
+    rays (f64: the oracle's trace of the same Philox draws with double geometry and double
+    sums): per-pixel relative error over the pixels the FP64 map lights (median, p99, max,
+    share above 1e-5) and over the pixels above 1 % of the peak, the total-power error, the
+    error of the accumulation alone (the FP32 map against its own exact integer sums), and
+    the share of rays whose receiver pixel, shading and blocking heliostat differ from the
+    FP64 model's on one block of heliostats (GPU records against FP64 records)."""
     import numpy as np
     import torch
 
@@ -138,9 +139,6 @@ def flux_error_vs_fp64(scene, maps, osc, nthreads: int, block: int = 2000) -> di
     scale = float(np.float32(scene.info["flux_scale"]))
     flux32 = maps.flux.cpu().numpy().astype(np.float64).ravel()
     energy = maps.energy.cpu().numpy().view(np.uint64).ravel()
-    t = time.perf_counter()
-    f64 = osc.trace_f64(nthreads=nthreads)
-    secs = time.perf_counter() - t
     flux64 = f64["energy"].ravel() * scale
     lit = flux64 > 0
     rel = np.abs(flux32[lit] - flux64[lit]) / flux64[lit]
@@ -168,8 +166,95 @@ def flux_error_vs_fp64(scene, maps, osc, nthreads: int, block: int = 2000) -> di
         "block_flip_rate": float((rg["block"] != r64["block"]).sum() / nr),
         "flip_sample": f"heliostats [{h0}, {h1}), {nr} rays",
         "per_pixel_1e-5_met": bool((rel > 1e-5).sum() == 0),
-        "fp64_seconds": secs,
     }
+
+
+def time_trace(scene, maps, steps: int, warmup: int) -> dict:
+    """Whole steps (clear maps, trace every heliostat, flux scaling and smoothing) timed
+    between synchronisations, and the trace launch (with its decode) by HIP events on the
+    launch stream."""
+    import numpy as np
+    import torch
+
+    H, R = scene.info["n_helio"], scene.info["total_rays"]
+    stream = torch.cuda.current_stream()
+
+    def step(ev0=None, ev1=None):
+        maps.counts.zero_()
+        maps.energy.zero_()
+        if ev0 is not None:
+            ev0.record(stream)
+        scene.trace_heliostats(0, H, maps)
+        if ev1 is not None:
+            ev1.record(stream)
+        scene.finalize(maps)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        step(e0, e1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"ms_per_step": 1e3 * dt / steps, "trace_ms": float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs])),
+            "rays_s": R / (dt / steps), "hits": int(maps.counts.sum().item())}
+
+
+def copy_probe(mib: int):
+    """An elementwise copy reading and writing mib MiB, three times: the reference kernel that
+    calibrates the HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    xs = torch.rand(n, device="cuda")
+    ys = torch.empty_like(xs)
+    for _ in range(3):
+        torch.mul(xs, 1.0, out=ys)
+    torch.cuda.synchronize()
+    del xs, ys
+    print(f"# copy probe: 3 x elementwise copy of {mib} MiB", file=sys.stderr, flush=True)
+
+
+def roofline(run: dict, workload: str, R: int, ops_ray: float, S: float, T: float) -> dict:
+    """VALU roofline of a run's trace kernel: the cost model's lane-ops per ray times rays
+    over the kernel time, against the VALU peak of its arithmetic (FP32: two 64-lane
+    instructions per CU clock; FP64: one), with the kernel's counters from its committed
+    rocprof run (VALU instructions per CU busy cycle against the issue probe of the same
+    arithmetic) and its HBM bytes per launch."""
+    fp64 = run["kernel"].startswith("trace_window64") or run["kernel"].startswith("trace64")
+    peak = P_VALU64 if fp64 else P_VALU
+    achieved = R * ops_ray / (run["trace_ms"] * 1e-3)
+    counters = None
+    vp = os.path.join(ROOT, "profiles", "valu_probe_kernel.json")
+    if os.path.exists(vp):
+        with open(vp) as f:
+            vj = json.load(f)
+        tw = vj.get(f"{run['kernel']}_{workload}")
+        probe = vj["probes"].get("probe_fma_f64" if fp64 else "probe_fma")
+        if tw and probe:
+            counters = {"valu_per_cu_busy_cycle": tw["valu_per_cu_busy_cycle"],
+                        "probe_valu_per_cu_busy_cycle": probe["valu_per_cu_busy_cycle"],
+                        "source": "profiles/valu_probe_kernel.json"}
+            if fp64:
+                f64i = sum(tw.get(k, 0.0) for k in ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+                                                    "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64"))
+                counters["fp64_share"] = f64i / tw["SQ_INSTS_VALU"]
+                counters["fp64_util"] = f64i * 64 / (tw["avg_ns"] * 1e-9) / P_VALU64
+            else:
+                counters["valu_util"] = tw["SQ_INSTS_VALU"] * 64 / (tw["avg_ns"] * 1e-9) / P_VALU
+                counters["of_probe"] = tw["valu_per_cu_busy_cycle"] / probe["valu_per_cu_busy_cycle"]
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"{run['kernel']}_{workload}.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            pj = json.load(f)
+        if pj.get("rays") == R:
+            traffic = pj["hbm_bytes_per_launch"]
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
+            "frac": achieved / peak, "traffic": traffic, "ops_per_ray": ops_ray, "dda_steps_per_ray": S,
+            "tests_per_ray": T, "arithmetic": "fp64" if fp64 else "fp32", "counters": counters}
 
 
 def main() -> int:
@@ -188,9 +273,10 @@ def main() -> int:
                     help="CPU baseline traces one block of 40 heliostats in every n "
                          "(default: the whole field, one block in 5 for large)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"],
-                    help="ray geometry: fp32 (the reference's, bit-exact against its host restatement) "
-                         "or fp64 (the FP64 model: per-pixel flux within 1e-5 of FP64)")
+    ap.add_argument("--precision", default="both", choices=["both", "fp32", "fp64"],
+                    help="ray geometry: fp32 (the reference's, bit-exact against its host restatement), "
+                         "fp64 (the FP64 model: per-pixel flux within 1e-5 of FP64), or both (default: "
+                         "the value is the fp64 run, the metric's accuracy; fp32 under modes)")
     ap.add_argument("--valu-probe", type=int, default=0, metavar="MI",
                     help="first run the VALU issue probes (om_math_eval 12..14) over MI Mi threads: "
                          "kernels with a known VALU instruction count that calibrate the VALU peak "
@@ -235,64 +321,38 @@ def main() -> int:
         desc, prm = F.config_small()
         wl = ("small: 100 heliostats 2x2 m, pillbox 4.65 mrad, slope 2 mrad, 16x16 Poisson "
               "x 64 rays, 64^3 grid, 128x128 map")
-    if args.precision == "fp64":
-        prm = dict(prm, precision=om.PREC_FP64)
-    t0 = time.time()
-    scene = om.SolarScene(desc, prm)
+    precisions = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
+    head = "fp64" if "fp64" in precisions else "fp32"
+    runs = {}
+    probe = None
+    for pr in precisions:
+        p = dict(prm, precision=om.PREC_FP64) if pr == "fp64" else prm
+        t0 = time.time()
+        scene = om.SolarScene(desc, p)
+        info = scene.info
+        H, R = info["n_helio"], info["total_rays"]
+        print(f"# {pr} scene built in {time.time() - t0:.1f}s: {H} heliostats, {R} rays, "
+              f"grid {info['grid_num']} nnz {info['grid_nnz']}", file=sys.stderr, flush=True)
+        maps = scene.new_maps(smooth=True)
+        # accumulation autotuner: per-tile packed atomics or the LDS window (identical maps)
+        acc = scene.autotune_accumulate(maps, reps=3)
+        kernel = ("trace_window" if acc["accumulate"] == "window" else "trace") + ("64" if pr == "fp64" else "") + "_kernel"
+        print(f"# {pr} accumulation autotune: packed {acc['packed_ms']:.3f} ms, window {acc['window_ms']:.3f} ms "
+              f"-> {acc['accumulate']}", file=sys.stderr, flush=True)
+        if probe is None and args.valu_probe > 0:
+            probe = valu_probe(args.valu_probe)
+        if pr == precisions[0] and args.copy_probe > 0:
+            copy_probe(args.copy_probe)
+        t = time_trace(scene, maps, args.steps, args.warmup)
+        print(f"# {pr}: {args.steps} steps, {t['ms_per_step']:.3f} ms/step, trace kernel {t['trace_ms']:.3f} ms, "
+              f"{t['rays_s']:.4g} rays/s, receiver hits {t['hits']}", file=sys.stderr, flush=True)
+        runs[pr] = dict(scene=scene, maps=maps, acc=acc, kernel=kernel, **t)
+    scene = runs[head]["scene"]
     info = scene.info
     H, R = info["n_helio"], info["total_rays"]
-    print(f"# scene built in {time.time() - t0:.1f}s: {H} heliostats, {R} rays, "
-          f"grid {info['grid_num']} nnz {info['grid_nnz']}", file=sys.stderr, flush=True)
-    maps = scene.new_maps(smooth=True)
-    stream = torch.cuda.current_stream()
-    # accumulation autotuner: per-tile packed atomics or the LDS window (identical maps)
-    acc = scene.autotune_accumulate(maps, reps=3)
-    kernel = "trace_window_kernel" if acc["accumulate"] == "window" else "trace_kernel"
-    if args.precision == "fp64":
-        kernel = "trace64_kernel"
-    print(f"# accumulation autotune: packed {acc['packed_ms']:.3f} ms, window {acc['window_ms']:.3f} ms "
-          f"-> {acc['accumulate']}", file=sys.stderr, flush=True)
-    probe = valu_probe(args.valu_probe) if args.valu_probe > 0 else None
-    if args.copy_probe > 0:
-        n = args.copy_probe * (1 << 20) // 4
-        xs = torch.rand(n, device="cuda")
-        ys = torch.empty_like(xs)
-        for _ in range(3):
-            torch.mul(xs, 1.0, out=ys)
-        torch.cuda.synchronize()
-        del xs, ys
-        print(f"# copy probe: 3 x elementwise copy of {args.copy_probe} MiB", file=sys.stderr,
-              flush=True)
 
-    def step(ev0=None, ev1=None):
-        maps.counts.zero_()
-        maps.energy.zero_()
-        if ev0 is not None:
-            ev0.record(stream)
-        scene.trace_heliostats(0, H, maps)
-        if ev1 is not None:
-            ev1.record(stream)
-        scene.finalize(maps)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for e0, e1 in evs:
-        step(e0, e1)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = 1e3 * dt / args.steps
-    trace_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
-    rays_s = R / (dt / args.steps)
-    hits = int(maps.counts.sum().item())
-    print(f"# {args.steps} steps: {ms_step:.3f} ms/step, trace kernel {trace_ms:.3f} ms, "
-          f"{rays_s:.4g} rays/s, receiver hits {hits}", file=sys.stderr, flush=True)
-
-    # oracle sample: step/test counts for the cost model and the CPU baseline
+    # oracle sample: step/test counts for the cost model and the CPU baseline (the FP32
+    # restatement of the reference's tracer)
     import oracle
     osc = oracle.OracleScene(desc, prm)
     every = args.cpu_every if args.cpu_every > 0 else (5 if args.workload == "large" else 1)
@@ -315,63 +375,46 @@ def main() -> int:
     cpu_rays = st_tot["rays"]
     nsample = sum(h1 - h0 for h0, h1 in blocks)
     ops_ray, S, T = model_ops(st_tot)
-    flux_err = None
+    # every run's flux against the FP64 model of the same rays (one FP64 trace of the field)
+    errs, fp64_model = {}, None
     if args.workload in ("mid", "small", "mid_grids") and not args.no_cpu:
-        flux_err = flux_error_vs_fp64(scene, maps, osc, nthreads)
-        print(f"# FP32 flux vs FP64 (whole field): per-pixel median {flux_err['median']:.2g}, p99 "
-              f"{flux_err['p99']:.2g}, {100 * flux_err['share_above_1e-5']:.1f} % of lit pixels above 1e-5, "
-              f"total power {flux_err['total_power']:.2g}, accumulation alone {flux_err['accumulation_only_max']:.2g}, "
-              f"pixel flips {flux_err['pixel_flip_rate']:.2g} of rays ({flux_err['fp64_seconds']:.1f} s)",
-              file=sys.stderr, flush=True)
-    achieved = R * ops_ray / (trace_ms * 1e-3) / 1e12
-    peak = P_VALU / 1e12
-    # counter-based VALU utilisation of the trace kernel from its committed rocprof run:
-    # SQ_INSTS_VALU x 64 lanes / kernel time against the same peak, and VALU instructions
-    # per CU busy cycle against what the VALU issue probe sustains
-    valu_util = None
-    vp = os.path.join(ROOT, "profiles", "valu_probe_kernel.json")
-    if os.path.exists(vp):
-        with open(vp) as f:
-            vj = json.load(f)
-        tw = vj.get(f"{kernel}_{args.workload}")
-        if tw:
-            valu_util = {"valu_util": tw["SQ_INSTS_VALU"] * 64 / (tw["avg_ns"] * 1e-9) / P_VALU,
-                         "valu_per_cu_busy_cycle": tw["valu_per_cu_busy_cycle"],
-                         "probe_valu_per_cu_busy_cycle": vj["probes"]["probe_fma"]["valu_per_cu_busy_cycle"],
-                         "of_probe": tw["valu_per_cu_busy_cycle"] / vj["probes"]["probe_fma"]["valu_per_cu_busy_cycle"],
-                         "source": "profiles/valu_probe_kernel.json"}
-    # HBM bytes per trace launch from the committed rocprof FETCH_SIZE / WRITE_SIZE runs
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", f"{kernel}_{args.workload}.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            pj = json.load(f)
-        if pj.get("rays") == R:
-            traffic = pj["hbm_bytes_per_launch"]
-
+        t2 = time.perf_counter()
+        f64 = osc.trace_f64(nthreads=nthreads)
+        fp64_model = {"value": R / (time.perf_counter() - t2), "unit": "rays/s", "cores": nthreads,
+                      "seconds": time.perf_counter() - t2,
+                      "sample": f"the whole field ({R} rays); oracle FP64 model, OpenMP"}
+        for pr, run in runs.items():
+            errs[pr] = flux_error_vs_fp64(run["scene"], run["maps"], osc, f64, nthreads)
+            e = errs[pr]
+            print(f"# {pr} flux vs FP64 (whole field): per-pixel median {e['median']:.2g}, p99 {e['p99']:.2g}, "
+                  f"{100 * e['share_above_1e-5']:.1f} % of lit pixels above 1e-5, total power {e['total_power']:.2g}, "
+                  f"pixel flips {e['pixel_flip_rate']:.2g} of rays", file=sys.stderr, flush=True)
+    rooflines = {pr: roofline(run, args.workload, R, ops_ray, S, T) for pr, run in runs.items()}
+    hr = runs[head]
     out = {
         "metric": METRIC,
-        "value": rays_s,
+        "value": hr["rays_s"],
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": hr["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "dtype": "f64" if head == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": wl, "precision": args.precision, "heliostats": H, "rays": R,
+        "config": {"workload": wl, "precision": head, "heliostats": H, "rays": R,
                    "grid": list(info["grid_num"]),
                    "map": [info["res_y"], info["res_x"]], "parallelism": "single GPU",
-                   "trace_kernel_ms": trace_ms, "receiver_hits": hits, "kernel": kernel,
-                   "accumulate": acc},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "ops_per_ray": ops_ray, "dda_steps_per_ray": S, "tests_per_ray": T,
-                     "counters": valu_util},
-        "flux_err_vs_fp64": flux_err,
+                   "trace_kernel_ms": hr["trace_ms"], "receiver_hits": hr["hits"], "kernel": hr["kernel"],
+                   "accumulate": hr["acc"],
+                   "geometry": ("FP64: sampling, frames, walks, intersections and receiver in double "
+                                "(om_params.precision = OM_PREC_FP64), energies in exact fixed point, "
+                                "FP32 flux map" if head == "fp64" else
+                                "FP32: the reference's ray geometry, bit-exact against its host restatement")},
+        "roofline": rooflines[head],
+        "flux_err_vs_fp64": errs.get(head),
         "valu_probe": probe,
         "cpu_baseline": None if args.no_cpu else {
             "value": cpu_rays / cpu_t, "unit": "rays/s", "cores": nthreads, "kind": "port",
@@ -379,9 +422,17 @@ def main() -> int:
             "sample": (f"the whole field ({nsample} heliostats, {cpu_rays} rays)" if every == 1 else
                        f"{len(blocks)} blocks of {blk} neighbouring heliostats, one in every "
                        f"{every} of the same field ({nsample} heliostats, {cpu_rays} rays)")
-                      + "; oracle FP32 tracer with the same early-exit walk, OpenMP",
-            "seconds": cpu_t},
+                      + "; oracle FP32 tracer (the restatement of the reference's) with the same "
+                        "early-exit walk, OpenMP",
+            "seconds": cpu_t, "fp64_model": fp64_model},
     }
+    if head == "fp64" and "fp32" in runs:
+        r32 = runs["fp32"]
+        out["modes"] = {"fp32_reference": {
+            "value": r32["rays_s"], "unit": "rays/s", "ms_per_step": r32["ms_per_step"],
+            "trace_kernel_ms": r32["trace_ms"], "kernel": r32["kernel"], "accumulate": r32["acc"],
+            "geometry": "FP32: the reference's ray geometry, bit-exact against its host restatement",
+            "flux_err_vs_fp64": errs.get("fp32"), "roofline": rooflines["fp32"]}}
     if args.workload == "mid" and not args.no_modes:
         # the batched and differentiable forms of the metric and the annual runs, on their own
         # workloads (a few timed steps each)
@@ -393,7 +444,7 @@ def main() -> int:
         sub_a.steps, sub_a.warmup = 1, 1
         an = bench_annual(sub_a)
         fa = bench_facet_annual(sub_a)
-        out["modes"] = {
+        out.setdefault("modes", {}).update({
             "batched": {"value": b["value"], "unit": "rays/s", "workload": b["config"]["workload"],
                         **{k: b["config"][k] for k in ("scenes", "rays", "batch_ms", "plain_ms",
                                                        "plain_rays_per_s", "batched_vs_plain",
@@ -408,7 +459,7 @@ def main() -> int:
                         "rays_per_s": d["value"], "workload": d["config"]["workload"],
                         **{k: d["config"][k] for k in ("fwd_bwd_ms", "forward_ms", "backward_ms",
                                                        "plain_trace_ms", "fwd_bwd_vs_plain")}},
-        }
+        })
     print(json.dumps(out), flush=True)
     return 0
 
