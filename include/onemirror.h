@@ -357,13 +357,20 @@ typedef struct om_batch om_batch;
 
 typedef struct {
     int workgroup;             /* threads per workgroup: 128, 256 or 512 (0 = 256), or
-                                  OM_BATCH_PER_SCENE                                     */
+                                  OM_BATCH_PER_SCENE or OM_BATCH_WINDOW                  */
     int rays_per_thread;       /* 1..8, rays per work item = workgroup x this (0 = auto)  */
 } om_launch;
 /* om_launch.workgroup: no persistent batch launch; each scene is traced as om_trace_plain
  * traces it with OM_ACC_WINDOW (the LDS-window kernel of 1024 threads), one after another
  * in the batch's graph.  Large scenes run faster this way; the maps are the same. */
 enum { OM_BATCH_PER_SCENE = 1024 };
+/* om_launch.workgroup: one persistent launch of 1024-thread workgroups, one per CU, that
+ * keeps the central block of the current scene's packed map in LDS (as OM_ACC_WINDOW does
+ * for one scene).  Waves take chunks of at most 2048 rays of one heliostat from 8 per-XCD
+ * queues in scene order; a workgroup adds its window to the scene's maps in device memory
+ * when its waves move on to another scene (or grid).  rays_per_thread is ignored.  The maps
+ * are the same. */
+enum { OM_BATCH_WINDOW = 2048 };
 
 /* Batch over scenes[0..n) with outputs outs[0..n) (counts and energy required, flux and
  * flux_smooth optional; all overwritten by each run).  The scenes and outputs must outlive
@@ -380,7 +387,7 @@ om_status om_trace_batched(om_ctx *ctx, om_scene *const *scenes, int n, const om
                            const om_launch *launch, void *stream);
 
 /* Autotuner of the batch launch: times every workgroup size in {128, 256, 512} x rays per
- * thread in {2, 4, 8}, and OM_BATCH_PER_SCENE, on the given batch (best of reps graph
+ * thread in {2, 4, 8}, OM_BATCH_WINDOW and OM_BATCH_PER_SCENE, on the given batch (best of reps graph
  * replays after a warm-up) and returns the fastest with the whole table.  Results do not
  * depend on these parameters. */
 typedef struct {
@@ -388,7 +395,7 @@ typedef struct {
     float     best_ms;
     double    rays_per_s;
     int       n_trials;
-    struct { int workgroup, rays_per_thread; float ms; } trials[10];
+    struct { int workgroup, rays_per_thread; float ms; } trials[12];
 } om_tune;
 om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs, int reps,
                       om_tune *result);

@@ -144,7 +144,7 @@ class TuneTrial(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("best", Launch), ("best_ms", ctypes.c_float), ("rays_per_s", ctypes.c_double),
-                ("n_trials", ctypes.c_int), ("trials", TuneTrial * 10)]
+                ("n_trials", ctypes.c_int), ("trials", TuneTrial * 12)]
 
 
 _EXPORTS = [
@@ -734,11 +734,14 @@ def _batch_arrays(scenes, maps):
 
 
 BATCH_PER_SCENE = 1024  # Batch workgroup: the window trace of each scene in turn (OM_BATCH_PER_SCENE)
+BATCH_WINDOW = 2048     # Batch workgroup: one persistent launch with LDS windows (OM_BATCH_WINDOW)
 
 
 class Batch:
-    """Many scenes traced by one persistent launch, or with workgroup=BATCH_PER_SCENE by the
-    window trace of each scene in turn, replayed from a hipGraph (om_batch_*).
+    """Many scenes traced by one persistent launch (workgroup 128/256/512, or BATCH_WINDOW
+    for the form that keeps each scene's map window in LDS), or with
+    workgroup=BATCH_PER_SCENE by the window trace of each scene in turn, replayed from a
+    hipGraph (om_batch_*).
 
     ``maps[i]`` receives the maps of ``scenes[i]``, equal to what ``scenes[i].trace_plain``
     gives; they are overwritten by every ``run``."""

@@ -45,14 +45,15 @@ struct om_batch {
     bool poisson = false;
     int wgs = 256, rpt = 8;
     bool per_scene = false;                     // OM_BATCH_PER_SCENE: window traces scene by scene
+    bool window = false;                        // OM_BATCH_WINDOW: persistent launch with LDS windows
     int n_items = 0, nblocks = 0, max_npx = 0;
     long long rays = 0;
     std::vector<om_scene *> scenes;
     std::vector<om_flux> outs;
     std::vector<float *> raw;                   // per scene, for smoothing
     om::TraceArgs *d_table = nullptr;
-    int4 *d_items = nullptr;
-    unsigned int *d_queue = nullptr;
+    int4 *d_items = nullptr;                    // work items, or OM_BATCH_WINDOW's chunks
+    unsigned int *d_queue = nullptr;            // one counter, or OM_BATCH_WINDOW's 8 queues
     om::ReduceSlot *d_slots = nullptr;
     unsigned long long *d_parts = nullptr;      // copies + carry + wrap of every scene
     float *d_raw = nullptr;
@@ -107,8 +108,13 @@ om_status batch_enqueue(om_batch *b, hipStream_t st)
         }
         return OM_OK;
     }
-    if ((rc = om::launch_fill_u32(b->d_queue, 1, st)) != OM_OK) return rc;
-    if (b->n_items > 0 &&
+    if ((rc = om::launch_fill_u32(b->d_queue, b->window ? 8 : 1, st)) != OM_OK) return rc;
+    if (b->window) {
+        if (b->n_items > 0 &&
+            (rc = om::launch_trace_batch_window(b->sunshape, b->poisson, b->d_table, b->d_items, b->n_items,
+                                                b->d_queue, b->nblocks, st)) != OM_OK)
+            return rc;
+    } else if (b->n_items > 0 &&
         (rc = om::launch_trace_batch(b->sunshape, b->poisson, b->wgs, b->d_table, b->d_items, b->n_items,
                                      b->d_queue, b->nblocks, st)) != OM_OK)
         return rc;
@@ -136,8 +142,9 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     b->n = n;
     b->wgs = (lp && lp->workgroup) ? lp->workgroup : 256;
     b->rpt = (lp && lp->rays_per_thread) ? lp->rays_per_thread : 0;
-    if (b->wgs != 128 && b->wgs != 256 && b->wgs != 512 && b->wgs != OM_BATCH_PER_SCENE) {
-        om::set_error("batch workgroup size must be 128, 256, 512 or OM_BATCH_PER_SCENE");
+    if (b->wgs != 128 && b->wgs != 256 && b->wgs != 512 && b->wgs != OM_BATCH_PER_SCENE &&
+        b->wgs != OM_BATCH_WINDOW) {
+        om::set_error("batch workgroup size must be 128, 256, 512, OM_BATCH_PER_SCENE or OM_BATCH_WINDOW");
         delete b;
         return OM_ERR_INVALID;
     }
@@ -178,7 +185,10 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     }
     b->sunshape = scenes[0]->prm.sunshape;
     b->poisson = scenes[0]->prm.micro_mode == OM_MICRO_POISSON;
-    if (b->rpt == 0) {
+    b->window = b->wgs == OM_BATCH_WINDOW;
+    if (b->window) {
+        b->rpt = 0;
+    } else if (b->rpt == 0) {
         // enough items to fill 256 CUs many times over
         b->rpt = om::RPT_LIMIT;
         while (b->rpt > 1 && b->rays / ((long long)b->wgs * b->rpt) < 16384) b->rpt /= 2;
@@ -214,9 +224,11 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     }
     // work items: (table entry, heliostat, first ray) blocks of wgs * rpt rays, scene by
     // scene; a table entry is one grid of one scene (its heliostats walk that grid)
+    // OM_BATCH_WINDOW: chunks (entry, heliostat, first ray, rays) of at most the window
+    // kernels' chunk size, in the same order
     std::vector<int4> items;
     std::vector<int> entry_scene, entry_grid;
-    const int block = b->wgs * b->rpt;
+    const int block = b->window ? om::window_chunk_rays() : b->wgs * b->rpt;
     for (int i = 0; i < n; ++i) {
         const om_scene *s = scenes[i];
         for (size_t g = 0; g < s->grids.size(); ++g) {
@@ -225,7 +237,8 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
             entry_grid.push_back((int)g);
             for (int h = s->grids[g].h0; h < s->grids[g].h1; ++h) {
                 const long long nr = (long long)s->mcount[h] * s->prm.rays_per_sample;
-                for (long long r = 0; r < nr; r += block) items.push_back(make_int4(e, h, (int)r, 0));
+                for (long long r = 0; r < nr; r += block)
+                    items.push_back(make_int4(e, h, (int)r, (int)std::min<long long>(block, nr - r)));
             }
         }
     }
@@ -245,7 +258,7 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     if ((rc = dalloc(b->d_raw, (size_t)sum_px)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_table, (size_t)n_entries)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_items, items.size())) != OM_OK) return fail(rc);
-    if ((rc = dalloc(b->d_queue, 1)) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_queue, 8)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_slots, (size_t)n)) != OM_OK) return fail(rc);
     std::vector<om::TraceArgs> table(n_entries);
     std::vector<om::ReduceSlot> slots(n);
@@ -267,6 +280,7 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
         a.copy_stride = npx;
         a.px_stride = 1;
         a.pack_shift = s->prm.pack_shift > 0 ? s->prm.pack_shift : 44;
+        om::window_rect(s->res_x, s->res_y, a.win_x0, a.win_y0, a.win_w, a.win_h);
         for (int e = 0; e < n_entries; ++e) {
             if (entry_scene[e] != i) continue;
             // the entry's grid; maps and launch shape as above
@@ -302,7 +316,10 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     // persistent workgroups: as many as are resident at once
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return fail(OM_ERR_HIP);
-    int per_cu = om::trace_batch_blocks_per_cu(b->sunshape, b->poisson, b->wgs);
+    int per_cu = b->window ? om::trace_batch_window_blocks_per_cu(b->sunshape, b->poisson)
+                           : om::trace_batch_blocks_per_cu(b->sunshape, b->poisson, b->wgs);
+    if (b->window && per_cu < 1)
+        return fail((om::set_error("trace_batch_window_kernel cannot be resident on this device"), OM_ERR_UNSUPPORTED));
     if (per_cu < 1) per_cu = 1;
     b->nblocks = std::max(1, std::min(b->n_items, per_cu * prop.multiProcessorCount));
     *out = b;
@@ -385,7 +402,7 @@ extern "C" om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, co
     if (!ctx || !result) return OM_ERR_INVALID;
     std::memset(result, 0, sizeof(*result));
     if (reps <= 0) reps = 2;
-    static const int wgs_list[] = {128, 256, 512, OM_BATCH_PER_SCENE};
+    static const int wgs_list[] = {128, 256, 512, OM_BATCH_WINDOW, OM_BATCH_PER_SCENE};
     static const std::vector<int> rpt_list = {2, 4, 8}, rpt_none = {0};
     OM_HIP(hipSetDevice(ctx->device));
     hipStream_t st;
@@ -396,7 +413,7 @@ extern "C" om_status om_autotune(om_ctx *ctx, om_scene *const *scenes, int n, co
     om_status rc = OM_OK;
     float best = 0.0f;
     for (int w : wgs_list)
-        for (int r : (w == OM_BATCH_PER_SCENE ? rpt_none : rpt_list)) {
+        for (int r : (w == OM_BATCH_PER_SCENE || w == OM_BATCH_WINDOW ? rpt_none : rpt_list)) {
             om_launch lp{w, r};
             om_batch *b = nullptr;
             if ((rc = batch_build(ctx, scenes, n, outs, &lp, &b)) != OM_OK) {

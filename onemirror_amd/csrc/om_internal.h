@@ -285,6 +285,15 @@ om_status launch_trace_batch(int sunshape, bool poisson, int wgs, const TraceArg
                              int n_items, unsigned int *queue, int nblocks, hipStream_t st);
 int trace_batch_blocks_per_cu(int sunshape, bool poisson, int wgs);
 om_status launch_reduce_batch(const ReduceSlot *slots, int nslot, int max_npx, hipStream_t st);
+// batched mode with LDS windows: chunks (entry, heliostat, first ray, rays) in entry order;
+// nwg persistent workgroups; queue: 8 counters, cleared
+om_status launch_trace_batch_window(int sunshape, bool poisson, const TraceArgs *table, const int4 *chunks,
+                                    int n_chunks, unsigned int *queue, int nwg, hipStream_t st);
+int trace_batch_window_blocks_per_cu(int sunshape, bool poisson);
+// the window of a res_x x res_y map the LDS holds: the whole map when it fits, else the
+// largest near-square block about the centre; the rays of one chunk of the window kernels
+void window_rect(int rx, int ry, int &x0, int &y0, int &w, int &h);
+int window_chunk_rays();
 om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st);
 constexpr int RPT_LIMIT = 8; // rays per thread a trace workgroup holds hit slots for
 // trace heliostats [h0, h1) into out (cleared first when clear is set), with the workspace ws

@@ -400,6 +400,201 @@ __global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__res
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Batched mode with LDS windows (OM_BATCH_WINDOW)
+// ---------------------------------------------------------------------------------------
+// One persistent workgroup of WIN_WGS threads per CU.  Its waves take chunks (table entry,
+// heliostat, first ray, rays) of at most WIN_CHUNK rays from 8 per-XCD queues over
+// contiguous eighths of the chunk list, which is in entry order (an entry is one grid of one
+// scene), so the workgroups of an XCD work through a few scenes at a time.  The workgroup
+// keeps the central block of one entry's packed map in LDS.  A wave traces the chunks it
+// takes while they belong to that entry; holding a chunk of another entry, it waits at the
+// workgroup barrier until every wave does (or has no chunk left).  There the entry of the
+// smallest held chunk becomes the window's: if it differs, the window's nonzero words are
+// added to the old entry's per-XCD copies (one returning atomic each, carries and wraps
+// counted as for single hits, which the decode then applies alike) and the window is
+// cleared.  Barriers come only at entry changes, so waves do not wait for one another
+// between chunks.  Hits outside the window go to the wave's LDS buffer and reach the copies
+// BW_OUT / 2 or more at a time, so a wave waits for memory-side atomics once per flush, not
+// per hit.
+constexpr int BW_WAVES = WIN_WGS / 64;
+constexpr int BW_OUT = 128; // out-of-window buffer entries per wave
+constexpr int BW_NONE = 0x7fffffff;
+
+// One hit or one window word w = (n << E) + e into the entry's copy of pixel px
+__device__ __forceinline__ void copy_add(const TraceArgs &a, unsigned long long *part, int px, unsigned long long w,
+                                         unsigned long long &old)
+{
+    old = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void copy_check(const TraceArgs &a, int px, unsigned long long w, unsigned long long old)
+{
+    const unsigned long long lowmask = (1ull << a.pack_shift) - 1ull;
+    if ((old & lowmask) + (w & lowmask) > lowmask) atomicAdd(a.carry + px, 1u);
+    if (old + w < old) atomicAdd(a.wrap + px, 1u);
+}
+
+// the per-XCD copy of this workgroup in entry a's packed map
+__device__ __forceinline__ unsigned long long *copy_of(const TraceArgs &a)
+{
+    const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
+    return a.part + (size_t)copy * a.copy_stride;
+}
+
+// Adds the wave's buffered out-of-window hits (n of them, n <= BW_OUT) to the copies.
+__device__ __forceinline__ void flush_out(const TraceArgs &a, const unsigned long long *buf, int n, int lane)
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); // the lanes' buffer stores are done
+    unsigned long long *part = copy_of(a);
+    const unsigned long long one = 1ull << a.pack_shift;
+    unsigned long long e[BW_OUT / 64], old[BW_OUT / 64];
+#pragma unroll
+    for (int b = 0; b < BW_OUT / 64; ++b) {
+        const int i = b * 64 + lane;
+        e[b] = i < n ? buf[i] : RING_EMPTY;
+        if (e[b] != RING_EMPTY) copy_add(a, part, (int)(e[b] >> 32), one + (e[b] & 0xffffffffull), old[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < BW_OUT / 64; ++b)
+        if (e[b] != RING_EMPTY) copy_check(a, (int)(e[b] >> 32), one + (e[b] & 0xffffffffull), old[b]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); // read before the buffer is refilled
+}
+
+// Adds the window's nonzero words to entry a's copies and clears the window (the whole
+// workgroup; barriers around it are the caller's).
+__device__ __forceinline__ void flush_window(const TraceArgs &a, unsigned long long *s_win)
+{
+    unsigned long long *part = copy_of(a);
+    const int wpx = a.win_w * a.win_h;
+    for (int i = threadIdx.x; i < wpx; i += WIN_WGS) {
+        const unsigned long long w = s_win[i];
+        if (w == 0ull) continue;
+        s_win[i] = 0ull;
+        const int row = i / a.win_w, col = i - row * a.win_w;
+        const int px = (a.win_y0 + row) * a.res_x + a.win_x0 + col;
+        unsigned long long old;
+        copy_add(a, part, px, w, old);
+        copy_check(a, px, w, old);
+    }
+}
+
+// nrays rays of heliostat ck.y from ray ck.z, traced by one wave: window hits as LDS atomics,
+// the others through the wave's buffer (nout entries, wave-uniform, flushed when the next
+// pass could overflow it; the caller flushes the rest)
+template <int SUN, bool POISSON>
+__device__ __forceinline__ void trace_chunk_batch(const TraceArgs &a, int4 ck, const int lane,
+                                                  unsigned long long *s_win, unsigned long long *buf, int &nout)
+{
+    const HelioTile t = helio_tile(a, ck.y);
+    const int L = a.L;
+    const int nrays = ck.w;
+    const int nr = min(t.hr->mcount * L, ck.z + nrays);
+    int r = ck.z + lane;
+    int m = r / L;
+    int l = r - m * L;
+    const int dm = 64 / L, dl = 64 - (64 / L) * L;
+    RayStart rs{};
+    int m_rs = -1;
+    for (int k = 0; k < nrays; k += 64, r += 64) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= L) {
+                l -= L;
+                ++m;
+            }
+        }
+        bool out = false;
+        int px = 0;
+        uint32_t q = 0u;
+        asm volatile("" ::: "memory");
+        if (r < nr) {
+            if (m != m_rs) {
+                rs = ray_start<POISSON>(a, t, m);
+                m_rs = m;
+            }
+            const RayOut o = trace_from<SUN, true>(a, t, rs, m, l, a.buie_bins, a.buie_guide);
+            if (o.rc.x >= 0) {
+                px = o.rc.x * a.res_x + o.rc.y;
+                q = o.q;
+                const uint32_t wr = (uint32_t)(o.rc.x - a.win_y0), wc = (uint32_t)(o.rc.y - a.win_x0);
+                if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w)
+                    packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+                else
+                    out = true;
+            }
+        }
+        const unsigned long long mk = __ballot(out);
+        if (out) {
+            const uint32_t j = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+            buf[nout + (int)j] = ((unsigned long long)px << 32) | q;
+        }
+        nout += __popcll(mk);
+        if (nout > BW_OUT - 64) { // the next pass could overflow the buffer
+            flush_out(a, buf, nout, lane);
+            nout = 0;
+        }
+    }
+}
+
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WIN_WGS) void trace_batch_window_kernel(const TraceArgs *__restrict__ table,
+                                                                     const int4 *__restrict__ chunks, int n_chunks,
+                                                                     unsigned int *queue)
+{
+    __shared__ unsigned long long s_win[WIN_PX_MAX];
+    __shared__ unsigned long long s_out[BW_WAVES][BW_OUT];
+    __shared__ int s_held[BW_WAVES];
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    for (int i = threadIdx.x; i < WIN_PX_MAX; i += WIN_WGS) s_win[i] = 0ull;
+    const int q = n_chunks >> 3, rmd = n_chunks & 7;
+    const uint32_t xcd = xcc_id() & 7u;
+    int qj = 0; // queues this wave has found empty: its own XCD's first, then the others in turn
+    // the next chunk for this wave, or -1 when every queue is empty
+    auto pop = [&]() -> int {
+        while (qj < 8) {
+            const int x = (int)((xcd + (uint32_t)qj) & 7u);
+            const int cnt = q + (x < rmd ? 1 : 0);
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(queue + x, 1u);
+            t = __builtin_amdgcn_readfirstlane(t); // the whole wave is active here
+            if (t < (uint32_t)cnt) return x * q + min(x, rmd) + (int)t;
+            ++qj;
+        }
+        return -1;
+    };
+    int held = pop();
+    int he = held >= 0 ? __builtin_amdgcn_readfirstlane(chunks[held].x) : BW_NONE; // entry of the held chunk
+    int cur = -1;  // entry whose window the LDS holds (uniform over the workgroup)
+    int nout = 0;  // entries in this wave's buffer
+    for (;;) {
+        if (lane == 0) s_held[wave] = he;
+        __syncthreads(); // every wave holds a chunk of another entry or none; the window is idle
+        int ne = BW_NONE;
+        for (int w = 0; w < BW_WAVES; ++w) ne = min(ne, s_held[w]);
+        ne = __builtin_amdgcn_readfirstlane(ne);
+        if (ne == BW_NONE) break;
+        if (ne != cur) {
+            if (cur >= 0) flush_window(table[cur], s_win);
+            cur = ne;
+        }
+        __syncthreads(); // the window is clear, s_held read, before hits and the next round
+        while (he == cur) {
+            const int4 c = chunks[held];
+            const int4 ck = make_int4(cur, __builtin_amdgcn_readfirstlane(c.y), __builtin_amdgcn_readfirstlane(c.z),
+                                      __builtin_amdgcn_readfirstlane(c.w));
+            trace_chunk_batch<SUN, POISSON>(table[cur], ck, lane, s_win, s_out[wave], nout);
+            held = pop();
+            he = held >= 0 ? __builtin_amdgcn_readfirstlane(chunks[held].x) : BW_NONE;
+        }
+        if (nout > 0) {
+            flush_out(table[cur], s_out[wave], nout, lane);
+            nout = 0;
+        }
+    }
+    if (cur >= 0) flush_window(table[cur], s_win);
+}
+
 // Batched decode: blockIdx.y selects the map; writes (not adds) counts and energy, the raw
 // float map for smoothing and the scaled flux, with the operations of reduce_packed_kernel
 // and finalize_kernel.
@@ -915,6 +1110,52 @@ int trace_batch_blocks_per_cu(int sunshape, bool poisson, int wgs)
     return poisson ? batch_occupancy_wgs<OM_SUN_PILLBOX, true>(wgs) : batch_occupancy_wgs<OM_SUN_PILLBOX, false>(wgs);
 }
 
+om_status launch_trace_batch_window(int sunshape, bool poisson, const TraceArgs *table, const int4 *chunks,
+                                    int n_chunks, unsigned int *queue, int nwg, hipStream_t st)
+{
+    const dim3 grid(nwg), block(WIN_WGS);
+    if (sunshape == OM_SUN_BUIE) {
+        if (poisson) hipLaunchKernelGGL((trace_batch_window_kernel<OM_SUN_BUIE, true>), grid, block, 0, st, table, chunks, n_chunks, queue);
+        else hipLaunchKernelGGL((trace_batch_window_kernel<OM_SUN_BUIE, false>), grid, block, 0, st, table, chunks, n_chunks, queue);
+    } else {
+        if (poisson) hipLaunchKernelGGL((trace_batch_window_kernel<OM_SUN_PILLBOX, true>), grid, block, 0, st, table, chunks, n_chunks, queue);
+        else hipLaunchKernelGGL((trace_batch_window_kernel<OM_SUN_PILLBOX, false>), grid, block, 0, st, table, chunks, n_chunks, queue);
+    }
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+int trace_batch_window_blocks_per_cu(int sunshape, bool poisson)
+{
+    int n = 0;
+    hipError_t e;
+    if (sunshape == OM_SUN_BUIE)
+        e = poisson ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_batch_window_kernel<OM_SUN_BUIE, true>, WIN_WGS, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_batch_window_kernel<OM_SUN_BUIE, false>, WIN_WGS, 0);
+    else
+        e = poisson ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_batch_window_kernel<OM_SUN_PILLBOX, true>, WIN_WGS, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_batch_window_kernel<OM_SUN_PILLBOX, false>, WIN_WGS, 0);
+    return e == hipSuccess ? n : 0;
+}
+
+void window_rect(int rx, int ry, int &x0, int &y0, int &w, int &h)
+{
+    if ((long long)rx * ry <= WIN_PX_MAX) {
+        w = rx;
+        h = ry;
+    } else {
+        int side = 1;
+        while ((side + 1) * (side + 1) <= WIN_PX_MAX) ++side;
+        w = std::min(rx, side);
+        h = std::min(ry, WIN_PX_MAX / w);
+        w = std::min(rx, WIN_PX_MAX / h);
+    }
+    x0 = (rx - w) / 2;
+    y0 = (ry - h) / 2;
+}
+
+int window_chunk_rays() { return WIN_CHUNK; }
+
 om_status launch_reduce_batch(const ReduceSlot *slots, int nslot, int max_npx, hipStream_t st)
 {
     hipLaunchKernelGGL(reduce_batch_kernel, dim3((max_npx + WG - 1) / WG, nslot), dim3(WG), 0, st, slots);
@@ -1019,20 +1260,8 @@ static om_status window_plan(om_scene *s, WindowRef &win, int &nwg)
         s->ctx->win_wgs = cus * per;
     }
     nwg = s->ctx->win_wgs;
-    const int rx = s->res_x, ry = s->res_y;
-    if ((long long)rx * ry <= WIN_PX_MAX) {
-        win.w = rx;
-        win.h = ry;
-    } else {
-        int side = 1;
-        while ((side + 1) * (side + 1) <= WIN_PX_MAX) ++side;
-        win.w = std::min(rx, side);
-        win.h = std::min(ry, WIN_PX_MAX / win.w);
-        win.w = std::min(rx, WIN_PX_MAX / win.h);
-    }
-    win.x0 = (rx - win.w) / 2;
-    win.y0 = (ry - win.h) / 2;
-    win.res_x = rx;
+    window_rect(s->res_x, s->res_y, win.x0, win.y0, win.w, win.h);
+    win.res_x = s->res_x;
     return OM_OK;
 }
 

@@ -1,5 +1,6 @@
-"""Batched mode on the GPU: one persistent launch over many scenes (or the window trace of
-each scene in turn, OM_BATCH_PER_SCENE), replayed from a hipGraph.
+"""Batched mode on the GPU: one persistent launch over many scenes (work items of 128-512
+threads, or OM_BATCH_WINDOW's 1024-thread workgroups with each scene's map window in LDS),
+or the window trace of each scene in turn (OM_BATCH_PER_SCENE), replayed from a hipGraph.
 
 Pin: every map of a batch equals the map the plain path gives for that scene alone (same
 Philox streams, integer accumulation), bit for bit, for every launch configuration and on
@@ -63,7 +64,7 @@ def pillbox_batch(om):
     return scenes, want
 
 
-@pytest.mark.parametrize("wg,rpt", [(0, 0), (128, 2), (512, 8), (256, 1), (1024, 0)])
+@pytest.mark.parametrize("wg,rpt", [(0, 0), (128, 2), (512, 8), (256, 1), (1024, 0), (2048, 0)])
 def test_batch_graph_replays_equal_plain(om, pillbox_batch, wg, rpt):
     import torch
     scenes, want = pillbox_batch
@@ -88,7 +89,7 @@ def test_trace_batched_one_shot(om, pillbox_batch):
         _same(m, want[i], f"scene {i}")
 
 
-@pytest.mark.parametrize("wg", [0, 1024])
+@pytest.mark.parametrize("wg", [0, 1024, 2048])
 def test_batch_buie_fields_equal_plain(om, wg):
     import torch
     d, p = F.config_mid()
@@ -109,10 +110,34 @@ def test_batch_buie_fields_equal_plain(om, wg):
             _same(m, want[i], f"buie scene {i} replay {rep}")
 
 
+def test_batch_window_two_grids_and_scene_changes(om):
+    """OM_BATCH_WINDOW over scenes of two grids each (a workgroup adds its window to the
+    maps when the next block belongs to another grid or scene) and over scenes whose maps
+    exceed the window (hits outside it go through each wave's buffer): maps equal the plain
+    ones on three replays."""
+    import torch
+    descs = []
+    d, p = F.config_two_grids()
+    for i, el in enumerate((50.0, 30.0, 65.0)):
+        descs.append((dict(d, sun_dir=F.sun_direction(el, 170.0 + 20 * i)),
+                      dict(p, scene_index=i, grid_select=om.GRIDS_ALL)))
+    scenes = [om.SolarScene(dd, pp) for dd, pp in descs]
+    assert all(s.info["n_grids"] == 2 for s in scenes)
+    want = [_plain(s) for s in scenes]
+    b = om.Batch(scenes, workgroup=om.BATCH_WINDOW)
+    for rep in range(3):
+        maps = b.run()
+        torch.cuda.synchronize()
+        for i, m in enumerate(maps):
+            _same(m, want[i], f"two-grid scene {i} replay {rep}")
+        assert float(torch.ones(1 << 20, device="cuda").sum()) == float(1 << 20)
+
+
 def test_autotune_launch_table(om, pillbox_batch):
     scenes, _ = pillbox_batch
     r = om.autotune_launch(scenes, reps=1)
-    assert len(r["trials"]) == 10 and (om.BATCH_PER_SCENE, 0) in [t[:2] for t in r["trials"]]
+    forms = [t[:2] for t in r["trials"]]
+    assert len(r["trials"]) == 11 and (om.BATCH_PER_SCENE, 0) in forms and (om.BATCH_WINDOW, 0) in forms
     assert all(ms > 0 for _, _, ms in r["trials"])
     assert (r["workgroup"], r["rays_per_thread"], r["ms"]) in r["trials"]
     assert r["rays_per_s"] > 0

@@ -4,7 +4,8 @@ azimuth 90-270 deg, 1e7-2e8 rays per scene, 256 x 256 maps; the role of the refe
 annual loop, main.cpp:56-116, which builds one scene per sun position).
 
 Pins:
-  * per scene, the persistent batch launch and the per-scene launch give the maps of a
+  * per scene, the persistent batch launches (work items, and LDS windows) and the
+    per-scene launch give the maps of a
     plain trace of that scene alone (counts, energy, flux, smoothed flux bit for bit);
   * per scene, three seeded heliostats traced through om_trace_records and
     om_trace_heliostats equal the oracle's records and maps of those heliostats;
@@ -52,7 +53,7 @@ def test_b_spans_the_heliostat_range(batch_b):
     assert min(rays) >= 0.9e7 and max(rays) <= 2.1e8, rays
 
 
-@pytest.mark.parametrize("wg", [0, 1024])
+@pytest.mark.parametrize("wg", [0, 1024, 2048])
 def test_b_batch_equals_plain(om, batch_b, wg):
     cfgs, scenes = batch_b
     want = [_host(s.trace_plain()) for s in scenes]
