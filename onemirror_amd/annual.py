@@ -17,7 +17,8 @@ from typing import Iterable, List, Optional, Sequence
 
 import numpy as np
 
-from . import BATCH_PER_SCENE, Batch, SolarScene, dni_hottel, facet_scene, lib, sun_direction, _f32p
+from . import (BATCH_PER_SCENE, BATCH_WINDOW, PREC_FP32, PREC_FP64, Batch, SolarScene, dni_hottel, facet_scene, lib,
+               sun_direction, _f32p)
 
 # main.cpp:34-44: the reference's site (Hangzhou) and year
 REFERENCE_SITE = {"latitude": 30.3, "longitude": 120.2, "timezone": 8.0, "elevation_m": 40.0}
@@ -107,9 +108,10 @@ class AnnualRun:
         """``mode``: "batch" traces the steps with the batched launch (hipGraph replays),
         "plain" with one om_trace_plain per step, in the accumulation mode of each scene
         (see SolarScene.autotune_accumulate); the maps are the same.  With ``workgroup`` 0
-        a batch of steps that each hold 1e7 rays or more (each fills the GPU by itself)
-        traces them one window trace after another (BATCH_PER_SCENE), smaller steps share
-        the persistent launch.  ``scenes`` (one per step) replaces the forks of ``scene``."""
+        a batch of steps is one persistent launch that keeps each step's map window in LDS
+        (BATCH_WINDOW), or, for FP64-geometry scenes, which that launch does not trace, one
+        window trace after another (BATCH_PER_SCENE).  ``scenes`` (one per step) replaces the
+        forks of ``scene``."""
         if mode not in ("batch", "plain"):
             raise ValueError("mode is 'batch' or 'plain'")
         self.scene = scene
@@ -129,8 +131,9 @@ class AnnualRun:
             for b in range(0, len(self.forks), self.batch_size):
                 part = self.forks[b: b + self.batch_size]
                 wg = workgroup
-                if wg == 0 and min(f.info["total_rays"] for f in part) >= 10_000_000:
-                    wg = BATCH_PER_SCENE
+                if wg == 0:
+                    fp64 = any(f.params.get("precision", PREC_FP32) == PREC_FP64 for f in part)
+                    wg = BATCH_PER_SCENE if fp64 else BATCH_WINDOW
                 self._batches.append(Batch(part, workgroup=wg, rays_per_thread=rays_per_thread))
         else:
             self._maps = [f.new_maps(smooth=True) for f in self.forks]

@@ -308,7 +308,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(TraceArgs)
 // passes none that is new to the wave.
 template <int SUN, bool POISSON>
 __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, const int lane, const int nrays,
-                                                   const WinRing &R, unsigned long long *s_win)
+                                                   WaveOut &wo, unsigned long long *s_win)
 {
     const HelioTile t = helio_tile(a, tl.x);
     const int L = a.L;
@@ -319,7 +319,6 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
     const int dm = 64 / L, dl = 64 - (64 / L) * L;
     RayStart rs{};
     int m_rs = -1; // microsample of rs
-    uint32_t head_seen = 0u; // a head read earlier: the true head is at least this
     for (int k = 0; k < nrays; k += 64, r += 64) {
         if (k > 0) {
             m += dm;
@@ -349,7 +348,7 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
                     out = true;
             }
         }
-        ring_push(R, out, px, q, lane, head_seen); // the whole wave takes part: no lane has left the loop
+        wo.push(a, out, px, q, lane); // the whole wave takes part: no lane has left the loop
     }
 }
 
@@ -363,11 +362,10 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window_kernel(TraceArgs)
 {
     const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ unsigned long long s_win[WIN_PX_MAX];
-    __shared__ unsigned long long s_ring[WIN_RING];
-    __shared__ uint32_t s_ctl[3];
-    window_body(a, s_win, s_ring, s_ctl,
-                [&](int2 tl, int lane, int nrays, const WinRing &R, unsigned long long *w) {
-                    trace_chunk_window<SUN, POISSON>(a, tl, lane, nrays, R, w);
+    __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
+    window_body(a, s_win, s_out,
+                [&](int2 tl, int lane, int nrays, WaveOut &wo, unsigned long long *w) {
+                    trace_chunk_window<SUN, POISSON>(a, tl, lane, nrays, wo, w);
                 });
 }
 
@@ -414,51 +412,9 @@ __global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__res
 // added to the old entry's per-XCD copies (one returning atomic each, carries and wraps
 // counted as for single hits, which the decode then applies alike) and the window is
 // cleared.  Barriers come only at entry changes, so waves do not wait for one another
-// between chunks.  Hits outside the window go to the wave's LDS buffer and reach the copies
-// BW_OUT / 2 or more at a time, so a wave waits for memory-side atomics once per flush, not
-// per hit.
-constexpr int BW_WAVES = WIN_WGS / 64;
-constexpr int BW_OUT = 128; // out-of-window buffer entries per wave
-constexpr int BW_NONE = 0x7fffffff;
-
-// One hit or one window word w = (n << E) + e into the entry's copy of pixel px
-__device__ __forceinline__ void copy_add(const TraceArgs &a, unsigned long long *part, int px, unsigned long long w,
-                                         unsigned long long &old)
-{
-    old = __hip_atomic_fetch_add(part + (size_t)px * a.px_stride, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void copy_check(const TraceArgs &a, int px, unsigned long long w, unsigned long long old)
-{
-    const unsigned long long lowmask = (1ull << a.pack_shift) - 1ull;
-    if ((old & lowmask) + (w & lowmask) > lowmask) atomicAdd(a.carry + px, 1u);
-    if (old + w < old) atomicAdd(a.wrap + px, 1u);
-}
-
-// the per-XCD copy of this workgroup in entry a's packed map
-__device__ __forceinline__ unsigned long long *copy_of(const TraceArgs &a)
-{
-    const uint32_t copy = (xcc_id() & 7u) | ((((uint32_t)blockIdx.x >> 3) & ((1u << a.copy_shift) - 1u)) << 3);
-    return a.part + (size_t)copy * a.copy_stride;
-}
-
-// Adds the wave's buffered out-of-window hits (n of them, n <= BW_OUT) to the copies.
-__device__ __forceinline__ void flush_out(const TraceArgs &a, const unsigned long long *buf, int n, int lane)
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); // the lanes' buffer stores are done
-    unsigned long long *part = copy_of(a);
-    const unsigned long long one = 1ull << a.pack_shift;
-    unsigned long long e[BW_OUT / 64], old[BW_OUT / 64];
-#pragma unroll
-    for (int b = 0; b < BW_OUT / 64; ++b) {
-        const int i = b * 64 + lane;
-        e[b] = i < n ? buf[i] : RING_EMPTY;
-        if (e[b] != RING_EMPTY) copy_add(a, part, (int)(e[b] >> 32), one + (e[b] & 0xffffffffull), old[b]);
-    }
-#pragma unroll
-    for (int b = 0; b < BW_OUT / 64; ++b)
-        if (e[b] != RING_EMPTY) copy_check(a, (int)(e[b] >> 32), one + (e[b] & 0xffffffffull), old[b]);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); // read before the buffer is refilled
-}
+// between chunks.  Hits outside the window go through the wave's buffer (WaveOut), which is
+// flushed before the wave reaches the barrier.
+constexpr int BW_NONE = 0x7fffffff; // no chunk held
 
 // Adds the window's nonzero words to entry a's copies and clears the window (the whole
 // workgroup; barriers around it are the caller's).
@@ -472,18 +428,15 @@ __device__ __forceinline__ void flush_window(const TraceArgs &a, unsigned long l
         s_win[i] = 0ull;
         const int row = i / a.win_w, col = i - row * a.win_w;
         const int px = (a.win_y0 + row) * a.res_x + a.win_x0 + col;
-        unsigned long long old;
-        copy_add(a, part, px, w, old);
-        copy_check(a, px, w, old);
+        copy_check(a, px, w, copy_add(a, part, px, w));
     }
 }
 
 // nrays rays of heliostat ck.y from ray ck.z, traced by one wave: window hits as LDS atomics,
-// the others through the wave's buffer (nout entries, wave-uniform, flushed when the next
-// pass could overflow it; the caller flushes the rest)
+// the others through the wave's buffer, which the caller flushes before the entry changes
 template <int SUN, bool POISSON>
 __device__ __forceinline__ void trace_chunk_batch(const TraceArgs &a, int4 ck, const int lane,
-                                                  unsigned long long *s_win, unsigned long long *buf, int &nout)
+                                                  unsigned long long *s_win, WaveOut &wo)
 {
     const HelioTile t = helio_tile(a, ck.y);
     const int L = a.L;
@@ -524,16 +477,7 @@ __device__ __forceinline__ void trace_chunk_batch(const TraceArgs &a, int4 ck, c
                     out = true;
             }
         }
-        const unsigned long long mk = __ballot(out);
-        if (out) {
-            const uint32_t j = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
-            buf[nout + (int)j] = ((unsigned long long)px << 32) | q;
-        }
-        nout += __popcll(mk);
-        if (nout > BW_OUT - 64) { // the next pass could overflow the buffer
-            flush_out(a, buf, nout, lane);
-            nout = 0;
-        }
+        wo.push(a, out, px, q, lane);
     }
 }
 
@@ -543,8 +487,8 @@ __global__ __launch_bounds__(WIN_WGS) void trace_batch_window_kernel(const Trace
                                                                      unsigned int *queue)
 {
     __shared__ unsigned long long s_win[WIN_PX_MAX];
-    __shared__ unsigned long long s_out[BW_WAVES][BW_OUT];
-    __shared__ int s_held[BW_WAVES];
+    __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
+    __shared__ int s_held[WIN_WAVES];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
     for (int i = threadIdx.x; i < WIN_PX_MAX; i += WIN_WGS) s_win[i] = 0ull;
     const int q = n_chunks >> 3, rmd = n_chunks & 7;
@@ -565,13 +509,13 @@ __global__ __launch_bounds__(WIN_WGS) void trace_batch_window_kernel(const Trace
     };
     int held = pop();
     int he = held >= 0 ? __builtin_amdgcn_readfirstlane(chunks[held].x) : BW_NONE; // entry of the held chunk
-    int cur = -1;  // entry whose window the LDS holds (uniform over the workgroup)
-    int nout = 0;  // entries in this wave's buffer
+    int cur = -1; // entry whose window the LDS holds (uniform over the workgroup)
+    WaveOut wo{s_out[wave], 0};
     for (;;) {
         if (lane == 0) s_held[wave] = he;
         __syncthreads(); // every wave holds a chunk of another entry or none; the window is idle
         int ne = BW_NONE;
-        for (int w = 0; w < BW_WAVES; ++w) ne = min(ne, s_held[w]);
+        for (int w = 0; w < WIN_WAVES; ++w) ne = min(ne, s_held[w]);
         ne = __builtin_amdgcn_readfirstlane(ne);
         if (ne == BW_NONE) break;
         if (ne != cur) {
@@ -583,14 +527,11 @@ __global__ __launch_bounds__(WIN_WGS) void trace_batch_window_kernel(const Trace
             const int4 c = chunks[held];
             const int4 ck = make_int4(cur, __builtin_amdgcn_readfirstlane(c.y), __builtin_amdgcn_readfirstlane(c.z),
                                       __builtin_amdgcn_readfirstlane(c.w));
-            trace_chunk_batch<SUN, POISSON>(table[cur], ck, lane, s_win, s_out[wave], nout);
+            trace_chunk_batch<SUN, POISSON>(table[cur], ck, lane, s_win, wo);
             held = pop();
             he = held >= 0 ? __builtin_amdgcn_readfirstlane(chunks[held].x) : BW_NONE;
         }
-        if (nout > 0) {
-            flush_out(table[cur], s_out[wave], nout, lane);
-            nout = 0;
-        }
+        wo.flush(table[cur], lane);
     }
     if (cur >= 0) flush_window(table[cur], s_win);
 }

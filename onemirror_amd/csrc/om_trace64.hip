@@ -369,10 +369,10 @@ __global__ __launch_bounds__(WG) void trace64_kernel(Trace64Launch)
 
 // nrays (a multiple of 64) rays of heliostat tl.x from tl.y traced by one wave of a window
 // workgroup, one ray per lane per pass: hits inside the LDS window are LDS atomics, the
-// others go to the ring (window_body, om_window.h).
+// others go to the wave's buffer (window_body, om_window.h).
 template <int SUN, bool POISSON>
 __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const Trace64Args &b, int2 tl, const int lane,
-                                                     const int nrays, const WinRing &R, unsigned long long *s_win)
+                                                     const int nrays, WaveOut &wo, unsigned long long *s_win)
 {
     const int h = tl.x;
     const HelioRec *hr = a.helio + h;
@@ -391,7 +391,6 @@ __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const T
     const bool clear = hr->clear != 0 && !a.literal_self;
     const int L = a.L;
     const int nr = min(hr->mcount * L, tl.y + nrays);
-    uint32_t head_seen = 0u;
     int m_o = -1; // microsample of orig
     d3 orig{0.0, 0.0, 0.0};
     for (int k = 0; k < nrays; k += 64) {
@@ -417,7 +416,7 @@ __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const T
                     out = true;
             }
         }
-        ring_push(R, out, px, q, lane, head_seen); // the whole wave takes part: no lane has left the loop
+        wo.push(a, out, px, q, lane); // the whole wave takes part: no lane has left the loop
     }
 }
 
@@ -426,11 +425,10 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
 {
     const Trace64Launch &args = *(const Trace64Launch *)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ unsigned long long s_win[WIN_PX_MAX];
-    __shared__ unsigned long long s_ring[WIN_RING];
-    __shared__ uint32_t s_ctl[3];
-    window_body(args.a, s_win, s_ring, s_ctl,
-                [&](int2 tl, int lane, int nrays, const WinRing &R, unsigned long long *w) {
-                    trace_chunk_window64<SUN, POISSON>(args.a, args.b, tl, lane, nrays, R, w);
+    __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
+    window_body(args.a, s_win, s_out,
+                [&](int2 tl, int lane, int nrays, WaveOut &wo, unsigned long long *w) {
+                    trace_chunk_window64<SUN, POISSON>(args.a, args.b, tl, lane, nrays, wo, w);
                 });
 }
 
