@@ -277,6 +277,9 @@ def main() -> int:
                     help="ray geometry: fp32 (the reference's, bit-exact against its host restatement), "
                          "fp64 (the FP64 model: per-pixel flux within 1e-5 of FP64), or both (default: "
                          "the value is the fp64 run, the metric's accuracy; fp32 under modes)")
+    ap.add_argument("--ray-compact", type=int, default=0, choices=[0, 1],
+                    help="fp32 window kernel: 1 packs the rays that pass the blocking test into full "
+                         "waves before the receiver stage (om_params.ray_compact)")
     ap.add_argument("--valu-probe", type=int, default=0, metavar="MI",
                     help="first run the VALU issue probes (om_math_eval 12..14) over MI Mi threads: "
                          "kernels with a known VALU instruction count that calibrate the VALU peak "
@@ -326,7 +329,7 @@ def main() -> int:
     runs = {}
     probe = None
     for pr in precisions:
-        p = dict(prm, precision=om.PREC_FP64) if pr == "fp64" else prm
+        p = dict(prm, precision=om.PREC_FP64) if pr == "fp64" else dict(prm, ray_compact=args.ray_compact)
         t0 = time.time()
         scene = om.SolarScene(desc, p)
         info = scene.info

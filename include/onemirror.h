@@ -131,6 +131,11 @@ typedef struct {
                                   in the same fixed point.  Plain traces, per-heliostat
                                   traces, records and per-scene batches; not the persistent
                                   batch launch or the differentiable mode.               */
+    int   ray_compact;         /* OM_ACC_WINDOW, FP32 geometry: 1 packs the rays that pass
+                                  the blocking test into full waves (ballot, mbcnt and one
+                                  ds_permute per ray value) before the receiver stage;
+                                  0 (default): each wave runs the receiver on the lanes its
+                                  walks leave alive.  Results are identical.              */
 } om_params;
 
 enum { OM_PREC_FP32 = 0, OM_PREC_FP64 = 1 };

@@ -91,6 +91,7 @@ class Params(ctypes.Structure):
         ("grid_select", ctypes.c_int),
         ("literal_self", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("ray_compact", ctypes.c_int),
     ]
 
 

@@ -95,12 +95,14 @@ __device__ __forceinline__ RayStart ray_start(const TraceArgs &a, const HelioTil
     return st;
 }
 
-// Ray l of microsample m of the tile's heliostat from its origin rs: map_tracing
-// (rectgrid_raytracing.cu:155-196).  FINE: the Buie bin comes from the split guide in device
-// memory (buie_bin_split), else from the tables s_bins / s_guide (in LDS).
+// Ray l of microsample m of the tile's heliostat from its origin rs, up to the receiver:
+// map_tracing (rectgrid_raytracing.cu:155-196) without its last step.  The ray goes on to
+// the receiver when shade and block are both -1; refl is then its reflected direction and
+// q its energy.  FINE: the Buie bin comes from the split guide in device memory
+// (buie_bin_split), else from the tables s_bins / s_guide (in LDS).
 template <int SUN, bool FINE>
-__device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile &t, const RayStart &rs, int m,
-                                             int l, const float4 *s_bins, const int *s_guide)
+__device__ __forceinline__ RayOut trace_walks(const TraceArgs &a, const HelioTile &t, const RayStart &rs, int m,
+                                              int l, const float4 *s_bins, const int *s_guide, v3 &refl)
 {
     RayOut o;
     o.shade = -1;
@@ -159,12 +161,23 @@ __device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile
     o.shade = t.clear ? -1 : dda(a, orig, c0, to_sun, t.self);
     if (o.shade >= 0) return o;
     if (!(SUN == OM_SUN_BUIE && FINE)) nrm = perturbed_normal(x, a.slope_sigma, t.hf);
-    const v3 refl = normalize<gfx>(reflect(-to_sun, nrm));
+    refl = normalize<gfx>(reflect(-to_sun, nrm));
     o.block = dda(a, orig, c0, refl, t.self);
     if (o.block >= 0) return o;
-    o.rc = receiver_rc(a, orig, refl);
     const float e = fabs_(dot(a.sun_n, nrm)) * 1.0f; // calEnergy with eta = 1
     o.q = (uint32_t)(e * 2147483648.0f);
+    return o;
+}
+
+// The whole of map_tracing (rectgrid_raytracing.cu:155-196) for one ray: trace_walks, then
+// the receiver.
+template <int SUN, bool FINE>
+__device__ __forceinline__ RayOut trace_from(const TraceArgs &a, const HelioTile &t, const RayStart &rs, int m,
+                                             int l, const float4 *s_bins, const int *s_guide)
+{
+    v3 refl;
+    RayOut o = trace_walks<SUN, FINE>(a, t, rs, m, l, s_bins, s_guide, refl);
+    if (o.shade < 0 && o.block < 0) o.rc = receiver_rc(a, rs.orig, refl);
     return o;
 }
 
@@ -352,21 +365,146 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
     }
 }
 
+// Ray compaction after the blocking test (om_params.ray_compact): rays that reach the
+// receiver stage are held, up to 63 of them, in lanes [0, n) of the wave until 64 fill the
+// wave, so that the receiver stage runs on full waves instead of on the two thirds of each
+// pass that the blocking walk leaves alive.  n is wave-uniform.
+struct Pending {
+    v3 orig, refl;
+    uint32_t q;
+    int n;
+};
+
+__device__ __forceinline__ float permute_f(int addr, float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_permute(addr, __float_as_int(v)));
+}
+
+// c ? x : y per component (a select of whole structs would select their addresses)
+__device__ __forceinline__ v3 sel3(bool c, const v3 &x, const v3 &y)
+{
+    return mk(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z);
+}
+
+// The receiver stage of the rays in the lanes where live: pixel, then the window (LDS) or
+// the wave's buffer.  The whole wave takes part.
+__device__ __forceinline__ void receive(const TraceArgs &a, bool live, const v3 &orig, const v3 &refl, uint32_t q,
+                                        const int lane, WaveOut &wo, unsigned long long *s_win)
+{
+    bool out = false;
+    int px = 0;
+    if (live) {
+        const int2 rc = receiver_rc(a, orig, refl);
+        if (rc.x >= 0) {
+            px = rc.x * a.res_x + rc.y;
+            const uint32_t wr = (uint32_t)(rc.x - a.win_y0), wc = (uint32_t)(rc.y - a.win_x0);
+            if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w)
+                packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+            else
+                out = true;
+        }
+    }
+    wo.push(a, out, px, q, lane);
+}
+
+// trace_chunk_window with the receiver stage on compacted waves.  After each pass's walks,
+// one forward permute (ds_permute) per ray value moves the live rays, ranked by mbcnt over
+// the ballot of live lanes, behind the held ones: ranks 0..63 fill the wave, the rest wrap
+// to lanes [0, n) as the new held rays; dead lanes take the other destinations, so the
+// permute is one-to-one.  The caller runs the held rays at the end (finish_compact).
+template <int SUN, bool POISSON>
+__device__ __forceinline__ void trace_chunk_compact(const TraceArgs &a, int2 tl, const int lane, const int nrays,
+                                                    WaveOut &wo, unsigned long long *s_win, Pending &P)
+{
+    const HelioTile t = helio_tile(a, tl.x);
+    const int L = a.L;
+    const int nr = min(t.hr->mcount * L, tl.y + nrays);
+    int r = tl.y + lane;
+    int m = r / L;
+    int l = r - m * L;
+    const int dm = 64 / L, dl = 64 - (64 / L) * L;
+    RayStart rs{};
+    int m_rs = -1;
+    for (int k = 0; k < nrays; k += 64, r += 64) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= L) {
+                l -= L;
+                ++m;
+            }
+        }
+        bool live = false;
+        v3 orig = mk(0.0f, 0.0f, 0.0f), refl = mk(0.0f, 0.0f, 0.0f);
+        uint32_t q = 0u;
+        asm volatile("" ::: "memory"); // see trace_window_kernel
+        if (r < nr) {
+            if (m != m_rs) {
+                rs = ray_start<POISSON>(a, t, m);
+                m_rs = m;
+            }
+            const RayOut o = trace_walks<SUN, true>(a, t, rs, m, l, a.buie_bins, a.buie_guide, refl);
+            live = o.shade < 0 && o.block < 0;
+            orig = rs.orig;
+            q = o.q;
+        }
+        const unsigned long long ms = __ballot(live);
+        const int ns = __popcll(ms);
+        const uint32_t rl = __builtin_amdgcn_mbcnt_hi((uint32_t)(ms >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ms, 0u));
+        const unsigned long long md = ~ms;
+        const uint32_t rd = __builtin_amdgcn_mbcnt_hi((uint32_t)(md >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)md, 0u));
+        const int addr = ((live ? P.n + (int)rl : P.n + ns + (int)rd) & 63) << 2;
+        const v3 ro = mk(permute_f(addr, orig.x), permute_f(addr, orig.y), permute_f(addr, orig.z));
+        const v3 rr = mk(permute_f(addr, refl.x), permute_f(addr, refl.y), permute_f(addr, refl.z));
+        const uint32_t rq = (uint32_t)__builtin_amdgcn_ds_permute(addr, (int)q);
+        const bool held = lane < P.n;
+        if (P.n + ns >= 64) {
+            receive(a, true, sel3(held, P.orig, ro), sel3(held, P.refl, rr), held ? P.q : rq, lane, wo, s_win);
+            P.orig = ro;
+            P.refl = rr;
+            P.q = rq;
+            P.n += ns - 64;
+        } else {
+            P.orig = sel3(held, P.orig, ro);
+            P.refl = sel3(held, P.refl, rr);
+            P.q = held ? P.q : rq;
+            P.n += ns;
+        }
+    }
+}
+
+__device__ __forceinline__ void finish_compact(const TraceArgs &a, const int lane, WaveOut &wo,
+                                               unsigned long long *s_win, Pending &P)
+{
+    if (P.n > 0) receive(a, lane < P.n, P.orig, P.refl, P.q, lane, wo, s_win);
+    P.n = 0;
+}
+
 // The window kernel of the FP32 geometry: window_body (om_window.h) over trace_chunk_window.
 // The arguments are read from the kernel-argument segment where each use needs them (a
 // compiler barrier per ray pass keeps the loads there): held in SGPRs across the ray loop,
 // they exceed the SGPR file and about a hundred of them spill to VGPR lanes, each use then
 // costing a v_readlane in the VALU-bound loop; scalar loads from the segment cost no VALU.
-template <int SUN, bool POISSON>
+// COMPACT: ray compaction after the blocking test (trace_chunk_compact).
+template <int SUN, bool POISSON, bool COMPACT>
 __global__ __launch_bounds__(WIN_WGS) void trace_window_kernel(TraceArgs)
 {
     const TraceArgs &a = *(const TraceArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ unsigned long long s_win[WIN_PX_MAX];
     __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
-    window_body(a, s_win, s_out,
-                [&](int2 tl, int lane, int nrays, WaveOut &wo, unsigned long long *w) {
-                    trace_chunk_window<SUN, POISSON>(a, tl, lane, nrays, wo, w);
-                });
+    if (COMPACT) {
+        Pending P{};
+        window_body(
+            a, s_win, s_out,
+            [&](int2 tl, int lane, int nrays, WaveOut &wo, unsigned long long *w) {
+                trace_chunk_compact<SUN, POISSON>(a, tl, lane, nrays, wo, w, P);
+            },
+            [&](int lane, WaveOut &wo, unsigned long long *w) { finish_compact(a, lane, wo, w, P); });
+    } else {
+        window_body(a, s_win, s_out, [&](int2 tl, int lane, int nrays, WaveOut &wo, unsigned long long *w) {
+            trace_chunk_window<SUN, POISSON>(a, tl, lane, nrays, wo, w);
+        });
+    }
 }
 
 // Batched mode: persistent workgroups take (scene, heliostat, first ray) items from a
@@ -1154,23 +1292,33 @@ template <int SUN, bool POISSON>
 static int window_blocks_per_cu()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_window_kernel<SUN, POISSON>, WIN_WGS, 0) !=
-        hipSuccess)
+    int nc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_window_kernel<SUN, POISSON, false>, WIN_WGS, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, trace_window_kernel<SUN, POISSON, true>, WIN_WGS, 0) !=
+            hipSuccess)
         return 0;
-    return n;
+    return std::min(n, nc); // one workgroup count for both forms
 }
 
-static om_status launch_trace_window(om_scene *s, const TraceArgs &a, int nwg, hipStream_t st)
+template <bool COMPACT>
+static void launch_trace_window_as(om_scene *s, const TraceArgs &a, int nwg, hipStream_t st)
 {
     const bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
     dim3 grid(nwg), block(WIN_WGS);
     if (s->prm.sunshape == OM_SUN_BUIE) {
-        if (poisson) hipLaunchKernelGGL((trace_window_kernel<OM_SUN_BUIE, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((trace_window_kernel<OM_SUN_BUIE, false>), grid, block, 0, st, a);
+        if (poisson) hipLaunchKernelGGL((trace_window_kernel<OM_SUN_BUIE, true, COMPACT>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((trace_window_kernel<OM_SUN_BUIE, false, COMPACT>), grid, block, 0, st, a);
     } else {
-        if (poisson) hipLaunchKernelGGL((trace_window_kernel<OM_SUN_PILLBOX, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((trace_window_kernel<OM_SUN_PILLBOX, false>), grid, block, 0, st, a);
+        if (poisson) hipLaunchKernelGGL((trace_window_kernel<OM_SUN_PILLBOX, true, COMPACT>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((trace_window_kernel<OM_SUN_PILLBOX, false, COMPACT>), grid, block, 0, st, a);
     }
+}
+
+static om_status launch_trace_window(om_scene *s, const TraceArgs &a, int nwg, hipStream_t st)
+{
+    if (s->prm.ray_compact) launch_trace_window_as<true>(s, a, nwg, st);
+    else launch_trace_window_as<false>(s, a, nwg, st);
     OM_HIP(hipGetLastError());
     return OM_OK;
 }

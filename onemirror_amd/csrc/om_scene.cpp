@@ -1430,6 +1430,10 @@ om_status validate(const om_scene_desc *d, const om_params *p)
         om::set_error("precision must be OM_PREC_FP32 or OM_PREC_FP64");
         return OM_ERR_INVALID;
     }
+    if (p->ray_compact != 0 && p->ray_compact != 1) {
+        om::set_error("ray_compact must be 0 or 1");
+        return OM_ERR_INVALID;
+    }
     if (p->sunshape == OM_SUN_BUIE && (p->buie_bins <= 0 || p->buie_bins > 8192 || !(p->csr > 0.0f))) {
         om::set_error("buie_bins must be in 1..8192 and csr positive");
         return OM_ERR_INVALID;

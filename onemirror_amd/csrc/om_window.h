@@ -109,10 +109,10 @@ struct WaveOut {
 // they trace a few neighbouring heliostats whose grid cells and blockers stay in that XCD's
 // L2).  The window is stored to this workgroup's slice of win_part at the end.
 // chunk(tl, lane, nrays, out, s_win) traces nrays rays of heliostat tl.x from ray tl.y with
-// one wave.
-template <class Chunk>
+// one wave; finish(lane, out, s_win) ends a wave's work after its last chunk.
+template <class Chunk, class Finish>
 __device__ __forceinline__ void window_body(const TraceArgs &a, unsigned long long *s_win,
-                                            unsigned long long (*s_out)[WIN_OUT], Chunk chunk_fn)
+                                            unsigned long long (*s_out)[WIN_OUT], Chunk chunk_fn, Finish finish_fn)
 {
     const int wpx = a.win_w * a.win_h; // <= WIN_PX_MAX, checked on the host
     for (int i = threadIdx.x; i < wpx; i += WIN_WGS) s_win[i] = 0ull;
@@ -138,10 +138,18 @@ __device__ __forceinline__ void window_body(const TraceArgs &a, unsigned long lo
             chunk_fn(make_int2(tl.x, tl.y + part * chunk), lane, chunk, out, s_win);
         }
     }
+    finish_fn(lane, out, s_win);
     out.flush(a, lane);
     __syncthreads();
     unsigned long long *dst = a.win_part + (size_t)blockIdx.x * wpx;
     for (int i = threadIdx.x; i < wpx; i += WIN_WGS) dst[i] = s_win[i];
+}
+
+template <class Chunk>
+__device__ __forceinline__ void window_body(const TraceArgs &a, unsigned long long *s_win,
+                                            unsigned long long (*s_out)[WIN_OUT], Chunk chunk_fn)
+{
+    window_body(a, s_win, s_out, chunk_fn, [](int, WaveOut &, unsigned long long *) {});
 }
 
 } // namespace om

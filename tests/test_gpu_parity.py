@@ -270,19 +270,21 @@ def test_mid_records_bitexact(mid, h0):
     _records_equal(g.trace_records(h0, h1), o.trace_f32(h0, h1, records=True), f"mid[{h0}:{h1}]")
 
 
-@pytest.mark.parametrize("mode,shift", [(0, 0), (1, 0), (0, 32), (0, 33), (0, 62), (0, 61),
-                                        (2, 0), (2, 32), (2, 33), (2, 62), (2, 61)])
-def test_accumulation_modes_bitexact(om, mode, shift):
+@pytest.mark.parametrize("mode,shift,compact", [(0, 0, 0), (1, 0, 0), (0, 32, 0), (0, 33, 0), (0, 62, 0),
+                                                (0, 61, 0), (2, 0, 0), (2, 32, 0), (2, 33, 0), (2, 62, 0),
+                                                (2, 61, 0), (2, 0, 1), (2, 33, 1), (2, 62, 1)])
+def test_accumulation_modes_bitexact(om, mode, shift, compact):
     """Packed single-atomic accumulation (with its carry and wrap bookkeeping driven hard by
     narrow fields: E = 32/33 carries out of the energy field every hit or two, E = 61/62
     wraps the count field every few hits), the same words kept in an LDS window of the map
     by persistent workgroups (mode 2: hits inside and outside the window, LDS words that
-    carry and wrap) and two device-scope atomics per hit all reproduce the oracle's integer
-    maps."""
+    carry and wrap; compact: with the rays that pass the blocking test packed into full
+    waves for the receiver stage) and two device-scope atomics per hit all reproduce the
+    oracle's integer maps."""
     import torch
     d, p = F.config_mid()
     d = F.subset(d, slice(1000, 1400))
-    p = dict(p, accumulate=mode, pack_shift=shift)
+    p = dict(p, accumulate=mode, pack_shift=shift, ray_compact=compact)
     g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
     maps = g.trace_plain()
     torch.cuda.synchronize()
@@ -292,19 +294,21 @@ def test_accumulation_modes_bitexact(om, mode, shift):
     assert int(orc["counts"].sum()) > 1_000_000
 
 
-@pytest.mark.parametrize("L,lattice", [(1, False), (7, False), (8, True), (40, True), (64, False),
-                                       (65, True), (100, False), (2048, True)])
-def test_window_sample_blocks_bitexact(om, L, lattice):
+@pytest.mark.parametrize("L,lattice,compact", [(1, False, 0), (7, False, 0), (8, True, 0), (40, True, 0),
+                                               (64, False, 0), (65, True, 0), (100, False, 0), (2048, True, 0),
+                                               (1, False, 1), (7, True, 1), (65, False, 1), (2048, True, 1)])
+def test_window_sample_blocks_bitexact(om, L, lattice, compact):
     """The window kernel traces sample blocks (one microsample per lane, its rays one per
     pass): rays per microsample below, at and above one part of 64, runs of several
     microsamples per lane (L = 1, 7) and a heliostat's last, partial run all give the
-    oracle's maps."""
+    oracle's maps, with and without ray compaction (rays held across passes, chunks and
+    heliostats until a wave is full, the rest run after the wave's last chunk)."""
     import torch
     d, p = F.config_mid()
     n = max(4, min(200, 8000 // L))
     d = F.subset(d, slice(1500, 1500 + n))
-    p = dict(F.with_lattice(p) if lattice else p, accumulate=2, rays_per_sample=L)
-    print(f"window blocks L={L} lattice={lattice} heliostats={n}", flush=True)
+    p = dict(F.with_lattice(p) if lattice else p, accumulate=2, rays_per_sample=L, ray_compact=compact)
+    print(f"window blocks L={L} lattice={lattice} compact={compact} heliostats={n}", flush=True)
     g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
     maps = g.trace_plain()
     torch.cuda.synchronize()
