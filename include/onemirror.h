@@ -194,6 +194,22 @@ void        om_default_params(om_params *p);
 om_status om_ctx_create(int device, om_ctx **out);
 om_status om_ctx_destroy(om_ctx *ctx);
 
+/* Abort and progress of the persistent launches of a context: the window traces
+ * (OM_ACC_WINDOW, FP32 and FP64) and both persistent batch launches.  No reference
+ * counterpart (the reference exits the process on any error, utils.h:13-22); the caller's
+ * watchdog uses them from any thread while a launch runs.
+ * om_ctx_abort(ctx, 1) sets a word of host-pinned memory that every wave reads each time
+ * it takes a chunk of work (one system-scope load beside the queue atomic).  The running
+ * launch, and each later one until om_ctx_abort(ctx, 0), stops within one chunk per wave.
+ * Its maps then hold only the chunks traced so far. */
+om_status om_ctx_abort(om_ctx *ctx, int on);
+/* Chunks of work taken so far by the persistent launch of this context that runs now, or
+ * that ran last, and its total.  The launch writes both into a queue block on its own
+ * stream, and this call copies the block on a stream of the context's own, so the two
+ * always belong to one launch.  An aborted launch counts at most one chunk per wave that
+ * it took but did not trace.  Do not call it while om_batch_destroy runs on the context. */
+om_status om_ctx_progress(om_ctx *ctx, int64_t *taken, int64_t *total);
+
 /* Scene from a .scn file + SolarScene::InitContent: SceneFileProc::SceneFileRead
  * (scene_file_proc.cpp:41) then sun, grid, receiver, heliostat aiming
  * (solar_scene.cpp:83-99).  sun_dir and dni are the solarenergy globals. */

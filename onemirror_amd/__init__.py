@@ -158,7 +158,7 @@ _EXPORTS = [
     "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
     "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel", "om_scn_write",
     "om_master_normal", "om_facet_heliostat", "om_facet_scene_desc", "om_savetxt_map", "om_savetxt_ratio",
-    "om_load_normals", "om_trace_plain_noalloc",
+    "om_load_normals", "om_trace_plain_noalloc", "om_ctx_abort", "om_ctx_progress",
 ]
 
 
@@ -176,6 +176,8 @@ def lib():
     L.om_default_params.argtypes = [ctypes.POINTER(Params)]
     L.om_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
     L.om_ctx_destroy.argtypes = [_vp]
+    L.om_ctx_abort.argtypes = [_vp, ctypes.c_int]
+    L.om_ctx_progress.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.om_scene_from_scn.argtypes = [_vp, ctypes.c_char_p, _f32p, ctypes.c_float,
                                     ctypes.POINTER(Params), ctypes.POINTER(_vp)]
     L.om_scn_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(SceneDesc)]
@@ -450,6 +452,18 @@ class Context:
         if getattr(self, "_h", None) and _LIB is not None:
             _LIB.om_ctx_destroy(self._h)
             self._h = None
+
+    def abort(self, on: bool = True):
+        """Stop the running persistent launches of this context, and the later ones until
+        abort(False) (om_ctx_abort); their maps hold only the work traced before."""
+        _check(lib().om_ctx_abort(self._h, int(bool(on))), "om_ctx_abort")
+
+    def progress(self) -> tuple:
+        """(chunks taken, chunks in all) of the persistent launch that runs now or ran last
+        (om_ctx_progress); callable from another thread while it runs."""
+        t, n = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().om_ctx_progress(self._h, ctypes.byref(t), ctypes.byref(n)), "om_ctx_progress")
+        return t.value, n.value
 
     def math_eval(self, fn: int, x):
         """Evaluate a device float function (parity self-test of the kernel arithmetic)."""

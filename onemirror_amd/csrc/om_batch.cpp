@@ -53,7 +53,7 @@ struct om_batch {
     std::vector<float *> raw;                   // per scene, for smoothing
     om::TraceArgs *d_table = nullptr;
     int4 *d_items = nullptr;                    // work items, or OM_BATCH_WINDOW's chunks
-    unsigned int *d_queue = nullptr;            // one counter, or OM_BATCH_WINDOW's 8 queues
+    unsigned int *d_queue = nullptr;            // queue block: one counter, or OM_BATCH_WINDOW's 8 queues
     om::ReduceSlot *d_slots = nullptr;
     unsigned long long *d_parts = nullptr;      // copies + carry + wrap of every scene
     float *d_raw = nullptr;
@@ -71,6 +71,11 @@ namespace {
 
 void batch_free(om_batch *b)
 {
+    {
+        // om_ctx_progress no longer reads this batch's queue block
+        std::lock_guard<std::mutex> lk(b->ctx->mon_mu);
+        if (b->ctx->cur_queue == b->d_queue) b->ctx->cur_queue = b->ctx->d_queue;
+    }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
@@ -108,7 +113,9 @@ om_status batch_enqueue(om_batch *b, hipStream_t st)
         }
         return OM_OK;
     }
-    if ((rc = om::launch_fill_u32(b->d_queue, b->window ? 8 : 1, st)) != OM_OK) return rc;
+    if ((rc = om::launch_queue_reset(b->d_queue, b->n_items, b->window ? om::QUEUE_EIGHTHS : om::QUEUE_ONE, st)) !=
+        OM_OK)
+        return rc;
     if (b->window) {
         if (b->n_items > 0 &&
             (rc = om::launch_trace_batch_window(b->sunshape, b->poisson, b->d_table, b->d_items, b->n_items,
@@ -258,7 +265,7 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     if ((rc = dalloc(b->d_raw, (size_t)sum_px)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_table, (size_t)n_entries)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_items, items.size())) != OM_OK) return fail(rc);
-    if ((rc = dalloc(b->d_queue, 8)) != OM_OK) return fail(rc);
+    if ((rc = dalloc(b->d_queue, om::QUEUE_WORDS)) != OM_OK) return fail(rc);
     if ((rc = dalloc(b->d_slots, (size_t)n)) != OM_OK) return fail(rc);
     std::vector<om::TraceArgs> table(n_entries);
     std::vector<om::ReduceSlot> slots(n);
@@ -354,6 +361,7 @@ extern "C" om_status om_batch_run(om_batch *b, void *stream)
         b->graph = g;
         OM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
+    om::set_current_queue(b->ctx, b->per_scene ? b->ctx->d_queue : b->d_queue);
     OM_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
     return OM_OK;
 }
@@ -385,6 +393,7 @@ extern "C" om_status om_trace_batched(om_ctx *ctx, om_scene *const *scenes, int 
     om_batch *b = nullptr;
     om_status rc = batch_build(ctx, scenes, n, outs, launch, &b);
     if (rc != OM_OK) return rc;
+    om::set_current_queue(ctx, b->per_scene ? ctx->d_queue : b->d_queue);
     rc = batch_enqueue(b, (hipStream_t)stream);
     if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && rc == OM_OK)
         rc = om::hip_fail(hipGetLastError(), "om_trace_batched");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -94,7 +95,8 @@ struct TraceArgs {
     // map are accumulated in LDS by persistent workgroups, which store their windows to
     // win_part[workgroup][win_h * win_w] at the end; chunks of tiles come from 8 queues
     unsigned long long *win_part;
-    uint32_t *win_queue;      // [8] chunks taken from each queue, zero before the launch
+    uint32_t *win_queue;      // queue block (QUEUE_WORDS): [8] chunks taken from each queue
+    const uint32_t *abort_flag; // host-pinned: nonzero stops every persistent launch (om_ctx_abort)
     int win_x0, win_y0, win_w, win_h;
     int literal_self;         // walks test the ray's own heliostat (om_params.literal_self)
 };
@@ -188,6 +190,15 @@ struct GridState {
 struct om_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // om_ctx_abort / om_ctx_progress: the abort word (host-pinned, read by the kernels), the
+    // queue block of the context's window traces, the queue block of the launch last
+    // enqueued (this one or a batch's), and a stream and pinned buffer to read it
+    uint32_t *h_abort = nullptr;
+    uint32_t *d_queue = nullptr;
+    uint32_t *cur_queue = nullptr;
+    std::mutex mon_mu;
+    hipStream_t mon_stream = nullptr;
+    uint32_t *h_mon = nullptr;
     om::DevBuf scratch;     // raw maps for finalize/smooth
     om::DevBuf scratch2;
     om::DevBuf parts;       // per-XCD partial maps
@@ -295,6 +306,15 @@ int trace_batch_window_blocks_per_cu(int sunshape, bool poisson);
 void window_rect(int rx, int ry, int &x0, int &y0, int &w, int &h);
 int window_chunk_rays();
 om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st);
+// A queue block of a persistent launch: QUEUE_COUNTERS counters of items taken, the item
+// count and the layout (QUEUE_EIGHTHS: 8 queues over contiguous eighths of the items;
+// QUEUE_ONE: counter 0 over all of them), written by launch_queue_reset on the launch's
+// stream, so that om_ctx_progress reads a consistent block of whichever launch runs.
+constexpr int QUEUE_COUNTERS = 8, QUEUE_WORDS = 16;
+enum { QUEUE_EIGHTHS = 0, QUEUE_ONE = 1 };
+om_status launch_queue_reset(uint32_t *q, long long n_items, int layout, hipStream_t st);
+// the queue block that om_ctx_progress reads from now on
+void set_current_queue(om_ctx *ctx, uint32_t *q);
 constexpr int RPT_LIMIT = 8; // rays per thread a trace workgroup holds hit slots for
 // trace heliostats [h0, h1) into out (cleared first when clear is set), with the workspace ws
 om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear, hipStream_t st,

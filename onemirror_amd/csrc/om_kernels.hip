@@ -508,9 +508,10 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window_kernel(TraceArgs)
 }
 
 // Batched mode: persistent workgroups take (scene, heliostat, first ray) items from a
-// queue until it is empty.  Items of one scene are consecutive, so the workgroups running
-// at any time mostly share one scene's grid and heliostats in their L2.  Every workgroup
-// reads the counter until it passes n_items, so the grid drains.
+// queue until it is empty (or the abort word of om_ctx_abort is set).  Items of one scene are
+// consecutive, so the workgroups running at any time mostly share one scene's grid and
+// heliostats in their L2.  Every workgroup reads the counter until it passes n_items, so
+// the grid drains.
 template <int SUN, bool POISSON, int WGS>
 __global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__restrict__ table,
                                                           const int4 *__restrict__ items, int n_items,
@@ -521,7 +522,11 @@ __global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__res
     uint32_t *s_hq = (uint32_t *)(s_hpx + RPT_MAX * WGS);
     __shared__ int s_item;
     for (;;) {
-        if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
+        if (threadIdx.x == 0) {
+            const int it = (int)atomicAdd(queue, 1u);
+            const bool stop = __hip_atomic_load(table[0].abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+            s_item = stop ? n_items : it; // om_ctx_abort: no further item
+        }
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads(); // every thread has read s_item before thread 0 overwrites it
@@ -637,9 +642,11 @@ __global__ __launch_bounds__(WIN_WGS) void trace_batch_window_kernel(const Trace
         while (qj < 8) {
             const int x = (int)((xcd + (uint32_t)qj) & 7u);
             const int cnt = q + (x < rmd ? 1 : 0);
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(queue + x, 1u);
-            t = __builtin_amdgcn_readfirstlane(t); // the whole wave is active here
+            const uint32_t t = take_chunk(queue + x, table[0].abort_flag, lane);
+            if (t == TAKE_ABORT) {
+                qj = 8;
+                break;
+            }
             if (t < (uint32_t)cnt) return x * q + min(x, rmd) + (int)t;
             ++qj;
         }
@@ -712,7 +719,6 @@ __global__ void reduce_batch_kernel(const ReduceSlot *__restrict__ slots)
 // the next trace.
 struct WindowRef {
     unsigned long long *part; // [nwin][w * h] window words of each workgroup
-    uint32_t *queue;          // [8] queue counters
     int nwin, x0, y0, w, h, res_x;
 };
 
@@ -721,7 +727,6 @@ __global__ void reduce_packed_kernel(unsigned long long *part, uint32_t *carry, 
                                      int copy_stride, int px_stride, WindowRef win)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (win.nwin > 0 && i < 8) win.queue[i] = 0u;
     if (i >= npx) return;
     const unsigned long long lowmask = (1ull << E) - 1ull;
     unsigned long long n = 0, e = 0;
@@ -1242,6 +1247,26 @@ om_status launch_reduce_batch(const ReduceSlot *slots, int nslot, int max_npx, h
     return OM_OK;
 }
 
+__global__ void queue_reset_kernel(uint32_t *q, uint32_t n_items, uint32_t layout)
+{
+    const int i = threadIdx.x;
+    if (i < QUEUE_WORDS) q[i] = i < QUEUE_COUNTERS ? 0u : (i == QUEUE_COUNTERS ? n_items : (i == QUEUE_COUNTERS + 1 ? layout : 0u));
+}
+
+om_status launch_queue_reset(uint32_t *q, long long n_items, int layout, hipStream_t st)
+{
+    if (n_items < 0 || n_items > 0x7fffffffLL) return OM_ERR_SIZE;
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, st, q, (uint32_t)n_items, (uint32_t)layout);
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+void set_current_queue(om_ctx *ctx, uint32_t *q)
+{
+    std::lock_guard<std::mutex> lk(ctx->mon_mu);
+    ctx->cur_queue = q;
+}
+
 om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st)
 {
     hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st, p, n);
@@ -1372,13 +1397,13 @@ using namespace om;
 namespace om {
 // Layout of a trace's workspace for scene s: 8 << shift packed copies of the map (same-pixel
 // atomics of concurrent workgroups then mostly land on different words), the carry and wrap
-// maps, and in window mode the 8 queue counters and one window slice per persistent
-// workgroup.  The copies double while twice as many still fit in copy_budget bytes (at least
+// maps, and in window mode one window slice per persistent workgroup (the queues are the
+// context's).  The copies double while twice as many still fit in copy_budget bytes (at least
 // one per XCD) and, with a cap on the total, halve again until the whole layout fits.
 struct WsPlan {
     int shift = 0;
-    size_t n16 = 0, bytes = 0, off_queue = 0, off_win = 0;
-    WindowRef win{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
+    size_t n16 = 0, bytes = 0, off_win = 0;
+    WindowRef win{nullptr, 0, 0, 0, 0, 0, 0};
     int nwg = 0;
 };
 
@@ -1397,10 +1422,8 @@ static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
         const size_t ncopy = (size_t)8 << shift;
         p.shift = shift;
         p.n16 = ncopy * npx;
-        p.off_queue = p.n16 * sizeof(unsigned long long) + 2 * npx * sizeof(uint32_t);
-        p.off_win = p.off_queue + 64;
-        p.bytes = p.nwg > 0 ? p.off_win + (size_t)p.nwg * p.win.w * p.win.h * sizeof(unsigned long long)
-                            : p.off_queue;
+        p.off_win = p.n16 * sizeof(unsigned long long) + 2 * npx * sizeof(uint32_t);
+        p.bytes = p.off_win + (p.nwg > 0 ? (size_t)p.nwg * p.win.w * p.win.h * sizeof(unsigned long long) : 0);
         if (p.bytes <= cap) return OM_OK;
         if (shift == 0) break;
     }
@@ -1450,16 +1473,19 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
         if (fp64 && !(p.nwg > 0 && a.ntiles > 0)) {
             if ((rc = launch_trace64(s, a, trace64_args(s, g), false, st)) != OM_OK) return rc;
         } else if (p.nwg > 0 && a.ntiles > 0) {
-            a.win_queue = (uint32_t *)(base + p.off_queue);
+            a.win_queue = s->ctx->d_queue;
             a.win_part = (unsigned long long *)(base + p.off_win);
             a.win_x0 = win.x0;
             a.win_y0 = win.y0;
             a.win_w = win.w;
             a.win_h = win.h;
-            win.queue = a.win_queue;
             win.part = a.win_part;
             win.nwin = p.nwg;
-            if ((rc = launch_fill_u32(a.win_queue, 8, st)) != OM_OK) return rc;
+            const int chunk = std::min(a.tile_rays, WIN_CHUNK);
+            if ((rc = launch_queue_reset(a.win_queue, (long long)a.ntiles * (a.tile_rays / chunk), QUEUE_EIGHTHS,
+                                         st)) != OM_OK)
+                return rc;
+            set_current_queue(s->ctx, a.win_queue);
             rc = fp64 ? launch_trace64_window(s, a, trace64_args(s, g), p.nwg, st) : launch_trace_window(s, a, p.nwg, st);
             if (rc != OM_OK) return rc;
         } else if ((rc = launch_trace(s, a, false, st)) != OM_OK) {
@@ -1495,7 +1521,7 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
             base = (char *)ws.grow->p;
         } else {
             base = (char *)ws.p;
-            if (ws.clear_each) clear_bytes = p.off_queue; // copies, carry and wrap maps
+            if (ws.clear_each) clear_bytes = p.off_win; // copies, carry and wrap maps
         }
         if (clear_bytes > 0 && (rc = launch_fill_u32((uint32_t *)base, (long long)(clear_bytes / 4), st)) != OM_OK)
             return rc;

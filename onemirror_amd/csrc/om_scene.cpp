@@ -120,7 +120,51 @@ extern "C" om_status om_ctx_create(int device, om_ctx **out)
     om_ctx *c = new om_ctx();
     c->device = device;
     c->static_scratch = sp;
+    // abort word and progress buffer in coherent pinned host memory, the window traces'
+    // queue block, the stream that reads queue blocks while launches run
+    hipError_t e = hipHostMalloc((void **)&c->h_abort, 64, hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_mon, om::QUEUE_WORDS * sizeof(uint32_t), hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_queue, om::QUEUE_WORDS * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(c->d_queue, 0, om::QUEUE_WORDS * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->mon_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        om_ctx_destroy(c);
+        return om::hip_fail(e, "context set-up");
+    }
+    c->h_abort[0] = 0u;
+    c->cur_queue = c->d_queue;
     *out = c;
+    return OM_OK;
+}
+
+extern "C" om_status om_ctx_abort(om_ctx *ctx, int on)
+{
+    if (!ctx || !ctx->h_abort) return OM_ERR_INVALID;
+    __atomic_store_n(ctx->h_abort, on ? 1u : 0u, __ATOMIC_SEQ_CST);
+    return OM_OK;
+}
+
+extern "C" om_status om_ctx_progress(om_ctx *ctx, int64_t *taken, int64_t *total)
+{
+    if (!ctx || !ctx->mon_stream) return OM_ERR_INVALID;
+    OM_HIP(hipSetDevice(ctx->device));
+    uint32_t q[om::QUEUE_WORDS];
+    {
+        std::lock_guard<std::mutex> lk(ctx->mon_mu);
+        OM_HIP(hipMemcpyAsync(ctx->h_mon, ctx->cur_queue, om::QUEUE_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              ctx->mon_stream));
+        OM_HIP(hipStreamSynchronize(ctx->mon_stream));
+        std::memcpy(q, ctx->h_mon, sizeof(q));
+    }
+    const long long n = q[om::QUEUE_COUNTERS];
+    long long done = 0;
+    if (q[om::QUEUE_COUNTERS + 1] == om::QUEUE_ONE) {
+        done = std::min<long long>(q[0], n);
+    } else {
+        for (int x = 0; x < 8; ++x) done += std::min<long long>(q[x], n / 8 + (x < n % 8 ? 1 : 0));
+    }
+    if (taken) *taken = done;
+    if (total) *total = n;
     return OM_OK;
 }
 
@@ -128,6 +172,10 @@ extern "C" om_status om_ctx_destroy(om_ctx *ctx)
 {
     if (!ctx) return OM_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
+    if (ctx->mon_stream) (void)hipStreamDestroy(ctx->mon_stream);
+    if (ctx->d_queue) (void)hipFree(ctx->d_queue);
+    if (ctx->h_mon) (void)hipHostFree(ctx->h_mon);
+    if (ctx->h_abort) (void)hipHostFree(ctx->h_abort);
     if (ctx->scratch.p) (void)hipFree(ctx->scratch.p);
     if (ctx->scratch2.p) (void)hipFree(ctx->scratch2.p);
     if (ctx->parts.p) (void)hipFree(ctx->parts.p);
@@ -1637,6 +1685,7 @@ TraceArgs make_trace_args(const om_scene *s, int g)
     TraceArgs a;
     std::memset(&a, 0, sizeof(a));
     a.literal_self = s->prm.literal_self ? 1 : 0;
+    a.abort_flag = s->ctx ? s->ctx->h_abort : nullptr;
     frame sf = make_frame(s->sun);
     a.sun_u = sf.u;
     a.sun_n = sf.n;

@@ -101,13 +101,30 @@ struct WaveOut {
     }
 };
 
+// One wave takes the next item of a queue: lane 0 adds 1 to the counter and, in the same
+// step (both in flight together), reads the abort word of om_ctx_abort in host memory.
+// Returns the item index, or TAKE_ABORT when the word is set.  The whole wave is active.
+constexpr uint32_t TAKE_ABORT = 0xffffffffu;
+__device__ __forceinline__ uint32_t take_chunk(uint32_t *counter, const uint32_t *abort_flag, int lane)
+{
+    uint32_t t = 0, stop = 0;
+    if (lane == 0) {
+        t = atomicAdd(counter, 1u);
+        stop = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    t = __builtin_amdgcn_readfirstlane(t);
+    stop = __builtin_amdgcn_readfirstlane(stop);
+    return stop != 0u ? TAKE_ABORT : t;
+}
+
 // The body of a window kernel: one persistent workgroup of WIN_WGS threads per CU.  The
 // window s_win is cleared, then every wave traces chunks of up to WIN_CHUNK rays of one
 // tile from 8 queues, one per XCD, each over a contiguous eighth of the tile list (a wave
 // drains its own XCD's queue first, then helps the others, so no wave waits for another
 // until all queues are empty; the waves of an XCD take consecutive chunks, so at any time
 // they trace a few neighbouring heliostats whose grid cells and blockers stay in that XCD's
-// L2).  The window is stored to this workgroup's slice of win_part at the end.
+// L2).  A wave stops taking chunks once the abort word is set.  The window is stored to
+// this workgroup's slice of win_part at the end.
 // chunk(tl, lane, nrays, out, s_win) traces nrays rays of heliostat tl.x from ray tl.y with
 // one wave; finish(lane, out, s_win) ends a wave's work after its last chunk.
 template <class Chunk, class Finish>
@@ -128,9 +145,8 @@ __device__ __forceinline__ void window_body(const TraceArgs &a, unsigned long lo
         const int x = (int)((xcd + j) & 7u);
         const int first = x * q + min(x, rmd), cnt = q + (x < rmd ? 1 : 0);
         for (;;) {
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(a.win_queue + x, 1u);
-            t = __builtin_amdgcn_readfirstlane(t); // the whole wave is active here
+            uint32_t t = take_chunk(a.win_queue + x, a.abort_flag, lane);
+            if (t == TAKE_ABORT) j = 8u; // the host asked every persistent launch to stop
             if (t >= (uint32_t)cnt) break;
             const int c = first + (int)t;
             const int tile = c / parts, part = c - tile * parts;
