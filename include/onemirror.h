@@ -198,10 +198,11 @@ om_status om_ctx_destroy(om_ctx *ctx);
  * (OM_ACC_WINDOW, FP32 and FP64) and both persistent batch launches.  No reference
  * counterpart (the reference exits the process on any error, utils.h:13-22); the caller's
  * watchdog uses them from any thread while a launch runs.
- * om_ctx_abort(ctx, 1) sets a word of host-pinned memory that every wave reads each time
- * it takes a chunk of work (one system-scope load beside the queue atomic).  The running
- * launch, and each later one until om_ctx_abort(ctx, 0), stops within one chunk per wave.
- * Its maps then hold only the chunks traced so far. */
+ * om_ctx_abort(ctx, 1) copies a nonzero word into device memory on a stream of the
+ * context's own; every wave reads it each time it takes a chunk of work (one system-scope
+ * load beside the queue atomic).  The running launch, and each later one until
+ * om_ctx_abort(ctx, 0), stops within one chunk per wave.  Its maps then hold only the
+ * chunks traced so far. */
 om_status om_ctx_abort(om_ctx *ctx, int on);
 /* Chunks of work taken so far by the persistent launch of this context that runs now, or
  * that ran last, and its total.  The launch writes both into a queue block on its own

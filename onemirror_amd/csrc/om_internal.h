@@ -96,7 +96,7 @@ struct TraceArgs {
     // win_part[workgroup][win_h * win_w] at the end; chunks of tiles come from 8 queues
     unsigned long long *win_part;
     uint32_t *win_queue;      // queue block (QUEUE_WORDS): [8] chunks taken from each queue
-    const uint32_t *abort_flag; // host-pinned: nonzero stops every persistent launch (om_ctx_abort)
+    const uint32_t *abort_flag; // nonzero stops every persistent launch (om_ctx_abort)
     int win_x0, win_y0, win_w, win_h;
     int literal_self;         // walks test the ray's own heliostat (om_params.literal_self)
 };
@@ -190,9 +190,11 @@ struct GridState {
 struct om_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // om_ctx_abort / om_ctx_progress: the abort word (host-pinned, read by the kernels), the
-    // queue block of the context's window traces, the queue block of the launch last
-    // enqueued (this one or a batch's), and a stream and pinned buffer to read it
+    // om_ctx_abort / om_ctx_progress: the abort word (device memory, read by the kernels;
+    // h_abort is its pinned staging copy), the queue block of the context's window traces,
+    // the queue block of the launch last enqueued (this one or a batch's), and a stream and
+    // pinned buffer that copy them while launches run
+    uint32_t *d_abort = nullptr;
     uint32_t *h_abort = nullptr;
     uint32_t *d_queue = nullptr;
     uint32_t *cur_queue = nullptr;

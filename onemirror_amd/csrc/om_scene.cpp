@@ -120,10 +120,14 @@ extern "C" om_status om_ctx_create(int device, om_ctx **out)
     om_ctx *c = new om_ctx();
     c->device = device;
     c->static_scratch = sp;
-    // abort word and progress buffer in coherent pinned host memory, the window traces'
-    // queue block, the stream that reads queue blocks while launches run
+    // the abort word and the window traces' queue block in device memory, their pinned
+    // staging buffers, and the stream that copies them while launches run.  The word is in
+    // device memory because the kernels read it once per chunk: read from host memory, those
+    // small reads over PCIe (about 1e5 per trace of 2e8 rays) cost more than the trace.
     hipError_t e = hipHostMalloc((void **)&c->h_abort, 64, hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_mon, om::QUEUE_WORDS * sizeof(uint32_t), hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_abort, 64);
+    if (e == hipSuccess) e = hipMemset(c->d_abort, 0, 64);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_queue, om::QUEUE_WORDS * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(c->d_queue, 0, om::QUEUE_WORDS * sizeof(uint32_t));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->mon_stream, hipStreamNonBlocking);
@@ -139,8 +143,14 @@ extern "C" om_status om_ctx_create(int device, om_ctx **out)
 
 extern "C" om_status om_ctx_abort(om_ctx *ctx, int on)
 {
-    if (!ctx || !ctx->h_abort) return OM_ERR_INVALID;
-    __atomic_store_n(ctx->h_abort, on ? 1u : 0u, __ATOMIC_SEQ_CST);
+    if (!ctx || !ctx->d_abort || !ctx->mon_stream) return OM_ERR_INVALID;
+    OM_HIP(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mon_mu);
+    ctx->h_abort[0] = on ? 1u : 0u;
+    // a copy on the context's stream: it lands while launches on other streams run, and the
+    // kernels' system-scope loads of the word read memory, not a cache
+    OM_HIP(hipMemcpyAsync(ctx->d_abort, ctx->h_abort, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->mon_stream));
+    OM_HIP(hipStreamSynchronize(ctx->mon_stream));
     return OM_OK;
 }
 
@@ -174,6 +184,7 @@ extern "C" om_status om_ctx_destroy(om_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->mon_stream) (void)hipStreamDestroy(ctx->mon_stream);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
+    if (ctx->d_abort) (void)hipFree(ctx->d_abort);
     if (ctx->h_mon) (void)hipHostFree(ctx->h_mon);
     if (ctx->h_abort) (void)hipHostFree(ctx->h_abort);
     if (ctx->scratch.p) (void)hipFree(ctx->scratch.p);
@@ -1685,7 +1696,7 @@ TraceArgs make_trace_args(const om_scene *s, int g)
     TraceArgs a;
     std::memset(&a, 0, sizeof(a));
     a.literal_self = s->prm.literal_self ? 1 : 0;
-    a.abort_flag = s->ctx ? s->ctx->h_abort : nullptr;
+    a.abort_flag = s->ctx ? s->ctx->d_abort : nullptr;
     frame sf = make_frame(s->sun);
     a.sun_u = sf.u;
     a.sun_n = sf.n;

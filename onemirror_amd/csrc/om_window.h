@@ -102,7 +102,9 @@ struct WaveOut {
 };
 
 // One wave takes the next item of a queue: lane 0 adds 1 to the counter and, in the same
-// step (both in flight together), reads the abort word of om_ctx_abort in host memory.
+// step (both in flight together), reads the abort word of om_ctx_abort, which the host
+// copies into device memory; the system-scope load reads memory past the L2, where the
+// copy lands.
 // Returns the item index, or TAKE_ABORT when the word is set.  The whole wave is active.
 constexpr uint32_t TAKE_ABORT = 0xffffffffu;
 __device__ __forceinline__ uint32_t take_chunk(uint32_t *counter, const uint32_t *abort_flag, int lane)
