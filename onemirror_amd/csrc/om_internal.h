@@ -126,6 +126,11 @@ struct Trace64Args {
     const Geo64 *cell_geo;                  // the launch grid's CSR entries in CSR order
     const double *cdf_x, *cdf_y, *slope;    // Buie table, nbins bins
     const int *guide;                       // [nbins]: the bin of value j / gscale
+    // Buie sunshape beyond 4.65 mrad: the polar angle (before the division by 1000) of the
+    // sunshape draw u = (2k + 1) 2^-24 for k = tail_k0 .. 2^23 - 1, the draws whose value u A
+    // reaches int465, each the same pow as the trace would evaluate (null: no such draw)
+    const double *tail;
+    uint32_t tail_k0;
     double gscale;                          // nbins / int465
     int nbins;
     double sun_n[3], sun_u[3], sun_v[3];
@@ -263,6 +268,7 @@ struct om_scene {
     // FP64 geometry: device block of Helio64[H], Geo64[H] and the double Buie table, and the
     // host-side values of the kernel arguments (om_params.precision = OM_PREC_FP64)
     char *d_f64 = nullptr;
+    double *d_tail64 = nullptr;            // Trace64Args::tail
     std::vector<om::Geo64 *> d_cell_geo64; // per grid: this scene's double vertices of its CSR entries
     om::Trace64Args a64{};
     // forks (om_scene_fork): the root scene whose grid, microsamples, tiles and Buie tables
@@ -284,6 +290,9 @@ om_status launch_poisson(om_scene *s, hipStream_t st);
 om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st);
 // the FP64 geometry trace (om_trace64.hip): the kernel arguments of s for grid g, the launch
 Trace64Args trace64_args(const om_scene *s, int g);
+// fills out[i] with the tail angle of draw k0 + i (Trace64Args::tail), i < n
+om_status build_buie_tail64(double *out, uint32_t k0, uint32_t n, double A, double int465, double B, double C,
+                            double inv_gp1);
 om_status launch_trace64(om_scene *s, const TraceArgs &a, const Trace64Args &b, bool records, hipStream_t st);
 // the FP64 trace with the LDS-window accumulation (nwg persistent workgroups)
 om_status launch_trace64_window(om_scene *s, const TraceArgs &a, const Trace64Args &b, int nwg, hipStream_t st);

@@ -1331,6 +1331,28 @@ om_status upload_fp64(om_scene *s)
     q.cdf_y = (const double *)(s->d_f64 + b_hel + b_tab);
     q.slope = (const double *)(s->d_f64 + b_hel + 2 * b_tab);
     q.guide = (const int *)(s->d_f64 + b_hel + 2 * b_tab + b_sl);
+    // the Buie tail: the draws k whose value u01(k << 9) A reaches int465 form the range
+    // [k0, 2^23) (u01 grows with k); their angles are evaluated once, on the device
+    dfree(s->d_tail64);
+    q.tail = nullptr;
+    q.tail_k0 = 1u << 23;
+    if (s->prm.sunshape == OM_SUN_BUIE) {
+        uint32_t lo = 0, hi = 1u << 23; // the first k in [lo, hi] with u A >= int465 (hi: none)
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if ((double)om::u01(mid << 9) * q.A >= q.int465) hi = mid;
+            else lo = mid + 1;
+        }
+        q.tail_k0 = lo;
+        const uint32_t n = (1u << 23) - lo;
+        if (n > 0) {
+            if (hipMalloc((void **)&s->d_tail64, sizeof(double) * n) != hipSuccess)
+                return om::hip_fail(hipGetLastError(), "hipMalloc (Buie tail table)");
+            if ((st = om::build_buie_tail64(s->d_tail64, lo, n, q.A, q.int465, q.B, q.C, q.inv_gp1)) != OM_OK)
+                return st;
+            q.tail = s->d_tail64;
+        }
+    }
     // the candidates of every grid cell in CSR order with their ids (a walk reads a
     // candidate's vertices without first reading its id)
     const om_scene *root = s->grid_src ? s->grid_src : s;
@@ -1389,6 +1411,7 @@ om_status choose_tiles(om_scene *s)
 void free_scene_dev(om_scene *s)
 {
     dfree(s->d_f64);
+    dfree(s->d_tail64);
     for (om::Geo64 *&p : s->d_cell_geo64) dfree(p);
     dfree(s->d_helio);
     dfree(s->d_geo);
@@ -1925,6 +1948,7 @@ extern "C" om_status om_scene_fork(const om_scene *src, const float sun_dir[3], 
     s->d_helio = nullptr;
     s->d_geo = nullptr;
     s->d_f64 = nullptr;
+    s->d_tail64 = nullptr;
     s->d_cell_geo64.assign(ng, nullptr);
     s->d_cell_geo.assign(ng, nullptr);
     s->diff_buf = om::DevBuf{};

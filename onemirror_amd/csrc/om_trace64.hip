@@ -256,7 +256,8 @@ __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dfr
             while (b.cdf_y[lo + 1] <= v) ++lo;
             th = (v - b.cdf_y[lo]) * b.slope[lo] + b.cdf_x[lo];
         } else {
-            th = pow64((v - b.int465) * b.B + b.C, b.inv_gp1);
+            // the tail: the table holds this pow for each draw that reaches here
+            th = b.tail[(x.x >> 9) - b.tail_k0];
         }
         th /= 1000.0;
     }
@@ -433,6 +434,28 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
 }
 
 } // namespace
+
+__global__ void buie_tail64_kernel(double *out, uint32_t k0, uint32_t n, double A, double int465, double B,
+                                   double C, double inv_gp1)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // the draw and the expression of trace_ray64, term for term
+    const double u0 = u01((k0 + i) << 9);
+    const double v = u0 * A;
+    out[i] = pow64((v - int465) * B + C, inv_gp1);
+}
+
+om_status build_buie_tail64(double *out, uint32_t k0, uint32_t n, double A, double int465, double B, double C,
+                            double inv_gp1)
+{
+    if (n == 0) return OM_OK;
+    hipLaunchKernelGGL(buie_tail64_kernel, dim3((n + 255) / 256), dim3(256), 0, nullptr, out, k0, n, A, int465, B, C,
+                       inv_gp1);
+    OM_HIP(hipGetLastError());
+    OM_HIP(hipDeviceSynchronize());
+    return OM_OK;
+}
 
 om_status launch_trace64_window(om_scene *s, const TraceArgs &a, const Trace64Args &b, int nwg, hipStream_t st)
 {
