@@ -456,7 +456,9 @@ om_status om_diff_visibility(om_scene *scene, const uint64_t *d_mask, int h0, in
  * in [2^-90, 2^60] over denominators in [2^-60, 2^60].  12..14 are VALU issue probes, not
  * math: 256 iterations over 8 independent chains per element of 8 v_fma_f32 (12), 8
  * v_fma_f32 + 8 v_add_u32 (13), 8 v_fma_f32 + 2 v_exp_f32 (14) or 8 v_fma_f64 (15) per
- * lane, for measuring the VALU issue rate.  in/out device arrays of n floats. */
+ * lane, for measuring the VALU issue rate.  in/out device arrays of n floats.  16: the
+ * FP64 geometry's slope-error erfinv(2u - 1) of the draw words in[i] (bits as uint32, u as
+ * in 0); out holds n doubles (2n floats). */
 om_status om_math_eval(om_ctx *ctx, int fn, const float *d_in, float *d_out, int n, void *stream);
 
 #ifdef __cplusplus

@@ -290,6 +290,8 @@ om_status launch_poisson(om_scene *s, hipStream_t st);
 om_status launch_trace(om_scene *s, const TraceArgs &a, bool records, hipStream_t st);
 // the FP64 geometry trace (om_trace64.hip): the kernel arguments of s for grid g, the launch
 Trace64Args trace64_args(const om_scene *s, int g);
+// om_math_eval 16: out[i] = the FP64 trace's erfinv(2u - 1) of draw word in[i]
+om_status launch_erfinv_ray(const uint32_t *in, double *out, int n, hipStream_t st);
 // fills out[i] with the tail angle of draw k0 + i (Trace64Args::tail), i < n
 om_status build_buie_tail64(double *out, uint32_t k0, uint32_t n, double A, double int465, double B, double C,
                             double inv_gp1);

@@ -1276,10 +1276,11 @@ om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st)
 
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st)
 {
-    if (fn < 0 || fn > 15) {
-        set_error("math function index must be in 0..15");
+    if (fn < 0 || fn > 16) {
+        set_error("math function index must be in 0..16");
         return OM_ERR_INVALID;
     }
+    if (fn == 16) return launch_erfinv_ray((const uint32_t *)in, (double *)out, n, st);
     const dim3 grid((n + WG - 1) / WG), block(WG);
     if (fn == 12) hipLaunchKernelGGL(valu_probe_kernel<0>, grid, block, 0, st, in, out, n);
     else if (fn == 13) hipLaunchKernelGGL(valu_probe_kernel<1>, grid, block, 0, st, in, out, n);

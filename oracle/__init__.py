@@ -102,6 +102,7 @@ def lib():
     L.oc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
     L.oc_philox_batch.argtypes = [_u32p, _u32p, _u32p, ctypes.c_int]
     L.oc_math_batch.argtypes = [ctypes.c_int, _f32p, _f32p, ctypes.c_int]
+    L.oc_erfinv_d_batch.argtypes = [_f64p, _f64p, ctypes.c_int]
     L.oc_scene_build.argtypes = [ctypes.POINTER(SceneDescC), ctypes.POINTER(ParamsC),
                                  ctypes.POINTER(ctypes.c_void_p)]
     L.oc_scene_build.restype = ctypes.c_int
@@ -179,6 +180,14 @@ def _f3(v):
 # ---------------------------------------------------------------------------------------
 _MATH = {"u01": 0, "sin2pi": 1, "cos2pi": 2, "sin_small": 3, "cos_small": 4, "log": 5,
          "exp": 6, "erfinv": 7, "gauss": 8, "sqrt": 9, "div": 10, "atan2": 11}
+
+
+def erfinv64(x) -> np.ndarray:
+    """The FP64 model's erfinv (om_oracle.c erfinv_d) element-wise."""
+    xd = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(xd)
+    lib().oc_erfinv_d_batch(_p(xd, _f64p), _p(out, _f64p), int(xd.size))
+    return out
 
 
 def math(fn: str, x) -> np.ndarray:

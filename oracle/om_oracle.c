@@ -1169,6 +1169,12 @@ static double erfinv_d(double x)
     return y;
 }
 
+/* the FP64 model's erfinv element-wise (tests) */
+void oc_erfinv_d_batch(const double *in, double *out, int n)
+{
+    for (int i = 0; i < n; ++i) out[i] = erfinv_d(in[i]);
+}
+
 typedef struct {
     const oc_scene *s;
     d3 sun, rv[4];

@@ -108,6 +108,8 @@ float oc_gauss_from_u(float u);
    5 log, 6 exp, 7 erfinv, 8 gauss_from_u, 9 sqrt */
 /* fn 10 and 11 take pairs (in[2i], in[2i+1]): division and atan2, n/2 outputs */
 void  oc_math_batch(int fn, const float *in, float *out, int n);
+/* the FP64 model's erfinv (Newton on libm's erf from oc_erfinvf), element-wise */
+void  oc_erfinv_d_batch(const double *in, double *out, int n);
 void  oc_philox_batch(const uint32_t *ctr /*[n][4]*/, const uint32_t key[2], uint32_t *out, int n);
 
 /* ---- scene construction: SolarScene::InitContent (solar_scene.cpp:83-99) ---- */
