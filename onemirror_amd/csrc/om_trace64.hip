@@ -167,10 +167,21 @@ __device__ __forceinline__ bool parallelogram64(d3 orig, d3 dir, d3 A, d3 E1, d3
 // index range (every later cell is empty).  The FP64 model divides by each direction
 // component once (absDivide's quotients as products with 1 / dir).  self: the id skipped,
 // or -1.
-__device__ int walk64(const TraceArgs &a, const Trace64Args &b, d3 o, d3 d, int self)
+// The grid cell of a ray origin, shared by both walks of the rays of one microsample
+struct Cell64 {
+    int g[3];
+};
+__device__ __forceinline__ Cell64 cell64(const Trace64Args &b, d3 o)
 {
     const d3 rel = sub(o, ld(b.gpos));
-    int g[3] = {(int)(rel.x / b.gint[0]), (int)(rel.y / b.gint[1]), (int)(rel.z / b.gint[2])};
+    return Cell64{{(int)(rel.x / b.gint[0]), (int)(rel.y / b.gint[1]), (int)(rel.z / b.gint[2])}};
+}
+
+// c: the cell of o (cell64)
+__device__ int walk64(const TraceArgs &a, const Trace64Args &b, d3 o, const Cell64 &c, d3 d, int self)
+{
+    const d3 rel = sub(o, ld(b.gpos));
+    int g[3] = {c.g[0], c.g[1], c.g[2]};
     const double dd[3] = {d.x, d.y, d.z}, cur[3] = {rel.x, rel.y, rel.z};
     const int gn[3] = {a.gnx, a.gny, a.gnz};
     int st[3];
@@ -295,6 +306,7 @@ struct Ray64 {
 // one ray (map_tracing, rectgrid_raytracing.cu:155-196) in double
 template <int SUN>
 __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dframe &hf, const dframe &sf, d3 orig,
+                             const Cell64 &c0,
                              int h, int m, int l, int self, bool clear)
 {
     Ray64 r{-1, -2, -1, make_int2(-1, -1), 0u};
@@ -325,10 +337,10 @@ __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dfr
     const double phs = 6.283185307179586 * u3;
     const double2 sts = sincos_polar(ths), sps = sincos_ray(phs);
     const d3 nrm = normalized(world(mkd(sts.x * sps.y, sts.y, sts.x * sps.x), hf));
-    r.shade = clear ? -1 : walk64(a, b, orig, to_sun, self);
+    r.shade = clear ? -1 : walk64(a, b, orig, c0, to_sun, self);
     if (r.shade >= 0) return r;
     const d3 refl = normalized(reflected(neg(to_sun), nrm));
-    r.block = walk64(a, b, orig, refl, self);
+    r.block = walk64(a, b, orig, c0, refl, self);
     if (r.block >= 0) return r;
     r.rc = receiver64(a, b, orig, refl);
     if (r.rc.x >= 0) {
@@ -378,6 +390,7 @@ __global__ __launch_bounds__(WG) void trace64_kernel(Trace64Launch)
     int nk = 0;
     int m_o = -1; // microsample of orig: its origin is found again only when m changes
     d3 orig{0.0, 0.0, 0.0};
+    Cell64 c0{};
     for (int k = 0; k < a.rays_per_thread; ++k) {
         const int r = tl.y + tid + k * WG;
         if (r >= nr) break;
@@ -386,9 +399,10 @@ __global__ __launch_bounds__(WG) void trace64_kernel(Trace64Launch)
         const int m = r / L, l = r - m * L;
         if (m != m_o) {
             orig = add(world(micro_local64<POISSON>(a, b, hr, h64, h, m), hf), hpos);
+            c0 = cell64(b, orig);
             m_o = m;
         }
-        const Ray64 o = trace_ray64<SUN>(a, b, hf, sf, orig, h, m, l, self, clear);
+        const Ray64 o = trace_ray64<SUN>(a, b, hf, sf, orig, c0, h, m, l, self, clear);
         if (RECORD) {
             const long long rid = (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
             a.rec_shade[rid] = o.shade;
@@ -451,6 +465,7 @@ __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const T
     const int nr = min(hr->mcount * L, tl.y + nrays);
     int m_o = -1; // microsample of orig
     d3 orig{0.0, 0.0, 0.0};
+    Cell64 c0{};
     for (int k = 0; k < nrays; k += 64) {
         const int r = tl.y + k + lane;
         bool out = false;
@@ -461,9 +476,10 @@ __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const T
             const int m = r / L, l = r - m * L;
             if (m != m_o) {
                 orig = add(world(micro_local64<POISSON>(a, b, hr, h64, h, m), hf), hpos);
+                c0 = cell64(b, orig);
                 m_o = m;
             }
-            const Ray64 o = trace_ray64<SUN>(a, b, hf, sf, orig, h, m, l, self, clear);
+            const Ray64 o = trace_ray64<SUN>(a, b, hf, sf, orig, c0, h, m, l, self, clear);
             if (o.px >= 0) {
                 px = o.px;
                 q = o.q;
