@@ -32,7 +32,27 @@ __device__ __forceinline__ d3 cross(d3 a, d3 b)
 {
     return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ d3 normalized(d3 v) { return scl(v, 1.0 / sqrt(dot(v, v))); }
+// 1 / d and 1 / sqrt(x) within about an ulp: the hardware estimates refined by two Newton
+// steps each.  The FP64 model rounds both correctly; a last-bit difference in a direction or
+// a crossing parameter moves a ray by some 1e-16 of the scene, far below the geometry's
+// tolerances.
+__device__ __forceinline__ double rcp64(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double rsq64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+__device__ __forceinline__ d3 normalized(d3 v) { return scl(v, rsq64(dot(v, v))); }
 __device__ __forceinline__ d3 reflected(d3 i, d3 n) { return sub(i, scl(n, 2.0 * dot(n, i))); }
 
 // The double transcendentals out of line: inlined, their general argument reduction holds
@@ -149,7 +169,7 @@ __device__ __forceinline__ bool parallelogram64(d3 orig, d3 dir, d3 A, d3 E1, d3
     const d3 pvec = cross(dir, E2);
     const double det = dot(E1, pvec);
     if (fabs(det) < 1e-6) return false;
-    const double invDet = 1 / det;
+    const double invDet = rcp64(det);
     const d3 T = sub(orig, A);
     u = dot(T, pvec) * invDet;
     if (u < 0 || u > 1) return false;
@@ -161,12 +181,6 @@ __device__ __forceinline__ bool parallelogram64(d3 orig, d3 dir, d3 A, d3 E1, d3
     return true;
 }
 
-// NotCollision + Intersect (rectgrid_raytracing.cu:21-117) in double: start cell by
-// truncation and never tested, absDivide's INT_MAX for |dir| <= 1e-6, the reference's
-// tie-break, first candidate in CSR order; the walk ends once an axis has passed the occupied
-// index range (every later cell is empty).  The FP64 model divides by each direction
-// component once (absDivide's quotients as products with 1 / dir).  self: the id skipped,
-// or -1.
 // The grid cell of a ray origin, shared by both walks of the rays of one microsample
 struct Cell64 {
     int g[3];
@@ -177,6 +191,12 @@ __device__ __forceinline__ Cell64 cell64(const Trace64Args &b, d3 o)
     return Cell64{{(int)(rel.x / b.gint[0]), (int)(rel.y / b.gint[1]), (int)(rel.z / b.gint[2])}};
 }
 
+// NotCollision + Intersect (rectgrid_raytracing.cu:21-117) in double: start cell by
+// truncation and never tested, absDivide's INT_MAX for |dir| <= 1e-6, the reference's
+// tie-break, first candidate in CSR order; the walk ends once an axis has passed the occupied
+// index range (every later cell is empty).  The FP64 model divides by each direction
+// component once (absDivide's quotients as products with 1 / dir).  self: the id skipped,
+// or -1.
 // c: the cell of o (cell64)
 __device__ int walk64(const TraceArgs &a, const Trace64Args &b, d3 o, const Cell64 &c, d3 d, int self)
 {
@@ -190,7 +210,7 @@ __device__ int walk64(const TraceArgs &a, const Trace64Args &b, d3 o, const Cell
     for (int k = 0; k < 3; ++k) {
         st[k] = dd[k] >= 0 ? 1 : -1;
         const double t = dd[k] >= 0 ? (g[k] + 1) * b.gint[k] - cur[k] : cur[k] - g[k] * b.gint[k];
-        const double inv = 1.0 / dd[k];
+        const double inv = rcp64(dd[k]); // used where |dd[k]| > 1e-6 only
         tm[k] = (fabs(dd[k]) <= 1e-6) ? 2147483647.0 : fabs(t * inv);
         td[k] = (fabs(dd[k]) <= 1e-6) ? 2147483647.0 : fabs(b.gint[k] * inv);
     }
