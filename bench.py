@@ -41,6 +41,15 @@ P_VALU = 256 * 128 * 2.4e9         # lane-op/s: two 64-lane VALU instructions pe
 P_VALU64 = 256 * 64 * 2.4e9        # FP64 lane-op/s: one 64-lane FP64 instruction per CU clock
 F0, C_STEP, C_TEST, C_SKIP = 569.0, 12.0, 58.0, 3.0
 C_SETUP = 57.0  # one walk set-up, the half of F0's two that a skipped shading walk saves
+GEOMETRY = {
+    "fp32": "FP32: the reference's ray geometry, bit-exact against its host restatement",
+    "fp64": ("FP64: sampling, frames, walks, intersections and receiver in double (om_params.precision = "
+             "OM_PREC_FP64), energies in exact fixed point, FP32 flux map"),
+    "mixed": ("mixed (om_params.precision = OM_PREC_MIXED): FP32 sampling and walks whose every decision is "
+              "certified by an error bound, directions, frames and receiver hit in double, rays with a decision "
+              "inside its bound traced again in FP64: the FP64 model's shading, blocking and pixel for every "
+              "ray; energies in exact fixed point, FP32 flux map"),
+}
 
 
 def model_ops(st: dict) -> tuple:
@@ -224,6 +233,7 @@ def roofline(run: dict, workload: str, R: int, ops_ray: float, S: float, T: floa
     rocprof run (VALU instructions per CU busy cycle against the issue probe of the same
     arithmetic) and its HBM bytes per launch."""
     fp64 = run["kernel"].startswith("trace_window64") or run["kernel"].startswith("trace64")
+    # the mixed kernel issues FP32 and FP64 instructions: priced against the FP32 peak
     peak = P_VALU64 if fp64 else P_VALU
     achieved = R * ops_ray / (run["trace_ms"] * 1e-3)
     counters = None
@@ -254,7 +264,8 @@ def roofline(run: dict, workload: str, R: int, ops_ray: float, S: float, T: floa
             traffic = pj["hbm_bytes_per_launch"]
     return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
             "frac": achieved / peak, "traffic": traffic, "ops_per_ray": ops_ray, "dda_steps_per_ray": S,
-            "tests_per_ray": T, "arithmetic": "fp64" if fp64 else "fp32", "counters": counters}
+            "tests_per_ray": T, "arithmetic": "fp64" if fp64 else ("fp32+fp64" if "mixed" in run["kernel"] else "fp32"),
+            "counters": counters}
 
 
 def main() -> int:
@@ -273,10 +284,13 @@ def main() -> int:
                     help="CPU baseline traces one block of 40 heliostats in every n "
                          "(default: the whole field, one block in 5 for large)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", default="both", choices=["both", "fp32", "fp64"],
+    ap.add_argument("--precision", default="all", choices=["all", "both", "fp32", "fp64", "mixed"],
                     help="ray geometry: fp32 (the reference's, bit-exact against its host restatement), "
-                         "fp64 (the FP64 model: per-pixel flux within 1e-5 of FP64), or both (default: "
-                         "the value is the fp64 run, the metric's accuracy; fp32 under modes)")
+                         "fp64 (the FP64 model: per-pixel flux within 1e-5 of FP64), mixed (FP32 walks "
+                         "with certified decisions, double directions and receiver, uncertain rays "
+                         "traced in FP64: the FP64 model's outcome per ray), both (fp32 and fp64) or "
+                         "all (default: the value is the mixed run, the metric's accuracy at the "
+                         "least cost; fp64 and fp32 under modes)")
     ap.add_argument("--ray-compact", type=int, default=0, choices=[0, 1],
                     help="fp32 window kernel: 1 packs the rays that pass the blocking test into full "
                          "waves before the receiver stage (om_params.ray_compact)")
@@ -324,12 +338,13 @@ def main() -> int:
         desc, prm = F.config_small()
         wl = ("small: 100 heliostats 2x2 m, pillbox 4.65 mrad, slope 2 mrad, 16x16 Poisson "
               "x 64 rays, 64^3 grid, 128x128 map")
-    precisions = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
-    head = "fp64" if "fp64" in precisions else "fp32"
+    precisions = {"all": ["fp32", "fp64", "mixed"], "both": ["fp32", "fp64"]}.get(args.precision, [args.precision])
+    head = "mixed" if "mixed" in precisions else ("fp64" if "fp64" in precisions else "fp32")
     runs = {}
     probe = None
     for pr in precisions:
-        p = dict(prm, precision=om.PREC_FP64) if pr == "fp64" else dict(prm, ray_compact=args.ray_compact)
+        p = {"fp64": dict(prm, precision=om.PREC_FP64), "mixed": dict(prm, precision=om.PREC_MIXED)}.get(
+            pr, dict(prm, ray_compact=args.ray_compact))
         t0 = time.time()
         scene = om.SolarScene(desc, p)
         info = scene.info
@@ -339,7 +354,8 @@ def main() -> int:
         maps = scene.new_maps(smooth=True)
         # accumulation autotuner: per-tile packed atomics or the LDS window (identical maps)
         acc = scene.autotune_accumulate(maps, reps=3)
-        kernel = ("trace_window" if acc["accumulate"] == "window" else "trace") + ("64" if pr == "fp64" else "") + "_kernel"
+        kernel = ("trace_window" if acc["accumulate"] == "window" else "trace") + \
+            {"fp64": "64", "mixed": "_mixed" if acc["accumulate"] == "window" else "64"}.get(pr, "") + "_kernel"
         print(f"# {pr} accumulation autotune: packed {acc['packed_ms']:.3f} ms, window {acc['window_ms']:.3f} ms "
               f"-> {acc['accumulate']}", file=sys.stderr, flush=True)
         if probe is None and args.valu_probe > 0:
@@ -347,8 +363,11 @@ def main() -> int:
         if pr == precisions[0] and args.copy_probe > 0:
             copy_probe(args.copy_probe)
         t = time_trace(scene, maps, args.steps, args.warmup)
+        if pr == "mixed":
+            t["redo_rays"] = scene.redo_count()
         print(f"# {pr}: {args.steps} steps, {t['ms_per_step']:.3f} ms/step, trace kernel {t['trace_ms']:.3f} ms, "
-              f"{t['rays_s']:.4g} rays/s, receiver hits {t['hits']}", file=sys.stderr, flush=True)
+              f"{t['rays_s']:.4g} rays/s, receiver hits {t['hits']}"
+              + (f", {t['redo_rays']} rays traced in FP64" if pr == "mixed" else ""), file=sys.stderr, flush=True)
         runs[pr] = dict(scene=scene, maps=maps, acc=acc, kernel=kernel, **t)
     scene = runs[head]["scene"]
     info = scene.info
@@ -405,17 +424,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f32",
+        "dtype": {"fp64": "f64", "mixed": "f32+f64"}.get(head, "f32"),
         "data": "synthetic",
         "config": {"workload": wl, "precision": head, "heliostats": H, "rays": R,
                    "grid": list(info["grid_num"]),
                    "map": [info["res_y"], info["res_x"]], "parallelism": "single GPU",
                    "trace_kernel_ms": hr["trace_ms"], "receiver_hits": hr["hits"], "kernel": hr["kernel"],
                    "accumulate": hr["acc"],
-                   "geometry": ("FP64: sampling, frames, walks, intersections and receiver in double "
-                                "(om_params.precision = OM_PREC_FP64), energies in exact fixed point, "
-                                "FP32 flux map" if head == "fp64" else
-                                "FP32: the reference's ray geometry, bit-exact against its host restatement")},
+                   "geometry": GEOMETRY[head]},
         "roofline": rooflines[head],
         "flux_err_vs_fp64": errs.get(head),
         "valu_probe": probe,
@@ -429,13 +445,16 @@ def main() -> int:
                         "early-exit walk, OpenMP",
             "seconds": cpu_t, "fp64_model": fp64_model},
     }
-    if head == "fp64" and "fp32" in runs:
-        r32 = runs["fp32"]
-        out["modes"] = {"fp32_reference": {
-            "value": r32["rays_s"], "unit": "rays/s", "ms_per_step": r32["ms_per_step"],
-            "trace_kernel_ms": r32["trace_ms"], "kernel": r32["kernel"], "accumulate": r32["acc"],
-            "geometry": "FP32: the reference's ray geometry, bit-exact against its host restatement",
-            "flux_err_vs_fp64": errs.get("fp32"), "roofline": rooflines["fp32"]}}
+    if head == "mixed":
+        out["config"]["redo_rays"] = hr["redo_rays"]
+    for pr, name in (("fp64", "fp64"), ("fp32", "fp32_reference")):
+        if pr == head or pr not in runs:
+            continue
+        r = runs[pr]
+        out.setdefault("modes", {})[name] = {
+            "value": r["rays_s"], "unit": "rays/s", "ms_per_step": r["ms_per_step"],
+            "trace_kernel_ms": r["trace_ms"], "kernel": r["kernel"], "accumulate": r["acc"],
+            "geometry": GEOMETRY[pr], "flux_err_vs_fp64": errs.get(pr), "roofline": rooflines[pr]}
     if args.workload == "mid" and not args.no_modes:
         # the batched and differentiable forms of the metric and the annual runs, on their own
         # workloads (a few timed steps each)

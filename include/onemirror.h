@@ -130,15 +130,30 @@ typedef struct {
                                   FP32 error is measured against), energies summed exactly
                                   in the same fixed point.  Plain traces, per-heliostat
                                   traces, records and per-scene batches; not the persistent
-                                  batch launch or the differentiable mode.               */
+                                  batch launch or the differentiable mode.
+                                  OM_PREC_MIXED: the FP64 model's outcome of every ray at
+                                  close to FP32 cost.  Sampling and both walks in FP32,
+                                  directions, frames and the receiver hit in double; every
+                                  FP32 decision (walk step, candidate test, pixel) is taken
+                                  only outside its rounding-error bound, and the rays with a
+                                  decision inside it are traced again in FP64.  Shading,
+                                  blocking and pixel of every ray are the FP64 model's;
+                                  energies differ from it by the FP32 draws (about 1e-8
+                                  relative).  Rectangle and cylinder receivers; scenes with
+                                  the cone receiver or literal_self trace in FP64.  Same
+                                  entry points as OM_PREC_FP64.                         */
     int   ray_compact;         /* OM_ACC_WINDOW, FP32 geometry: 1 packs the rays that pass
                                   the blocking test into full waves (ballot, mbcnt and one
                                   ds_permute per ray value) before the receiver stage;
                                   0 (default): each wave runs the receiver on the lanes its
                                   walks leave alive.  Results are identical.              */
+    int   redo_cap;            /* OM_PREC_MIXED: entries of a launch's list of rays for the
+                                  FP64 trace (0, the default: one per 64 rays, 2^16 to
+                                  2^22).  A launch whose list fills up is traced again
+                                  wholly in FP64: the same results, more time.           */
 } om_params;
 
-enum { OM_PREC_FP32 = 0, OM_PREC_FP64 = 1 };
+enum { OM_PREC_FP32 = 0, OM_PREC_FP64 = 1, OM_PREC_MIXED = 2 };
 
 enum { OM_GRIDS_ALL = -1 };
 
@@ -331,6 +346,11 @@ om_status om_trace_heliostats(om_scene *scene, int h0, int h1, const om_flux *ou
  * Device int32 arrays of at least ray_offset[h1]-ray_offset[h0] elements. */
 om_status om_trace_records(om_scene *scene, int h0, int h1, int32_t *shade, int32_t *block,
                            int32_t *pixel, int64_t capacity, void *stream);
+
+/* OM_PREC_MIXED: the number of rays of the scene's last trace (or records call) that a
+ * decision within its error bound sent to the FP64 trace.  Waits for the device.  No
+ * counterpart in the reference (its traces are FP32 only). */
+om_status om_scene_redo_count(const om_scene *scene, int64_t *rays);
 
 /* Energy/fixed point -> flux maps (the scaling loop of Gtest.cu:37-46) for a scene. */
 om_status om_flux_finalize(om_scene *scene, const om_flux *out, void *stream);

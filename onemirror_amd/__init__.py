@@ -92,11 +92,12 @@ class Params(ctypes.Structure):
         ("literal_self", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("ray_compact", ctypes.c_int),
+        ("redo_cap", ctypes.c_int),
     ]
 
 
 ACC_PACKED, ACC_DEVICE, ACC_WINDOW = 0, 1, 2
-PREC_FP32, PREC_FP64 = 0, 1
+PREC_FP32, PREC_FP64, PREC_MIXED = 0, 1, 2
 GRIDS_ALL = -1
 
 
@@ -158,7 +159,7 @@ _EXPORTS = [
     "om_autotune", "om_autotune_accumulate", "om_diff_mask_words", "om_diff_forward", "om_diff_backward",
     "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel", "om_scn_write",
     "om_master_normal", "om_facet_heliostat", "om_facet_scene_desc", "om_savetxt_map", "om_savetxt_ratio",
-    "om_load_normals", "om_trace_plain_noalloc", "om_ctx_abort", "om_ctx_progress",
+    "om_load_normals", "om_trace_plain_noalloc", "om_ctx_abort", "om_ctx_progress", "om_scene_redo_count",
 ]
 
 
@@ -207,6 +208,7 @@ def lib():
     L.om_autotune_accumulate.argtypes = [_vp, ctypes.POINTER(Flux), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]
     L.om_trace_heliostats.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Flux), _vp]
+    L.om_scene_redo_count.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
     L.om_trace_records.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                    ctypes.c_int64, _vp]
     L.om_flux_finalize.argtypes = [_vp, ctypes.POINTER(Flux), _vp]
@@ -617,6 +619,13 @@ class SolarScene:
     def ray_offset(self) -> np.ndarray:
         n = self.info["n_helio"]
         return self.export("ray_offset", 8 * (n + 1), np.int64)
+
+    def redo_count(self) -> int:
+        """OM_PREC_MIXED: rays of the last trace (or records call) whose FP32 decisions fell
+        within their error bounds and were traced by the FP64 trace (om_scene_redo_count)."""
+        n = ctypes.c_int64(0)
+        _check(lib().om_scene_redo_count(self._h, ctypes.byref(n)), "om_scene_redo_count")
+        return int(n.value)
 
     def trace_records(self, h0: int = 0, h1: Optional[int] = None, stream=None) -> dict:
         import torch

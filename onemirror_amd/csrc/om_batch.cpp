@@ -174,8 +174,8 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
             return OM_ERR_SIZE;
         }
         if (s->prm.precision != OM_PREC_FP32 && b->wgs != OM_BATCH_PER_SCENE) {
-            om::set_error("the persistent batch launch traces in FP32 geometry: FP64 scenes need "
-                          "OM_BATCH_PER_SCENE");
+            om::set_error("the persistent batch launch traces in FP32 geometry: FP64 and mixed-precision "
+                          "scenes need OM_BATCH_PER_SCENE");
             delete b;
             return OM_ERR_UNSUPPORTED;
         }

@@ -637,7 +637,8 @@ static om_status diff_prepare(om_scene *s, DiffArgs &p, hipStream_t st)
         return OM_ERR_UNSUPPORTED;
     }
     if (s->prm.precision != OM_PREC_FP32) {
-        set_error("differentiable mode traces in FP32 geometry (precision OM_PREC_FP64 is not supported)");
+        set_error("differentiable mode traces in FP32 geometry (precisions OM_PREC_FP64 and OM_PREC_MIXED are "
+                  "not supported)");
         return OM_ERR_UNSUPPORTED;
     }
     if (!s->normal_in.empty()) {

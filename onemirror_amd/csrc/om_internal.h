@@ -140,6 +140,60 @@ struct Trace64Args {
     double gpos[3], gint[3];                // the launch's grid
     double rA[3], rE1[3], rE2[3];           // rectangle receiver: v0, v1 - v0, v3 - v0
     double cyl_base[3], cyl_r, cyl_h, cone_r0, cone_r1;
+    // window trace of a mixed launch: runs only when *gate > gate_cap (its redo list filled
+    // up); null: always
+    const uint32_t *gate;
+    uint32_t gate_cap;
+};
+
+// ---- Certified FP32 geometry (om_params.precision = OM_PREC_MIXED) ----------------------
+// A ray is sampled in FP32, its directions, frames and receiver hit are formed in double
+// from the FP64 model's scene, and both walks run in FP32 over float images of the model's
+// vertices.  Every decision of the FP32 arithmetic (a walk step's axis, a candidate's hit
+// test, the receiver pixel) is taken only when it holds with the error bounds below: a ray
+// with a decision inside its bound is traced again by the FP64 trace.  The bounds, for every
+// ray of the scene:
+//   e_dir   |d - d_model| (2-norm) of a direction formed from the FP32 draws, plus the FP32
+//           rounding of its components in the walks: 2^-19 (theta_max + 2 theta_s,max)
+//   e_pos   |float(origin) - origin| + |float(vertex) - vertex|: 2^-22 of the largest
+//           coordinate of the scene
+//   e_cell  error of a crossing numerator k * edge - (origin - grid pos) in FP32: 2^-21 of
+//           the grid's extent
+// Per heliostat, the FP64 model's values rounded to float: the frame of the aimed normal n0
+// (the identity axes for an identity frame), the reflection a = i0 - 2 (n0.i0) n0 of the
+// nominal light direction i0 (the sun frame's axis), c0 = n0.i0, and the nominal ray's
+// point nearest the receiver's reference point F: D = (F - pos).a, qf = pos + D a - F.  A ray
+// is then a + dr with dr formed from small quantities only, and the receiver is met at
+// F + c + tau d with c = qf + (o - pos) + D dr: no FP32 operand carries a coordinate of the
+// field, so the rounding stays at the size of the deviations.
+struct alignas(16) HelioMix {
+    float u[3], n[3], v[3];
+    float a[3];
+    float qf[3];
+    float c0, D;
+    float e0;   // n0.sun: the energy |sun.n| = |e0 + sun.dn|
+    float pad[2];
+};
+static_assert(sizeof(HelioMix) == 80, "HelioMix is 20 words");
+
+struct MixArgs {
+    float e_dir, e_pos, e_cell;
+    float e_dr;        // |dr - dr_model| (2-norm): 2^-19 (theta_max + 2 theta_s,max)
+    float buie_vmax;   // the largest float value below the top of the Buie table's image
+    const HelioMix *hmix;
+    v3 s_u, s_n, s_v;  // the model's sun frame rounded (identity axes for an identity frame)
+    v3 sun;            // the sun direction rounded (the energy's)
+    // rectangle receiver: unit normal, E2 x E1 (det = d.nr), dual vectors of E1 and E2 in the
+    // plane (u = uF + (P - F).g1, v = vF + (P - F).g2) with their 1-norms
+    v3 nn, nr, g1, g2;
+    float uF, vF, g1n, g2n, nrn;
+    // cylinder: radius, height, half height (F is the axis point at mid-height)
+    float cyl_r, cyl_h;
+    uint2 *redo;       // [redo_cap] (heliostat, ray of that heliostat) to trace again in FP64
+    uint32_t *redo_n;  // [0]: rays put on the list this launch (may exceed redo_cap);
+                       // [1]: rays put on the lists of the scene's last mixed trace;
+                       // [2]: launches whose list filled up (traced again wholly in FP64)
+    int redo_cap;
 };
 
 // One map of a batch for the decode (reduce_batch_kernel): packed copies and side maps,
@@ -271,6 +325,17 @@ struct om_scene {
     double *d_tail64 = nullptr;            // Trace64Args::tail
     std::vector<om::Geo64 *> d_cell_geo64; // per grid: this scene's double vertices of its CSR entries
     om::Trace64Args a64{};
+    // OM_PREC_MIXED: per grid, float images of the double CSR vertices with their edge
+    // lengths ((A, id bits), (E1, |E1|), (E2, |E2|)); the double Buie table rounded to float
+    // with its split guide; the error bounds; the redo counters (MixArgs::redo_n)
+    std::vector<float4 *> d_cell_geo_m;
+    om::HelioMix *d_hmix = nullptr;
+    om::MixArgs mix{};       // the scene's part of the mixed kernels' arguments
+    float4 *d_buie_bins_m = nullptr;
+    int2 *d_buie_split_m = nullptr;
+    float buie_A_m = 0.0f, buie_vmax_m = 0.0f, buie_gscale_m = 0.0f;
+    float mix_e_dir = 0.0f, mix_e_pos = 0.0f;
+    uint32_t *d_mix_n = nullptr;
     // forks (om_scene_fork): the root scene whose grid, microsamples, tiles and Buie tables
     // this scene's device pointers share, and whose host CSR it reads; null for a root
     const om_scene *grid_src = nullptr;
@@ -296,6 +361,18 @@ om_status launch_erfinv_ray(const uint32_t *in, double *out, int n, hipStream_t 
 om_status build_buie_tail64(double *out, uint32_t k0, uint32_t n, double A, double int465, double B, double C,
                             double inv_gp1);
 om_status launch_trace64(om_scene *s, const TraceArgs &a, const Trace64Args &b, bool records, hipStream_t st);
+// OM_PREC_MIXED: the kernel arguments of s for grid g (the FP32 fields of a replaced by the
+// float images of the FP64 scene); whether s can take the mixed path (else it traces in FP64)
+void mixed_args(const om_scene *s, int g, TraceArgs &a, MixArgs &x);
+bool mixed_supported(const om_scene *s);
+// the window launch (nwg persistent workgroups) and the FP64 trace of its redo list; the
+// per-ray records (rays on the redo list traced in FP64 in place)
+// acc: the packed copies and side maps of the launch (acc_words words), cleared when its redo
+// list fills up
+om_status launch_trace_mixed_window(om_scene *s, const TraceArgs &a, const Trace64Args &b, const MixArgs &x, int nwg,
+                                    uint32_t *acc, long long acc_words, hipStream_t st);
+om_status launch_trace_mixed_records(om_scene *s, const TraceArgs &a, const Trace64Args &b, const MixArgs &x,
+                                     hipStream_t st);
 // the FP64 trace with the LDS-window accumulation (nwg persistent workgroups)
 om_status launch_trace64_window(om_scene *s, const TraceArgs &a, const Trace64Args &b, int nwg, hipStream_t st);
 om_status launch_finalize(const unsigned long long *energy, float *raw, float *flux, float scale,

@@ -1406,7 +1406,15 @@ struct WsPlan {
     size_t n16 = 0, bytes = 0, off_win = 0;
     WindowRef win{nullptr, 0, 0, 0, 0, 0, 0};
     int nwg = 0;
+    size_t off_redo = 0; // OM_PREC_MIXED window traces: the redo list of redo_cap entries
+    int redo_cap = 0;
 };
+
+// whether a trace of s takes the mixed path (else OM_PREC_MIXED traces in FP64)
+static bool mixed_window(const om_scene *s)
+{
+    return s->prm.precision == OM_PREC_MIXED && s->prm.accumulate == OM_ACC_WINDOW && mixed_supported(s);
+}
 
 static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
 {
@@ -1419,14 +1427,26 @@ static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
     }
     int shift = 0;
     while (shift < COPY_SHIFT_MAX && ((size_t)16 << shift) * npx * sizeof(unsigned long long) <= copy_budget) ++shift;
-    for (;; --shift) {
-        const size_t ncopy = (size_t)8 << shift;
-        p.shift = shift;
-        p.n16 = ncopy * npx;
-        p.off_win = p.n16 * sizeof(unsigned long long) + 2 * npx * sizeof(uint32_t);
-        p.bytes = p.off_win + (p.nwg > 0 ? (size_t)p.nwg * p.win.w * p.win.h * sizeof(unsigned long long) : 0);
-        if (p.bytes <= cap) return OM_OK;
-        if (shift == 0) break;
+    // the redo list of a mixed trace: one entry per 64 rays, 2^16 to 2^22 entries (a full
+    // list sends its further rays to the FP64 trace inside the window kernel)
+    long long want = !mixed_window(s) ? 0
+                     : s->prm.redo_cap > 0 ? s->prm.redo_cap
+                                           : std::min<long long>(std::max<long long>(s->total_rays / 64, 1 << 16), 1 << 22);
+    for (;;) {
+        for (int sh = shift;; --sh) {
+            const size_t ncopy = (size_t)8 << sh;
+            p.shift = sh;
+            p.n16 = ncopy * npx;
+            p.off_win = p.n16 * sizeof(unsigned long long) + 2 * npx * sizeof(uint32_t);
+            p.bytes = p.off_win + (p.nwg > 0 ? (size_t)p.nwg * p.win.w * p.win.h * sizeof(unsigned long long) : 0);
+            p.off_redo = (p.bytes + 255) & ~(size_t)255;
+            p.redo_cap = (int)want;
+            if (want > 0) p.bytes = p.off_redo + (size_t)want * sizeof(uint2);
+            if (p.bytes <= cap) return OM_OK;
+            if (sh == 0) break;
+        }
+        if (want <= 4096 || s->prm.redo_cap > 0) break;
+        want /= 2;
     }
     set_error("the trace needs " + std::to_string(p.bytes) + " bytes of workspace, more than the " +
               std::to_string(cap) + " available");
@@ -1457,7 +1477,10 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
     a.ntiles = t1 - t0;
     const int npx = s->res_x * s->res_y;
     om_status rc;
-    const bool fp64 = s->prm.precision == OM_PREC_FP64;
+    // OM_PREC_MIXED takes the mixed window trace; without the window accumulation, or where
+    // the mixed path does not apply, it traces in FP64 (the same outcome for every ray)
+    const bool mixed = mixed_window(s);
+    const bool fp64 = s->prm.precision == OM_PREC_FP64 || (s->prm.precision == OM_PREC_MIXED && !mixed);
     if (s->prm.accumulate == OM_ACC_PACKED || s->prm.accumulate == OM_ACC_WINDOW) {
         const int ncopy = 8 << p.shift;
         a.part = (unsigned long long *)base;
@@ -1487,7 +1510,16 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
                                          st)) != OM_OK)
                 return rc;
             set_current_queue(s->ctx, a.win_queue);
-            rc = fp64 ? launch_trace64_window(s, a, trace64_args(s, g), p.nwg, st) : launch_trace_window(s, a, p.nwg, st);
+            if (mixed) {
+                MixArgs x;
+                mixed_args(s, g, a, x);
+                x.redo = (uint2 *)(base + p.off_redo);
+                x.redo_cap = p.redo_cap;
+                rc = launch_trace_mixed_window(s, a, trace64_args(s, g), x, p.nwg, (uint32_t *)base,
+                                               (long long)(p.off_win / 4), st);
+            } else {
+                rc = fp64 ? launch_trace64_window(s, a, trace64_args(s, g), p.nwg, st) : launch_trace_window(s, a, p.nwg, st);
+            }
             if (rc != OM_OK) return rc;
         } else if ((rc = launch_trace(s, a, false, st)) != OM_OK) {
             return rc;
@@ -1527,6 +1559,8 @@ om_status trace_into(om_scene *s, int h0, int h1, const om_flux *out, bool clear
         if (clear_bytes > 0 && (rc = launch_fill_u32((uint32_t *)base, (long long)(clear_bytes / 4), st)) != OM_OK)
             return rc;
     }
+    // the count of rays the mixed trace sends to FP64 starts again
+    if (mixed_window(s) && (rc = launch_fill_u32(s->d_mix_n + 1, 2, st)) != OM_OK) return rc;
     // one launch (and decode, which adds into out) per grid: a grid's CSR holds its own
     // heliostats only
     for (size_t g = 0; g < s->grids.size(); ++g) {
@@ -1732,6 +1766,10 @@ extern "C" om_status om_trace_records(om_scene *s, int h0, int h1, int32_t *shad
     OM_HIP(hipSetDevice(s->ctx->device));
     // records never touch the maps; any valid pointers do
     if (om_status st = ensure_buf(s->ctx->scratch2, 16 * (size_t)s->res_x * s->res_y); st != OM_OK) return st;
+    const bool mixed = s->prm.precision == OM_PREC_MIXED && mixed_supported(s);
+    if (mixed) {
+        if (om_status st = launch_fill_u32(s->d_mix_n + 1, 2, (hipStream_t)stream); st != OM_OK) return st;
+    }
     for (size_t g = 0; g < s->grids.size(); ++g) {
         const int a0 = std::max(h0, s->grids[g].h0), a1 = std::min(h1, s->grids[g].h1);
         if (a0 >= a1) continue;
@@ -1746,11 +1784,31 @@ extern "C" om_status om_trace_records(om_scene *s, int h0, int h1, int32_t *shad
         a.rec_block = block;
         a.rec_pixel = pixel;
         a.rec_base = s->ray_offset[h0];
-        const om_status st = s->prm.precision == OM_PREC_FP64
-                                 ? launch_trace64(s, a, trace64_args(s, (int)g), true, (hipStream_t)stream)
-                                 : launch_trace(s, a, true, (hipStream_t)stream);
+        om_status st;
+        if (mixed) {
+            MixArgs x;
+            mixed_args(s, (int)g, a, x);
+            st = launch_trace_mixed_records(s, a, trace64_args(s, (int)g), x, (hipStream_t)stream);
+        } else if (s->prm.precision != OM_PREC_FP32) {
+            st = launch_trace64(s, a, trace64_args(s, (int)g), true, (hipStream_t)stream);
+        } else {
+            st = launch_trace(s, a, true, (hipStream_t)stream);
+        }
         if (st != OM_OK) return st;
     }
+    return OM_OK;
+}
+
+extern "C" om_status om_scene_redo_count(const om_scene *s, int64_t *rays)
+{
+    if (!s || !rays) return OM_ERR_INVALID;
+    *rays = 0;
+    if (!s->ctx || !s->d_mix_n) return OM_OK;
+    OM_HIP(hipSetDevice(s->ctx->device));
+    OM_HIP(hipDeviceSynchronize());
+    uint32_t n = 0;
+    OM_HIP(hipMemcpy(&n, s->d_mix_n + 1, sizeof(n), hipMemcpyDeviceToHost));
+    *rays = n;
     return OM_OK;
 }
 

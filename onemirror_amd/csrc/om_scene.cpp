@@ -1181,6 +1181,171 @@ D3 dworld(D3 d, const DFrame &f)
               d.x * f.u.z + d.y * f.n.z + d.z * f.v.z);
 }
 
+// The float nearest x from above (a magnitude the error bounds may overstate, never understate)
+float up_f(double x)
+{
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+// OM_PREC_MIXED: float images of the FP64 scene for the FP32 walks and sampling, and the
+// scene's error bounds (MixArgs).  geo: each heliostat's double vertices; cx, cy, sl: the
+// double Buie table.
+// OM_PREC_MIXED: per heliostat and for the receiver, the values the mixed kernels take from the
+// FP64 model (om_internal.h, HelioMix and MixArgs): frames, the nominal reflected direction
+// and the nominal ray's point nearest the receiver's reference point F (the rectangle's
+// centre, the cylinder's axis point at mid-height), all formed in double and rounded once.
+void mix_helio(om_scene *s, const std::vector<om::Helio64> &hel, const DFrame &sf, D3 F, std::vector<om::HelioMix> &out)
+{
+    const D3 i0 = sf.ident ? d3(0, 1, 0) : sf.n;
+    out.assign((size_t)std::max(s->H, 1), om::HelioMix{});
+    for (int i = 0; i < s->H; ++i) {
+        const om::Helio64 &h = hel[(size_t)i];
+        D3 u = d3(h.u[0], h.u[1], h.u[2]), n = d3(h.n[0], h.n[1], h.n[2]), v = d3(h.v[0], h.v[1], h.v[2]);
+        if (h.ident) {
+            u = d3(1, 0, 0);
+            n = d3(0, 1, 0);
+            v = d3(0, 0, 1);
+        }
+        const double c0 = ddot(n, i0);
+        const D3 a = dsub(i0, dscl(n, 2.0 * c0));
+        const D3 pos = d3(h.pos[0], h.pos[1], h.pos[2]);
+        const double D = ddot(dsub(F, pos), a);
+        const D3 qf = dsub(dadd(pos, dscl(a, D)), F);
+        om::HelioMix &m = out[(size_t)i];
+        const D3 vs[5] = {u, n, v, a, qf};
+        float *dst[5] = {m.u, m.n, m.v, m.a, m.qf};
+        for (int k = 0; k < 5; ++k) {
+            dst[k][0] = (float)vs[k].x;
+            dst[k][1] = (float)vs[k].y;
+            dst[k][2] = (float)vs[k].z;
+        }
+        m.c0 = (float)c0;
+        m.D = (float)D;
+        m.e0 = (float)ddot(n, sf.n);
+    }
+}
+
+om::v3 fv3(D3 a) { return om::mk((float)a.x, (float)a.y, (float)a.z); }
+float norm1_up(D3 a) { return up_f((std::fabs(a.x) + std::fabs(a.y) + std::fabs(a.z)) * (1.0 + 1e-6)); }
+
+om_status upload_mixed(om_scene *s, const std::vector<om::Helio64> &hel, const std::vector<om::Geo64> &geo,
+                       const std::vector<double> &cx, const std::vector<double> &cy, const std::vector<double> &sl,
+                       const DFrame &sf, D3 focus, const D3 *rv)
+{
+    om_status st;
+    om::MixArgs &x = s->mix;
+    x = om::MixArgs{};
+    // sun frame (the identity axes for an identity frame: world() is then the identity)
+    x.s_u = sf.ident ? om::mk(1, 0, 0) : fv3(sf.u);
+    x.s_n = sf.ident ? om::mk(0, 1, 0) : fv3(sf.n);
+    x.s_v = sf.ident ? om::mk(0, 0, 1) : fv3(sf.v);
+    x.sun = fv3(sf.n);
+    // receiver: the reference point F and the plane's constants (rectangle) or the axis
+    // point at mid-height (cylinder)
+    D3 F = focus;
+    if (s->recv_type == 0) {
+        const D3 A = rv[0], E1 = dsub(rv[1], rv[0]), E2 = dsub(rv[3], rv[0]);
+        const D3 nr = dcross(E2, E1), nn = dnorm(nr);
+        const D3 g1r = dcross(E2, nn), g2r = dcross(nn, E1);
+        const D3 g1 = dscl(g1r, 1.0 / ddot(E1, g1r)), g2 = dscl(g2r, 1.0 / ddot(E2, g2r));
+        x.nn = fv3(nn);
+        x.nr = fv3(nr);
+        x.g1 = fv3(g1);
+        x.g2 = fv3(g2);
+        x.uF = (float)ddot(dsub(F, A), g1);
+        x.vF = (float)ddot(dsub(F, A), g2);
+        x.g1n = norm1_up(g1);
+        x.g2n = norm1_up(g2);
+        x.nrn = norm1_up(nr);
+    } else if (s->recv_type == 1) {
+        F = dload(s->recv_pos);
+        x.cyl_r = (float)s->recv_size[0];
+        x.cyl_h = (float)s->recv_size[1];
+    }
+    std::vector<om::HelioMix> hm;
+    mix_helio(s, hel, sf, F, hm);
+    if ((st = upload(s->d_hmix, hm.data(), hm.size())) != OM_OK) return st;
+    x.hmix = s->d_hmix;
+    // Origins lie on the mirrors, so no coordinate of an origin or a vertex exceeds the
+    // vertices' largest; the float of each is within 2^-24 of it per component, and the two
+    // together within 2^-22 of it (2-norm, with a margin).
+    double rmax = 1.0;
+    for (int i = 0; i < s->H; ++i) {
+        const om::Geo64 &G = geo[(size_t)i];
+        for (int c = 0; c < 4; ++c)
+            for (int k = 0; k < 3; ++k)
+                rmax = std::max(rmax, std::fabs(G.A[k] + ((c & 1) ? G.E1[k] : 0.0) + ((c & 2) ? G.E2[k] : 0.0)));
+    }
+    s->mix_e_pos = up_f(std::ldexp(rmax, -22));
+    // Directions from the FP32 draws: the sunshape polar angle theta (at most 9.3 mrad, the
+    // Buie tail's end, or the pillbox half angle) and the slope-error angle theta_s (at most
+    // 5.3 sigma, the deviate of the extreme draw 1 - 2^-24) enter as sines and cosines of
+    // relative error a few 2^-24 and as products with the double frames, so a direction is
+    // within about 2^-21 (theta + 2 theta_s) of the model's (the reflection doubles the
+    // normal's error); the bound takes four times that.
+    const double th_max = s->prm.sunshape == OM_SUN_BUIE ? 9.3e-3 : (double)s->prm.pillbox_half_angle;
+    const double ths_max = 5.5 * (double)s->prm.slope_sigma;
+    s->mix_e_dir = up_f(std::ldexp(th_max + 2.0 * ths_max, -19) + std::ldexp(1.0, -40));
+    // the deviations dr carry that error; a direction a + dr, rounded to float in the walks,
+    // adds three roundings of its components (within 2^-21 in 2-norm)
+    x.e_dr = s->mix_e_dir;
+    x.e_dir = s->mix_e_dir + std::ldexp(1.0f, -21);
+    x.e_pos = s->mix_e_pos;
+    // the double Buie table rounded to float, with the split guide of the FP32 lookup
+    dfree(s->d_buie_bins_m);
+    dfree(s->d_buie_split_m);
+    const int ng = s->nbins;
+    if (s->prm.sunshape == OM_SUN_BUIE && ng > 0) {
+        std::vector<float> fy((size_t)ng + 1);
+        for (int i = 0; i <= ng; ++i) fy[i] = (float)cy[i];
+        std::vector<float4> bins((size_t)ng);
+        for (int i = 0; i < ng; ++i) bins[i] = make_float4(fy[i], fy[i + 1], (float)sl[i], (float)cx[i]);
+        const float top = fy[ng];
+        s->buie_gscale_m = (float)ng / top;
+        std::vector<int2> split((size_t)ng);
+        for (int j = 0, i = 0; j < ng; ++j) {
+            const float vj = (float)j / s->buie_gscale_m;
+            while (i < ng - 1 && fy[i + 1] <= vj) ++i;
+            const float edge = fy[i + 1];
+            int bits;
+            std::memcpy(&bits, &edge, 4);
+            split[j] = make_int2(i, bits);
+        }
+        s->buie_A_m = (float)s->a64.A;
+        // the draws below the tail (k < tail_k0) have u A < int465 in double; their float
+        // value is held below the table's top so that the lookup stays in the table
+        s->buie_vmax_m = std::nextafter(top, 0.0f);
+        if ((st = upload(s->d_buie_bins_m, bins.data(), bins.size())) != OM_OK) return st;
+        if ((st = upload(s->d_buie_split_m, split.data(), split.size())) != OM_OK) return st;
+    }
+    // the candidates of every grid cell in CSR order: (A, id bits), (E1, |E1|), (E2, |E2|)
+    const om_scene *root = s->grid_src ? s->grid_src : s;
+    for (float4 *&p : s->d_cell_geo_m) dfree(p);
+    s->d_cell_geo_m.assign(s->grids.size(), nullptr);
+    for (size_t k = 0; k < s->grids.size(); ++k) {
+        const std::vector<int> &items = root->grids[k].cell_items;
+        std::vector<float4> cg(3 * items.size());
+        for (size_t i = 0; i < items.size(); ++i) {
+            const int h = items[i];
+            const om::Geo64 &G = geo[(size_t)h];
+            const double l1 = std::sqrt(G.E1[0] * G.E1[0] + G.E1[1] * G.E1[1] + G.E1[2] * G.E1[2]);
+            const double l2 = std::sqrt(G.E2[0] * G.E2[0] + G.E2[1] * G.E2[1] + G.E2[2] * G.E2[2]);
+            cg[3 * i] = make_float4((float)G.A[0], (float)G.A[1], (float)G.A[2], 0.0f);
+            std::memcpy(&cg[3 * i].w, &h, 4);
+            cg[3 * i + 1] = make_float4((float)G.E1[0], (float)G.E1[1], (float)G.E1[2], up_f(l1 * (1.0 + 1e-6)));
+            cg[3 * i + 2] = make_float4((float)G.E2[0], (float)G.E2[1], (float)G.E2[2], up_f(l2 * (1.0 + 1e-6)));
+        }
+        if ((st = upload(s->d_cell_geo_m[k], cg.data(), cg.size())) != OM_OK) return st;
+    }
+    if (!s->d_mix_n) {
+        const uint32_t zero[3] = {0u, 0u, 0u};
+        if ((st = upload(s->d_mix_n, zero, 3)) != OM_OK) return st;
+    }
+    return OM_OK;
+}
+
 om_status upload_fp64(om_scene *s)
 {
     om::Trace64Args &q = s->a64;
@@ -1364,12 +1529,13 @@ om_status upload_fp64(om_scene *s)
         for (size_t i = 0; i < items.size(); ++i) cg[i] = geo[(size_t)items[i]];
         if ((st = upload(s->d_cell_geo64[k], cg.data(), cg.size())) != OM_OK) return st;
     }
+    if (s->prm.precision == OM_PREC_MIXED) return upload_mixed(s, hel, geo, cx, cy, sl, sf, focus, rv);
     return OM_OK;
 }
 
 om_status upload_geometry(om_scene *s)
 {
-    if (s->prm.precision == OM_PREC_FP64) {
+    if (s->prm.precision == OM_PREC_FP64 || s->prm.precision == OM_PREC_MIXED) {
         om_status st = upload_fp64(s);
         if (st != OM_OK) return st;
     }
@@ -1413,6 +1579,11 @@ void free_scene_dev(om_scene *s)
     dfree(s->d_f64);
     dfree(s->d_tail64);
     for (om::Geo64 *&p : s->d_cell_geo64) dfree(p);
+    for (float4 *&p : s->d_cell_geo_m) dfree(p);
+    dfree(s->d_hmix);
+    dfree(s->d_buie_bins_m);
+    dfree(s->d_buie_split_m);
+    dfree(s->d_mix_n);
     dfree(s->d_helio);
     dfree(s->d_geo);
     for (float4 *&p : s->d_cell_geo) dfree(p);
@@ -1508,8 +1679,12 @@ om_status validate(const om_scene_desc *d, const om_params *p)
         om::set_error("unknown microsample mode");
         return OM_ERR_INVALID;
     }
-    if (p->precision != OM_PREC_FP32 && p->precision != OM_PREC_FP64) {
-        om::set_error("precision must be OM_PREC_FP32 or OM_PREC_FP64");
+    if (p->precision != OM_PREC_FP32 && p->precision != OM_PREC_FP64 && p->precision != OM_PREC_MIXED) {
+        om::set_error("precision must be OM_PREC_FP32, OM_PREC_FP64 or OM_PREC_MIXED");
+        return OM_ERR_INVALID;
+    }
+    if (p->redo_cap < 0) {
+        om::set_error("redo_cap must be 0 or positive");
         return OM_ERR_INVALID;
     }
     if (p->ray_compact != 0 && p->ray_compact != 1) {
@@ -1950,6 +2125,11 @@ extern "C" om_status om_scene_fork(const om_scene *src, const float sun_dir[3], 
     s->d_f64 = nullptr;
     s->d_tail64 = nullptr;
     s->d_cell_geo64.assign(ng, nullptr);
+    s->d_cell_geo_m.assign(ng, nullptr);
+    s->d_hmix = nullptr;
+    s->d_buie_bins_m = nullptr;
+    s->d_buie_split_m = nullptr;
+    s->d_mix_n = nullptr;
     s->d_cell_geo.assign(ng, nullptr);
     s->diff_buf = om::DevBuf{};
     s->prm.scene_index = scene_index;

@@ -13,6 +13,9 @@
 // scene_instance_process.cu:6-226, receiver_drawing (:135-153).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "om_window.h"
 
 namespace om {
@@ -330,7 +333,10 @@ __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dfr
                              int h, int m, int l, int self, bool clear)
 {
     Ray64 r{-1, -2, -1, make_int2(-1, -1), 0u};
-    const u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3}, a.key0, a.key1);
+    // the key words read as wave-uniform values (in ray64_at, an out-of-line function, a's
+    // fields arrive through a vector address)
+    const u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3},
+                                    __builtin_amdgcn_readfirstlane(a.key0), __builtin_amdgcn_readfirstlane(a.key1));
     const double u0 = u01(x.x), u1 = u01(x.y), u2 = u01(x.z), u3 = u01(x.w);
     double th;
     if (SUN == OM_SUN_PILLBOX) {
@@ -518,6 +524,8 @@ template <int SUN, bool POISSON>
 __global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
 {
     const Trace64Launch &args = *(const Trace64Launch *)__builtin_amdgcn_kernarg_segment_ptr();
+    // the FP64 trace of a mixed launch whose redo list filled up: nothing to do otherwise
+    if (args.b.gate && *args.b.gate <= args.b.gate_cap) return;
     __shared__ unsigned long long s_win[WIN_PX_MAX];
     __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
     window_body(args.a, s_win, s_out,
@@ -526,7 +534,556 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
                 });
 }
 
+// ---------------------------------------------------------------------------------------
+// OM_PREC_MIXED: the FP64 model's outcome of every ray at close to FP32 cost
+// ---------------------------------------------------------------------------------------
+// A ray is sampled in FP32 (the same Philox draws); its sun direction, perturbed normal and
+// reflection are formed in double with the model's frames, and its receiver hit in double
+// with the model's receiver.  Both walks run in FP32 over float images of the model's
+// vertices.  Each FP32 decision is taken only when it holds for every value within its
+// error bound (MixArgs); a ray with a decision inside its bound is put on a list and traced
+// by the FP64 trace afterwards.  So every ray's shading heliostat, blocking heliostat and
+// pixel are the model's; its energy |sun.n| is the model's to the FP32 draws' error.
+
+constexpr int MIX_UNSURE = -3;
+constexpr float U24 = 5.9604644775390625e-08f; // 2^-24
+constexpr int REDO_BLOCKS = 1024;              // workgroups of the redo list's FP64 trace
+
+struct MixLaunch {
+    TraceArgs a; // its FP32 scene fields hold the float images of the FP64 scene (mixed_args)
+    Trace64Args b;
+    MixArgs x;
+};
+
+__device__ __forceinline__ v3 f3(d3 v) { return mk((float)v.x, (float)v.y, (float)v.z); }
+
+// rayParallelogramIntersect (global_function.cuh:78-105) in FP32, certified against the same
+// test in double on the model's ray: 1 hit, 0 miss, -1 when a tested quantity lies within its
+// error bound of its threshold.  With T = o - A, c0 = e_pos + |T| (e_dir + 11 2^-24) (the
+// error of T and of the ray's position along T) and c1 = e_dir + 8 2^-24 (the direction's):
+//   |det - det_model| <= |E1||E2| c1
+//   |u - u_model| <= (|E2| c0 + |u| |E1||E2| c1) / |det| + 4 2^-24 |u|
+//   |v - v_model| <= (|E1| c0 + |v| |E1||E2| c1) / |det| + 4 2^-24 |v|
+//   |t - t_model| <= (|E1||E2| c0 + |t| |E1||E2| c1) / |det| + 4 2^-24 |t|
+// (first order; the reciprocal is the hardware estimate, within 2^-23).  |T| is bounded by
+// its 1-norm and |E1|, |E2| (rounded up) ride in the w components of the vertex records.
+__device__ __forceinline__ int parallelogram_cert(const MixArgs &x, v3 o, v3 d, const float4 &A, const float4 &E1,
+                                                  const float4 &E2)
+{
+    const v3 e1 = mk(E1.x, E1.y, E1.z), e2 = mk(E2.x, E2.y, E2.z);
+    const v3 p = cross(d, e2);
+    const float det = dot(e1, p);
+    const float inv = __builtin_amdgcn_rcpf(det);
+    const v3 T = o - mk(A.x, A.y, A.z);
+    const v3 q = cross(T, e1);
+    const float u = dot(T, p) * inv, v = dot(d, q) * inv, t = dot(e2, q) * inv;
+    const float tn = fabs_(T.x) + fabs_(T.y) + fabs_(T.z);
+    const float c0 = fma_(tn, x.e_dir + 11.0f * U24, x.e_pos);
+    const float mdet = E1.w * E2.w * (x.e_dir + 8.0f * U24);
+    const float ai = fabs_(inv), adet = fabs_(det);
+    const float au = fabs_(u), av = fabs_(v), at = fabs_(t);
+    const float mu = fma_(au, mdet, E2.w * c0) * ai + 4.0f * U24 * au;
+    const float mv = fma_(av, mdet, E1.w * c0) * ai + 4.0f * U24 * av;
+    const float mt = fma_(at, mdet, E1.w * E2.w * c0) * ai + 4.0f * U24 * at;
+    const bool miss = adet < 1e-6f - mdet || u < -mu || u > 1.0f + mu || v < -mv || v > 1.0f + mv || t < 1e-6f - mt;
+    const bool hit = adet >= 1e-6f + mdet && u >= mu && u <= 1.0f - mu && v >= mv && v <= 1.0f - mv && t >= 1e-6f + mt;
+    return hit ? 1 : (miss ? 0 : -1);
+}
+
+// The origin of a microsample's rays: the float origin o = pos + w with w = o - pos formed in
+// float from the microsample's local position and the model's frame, and the grid cell of
+// o with its float offset from the grid position.  The model's cell is the truncation of
+// (o - gpos) / edge in double; the float one is certain when each axis's quotient lies
+// farther from an integer than its error (e_pos and the roundings, over the edge).  A
+// microsample's facet is certain likewise (the quotient of the Poisson point by the facet
+// length); ok is false when either is not, and every ray of the microsample goes to FP64.
+struct MixStart {
+    v3 of, w;
+    cell3 c;
+    bool ok;
+};
+template <bool POISSON>
+__device__ __forceinline__ MixStart mix_start(const TraceArgs &a, const MixArgs &x, const HelioRec *hr,
+                                              const HelioMix &hm, int h, int m)
+{
+    MixStart s;
+    s.ok = true;
+    v3 loc;
+    if (POISSON) {
+        const float2 pt = a.poisson[(size_t)h * a.poisson_cap + m];
+        int br = 0, bc = 0;
+        if (hr->rows > 1) {
+            const float q = pt.y / hr->subrow_len, g = fabs_(q - rintf(q));
+            br = trunc_i(q);
+            s.ok = s.ok && g > fma_(fabs_(q), 4.0f * U24, 1e-6f);
+        }
+        if (hr->cols > 1) {
+            const float q = pt.x / hr->subcol_len, g = fabs_(q - rintf(q));
+            bc = trunc_i(q);
+            s.ok = s.ok && g > fma_(fabs_(q), 4.0f * U24, 1e-6f);
+        }
+        loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
+    } else {
+        const int row = m / hr->cols_all, col = m - row * hr->cols_all;
+        const int br = row / hr->sub_rows, bc = col / hr->sub_cols;
+        loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
+                 (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
+    }
+    const v3 u = mk(hm.u[0], hm.u[1], hm.u[2]), n = mk(hm.n[0], hm.n[1], hm.n[2]), v = mk(hm.v[0], hm.v[1], hm.v[2]);
+    s.w = loc.x * u + loc.y * n + loc.z * v;
+    s.of = mk(hr->pos[0], hr->pos[1], hr->pos[2]) + s.w;
+    s.c.rx = s.of.x - a.gpx;
+    s.c.ry = s.of.y - a.gpy;
+    s.c.rz = s.of.z - a.gpz;
+    const float qx = div_rn(s.c.rx, a.gix), qy = div_rn(s.c.ry, a.giy), qz = div_rn(s.c.rz, a.giz);
+    s.c.gx = trunc_i(qx);
+    s.c.gy = trunc_i(qy);
+    s.c.gz = trunc_i(qz);
+    s.c.addr = (s.c.gx * a.gny + s.c.gy) * a.gnz + s.c.gz;
+    const float ex = fma_(fabs_(s.c.rx), 2.0f * U24, x.e_pos), ey = fma_(fabs_(s.c.ry), 2.0f * U24, x.e_pos),
+                ez = fma_(fabs_(s.c.rz), 2.0f * U24, x.e_pos);
+    s.ok = s.ok && fabs_(qx - rintf(qx)) > fma_(qx, 2.0f * U24, div_rn(ex, a.gix)) &&
+           fabs_(qy - rintf(qy)) > fma_(qy, 2.0f * U24, div_rn(ey, a.giy)) &&
+           fabs_(qz - rintf(qz)) > fma_(qz, 2.0f * U24, div_rn(ez, a.giz));
+    return s;
+}
+
+// NotCollision (rectgrid_raytracing.cu:38-117) in FP32 with certified decisions, in the form
+// of dda (om_device.h): the first candidate hit, -1 for none, MIX_UNSURE.  c: the model's
+// start cell.  A step's axis is certain when the chosen crossing parameter m_c lies below
+// each other axis's m_j by more than the sum of their bounds
+//   |m_k - m_k,model| <= e_cell / |d_k| + m_k (e_dir / |d_k| + (n + 4) 2^-24)
+// (the numerator's error, the direction's, the division and the n additions of the walk so
+// far).  A direction component within 2 e_dir + 1e-6 of zero could differ from the model's in
+// sign or in absDivide's 1e-6 test: such a ray is unsure.
+__device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3 &c, v3 d, int self)
+{
+    const bool fx = d.x >= 0.0f, fy = d.y >= 0.0f, fz = d.z >= 0.0f;
+    int nx = fx ? a.occ_hi[0] - c.gx : c.gx - a.occ_lo[0];
+    int ny = fy ? a.occ_hi[1] - c.gy : c.gy - a.occ_lo[1];
+    int nz = fz ? a.occ_hi[2] - c.gz : c.gz - a.occ_lo[2];
+    const float near0 = 2.0f * x.e_dir + 1.0001e-6f;
+    if (fabs_(d.x) <= near0 || fabs_(d.y) <= near0 || fabs_(d.z) <= near0) return MIX_UNSURE;
+    if ((nx | ny | nz) < 0) return -1;
+    if ((nx | ny | nz) == 0) return -1;
+    float mx = first_crossing(d.x, fx, a.gix, c.gx, c.rx);
+    float my = first_crossing(d.y, fy, a.giy, c.gy, c.ry);
+    float mz = first_crossing(d.z, fz, a.giz, c.gz, c.rz);
+    const float dx = cell_crossing(d.x, a.gix), dy = cell_crossing(d.y, a.giy), dz = cell_crossing(d.z, a.giz);
+    // 1 / |d_k| from the estimate, rounded up by more than its error
+    const float rx = __builtin_amdgcn_rcpf(fabs_(d.x)) * 1.0000005f;
+    const float ry = __builtin_amdgcn_rcpf(fabs_(d.y)) * 1.0000005f;
+    const float rz = __builtin_amdgcn_rcpf(fabs_(d.z)) * 1.0000005f;
+    const float ax = x.e_cell * rx, ay = x.e_cell * ry, az = x.e_cell * rz;
+    const float bx = x.e_dir * rx, by = x.e_dir * ry, bz = x.e_dir * rz;
+    float sn = 4.0f * U24; // (n + 4) 2^-24
+    const int yz = a.gny * a.gnz;
+    const int sax = fx ? yz : -yz, say = fy ? a.gnz : -a.gnz, saz = fz ? 1 : -1;
+    int addr = c.addr;
+#define OM_CERT_STEP(more, sure)                                                                   \
+    do {                                                                                         \
+        const bool sx = mx < my && mx < mz;                                                      \
+        const bool sy = !(mx < my) && my < mz;                                                   \
+        const bool sz = !sx && !sy;                                                              \
+        const float ex = fma_(mx, bx + sn, ax), ey = fma_(my, by + sn, ay), ez = fma_(mz, bz + sn, az); \
+        const float mc = sx ? mx : (sy ? my : mz), ec = sx ? ex : (sy ? ey : ez);                \
+        sure = (sx || mx - mc > ex + ec) && (sy || my - mc > ey + ec) && (sz || mz - mc > ez + ec); \
+        sn += U24;                                                                               \
+        nx -= sx ? 1 : 0;                                                                        \
+        ny -= sy ? 1 : 0;                                                                        \
+        nz -= sz ? 1 : 0;                                                                        \
+        more = (nx | ny | nz) >= 0;                                                              \
+        addr += sx ? sax : (sy ? say : saz);                                                     \
+        mx = sx ? mx + dx : mx;                                                                  \
+        my = sy ? my + dy : my;                                                                  \
+        mz = sz ? mz + dz : mz;                                                                  \
+    } while (0)
+    bool first, sure;
+    OM_CERT_STEP(first, sure);
+    if (!sure) return MIX_UNSURE;
+    if (!first) return -1;
+    const int *cs = a.cell_start + (uint32_t)addr;
+    int i0 = cs[0], i1 = cs[1];
+    for (;;) {
+        // the next step and its range request before this cell's candidates (as dda); the
+        // step's certainty matters only when the walk goes on past this cell
+        bool more;
+        OM_CERT_STEP(more, sure);
+        int n0 = 0, n1 = 0;
+        if (more) {
+            const int *ns = a.cell_start + (uint32_t)addr;
+            n0 = ns[0];
+            n1 = ns[1];
+        }
+        for (int i = i0; i < i1; ++i) {
+            const float4 *g = a.cell_geo + 3u * (uint32_t)i;
+            const float4 A = g[0];
+            const int h = (int)f2u(A.w);
+            if (h == self) continue;
+            const int r = parallelogram_cert(x, o, d, A, g[1], g[2]);
+            if (r != 0) return r > 0 ? h : MIX_UNSURE;
+        }
+        if (!sure) return MIX_UNSURE;
+        if (!more) return -1;
+        i0 = n0;
+        i1 = n1;
+    }
+#undef OM_CERT_STEP
+}
+
+// distance of x from the nearest integer
+__device__ __forceinline__ float int_gapf(float x) { return fabs_(x - rintf(x)); }
+
+// sin x and cos x - 1 of a polar angle |x| < 0.25 (the series of sincos_small with the 1 of
+// the cosine left out, so that the deviation keeps its relative precision)
+__device__ __forceinline__ void sincos_m1(float x, float &s, float &cm1)
+{
+    const float x2 = x * x;
+    float ps = fma_(x2, -1.98412698e-04f, 8.33333333e-03f);
+    ps = fma_(x2, ps, -1.66666667e-01f);
+    s = fma_(x * x2, ps, x);
+    float pc = fma_(x2, 2.48015873e-05f, -1.38888889e-03f);
+    pc = fma_(x2, pc, 4.16666667e-02f);
+    pc = fma_(x2, pc, -0.5f);
+    cm1 = x2 * pc;
+}
+
+__device__ __forceinline__ float norm1(v3 a) { return fabs_(a.x) + fabs_(a.y) + fabs_(a.z); }
+
+// receiver_drawing (rectgrid_raytracing.cu:135-153) for the rectangle and the cylinder, from
+// the receiver's reference point F: the ray meets the receiver at F + c + tau d.  (row,
+// column), row -1 for a miss, row MIX_UNSURE when the hit or the pixel lies within its bound
+// of an edge.  bc bounds the error of c against the model's ray (its point nearest F); a
+// point tau along the ray is within bc + |tau| e_dir of the model's, and the hit on the
+// surface within that over the cosine of incidence.
+__device__ __forceinline__ int2 receiver_local(const TraceArgs &a, const MixArgs &x, v3 c, v3 d, float D, float bc)
+{
+    const int2 miss = make_int2(-1, -1), unsure = make_int2(MIX_UNSURE, 0);
+    const float rx = (float)a.res_x, ry = (float)a.res_y;
+    const float cn1 = norm1(c);
+    if (a.recv_type == 1) {
+        const float r = x.cyl_r;
+        const float A2 = d.x * d.x + d.z * d.z, B2 = c.x * d.x + c.z * d.z, C2 = c.x * c.x + c.z * c.z - r * r;
+        if (fabs_(A2 - 1e-6f) <= 4.0f * x.e_dir) return unsure;
+        if (A2 < 1e-6f) return miss;
+        // the line's distance from the axis, rho^2 = r^2 - disc / A2, against r
+        const float disc = B2 * B2 - A2 * C2;
+        const float ia = __builtin_amdgcn_rcpf(A2);
+        const float tc = fabs_(B2) * ia; // the parameter of the closest approach
+        const float dl = bc + tc * x.e_dir + fma_(cn1 + tc, 4.0f * U24, 0.0f) + r * x.e_dir * __builtin_amdgcn_rsqf(A2);
+        const float rho2 = r * r - disc * ia;
+        const float lo = r > dl ? (r - dl) * (r - dl) : 0.0f, hi = (r + dl) * (r + dl);
+        if (fabs_(rho2 - r * r) <= 8.0f * U24 * (r * r + fabs_(disc * ia)) || (rho2 >= lo && rho2 <= hi))
+            return unsure;
+        if (disc <= 0.0f) return miss;
+        const float sq = sqrt_rn(disc);
+        const float q = -(B2 + (B2 >= 0.0f ? sq : -sq));
+        const float ta = q * ia, tb = div_rn(C2, q);
+        const float tau = fminf(ta, tb);
+        const v3 W = c + tau * d;
+        const float t = D + tau;
+        const float at = fabs_(tau);
+        const float ci = fabs_(d.x * W.x + d.z * W.z) / r; // cosine of incidence (|d| = 1)
+        const float mw = div_rn(bc + at * x.e_dir, fmaxf(ci, 1e-30f)) + (cn1 + at) * 4.0f * U24;
+        if (fabs_(t - 1e-6f) <= mw + fabs_(t) * 4.0f * U24) return unsure;
+        if (t < 1e-6f) return miss;
+        const float Y = (W.y + 0.5f * x.cyl_h) / x.cyl_h * ry, mY = (mw + fabs_(W.y) * 4.0f * U24) / x.cyl_h * ry + ry * 4.0f * U24;
+        if (Y < -mY || Y > ry + mY) return miss;
+        float th = atan2_(W.z, W.x);
+        if (th < 0.0f) th += 6.28318548f;
+        const float X = th * 0.159154937f * rx;
+        const float mX = (mw / r + 8.0f * U24 * 6.2831853f) * 0.159154937f * rx + rx * 4.0f * U24;
+        if (int_gapf(Y) <= mY || int_gapf(X) <= mX || X >= rx - mX) return unsure;
+        return make_int2(min(trunc_i(Y), a.res_y - 1), min(trunc_i(X), a.res_x - 1));
+    }
+    const float dn = dot(d, x.nn), ad = fabs_(dn);
+    const float det = dot(d, x.nr);
+    if (fabs_(fabs_(det) - 1e-6f) <= x.nrn * (x.e_dir + 4.0f * U24)) return unsure;
+    if (fabs_(det) < 1e-6f) return miss;
+    const float tau = -dot(c, x.nn) * __builtin_amdgcn_rcpf(dn);
+    const v3 P = c + tau * d;
+    const float at = fabs_(tau);
+    const float mp = (bc + at * x.e_dir) * __builtin_amdgcn_rcpf(ad) * 1.0001f + (cn1 + at) * 8.0f * U24;
+    const float t = D + tau;
+    if (fabs_(t - 1e-6f) <= mp + fabs_(t) * 4.0f * U24) return unsure;
+    const float pn = norm1(P);
+    const float U = (x.uF + dot(P, x.g1)) * ry, V = (x.vF + dot(P, x.g2)) * rx;
+    const float mU = ((mp + pn * 8.0f * U24) * x.g1n + 4.0f * U24) * ry + ry * 4.0f * U24;
+    const float mV = ((mp + pn * 8.0f * U24) * x.g2n + 4.0f * U24) * rx + rx * 4.0f * U24;
+    if (U < -mU || U > ry + mU || V < -mV || V > rx + mV || t < 1e-6f) return miss;
+    if (int_gapf(U) <= mU || int_gapf(V) <= mV) return unsure;
+    return make_int2(min(trunc_i(U), a.res_y - 1), min(trunc_i(V), a.res_x - 1));
+}
+
+struct MixRay {
+    int shade, block; // as Ray64; shade MIX_UNSURE: the ray goes to the FP64 trace
+    int2 rc;
+    uint32_t q;
+};
+
+// One ray (map_tracing, rectgrid_raytracing.cu:155-196) of the mixed trace, in FP32.  The
+// draws are those of trace_ray64: the sunshape angle from the float image of the model's Buie
+// table (or, for the tail draws k >= tail_k0, the power law in float), the slope-error angle
+// from the FP32 erfinv.  Directions are the nominal ones plus deviations:
+// the light i = i0 + di, the normal n = n0 + dn, and the reflection
+//   r = i - 2 (n.i) n = a + dr,   dr = di - 2 c1 n0 - 2 (c0 + c1) dn,   c1 = n0.di + dn.i0 + dn.di
+// (c0 = n0.i0, a = i0 - 2 c0 n0 from the model); |r| = 1 to the FP32 rounding of the draws,
+// as the model's normalisations give.
+template <int SUN>
+__device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trace64Args &b, const MixArgs &x,
+                                                  const HelioMix &hm, const MixStart &ms, int h, int m, int l,
+                                                  int self, bool clear)
+{
+    MixRay r{-1, -2, make_int2(-1, -1), 0u};
+    if (!ms.ok) {
+        r.shade = MIX_UNSURE;
+        return r;
+    }
+    const u4 xr = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3}, a.key0, a.key1);
+    const v3 n0 = mk(hm.n[0], hm.n[1], hm.n[2]);
+    float th, ths, st, cm, sp, cp, sts, cms, sps, cps;
+    v3 dn;
+    if (SUN == OM_SUN_BUIE) {
+        // The two loads of the Buie lookup (buie_bin_split) are issued ahead of the work that
+        // does not depend on them (the slope-error normal and the sunshape azimuth), which
+        // then runs while they are in flight.  The tail draws (k >= tail_k0, the model's
+        // choice) take the power law in float, within e_dr of the model's angle.
+        const uint32_t k = xr.x >> 9;
+        const float v = fminf(u01(xr.x) * a.buie_A, x.buie_vmax);
+        const int j = min(max(trunc_i(v * a.buie_gscale), 0), a.nbins - 1);
+        const int2 g = a.buie_split[j];
+        __builtin_amdgcn_sched_barrier(0);
+        ths = gauss_<gfx>(u01(xr.z)) * a.slope_sigma;
+        sincos_m1(ths, sts, cms);
+        sincos2pi(u01(xr.w), sps, cps);
+        dn = cms * n0 + sts * (cps * mk(hm.u[0], hm.u[1], hm.u[2]) + sps * mk(hm.v[0], hm.v[1], hm.v[2]));
+        asm volatile("" ::"v"(dn.x), "v"(dn.y), "v"(dn.z)); // computed here
+        __builtin_amdgcn_sched_barrier(0);
+        int i = min(g.x + (v >= u2f((uint32_t)g.y) ? 1 : 0), a.nbins - 1);
+        float4 bb = a.buie_bins[i];
+        __builtin_amdgcn_sched_barrier(0);
+        sincos2pi(u01(xr.y), sp, cp);
+        asm volatile("" ::"v"(sp), "v"(cp));
+        __builtin_amdgcn_sched_barrier(0);
+        if (k < b.tail_k0) {
+            while (bb.x > v) bb = a.buie_bins[--i];
+            while (bb.y <= v) bb = a.buie_bins[++i];
+            th = div_rn((v - bb.x) * bb.z + bb.w, 1000.0f);
+        } else {
+            const float vt = u01(xr.x) * a.buie_A;
+            th = div_rn(pow_<gfx>((vt - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1), 1000.0f);
+        }
+    } else {
+        th = a.pillbox * sqrt_rn(u01(xr.x));
+        ths = gauss_<gfx>(u01(xr.z)) * a.slope_sigma;
+        sincos_m1(ths, sts, cms);
+        sincos2pi(u01(xr.w), sps, cps);
+        dn = cms * n0 + sts * (cps * mk(hm.u[0], hm.u[1], hm.u[2]) + sps * mk(hm.v[0], hm.v[1], hm.v[2]));
+        sincos2pi(u01(xr.y), sp, cp);
+    }
+    if (!(fabs_(th) < 0.25f && fabs_(ths) < 0.25f)) { // the series' range
+        r.shade = MIX_UNSURE;
+        return r;
+    }
+    sincos_m1(th, st, cm);
+    const v3 di = cm * x.s_n + st * (cp * x.s_u + sp * x.s_v);
+    if (!clear) {
+        r.shade = walk_cert(a, x, ms.of, ms.c, -(x.s_n + di), self);
+        if (r.shade != -1) return r; // a blocker, or MIX_UNSURE
+    }
+    const float c1 = dot(n0, di) + dot(dn, x.s_n) + dot(dn, di);
+    const v3 dr = di - (2.0f * c1) * n0 - (2.0f * (hm.c0 + c1)) * dn;
+    const v3 d = mk(hm.a[0], hm.a[1], hm.a[2]) + dr;
+    r.block = walk_cert(a, x, ms.of, ms.c, d, self);
+    if (r.block != -1) {
+        if (r.block == MIX_UNSURE) r.shade = MIX_UNSURE;
+        return r;
+    }
+    // c = qf + w + D dr with its error bound: the model's qf and D rounded, w and D dr formed
+    // in float, dr within e_dr of the model's
+    const v3 qf = mk(hm.qf[0], hm.qf[1], hm.qf[2]);
+    const v3 c = qf + ms.w + hm.D * dr;
+    const float bc = (norm1(qf) + norm1(ms.w) + hm.D * norm1(dr)) * 8.0f * U24 + hm.D * x.e_dr;
+    r.rc = receiver_local(a, x, c, d, hm.D, bc);
+    if (r.rc.x == MIX_UNSURE) {
+        r.shade = MIX_UNSURE;
+    } else if (r.rc.x >= 0) {
+        // |sun.n| = |sun.n0 + sun.dn|
+        r.q = (uint32_t)(fabs_(hm.e0 + dot(x.sun, dn)) * 2147483648.0f);
+    }
+    return r;
+}
+
+// Ray r of heliostat h traced by trace_ray64 from nothing but its indices: the rays of the
+// redo list, and of a full list.  Out of line: the FP64 ray's registers are held only here.
+template <int SUN, bool POISSON>
+__device__ __noinline__ Ray64 ray64_at(const TraceArgs &a, const Trace64Args &b, int h, int r)
+{
+    const HelioRec *hr = a.helio + h;
+    const Helio64 &h64 = b.helio[h];
+    dframe hf, sf;
+    hf.n = ld(h64.n);
+    hf.u = ld(h64.u);
+    hf.v = ld(h64.v);
+    hf.ident = h64.ident != 0;
+    sf.n = ld(b.sun_n);
+    sf.u = ld(b.sun_u);
+    sf.v = ld(b.sun_v);
+    sf.ident = b.sun_ident != 0;
+    const int m = r / a.L, l = r - m * a.L;
+    const d3 orig = add(world(micro_local64<POISSON>(a, b, hr, h64, h, m), hf), ld(h64.pos));
+    return trace_ray64<SUN>(a, b, hf, sf, orig, cell64(b, orig), h, m, l, h, hr->clear != 0);
+}
+
+// A hit of pixel (row, col) into the LDS window, or marked for the wave's buffer (out)
+__device__ __forceinline__ void window_hit(const TraceArgs &a, unsigned long long *s_win, int2 rc, int px, uint32_t q,
+                                           bool &out)
+{
+    const uint32_t wr = (uint32_t)(rc.x - a.win_y0), wc = (uint32_t)(rc.y - a.win_x0);
+    if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w) packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+    else out = true;
+}
+
+// nrays (a multiple of 64) rays of heliostat tl.x from tl.y traced by one wave of a window
+// workgroup, one ray per lane per pass (as trace_chunk_window, om_kernels.hip)
+template <int SUN, bool POISSON>
+__device__ __forceinline__ void trace_chunk_mixed(const MixLaunch &L, int2 tl, const int lane, const int nrays,
+                                                  WaveOut &wo, unsigned long long *s_win)
+{
+    const TraceArgs &a = L.a;
+    wo.mx = &L.x;
+    const int h = tl.x;
+    const HelioRec *hr = a.helio + h;
+    const HelioMix hm = L.x.hmix[h];
+    const bool clear = hr->clear != 0;
+    const int Lr = a.L;
+    const int nr = min(hr->mcount * Lr, tl.y + nrays);
+    int r = tl.y + lane;
+    int m = r / Lr;
+    int l = r - m * Lr;
+    const int dm = 64 / Lr, dl = 64 - (64 / Lr) * Lr;
+    int m_o = -1; // microsample of ms
+    MixStart ms{};
+    for (int k = 0; k < nrays; k += 64, r += 64) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= Lr) {
+                l -= Lr;
+                ++m;
+            }
+        }
+        bool out = false, flag = false;
+        int px = 0;
+        uint32_t q = 0u;
+        asm volatile("" ::: "memory"); // arguments re-read per ray pass
+        if (r < nr) {
+            if (m != m_o) {
+                ms = mix_start<POISSON>(a, L.x, hr, hm, h, m);
+                m_o = m;
+            }
+            const MixRay o = trace_ray_mixed<SUN>(a, L.b, L.x, hm, ms, h, m, l, h, clear);
+            if (o.shade == MIX_UNSURE) {
+                flag = true;
+            } else if (o.rc.x >= 0) {
+                px = o.rc.x * a.res_x + o.rc.y;
+                q = o.q;
+                window_hit(a, s_win, o.rc, px, q, out);
+            }
+        }
+        // the whole wave takes part: no lane has left the loop.  A ray that finds the redo
+        // list full is dropped: the whole launch is then traced again in FP64
+        // (launch_trace_mixed_window).
+        wo.push_redo(a, flag, h, r, lane);
+        wo.push(a, out, px, q, lane);
+    }
+}
+
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WIN_WGS) void trace_window_mixed_kernel(MixLaunch)
+{
+    const MixLaunch &L = *(const MixLaunch *)__builtin_amdgcn_kernarg_segment_ptr();
+    __shared__ unsigned long long s_win[WIN_PX_MAX];
+    __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
+    window_body(L.a, s_win, s_out, [&](int2 tl, int lane, int nrays, WaveOut &wo, unsigned long long *w) {
+        trace_chunk_mixed<SUN, POISSON>(L, tl, lane, nrays, wo, w);
+    });
+}
+
+// The rays of the redo list traced by the FP64 trace into the packed per-XCD copies; thread
+// 0 adds the launch's count to the scene's running total (and counts a full list in
+// redo_n[2]).  A full list leaves everything to the FP64 trace of the whole launch.
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WG) void trace_redo_kernel(MixLaunch)
+{
+    const MixLaunch &L = *(const MixLaunch *)__builtin_amdgcn_kernarg_segment_ptr();
+    const TraceArgs &a = L.a;
+    const uint32_t cnt = *L.x.redo_n;
+    const bool full = cnt > (uint32_t)L.x.redo_cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(L.x.redo_n + 1, cnt);
+        if (full) atomicAdd(L.x.redo_n + 2, 1u);
+    }
+    if (full) return;
+    const uint32_t n = cnt;
+    unsigned long long *part = copy_of(a);
+    const unsigned long long one = 1ull << a.pack_shift;
+    for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
+        const uint2 e = L.x.redo[i];
+        const Ray64 o = ray64_at<SUN, POISSON>(a, L.b, (int)e.x, (int)e.y);
+        if (o.px >= 0) {
+            const unsigned long long w = one + o.q;
+            copy_check(a, o.px, w, copy_add(a, part, o.px, w));
+        }
+    }
+}
+
+// Per-ray records of the mixed trace (one workgroup per tile, as trace64_kernel): a ray with
+// an unsure decision is traced by the FP64 trace in place and counted in redo_n[1].
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WG) void trace_mixed_records_kernel(MixLaunch)
+{
+    const MixLaunch &L = *(const MixLaunch *)__builtin_amdgcn_kernarg_segment_ptr();
+    const TraceArgs &a = L.a;
+    const int tid = (int)threadIdx.x;
+    const int2 tl = a.tiles[a.tile_base + xcd_chunk(blockIdx.x, gridDim.x)];
+    const int h = tl.x;
+    const HelioRec *hr = a.helio + h;
+    const HelioMix hm = L.x.hmix[h];
+    const bool clear = hr->clear != 0;
+    const int Lr = a.L;
+    const int nr = min(hr->mcount * Lr, tl.y + a.tile_rays);
+    for (int k = 0; k < a.rays_per_thread; ++k) {
+        const int r = tl.y + tid + k * WG;
+        if (r >= nr) break;
+        asm volatile("" ::: "memory");
+        const int m = r / Lr, l = r - m * Lr;
+        const MixStart ms = mix_start<POISSON>(a, L.x, hr, hm, h, m);
+        const MixRay o = trace_ray_mixed<SUN>(a, L.b, L.x, hm, ms, h, m, l, h, clear);
+        int shade = o.shade, block = o.block, px = o.rc.x >= 0 ? o.rc.x * a.res_x + o.rc.y : -1;
+        if (o.shade == MIX_UNSURE) {
+            atomicAdd(L.x.redo_n + 1, 1u);
+            const Ray64 f = ray64_at<SUN, POISSON>(a, L.b, h, r);
+            shade = f.shade;
+            block = f.block;
+            px = f.px;
+        }
+        const long long rid = (((long long)hr->ray_off_hi << 32) | (unsigned)hr->ray_off_lo) + r - a.rec_base;
+        a.rec_shade[rid] = shade;
+        a.rec_block[rid] = block;
+        a.rec_pixel[rid] = px;
+    }
+}
+
 } // namespace
+
+// Clears n words at p when the mixed launch's redo list filled up (*gate > cap)
+__global__ void clear_if_full_kernel(const uint32_t *gate, uint32_t cap, uint32_t *p, long long n)
+{
+    if (*gate <= cap) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
 
 // om_math_eval 16: erfinv_ray(2u - 1) of the draw words in[i], as the FP64 trace evaluates it
 __global__ void erfinv_ray_kernel(const uint32_t *in, double *out, int n)
@@ -581,6 +1138,103 @@ om_status launch_trace64_window(om_scene *s, const TraceArgs &a, const Trace64Ar
     } else {
         if (poisson) hipLaunchKernelGGL((trace_window64_kernel<OM_SUN_PILLBOX, true>), grid, block, 0, st, ab);
         else hipLaunchKernelGGL((trace_window64_kernel<OM_SUN_PILLBOX, false>), grid, block, 0, st, ab);
+    }
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+bool mixed_supported(const om_scene *s)
+{
+    // the cone receiver and the literal self-intersection rule trace in FP64
+    return s->recv_type != 2 && !s->prm.literal_self && s->d_mix_n != nullptr && s->d_hmix != nullptr;
+}
+
+void mixed_args(const om_scene *s, int g, TraceArgs &a, MixArgs &x)
+{
+    a.cell_geo = g < (int)s->d_cell_geo_m.size() ? s->d_cell_geo_m[g] : nullptr;
+    a.buie_bins = s->d_buie_bins_m;
+    a.buie_split = s->d_buie_split_m;
+    a.buie_A = s->buie_A_m;
+    a.buie_gscale = s->buie_gscale_m;
+    // the tail's power law from the model's constants
+    a.buie_int465 = (float)s->a64.int465;
+    a.buie_B = (float)s->a64.B;
+    a.buie_C = (float)s->a64.C;
+    a.buie_inv_gp1 = (float)s->a64.inv_gp1;
+    x = s->mix;
+    const GridState &G = s->grids[g];
+    double ext = 0.0;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)G.gn[k] * (double)G.interval[k]);
+    // a crossing numerator k * edge - (o - gpos): its roundings, and the origin's error
+    x.e_cell = (float)std::ldexp(ext, -21) * 1.0001f + x.e_pos;
+    x.buie_vmax = s->buie_vmax_m;
+    x.redo_n = s->d_mix_n;
+}
+
+om_status launch_trace_mixed_window(om_scene *s, const TraceArgs &a, const Trace64Args &b, const MixArgs &x, int nwg,
+                                    uint32_t *acc, long long acc_words, hipStream_t st)
+{
+    if (a.ntiles <= 0) return OM_OK;
+    if (!b.helio || !b.cell_geo || !a.cell_geo || !x.hmix || !x.redo_n || !x.redo || x.redo_cap <= 0) {
+        set_error("scene has no mixed-precision geometry or the trace no redo list");
+        return OM_ERR_INVALID;
+    }
+    om_status rc = launch_fill_u32(x.redo_n, 1, st); // this launch's list is empty
+    if (rc != OM_OK) return rc;
+    const bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
+    const MixLaunch ml{a, b, x};
+    const dim3 grid(nwg), block(WIN_WGS), rgrid(REDO_BLOCKS), rblock(WG);
+#define OM_MIX(SUN, P)                                                                     \
+    do {                                                                                   \
+        hipLaunchKernelGGL((trace_window_mixed_kernel<SUN, P>), grid, block, 0, st, ml);   \
+        hipLaunchKernelGGL((trace_redo_kernel<SUN, P>), rgrid, rblock, 0, st, ml);        \
+    } while (0)
+    if (s->prm.sunshape == OM_SUN_BUIE) {
+        if (poisson) OM_MIX(OM_SUN_BUIE, true);
+        else OM_MIX(OM_SUN_BUIE, false);
+    } else {
+        if (poisson) OM_MIX(OM_SUN_PILLBOX, true);
+        else OM_MIX(OM_SUN_PILLBOX, false);
+    }
+#undef OM_MIX
+    OM_HIP(hipGetLastError());
+    // A full list: the packed copies and side maps are cleared and the whole launch is traced
+    // again by the FP64 window trace, whose workgroups overwrite their window slices.  Both
+    // kernels read the list's count and return at once when it fitted.
+    const int chunk = std::min(a.tile_rays, window_chunk_rays());
+    if ((rc = launch_queue_reset(a.win_queue, (long long)a.ntiles * (a.tile_rays / chunk), QUEUE_EIGHTHS, st)) !=
+        OM_OK)
+        return rc;
+    hipLaunchKernelGGL(clear_if_full_kernel, dim3(1024), dim3(WG), 0, st, x.redo_n, (uint32_t)x.redo_cap, acc,
+                       acc_words);
+    OM_HIP(hipGetLastError());
+    Trace64Args bg = b;
+    bg.gate = x.redo_n;
+    bg.gate_cap = (uint32_t)x.redo_cap;
+    return launch_trace64_window(s, a, bg, nwg, st);
+}
+
+om_status launch_trace_mixed_records(om_scene *s, const TraceArgs &a, const Trace64Args &b, const MixArgs &x,
+                                     hipStream_t st)
+{
+    if (a.ntiles <= 0) return OM_OK;
+    if (a.rays_per_thread > RPT_MAX) {
+        set_error("rays per thread above the hit-slot count");
+        return OM_ERR_INVALID;
+    }
+    if (!b.helio || !b.cell_geo || !a.cell_geo || !x.hmix || !x.redo_n) {
+        set_error("scene has no mixed-precision geometry");
+        return OM_ERR_INVALID;
+    }
+    const bool poisson = s->prm.micro_mode == OM_MICRO_POISSON;
+    const MixLaunch ml{a, b, x};
+    const dim3 grid(a.ntiles), block(WG);
+    if (s->prm.sunshape == OM_SUN_BUIE) {
+        if (poisson) hipLaunchKernelGGL((trace_mixed_records_kernel<OM_SUN_BUIE, true>), grid, block, 0, st, ml);
+        else hipLaunchKernelGGL((trace_mixed_records_kernel<OM_SUN_BUIE, false>), grid, block, 0, st, ml);
+    } else {
+        if (poisson) hipLaunchKernelGGL((trace_mixed_records_kernel<OM_SUN_PILLBOX, true>), grid, block, 0, st, ml);
+        else hipLaunchKernelGGL((trace_mixed_records_kernel<OM_SUN_PILLBOX, false>), grid, block, 0, st, ml);
     }
     OM_HIP(hipGetLastError());
     return OM_OK;

@@ -60,10 +60,44 @@ __device__ __forceinline__ void copy_check(const TraceArgs &a, int px, unsigned 
     if (old + w < old) atomicAdd(a.wrap + px, 1u);
 }
 
-// A wave's buffer of hits outside the window; n is wave-uniform.
+// A wave's buffer of hits outside the window; n is wave-uniform.  In the mixed trace it also
+// holds the rays to trace again in FP64 (entries with REDO_TAG: the heliostat in bits 32..62,
+// the ray of that heliostat in bits 0..31), which a flush appends to the redo list with one
+// counter atomic for all of them: a returning atomic per flagged pass on the one counter of
+// the whole launch would serialise there and hold back each wave's loads behind it.
+constexpr unsigned long long REDO_TAG = 1ull << 63;
 struct WaveOut {
     unsigned long long *buf; // [WIN_OUT] in LDS
     int n;
+    const MixArgs *mx = nullptr; // the mixed trace's redo list (null: no REDO_TAG entries)
+
+    // Appends the REDO_TAG entries of e[] to the redo list and empties their slots; an entry
+    // past the list's capacity is dropped (the launch is then traced again in FP64).
+    __device__ __forceinline__ void flush_redo(unsigned long long (&e)[WIN_OUT / 64], int lane)
+    {
+        unsigned long long m[WIN_OUT / 64];
+        uint32_t cnt = 0u;
+#pragma unroll
+        for (int b = 0; b < WIN_OUT / 64; ++b) {
+            m[b] = __ballot(e[b] != OUT_EMPTY && (e[b] & REDO_TAG) != 0ull);
+            cnt += (uint32_t)__popcll(m[b]);
+        }
+        if (cnt == 0u) return;
+        uint32_t base = 0u;
+        if (lane == 0) base = atomicAdd(mx->redo_n, cnt);
+        base = __builtin_amdgcn_readfirstlane(base);
+#pragma unroll
+        for (int b = 0; b < WIN_OUT / 64; ++b) {
+            if ((m[b] >> lane) & 1ull) {
+                const uint32_t i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m[b] >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m[b], 0u));
+                if (i < (uint32_t)mx->redo_cap)
+                    mx->redo[i] = make_uint2((uint32_t)(e[b] >> 32) & 0x7fffffffu, (uint32_t)e[b]);
+                e[b] = OUT_EMPTY;
+            }
+            base += (uint32_t)__popcll(m[b]);
+        }
+    }
 
     // Adds the buffered hits to the copies: all returning atomics first, then their checks.
     __device__ __forceinline__ void flush(const TraceArgs &a, int lane)
@@ -77,6 +111,10 @@ struct WaveOut {
         for (int b = 0; b < WIN_OUT / 64; ++b) {
             const int i = b * 64 + lane;
             e[b] = i < n ? buf[i] : OUT_EMPTY;
+        }
+        if (mx) flush_redo(e, lane);
+#pragma unroll
+        for (int b = 0; b < WIN_OUT / 64; ++b) {
             old[b] = 0ull;
             if (e[b] != OUT_EMPTY) old[b] = copy_add(a, part, (int)(e[b] >> 32), one + (e[b] & 0xffffffffull));
         }
@@ -95,6 +133,20 @@ struct WaveOut {
         if (out) {
             const uint32_t j = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
             buf[n + (int)j] = ((unsigned long long)px << 32) | q;
+        }
+        n += __popcll(m);
+        if (n > WIN_OUT - 64) flush(a, lane);
+    }
+
+    // The mixed trace's rays to trace again in FP64 (flag), ray r of heliostat h.  The whole
+    // wave takes part.
+    __device__ __forceinline__ void push_redo(const TraceArgs &a, bool flag, int h, int r, int lane)
+    {
+        const unsigned long long m = __ballot(flag);
+        if (m == 0ull) return;
+        if (flag) {
+            const uint32_t j = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            buf[n + (int)j] = REDO_TAG | ((unsigned long long)(uint32_t)h << 32) | (uint32_t)r;
         }
         n += __popcll(m);
         if (n > WIN_OUT - 64) flush(a, lane);

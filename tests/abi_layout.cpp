@@ -67,7 +67,7 @@ void layout()
     OM_FIELD(om_params, poisson_k); OM_FIELD(om_params, rays_per_sample); OM_FIELD(om_params, reflectivity);
     OM_FIELD(om_params, smooth_radius); OM_FIELD(om_params, smooth_ratio); OM_FIELD(om_params, accumulate);
     OM_FIELD(om_params, pack_shift); OM_FIELD(om_params, grid_select); OM_FIELD(om_params, literal_self);
-    OM_FIELD(om_params, precision); OM_FIELD(om_params, ray_compact);
+    OM_FIELD(om_params, precision); OM_FIELD(om_params, ray_compact); OM_FIELD(om_params, redo_cap);
     end_struct();
 
     OM_STRUCT(om_flux);
