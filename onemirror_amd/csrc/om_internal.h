@@ -154,7 +154,7 @@ struct Trace64Args {
 // with a decision inside its bound is traced again by the FP64 trace.  The bounds, for every
 // ray of the scene:
 //   e_dir   |d - d_model| (2-norm) of a direction formed from the FP32 draws, plus the FP32
-//           rounding of its components in the walks: 2^-19 (theta_max + 2 theta_s,max)
+//           rounding of its components in the walks: e_dr + 2^-21
 //   e_pos   |float(origin) - origin| + |float(vertex) - vertex|: 2^-22 of the largest
 //           coordinate of the scene
 //   e_cell  error of a crossing numerator k * edge - (origin - grid pos) in FP32: 2^-21 of
@@ -178,8 +178,12 @@ static_assert(sizeof(HelioMix) == 80, "HelioMix is 20 words");
 
 struct MixArgs {
     float e_dir, e_pos, e_cell;
-    float e_dr;        // |dr - dr_model| (2-norm): 2^-19 (theta_max + 2 theta_s,max)
+    float e_dr;        // |dr - dr_model| (2-norm): 2^-19 (theta_max + 4 theta_s,max) + 2^-26
     float buie_vmax;   // the largest float value below the top of the Buie table's image
+    // Buie tail draws k >= tail_k0 of Trace64Args: the value's excess over int465,
+    // (k - tail_k0) tail_du + tail_dv0 (tail_du = 2^-23 A, tail_dv0 = u(tail_k0) A - int465 in
+    // double), free of the cancellation of u A - int465 in float
+    float tail_du, tail_dv0;
     const HelioMix *hmix;
     v3 s_u, s_n, s_v;  // the model's sun frame rounded (identity axes for an identity frame)
     v3 sun;            // the sun direction rounded (the energy's)

@@ -1282,12 +1282,14 @@ om_status upload_mixed(om_scene *s, const std::vector<om::Helio64> &hel, const s
     // Directions from the FP32 draws: the sunshape polar angle theta (at most 9.3 mrad, the
     // Buie tail's end, or the pillbox half angle) and the slope-error angle theta_s (at most
     // 5.3 sigma, the deviate of the extreme draw 1 - 2^-24) enter as sines and cosines of
-    // relative error a few 2^-24 and as products with the double frames, so a direction is
-    // within about 2^-21 (theta + 2 theta_s) of the model's (the reflection doubles the
-    // normal's error); the bound takes four times that.
+    // relative error a few 2^-24 (the erfinv's about 2^-21), the deviations from the nominal
+    // directions scale with them and the reflection doubles the normal's, so a direction's
+    // deviation is within about 2^-21 (theta + 2 theta_s) of the model's, plus at most about
+    // 3e-9 rad of the float Buie lookup; the bound takes 2^-19 (theta + 4 theta_s) + 2^-26 at
+    // the largest angles (the kernels take it per ray at the ray's angles).
     const double th_max = s->prm.sunshape == OM_SUN_BUIE ? 9.3e-3 : (double)s->prm.pillbox_half_angle;
     const double ths_max = 5.5 * (double)s->prm.slope_sigma;
-    s->mix_e_dir = up_f(std::ldexp(th_max + 2.0 * ths_max, -19) + std::ldexp(1.0, -40));
+    s->mix_e_dir = up_f(std::ldexp(th_max + 4.0 * ths_max, -19) + std::ldexp(1.0, -26));
     // the deviations dr carry that error; a direction a + dr, rounded to float in the walks,
     // adds three roundings of its components (within 2^-21 in 2-norm)
     x.e_dr = s->mix_e_dir;

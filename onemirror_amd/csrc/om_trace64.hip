@@ -649,13 +649,15 @@ __device__ __forceinline__ MixStart mix_start(const TraceArgs &a, const MixArgs 
 }
 
 // NotCollision (rectgrid_raytracing.cu:38-117) in FP32 with certified decisions, in the form
-// of dda (om_device.h): the first candidate hit, -1 for none, MIX_UNSURE.  c: the model's
-// start cell.  A step's axis is certain when the chosen crossing parameter m_c lies below
-// each other axis's m_j by more than the sum of their bounds
-//   |m_k - m_k,model| <= e_cell / |d_k| + m_k (e_dir / |d_k| + (n + 4) 2^-24)
-// (the numerator's error, the direction's, the division and the n additions of the walk so
-// far).  A direction component within 2 e_dir + 1e-6 of zero could differ from the model's in
-// sign or in absDivide's 1e-6 test: such a ray is unsure.
+// of dda (om_device.h): the first candidate hit, -1 for none, MIX_UNSURE.  c: the start cell
+// (certified by mix_start).  The crossing parameters m_k = |(k edge - cur) / d_k| are formed
+// with the hardware reciprocal (within 1 ulp); each is within
+//   e_k = e_cell / |d_k| + m_k (e_dir / |d_k| + (n + 8) 2^-24)
+// of the model's after n steps (the numerator's error, the direction's, the reciprocal and
+// product, the n additions).  A step's axis is certain when the chosen m_c lies below each
+// other axis's m_j by more than e_c + e_j.  A direction component within 2 e_dir + 1e-6 of
+// zero could differ from the model's in sign or in absDivide's 1e-6 test: such a ray is
+// unsure.
 __device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3 &c, v3 d, int self)
 {
     const bool fx = d.x >= 0.0f, fy = d.y >= 0.0f, fz = d.z >= 0.0f;
@@ -666,17 +668,16 @@ __device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3
     if (fabs_(d.x) <= near0 || fabs_(d.y) <= near0 || fabs_(d.z) <= near0) return MIX_UNSURE;
     if ((nx | ny | nz) < 0) return -1;
     if ((nx | ny | nz) == 0) return -1;
-    float mx = first_crossing(d.x, fx, a.gix, c.gx, c.rx);
-    float my = first_crossing(d.y, fy, a.giy, c.gy, c.ry);
-    float mz = first_crossing(d.z, fz, a.giz, c.gz, c.rz);
-    const float dx = cell_crossing(d.x, a.gix), dy = cell_crossing(d.y, a.giy), dz = cell_crossing(d.z, a.giz);
-    // 1 / |d_k| from the estimate, rounded up by more than its error
-    const float rx = __builtin_amdgcn_rcpf(fabs_(d.x)) * 1.0000005f;
-    const float ry = __builtin_amdgcn_rcpf(fabs_(d.y)) * 1.0000005f;
-    const float rz = __builtin_amdgcn_rcpf(fabs_(d.z)) * 1.0000005f;
-    const float ax = x.e_cell * rx, ay = x.e_cell * ry, az = x.e_cell * rz;
-    const float bx = x.e_dir * rx, by = x.e_dir * ry, bz = x.e_dir * rz;
-    float sn = 4.0f * U24; // (n + 4) 2^-24
+    const float rx = __builtin_amdgcn_rcpf(d.x), ry = __builtin_amdgcn_rcpf(d.y), rz = __builtin_amdgcn_rcpf(d.z);
+    float mx = fabs_(((float)(c.gx + (fx ? 1 : 0)) * a.gix - c.rx) * rx);
+    float my = fabs_(((float)(c.gy + (fy ? 1 : 0)) * a.giy - c.ry) * ry);
+    float mz = fabs_(((float)(c.gz + (fz ? 1 : 0)) * a.giz - c.rz) * rz);
+    const float dx = fabs_(a.gix * rx), dy = fabs_(a.giy * ry), dz = fabs_(a.giz * rz);
+    // 1 / |d_k| rounded up by more than the estimate's error
+    const float ux = fabs_(rx) * 1.000001f, uy = fabs_(ry) * 1.000001f, uz = fabs_(rz) * 1.000001f;
+    const float ax = x.e_cell * ux, ay = x.e_cell * uy, az = x.e_cell * uz;
+    const float bx = x.e_dir * ux, by = x.e_dir * uy, bz = x.e_dir * uz;
+    float sn = 8.0f * U24; // (n + 8) 2^-24
     const int yz = a.gny * a.gnz;
     const int sax = fx ? yz : -yz, say = fy ? a.gnz : -a.gnz, saz = fz ? 1 : -1;
     int addr = c.addr;
@@ -788,7 +789,7 @@ __device__ __forceinline__ int2 receiver_local(const TraceArgs &a, const MixArgs
         if (fabs_(t - 1e-6f) <= mw + fabs_(t) * 4.0f * U24) return unsure;
         if (t < 1e-6f) return miss;
         const float Y = (W.y + 0.5f * x.cyl_h) / x.cyl_h * ry, mY = (mw + fabs_(W.y) * 4.0f * U24) / x.cyl_h * ry + ry * 4.0f * U24;
-        if (Y < -mY || Y > ry + mY) return miss;
+        if (Y < -mY || Y - ry > mY) return miss; // Y - ry is exact near ry (ry + mY would round)
         float th = atan2_(W.z, W.x);
         if (th < 0.0f) th += 6.28318548f;
         const float X = th * 0.159154937f * rx;
@@ -810,7 +811,10 @@ __device__ __forceinline__ int2 receiver_local(const TraceArgs &a, const MixArgs
     const float U = (x.uF + dot(P, x.g1)) * ry, V = (x.vF + dot(P, x.g2)) * rx;
     const float mU = ((mp + pn * 8.0f * U24) * x.g1n + 4.0f * U24) * ry + ry * 4.0f * U24;
     const float mV = ((mp + pn * 8.0f * U24) * x.g2n + 4.0f * U24) * rx + rx * 4.0f * U24;
-    if (U < -mU || U > ry + mU || V < -mV || V > rx + mV || t < 1e-6f) return miss;
+    // the edges: U - ry is exact near ry, where ry + mU would round to float and leave a sliver
+    // (ry + mU, fl(ry + mU)] that neither test catches; nearer an integer than the bound
+    // (an edge included) the pixel is unsure; else the hit is certain
+    if (U < -mU || U - ry > mU || V < -mV || V - rx > mV || t < 1e-6f) return miss;
     if (int_gapf(U) <= mU || int_gapf(V) <= mV) return unsure;
     return make_int2(min(trunc_i(U), a.res_y - 1), min(trunc_i(V), a.res_x - 1));
 }
@@ -847,7 +851,8 @@ __device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trac
         // The two loads of the Buie lookup (buie_bin_split) are issued ahead of the work that
         // does not depend on them (the slope-error normal and the sunshape azimuth), which
         // then runs while they are in flight.  The tail draws (k >= tail_k0, the model's
-        // choice) take the power law in float, within e_dr of the model's angle.
+        // choice) take the power law in float, from the value's excess over int465 formed
+        // without cancellation (MixArgs::tail_du), within 2^-21 of the model's angle.
         const uint32_t k = xr.x >> 9;
         const float v = fminf(u01(xr.x) * a.buie_A, x.buie_vmax);
         const int j = min(max(trunc_i(v * a.buie_gscale), 0), a.nbins - 1);
@@ -868,13 +873,13 @@ __device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trac
         if (k < b.tail_k0) {
             while (bb.x > v) bb = a.buie_bins[--i];
             while (bb.y <= v) bb = a.buie_bins[++i];
-            th = div_rn((v - bb.x) * bb.z + bb.w, 1000.0f);
+            th = ((v - bb.x) * bb.z + bb.w) * 1e-3f;
         } else {
-            const float vt = u01(xr.x) * a.buie_A;
-            th = div_rn(pow_<gfx>((vt - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1), 1000.0f);
+            const float dv = fma_((float)(k - b.tail_k0), x.tail_du, x.tail_dv0);
+            th = pow_<gfx>(dv * a.buie_B + a.buie_C, a.buie_inv_gp1) * 1e-3f;
         }
     } else {
-        th = a.pillbox * sqrt_rn(u01(xr.x));
+        th = a.pillbox * __builtin_amdgcn_sqrtf(u01(xr.x));
         ths = gauss_<gfx>(u01(xr.z)) * a.slope_sigma;
         sincos_m1(ths, sts, cms);
         sincos2pi(u01(xr.w), sps, cps);
@@ -903,7 +908,13 @@ __device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trac
     // in float, dr within e_dr of the model's
     const v3 qf = mk(hm.qf[0], hm.qf[1], hm.qf[2]);
     const v3 c = qf + ms.w + hm.D * dr;
-    const float bc = (norm1(qf) + norm1(ms.w) + hm.D * norm1(dr)) * 8.0f * U24 + hm.D * x.e_dr;
+    // dr is within 2^-19 (theta + 4 theta_s) + 2^-26 of the model's deviation: the draws'
+    // relative errors (a few 2^-24; the FP32 erfinv's, about 2^-21) scale with the angles and
+    // the slope error's count twice in the reflection; the float Buie lookup's absolute error
+    // (at most about 3e-9 rad, where the table's slope is steepest) does not.  The scene bound
+    // e_dr is this at the largest angles.
+    const float edr = fminf(fma_(fabs_(th) + 4.0f * fabs_(ths), 1.9073486e-06f, 1.4901161e-08f), x.e_dr);
+    const float bc = (norm1(qf) + norm1(ms.w) + hm.D * norm1(dr)) * 8.0f * U24 + hm.D * edr;
     r.rc = receiver_local(a, x, c, d, hm.D, bc);
     if (r.rc.x == MIX_UNSURE) {
         r.shade = MIX_UNSURE;
@@ -1162,6 +1173,10 @@ void mixed_args(const om_scene *s, int g, TraceArgs &a, MixArgs &x)
     a.buie_C = (float)s->a64.C;
     a.buie_inv_gp1 = (float)s->a64.inv_gp1;
     x = s->mix;
+    x.tail_du = (float)std::ldexp(s->a64.A, -23);
+    x.tail_dv0 = s->a64.tail_k0 < (1u << 23)
+                     ? (float)((double)u01(s->a64.tail_k0 << 9) * s->a64.A - s->a64.int465)
+                     : 0.0f;
     const GridState &G = s->grids[g];
     double ext = 0.0;
     for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)G.gn[k] * (double)G.interval[k]);
