@@ -148,8 +148,8 @@ typedef struct {
                                   0 (default): each wave runs the receiver on the lanes its
                                   walks leave alive.  Results are identical.              */
     int   redo_cap;            /* OM_PREC_MIXED: entries of a launch's list of rays for the
-                                  FP64 trace (0, the default: one per 64 rays, 2^16 to
-                                  2^22).  A launch whose list fills up is traced again
+                                  FP64 trace (0, the default: one per 32 rays, 2^16 to
+                                  2^24).  A launch whose list fills up is traced again
                                   wholly in FP64: the same results, more time.           */
 } om_params;
 

@@ -1427,11 +1427,11 @@ static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
     }
     int shift = 0;
     while (shift < COPY_SHIFT_MAX && ((size_t)16 << shift) * npx * sizeof(unsigned long long) <= copy_budget) ++shift;
-    // the redo list of a mixed trace: one entry per 64 rays, 2^16 to 2^22 entries (a full
-    // list sends its further rays to the FP64 trace inside the window kernel)
+    // the redo list of a mixed trace: one entry per 32 rays, 2^16 to 2^24 entries (a launch
+    // whose list fills up is traced again wholly in FP64)
     long long want = !mixed_window(s) ? 0
                      : s->prm.redo_cap > 0 ? s->prm.redo_cap
-                                           : std::min<long long>(std::max<long long>(s->total_rays / 64, 1 << 16), 1 << 22);
+                                           : std::min<long long>(std::max<long long>(s->total_rays / 32, 1 << 16), 1 << 24);
     for (;;) {
         for (int sh = shift;; --sh) {
             const size_t ncopy = (size_t)8 << sh;

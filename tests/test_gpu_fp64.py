@@ -128,8 +128,9 @@ def test_large_whole_field_on_a_1pct_pixel_subset(om):
     """The large field (25,000 heliostats, 9.95e8 rays, 512 x 256 cylinder map) against the
     FP64 model of the whole field (about 30 s on 16 host cores), compared on a seeded 1 %
     subset of the lit pixels, as the large configuration's accuracy is quoted:
-      * the FP64-geometry trace: counts equal the model's on every pixel, flux within 1e-5
-        relative on the subset and on every lit pixel;
+      * the FP64-geometry trace and the mixed-precision trace (OM_PREC_MIXED, window
+        accumulation): counts equal the model's on every pixel, flux within 1e-5 relative on
+        the subset and on every lit pixel;
       * the FP32-geometry trace (reference semantics): its error on the subset is printed
         and bounded by the figures stated for it, with a 10 % margin."""
     import time
@@ -145,10 +146,13 @@ def test_large_whole_field_on_a_1pct_pixel_subset(om):
     rng = np.random.default_rng(1)
     sub = np.sort(rng.choice(lit, max(1, lit.size // 100), replace=False))
     out = {}
-    for name, prm in (("fp64", _fp64(p, om)), ("fp32", p)):
+    for name, prm in (("fp64", _fp64(p, om)), ("mixed", dict(p, precision=om.PREC_MIXED, accumulate=om.ACC_WINDOW)),
+                      ("fp32", p)):
         g = om.SolarScene(d, prm)
         m = g.trace_plain()
         torch.cuda.synchronize()
+        if name == "mixed":
+            print(f"mixed: {g.redo_count()} of {g.info['total_rays']} rays traced in FP64", flush=True)
         counts = m.counts.cpu().numpy().view(np.uint32).ravel()
         flux = m.flux.cpu().numpy().astype(np.float64).ravel()
         rel = np.abs(flux - flux64)[lit] / flux64[lit]
@@ -158,8 +162,9 @@ def test_large_whole_field_on_a_1pct_pixel_subset(om):
                          share=float((rs > 1e-5).mean()), max_sub=float(rs.max()))
         print(name, sub.size, "subset pixels of", lit.size, out[name], flush=True)
         del g, m
-    assert out["fp64"]["count_px"] == 0
-    assert out["fp64"]["max_sub"] <= 1e-5 and out["fp64"]["max_all"] <= 1e-5
+    for name in ("fp64", "mixed"):
+        assert out[name]["count_px"] == 0, name
+        assert out[name]["max_sub"] <= 1e-5 and out[name]["max_all"] <= 1e-5, name
     # FP32 geometry on the whole large field, as stated (subset median 8.09e-4, p99 5.48e-3,
     # max 9.52e-3; 98.9 % of the subset above 1e-5)
     for k, v in (("median", 8.09e-4), ("p99", 5.48e-3), ("max_sub", 9.52e-3)):
