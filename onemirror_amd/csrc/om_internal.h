@@ -157,8 +157,8 @@ struct Trace64Args {
 //           rounding of its components in the walks: e_dr + 2^-21
 //   e_pos   |float(origin) - origin| + |float(vertex) - vertex|: 2^-22 of the largest
 //           coordinate of the scene
-//   e_cell  error of a crossing numerator k * edge - (origin - grid pos) in FP32: 2^-21 of
-//           the grid's extent
+// (the walks' crossings start from the origin's offset inside its cell, formed in double:
+// their error is that of the microsample's offset w from its heliostat, 2^-21 |w|_1)
 // Per heliostat, the FP64 model's values rounded to float: the frame of the aimed normal n0
 // (the identity axes for an identity frame), the reflection a = i0 - 2 (n0.i0) n0 of the
 // nominal light direction i0 (the sun frame's axis), c0 = n0.i0, and the nominal ray's
@@ -177,7 +177,8 @@ struct alignas(16) HelioMix {
 static_assert(sizeof(HelioMix) == 80, "HelioMix is 20 words");
 
 struct MixArgs {
-    float e_dir, e_pos, e_cell;
+    float e_dir, e_pos;
+    float gi_max;      // the launch grid's largest cell edge
     float e_dr;        // |dr - dr_model| (2-norm): 2^-19 (theta_max + 4 theta_s,max) + 2^-26
     float buie_vmax;   // the largest float value below the top of the Buie table's image
     // Buie tail draws k >= tail_k0 of Trace64Args: the value's excess over int465,

@@ -549,6 +549,8 @@ constexpr int MIX_UNSURE = -3;
 constexpr float U24 = 5.9604644775390625e-08f; // 2^-24
 constexpr int REDO_BLOCKS = 1024;              // workgroups of the redo list's FP64 trace
 
+__device__ __forceinline__ float norm1(v3 a) { return fabs_(a.x) + fabs_(a.y) + fabs_(a.z); }
+
 struct MixLaunch {
     TraceArgs a; // its FP32 scene fields hold the float images of the FP64 scene (mixed_args)
     Trace64Args b;
@@ -591,15 +593,18 @@ __device__ __forceinline__ int parallelogram_cert(const MixArgs &x, v3 o, v3 d, 
 }
 
 // The origin of a microsample's rays: the float origin o = pos + w with w = o - pos formed in
-// float from the microsample's local position and the model's frame, and the grid cell of
-// o with its float offset from the grid position.  The model's cell is the truncation of
-// (o - gpos) / edge in double; the float one is certain when each axis's quotient lies
-// farther from an integer than its error (e_pos and the roundings, over the edge).  A
-// microsample's facet is certain likewise (the quotient of the Poisson point by the facet
+// float from the microsample's local position and the model's frame (within ew = 2^-21 |w|_1
+// of the model's w), the grid cell of the origin and the origin's offset inside it.  pos and
+// w are floats, so pos + w - gpos is formed exactly in double, once per microsample: the
+// cell is the model's truncation of (o - gpos) / edge unless the quotient lies within ew /
+// edge of an integer, and the offset inside the cell, rounded to float, carries the walk's
+// crossings with an error of ew + 2^-23 edge instead of the rounding of field coordinates.
+// A microsample's facet is certain likewise (the quotient of the Poisson point by the facet
 // length); ok is false when either is not, and every ray of the microsample goes to FP64.
 struct MixStart {
     v3 of, w;
-    cell3 c;
+    cell3 c; // rx, ry, rz: the offset of the origin inside its cell
+    float ew;
     bool ok;
 };
 template <bool POISSON>
@@ -631,34 +636,38 @@ __device__ __forceinline__ MixStart mix_start(const TraceArgs &a, const MixArgs 
     }
     const v3 u = mk(hm.u[0], hm.u[1], hm.u[2]), n = mk(hm.n[0], hm.n[1], hm.n[2]), v = mk(hm.v[0], hm.v[1], hm.v[2]);
     s.w = loc.x * u + loc.y * n + loc.z * v;
-    s.of = mk(hr->pos[0], hr->pos[1], hr->pos[2]) + s.w;
-    s.c.rx = s.of.x - a.gpx;
-    s.c.ry = s.of.y - a.gpy;
-    s.c.rz = s.of.z - a.gpz;
-    const float qx = div_rn(s.c.rx, a.gix), qy = div_rn(s.c.ry, a.giy), qz = div_rn(s.c.rz, a.giz);
-    s.c.gx = trunc_i(qx);
-    s.c.gy = trunc_i(qy);
-    s.c.gz = trunc_i(qz);
+    const v3 pos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
+    s.of = pos + s.w;
+    s.ew = fma_(norm1(s.w), 4.7683716e-07f, 1e-12f);
+    const double rx = ((double)pos.x + (double)s.w.x) - (double)a.gpx;
+    const double ry = ((double)pos.y + (double)s.w.y) - (double)a.gpy;
+    const double rz = ((double)pos.z + (double)s.w.z) - (double)a.gpz;
+    const double qx = rx / (double)a.gix, qy = ry / (double)a.giy, qz = rz / (double)a.giz;
+    s.c.gx = (int)qx;
+    s.c.gy = (int)qy;
+    s.c.gz = (int)qz;
     s.c.addr = (s.c.gx * a.gny + s.c.gy) * a.gnz + s.c.gz;
-    const float ex = fma_(fabs_(s.c.rx), 2.0f * U24, x.e_pos), ey = fma_(fabs_(s.c.ry), 2.0f * U24, x.e_pos),
-                ez = fma_(fabs_(s.c.rz), 2.0f * U24, x.e_pos);
-    s.ok = s.ok && fabs_(qx - rintf(qx)) > fma_(qx, 2.0f * U24, div_rn(ex, a.gix)) &&
-           fabs_(qy - rintf(qy)) > fma_(qy, 2.0f * U24, div_rn(ey, a.giy)) &&
-           fabs_(qz - rintf(qz)) > fma_(qz, 2.0f * U24, div_rn(ez, a.giz));
+    s.c.rx = (float)(rx - (double)s.c.gx * (double)a.gix);
+    s.c.ry = (float)(ry - (double)s.c.gy * (double)a.giy);
+    s.c.rz = (float)(rz - (double)s.c.gz * (double)a.giz);
+    const double gx = qx - (double)s.c.gx, gy = qy - (double)s.c.gy, gz = qz - (double)s.c.gz;
+    const float mx = s.ew / a.gix * 1.01f, my = s.ew / a.giy * 1.01f, mz = s.ew / a.giz * 1.01f;
+    s.ok = s.ok && (float)fmin(gx, 1.0 - gx) > mx && (float)fmin(gy, 1.0 - gy) > my && (float)fmin(gz, 1.0 - gz) > mz;
     return s;
 }
 
 // NotCollision (rectgrid_raytracing.cu:38-117) in FP32 with certified decisions, in the form
 // of dda (om_device.h): the first candidate hit, -1 for none, MIX_UNSURE.  c: the start cell
-// (certified by mix_start).  The crossing parameters m_k = |(k edge - cur) / d_k| are formed
-// with the hardware reciprocal (within 1 ulp); each is within
-//   e_k = e_cell / |d_k| + m_k (e_dir / |d_k| + (n + 8) 2^-24)
-// of the model's after n steps (the numerator's error, the direction's, the reciprocal and
-// product, the n additions).  A step's axis is certain when the chosen m_c lies below each
-// other axis's m_j by more than e_c + e_j.  A direction component within 2 e_dir + 1e-6 of
-// zero could differ from the model's in sign or in absDivide's 1e-6 test: such a ray is
-// unsure.
-__device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3 &c, v3 d, int self)
+// with the origin's offset inside it (mix_start), so that the first crossing numerators,
+// edge - r or r, are within e_num = ew + 2^-22 edge of the model's.  The crossing parameters
+// m_k = numerator / |d_k| are formed with the hardware reciprocal (within 1 ulp); each is
+// within e_k = e_num / |d_k| + m_k (e_dir / |d_k| + (n + 8) 2^-24) of the model's after n
+// steps (the direction's error, the reciprocal and product, the n additions).  A step's axis
+// is certain when the chosen m_c lies below the runner-up m_2 by more than
+// 2 e_num r + (m_c + m_2) (e_dir r + (n + 8) 2^-24), r the largest 1 / |d_k|, which bounds
+// e_c + e_2.  A direction component within 2 e_dir + 1e-6 of zero could differ from the
+// model's in sign or in absDivide's 1e-6 test: such a ray is unsure.
+__device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3 &c, float ew, v3 d, int self)
 {
     const bool fx = d.x >= 0.0f, fy = d.y >= 0.0f, fz = d.z >= 0.0f;
     int nx = fx ? a.occ_hi[0] - c.gx : c.gx - a.occ_lo[0];
@@ -668,16 +677,15 @@ __device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3
     if (fabs_(d.x) <= near0 || fabs_(d.y) <= near0 || fabs_(d.z) <= near0) return MIX_UNSURE;
     if ((nx | ny | nz) < 0) return -1;
     if ((nx | ny | nz) == 0) return -1;
-    const float rx = __builtin_amdgcn_rcpf(d.x), ry = __builtin_amdgcn_rcpf(d.y), rz = __builtin_amdgcn_rcpf(d.z);
-    float mx = fabs_(((float)(c.gx + (fx ? 1 : 0)) * a.gix - c.rx) * rx);
-    float my = fabs_(((float)(c.gy + (fy ? 1 : 0)) * a.giy - c.ry) * ry);
-    float mz = fabs_(((float)(c.gz + (fz ? 1 : 0)) * a.giz - c.rz) * rz);
-    const float dx = fabs_(a.gix * rx), dy = fabs_(a.giy * ry), dz = fabs_(a.giz * rz);
-    // 1 / |d_k| rounded up by more than the estimate's error
-    const float ux = fabs_(rx) * 1.000001f, uy = fabs_(ry) * 1.000001f, uz = fabs_(rz) * 1.000001f;
-    const float ax = x.e_cell * ux, ay = x.e_cell * uy, az = x.e_cell * uz;
-    const float bx = x.e_dir * ux, by = x.e_dir * uy, bz = x.e_dir * uz;
-    float sn = 8.0f * U24; // (n + 8) 2^-24
+    const float rx = fabs_(__builtin_amdgcn_rcpf(d.x)), ry = fabs_(__builtin_amdgcn_rcpf(d.y)),
+                rz = fabs_(__builtin_amdgcn_rcpf(d.z));
+    float mx = (fx ? a.gix - c.rx : c.rx) * rx;
+    float my = (fy ? a.giy - c.ry : c.ry) * ry;
+    float mz = (fz ? a.giz - c.rz : c.rz) * rz;
+    const float dx = a.gix * rx, dy = a.giy * ry, dz = a.giz * rz;
+    const float rm = fmaxf(fmaxf(rx, ry), rz) * 1.000001f;
+    const float ea = 2.0f * fma_(x.gi_max, 2.3841858e-07f, ew) * rm; // 2 (ew + 2^-22 edge) r
+    float eb = fma_(x.e_dir, rm, 8.0f * U24);                          // e_dir r + (n + 8) 2^-24
     const int yz = a.gny * a.gnz;
     const int sax = fx ? yz : -yz, say = fy ? a.gnz : -a.gnz, saz = fz ? 1 : -1;
     int addr = c.addr;
@@ -686,10 +694,10 @@ __device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3
         const bool sx = mx < my && mx < mz;                                                      \
         const bool sy = !(mx < my) && my < mz;                                                   \
         const bool sz = !sx && !sy;                                                              \
-        const float ex = fma_(mx, bx + sn, ax), ey = fma_(my, by + sn, ay), ez = fma_(mz, bz + sn, az); \
-        const float mc = sx ? mx : (sy ? my : mz), ec = sx ? ex : (sy ? ey : ez);                \
-        sure = (sx || mx - mc > ex + ec) && (sy || my - mc > ey + ec) && (sz || mz - mc > ez + ec); \
-        sn += U24;                                                                               \
+        const float mc = sx ? mx : (sy ? my : mz);                                               \
+        const float m2 = sx ? fminf(my, mz) : (sy ? fminf(mx, mz) : fminf(mx, my));              \
+        sure = m2 - mc > fma_(mc + m2, eb, ea);                                                  \
+        eb += U24;                                                                               \
         nx -= sx ? 1 : 0;                                                                        \
         ny -= sy ? 1 : 0;                                                                        \
         nz -= sz ? 1 : 0;                                                                        \
@@ -748,8 +756,6 @@ __device__ __forceinline__ void sincos_m1(float x, float &s, float &cm1)
     pc = fma_(x2, pc, -0.5f);
     cm1 = x2 * pc;
 }
-
-__device__ __forceinline__ float norm1(v3 a) { return fabs_(a.x) + fabs_(a.y) + fabs_(a.z); }
 
 // receiver_drawing (rectgrid_raytracing.cu:135-153) for the rectangle and the cylinder, from
 // the receiver's reference point F: the ray meets the receiver at F + c + tau d.  (row,
@@ -893,13 +899,13 @@ __device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trac
     sincos_m1(th, st, cm);
     const v3 di = cm * x.s_n + st * (cp * x.s_u + sp * x.s_v);
     if (!clear) {
-        r.shade = walk_cert(a, x, ms.of, ms.c, -(x.s_n + di), self);
+        r.shade = walk_cert(a, x, ms.of, ms.c, ms.ew, -(x.s_n + di), self);
         if (r.shade != -1) return r; // a blocker, or MIX_UNSURE
     }
     const float c1 = dot(n0, di) + dot(dn, x.s_n) + dot(dn, di);
     const v3 dr = di - (2.0f * c1) * n0 - (2.0f * (hm.c0 + c1)) * dn;
     const v3 d = mk(hm.a[0], hm.a[1], hm.a[2]) + dr;
-    r.block = walk_cert(a, x, ms.of, ms.c, d, self);
+    r.block = walk_cert(a, x, ms.of, ms.c, ms.ew, d, self);
     if (r.block != -1) {
         if (r.block == MIX_UNSURE) r.shade = MIX_UNSURE;
         return r;
@@ -1178,10 +1184,7 @@ void mixed_args(const om_scene *s, int g, TraceArgs &a, MixArgs &x)
                      ? (float)((double)u01(s->a64.tail_k0 << 9) * s->a64.A - s->a64.int465)
                      : 0.0f;
     const GridState &G = s->grids[g];
-    double ext = 0.0;
-    for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)G.gn[k] * (double)G.interval[k]);
-    // a crossing numerator k * edge - (o - gpos): its roundings, and the origin's error
-    x.e_cell = (float)std::ldexp(ext, -21) * 1.0001f + x.e_pos;
+    x.gi_max = std::max(std::max(G.interval[0], G.interval[1]), G.interval[2]);
     x.buie_vmax = s->buie_vmax_m;
     x.redo_n = s->d_mix_n;
 }
