@@ -858,7 +858,7 @@ __device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trac
         // does not depend on them (the slope-error normal and the sunshape azimuth), which
         // then runs while they are in flight.  The tail draws (k >= tail_k0, the model's
         // choice) take the power law in float, from the value's excess over int465 formed
-        // without cancellation (MixArgs::tail_du), within 2^-21 of the model's angle.
+        // without cancellation (MixArgs::tail_du), within 2^-20 relative of the model's angle.
         const uint32_t k = xr.x >> 9;
         const float v = fminf(u01(xr.x) * a.buie_A, x.buie_vmax);
         const int j = min(max(trunc_i(v * a.buie_gscale), 0), a.nbins - 1);
@@ -915,7 +915,8 @@ __device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trac
     const v3 qf = mk(hm.qf[0], hm.qf[1], hm.qf[2]);
     const v3 c = qf + ms.w + hm.D * dr;
     // dr is within 2^-19 (theta + 4 theta_s) + 2^-26 of the model's deviation: the draws'
-    // relative errors (a few 2^-24; the FP32 erfinv's, about 2^-21) scale with the angles and
+    // relative errors (a few 2^-24; the FP32 erfinv's and the float Buie tail's, at most 2^-21
+    // and 2^-20 over all 2^23 draws) scale with the angles and
     // the slope error's count twice in the reflection; the float Buie lookup's absolute error
     // (at most about 3e-9 rad, where the table's slope is steepest) does not.  The scene bound
     // e_dr is this at the largest angles.

@@ -103,6 +103,8 @@ def lib():
     L.oc_philox_batch.argtypes = [_u32p, _u32p, _u32p, ctypes.c_int]
     L.oc_math_batch.argtypes = [ctypes.c_int, _f32p, _f32p, ctypes.c_int]
     L.oc_erfinv_d_batch.argtypes = [_f64p, _f64p, ctypes.c_int]
+    L.oc_buie_table_d.argtypes = [ctypes.c_double, ctypes.c_int, _f64p, _f64p, _f64p, _f64p]
+    L.oc_buie_theta_d_batch.argtypes = [ctypes.c_double, ctypes.c_int, _f64p, _f64p, ctypes.c_int]
     L.oc_scene_build.argtypes = [ctypes.POINTER(SceneDescC), ctypes.POINTER(ParamsC),
                                  ctypes.POINTER(ctypes.c_void_p)]
     L.oc_scene_build.restype = ctypes.c_int
@@ -187,6 +189,22 @@ def erfinv64(x) -> np.ndarray:
     xd = np.ascontiguousarray(x, dtype=np.float64)
     out = np.empty_like(xd)
     lib().oc_erfinv_d_batch(_p(xd, _f64p), _p(out, _f64p), int(xd.size))
+    return out
+
+
+def buie_table64(csr: float, ng: int) -> dict:
+    """The FP64 model's Buie table (om_oracle.c oc_buie_table_d): cx, cy [ng + 1], slope [ng]
+    and the tail's int465, A, B, C, inv_gp1."""
+    cx, cy, sl, c = np.zeros(ng + 1), np.zeros(ng + 1), np.zeros(ng), np.zeros(5)
+    lib().oc_buie_table_d(float(csr), int(ng), _p(cx, _f64p), _p(cy, _f64p), _p(sl, _f64p), _p(c, _f64p))
+    return dict(cx=cx, cy=cy, slope=sl, **dict(zip(("int465", "A", "B", "C", "inv_gp1"), c.tolist())))
+
+
+def buie_theta64(csr: float, ng: int, u) -> np.ndarray:
+    """The FP64 model's sunshape angle in mrad of the draws u (om_oracle.c buie_theta_d)."""
+    ud = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.empty_like(ud)
+    lib().oc_buie_theta_d_batch(float(csr), int(ng), _p(ud, _f64p), _p(out, _f64p), int(ud.size))
     return out
 
 

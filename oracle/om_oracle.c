@@ -1249,6 +1249,64 @@ static int cylinder_pixeld(d3 o, d3 d, const oc_scene64 *q, int res_x, int res_y
     return row * res_x + col;
 }
 
+/* The FP64 model's Buie table (inter_larger_465_intensity and parameters_generate,
+ * scene_instance_process.cu:54-226, in double): the trapezoid CDF of the profile
+ * cos(0.326 x) / cos(0.308 x) on ng bins of [0, 4.65] mrad, cx, cy [ng + 1] and the slopes
+ * [ng]; c = {int465, A, B, C, 1 / (gamma + 1)} of the power-law tail to 9.3 mrad. */
+void oc_buie_table_d(double csr, int ng, double *cx, double *cy, double *sl, double c[5])
+{
+    double upper = 4.65;
+    double k = 0.9 * log(13.5 * csr) * pow(csr, -0.3);
+    double g = 2.2 * log(0.52 * csr) * pow(csr, 0.43) - 0.1;
+    double i93 = exp(k) / (g + 1) * pow(9.3, g + 1), i465 = exp(k) / (g + 1) * pow(upper, g + 1);
+    double li = upper / ng;
+    cx[0] = 0; cy[0] = 0;
+    double hp = 1.0;
+    for (int i = 1; i <= ng; ++i) {
+        double x = li * i;
+        double hc = cos(0.326 * x) / cos(0.308 * x);
+        double sa = (hp + hc) * li / 2;
+        cx[i] = x;
+        cy[i] = sa + cy[i - 1];
+        sl[i - 1] = li / sa;
+        hp = hc;
+    }
+    c[0] = cy[ng];
+    c[1] = c[0] + (i93 - i465);
+    c[2] = (g + 1) / exp(k);
+    c[3] = pow(upper, g + 1);
+    c[4] = 1 / (g + 1);
+}
+
+/* The FP64 model's sunshape angle in mrad of the draw u (sun_theta in double): the value
+ * v = u A below int465 interpolated in the table's bin (the largest lo with cy[lo] <= v),
+ * else the power-law tail. */
+static double buie_theta_d(const oc_scene64 *q, int ng, double u)
+{
+    double v = u * q->A;
+    if (v < q->int465) {
+        int lo = 0, hi = ng;
+        while (hi - lo > 1) { int md = (lo + hi) / 2; if (q->cdf_y[md] <= v) lo = md; else hi = md; }
+        return (v - q->cdf_y[lo]) * q->slope[lo] + q->cdf_x[lo];
+    }
+    return pow((v - q->int465) * q->B + q->C, q->inv_gp1);
+}
+
+/* buie_theta_d element-wise on the table of (csr, ng) (tests) */
+void oc_buie_theta_d_batch(double csr, int ng, const double *u, double *th, int n)
+{
+    oc_scene64 q;
+    memset(&q, 0, sizeof q);
+    q.cdf_x = malloc(sizeof(double) * (ng + 1));
+    q.cdf_y = malloc(sizeof(double) * (ng + 1));
+    q.slope = malloc(sizeof(double) * ng);
+    double c[5];
+    oc_buie_table_d(csr, ng, q.cdf_x, q.cdf_y, q.slope, c);
+    q.int465 = c[0]; q.A = c[1]; q.B = c[2]; q.C = c[3]; q.inv_gp1 = c[4];
+    for (int i = 0; i < n; ++i) th[i] = buie_theta_d(&q, ng, u[i]);
+    free(q.cdf_x); free(q.cdf_y); free(q.slope);
+}
+
 static void build64(const oc_scene *s, oc_scene64 *q)
 {
     int H = s->H;
@@ -1318,30 +1376,12 @@ static void build64(const oc_scene *s, oc_scene64 *q)
     }
     /* Buie table in double */
     int ng = s->nbins;
-    double csr = s->p.csr, upper = 4.65;
-    double k = 0.9 * log(13.5 * csr) * pow(csr, -0.3);
-    double g = 2.2 * log(0.52 * csr) * pow(csr, 0.43) - 0.1;
-    double i93 = exp(k) / (g + 1) * pow(9.3, g + 1), i465 = exp(k) / (g + 1) * pow(upper, g + 1);
     q->cdf_x = malloc(sizeof(double) * (ng + 1));
     q->cdf_y = malloc(sizeof(double) * (ng + 1));
     q->slope = malloc(sizeof(double) * ng);
-    double li = upper / ng;
-    q->cdf_x[0] = 0; q->cdf_y[0] = 0;
-    double hp = 1.0;
-    for (int i = 1; i <= ng; ++i) {
-        double x = li * i;
-        double hc = cos(0.326 * x) / cos(0.308 * x);
-        double sa = (hp + hc) * li / 2;
-        q->cdf_x[i] = x;
-        q->cdf_y[i] = sa + q->cdf_y[i - 1];
-        q->slope[i - 1] = li / sa;
-        hp = hc;
-    }
-    q->int465 = q->cdf_y[ng];
-    q->A = q->int465 + (i93 - i465);
-    q->B = (g + 1) / exp(k);
-    q->C = pow(upper, g + 1);
-    q->inv_gp1 = 1 / (g + 1);
+    double c[5];
+    oc_buie_table_d(s->p.csr, ng, q->cdf_x, q->cdf_y, q->slope, c);
+    q->int465 = c[0]; q->A = c[1]; q->B = c[2]; q->C = c[3]; q->inv_gp1 = c[4];
 }
 
 static void free64(oc_scene64 *q)
@@ -1450,15 +1490,7 @@ int oc_trace_f64(const oc_scene *s, int h0, int h1,
                     if (s->p.sunshape == OC_SUN_PILLBOX) {
                         th = (double)s->p.pillbox_half_angle * sqrt(u0);
                     } else {
-                        double v = u0 * q.A;
-                        if (v < q.int465) {
-                            int lo = 0, hi = s->nbins;
-                            while (hi - lo > 1) { int md = (lo + hi) / 2; if (q.cdf_y[md] <= v) lo = md; else hi = md; }
-                            th = (v - q.cdf_y[lo]) * q.slope[lo] + q.cdf_x[lo];
-                        } else {
-                            th = pow((v - q.int465) * q.B + q.C, q.inv_gp1);
-                        }
-                        th /= 1000.0;
+                        th = buie_theta_d(&q, s->nbins, u0) / 1000.0;
                     }
                     double ph = 6.283185307179586 * u1;
                     d3 smp = mkd(sin(th) * cos(ph), cos(th), sin(th) * sin(ph));
@@ -1781,15 +1813,7 @@ int oc_diff_splat64(const oc_scene *s, const double *theta, int h0, int h1, cons
                     if (s->p.sunshape == OC_SUN_PILLBOX) {
                         thv = (double)s->p.pillbox_half_angle * sqrt(u0);
                     } else {
-                        double v = u0 * q.A;
-                        if (v < q.int465) {
-                            int lo = 0, hi = s->nbins;
-                            while (hi - lo > 1) { int md = (lo + hi) / 2; if (q.cdf_y[md] <= v) lo = md; else hi = md; }
-                            thv = (v - q.cdf_y[lo]) * q.slope[lo] + q.cdf_x[lo];
-                        } else {
-                            thv = pow((v - q.int465) * q.B + q.C, q.inv_gp1);
-                        }
-                        thv /= 1000.0;
+                        thv = buie_theta_d(&q, s->nbins, u0) / 1000.0;
                     }
                     double ph = 6.283185307179586 * u1;
                     d3 smp = mkd(sin(thv) * cos(ph), cos(thv), sin(thv) * sin(ph));

@@ -110,6 +110,10 @@ float oc_gauss_from_u(float u);
 void  oc_math_batch(int fn, const float *in, float *out, int n);
 /* the FP64 model's erfinv (Newton on libm's erf from oc_erfinvf), element-wise */
 void  oc_erfinv_d_batch(const double *in, double *out, int n);
+/* the FP64 model's Buie table: cx, cy [ng + 1], slopes [ng], c = {int465, A, B, C, 1/(gamma+1)};
+   and its sunshape angle (mrad) of draws u, element-wise */
+void  oc_buie_table_d(double csr, int ng, double *cx, double *cy, double *sl, double c[5]);
+void  oc_buie_theta_d_batch(double csr, int ng, const double *u, double *th, int n);
 void  oc_philox_batch(const uint32_t *ctr /*[n][4]*/, const uint32_t key[2], uint32_t *out, int n);
 
 /* ---- scene construction: SolarScene::InitContent (solar_scene.cpp:83-99) ---- */
