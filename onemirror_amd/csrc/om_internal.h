@@ -185,6 +185,7 @@ struct MixArgs {
     // (k - tail_k0) tail_du + tail_dv0 (tail_du = 2^-23 A, tail_dv0 = u(tail_k0) A - int465 in
     // double), free of the cancellation of u A - int465 in float
     float tail_du, tail_dv0;
+    double rgi[3];     // 1 / the launch grid's cell edges, rounded (the start cell's quotients)
     const HelioMix *hmix;
     v3 s_u, s_n, s_v;  // the model's sun frame rounded (identity axes for an identity frame)
     v3 sun;            // the sun direction rounded (the energy's)

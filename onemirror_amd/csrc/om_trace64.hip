@@ -617,13 +617,15 @@ __device__ __forceinline__ MixStart mix_start(const TraceArgs &a, const MixArgs 
     if (POISSON) {
         const float2 pt = a.poisson[(size_t)h * a.poisson_cap + m];
         int br = 0, bc = 0;
+        // the facet of the point: quotients by the hardware reciprocal (within 1.5 ulp of
+        // the correctly rounded ones, inside the margin's 4 2^-24 |q|)
         if (hr->rows > 1) {
-            const float q = pt.y / hr->subrow_len, g = fabs_(q - rintf(q));
+            const float q = pt.y * __builtin_amdgcn_rcpf(hr->subrow_len), g = fabs_(q - rintf(q));
             br = trunc_i(q);
             s.ok = s.ok && g > fma_(fabs_(q), 4.0f * U24, 1e-6f);
         }
         if (hr->cols > 1) {
-            const float q = pt.x / hr->subcol_len, g = fabs_(q - rintf(q));
+            const float q = pt.x * __builtin_amdgcn_rcpf(hr->subcol_len), g = fabs_(q - rintf(q));
             bc = trunc_i(q);
             s.ok = s.ok && g > fma_(fabs_(q), 4.0f * U24, 1e-6f);
         }
@@ -642,7 +644,9 @@ __device__ __forceinline__ MixStart mix_start(const TraceArgs &a, const MixArgs 
     const double rx = ((double)pos.x + (double)s.w.x) - (double)a.gpx;
     const double ry = ((double)pos.y + (double)s.w.y) - (double)a.gpy;
     const double rz = ((double)pos.z + (double)s.w.z) - (double)a.gpz;
-    const double qx = rx / (double)a.gix, qy = ry / (double)a.giy, qz = rz / (double)a.giz;
+    // the quotients by the rounded reciprocals: within 2^-51 |q| of r / gi, which the margins
+    // below cover with their 2^-30 (|q| < 2^16 cells)
+    const double qx = rx * x.rgi[0], qy = ry * x.rgi[1], qz = rz * x.rgi[2];
     s.c.gx = (int)qx;
     s.c.gy = (int)qy;
     s.c.gz = (int)qz;
@@ -651,7 +655,8 @@ __device__ __forceinline__ MixStart mix_start(const TraceArgs &a, const MixArgs 
     s.c.ry = (float)(ry - (double)s.c.gy * (double)a.giy);
     s.c.rz = (float)(rz - (double)s.c.gz * (double)a.giz);
     const double gx = qx - (double)s.c.gx, gy = qy - (double)s.c.gy, gz = qz - (double)s.c.gz;
-    const float mx = s.ew / a.gix * 1.01f, my = s.ew / a.giy * 1.01f, mz = s.ew / a.giz * 1.01f;
+    const float mx = fma_(s.ew * a.inv_gix, 1.01f, 9.3132257e-10f), my = fma_(s.ew * a.inv_giy, 1.01f, 9.3132257e-10f),
+                mz = fma_(s.ew * a.inv_giz, 1.01f, 9.3132257e-10f);
     s.ok = s.ok && (float)fmin(gx, 1.0 - gx) > mx && (float)fmin(gy, 1.0 - gy) > my && (float)fmin(gz, 1.0 - gz) > mz;
     return s;
 }
@@ -1186,6 +1191,7 @@ void mixed_args(const om_scene *s, int g, TraceArgs &a, MixArgs &x)
                      : 0.0f;
     const GridState &G = s->grids[g];
     x.gi_max = std::max(std::max(G.interval[0], G.interval[1]), G.interval[2]);
+    for (int k = 0; k < 3; ++k) x.rgi[k] = 1.0 / (double)G.interval[k];
     x.buie_vmax = s->buie_vmax_m;
     x.redo_n = s->d_mix_n;
 }
