@@ -210,7 +210,7 @@ om_status om_ctx_create(int device, om_ctx **out);
 om_status om_ctx_destroy(om_ctx *ctx);
 
 /* Abort and progress of the persistent launches of a context: the window traces
- * (OM_ACC_WINDOW, FP32 and FP64) and both persistent batch launches.  No reference
+ * (OM_ACC_WINDOW, every precision) and both persistent batch launches.  No reference
  * counterpart (the reference exits the process on any error, utils.h:13-22); the caller's
  * watchdog uses them from any thread while a launch runs.
  * om_ctx_abort(ctx, 1) copies a nonzero word into device memory on a stream of the

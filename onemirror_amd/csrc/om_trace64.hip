@@ -1100,10 +1100,14 @@ __global__ __launch_bounds__(WG) void trace_mixed_records_kernel(MixLaunch)
 
 } // namespace
 
-// Clears n words at p when the mixed launch's redo list filled up (*gate > cap)
-__global__ void clear_if_full_kernel(const uint32_t *gate, uint32_t cap, uint32_t *p, long long n)
+// When the mixed launch's redo list filled up (*gate > cap): clears n words at p and the
+// chunk counters of the launch's queue (whose item count and layout the FP64 window trace
+// shares), so that the FP64 trace takes every chunk again.  Otherwise the queue keeps the
+// mixed launch's counts, which om_ctx_progress reports.
+__global__ void clear_if_full_kernel(const uint32_t *gate, uint32_t cap, uint32_t *p, long long n, uint32_t *queue)
 {
     if (*gate <= cap) return;
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)QUEUE_COUNTERS) queue[threadIdx.x] = 0u;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         p[i] = 0u;
 }
@@ -1223,15 +1227,12 @@ om_status launch_trace_mixed_window(om_scene *s, const TraceArgs &a, const Trace
     }
 #undef OM_MIX
     OM_HIP(hipGetLastError());
-    // A full list: the packed copies and side maps are cleared and the whole launch is traced
-    // again by the FP64 window trace, whose workgroups overwrite their window slices.  Both
-    // kernels read the list's count and return at once when it fitted.
-    const int chunk = std::min(a.tile_rays, window_chunk_rays());
-    if ((rc = launch_queue_reset(a.win_queue, (long long)a.ntiles * (a.tile_rays / chunk), QUEUE_EIGHTHS, st)) !=
-        OM_OK)
-        return rc;
+    // A full list: the packed copies and side maps are cleared, the queue's chunk counters
+    // zeroed, and the whole launch is traced again by the FP64 window trace, whose workgroups
+    // overwrite their window slices.  Both kernels read the list's count and return at once
+    // when it fitted.
     hipLaunchKernelGGL(clear_if_full_kernel, dim3(1024), dim3(WG), 0, st, x.redo_n, (uint32_t)x.redo_cap, acc,
-                       acc_words);
+                       acc_words, a.win_queue);
     OM_HIP(hipGetLastError());
     Trace64Args bg = b;
     bg.gate = x.redo_n;

@@ -2,12 +2,14 @@
 the host: the kernels read a flag the host sets, and every launch ends normally.
 
 Pins:
-  * with the abort word set before a window trace (FP32 and FP64 geometry) or a persistent
+  * with the abort word set before a window trace (FP32, FP64 and mixed geometry) or a persistent
     batch (LDS windows, and work items), no chunk is traced: the maps are all zero, and
     progress reports the launch's total with at most one chunk per wave (or per workgroup
     for work items) taken;
   * after om_ctx_abort(ctx, 0) the same calls give the maps of a context that never aborted
-    (for the window traces, the oracle's), and progress reports every chunk taken;
+    (for the window traces, the oracle's: the FP64 model's counts for FP64 and mixed), and
+    progress reports every chunk taken (for the mixed trace, its own launch's: the FP64
+    fallback of a full redo list does not run);
   * a watchdog thread that polls progress while a stream of 40 traces of the 5,000-heliostat
     field runs, and sets the abort word once it has seen progress, ends the stream early (its
     wall time is under half of the stream's without abort); the progress it read never
@@ -37,14 +39,14 @@ def _maps(m):
     return m.counts.cpu().numpy().view(np.uint32).copy(), m.energy.cpu().numpy().view(np.uint64).copy()
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "mixed"])
 def test_window_trace_aborted_then_cleared(om, precision):
     ctx = om.Context(0)
     d, p = F.config_mid()
     d = F.subset(d, slice(2000, 2300))
     p = dict(p, accumulate=om.ACC_WINDOW)
-    if precision == "fp64":
-        p = dict(p, precision=om.PREC_FP64)
+    if precision != "fp32":
+        p = dict(p, precision=om.PREC_FP64 if precision == "fp64" else om.PREC_MIXED)
     g = om.SolarScene(d, p, ctx=ctx)
     m = g.new_maps(smooth=False)
     print(f"abort: window trace {precision}", flush=True)
@@ -61,7 +63,7 @@ def test_window_trace_aborted_then_cleared(om, precision):
     counts, energy = _maps(m)
     assert ctx.progress() == (total, total)
     o = oracle.OracleScene(d, p)
-    want = o.trace_f64() if precision == "fp64" else o.trace_f32()
+    want = o.trace_f32() if precision == "fp32" else o.trace_f64()
     assert np.array_equal(counts.ravel(), want["counts"].ravel())
     if precision == "fp32":
         assert np.array_equal(energy.ravel(), want["energy"].ravel())

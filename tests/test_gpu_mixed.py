@@ -14,7 +14,8 @@ the pins are those of the FP64 trace:
   * the FP32 flux map is within 1e-5 relative per lit pixel of the FP64 model's flux;
   * the share of rays sent to the FP64 trace is small (printed; bounded at 2 %);
   * a redo list that fills up sends the whole launch to the FP64 trace: the maps are then
-    the FP64 trace's, bit for bit;
+    the FP64 trace's, bit for bit, and om_ctx_progress reports that trace's chunks, all
+    taken; a list that fits leaves the mixed launch's own chunk counts to om_ctx_progress;
   * a per-scene batch of mixed scenes equals their plain traces; the persistent batch launch
     and the differentiable mode refuse mixed scenes.
 """
@@ -110,6 +111,9 @@ def test_mid_whole_field_counts_equal_the_fp64_model(om):
     g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
     redo = _check_against_model(g, o.trace_f64(), "mid whole field")
     assert redo > 0  # the bounds do send rays to FP64 on a field of this size
+    # the list fitted, so progress reports the mixed launch's own chunks, all taken
+    taken, total = g.ctx.progress()
+    assert total > 0 and taken == total
 
 
 def test_full_redo_list_traces_the_launch_in_fp64(om):
@@ -121,6 +125,9 @@ def test_full_redo_list_traces_the_launch_in_fp64(om):
     got = _maps(g)
     assert g.redo_count() > 1
     assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    # the FP64 retrace took every chunk of the launch's queue again
+    taken, total = g.ctx.progress()
+    assert total > 0 and taken == total
     # a second trace on the same scene starts from cleared buffers
     again = _maps(g)
     assert np.array_equal(again[0], want[0]) and np.array_equal(again[1], want[1])
