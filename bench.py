@@ -45,10 +45,11 @@ GEOMETRY = {
     "fp32": "FP32: the reference's ray geometry, bit-exact against its host restatement",
     "fp64": ("FP64: sampling, frames, walks, intersections and receiver in double (om_params.precision = "
              "OM_PREC_FP64), energies in exact fixed point, FP32 flux map"),
-    "mixed": ("mixed (om_params.precision = OM_PREC_MIXED): FP32 sampling and walks whose every decision is "
-              "certified by an error bound, directions, frames and receiver hit in double, rays with a decision "
-              "inside its bound traced again in FP64: the FP64 model's shading, blocking and pixel for every "
-              "ray; energies in exact fixed point, FP32 flux map"),
+    "mixed": ("mixed (om_params.precision = OM_PREC_MIXED): FP32 sampling, walks and receiver hit, the "
+              "direction as an FP32 deviation from the FP64 model's nominal reflection, every decision "
+              "certified by an error bound; rays with a decision inside its bound traced again in FP64: the "
+              "FP64 model's shading, blocking and pixel for every ray; energies in exact fixed point, FP32 "
+              "flux map"),
 }
 
 

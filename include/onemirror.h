@@ -132,11 +132,13 @@ typedef struct {
                                   traces, records and per-scene batches; not the persistent
                                   batch launch or the differentiable mode.
                                   OM_PREC_MIXED: the FP64 model's outcome of every ray at
-                                  close to FP32 cost.  Sampling and both walks in FP32,
-                                  directions, frames and the receiver hit in double; every
-                                  FP32 decision (walk step, candidate test, pixel) is taken
-                                  only outside its rounding-error bound, and the rays with a
-                                  decision inside it are traced again in FP64.  Shading,
+                                  close to FP32 cost.  Sampling, both walks and the receiver
+                                  hit in FP32, each direction an FP32 deviation from the
+                                  model's nominal reflection of its heliostat (computed in
+                                  double); every FP32 decision (start cell, walk step,
+                                  candidate test, pixel) is taken only outside its error
+                                  bound, and the rays with a decision inside it are traced
+                                  again in FP64.  Shading,
                                   blocking and pixel of every ray are the FP64 model's;
                                   energies differ from it by the FP32 draws (about 1e-8
                                   relative).  Rectangle and cylinder receivers; scenes with

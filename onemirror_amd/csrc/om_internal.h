@@ -147,9 +147,10 @@ struct Trace64Args {
 };
 
 // ---- Certified FP32 geometry (om_params.precision = OM_PREC_MIXED) ----------------------
-// A ray is sampled in FP32, its directions, frames and receiver hit are formed in double
-// from the FP64 model's scene, and both walks run in FP32 over float images of the model's
-// vertices.  Every decision of the FP32 arithmetic (a walk step's axis, a candidate's hit
+// A ray is sampled in FP32; its direction is the FP64 model's nominal reflection of its
+// heliostat (HelioMix, from the model's doubles) plus an FP32 deviation, its receiver hit is
+// found in FP32 relative to the receiver's reference point, and both walks run in FP32 over
+// float images of the model's vertices.  Every decision of the FP32 arithmetic (a walk step's axis, a candidate's hit
 // test, the receiver pixel) is taken only when it holds with the error bounds below: a ray
 // with a decision inside its bound is traced again by the FP64 trace.  The bounds, for every
 // ray of the scene:

@@ -537,12 +537,15 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
 // ---------------------------------------------------------------------------------------
 // OM_PREC_MIXED: the FP64 model's outcome of every ray at close to FP32 cost
 // ---------------------------------------------------------------------------------------
-// A ray is sampled in FP32 (the same Philox draws); its sun direction, perturbed normal and
-// reflection are formed in double with the model's frames, and its receiver hit in double
-// with the model's receiver.  Both walks run in FP32 over float images of the model's
-// vertices.  Each FP32 decision is taken only when it holds for every value within its
-// error bound (MixArgs); a ray with a decision inside its bound is put on a list and traced
-// by the FP64 trace afterwards.  So every ray's shading heliostat, blocking heliostat and
+// A ray is sampled in FP32 (the same Philox draws).  Its direction is the model's nominal
+// reflection of the heliostat (computed in double at upload, HelioMix) plus a deviation
+// formed in FP32 from the small sunshape and slope-error angles, and its receiver hit is
+// found in FP32 relative to the receiver's reference point, from the model's nominal miss
+// vector and the deviation.  Both walks run in FP32 over float images of the model's
+// vertices, from the origin's offset inside its cell (formed in double).  Each FP32
+// decision is taken only when it holds for every value within its error bound (MixArgs); a
+// ray with a decision inside its bound is put on a list and traced by the FP64 trace
+// afterwards.  So every ray's shading heliostat, blocking heliostat and
 // pixel are the model's; its energy |sun.n| is the model's to the FP32 draws' error.
 
 constexpr int MIX_UNSURE = -3;

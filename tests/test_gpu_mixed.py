@@ -1,9 +1,10 @@
 """The mixed-precision trace (om_params.precision = OM_PREC_MIXED) on the GPU against the
 oracle's FP64 model of the same rays (oracle trace_f64).
 
-The mixed trace samples in FP32, forms directions and the receiver hit in double and walks in
-FP32 with every decision certified by an error bound; rays with a decision inside its bound
-are traced again by the FP64 trace.  Its claim is the FP64 model's outcome for every ray, so
+The mixed trace samples, walks and meets the receiver in FP32, with each ray's direction an
+FP32 deviation from the FP64 model's nominal reflection of its heliostat, and every decision
+certified by an error bound; rays with a decision inside its bound are traced again by the
+FP64 trace.  Its claim is the FP64 model's outcome for every ray, so
 the pins are those of the FP64 trace:
   * per-ray records (shading heliostat, blocking heliostat, receiver pixel) equal the FP64
     model's on the 100-heliostat field (Poisson and lattice microsamples, pillbox sun), the
