@@ -333,8 +333,8 @@ __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dfr
                              int h, int m, int l, int self, bool clear)
 {
     Ray64 r{-1, -2, -1, make_int2(-1, -1), 0u};
-    // the key words read as wave-uniform values (in ray64_at, an out-of-line function, a's
-    // fields arrive through a vector address)
+    // the key words read as wave-uniform values (a's fields can arrive through a vector
+    // address where the caller's argument struct is not the kernel's)
     const u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3},
                                     __builtin_amdgcn_readfirstlane(a.key0), __builtin_amdgcn_readfirstlane(a.key1));
     const double u0 = u01(x.x), u1 = u01(x.y), u2 = u01(x.z), u3 = u01(x.w);
@@ -941,9 +941,9 @@ __device__ __forceinline__ MixRay trace_ray_mixed(const TraceArgs &a, const Trac
 }
 
 // Ray r of heliostat h traced by trace_ray64 from nothing but its indices: the rays of the
-// redo list, and of a full list.  Out of line: the FP64 ray's registers are held only here.
+// redo list, and the unsure rays of the records kernel.
 template <int SUN, bool POISSON>
-__device__ __noinline__ Ray64 ray64_at(const TraceArgs &a, const Trace64Args &b, int h, int r)
+__device__ __forceinline__ Ray64 ray64_at(const TraceArgs &a, const Trace64Args &b, int h, int r)
 {
     const HelioRec *hr = a.helio + h;
     const Helio64 &h64 = b.helio[h];
