@@ -988,8 +988,14 @@ __device__ __forceinline__ void trace_chunk_mixed(const MixLaunch &L, int2 tl, c
     int m = r / Lr;
     int l = r - m * Lr;
     const int dm = 64 / Lr, dl = 64 - (64 / Lr) * Lr;
-    int m_o = -1; // microsample of ms
-    MixStart ms{};
+    // The starts of 64 consecutive microsamples mt, mt + 1, ..., one per lane: w, the cell
+    // and the offset inside it (rx NaN: not certain).  A pass spans at most 64 microsamples,
+    // so the table is refilled only when a pass reaches past it: once per L passes, once per
+    // chunk for L >= 32.  Each pass reads its rays' starts from the lanes that hold them.
+    int mt = -(1 << 30);
+    float tw0 = 0.0f, tw1 = 0.0f, tw2 = 0.0f, tr0 = 0.0f, tr1 = 0.0f, tr2 = 0.0f;
+    int tg0 = 0, tg1 = 0, tg2 = 0;
+    const v3 pos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
     for (int k = 0; k < nrays; k += 64, r += 64) {
         if (k > 0) {
             m += dm;
@@ -1003,11 +1009,40 @@ __device__ __forceinline__ void trace_chunk_mixed(const MixLaunch &L, int2 tl, c
         int px = 0;
         uint32_t q = 0u;
         asm volatile("" ::: "memory"); // arguments re-read per ray pass
-        if (r < nr) {
-            if (m != m_o) {
-                ms = mix_start<POISSON>(a, L.x, hr, hm, h, m);
-                m_o = m;
+        // the whole wave takes part in the refill and the reads
+        if (__ballot(r < nr && m - mt >= 64) != 0ull) {
+            mt = __builtin_amdgcn_readfirstlane(m); // lane 0's: the pass's first microsample
+            const int mj = mt + lane;
+            if (mj < hr->mcount) {
+                const MixStart t = mix_start<POISSON>(a, L.x, hr, hm, h, mj);
+                tw0 = t.w.x;
+                tw1 = t.w.y;
+                tw2 = t.w.z;
+                tg0 = t.c.gx;
+                tg1 = t.c.gy;
+                tg2 = t.c.gz;
+                tr0 = t.ok ? t.c.rx : __builtin_nanf("");
+                tr1 = t.c.ry;
+                tr2 = t.c.rz;
             }
+        }
+        const int src = min(max(m - mt, 0), 63) << 2;
+        MixStart ms;
+        ms.w = mk(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(tw0))),
+                  __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(tw1))),
+                  __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(tw2))));
+        ms.c.gx = __builtin_amdgcn_ds_bpermute(src, tg0);
+        ms.c.gy = __builtin_amdgcn_ds_bpermute(src, tg1);
+        ms.c.gz = __builtin_amdgcn_ds_bpermute(src, tg2);
+        ms.c.rx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(tr0)));
+        ms.c.ry = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(tr1)));
+        ms.c.rz = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(tr2)));
+        if (r < nr) {
+            // as mix_start forms them
+            ms.of = pos + ms.w;
+            ms.ew = fma_(norm1(ms.w), 4.7683716e-07f, 1e-12f);
+            ms.c.addr = (ms.c.gx * a.gny + ms.c.gy) * a.gnz + ms.c.gz;
+            ms.ok = ms.c.rx == ms.c.rx;
             const MixRay o = trace_ray_mixed<SUN>(a, L.b, L.x, hm, ms, h, m, l, h, clear);
             if (o.shade == MIX_UNSURE) {
                 flag = true;

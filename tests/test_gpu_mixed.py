@@ -7,8 +7,9 @@ certified by an error bound; rays with a decision inside its bound are traced ag
 FP64 trace.  Its claim is the FP64 model's outcome for every ray, so
 the pins are those of the FP64 trace:
   * per-ray records (shading heliostat, blocking heliostat, receiver pixel) equal the FP64
-    model's on the 100-heliostat field (Poisson and lattice microsamples, pillbox sun), the
-    cylinder field, and slices of the 5,000- and 25,000-heliostat fields (Buie sun);
+    model's on the 100-heliostat field (Poisson and lattice microsamples, pillbox sun; 64, 5
+    and 1 rays per microsample), the cylinder field, and slices of the 5,000- and
+    25,000-heliostat fields (Buie sun);
   * the integer count maps of the window trace equal the FP64 model's counts, on the small
     fields and on the whole 5,000-heliostat field (2e8 rays): a ray whose FP32 decision
     differed from the model's without being sent to FP64 would show as a count off by one;
@@ -69,14 +70,19 @@ def _check_against_model(g, want, label):
     return redo
 
 
-@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder"])
+@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder", "small_L5", "small_L1"])
 def test_small_fields_records_and_maps(om, cfg):
+    """small_L5, small_L1: 5 and 1 rays per microsample, so that a wave's chunk of rays spans
+    more than the 64 microsample starts a wave holds at a time (its table is refilled every
+    5 passes, and every pass)."""
     if cfg == "cylinder":
         d, p = F.config_small_cylinder()
     else:
         d, p = F.config_small()
         if cfg == "small_lattice":
             p = F.with_lattice(p)
+        elif cfg.startswith("small_L"):
+            p = dict(p, rays_per_sample=int(cfg[7:]))
     p = _mixed(p, om)
     g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
     want = o.trace_f64(records=True)
