@@ -95,6 +95,54 @@ __device__ __forceinline__ RayStart ray_start(const TraceArgs &a, const HelioTil
     return st;
 }
 
+// The starts of 64 consecutive microsamples mt, mt + 1, ... of a tile's heliostat, one per
+// lane of a wave (the origin and its cell).  A wave's pass of 64 rays spans at most 64
+// microsamples, so the table is refilled only when a pass reaches past it: once per L
+// passes, once per chunk of 2048 rays for L >= 32.  Each pass reads its rays' starts from
+// the lanes that hold them (ds_bpermute) instead of finding them again: with fewer than 64
+// rays per microsample a lane's microsample changes at every pass.  The whole wave calls
+// both members.
+struct StartTable {
+    int mt = -(1 << 30);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    int gx = 0, gy = 0, gz = 0;
+
+    template <bool POISSON>
+    __device__ __forceinline__ void update(const TraceArgs &a, const HelioTile &t, bool active, int m, int lane)
+    {
+        if (__ballot(active && m - mt >= 64) == 0ull) return;
+        mt = __builtin_amdgcn_readfirstlane(m); // lane 0's: the pass's first microsample
+        const int mj = mt + lane;
+        if (mj < t.hr->mcount) {
+            const RayStart st = ray_start<POISSON>(a, t, mj);
+            ox = st.orig.x;
+            oy = st.orig.y;
+            oz = st.orig.z;
+            gx = st.c0.gx;
+            gy = st.c0.gy;
+            gz = st.c0.gz;
+        }
+    }
+    // the start of microsample m, mt <= m < mt + 64, as ray_start gives it
+    __device__ __forceinline__ RayStart get(const TraceArgs &a, int m) const
+    {
+        const int src = min(max(m - mt, 0), 63) << 2;
+        RayStart st;
+        st.orig = mk(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(ox))),
+                     __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(oy))),
+                     __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(oz))));
+        st.c0.gx = __builtin_amdgcn_ds_bpermute(src, gx);
+        st.c0.gy = __builtin_amdgcn_ds_bpermute(src, gy);
+        st.c0.gz = __builtin_amdgcn_ds_bpermute(src, gz);
+        // as start_cell forms them
+        st.c0.rx = st.orig.x - a.gpx;
+        st.c0.ry = st.orig.y - a.gpy;
+        st.c0.rz = st.orig.z - a.gpz;
+        st.c0.addr = (st.c0.gx * a.gny + st.c0.gy) * a.gnz + st.c0.gz;
+        return st;
+    }
+};
+
 // Ray l of microsample m of the tile's heliostat from its origin rs, up to the receiver:
 // map_tracing (rectgrid_raytracing.cu:155-196) without its last step.  The ray goes on to
 // the receiver when shade and block are both -1; refl is then its reflected direction and
@@ -330,8 +378,7 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
     int m = r / L;
     int l = r - m * L;
     const int dm = 64 / L, dl = 64 - (64 / L) * L;
-    RayStart rs{};
-    int m_rs = -1; // microsample of rs
+    StartTable tab;
     for (int k = 0; k < nrays; k += 64, r += 64) {
         if (k > 0) {
             m += dm;
@@ -345,11 +392,9 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
         int px = 0;
         uint32_t q = 0u;
         asm volatile("" ::: "memory"); // see trace_window_kernel
+        tab.update<POISSON>(a, t, r < nr, m, lane);
+        const RayStart rs = tab.get(a, m);
         if (r < nr) {
-            if (m != m_rs) {
-                rs = ray_start<POISSON>(a, t, m);
-                m_rs = m;
-            }
             const RayOut o = trace_from<SUN, true>(a, t, rs, m, l, a.buie_bins, a.buie_guide);
             if (o.rc.x >= 0) {
                 px = o.rc.x * a.res_x + o.rc.y;
@@ -423,8 +468,7 @@ __device__ __forceinline__ void trace_chunk_compact(const TraceArgs &a, int2 tl,
     int m = r / L;
     int l = r - m * L;
     const int dm = 64 / L, dl = 64 - (64 / L) * L;
-    RayStart rs{};
-    int m_rs = -1;
+    StartTable tab;
     for (int k = 0; k < nrays; k += 64, r += 64) {
         if (k > 0) {
             m += dm;
@@ -438,11 +482,9 @@ __device__ __forceinline__ void trace_chunk_compact(const TraceArgs &a, int2 tl,
         v3 orig = mk(0.0f, 0.0f, 0.0f), refl = mk(0.0f, 0.0f, 0.0f);
         uint32_t q = 0u;
         asm volatile("" ::: "memory"); // see trace_window_kernel
+        tab.update<POISSON>(a, t, r < nr, m, lane);
+        const RayStart rs = tab.get(a, m);
         if (r < nr) {
-            if (m != m_rs) {
-                rs = ray_start<POISSON>(a, t, m);
-                m_rs = m;
-            }
             const RayOut o = trace_walks<SUN, true>(a, t, rs, m, l, a.buie_bins, a.buie_guide, refl);
             live = o.shade < 0 && o.block < 0;
             orig = rs.orig;
@@ -589,8 +631,7 @@ __device__ __forceinline__ void trace_chunk_batch(const TraceArgs &a, int4 ck, c
     int m = r / L;
     int l = r - m * L;
     const int dm = 64 / L, dl = 64 - (64 / L) * L;
-    RayStart rs{};
-    int m_rs = -1;
+    StartTable tab;
     for (int k = 0; k < nrays; k += 64, r += 64) {
         if (k > 0) {
             m += dm;
@@ -604,11 +645,9 @@ __device__ __forceinline__ void trace_chunk_batch(const TraceArgs &a, int4 ck, c
         int px = 0;
         uint32_t q = 0u;
         asm volatile("" ::: "memory");
+        tab.update<POISSON>(a, t, r < nr, m, lane);
+        const RayStart rs = tab.get(a, m);
         if (r < nr) {
-            if (m != m_rs) {
-                rs = ray_start<POISSON>(a, t, m);
-                m_rs = m;
-            }
             const RayOut o = trace_from<SUN, true>(a, t, rs, m, l, a.buie_bins, a.buie_guide);
             if (o.rc.x >= 0) {
                 px = o.rc.x * a.res_x + o.rc.y;

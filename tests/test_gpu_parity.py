@@ -298,11 +298,13 @@ def test_accumulation_modes_bitexact(om, mode, shift, compact):
                                                (64, False, 0), (65, True, 0), (100, False, 0), (2048, True, 0),
                                                (1, False, 1), (7, True, 1), (65, False, 1), (2048, True, 1)])
 def test_window_sample_blocks_bitexact(om, L, lattice, compact):
-    """The window kernel traces sample blocks (one microsample per lane, its rays one per
-    pass): rays per microsample below, at and above one part of 64, runs of several
-    microsamples per lane (L = 1, 7) and a heliostat's last, partial run all give the
-    oracle's maps, with and without ray compaction (rays held across passes, chunks and
-    heliostats until a wave is full, the rest run after the wave's last chunk)."""
+    """The window kernel traces passes of 64 consecutive rays and holds the starts of 64
+    consecutive microsamples, one per lane, refilled when a pass reaches past them: rays per
+    microsample below, at and above 64 (a pass within one microsample, across several, and
+    refills at every pass for L = 1 or every few for L = 7, 8), and a heliostat's last,
+    partial pass all give the oracle's maps, with and without ray compaction (rays held
+    across passes, chunks and heliostats until a wave is full, the rest run after the
+    wave's last chunk)."""
     import torch
     d, p = F.config_mid()
     n = max(4, min(200, 8000 // L))
