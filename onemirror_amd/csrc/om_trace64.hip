@@ -468,6 +468,15 @@ __global__ __launch_bounds__(WG) void trace64_kernel(Trace64Launch)
 // nrays (a multiple of 64) rays of heliostat tl.x from tl.y traced by one wave of a window
 // workgroup, one ray per lane per pass: hits inside the LDS window are LDS atomics, the
 // others go to the wave's buffer (window_body, om_window.h).
+// v of the lane src / 4 (ds_bpermute of both halves)
+__device__ __forceinline__ double bperm_d(int src, double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <int SUN, bool POISSON>
 __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const Trace64Args &b, int2 tl, const int lane,
                                                      const int nrays, WaveOut &wo, unsigned long long *s_win)
@@ -489,22 +498,45 @@ __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const T
     const bool clear = hr->clear != 0 && !a.literal_self;
     const int L = a.L;
     const int nr = min(hr->mcount * L, tl.y + nrays);
-    int m_o = -1; // microsample of orig
-    d3 orig{0.0, 0.0, 0.0};
-    Cell64 c0{};
-    for (int k = 0; k < nrays; k += 64) {
-        const int r = tl.y + k + lane;
+    int r = tl.y + lane;
+    int m = r / L;
+    int l = r - m * L;
+    const int dm = 64 / L, dl = 64 - (64 / L) * L;
+    // The origins and cells of 64 consecutive microsamples mt, mt + 1, ..., one per lane,
+    // refilled when a pass reaches past them (once per chunk for L >= 32); each pass reads
+    // its rays' from the lanes that hold them, as the FP32 and mixed window traces do.
+    int mt = -(1 << 30);
+    d3 to{0.0, 0.0, 0.0};
+    int tg0 = 0, tg1 = 0, tg2 = 0;
+    for (int k = 0; k < nrays; k += 64, r += 64) {
+        if (k > 0) {
+            m += dm;
+            l += dl;
+            if (l >= L) {
+                l -= L;
+                ++m;
+            }
+        }
         bool out = false;
         int px = 0;
         uint32_t q = 0u;
         asm volatile("" ::: "memory"); // arguments re-read per ray pass
-        if (r < nr) {
-            const int m = r / L, l = r - m * L;
-            if (m != m_o) {
-                orig = add(world(micro_local64<POISSON>(a, b, hr, h64, h, m), hf), hpos);
-                c0 = cell64(b, orig);
-                m_o = m;
+        if (__ballot(r < nr && m - mt >= 64) != 0ull) { // the whole wave takes part
+            mt = __builtin_amdgcn_readfirstlane(m);     // lane 0's: the pass's first microsample
+            const int mj = mt + lane;
+            if (mj < hr->mcount) {
+                to = add(world(micro_local64<POISSON>(a, b, hr, h64, h, mj), hf), hpos);
+                const Cell64 c = cell64(b, to);
+                tg0 = c.g[0];
+                tg1 = c.g[1];
+                tg2 = c.g[2];
             }
+        }
+        const int src = min(max(m - mt, 0), 63) << 2;
+        const d3 orig{bperm_d(src, to.x), bperm_d(src, to.y), bperm_d(src, to.z)};
+        const Cell64 c0{{__builtin_amdgcn_ds_bpermute(src, tg0), __builtin_amdgcn_ds_bpermute(src, tg1),
+                         __builtin_amdgcn_ds_bpermute(src, tg2)}};
+        if (r < nr) {
             const Ray64 o = trace_ray64<SUN>(a, b, hf, sf, orig, c0, h, m, l, self, clear);
             if (o.px >= 0) {
                 px = o.px;

@@ -55,7 +55,8 @@ def _flux_err(g, o, want):
     return np.abs(flux32[lit] - flux64[lit]) / flux64[lit]
 
 
-@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder", "cone", "small_window", "cylinder_window"])
+@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder", "cone", "small_window", "cylinder_window",
+                                 "small_L7_window"])
 def test_small_fields_records_counts_flux(om, cfg):
     if cfg.startswith("cylinder"):
         d, p = F.config_small_cylinder()
@@ -65,6 +66,8 @@ def test_small_fields_records_counts_flux(om, cfg):
         d, p = F.config_small()
         if cfg == "small_lattice":
             p = F.with_lattice(p)
+        elif cfg == "small_L7_window":  # 7 rays per microsample: the window trace's table
+            p = dict(p, rays_per_sample=7)  # of microsample starts is refilled every 7 passes
     # "_window": the LDS-window accumulation (the map equals the packed one)
     p = _fp64(p, om)
     if cfg.endswith("_window"):
