@@ -619,9 +619,12 @@ __device__ __forceinline__ int parallelogram_cert(const MixArgs &x, v3 o, v3 d, 
     const float mdet = E1.w * E2.w * (x.e_dir + 8.0f * U24);
     const float ai = fabs_(inv), adet = fabs_(det);
     const float au = fabs_(u), av = fabs_(v), at = fabs_(t);
-    const float mu = fma_(au, mdet, E2.w * c0) * ai + 4.0f * U24 * au;
-    const float mv = fma_(av, mdet, E1.w * c0) * ai + 4.0f * U24 * av;
-    const float mt = fma_(at, mdet, E1.w * E2.w * c0) * ai + 4.0f * U24 * at;
+    // (|u| mdet + |E2| c0) / |det| + 4 2^-24 |u|, and likewise, with the factors shared by the
+    // three formed once (four roundings each, as in the direct form)
+    const float k4 = fma_(mdet, ai, 4.0f * U24), ci = c0 * ai;
+    const float mu = fma_(au, k4, E2.w * ci);
+    const float mv = fma_(av, k4, E1.w * ci);
+    const float mt = fma_(at, k4, E1.w * E2.w * ci);
     const bool miss = adet < 1e-6f - mdet || u < -mu || u > 1.0f + mu || v < -mv || v > 1.0f + mv || t < 1e-6f - mt;
     const bool hit = adet >= 1e-6f + mdet && u >= mu && u <= 1.0f - mu && v >= mv && v <= 1.0f - mv && t >= 1e-6f + mt;
     return hit ? 1 : (miss ? 0 : -1);
@@ -734,8 +737,9 @@ __device__ int walk_cert(const TraceArgs &a, const MixArgs &x, v3 o, const cell3
         const bool sx = mx < my && mx < mz;                                                      \
         const bool sy = !(mx < my) && my < mz;                                                   \
         const bool sz = !sx && !sy;                                                              \
-        const float mc = sx ? mx : (sy ? my : mz);                                               \
-        const float m2 = sx ? fminf(my, mz) : (sy ? fminf(mx, mz) : fminf(mx, my));              \
+        /* the chosen axis holds the smallest value (ties too), m2 the second smallest */       \
+        const float mc = fminf(fminf(mx, my), mz);                                               \
+        const float m2 = __builtin_amdgcn_fmed3f(mx, my, mz);                                    \
         sure = m2 - mc > fma_(mc + m2, eb, ea);                                                  \
         eb += U24;                                                                               \
         nx -= sx ? 1 : 0;                                                                        \
