@@ -138,12 +138,12 @@ typedef struct {
                                   double); every FP32 decision (start cell, walk step,
                                   candidate test, pixel) is taken only outside its error
                                   bound, and the rays with a decision inside it are traced
-                                  again in FP64.  Shading,
-                                  blocking and pixel of every ray are the FP64 model's;
-                                  energies differ from it by the FP32 draws (about 1e-8
-                                  relative).  Rectangle and cylinder receivers; scenes with
-                                  the cone receiver or literal_self trace in FP64.  Same
-                                  entry points as OM_PREC_FP64.                         */
+                                  again in FP64.  Shading, blocking and pixel of every ray
+                                  are the FP64 model's; energies differ from it by the FP32
+                                  draws (about 1e-8 relative).  Rectangle and cylinder receivers with
+                                  OM_ACC_WINDOW; scenes with the cone receiver or
+                                  literal_self, and other accumulation modes, trace in
+                                  FP64.  Same entry points as OM_PREC_FP64.              */
     int   ray_compact;         /* OM_ACC_WINDOW, FP32 geometry: 1 packs the rays that pass
                                   the blocking test into full waves (ballot, mbcnt and one
                                   ds_permute per ray value) before the receiver stage;
