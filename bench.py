@@ -104,6 +104,46 @@ def valu_probe(mi: int, reps: int = 5) -> dict:
     return out
 
 
+ATOMIC_PROBES = {17: "global_u64_no_return", 18: "global_u64_returning", 19: "lds_u64_returning"}
+ATOM_ITERS = 64
+
+
+def atomic_probe(mi: int, reps: int = 5) -> dict:
+    """Runs each atomic throughput probe (om_math_eval 17..19: 64 atomic adds of 64 bits per
+    thread at hashed addresses, in device memory over mi / 2 Mi words or in a window-sized LDS
+    array per workgroup) over mi Mi threads, and reports its time (HIP events on the launch
+    stream) and the atomic operations per second it sustains."""
+    import torch
+
+    import onemirror_amd as om
+
+    n = mi << 20
+    x = torch.rand(n, device="cuda")
+    y = torch.zeros_like(x)
+    ctx = om.default_context()
+    stream = torch.cuda.current_stream()
+    out = {}
+    for fn, name in ATOMIC_PROBES.items():
+        def run():
+            om._check(om.lib().om_math_eval(ctx._h, fn, om._vp(x.data_ptr()), om._vp(y.data_ptr()), n,
+                                            om._stream_ptr()), "om_math_eval")
+        run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        ops = n * ATOM_ITERS
+        out[name] = {"ms": ms, "atomics": ops, "atomics_per_s": ops / (ms * 1e-3),
+                     "footprint": "LDS, 18,400 words per workgroup" if fn == 19 else f"{n // 2} words of device memory"}
+        print(f"# atomic probe {name}: {ms:.3f} ms, {ops:.4g} atomics, {ops / (ms * 1e-3):.4g} per s",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def host_cores() -> dict:
     """Host cores this process may run on, the machine's count, the CPU model, and the
     threads the CPU baseline uses: every core it may run on, or fewer where OMP_NUM_THREADS
@@ -263,6 +303,23 @@ def roofline(run: dict, workload: str, R: int, ops_ray: float, S: float, T: floa
             pj = json.load(f)
         if pj.get("rays") == R:
             traffic = pj["hbm_bytes_per_launch"]
+            ap = os.path.join(ROOT, "profiles", "atomic_probe.json")
+            if counters is not None and os.path.exists(ap) and pj.get("tcc_atomic"):
+                # the kernel's atomics per second against the probes': memory-side atomics
+                # (the hits outside the LDS window) and LDS atomics (the hits inside it)
+                with open(ap) as f:
+                    aj = json.load(f)["probes"]
+                mem = pj["tcc_atomic"] / (pj["avg_ns"] * 1e-9)
+                lds = max(run["hits"] - pj["tcc_atomic"], 0.0) / (pj["avg_ns"] * 1e-9) \
+                    if run["kernel"].startswith("trace_window") else None
+                counters["atomics"] = {
+                    "memory_side_per_s": mem,
+                    "probe_memory_side_per_s": aj["global_u64_returning"]["atomics_per_s"],
+                    "memory_side_of_probe": mem / aj["global_u64_returning"]["atomics_per_s"],
+                    "lds_per_s": lds,
+                    "probe_lds_per_s": aj["lds_u64_returning"]["atomics_per_s"],
+                    "lds_of_probe": None if lds is None else lds / aj["lds_u64_returning"]["atomics_per_s"],
+                    "source": "profiles/atomic_probe.json, TCC_ATOMIC_sum of the kernel's profile"}
     return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
             "frac": achieved / peak, "traffic": traffic, "ops_per_ray": ops_ray, "dda_steps_per_ray": S,
             "tests_per_ray": T, "arithmetic": "fp64" if fp64 else ("fp32+fp64" if "mixed" in run["kernel"] else "fp32"),
@@ -302,6 +359,11 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run an elementwise copy of MIB MiB (reads and writes MIB MiB "
                          "each): the reference kernel that calibrates HBM byte counters")
+    ap.add_argument("--atomic-probe", type=int, default=0, metavar="MI",
+                    help="first run the atomic throughput probes (om_math_eval 17..19) over MI Mi "
+                         "threads: 64-bit atomic adds in device memory (without and with the old "
+                         "value returned) and in a window-sized LDS array, the ceilings the trace "
+                         "kernels' atomic rates are set against")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -342,7 +404,7 @@ def main() -> int:
     precisions = {"all": ["fp32", "fp64", "mixed"], "both": ["fp32", "fp64"]}.get(args.precision, [args.precision])
     head = "mixed" if "mixed" in precisions else ("fp64" if "fp64" in precisions else "fp32")
     runs = {}
-    probe = None
+    probe = atomics = None
     for pr in precisions:
         p = {"fp64": dict(prm, precision=om.PREC_FP64), "mixed": dict(prm, precision=om.PREC_MIXED)}.get(
             pr, dict(prm, ray_compact=args.ray_compact))
@@ -363,6 +425,8 @@ def main() -> int:
             probe = valu_probe(args.valu_probe)
         if pr == precisions[0] and args.copy_probe > 0:
             copy_probe(args.copy_probe)
+        if pr == precisions[0] and args.atomic_probe > 0:
+            atomics = atomic_probe(args.atomic_probe)
         t = time_trace(scene, maps, args.steps, args.warmup)
         if pr == "mixed":
             t["redo_rays"] = scene.redo_count()
@@ -436,6 +500,7 @@ def main() -> int:
         "roofline": rooflines[head],
         "flux_err_vs_fp64": errs.get(head),
         "valu_probe": probe,
+        "atomic_probe": atomics,
         "cpu_baseline": None if args.no_cpu else {
             "value": cpu_rays / cpu_t, "unit": "rays/s", "cores": nthreads, "kind": "port",
             "host_cores": cores,

@@ -480,7 +480,11 @@ om_status om_diff_visibility(om_scene *scene, const uint64_t *d_mask, int h0, in
  * v_fma_f32 + 8 v_add_u32 (13), 8 v_fma_f32 + 2 v_exp_f32 (14) or 8 v_fma_f64 (15) per
  * lane, for measuring the VALU issue rate.  in/out device arrays of n floats.  16: the
  * FP64 geometry's slope-error erfinv(2u - 1) of the draw words in[i] (bits as uint32, u as
- * in 0); out holds n doubles (2n floats). */
+ * in 0); out holds n doubles (2n floats).  17..19 are atomic throughput probes: n threads
+ * each add 64 times with 64-bit atomics at hashed addresses, into out as n / 2 words of
+ * device memory without (17) and with (18) the old value returned, or into an LDS array of
+ * the window kernels' size in each 1024-thread workgroup (19, old value returned; out[i]
+ * gets a digest).  out is overwritten. */
 om_status om_math_eval(om_ctx *ctx, int fn, const float *d_in, float *d_out, int n, void *stream);
 
 #ifdef __cplusplus

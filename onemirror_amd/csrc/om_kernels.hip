@@ -1079,6 +1079,53 @@ __global__ __launch_bounds__(WG) void valu_probe_kernel(const float *in, float *
     out[i] = s + (MIX == 1 ? (float)(t & 1u) : 0.0f);
 }
 
+// Atomic throughput probes (om_math_eval fn 17..19): ATOM_ITERS 64-bit atomic adds per
+// thread at addresses spread by a multiplicative hash, so that the trace kernels' atomic
+// rates can be set against what this GPU sustains in the same forms:
+//   17: device memory (out, as n / 2 words), not returning;
+//   18: device memory, returning (the form of the window kernels' flushes, whose carry check
+//       needs the old value);
+//   19: LDS, returning, over a WIN_PX_MAX-word array in one 1024-thread workgroup per CU
+//       (the window kernels' hits); out[i] gets the thread's sum of the old values.
+// 17 and 18 leave their sums in out's words; 18 keeps the old values alive in registers.
+constexpr int ATOM_ITERS = 64;
+
+__device__ __forceinline__ uint32_t probe_slot(uint32_t i, uint32_t j, uint32_t n)
+{
+    const uint32_t h = (i * (uint32_t)ATOM_ITERS + j) * 0x9E3779B1u;
+    return (uint32_t)(((unsigned long long)h * n) >> 32); // in [0, n)
+}
+
+template <int KIND>
+__global__ __launch_bounds__(WIN_WGS) void atomic_probe_kernel(const float *in, float *out, int n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long v = 1ull + (i < (uint32_t)n ? (unsigned long long)(f2u(in[i]) & 0xffu) : 0ull);
+    if (KIND == 19) {
+        __shared__ unsigned long long s[WIN_PX_MAX];
+        for (int k = threadIdx.x; k < WIN_PX_MAX; k += WIN_WGS) s[k] = 0ull;
+        __syncthreads();
+        unsigned long long acc = 0ull;
+#pragma unroll 8
+        for (int j = 0; j < ATOM_ITERS; ++j)
+            acc += __hip_atomic_fetch_add(s + probe_slot(i, (uint32_t)j, (uint32_t)WIN_PX_MAX), v, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (i < (uint32_t)n) out[i] = (float)(acc & 0xffffffu);
+        return;
+    }
+    if (i >= (uint32_t)n) return;
+    unsigned long long *w = (unsigned long long *)out;
+    const uint32_t nw = (uint32_t)n / 2u;
+    unsigned long long acc = 0ull;
+#pragma unroll 8
+    for (int j = 0; j < ATOM_ITERS; ++j) {
+        unsigned long long *p = w + probe_slot(i, (uint32_t)j, nw);
+        if (KIND == 18) acc += __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (KIND == 18) asm volatile("" ::"v"(acc));
+}
+
 // ---------------------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------------------
@@ -1315,11 +1362,23 @@ om_status launch_fill_u32(uint32_t *p, long long n, hipStream_t st)
 
 om_status launch_math(int fn, const float *in, float *out, int n, hipStream_t st)
 {
-    if (fn < 0 || fn > 16) {
-        set_error("math function index must be in 0..16");
+    if (fn < 0 || fn > 19) {
+        set_error("math function index must be in 0..19");
         return OM_ERR_INVALID;
     }
     if (fn == 16) return launch_erfinv_ray((const uint32_t *)in, (double *)out, n, st);
+    if (fn >= 17) {
+        if (fn != 19 && n < 2) {
+            set_error("the device-memory atomic probes need n >= 2");
+            return OM_ERR_INVALID;
+        }
+        const dim3 ag((unsigned)((n + WIN_WGS - 1) / WIN_WGS)), ab(WIN_WGS);
+        if (fn == 17) hipLaunchKernelGGL(atomic_probe_kernel<17>, ag, ab, 0, st, in, out, n);
+        else if (fn == 18) hipLaunchKernelGGL(atomic_probe_kernel<18>, ag, ab, 0, st, in, out, n);
+        else hipLaunchKernelGGL(atomic_probe_kernel<19>, ag, ab, 0, st, in, out, n);
+        OM_HIP(hipGetLastError());
+        return OM_OK;
+    }
     const dim3 grid((n + WG - 1) / WG), block(WG);
     if (fn == 12) hipLaunchKernelGGL(valu_probe_kernel<0>, grid, block, 0, st, in, out, n);
     else if (fn == 13) hipLaunchKernelGGL(valu_probe_kernel<1>, grid, block, 0, st, in, out, n);

@@ -56,6 +56,32 @@ def test_device_math_bitexact(om, fn):
         f"{names[fn]}: {(got != want).sum()} mismatches"
 
 
+def test_atomic_probes_add_every_value(om):
+    """The atomic throughput probes (om_math_eval 17 and 18) add 64 values of 64 bits per
+    thread into device memory at hashed addresses: the words' total equals the sum of every
+    add, returned or not, so each probe does the n x 64 atomics its rate is counted on.  The
+    LDS probe (19) runs to completion and leaves each thread's digest (24 bits)."""
+    import ctypes
+    import torch
+    ctx = om.Context(0)
+    n = 1 << 16
+    x = torch.rand(n, device="cuda")
+    v = (x.cpu().numpy().view(np.uint32) & 0xff).astype(np.uint64) + 1
+    want = int(v.sum()) * 64
+    for fn in (17, 18):
+        y = torch.zeros(n, device="cuda")
+        assert om.lib().om_math_eval(ctx._h, fn, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                     n, None) == 0
+        torch.cuda.synchronize()
+        assert int(y.cpu().numpy().view(np.uint64).sum()) == want, fn
+    y = torch.full((n,), -1.0, device="cuda")
+    assert om.lib().om_math_eval(ctx._h, 19, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                 n, None) == 0
+    torch.cuda.synchronize()
+    d = y.cpu().numpy()
+    assert np.all((d >= 0) & (d < 2 ** 24))
+
+
 def test_device_sqrt_exhaustive(om):
     """The kernel's short square root equals IEEE sqrt on every float in [1, 4); an input
     scaled by 4^k (no denormal or overflow on either side) scales its root by 2^k exactly,
