@@ -98,6 +98,14 @@ struct TraceArgs {
     uint32_t *win_queue;      // queue block (QUEUE_WORDS): [8] chunks taken from each queue
     const uint32_t *abort_flag; // nonzero stops every persistent launch (om_ctx_abort)
     int win_x0, win_y0, win_w, win_h;
+    // Window of each queue's workgroups (azimuthal maps, whole-grid traces): the queue of XCD
+    // x takes the tiles win_order[first..] (a permutation of the launch's tiles, null: their
+    // order) and its workgroups hold rows [win_qy0[x], + win_h) and columns [win_qx0[x],
+    // + win_w) modulo res_x; each workgroup records its queue in win_slice_q[workgroup] for
+    // the decode (null: every workgroup holds win_x0, win_y0)
+    const int *win_order;
+    int *win_slice_q;
+    int win_qx0[8], win_qy0[8];
     int literal_self;         // walks test the ray's own heliostat (om_params.literal_self)
 };
 
@@ -249,6 +257,12 @@ struct GridState {
     std::vector<int> cell_start, cell_items; // host CSR (root scenes only)
     int *d_cell_start = nullptr, *d_cell_items = nullptr;
     int t0 = 0, t1 = 0;
+    // azimuthal maps (cylinder, cone): the grid's tiles in the order of 8 azimuth sectors of
+    // its heliostats about the receiver axis (d_win_order, relative to t0), and each window
+    // queue's window origin (win_sectors false: the centred window of every queue)
+    bool win_sectors = false;
+    int *d_win_order = nullptr;
+    int win_qx0[8] = {0, 0, 0, 0, 0, 0, 0, 0}, win_qy0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 } // namespace om

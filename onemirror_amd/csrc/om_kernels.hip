@@ -399,9 +399,9 @@ __device__ __forceinline__ void trace_chunk_window(const TraceArgs &a, int2 tl, 
             if (o.rc.x >= 0) {
                 px = o.rc.x * a.res_x + o.rc.y;
                 q = o.q;
-                const uint32_t wr = (uint32_t)(o.rc.x - a.win_y0), wc = (uint32_t)(o.rc.y - a.win_x0);
-                if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w)
-                    packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+                const int wi = wo.window_word(a, o.rc.x, o.rc.y);
+                if (wi >= 0)
+                    packed_add_lds(a, s_win + wi, px, q);
                 else
                     out = true;
             }
@@ -442,9 +442,9 @@ __device__ __forceinline__ void receive(const TraceArgs &a, bool live, const v3 
         const int2 rc = receiver_rc(a, orig, refl);
         if (rc.x >= 0) {
             px = rc.x * a.res_x + rc.y;
-            const uint32_t wr = (uint32_t)(rc.x - a.win_y0), wc = (uint32_t)(rc.y - a.win_x0);
-            if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w)
-                packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+            const int wi = wo.window_word(a, rc.x, rc.y);
+            if (wi >= 0)
+                packed_add_lds(a, s_win + wi, px, q);
             else
                 out = true;
         }
@@ -759,6 +759,10 @@ __global__ void reduce_batch_kernel(const ReduceSlot *__restrict__ slots)
 struct WindowRef {
     unsigned long long *part; // [nwin][w * h] window words of each workgroup
     int nwin, x0, y0, w, h, res_x;
+    // the queue of each workgroup and the window origin of each queue (columns modulo res_x);
+    // null: every workgroup's window is at (x0, y0)
+    const int *slice_q;
+    int qx0[8], qy0[8];
 };
 
 __global__ void reduce_packed_kernel(unsigned long long *part, uint32_t *carry, uint32_t *wrap, int E,
@@ -777,7 +781,31 @@ __global__ void reduce_packed_kernel(unsigned long long *part, uint32_t *carry, 
         e += wv & lowmask;
         w = 0ull;
     }
-    if (win.nwin > 0) {
+    if (win.nwin > 0 && win.slice_q) {
+        // the slices of the queues whose window holds the pixel
+        const int row = i / win.res_x, col = i - row * win.res_x;
+        uint32_t held = 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int c = col - win.qx0[q];
+            if (c < 0) c += win.res_x;
+            if ((uint32_t)(row - win.qy0[q]) < (uint32_t)win.h && (uint32_t)c < (uint32_t)win.w) held |= 1u << q;
+        }
+        if (held) {
+            const size_t wpx = (size_t)win.w * win.h;
+            for (int g = 0; g < win.nwin; ++g) {
+                const int q = win.slice_q[g];
+                if (!((held >> q) & 1u)) continue;
+                int c = col - win.qx0[q];
+                if (c < 0) c += win.res_x;
+                unsigned long long *p = win.part + (size_t)g * wpx + (size_t)(row - win.qy0[q]) * win.w + c;
+                const unsigned long long wv = *p;
+                n += wv >> E;
+                e += wv & lowmask;
+                *p = 0ull;
+            }
+        }
+    } else if (win.nwin > 0) {
         const int row = i / win.res_x, col = i - row * win.res_x;
         const uint32_t wr = (uint32_t)(row - win.y0), wc = (uint32_t)(col - win.x0);
         if (wr < (uint32_t)win.h && wc < (uint32_t)win.w) {
@@ -1502,7 +1530,8 @@ namespace om {
 struct WsPlan {
     int shift = 0;
     size_t n16 = 0, bytes = 0, off_win = 0;
-    WindowRef win{nullptr, 0, 0, 0, 0, 0, 0};
+    size_t off_slice_q = 0; // window mode: the queue of each workgroup (nwg ints)
+    WindowRef win{nullptr, 0, 0, 0, 0, 0, 0, nullptr, {}, {}};
     int nwg = 0;
     size_t off_redo = 0; // OM_PREC_MIXED window traces: the redo list of redo_cap entries
     int redo_cap = 0;
@@ -1537,6 +1566,8 @@ static om_status ws_plan(om_scene *s, size_t copy_budget, size_t cap, WsPlan &p)
             p.n16 = ncopy * npx;
             p.off_win = p.n16 * sizeof(unsigned long long) + 2 * npx * sizeof(uint32_t);
             p.bytes = p.off_win + (p.nwg > 0 ? (size_t)p.nwg * p.win.w * p.win.h * sizeof(unsigned long long) : 0);
+            p.off_slice_q = p.bytes;
+            if (p.nwg > 0) p.bytes += (size_t)p.nwg * sizeof(int);
             p.off_redo = (p.bytes + 255) & ~(size_t)255;
             p.redo_cap = (int)want;
             if (want > 0) p.bytes = p.off_redo + (size_t)want * sizeof(uint2);
@@ -1603,6 +1634,18 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
             a.win_h = win.h;
             win.part = a.win_part;
             win.nwin = p.nwg;
+            // a whole-grid trace of an azimuthal map: the queues take the grid's azimuth
+            // sectors, each with its own window
+            const GridState &G = s->grid_src ? s->grid_src->grids[(size_t)g] : s->grids[(size_t)g];
+            if (G.win_sectors && G.d_win_order && t0 == G.t0 && t1 == G.t1) {
+                a.win_order = G.d_win_order;
+                a.win_slice_q = (int *)(base + p.off_slice_q);
+                win.slice_q = a.win_slice_q;
+                for (int q = 0; q < 8; ++q) {
+                    a.win_qx0[q] = win.qx0[q] = G.win_qx0[q];
+                    a.win_qy0[q] = win.qy0[q] = G.win_qy0[q];
+                }
+            }
             const int chunk = std::min(a.tile_rays, WIN_CHUNK);
             if ((rc = launch_queue_reset(a.win_queue, (long long)a.ntiles * (a.tile_rays / chunk), QUEUE_EIGHTHS,
                                          st)) != OM_OK)

@@ -1565,6 +1565,83 @@ om_status upload_geometry(om_scene *s)
     return OM_OK;
 }
 
+// Azimuthal maps (cylinder, cone): a heliostat's rays land on the side of the receiver that
+// faces it, so the LDS-window traces give each of their 8 queues one azimuth sector of the
+// grid's heliostats about the receiver axis (sectors of equal tile counts; heliostat order
+// within a sector keeps neighbouring heliostats together), and the workgroups of a queue
+// hold the window centred on its sector's mean azimuth.  On the 25,000-heliostat cylinder
+// field the windows then take 68 % of the hits instead of the centred window's 22 %.  Maps
+// that fit one window whole keep it.  The window kernels split a grid's n = tiles x parts
+// chunks into queues x of q = n / 8 (+1 for x < n % 8), as here.
+om_status build_win_sectors(om_scene *s)
+{
+    for (om::GridState &G : s->grids) {
+        dfree(G.d_win_order);
+        G.win_sectors = false;
+    }
+    if (s->recv_type != 1 && s->recv_type != 2) return OM_OK;
+    int x0, y0, w, h;
+    om::window_rect(s->res_x, s->res_y, x0, y0, w, h);
+    if (w >= s->res_x && h >= s->res_y) return OM_OK;
+    const int tile_rays = 256 * s->rays_per_thread;
+    const int parts = tile_rays / std::min(tile_rays, om::window_chunk_rays());
+    const double two_pi = 6.283185307179586;
+    for (om::GridState &G : s->grids) {
+        const int nt = G.t1 - G.t0;
+        if (nt < 8 || G.h1 <= G.h0) continue;
+        // each heliostat's first tile and tile count, and its azimuth about the axis
+        const int nh = G.h1 - G.h0;
+        std::vector<int> first(nh, 0), count(nh, 0);
+        for (int t = G.t0; t < G.t1; ++t) {
+            const int k = s->tiles[(size_t)t].x - G.h0;
+            if (count[k]++ == 0) first[k] = t - G.t0;
+        }
+        std::vector<double> th(nh);
+        for (int k = 0; k < nh; ++k) {
+            const int hh = G.h0 + k;
+            double a = std::atan2((double)s->pos[3 * (size_t)hh + 2] - s->cyl_base.z,
+                                  (double)s->pos[3 * (size_t)hh] - s->cyl_base.x);
+            th[k] = a < 0.0 ? a + two_pi : a;
+        }
+        std::vector<int> by_az(nh);
+        for (int k = 0; k < nh; ++k) by_az[k] = k;
+        std::stable_sort(by_az.begin(), by_az.end(), [&](int u, int v) { return th[u] < th[v]; });
+        std::vector<int> sector(nh, 0);
+        long long cum = 0;
+        for (int k : by_az) {
+            sector[k] = (int)std::min<long long>(7, (8 * (2 * cum + count[k])) / (2LL * nt));
+            cum += count[k];
+        }
+        std::vector<int> order_h(nh);
+        for (int k = 0; k < nh; ++k) order_h[k] = k;
+        std::stable_sort(order_h.begin(), order_h.end(), [&](int u, int v) { return sector[u] < sector[v]; });
+        std::vector<int> order;
+        order.reserve((size_t)nt);
+        for (int k : order_h)
+            for (int j = 0; j < count[k]; ++j) order.push_back(first[k] + j);
+        // each queue's window: centred on the circular mean azimuth of its tiles
+        const long long n = (long long)nt * parts, q = n / 8, rmd = n % 8;
+        for (int x = 0; x < 8; ++x) {
+            const long long c0 = x * q + std::min<long long>(x, rmd), cnt = q + (x < rmd ? 1 : 0);
+            double cs = 0.0, sn = 0.0;
+            for (long long c = c0; c < c0 + cnt; c += parts) {
+                const int k = s->tiles[(size_t)(G.t0 + order[(size_t)(c / parts)])].x - G.h0;
+                cs += std::cos(th[k]);
+                sn += std::sin(th[k]);
+            }
+            double a = std::atan2(sn, cs);
+            if (a < 0.0) a += two_pi;
+            const int col = (int)std::floor(a / two_pi * s->res_x - 0.5 * w);
+            G.win_qx0[x] = ((col % s->res_x) + s->res_x) % s->res_x;
+            G.win_qy0[x] = y0;
+        }
+        om_status st = upload(G.d_win_order, order.data(), order.size());
+        if (st != OM_OK) return st;
+        G.win_sectors = true;
+    }
+    return OM_OK;
+}
+
 om_status choose_tiles(om_scene *s)
 {
     // enough workgroups to fill 256 CUs several times over, tiles of at least 256 rays
@@ -1573,7 +1650,9 @@ om_status choose_tiles(om_scene *s)
     while (rpt > 1 && total / (256LL * rpt) < 8192) rpt /= 2;
     s->rays_per_thread = rpt;
     build_tiles(s);
-    return upload(s->d_tiles, s->tiles.data(), s->tiles.size());
+    om_status st = upload(s->d_tiles, s->tiles.data(), s->tiles.size());
+    if (st != OM_OK) return st;
+    return build_win_sectors(s);
 }
 
 void free_scene_dev(om_scene *s)
@@ -1596,6 +1675,7 @@ void free_scene_dev(om_scene *s)
     for (om::GridState &G : s->grids) {
         dfree(G.d_cell_start);
         dfree(G.d_cell_items);
+        dfree(G.d_win_order);
     }
     dfree(s->d_buie_bins);
     dfree(s->d_buie_guide);

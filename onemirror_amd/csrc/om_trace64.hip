@@ -541,9 +541,9 @@ __device__ __forceinline__ void trace_chunk_window64(const TraceArgs &a, const T
             if (o.px >= 0) {
                 px = o.px;
                 q = o.q;
-                const uint32_t wr = (uint32_t)(o.rc.x - a.win_y0), wc = (uint32_t)(o.rc.y - a.win_x0);
-                if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w)
-                    packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+                const int wi = wo.window_word(a, o.rc.x, o.rc.y);
+                if (wi >= 0)
+                    packed_add_lds(a, s_win + wi, px, q);
                 else
                     out = true;
             }
@@ -997,12 +997,13 @@ __device__ __forceinline__ Ray64 ray64_at(const TraceArgs &a, const Trace64Args 
     return trace_ray64<SUN>(a, b, hf, sf, orig, cell64(b, orig), h, m, l, h, hr->clear != 0);
 }
 
-// A hit of pixel (row, col) into the LDS window, or marked for the wave's buffer (out)
-__device__ __forceinline__ void window_hit(const TraceArgs &a, unsigned long long *s_win, int2 rc, int px, uint32_t q,
-                                           bool &out)
+// A hit of pixel (row, col) into the workgroup's LDS window, or marked for the wave's buffer
+// (out)
+__device__ __forceinline__ void window_hit(const TraceArgs &a, const WaveOut &wo, unsigned long long *s_win, int2 rc,
+                                           int px, uint32_t q, bool &out)
 {
-    const uint32_t wr = (uint32_t)(rc.x - a.win_y0), wc = (uint32_t)(rc.y - a.win_x0);
-    if (wr < (uint32_t)a.win_h && wc < (uint32_t)a.win_w) packed_add_lds(a, s_win + wr * (uint32_t)a.win_w + wc, px, q);
+    const int wi = wo.window_word(a, rc.x, rc.y);
+    if (wi >= 0) packed_add_lds(a, s_win + wi, px, q);
     else out = true;
 }
 
@@ -1085,7 +1086,7 @@ __device__ __forceinline__ void trace_chunk_mixed(const MixLaunch &L, int2 tl, c
             } else if (o.rc.x >= 0) {
                 px = o.rc.x * a.res_x + o.rc.y;
                 q = o.q;
-                window_hit(a, s_win, o.rc, px, q, out);
+                window_hit(a, wo, s_win, o.rc, px, q, out);
             }
         }
         // the whole wave takes part: no lane has left the loop.  A ray that finds the redo

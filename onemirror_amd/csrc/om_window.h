@@ -70,6 +70,17 @@ struct WaveOut {
     unsigned long long *buf; // [WIN_OUT] in LDS
     int n;
     const MixArgs *mx = nullptr; // the mixed trace's redo list (null: no REDO_TAG entries)
+    int wx0 = 0, wy0 = 0;        // the origin of this workgroup's window (window_body)
+
+    // The word of pixel (row, col) in this workgroup's window (columns modulo res_x), or -1
+    // when the pixel lies outside it.
+    __device__ __forceinline__ int window_word(const TraceArgs &a, int row, int col) const
+    {
+        const uint32_t wr = (uint32_t)(row - wy0);
+        int c = col - wx0;
+        if (c < 0) c += a.res_x;
+        return wr < (uint32_t)a.win_h && (uint32_t)c < (uint32_t)a.win_w ? (int)wr * a.win_w + c : -1;
+    }
 
     // Appends the REDO_TAG entries of e[] to the redo list and empties their slots; an entry
     // past the list's capacity is dropped (the launch is then traced again in FP64).
@@ -189,12 +200,15 @@ __device__ __forceinline__ void window_body(const TraceArgs &a, unsigned long lo
     for (int i = threadIdx.x; i < wpx; i += WIN_WGS) s_win[i] = 0ull;
     __syncthreads();
     const int lane = (int)threadIdx.x & 63;
+    const uint32_t xcd = xcc_id() & 7u;
     WaveOut out{s_out[threadIdx.x >> 6], 0};
+    // the window of this XCD's queue (the same for every queue unless win_slice_q is set)
+    out.wx0 = a.win_slice_q ? a.win_qx0[xcd] : a.win_x0;
+    out.wy0 = a.win_slice_q ? a.win_qy0[xcd] : a.win_y0;
     // tiles hold 64..2048 rays (a power of two): chunks of min(tile, WIN_CHUNK) rays
     const int chunk = min(a.tile_rays, WIN_CHUNK);
     const int parts = a.tile_rays / chunk;
     const int n = a.ntiles * parts, q = n >> 3, rmd = n & 7;
-    const uint32_t xcd = xcc_id() & 7u;
     for (uint32_t j = 0; j < 8u; ++j) {
         const int x = (int)((xcd + j) & 7u);
         const int first = x * q + min(x, rmd), cnt = q + (x < rmd ? 1 : 0);
@@ -204,7 +218,7 @@ __device__ __forceinline__ void window_body(const TraceArgs &a, unsigned long lo
             if (t >= (uint32_t)cnt) break;
             const int c = first + (int)t;
             const int tile = c / parts, part = c - tile * parts;
-            const int2 tl = a.tiles[a.tile_base + tile];
+            const int2 tl = a.tiles[a.tile_base + (a.win_order ? a.win_order[tile] : tile)];
             chunk_fn(make_int2(tl.x, tl.y + part * chunk), lane, chunk, out, s_win);
         }
     }
@@ -213,6 +227,7 @@ __device__ __forceinline__ void window_body(const TraceArgs &a, unsigned long lo
     __syncthreads();
     unsigned long long *dst = a.win_part + (size_t)blockIdx.x * wpx;
     for (int i = threadIdx.x; i < wpx; i += WIN_WGS) dst[i] = s_win[i];
+    if (a.win_slice_q && threadIdx.x == 0) a.win_slice_q[blockIdx.x] = (int)xcd;
 }
 
 template <class Chunk>

@@ -135,7 +135,8 @@ def test_large_whole_field_on_a_1pct_pixel_subset(om):
         accumulation): counts equal the model's on every pixel, flux within 1e-5 relative on
         the subset and on every lit pixel;
       * the FP32-geometry trace (reference semantics): its error on the subset is printed
-        and bounded by the figures stated for it, with a 10 % margin."""
+        and bounded by the figures stated for it, with a 10 % margin; its window trace's maps
+        equal the oracle's FP32 restatement of the whole field bit for bit."""
     import time
     import torch
     d, p = F.config_large()
@@ -168,6 +169,16 @@ def test_large_whole_field_on_a_1pct_pixel_subset(om):
     for name in ("fp64", "mixed"):
         assert out[name]["count_px"] == 0, name
         assert out[name]["max_sub"] <= 1e-5 and out[name]["max_all"] <= 1e-5, name
+    # the FP32 window trace of the whole field (its queues take azimuth sectors, each with
+    # its own window) equals the oracle's FP32 restatement bit for bit
+    t0 = time.time()
+    want32 = o.trace_f32()
+    print(f"FP32 restatement of the large field: {time.time() - t0:.1f} s", flush=True)
+    g = om.SolarScene(d, dict(p, accumulate=om.ACC_WINDOW))
+    m = g.trace_plain()
+    torch.cuda.synchronize()
+    assert np.array_equal(m.counts.cpu().numpy().view(np.uint32).ravel(), want32["counts"].ravel())
+    assert np.array_equal(m.energy.cpu().numpy().view(np.uint64).ravel(), want32["energy"].ravel())
     # FP32 geometry on the whole large field, as stated (subset median 8.09e-4, p99 5.48e-3,
     # max 9.52e-3; 98.9 % of the subset above 1e-5)
     for k, v in (("median", 8.09e-4), ("p99", 5.48e-3), ("max_sub", 9.52e-3)):

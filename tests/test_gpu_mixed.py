@@ -18,7 +18,8 @@ the pins are those of the FP64 trace:
   * a redo list that fills up sends the whole launch to the FP64 trace: the maps are then
     the FP64 trace's, bit for bit, and om_ctx_progress reports that trace's chunks, all
     taken; a list that fits leaves the mixed launch's own chunk counts to om_ctx_progress;
-  * a per-scene batch of mixed scenes equals their plain traces; the persistent batch launch
+  * a per-scene batch of mixed scenes equals their plain traces on three replays of its graph
+    with eager traces between them; the persistent batch launch
     and the differentiable mode refuse mixed scenes.
 """
 import numpy as np
@@ -70,13 +71,15 @@ def _check_against_model(g, want, label):
     return redo
 
 
-@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder", "small_L5", "small_L1"])
+@pytest.mark.parametrize("cfg", ["small", "small_lattice", "cylinder", "small_L5", "small_L1", "cylinder_sectors"])
 def test_small_fields_records_and_maps(om, cfg):
     """small_L5, small_L1: 5 and 1 rays per microsample, so that a wave's chunk of rays spans
     more than the 64 microsample starts a wave holds at a time (its table is refilled every
     5 passes, and every pass)."""
-    if cfg == "cylinder":
+    if cfg.startswith("cylinder"):
         d, p = F.config_small_cylinder()
+        if cfg == "cylinder_sectors":  # a map larger than one window: per-sector windows
+            d = dict(d, recv_res_x=320, recv_res_y=160)
     else:
         d, p = F.config_small()
         if cfg == "small_lattice":
@@ -146,11 +149,16 @@ def test_mixed_batch_forms(om):
     scenes = [om.SolarScene(dict(d, sun_dir=F.sun_direction(el, 180.0)), _mixed(dict(p, scene_index=i), om))
               for i, el in enumerate((45.0, 30.0))]
     want = [_maps(s)[:2] for s in scenes]
-    maps = om.Batch(scenes, workgroup=om.BATCH_PER_SCENE).run()
-    torch.cuda.synchronize()
-    for i, m in enumerate(maps):
-        assert np.array_equal(m.counts.cpu().numpy().view(np.uint32).ravel(), want[i][0]), i
-        assert np.array_equal(m.energy.cpu().numpy().view(np.uint64).ravel(), want[i][1]), i
+    batch = om.Batch(scenes, workgroup=om.BATCH_PER_SCENE)
+    # three replays of the batch's graph, with eager traces of the scenes between them (the
+    # redo counters and lists are cleared by fill kernels inside the graph)
+    for rep in range(3):
+        maps = batch.run()
+        torch.cuda.synchronize()
+        for i, m in enumerate(maps):
+            assert np.array_equal(m.counts.cpu().numpy().view(np.uint32).ravel(), want[i][0]), (rep, i)
+            assert np.array_equal(m.energy.cpu().numpy().view(np.uint64).ravel(), want[i][1]), (rep, i)
+        assert np.array_equal(_maps(scenes[rep % 2])[0], want[rep % 2][0])
     with pytest.raises(om.OMError, match="mixed"):
         om.Batch(scenes, workgroup=256)
 

@@ -393,6 +393,35 @@ def test_cylinder_small_records_and_maps_bitexact(om, lattice):
     assert np.array_equal(maps.flux.cpu().numpy().view(np.uint32).ravel(), flux.view(np.uint32).ravel())
 
 
+@pytest.mark.parametrize("compact", [0, 1])
+def test_cylinder_sector_windows_bitexact(om, compact):
+    """A cylinder map larger than one LDS window (320 x 160 pixels): the window trace gives
+    each of its 8 queues one azimuth sector of the heliostats and the window centred on it
+    (columns wrapping around the cylinder), and the decode adds each workgroup's window where
+    its queue's lies.  The maps equal the oracle's bit for bit, with and without ray
+    compaction, on two traces in a row; a trace of a heliostat range (the centred window for
+    every queue) in between gives its own maps."""
+    import torch
+    d, p = F.config_small_cylinder()
+    d = dict(d, recv_res_x=320, recv_res_y=160)
+    p = dict(p, accumulate=om.ACC_WINDOW, ray_compact=compact)
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    assert g.info["res_x"] * g.info["res_y"] == 320 * 160
+    orc = o.trace_f32()
+    part = o.trace_f32(20, 60)
+    for rep in range(2):
+        maps = g.trace_plain()
+        torch.cuda.synchronize()
+        assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32).ravel(), orc["counts"].ravel()), rep
+        assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64).ravel(), orc["energy"].ravel()), rep
+        m = g.new_maps(smooth=False)
+        g.trace_heliostats(20, 60, m)
+        torch.cuda.synchronize()
+        assert np.array_equal(m.counts.cpu().numpy().view(np.uint32).ravel(), part["counts"].ravel()), rep
+        assert np.array_equal(m.energy.cpu().numpy().view(np.uint64).ravel(), part["energy"].ravel()), rep
+    assert int(orc["counts"].sum()) > 100_000
+
+
 @pytest.fixture(scope="module")
 def large(om):
     d, p = F.config_large()
