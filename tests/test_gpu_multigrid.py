@@ -142,6 +142,26 @@ def test_fork_of_a_multigrid_scene_equals_fresh(om, two):
     _same_maps(_maps(f), _maps(fresh), "fork")
 
 
+def test_two_grids_with_sector_windows(om, two):
+    """Both grids around a cylinder whose map is larger than one LDS window (320 x 160): each
+    grid's window trace gives its queues that grid's azimuth sectors.  The maps equal the
+    oracle's bit for bit, for the root scene and for a fork at another sun (a fork uses its
+    root's sector order)."""
+    d, p = two
+    d = dict(d, recv_type=1, recv_pos=np.array([0.0, 30.0, 0.0], np.float32),
+             recv_size=np.array([2.0, 4.0, 0.0], np.float32), recv_res_x=320, recv_res_y=160)
+    p = dict(p, grid_select=om.GRIDS_ALL, accumulate=om.ACC_WINDOW)
+    g, o = om.SolarScene(d, p), oracle.OracleScene(d, p)
+    assert g.info["n_grids"] == 2
+    orc = o.trace_f32()
+    _oracle_equal(_maps(g), orc, "two grids, sector windows")
+    assert int(orc["counts"].sum()) > 100_000
+    sun = F.sun_direction(35.0, 205.0)
+    f = g.fork(sun, 800.0, scene_index=0)
+    d2 = dict(d, sun_dir=sun, dni=np.float32(800.0))
+    _oracle_equal(_maps(f), oracle.OracleScene(d2, p).trace_f32(), "fork, sector windows")
+
+
 def test_diff_forward_and_gradients_over_two_grids(om, two):
     """The differentiable mode walks each heliostat's grid for its visibility ballots.  At
     zero parameters its rays are the plain trace's up to the rounding of one more
