@@ -534,9 +534,10 @@ def main() -> int:
         fa = bench_facet_annual(sub_a)
         out.setdefault("modes", {}).update({
             "batched": {"value": b["value"], "unit": "rays/s", "workload": b["config"]["workload"],
-                        **{k: b["config"][k] for k in ("scenes", "rays", "batch_ms", "plain_ms",
+                        **{k: b["config"][k] for k in ("precision", "scenes", "rays", "batch_ms", "plain_ms",
                                                        "plain_rays_per_s", "batched_vs_plain",
-                                                       "grid_cells", "workgroup", "rays_per_thread")}},
+                                                       "grid_cells", "workgroup", "rays_per_thread",
+                                                       "redo_rays", "full_lists", "fp32_reference")}},
             "annual": {"value": an["value"], "unit": "rays/s", "workload": an["config"]["workload"],
                        **{k: an["config"][k] for k in ("time_steps", "rays", "mode", "batch_ms", "plain_ms",
                                                         "time_steps_per_s", "annual_ratio")}},
@@ -701,7 +702,9 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
     256 cells per axis by timing the persistent kernel on a sample) in one om_batch replayed
     from a hipGraph, with the launch the autotuner picks: one persistent launch over every
     scene, or the window trace of each scene in turn; the plain path (one om_trace_plain per
-    scene, back to back on the stream) timed on the same scenes."""
+    scene, back to back on the stream) timed on the same scenes.  The value is the mixed
+    trace's (OM_PREC_MIXED: the FP64 model's outcome of every ray, the metric's accuracy);
+    the same batch in FP32 reference geometry is reported beside it."""
     import torch
 
     import onemirror_amd as om
@@ -724,45 +727,61 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
         d.update(F.grid_for(d["helio_pos"], d["helio_size"], float(d["grid_pos"][1] + d["grid_size"][1]),
                             grid_cells))
         configs.append((d, p))
-    scenes = [om.SolarScene(d, p) for d, p in configs]
-    torch.cuda.synchronize()
-    R = sum(s.info["total_rays"] for s in scenes)
-    Hs = [s.info["n_helio"] for s in scenes]
-    print(f"# {len(scenes)} scenes built in {time.time() - t1:.1f}s: {sum(Hs)} heliostats "
-          f"({min(Hs)}..{max(Hs)}), {R} rays, grids {grid_cells}^3", file=sys.stderr, flush=True)
-    maps = [s.new_maps(smooth=True) for s in scenes]
-    tl = om.autotune_launch(scenes, maps, reps=2)
-    print(f"# launch autotune: workgroup {tl['workgroup']} x {tl['rays_per_thread']} rays/thread",
-          file=sys.stderr, flush=True)
-    b = om.Batch(scenes, maps, workgroup=tl["workgroup"], rays_per_thread=tl["rays_per_thread"])
-    stream = torch.cuda.current_stream()
-    for _ in range(max(1, args.warmup)):
-        b.run()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        b.run()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t) / args.steps
-    batch_ms = e0.elapsed_time(e1) / args.steps
-    # the plain path on the same scenes: one trace + finalize + smoothing per scene, each scene
-    # with the accumulation its own autotuner picks (packed or LDS window)
-    for s, m in zip(scenes, maps):
-        s.autotune_accumulate(m, reps=2)
-        s.trace_plain(m)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
+    def run(precision):
+        """The batch of the scenes in this precision: autotuned launch, graph replays, and
+        the plain path (one accumulation-autotuned om_trace_plain per scene) on the same
+        scenes."""
+        pc = {"fp32": om.PREC_FP32, "mixed": om.PREC_MIXED}[precision]
+        t1 = time.time()
+        scenes = [om.SolarScene(d, dict(p, precision=pc)) for d, p in configs]
+        torch.cuda.synchronize()
+        R = sum(s.info["total_rays"] for s in scenes)
+        Hs = [s.info["n_helio"] for s in scenes]
+        print(f"# {precision}: {len(scenes)} scenes built in {time.time() - t1:.1f}s: {sum(Hs)} heliostats "
+              f"({min(Hs)}..{max(Hs)}), {R} rays, grids {grid_cells}^3", file=sys.stderr, flush=True)
+        maps = [s.new_maps(smooth=True) for s in scenes]
+        tl = om.autotune_launch(scenes, maps, reps=2)
+        print(f"# {precision} launch autotune: workgroup {tl['workgroup']} x {tl['rays_per_thread']} rays/thread "
+              f"({', '.join(f'{w}x{r} {ms:.1f} ms' for w, r, ms in tl['trials'])})", file=sys.stderr, flush=True)
+        b = om.Batch(scenes, maps, workgroup=tl["workgroup"], rays_per_thread=tl["rays_per_thread"])
+        stream = torch.cuda.current_stream()
+        for _ in range(max(1, args.warmup)):
+            b.run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            b.run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        batch_ms = e0.elapsed_time(e1) / args.steps
+        redo, full = b.redo_count()
         for s, m in zip(scenes, maps):
+            s.autotune_accumulate(m, reps=2)
             s.trace_plain(m)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    plain_ms = e0.elapsed_time(e1) / args.steps
-    print(f"# batched {batch_ms:.2f} ms ({R / (batch_ms * 1e-3):.4g} rays/s), plain per scene "
-          f"{plain_ms:.2f} ms ({R / (plain_ms * 1e-3):.4g} rays/s)", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            for s, m in zip(scenes, maps):
+                s.trace_plain(m)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms = e0.elapsed_time(e1) / args.steps
+        print(f"# {precision} batched {batch_ms:.2f} ms ({R / (batch_ms * 1e-3):.4g} rays/s), plain per scene "
+              f"{plain_ms:.2f} ms ({R / (plain_ms * 1e-3):.4g} rays/s)"
+              + (f", {redo} rays traced again in FP64, {full} full lists" if precision == "mixed" else ""),
+              file=sys.stderr, flush=True)
+        del b, maps, scenes
+        torch.cuda.synchronize()
+        return {"rays": R, "heliostats": sum(Hs), "dt": dt, "batch_ms": batch_ms, "plain_ms": plain_ms,
+                "tl": tl, "redo_rays": redo, "full_lists": full}
+
+    runs = {pr: run(pr) for pr in ("mixed", "fp32")}
+    head = runs["mixed"]
+    R, batch_ms, plain_ms, tl, dt = head["rays"], head["batch_ms"], head["plain_ms"], head["tl"], head["dt"]
+    Hs_total = head["heliostats"]
     sample_cfg = configs[:: max(1, len(configs) // 8)]
     if not with_cpu:
         sample_cfg = []
@@ -775,14 +794,23 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
         "metric": METRIC, "value": R / dt, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"batched: {len(scenes)} scenes, heliostats log-uniform 500-20000, "
+        "config": {"workload": f"batched: {len(configs)} scenes, heliostats log-uniform 500-20000, "
                                "sun elevation 10-80 deg azimuth 90-270 deg, rays per scene "
                                "proportional to heliostats in [1e7, 2e8], Buie CSR 0.1, 256x256 maps",
-                   "scenes": len(scenes), "heliostats": sum(Hs), "rays": R,
+                   "precision": "mixed", "geometry": GEOMETRY["mixed"],
+                   "scenes": len(configs), "heliostats": Hs_total, "rays": R,
                    "grid_cells": grid_cells, "workgroup": tl["workgroup"],
                    "rays_per_thread": tl["rays_per_thread"], "batch_ms": batch_ms,
                    "plain_ms": plain_ms, "plain_rays_per_s": R / (plain_ms * 1e-3),
                    "batched_vs_plain": plain_ms / batch_ms,
+                   "redo_rays": head["redo_rays"], "full_lists": head["full_lists"],
+                   "fp32_reference": {
+                       "value": runs["fp32"]["rays"] / runs["fp32"]["dt"], "unit": "rays/s",
+                       "batch_ms": runs["fp32"]["batch_ms"], "plain_ms": runs["fp32"]["plain_ms"],
+                       "batched_vs_plain": runs["fp32"]["plain_ms"] / runs["fp32"]["batch_ms"],
+                       "workgroup": runs["fp32"]["tl"]["workgroup"],
+                       "rays_per_thread": runs["fp32"]["tl"]["rays_per_thread"],
+                       "geometry": GEOMETRY["fp32"]},
                    "grid_tuning": tune_grid["trials"] if tune_grid else None,
                    "launch_tuning": tl["trials"], "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",

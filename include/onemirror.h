@@ -413,7 +413,9 @@ enum { OM_BATCH_PER_SCENE = 1024 };
  * for one scene).  Waves take chunks of at most 2048 rays of one heliostat from 8 per-XCD
  * queues in scene order; a workgroup adds its window to the scene's maps in device memory
  * when its waves move on to another scene (or grid).  rays_per_thread is ignored.  The maps
- * are the same. */
+ * are the same.  It also takes OM_PREC_MIXED scenes (all scenes of the batch then mixed):
+ * the mixed trace's rays with an uncertain decision are listed per scene and grid and traced
+ * in FP64 after the launch, and a scene whose list fills up is traced again wholly in FP64. */
 enum { OM_BATCH_WINDOW = 2048 };
 
 /* Batch over scenes[0..n) with outputs outs[0..n) (counts and energy required, flux and
@@ -425,6 +427,11 @@ om_status om_batch_create(om_ctx *ctx, om_scene *const *scenes, int n, const om_
  * every run replays it.  Calls on one batch must not overlap. */
 om_status om_batch_run(om_batch *batch, void *stream);
 om_status om_batch_info(const om_batch *batch, om_launch *launch, int64_t *items, int64_t *rays);
+/* OM_BATCH_WINDOW over OM_PREC_MIXED scenes: the rays of the batch's last run that a decision
+ * within its error bound sent to the FP64 trace, and the scenes whose redo list filled up
+ * (each then traced wholly in FP64).  Zero for other batches.  Waits for the device.  No
+ * counterpart in the reference (its traces are FP32 only). */
+om_status om_batch_redo_count(const om_batch *batch, int64_t *rays, int64_t *full_scenes);
 om_status om_batch_destroy(om_batch *batch);
 /* One-shot batch without a graph; synchronous (waits for the stream). */
 om_status om_trace_batched(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux *outs,

@@ -160,6 +160,7 @@ _EXPORTS = [
     "om_diff_visibility", "om_scene_fork", "om_sun_direction", "om_dni_hottel", "om_scn_write",
     "om_master_normal", "om_facet_heliostat", "om_facet_scene_desc", "om_savetxt_map", "om_savetxt_ratio",
     "om_load_normals", "om_trace_plain_noalloc", "om_ctx_abort", "om_ctx_progress", "om_scene_redo_count",
+    "om_batch_redo_count",
 ]
 
 
@@ -223,6 +224,7 @@ def lib():
     L.om_batch_run.argtypes = [_vp, _vp]
     L.om_batch_info.argtypes = [_vp, ctypes.POINTER(Launch), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(ctypes.c_int64)]
+    L.om_batch_redo_count.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.om_batch_destroy.argtypes = [_vp]
     L.om_trace_batched.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(Flux),
                                    ctypes.POINTER(Launch), _vp]
@@ -787,6 +789,13 @@ class Batch:
     def run(self, stream=None):
         _check(lib().om_batch_run(self._h, _stream_ptr(stream)), "om_batch_run")
         return self.maps
+
+    def redo_count(self):
+        """Mixed-precision persistent batches (om_batch_redo_count): the rays of the last run
+        traced again in FP64, and the scenes whose redo list filled up.  Waits for the device."""
+        r, f = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().om_batch_redo_count(self._h, ctypes.byref(r), ctypes.byref(f)), "om_batch_redo_count")
+        return r.value, f.value
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _LIB is not None:

@@ -52,6 +52,16 @@ struct om_batch {
     std::vector<om_flux> outs;
     std::vector<float *> raw;                   // per scene, for smoothing
     om::TraceArgs *d_table = nullptr;
+    // OM_BATCH_WINDOW over OM_PREC_MIXED scenes: one MixLaunch per entry, the entries' redo
+    // lists and counters (4 words per entry, then a flag per scene), each entry's scene and
+    // range of chunks
+    bool mixed = false;
+    void *d_mtable = nullptr;
+    uint2 *d_redo = nullptr;
+    uint32_t *d_mixc = nullptr;
+    int *d_entry_scene = nullptr;
+    int2 *d_entry_chunks = nullptr;
+    int n_entries = 0;
     int4 *d_items = nullptr;                    // work items, or OM_BATCH_WINDOW's chunks
     unsigned int *d_queue = nullptr;            // queue block: one counter, or OM_BATCH_WINDOW's 8 queues
     om::ReduceSlot *d_slots = nullptr;
@@ -80,6 +90,11 @@ void batch_free(om_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     dfree(b->d_table);
+    if (b->d_mtable) (void)hipFree(b->d_mtable);
+    dfree(b->d_redo);
+    dfree(b->d_mixc);
+    dfree(b->d_entry_scene);
+    dfree(b->d_entry_chunks);
     dfree(b->d_items);
     dfree(b->d_queue);
     dfree(b->d_slots);
@@ -116,7 +131,23 @@ om_status batch_enqueue(om_batch *b, hipStream_t st)
     if ((rc = om::launch_queue_reset(b->d_queue, b->n_items, b->window ? om::QUEUE_EIGHTHS : om::QUEUE_ONE, st)) !=
         OM_OK)
         return rc;
-    if (b->window) {
+    if (b->mixed) {
+        om::MixBatch m;
+        m.sunshape = b->sunshape;
+        m.poisson = b->poisson;
+        m.table = b->d_mtable;
+        m.chunks = b->d_items;
+        m.n_chunks = b->n_items;
+        m.queue = b->d_queue;
+        m.nwg = b->nblocks;
+        m.entry_scene = b->d_entry_scene;
+        m.entry_chunks = b->d_entry_chunks;
+        m.n_entries = b->n_entries;
+        m.counters = b->d_mixc;
+        m.slots = b->d_slots;
+        m.n_scenes = b->n;
+        if ((rc = om::launch_trace_batch_mixed(m, st)) != OM_OK) return rc;
+    } else if (b->window) {
         if (b->n_items > 0 &&
             (rc = om::launch_trace_batch_window(b->sunshape, b->poisson, b->d_table, b->d_items, b->n_items,
                                                 b->d_queue, b->nblocks, st)) != OM_OK)
@@ -173,9 +204,16 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
             delete b;
             return OM_ERR_SIZE;
         }
-        if (s->prm.precision != OM_PREC_FP32 && b->wgs != OM_BATCH_PER_SCENE) {
-            om::set_error("the persistent batch launch traces in FP32 geometry: FP64 and mixed-precision "
-                          "scenes need OM_BATCH_PER_SCENE");
+        if (s->prm.precision != OM_PREC_FP32 && b->wgs != OM_BATCH_PER_SCENE &&
+            !(s->prm.precision == OM_PREC_MIXED && b->wgs == OM_BATCH_WINDOW)) {
+            om::set_error("the persistent batch launch of 128, 256 or 512 threads traces in FP32 geometry: "
+                          "mixed-precision scenes need OM_BATCH_WINDOW or OM_BATCH_PER_SCENE, FP64 scenes "
+                          "OM_BATCH_PER_SCENE");
+            delete b;
+            return OM_ERR_UNSUPPORTED;
+        }
+        if (i > 0 && b->wgs == OM_BATCH_WINDOW && s->prm.precision != scenes[0]->prm.precision) {
+            om::set_error("scenes of a persistent batch launch must share the precision");
             delete b;
             return OM_ERR_UNSUPPORTED;
         }
@@ -193,6 +231,7 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     b->sunshape = scenes[0]->prm.sunshape;
     b->poisson = scenes[0]->prm.micro_mode == OM_MICRO_POISSON;
     b->window = b->wgs == OM_BATCH_WINDOW;
+    b->mixed = b->window && scenes[0]->prm.precision == OM_PREC_MIXED;
     if (b->window) {
         b->rpt = 0;
     } else if (b->rpt == 0) {
@@ -235,6 +274,8 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     // kernels' chunk size, in the same order
     std::vector<int4> items;
     std::vector<int> entry_scene, entry_grid;
+    std::vector<int2> entry_chunks;
+    std::vector<long long> entry_rays;
     const int block = b->window ? om::window_chunk_rays() : b->wgs * b->rpt;
     for (int i = 0; i < n; ++i) {
         const om_scene *s = scenes[i];
@@ -242,11 +283,16 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
             const int e = (int)entry_scene.size();
             entry_scene.push_back(i);
             entry_grid.push_back((int)g);
+            const int c0 = (int)std::min<size_t>(items.size(), INT32_MAX);
+            long long rays = 0;
             for (int h = s->grids[g].h0; h < s->grids[g].h1; ++h) {
                 const long long nr = (long long)s->mcount[h] * s->prm.rays_per_sample;
+                rays += nr;
                 for (long long r = 0; r < nr; r += block)
                     items.push_back(make_int4(e, h, (int)r, (int)std::min<long long>(block, nr - r)));
             }
+            entry_chunks.push_back(make_int2(c0, (int)std::min<size_t>(items.size(), INT32_MAX)));
+            entry_rays.push_back(rays);
         }
     }
     const int n_entries = (int)entry_scene.size();
@@ -312,6 +358,41 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
         b->raw.push_back(r.raw);
         px_off += (size_t)npx;
     }
+    if (b->mixed) {
+        // each entry's redo list: one entry per 64 rays, 2^12 to 2^24 (om_params.redo_cap
+        // when set); a list that fills up sends its whole scene to the FP64 trace
+        b->n_entries = n_entries;
+        std::vector<size_t> redo_off(n_entries);
+        size_t redo_total = 0;
+        std::vector<int> caps(n_entries);
+        for (int e = 0; e < n_entries; ++e) {
+            const om_scene *s = scenes[entry_scene[e]];
+            caps[e] = s->prm.redo_cap > 0
+                          ? s->prm.redo_cap
+                          : (int)std::min<long long>(std::max<long long>(entry_rays[e] / 64, 1 << 12), 1 << 24);
+            redo_off[e] = redo_total;
+            redo_total += (size_t)caps[e];
+        }
+        const size_t mlb = om::mix_launch_bytes();
+        std::vector<unsigned char> mt(mlb * (size_t)n_entries);
+        if ((rc = dalloc(b->d_redo, redo_total)) != OM_OK) return fail(rc);
+        if ((rc = dalloc(b->d_mixc, (size_t)4 * n_entries + n)) != OM_OK) return fail(rc);
+        if ((rc = dalloc(b->d_entry_scene, (size_t)n_entries)) != OM_OK) return fail(rc);
+        if ((rc = dalloc(b->d_entry_chunks, (size_t)n_entries)) != OM_OK) return fail(rc);
+        if (hipMalloc(&b->d_mtable, mt.size()) != hipSuccess)
+            return fail(om::hip_fail(hipGetLastError(), "hipMalloc (batch table)"));
+        for (int e = 0; e < n_entries; ++e)
+            if ((rc = om::mix_batch_entry(scenes[entry_scene[e]], entry_grid[e], table[e], b->d_redo + redo_off[e],
+                                          caps[e], b->d_mixc + 4 * (size_t)e, mt.data() + mlb * (size_t)e)) != OM_OK)
+                return fail(rc);
+        if (hipMemcpy(b->d_mtable, mt.data(), mt.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->d_entry_scene, entry_scene.data(), sizeof(int) * n_entries, hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(b->d_entry_chunks, entry_chunks.data(), sizeof(int2) * n_entries, hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemset(b->d_mixc, 0, sizeof(uint32_t) * ((size_t)4 * n_entries + n)) != hipSuccess)
+            return fail(om::hip_fail(hipGetLastError(), "hipMemcpy (mixed batch tables)"));
+    }
     if (hipMemcpy(b->d_table, table.data(), sizeof(om::TraceArgs) * n_entries, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_slots, slots.data(), sizeof(om::ReduceSlot) * n, hipMemcpyHostToDevice) != hipSuccess ||
         (!items.empty() && hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(),
@@ -323,8 +404,9 @@ om_status batch_build(om_ctx *ctx, om_scene *const *scenes, int n, const om_flux
     // persistent workgroups: as many as are resident at once
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return fail(OM_ERR_HIP);
-    int per_cu = b->window ? om::trace_batch_window_blocks_per_cu(b->sunshape, b->poisson)
-                           : om::trace_batch_blocks_per_cu(b->sunshape, b->poisson, b->wgs);
+    int per_cu = b->mixed    ? om::trace_batch_window_mixed_blocks_per_cu(b->sunshape, b->poisson)
+                 : b->window ? om::trace_batch_window_blocks_per_cu(b->sunshape, b->poisson)
+                             : om::trace_batch_blocks_per_cu(b->sunshape, b->poisson, b->wgs);
     if (b->window && per_cu < 1)
         return fail((om::set_error("trace_batch_window_kernel cannot be resident on this device"), OM_ERR_UNSUPPORTED));
     if (per_cu < 1) per_cu = 1;
@@ -375,6 +457,24 @@ extern "C" om_status om_batch_info(const om_batch *b, om_launch *launch, int64_t
     }
     if (items) *items = b->n_items;
     if (rays) *rays = b->rays;
+    return OM_OK;
+}
+
+extern "C" om_status om_batch_redo_count(const om_batch *b, int64_t *rays, int64_t *full_scenes)
+{
+    if (!b) return OM_ERR_INVALID;
+    if (rays) *rays = 0;
+    if (full_scenes) *full_scenes = 0;
+    if (!b->mixed) return OM_OK;
+    OM_HIP(hipSetDevice(b->ctx->device));
+    OM_HIP(hipDeviceSynchronize());
+    std::vector<uint32_t> c((size_t)4 * b->n_entries + b->n);
+    OM_HIP(hipMemcpy(c.data(), b->d_mixc, sizeof(uint32_t) * c.size(), hipMemcpyDeviceToHost));
+    int64_t r = 0, f = 0;
+    for (int e = 0; e < b->n_entries; ++e) r += c[4 * (size_t)e];
+    for (int i = 0; i < b->n; ++i) f += c[4 * (size_t)b->n_entries + i] != 0u;
+    if (rays) *rays = r;
+    if (full_scenes) *full_scenes = f;
     return OM_OK;
 }
 

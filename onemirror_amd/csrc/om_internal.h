@@ -412,6 +412,33 @@ om_status launch_reduce_batch(const ReduceSlot *slots, int nslot, int max_npx, h
 om_status launch_trace_batch_window(int sunshape, bool poisson, const TraceArgs *table, const int4 *chunks,
                                     int n_chunks, unsigned int *queue, int nwg, hipStream_t st);
 int trace_batch_window_blocks_per_cu(int sunshape, bool poisson);
+// batched mode with LDS windows over OM_PREC_MIXED scenes (om_trace64.hip): table holds
+// n_entries entry arguments of mix_launch_bytes() each, made by mix_batch_entry from the
+// entry's batch TraceArgs a (its copies and window), its redo list of redo_cap entries and its
+// 4 counter words; counters holds 4 words per entry, then one flag per scene.  One launch
+// runs: counter reset, the persistent mixed kernel, the FP64 trace of every redo list, and
+// for each scene whose list filled up the clear of its maps and the FP64 trace of the whole
+// scene (slots: the scenes' maps, entry_chunks: each entry's range of chunks).
+struct MixBatch {
+    int sunshape = 0;
+    bool poisson = false;
+    const void *table = nullptr;
+    const int4 *chunks = nullptr;
+    int n_chunks = 0;
+    unsigned int *queue = nullptr;
+    int nwg = 0;
+    const int *entry_scene = nullptr;
+    const int2 *entry_chunks = nullptr;
+    int n_entries = 0;
+    uint32_t *counters = nullptr;
+    const ReduceSlot *slots = nullptr;
+    int n_scenes = 0;
+};
+size_t mix_launch_bytes();
+om_status mix_batch_entry(const om_scene *s, int g, const TraceArgs &a, uint2 *redo, int redo_cap,
+                          uint32_t *redo_n, void *dst);
+om_status launch_trace_batch_mixed(const MixBatch &m, hipStream_t st);
+int trace_batch_window_mixed_blocks_per_cu(int sunshape, bool poisson);
 // the window of a res_x x res_y map the LDS holds: the whole map when it fits, else the
 // largest near-square block about the centre; the rays of one chunk of the window kernels
 void window_rect(int rx, int ry, int &x0, int &y0, int &w, int &h);

@@ -583,40 +583,6 @@ __global__ __launch_bounds__(WGS) void trace_batch_kernel(const TraceArgs *__res
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// Batched mode with LDS windows (OM_BATCH_WINDOW)
-// ---------------------------------------------------------------------------------------
-// One persistent workgroup of WIN_WGS threads per CU.  Its waves take chunks (table entry,
-// heliostat, first ray, rays) of at most WIN_CHUNK rays from 8 per-XCD queues over
-// contiguous eighths of the chunk list, which is in entry order (an entry is one grid of one
-// scene), so the workgroups of an XCD work through a few scenes at a time.  The workgroup
-// keeps the central block of one entry's packed map in LDS.  A wave traces the chunks it
-// takes while they belong to that entry; holding a chunk of another entry, it waits at the
-// workgroup barrier until every wave does (or has no chunk left).  There the entry of the
-// smallest held chunk becomes the window's: if it differs, the window's nonzero words are
-// added to the old entry's per-XCD copies (one returning atomic each, carries and wraps
-// counted as for single hits, which the decode then applies alike) and the window is
-// cleared.  Barriers come only at entry changes, so waves do not wait for one another
-// between chunks.  Hits outside the window go through the wave's buffer (WaveOut), which is
-// flushed before the wave reaches the barrier.
-constexpr int BW_NONE = 0x7fffffff; // no chunk held
-
-// Adds the window's nonzero words to entry a's copies and clears the window (the whole
-// workgroup; barriers around it are the caller's).
-__device__ __forceinline__ void flush_window(const TraceArgs &a, unsigned long long *s_win)
-{
-    unsigned long long *part = copy_of(a);
-    const int wpx = a.win_w * a.win_h;
-    for (int i = threadIdx.x; i < wpx; i += WIN_WGS) {
-        const unsigned long long w = s_win[i];
-        if (w == 0ull) continue;
-        s_win[i] = 0ull;
-        const int row = i / a.win_w, col = i - row * a.win_w;
-        const int px = (a.win_y0 + row) * a.res_x + a.win_x0 + col;
-        copy_check(a, px, w, copy_add(a, part, px, w));
-    }
-}
-
 // nrays rays of heliostat ck.y from ray ck.z, traced by one wave: window hits as LDS atomics,
 // the others through the wave's buffer, which the caller flushes before the entry changes
 template <int SUN, bool POISSON>
@@ -663,6 +629,7 @@ __device__ __forceinline__ void trace_chunk_batch(const TraceArgs &a, int4 ck, c
     }
 }
 
+// OM_BATCH_WINDOW in FP32 geometry (batch_window_body, om_window.h)
 template <int SUN, bool POISSON>
 __global__ __launch_bounds__(WIN_WGS) void trace_batch_window_kernel(const TraceArgs *__restrict__ table,
                                                                      const int4 *__restrict__ chunks, int n_chunks,
@@ -671,53 +638,11 @@ __global__ __launch_bounds__(WIN_WGS) void trace_batch_window_kernel(const Trace
     __shared__ unsigned long long s_win[WIN_PX_MAX];
     __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
     __shared__ int s_held[WIN_WAVES];
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
-    for (int i = threadIdx.x; i < WIN_PX_MAX; i += WIN_WGS) s_win[i] = 0ull;
-    const int q = n_chunks >> 3, rmd = n_chunks & 7;
-    const uint32_t xcd = xcc_id() & 7u;
-    int qj = 0; // queues this wave has found empty: its own XCD's first, then the others in turn
-    // the next chunk for this wave, or -1 when every queue is empty
-    auto pop = [&]() -> int {
-        while (qj < 8) {
-            const int x = (int)((xcd + (uint32_t)qj) & 7u);
-            const int cnt = q + (x < rmd ? 1 : 0);
-            const uint32_t t = take_chunk(queue + x, table[0].abort_flag, lane);
-            if (t == TAKE_ABORT) {
-                qj = 8;
-                break;
-            }
-            if (t < (uint32_t)cnt) return x * q + min(x, rmd) + (int)t;
-            ++qj;
-        }
-        return -1;
-    };
-    int held = pop();
-    int he = held >= 0 ? __builtin_amdgcn_readfirstlane(chunks[held].x) : BW_NONE; // entry of the held chunk
-    int cur = -1; // entry whose window the LDS holds (uniform over the workgroup)
-    WaveOut wo{s_out[wave], 0};
-    for (;;) {
-        if (lane == 0) s_held[wave] = he;
-        __syncthreads(); // every wave holds a chunk of another entry or none; the window is idle
-        int ne = BW_NONE;
-        for (int w = 0; w < WIN_WAVES; ++w) ne = min(ne, s_held[w]);
-        ne = __builtin_amdgcn_readfirstlane(ne);
-        if (ne == BW_NONE) break;
-        if (ne != cur) {
-            if (cur >= 0) flush_window(table[cur], s_win);
-            cur = ne;
-        }
-        __syncthreads(); // the window is clear, s_held read, before hits and the next round
-        while (he == cur) {
-            const int4 c = chunks[held];
-            const int4 ck = make_int4(cur, __builtin_amdgcn_readfirstlane(c.y), __builtin_amdgcn_readfirstlane(c.z),
-                                      __builtin_amdgcn_readfirstlane(c.w));
-            trace_chunk_batch<SUN, POISSON>(table[cur], ck, lane, s_win, wo);
-            held = pop();
-            he = held >= 0 ? __builtin_amdgcn_readfirstlane(chunks[held].x) : BW_NONE;
-        }
-        wo.flush(table[cur], lane);
-    }
-    if (cur >= 0) flush_window(table[cur], s_win);
+    batch_window_body(
+        table, chunks, n_chunks, queue, s_win, s_out, s_held, [](const TraceArgs &a) -> const TraceArgs & { return a; },
+        [](const TraceArgs &a, int4 ck, int lane, unsigned long long *w, WaveOut &wo) {
+            trace_chunk_batch<SUN, POISSON>(a, ck, lane, w, wo);
+        });
 }
 
 // Batched decode: blockIdx.y selects the map; writes (not adds) counts and energy, the raw

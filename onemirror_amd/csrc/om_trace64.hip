@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "om_window.h"
 
@@ -583,6 +584,7 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
 constexpr int MIX_UNSURE = -3;
 constexpr float U24 = 5.9604644775390625e-08f; // 2^-24
 constexpr int REDO_BLOCKS = 1024;              // workgroups of the redo list's FP64 trace
+constexpr int REDO_BATCH_BLOCKS = 64;          // per entry, in the mixed batch launch
 
 __device__ __forceinline__ float norm1(v3 a) { return fabs_(a.x) + fabs_(a.y) + fabs_(a.z); }
 
@@ -1173,6 +1175,118 @@ __global__ __launch_bounds__(WG) void trace_mixed_records_kernel(MixLaunch)
     }
 }
 
+
+// ---------------------------------------------------------------------------------------
+// OM_BATCH_WINDOW with OM_PREC_MIXED: the persistent batch launch of mixed scenes
+// ---------------------------------------------------------------------------------------
+// The table holds one MixLaunch per entry (one grid of one scene), with the batch's packed
+// copies of the scene's map, the entry's own redo list and its counters (redo_n[0]: rays
+// listed by this run, redo_n[2]: set when the list filled up).  The persistent kernel is the
+// FP32 batch window kernel's body with the mixed trace's chunks; a wave's buffer sends its
+// redo entries to the current entry's list before the window moves on to another entry.
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WIN_WGS) void trace_batch_window_mixed_kernel(const MixLaunch *__restrict__ table,
+                                                                           const int4 *__restrict__ chunks,
+                                                                           int n_chunks, unsigned int *queue)
+{
+    __shared__ unsigned long long s_win[WIN_PX_MAX];
+    __shared__ unsigned long long s_out[WIN_WAVES][WIN_OUT];
+    __shared__ int s_held[WIN_WAVES];
+    batch_window_body(
+        table, chunks, n_chunks, queue, s_win, s_out, s_held, [](const MixLaunch &L) -> const TraceArgs & { return L.a; },
+        [](const MixLaunch &L, int4 ck, int lane, unsigned long long *w, WaveOut &wo) {
+            trace_chunk_mixed<SUN, POISSON>(L, make_int2(ck.y, ck.z), lane, ck.w, wo, w);
+        });
+}
+
+// Every entry's redo list traced in FP64 into its scene's copies (blockIdx.y: the entry).  A
+// list that filled up marks the entry (redo_n[2]) and its scene (scene_full), whose maps are
+// then cleared and traced again wholly in FP64 (trace64_batch_full_kernel).
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WG) void trace_redo_batch_kernel(const MixLaunch *__restrict__ table,
+                                                              const int *__restrict__ entry_scene,
+                                                              uint32_t *scene_full)
+{
+    const MixLaunch &L = table[blockIdx.y];
+    const TraceArgs &a = L.a;
+    const uint32_t cnt = *L.x.redo_n;
+    if (cnt > (uint32_t)L.x.redo_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            L.x.redo_n[2] = 1u;
+            atomicOr(scene_full + entry_scene[blockIdx.y], 1u);
+        }
+        return;
+    }
+    unsigned long long *part = copy_of(a);
+    const unsigned long long one = 1ull << a.pack_shift;
+    for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < cnt; i += gridDim.x * WG) {
+        const uint2 e = L.x.redo[i];
+        const Ray64 o = ray64_at<SUN, POISSON>(a, L.b, (int)e.x, (int)e.y);
+        if (o.px >= 0) {
+            const unsigned long long w = one + o.q;
+            copy_check(a, o.px, w, copy_add(a, part, o.px, w));
+        }
+    }
+}
+
+// The scenes whose redo list filled up (blockIdx.y: the scene): their packed copies and side
+// maps cleared for the FP64 trace of the whole scene.
+__global__ void clear_full_scenes_kernel(const ReduceSlot *__restrict__ slots, const uint32_t *__restrict__ scene_full)
+{
+    if (scene_full[blockIdx.y] == 0u) return;
+    const ReduceSlot &s = slots[blockIdx.y];
+    const long long nw = (long long)s.ncopy * s.copy_stride, step = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nw; i += step) s.part[i] = 0ull;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < s.npx; i += step) {
+        s.carry[i] = 0u;
+        s.wrap[i] = 0u;
+    }
+}
+
+// The FP64 trace of every chunk of the scenes whose redo list filled up, one ray per thread
+// from its indices (ray64_at), into the scene's copies; entry_chunks[e] is the range of
+// entry e's chunks.  Scenes that fitted are skipped without reading their chunks.
+template <int SUN, bool POISSON>
+__global__ __launch_bounds__(WG) void trace64_batch_full_kernel(const MixLaunch *__restrict__ table,
+                                                                const int4 *__restrict__ chunks,
+                                                                const int2 *__restrict__ entry_chunks,
+                                                                const int *__restrict__ entry_scene,
+                                                                const uint32_t *__restrict__ scene_full, int n_entries)
+{
+    for (int e = 0; e < n_entries; ++e) {
+        if (scene_full[entry_scene[e]] == 0u) continue;
+        const MixLaunch &L = table[e];
+        const TraceArgs &a = L.a;
+        unsigned long long *part = copy_of(a);
+        const unsigned long long one = 1ull << a.pack_shift;
+        const int2 cr = entry_chunks[e];
+        for (int c = cr.x + (int)blockIdx.x; c < cr.y; c += (int)gridDim.x) {
+            const int4 ck = chunks[c];
+            for (int i = (int)threadIdx.x; i < ck.w; i += WG) {
+                const Ray64 o = ray64_at<SUN, POISSON>(a, L.b, ck.y, ck.z + i);
+                if (o.px >= 0) {
+                    const unsigned long long w = one + o.q;
+                    copy_check(a, o.px, w, copy_add(a, part, o.px, w));
+                }
+            }
+        }
+    }
+}
+
+// Each run starts with empty lists: redo_n[0] and redo_n[2] of every entry (4 words per
+// entry at c) and the scene flags after them.
+__global__ void batch_mix_reset_kernel(uint32_t *c, int n_entries, int n_scenes)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_entries + n_scenes; i += gridDim.x * blockDim.x) {
+        if (i < n_entries) {
+            c[4 * i] = 0u;
+            c[4 * i + 2] = 0u;
+        } else {
+            c[4 * n_entries + (i - n_entries)] = 0u;
+        }
+    }
+}
+
 } // namespace
 
 // When the mixed launch's redo list filled up (*gate > cap): clears n words at p and the
@@ -1339,6 +1453,75 @@ om_status launch_trace_mixed_records(om_scene *s, const TraceArgs &a, const Trac
     }
     OM_HIP(hipGetLastError());
     return OM_OK;
+}
+
+size_t mix_launch_bytes() { return sizeof(MixLaunch); }
+
+om_status mix_batch_entry(const om_scene *s, int g, const TraceArgs &a, uint2 *redo, int redo_cap,
+                          uint32_t *redo_n, void *dst)
+{
+    if (!mixed_supported(s)) {
+        set_error("a scene of the batch cannot take the mixed path (cone receiver or literal self-intersection): "
+                  "OM_BATCH_PER_SCENE traces it in FP64");
+        return OM_ERR_UNSUPPORTED;
+    }
+    MixLaunch L{a, trace64_args(s, g), MixArgs{}};
+    mixed_args(s, g, L.a, L.x);
+    if (!L.b.helio || !L.b.cell_geo || !L.a.cell_geo || !L.x.hmix) {
+        set_error("scene has no mixed-precision geometry");
+        return OM_ERR_INVALID;
+    }
+    L.x.redo = redo;
+    L.x.redo_cap = redo_cap;
+    L.x.redo_n = redo_n;
+    std::memcpy(dst, &L, sizeof(L));
+    return OM_OK;
+}
+
+om_status launch_trace_batch_mixed(const MixBatch &m, hipStream_t st)
+{
+    if (m.n_chunks <= 0 || m.n_entries <= 0) return OM_OK;
+    const MixLaunch *table = (const MixLaunch *)m.table;
+    hipLaunchKernelGGL(batch_mix_reset_kernel, dim3((m.n_entries + m.n_scenes + WG - 1) / WG), dim3(WG), 0, st,
+                       m.counters, m.n_entries, m.n_scenes);
+    uint32_t *full = m.counters + 4 * (size_t)m.n_entries;
+    const dim3 grid(m.nwg), block(WIN_WGS), rgrid(REDO_BATCH_BLOCKS, m.n_entries), wblock(WG);
+#define OM_MIXB(SUN, P)                                                                                             \
+    do {                                                                                                            \
+        hipLaunchKernelGGL((trace_batch_window_mixed_kernel<SUN, P>), grid, block, 0, st, table, m.chunks,           \
+                           m.n_chunks, m.queue);                                                                    \
+        hipLaunchKernelGGL((trace_redo_batch_kernel<SUN, P>), rgrid, wblock, 0, st, table, m.entry_scene, full);     \
+        hipLaunchKernelGGL(clear_full_scenes_kernel, dim3(256, m.n_scenes), wblock, 0, st, m.slots, full);           \
+        hipLaunchKernelGGL((trace64_batch_full_kernel<SUN, P>), dim3(REDO_BLOCKS), wblock, 0, st, table, m.chunks,   \
+                           m.entry_chunks, m.entry_scene, full, m.n_entries);                                       \
+    } while (0)
+    if (m.sunshape == OM_SUN_BUIE) {
+        if (m.poisson) OM_MIXB(OM_SUN_BUIE, true);
+        else OM_MIXB(OM_SUN_BUIE, false);
+    } else {
+        if (m.poisson) OM_MIXB(OM_SUN_PILLBOX, true);
+        else OM_MIXB(OM_SUN_PILLBOX, false);
+    }
+#undef OM_MIXB
+    OM_HIP(hipGetLastError());
+    return OM_OK;
+}
+
+int trace_batch_window_mixed_blocks_per_cu(int sunshape, bool poisson)
+{
+    int n = 0;
+    hipError_t e;
+    if (sunshape == OM_SUN_BUIE)
+        e = poisson ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_batch_window_mixed_kernel<OM_SUN_BUIE, true>,
+                                                                   WIN_WGS, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, trace_batch_window_mixed_kernel<OM_SUN_BUIE, false>,
+                                                                   WIN_WGS, 0);
+    else
+        e = poisson ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &n, trace_batch_window_mixed_kernel<OM_SUN_PILLBOX, true>, WIN_WGS, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &n, trace_batch_window_mixed_kernel<OM_SUN_PILLBOX, false>, WIN_WGS, 0);
+    return e == hipSuccess ? n : 0;
 }
 
 Trace64Args trace64_args(const om_scene *s, int g)

@@ -19,8 +19,9 @@ the pins are those of the FP64 trace:
     the FP64 trace's, bit for bit, and om_ctx_progress reports that trace's chunks, all
     taken; a list that fits leaves the mixed launch's own chunk counts to om_ctx_progress;
   * a per-scene batch of mixed scenes equals their plain traces on three replays of its graph
-    with eager traces between them; the persistent batch launch
-    and the differentiable mode refuse mixed scenes.
+    with eager traces between them; the persistent batch launch of 128-512 threads and the
+    differentiable mode refuse mixed scenes (the persistent launch with LDS windows takes
+    them: test_gpu_batch_mixed.py).
 """
 import numpy as np
 import pytest
