@@ -539,21 +539,43 @@ def main() -> int:
                                                        "grid_cells", "workgroup", "rays_per_thread",
                                                        "redo_rays", "full_lists", "fp32_reference")}},
             "annual": {"value": an["value"], "unit": "rays/s", "workload": an["config"]["workload"],
-                       **{k: an["config"][k] for k in ("time_steps", "rays", "mode", "batch_ms", "plain_ms",
-                                                        "time_steps_per_s", "annual_ratio")}},
+                       **{k: an["config"][k] for k in ("precision", "time_steps", "rays", "mode", "batch_ms",
+                                                        "plain_ms", "time_steps_per_s", "annual_ratio",
+                                                        "fp32_reference")}},
             "facet_annual": {"value": fa["value"], "unit": "rays/s", "workload": fa["config"]["workload"],
-                             **{k: fa["config"][k] for k in ("time_steps", "rays", "mode", "batch_ms", "plain_ms",
-                                                              "time_steps_per_s", "annual_ratio")}},
+                             **{k: fa["config"][k] for k in ("precision", "time_steps", "rays", "mode", "batch_ms",
+                                                              "plain_ms", "time_steps_per_s", "annual_ratio",
+                                                              "fp32_reference")}},
             "fwd_bwd": {"value": d["config"]["fwd_bwd_per_s"], "unit": "fwd+bwd/s",
                         "rays_per_s": d["value"], "workload": d["config"]["workload"],
                         **{k: d["config"][k] for k in ("fwd_bwd_ms", "forward_ms", "backward_ms",
-                                                       "plain_trace_ms", "fwd_bwd_vs_plain")}},
+                                                       "plain_trace_ms", "fwd_bwd_vs_plain",
+                                                       "plain_trace_mixed_ms", "fwd_bwd_vs_plain_mixed")}},
         })
     print(json.dumps(out), flush=True)
     return 0
 
 
+def with_fp32_reference(fn, args) -> dict:
+    """fn's line in the mixed precision (the metric's accuracy), with fn's FP32 run of the
+    same workload under config.fp32_reference."""
+    out = fn(args, "mixed")
+    ref = fn(args, "fp32")
+    out["config"]["fp32_reference"] = {
+        "value": ref["value"], "unit": ref["unit"], "ms_per_step": ref["ms_per_step"],
+        **{k: ref["config"][k] for k in ("mode", "batch_ms", "plain_ms", "time_steps_per_s")}}
+    return out
+
+
 def bench_annual(args) -> dict:
+    return with_fp32_reference(annual_once, args)
+
+
+def bench_facet_annual(args) -> dict:
+    return with_fp32_reference(facet_annual_once, args)
+
+
+def annual_once(args, precision: str) -> dict:
     """Annual driver (main.cpp:56-116): the mid field at every time step of the reference's
     schedule with the sun above 15 degrees, each step a fork of the field, traced by the
     batched launch and by one plain (accumulation-autotuned) trace per step; the faster is
@@ -565,13 +587,14 @@ def bench_annual(args) -> dict:
     from onemirror_amd import fields as F
 
     d, p = F.config_mid()
+    p = dict(p, precision={"fp32": om.PREC_FP32, "mixed": om.PREC_MIXED}[precision])
     t0 = time.time()
     root = om.SolarScene(d, p)
     acc = root.autotune_accumulate(reps=2)
     steps = A.annual_steps()
     runs = {m: A.AnnualRun(root, steps, batch_size=64, mode=m) for m in ("batch", "plain")}
     R = runs["batch"].rays
-    print(f"# annual: {len(steps)} time steps (zenith <= {A.ZENITH_MAX}), {R} rays, forks built in "
+    print(f"# annual {precision}: {len(steps)} time steps (zenith <= {A.ZENITH_MAX}), {R} rays, forks built in "
           f"{time.time() - t0:.1f}s; accumulation {acc['accumulate']}", file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream()
     ms = {}
@@ -586,7 +609,7 @@ def bench_annual(args) -> dict:
         e1.record(stream)
         torch.cuda.synchronize()
         ms[m] = e0.elapsed_time(e1) / args.steps
-        print(f"# annual {m}: {ms[m]:.1f} ms per year ({R / (ms[m] * 1e-3):.4g} rays/s)", file=sys.stderr, flush=True)
+        print(f"# annual {precision} {m}: {ms[m]:.1f} ms per year ({R / (ms[m] * 1e-3):.4g} rays/s)", file=sys.stderr, flush=True)
     best = min(ms, key=ms.get)
     res = runs[best].results()
     print(f"# annual intercept ratio {res['annual_ratio']:.4f}, intercepted {res['intercepted_w_sum'] / 1e6:.1f} "
@@ -599,6 +622,7 @@ def bench_annual(args) -> dict:
                                 "every 3, hours 6-18 every 2) at 30.3N 120.2E UTC+8, 40 m, zenith <= 75 deg "
                                 "(approximate solar positions), clear-sky DNI of getDNI; one step = the "
                                 "whole year"),
+                   "precision": precision, "geometry": GEOMETRY[precision],
                    "time_steps": len(steps), "rays": R, "mode": best,
                    "batch_ms": ms["batch"], "plain_ms": ms["plain"],
                    "time_steps_per_s": len(steps) / (ms[best] * 1e-3),
@@ -608,7 +632,7 @@ def bench_annual(args) -> dict:
     }
 
 
-def bench_facet_annual(args) -> dict:
+def facet_annual_once(args, precision: str) -> dict:
     """The reference driver's own run (main.cpp:56-116): the faceted paraboloid heliostat
     rebuilt and traced for every time step of the year with the reference's default
     parameters, batched and one plain trace per step; the faster is the value."""
@@ -621,7 +645,7 @@ def bench_facet_annual(args) -> dict:
     site = F.facet_site()
     t0 = time.time()
     steps = A.annual_steps()
-    p = om.default_params()
+    p = dict(om.default_params(), precision={"fp32": om.PREC_FP32, "mixed": om.PREC_MIXED}[precision])
     # accumulation autotune on the middle step's scene, kept for every step's plain trace
     mid = steps[len(steps) // 2]
     probe = om.SolarScene(om.facet_scene(site["a"], site["b"], site["X"], site["Y"], site["Z"], mid["sun_dir"],
@@ -635,7 +659,7 @@ def bench_facet_annual(args) -> dict:
                                   p if m == "batch" else p_plain, batch_size=len(steps), mode=m)
             for m in ("batch", "plain")}
     R = runs["batch"].rays
-    print(f"# facet annual: {len(steps)} time steps, {R} rays ({R // len(steps)} a step), scenes built in "
+    print(f"# facet annual {precision}: {len(steps)} time steps, {R} rays ({R // len(steps)} a step), scenes built in "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream()
     ms = {}
@@ -650,7 +674,7 @@ def bench_facet_annual(args) -> dict:
         e1.record(stream)
         torch.cuda.synchronize()
         ms[m] = e0.elapsed_time(e1) / args.steps
-        print(f"# facet annual {m}: {ms[m]:.1f} ms per year ({R / (ms[m] * 1e-3):.4g} rays/s)", file=sys.stderr,
+        print(f"# facet annual {precision} {m}: {ms[m]:.1f} ms per year ({R / (ms[m] * 1e-3):.4g} rays/s)", file=sys.stderr,
               flush=True)
     best = min(ms, key=ms.get)
     res = runs[best].results()
@@ -664,6 +688,7 @@ def bench_facet_annual(args) -> dict:
                                 "normal, 1.4 m receiver, reference defaults (Buie CSR 0.1, 1024 bins, slope "
                                 "1 mrad, lattice 0.01 m x 2048 rays); the reference schedule at 30.3N 120.2E, "
                                 "zenith <= 75 deg; one step = the whole year" % site["a"]),
+                   "precision": precision, "geometry": GEOMETRY[precision],
                    "time_steps": len(steps), "rays": R, "mode": best, "batch_ms": ms["batch"],
                    "plain_ms": ms["plain"], "time_steps_per_s": len(steps) / (ms[best] * 1e-3),
                    "accumulate": acc, "annual_ratio": res["annual_ratio"], "parallelism": "single GPU"},
@@ -883,8 +908,21 @@ def bench_diff(args) -> dict:
     torch.cuda.synchronize()
     fwd_ms = e[0].elapsed_time(e[1]) / args.steps
     plain_ms = e[2].elapsed_time(e[3]) / args.steps
+    # the plain trace at the metric's accuracy (OM_PREC_MIXED) of the same field
+    mixed = om.SolarScene(desc, dict(prm, precision=om.PREC_MIXED))
+    mmaps = mixed.new_maps(smooth=False)
+    mixed.autotune_accumulate(mmaps, reps=2)
+    mixed.trace_heliostats(0, H, mmaps)
+    e[2].record(stream)
+    for _ in range(args.steps):
+        mixed.trace_heliostats(0, H, mmaps)
+    e[3].record(stream)
+    torch.cuda.synchronize()
+    plain_mixed_ms = e[2].elapsed_time(e[3]) / args.steps
+    del mixed, mmaps
     print(f"# fwd+bwd {fb_ms:.2f} ms ({1e3 / fb_ms:.2f}/s), forward {fwd_ms:.2f} ms, plain trace "
-          f"{plain_ms:.2f} ms, loss {float(L):.6f}", file=sys.stderr, flush=True)
+          f"{plain_ms:.2f} ms (FP32), {plain_mixed_ms:.2f} ms (mixed), loss {float(L.detach()):.6f}",
+          file=sys.stderr, flush=True)
     return {
         "metric": METRIC, "value": R / dt, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
@@ -895,7 +933,8 @@ def bench_diff(args) -> dict:
                                "adjoint",
                    "heliostats": H, "rays": R, "fwd_bwd_per_s": 1e3 / fb_ms, "fwd_bwd_ms": fb_ms,
                    "forward_ms": fwd_ms, "backward_ms": fb_ms - fwd_ms, "plain_trace_ms": plain_ms,
-                   "fwd_bwd_vs_plain": plain_ms / fb_ms, "parallelism": "single GPU"},
+                   "fwd_bwd_vs_plain": plain_ms / fb_ms, "plain_trace_mixed_ms": plain_mixed_ms,
+                   "fwd_bwd_vs_plain_mixed": plain_mixed_ms / fb_ms, "parallelism": "single GPU"},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -16,7 +16,9 @@ and after the launch traces every list in FP64.  Pins:
     the plain FP64 trace's bit for bit, while the other scenes of the batch keep their mixed
     maps; om_batch_redo_count reports the listed rays and the one full scene;
   * scenes of differing precision, and mixed scenes that trace in FP64 (the cone receiver),
-    are refused by the persistent launch.
+    are refused by the persistent launch;
+  * the annual driver's batched steps over mixed forks have the FP64 model's counts of each
+    step's sun (on the step's own Philox stream) and equal plain traces of the forks.
 """
 import numpy as np
 import pytest
@@ -151,3 +153,37 @@ def test_persistent_launch_refusals(om):
         om.Batch([cone], workgroup=om.BATCH_WINDOW)
     with pytest.raises(om.OMError, match="mixed"):
         om.Batch([mixed], workgroup=256)
+
+
+def test_annual_mixed_batch_steps_equal_the_fp64_model(om):
+    """The annual driver over mixed-precision forks takes the persistent launch with LDS
+    windows: every time step (fork i, Philox stream i) has the counts of the FP64 model of
+    that step's sun and DNI, its flux is within 1e-5 of the model's on every lit pixel, and
+    the batched maps equal plain traces of the same forks bit for bit."""
+    import torch
+    from onemirror_amd import annual as A
+    d, p = F.config_small()
+    p = _mixed(p, om)
+    root = om.SolarScene(d, p)
+    steps = A.annual_steps([(3, 1, 8), (6, 16, 12), (9, 10, 16), (12, 4, 10)])
+    run = A.AnnualRun(root, steps, batch_size=4, mode="batch")
+    assert run._batches[0].workgroup == om.BATCH_WINDOW
+    run.run(keep_maps=True)
+    for i, st in enumerate(steps):
+        o = oracle.OracleScene(dict(d, sun_dir=np.float32(st["sun_dir"]), dni=np.float32(st["dni"])), p)
+        o.set_ray_stream(i)
+        want = o.trace_f64()
+        got = run.maps()[i]
+        torch.cuda.synchronize()
+        counts = got.counts.cpu().numpy().view(np.uint32).ravel()
+        assert np.array_equal(counts, want["counts"].ravel()), i
+        scale = float(np.float32(run.forks[i].info["flux_scale"]))
+        flux64 = want["energy"].ravel() * scale
+        lit = flux64 > 0
+        rel = np.abs(got.flux.cpu().numpy().astype(np.float64).ravel()[lit] - flux64[lit]) / flux64[lit]
+        assert rel.size > 0 and rel.max() <= 1e-5, (i, rel.max())
+        plain = run.forks[i].trace_plain()
+        torch.cuda.synchronize()
+        assert np.array_equal(plain.counts.cpu().numpy().view(np.uint32).ravel(), counts), i
+        assert np.array_equal(plain.energy.cpu().numpy().view(np.uint64).ravel(),
+                              got.energy.cpu().numpy().view(np.uint64).ravel()), i
