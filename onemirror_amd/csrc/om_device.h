@@ -277,6 +277,122 @@ __device__ __forceinline__ u4 philox_uniform_key(u4 c, uint32_t k0, uint32_t k1)
     return c;
 }
 
+// Per-tile state of one heliostat: its record, frame and position, and the sun frame.
+struct HelioTile {
+    const HelioRec *hr;
+    int h;
+    frame hf, sf;
+    v3 hpos;
+    bool one_row, one_col;
+    bool clear; // the shading walk finds nothing for any ray of this heliostat
+    int self;   // the id a walk skips: the heliostat's own, or -1 (none) for the literal rule
+};
+
+__device__ __forceinline__ HelioTile helio_tile(const TraceArgs &a, int h)
+{
+    HelioTile t;
+    t.h = h;
+    t.hr = a.helio + h;
+    const HelioRec *hr = t.hr;
+    t.hf.n = mk(hr->n[0], hr->n[1], hr->n[2]);
+    t.hf.u = mk(hr->fu[0], hr->fu[1], hr->fu[2]);
+    t.hf.v = mk(hr->fv[0], hr->fv[1], hr->fv[2]);
+    t.hf.identity = hr->ident;
+    t.sf.n = a.sun_n;
+    t.sf.u = a.sun_u;
+    t.sf.v = a.sun_v;
+    t.sf.identity = a.sun_ident;
+    t.hpos = mk(hr->pos[0], hr->pos[1], hr->pos[2]);
+    t.one_row = hr->rows == 1;
+    t.one_col = hr->cols == 1;
+    // under the literal rule a walk may stop at the ray's own heliostat, which the clear
+    // test (other heliostats only) does not cover
+    t.clear = hr->clear != 0 && !a.literal_self;
+    t.self = a.literal_self ? -1 : h;
+    return t;
+}
+
+// Origin of the rays of microsample m, shared by its L rays: the microsample centre in world
+// coordinates and the grid cell that holds it.
+struct RayStart {
+    v3 orig;
+    cell3 c0;
+};
+
+// t: any per-tile state with the heliostat record hr, its index h, one_row, one_col, the
+// frame hf and the position hpos (HelioTile, or the differentiable kernels' tile)
+template <bool POISSON, class Tile>
+__device__ __forceinline__ RayStart ray_start(const TraceArgs &a, const Tile &t, int m)
+{
+    const HelioRec *hr = t.hr;
+    // microsample centre (steps_for_raytracing.cu:4-38, :53-65)
+    v3 loc;
+    if (POISSON) {
+        float2 pt = a.poisson[(size_t)t.h * a.poisson_cap + m];
+        // valid points lie inside (0, width) x (0, height): one facet -> block 0
+        int br = t.one_row ? 0 : trunc_i(pt.y / hr->subrow_len);
+        int bc = t.one_col ? 0 : trunc_i(pt.x / hr->subcol_len);
+        loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
+    } else {
+        int row = m / hr->cols_all, col = m - row * hr->cols_all;
+        int br = row / hr->sub_rows, bc = col / hr->sub_cols;
+        loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
+                 (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
+    }
+    RayStart st;
+    st.orig = to_world(loc, t.hf) + t.hpos;
+    st.c0 = start_cell(a, st.orig);
+    return st;
+}
+
+// The starts of 64 consecutive microsamples mt, mt + 1, ... of a tile's heliostat, one per
+// lane of a wave (the origin and its cell).  A wave's pass of 64 rays spans at most 64
+// microsamples, so the table is refilled only when a pass reaches past it: once per L
+// passes, once per chunk of 2048 rays for L >= 32.  Each pass reads its rays' starts from
+// the lanes that hold them (ds_bpermute) instead of finding them again: with fewer than 64
+// rays per microsample a lane's microsample changes at every pass.  The whole wave calls
+// both members.
+struct StartTable {
+    int mt = -(1 << 30);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    int gx = 0, gy = 0, gz = 0;
+
+    template <bool POISSON, class Tile>
+    __device__ __forceinline__ void update(const TraceArgs &a, const Tile &t, bool active, int m, int lane)
+    {
+        if (__ballot(active && m - mt >= 64) == 0ull) return;
+        mt = __builtin_amdgcn_readfirstlane(m); // lane 0's: the pass's first microsample
+        const int mj = mt + lane;
+        if (mj < t.hr->mcount) {
+            const RayStart st = ray_start<POISSON>(a, t, mj);
+            ox = st.orig.x;
+            oy = st.orig.y;
+            oz = st.orig.z;
+            gx = st.c0.gx;
+            gy = st.c0.gy;
+            gz = st.c0.gz;
+        }
+    }
+    // the start of microsample m, mt <= m < mt + 64, as ray_start gives it
+    __device__ __forceinline__ RayStart get(const TraceArgs &a, int m) const
+    {
+        const int src = min(max(m - mt, 0), 63) << 2;
+        RayStart st;
+        st.orig = mk(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(ox))),
+                     __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(oy))),
+                     __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(oz))));
+        st.c0.gx = __builtin_amdgcn_ds_bpermute(src, gx);
+        st.c0.gy = __builtin_amdgcn_ds_bpermute(src, gy);
+        st.c0.gz = __builtin_amdgcn_ds_bpermute(src, gz);
+        // as start_cell forms them
+        st.c0.rx = st.orig.x - a.gpx;
+        st.c0.ry = st.orig.y - a.gpy;
+        st.c0.rz = st.orig.z - a.gpz;
+        st.c0.addr = (st.c0.gx * a.gny + st.c0.gy) * a.gnz + st.c0.gz;
+        return st;
+    }
+};
+
 // XCD (accelerator complex die) that runs the calling wave.  The packed map words are kept
 // one copy per XCD to spread same-pixel atomics; the atomics themselves are agent-scope, so
 // the choice of copy affects contention only, never the result.

@@ -154,24 +154,13 @@ struct DiffTile {
     bool clear; // the shading walk finds nothing for any ray of this heliostat
 };
 
-template <int SUN, bool POISSON>
-__device__ __forceinline__ void diff_sample(const TraceArgs &a, const DiffTile &T, int m, int l, v3 &orig,
-                                            v3 &to_sun, u4 &x)
+// The sun ray of ray l of microsample m (the plain path's Philox draws and sunshape): the
+// direction toward the sun and the draws, whose slope-error words the caller uses.  The
+// microsample's origin comes from the wave's start table (StartTable, om_device.h), the same
+// bits as ray_start.
+template <int SUN>
+__device__ __forceinline__ void diff_dir(const TraceArgs &a, const DiffTile &T, int m, int l, v3 &to_sun, u4 &x)
 {
-    const HelioRec *hr = T.hr;
-    v3 loc;
-    if (POISSON) {
-        float2 pt = a.poisson[(size_t)T.h * a.poisson_cap + m];
-        int br = T.one_row ? 0 : trunc_i(pt.y / hr->subrow_len);
-        int bc = T.one_col ? 0 : trunc_i(pt.x / hr->subcol_len);
-        loc = mk(pt.x + (float)bc * hr->gx - hr->sx2, hr->sy2, pt.y + (float)br * hr->gy - hr->sz2);
-    } else {
-        int row = m / hr->cols_all, col = m - row * hr->cols_all;
-        int br = row / hr->sub_rows, bc = col / hr->sub_cols;
-        loc = mk((float)col * a.pitch + (float)bc * hr->gx + a.pitch2 - hr->sx2, hr->sy2,
-                 (float)row * a.pitch + (float)br * hr->gy + a.pitch2 - hr->sz2);
-    }
-    orig = to_world(loc, T.hf) + T.hpos;
     x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)T.h, a.w3}, a.key0, a.key1);
     float th;
     if (SUN == OM_SUN_PILLBOX) {
@@ -237,6 +226,7 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs, DiffArgs)
     int m = r / T.L;
     int l = r - m * T.L;
     const int dm = WG / T.L, dl = WG - (WG / T.L) * T.L;
+    StartTable tab; // the wave's rays of a pass are 64 consecutive rays
     for (int k = 0; k < a.rays_per_thread; ++k, r += WG) {
         if (k > 0) {
             m += dm;
@@ -250,11 +240,14 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs, DiffArgs)
         DiffRay o;
         int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
         asm volatile("" ::: "memory");
+        tab.update<POISSON>(a, T, r < T.nr, m, lane); // the whole wave
+        const RayStart rs = tab.get(a, m);
         if (r < T.nr) {
-            v3 orig, to_sun;
+            v3 to_sun;
             u4 x;
-            diff_sample<SUN, POISSON>(a, T, m, l, orig, to_sun, x);
-            const cell3 c0 = start_cell(a, orig);
+            diff_dir<SUN>(a, T, m, l, to_sun, x);
+            const v3 orig = rs.orig;
+            const cell3 c0 = rs.c0;
             if (T.clear || dda(a, orig, c0, to_sun, T.h) < 0) {
                 const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
                 if (diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o) && dda(a, orig, c0, o.r, T.h) < 0) {
@@ -442,6 +435,7 @@ __global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs, DiffArgs)
     int m = r / T.L;
     int l = r - m * T.L;
     const int dm = WG / T.L, dl = WG - (WG / T.L) * T.L;
+    StartTable tab; // the wave's rays of a pass are 64 consecutive rays
     for (int k = 0; k < a.rays_per_thread; ++k, r += WG) {
         if (k > 0) {
             m += dm;
@@ -451,13 +445,17 @@ __global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs, DiffArgs)
                 ++m;
             }
         }
-        if (r >= T.nr) break;
+        // lane 0 holds the pass's first ray: past the heliostat's rays, every lane is
+        if (__builtin_amdgcn_readfirstlane(r) >= T.nr) break;
         asm volatile("" ::: "memory");
         const unsigned long long bits = p.mask[((size_t)tile * a.rays_per_thread + k) * (WG / 64) + wave];
-        if (!((bits >> lane) & 1ull)) continue;
-        v3 orig, to_sun;
+        tab.update<POISSON>(a, T, r < T.nr, m, lane); // the whole wave
+        const RayStart rs = tab.get(a, m);
+        if (r >= T.nr || !((bits >> lane) & 1ull)) continue;
+        v3 to_sun;
         u4 x;
-        diff_sample<SUN, POISSON>(a, T, m, l, orig, to_sun, x);
+        diff_dir<SUN>(a, T, m, l, to_sun, x);
+        const v3 orig = rs.orig;
         const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
         DiffRay o;
         if (!diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o)) continue; // (not reached)
