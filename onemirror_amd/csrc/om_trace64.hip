@@ -584,7 +584,6 @@ __global__ __launch_bounds__(WIN_WGS) void trace_window64_kernel(Trace64Launch)
 constexpr int MIX_UNSURE = -3;
 constexpr float U24 = 5.9604644775390625e-08f; // 2^-24
 constexpr int REDO_BLOCKS = 1024;              // workgroups of the redo list's FP64 trace
-constexpr int REDO_BATCH_BLOCKS = 64;          // per entry, in the mixed batch launch
 
 __device__ __forceinline__ float norm1(v3 a) { return fabs_(a.x) + fabs_(a.y) + fabs_(a.z); }
 
@@ -1199,32 +1198,40 @@ __global__ __launch_bounds__(WIN_WGS) void trace_batch_window_mixed_kernel(const
         });
 }
 
-// Every entry's redo list traced in FP64 into its scene's copies (blockIdx.y: the entry).  A
-// list that filled up marks the entry (redo_n[2]) and its scene (scene_full), whose maps are
-// then cleared and traced again wholly in FP64 (trace64_batch_full_kernel).
+// Every entry's redo list traced in FP64 into its scene's copies.  Each workgroup visits
+// every entry in turn and takes a strided share of its list, starting at a workgroup that
+// moves with the entry, so that the rays of all lists spread over the whole grid: a grid
+// per entry left the lists of the large scenes to a few workgroups each.  A list that
+// filled up marks the entry (redo_n[2]) and its scene (scene_full), whose maps are then
+// cleared and traced again wholly in FP64 (trace64_batch_full_kernel).
 template <int SUN, bool POISSON>
 __global__ __launch_bounds__(WG) void trace_redo_batch_kernel(const MixLaunch *__restrict__ table,
                                                               const int *__restrict__ entry_scene,
-                                                              uint32_t *scene_full)
+                                                              uint32_t *scene_full, int n_entries)
 {
-    const MixLaunch &L = table[blockIdx.y];
-    const TraceArgs &a = L.a;
-    const uint32_t cnt = *L.x.redo_n;
-    if (cnt > (uint32_t)L.x.redo_cap) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            L.x.redo_n[2] = 1u;
-            atomicOr(scene_full + entry_scene[blockIdx.y], 1u);
+    const uint32_t nthreads = gridDim.x * WG;
+    for (int en = 0; en < n_entries; ++en) {
+        const MixLaunch &L = table[en];
+        const TraceArgs &a = L.a;
+        const uint32_t cnt = *L.x.redo_n;
+        if (cnt > (uint32_t)L.x.redo_cap) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                L.x.redo_n[2] = 1u;
+                atomicOr(scene_full + entry_scene[en], 1u);
+            }
+            continue;
         }
-        return;
-    }
-    unsigned long long *part = copy_of(a);
-    const unsigned long long one = 1ull << a.pack_shift;
-    for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < cnt; i += gridDim.x * WG) {
-        const uint2 e = L.x.redo[i];
-        const Ray64 o = ray64_at<SUN, POISSON>(a, L.b, (int)e.x, (int)e.y);
-        if (o.px >= 0) {
-            const unsigned long long w = one + o.q;
-            copy_check(a, o.px, w, copy_add(a, part, o.px, w));
+        const uint32_t first = ((blockIdx.x + (uint32_t)en * 97u) % gridDim.x) * WG + threadIdx.x;
+        if (first >= cnt) continue;
+        unsigned long long *part = copy_of(a);
+        const unsigned long long one = 1ull << a.pack_shift;
+        for (uint32_t i = first; i < cnt; i += nthreads) {
+            const uint2 e = L.x.redo[i];
+            const Ray64 o = ray64_at<SUN, POISSON>(a, L.b, (int)e.x, (int)e.y);
+            if (o.px >= 0) {
+                const unsigned long long w = one + o.q;
+                copy_check(a, o.px, w, copy_add(a, part, o.px, w));
+            }
         }
     }
 }
@@ -1485,12 +1492,13 @@ om_status launch_trace_batch_mixed(const MixBatch &m, hipStream_t st)
     hipLaunchKernelGGL(batch_mix_reset_kernel, dim3((m.n_entries + m.n_scenes + WG - 1) / WG), dim3(WG), 0, st,
                        m.counters, m.n_entries, m.n_scenes);
     uint32_t *full = m.counters + 4 * (size_t)m.n_entries;
-    const dim3 grid(m.nwg), block(WIN_WGS), rgrid(REDO_BATCH_BLOCKS, m.n_entries), wblock(WG);
+    const dim3 grid(m.nwg), block(WIN_WGS), rgrid(REDO_BLOCKS), wblock(WG);
 #define OM_MIXB(SUN, P)                                                                                             \
     do {                                                                                                            \
         hipLaunchKernelGGL((trace_batch_window_mixed_kernel<SUN, P>), grid, block, 0, st, table, m.chunks,           \
                            m.n_chunks, m.queue);                                                                    \
-        hipLaunchKernelGGL((trace_redo_batch_kernel<SUN, P>), rgrid, wblock, 0, st, table, m.entry_scene, full);     \
+        hipLaunchKernelGGL((trace_redo_batch_kernel<SUN, P>), rgrid, wblock, 0, st, table, m.entry_scene, full,      \
+                           m.n_entries);                                                                            \
         hipLaunchKernelGGL(clear_full_scenes_kernel, dim3(256, m.n_scenes), wblock, 0, st, m.slots, full);           \
         hipLaunchKernelGGL((trace64_batch_full_kernel<SUN, P>), dim3(REDO_BLOCKS), wblock, 0, st, table, m.chunks,   \
                            m.entry_chunks, m.entry_scene, full, m.n_entries);                                       \
