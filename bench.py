@@ -534,6 +534,7 @@ def main() -> int:
         fa = bench_facet_annual(sub_a)
         out.setdefault("modes", {}).update({
             "batched": {"value": b["value"], "unit": "rays/s", "workload": b["config"]["workload"],
+                        "roofline": b["roofline"],
                         **{k: b["config"][k] for k in ("precision", "scenes", "rays", "batch_ms", "plain_ms",
                                                        "plain_rays_per_s", "batched_vs_plain",
                                                        "grid_cells", "workgroup", "rays_per_thread",
@@ -815,6 +816,25 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
     ops_ray, S, T = model_ops(st)
     achieved = R * ops_ray / (batch_ms * 1e-3) / 1e12
     peak = P_VALU / 1e12
+    # HBM bytes and VALU issue of the batch kernel from its committed rocprof run, when that
+    # run traced this batch (the launch the autotuner picked)
+    traffic, counters = None, None
+    if tl["workgroup"] == 2048:
+        prof = os.path.join(ROOT, "profiles", "trace_batch_window_mixed_kernel_batched.json")
+        vp = os.path.join(ROOT, "profiles", "valu_probe_kernel.json")
+        if os.path.exists(prof) and os.path.exists(vp):
+            with open(prof) as f:
+                pj = json.load(f)
+            with open(vp) as f:
+                probe = json.load(f)["probes"]["probe_fma"]
+            if pj.get("rays") == R:
+                traffic = pj["hbm_bytes_per_launch"]
+                counters = {"valu_per_cu_busy_cycle": pj["valu_per_cu_busy_cycle"],
+                            "probe_valu_per_cu_busy_cycle": probe["valu_per_cu_busy_cycle"],
+                            "of_probe": pj["valu_per_cu_busy_cycle"] / probe["valu_per_cu_busy_cycle"],
+                            "valu_util": pj["SQ_INSTS_VALU"] * 64 / (pj["avg_ns"] * 1e-9) / P_VALU,
+                            "memory_side_atomics": pj["tcc_atomic"],
+                            "source": "profiles/trace_batch_window_mixed_kernel_batched.json"}
     return {
         "metric": METRIC, "value": R / dt, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
@@ -839,8 +859,8 @@ def bench_batched(args, with_cpu: bool = True) -> dict:
                    "grid_tuning": tune_grid["trials"] if tune_grid else None,
                    "launch_tuning": tl["trials"], "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tlane-op/s",
-                     "frac": achieved / peak, "traffic": None, "ops_per_ray": ops_ray,
-                     "dda_steps_per_ray": S, "tests_per_ray": T},
+                     "frac": achieved / peak, "traffic": traffic, "ops_per_ray": ops_ray,
+                     "dda_steps_per_ray": S, "tests_per_ray": T, "counters": counters},
         "cpu_baseline": None if not sample_cfg else {"value": st["rays"] / cpu_t, "unit": "rays/s", "cores": nthreads,
                          "kind": "port",
                          "sample": f"the first 40 heliostats of {len(sample_cfg)} of the scenes "
