@@ -155,29 +155,46 @@ struct DiffTile {
 };
 
 // The sun ray of ray l of microsample m (the plain path's Philox draws and sunshape): the
-// direction toward the sun and the draws, whose slope-error words the caller uses.  The
+// direction toward the sun, and the slope-error normal nrm0 about the nominal normal (the
+// plain path's perturbed_normal of the same draws).  With the Buie sunshape the two loads of
+// the bin lookup (buie_bin_split) are issued ahead of the normal and the sunshape azimuth,
+// which do not depend on them and run while they are in flight, as in the plain trace.  The
 // microsample's origin comes from the wave's start table (StartTable, om_device.h), the same
 // bits as ray_start.
 template <int SUN>
-__device__ __forceinline__ void diff_dir(const TraceArgs &a, const DiffTile &T, int m, int l, v3 &to_sun, u4 &x)
+__device__ __forceinline__ void diff_dir(const TraceArgs &a, const DiffTile &T, int m, int l, v3 &to_sun, v3 &nrm0)
 {
-    x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)T.h, a.w3}, a.key0, a.key1);
-    float th;
+    const u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)T.h, a.w3}, a.key0, a.key1);
+    float th, sp, cp;
     if (SUN == OM_SUN_PILLBOX) {
         th = a.pillbox * sqrt_rn(u01(x.x));
+        nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
+        sincos2pi(u01(x.y), sp, cp);
     } else {
-        float v = u01(x.x) * a.buie_A;
+        const float v = u01(x.x) * a.buie_A;
+        const int j = min(max(trunc_i(v * a.buie_gscale), 0), a.nbins - 1);
+        const int2 g = a.buie_split[j];
+        __builtin_amdgcn_sched_barrier(0); // the load above is issued before this work
+        nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
+        asm volatile("" ::"v"(nrm0.x), "v"(nrm0.y), "v"(nrm0.z)); // computed here
+        __builtin_amdgcn_sched_barrier(0);
+        int i = min(g.x + (v >= u2f((uint32_t)g.y) ? 1 : 0), a.nbins - 1);
+        float4 bb = a.buie_bins[i];
+        __builtin_amdgcn_sched_barrier(0);
+        sincos2pi(u01(x.y), sp, cp);
+        asm volatile("" ::"v"(sp), "v"(cp));
+        __builtin_amdgcn_sched_barrier(0);
         if (v < a.buie_int465) {
-            const float4 b = buie_bin_split(a, v);
-            th = (v - b.x) * b.z + b.w;
+            while (bb.x > v) bb = a.buie_bins[--i];
+            while (bb.y <= v) bb = a.buie_bins[++i];
+            th = (v - bb.x) * bb.z + bb.w;
         } else {
             th = pow_<gfx>((v - a.buie_int465) * a.buie_B + a.buie_C, a.buie_inv_gp1);
         }
         th = div_rn(th, 1000.0f);
     }
-    float st, ct, sp, cp;
+    float st, ct;
     sincos_small(th, st, ct);
-    sincos2pi(u01(x.y), sp, cp);
     to_sun = -to_world(angle2xyz(st, ct, sp, cp), T.sf);
 }
 
@@ -243,13 +260,11 @@ __global__ __launch_bounds__(WG) void diff_forward_kernel(TraceArgs, DiffArgs)
         tab.update<POISSON>(a, T, r < T.nr, m, lane); // the whole wave
         const RayStart rs = tab.get(a, m);
         if (r < T.nr) {
-            v3 to_sun;
-            u4 x;
-            diff_dir<SUN>(a, T, m, l, to_sun, x);
+            v3 to_sun, nrm0;
+            diff_dir<SUN>(a, T, m, l, to_sun, nrm0);
             const v3 orig = rs.orig;
             const cell3 c0 = rs.c0;
             if (T.clear || dda(a, orig, c0, to_sun, T.h) < 0) {
-                const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
                 if (diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o) && dda(a, orig, c0, o.r, T.h) < 0) {
                     vis = true;
                     esum += (unsigned long long)((double)o.e * 4294967296.0);
@@ -452,11 +467,9 @@ __global__ __launch_bounds__(WG) void diff_backward_kernel(TraceArgs, DiffArgs)
         tab.update<POISSON>(a, T, r < T.nr, m, lane); // the whole wave
         const RayStart rs = tab.get(a, m);
         if (r >= T.nr || !((bits >> lane) & 1ull)) continue;
-        v3 to_sun;
-        u4 x;
-        diff_dir<SUN>(a, T, m, l, to_sun, x);
+        v3 to_sun, nrm0;
+        diff_dir<SUN>(a, T, m, l, to_sun, nrm0);
         const v3 orig = rs.orig;
-        const v3 nrm0 = perturbed_normal(x, a.slope_sigma, T.hf);
         DiffRay o;
         if (!diff_ray(p, orig, to_sun, nrm0, T.hf.n, T.nth, o)) continue; // (not reached)
         // gather over the kernel window (none when the window misses the map: then only the
