@@ -1480,6 +1480,9 @@ static om_status trace_grid(om_scene *s, int g, int h0, int h1, const om_flux *o
                            a.wrap, a.pack_shift, out->counts, (unsigned long long *)out->energy, npx,
                            ncopy, a.copy_stride, a.px_stride, win);
         OM_HIP(hipGetLastError());
+        // the workgroups' queue numbers are read by the decode, which leaves them: cleared
+        // here, since a later trace may take these words as packed copies
+        if (win.slice_q) return launch_fill_u32((uint32_t *)win.slice_q, p.nwg, st);
         return OM_OK;
     }
     a.counts = out->counts;

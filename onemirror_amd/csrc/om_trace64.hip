@@ -339,15 +339,25 @@ __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dfr
     const u4 x = philox_uniform_key(u4{(uint32_t)l, (uint32_t)m, (uint32_t)h, a.w3},
                                     __builtin_amdgcn_readfirstlane(a.key0), __builtin_amdgcn_readfirstlane(a.key1));
     const double u0 = u01(x.x), u1 = u01(x.y), u2 = u01(x.z), u3 = u01(x.w);
+    // the slope-error normal does not depend on the sunshape draw: with the Buie sunshape it
+    // is formed while the guide entry's load is in flight
+    const double v = u0 * b.A;
+    const bool head = SUN != OM_SUN_PILLBOX && v < b.int465;
+    int lo = 0;
+    if (head) lo = b.guide[min(max((int)(v * b.gscale), 0), b.nbins - 1)];
+    __builtin_amdgcn_sched_barrier(0); // the load above is issued before this work
+    const double ths = erfinv_ray(2.0 * u2 - 1.0) * 1.4142135623730951 * b.sigma;
+    const double phs = 6.283185307179586 * u3;
+    const double2 sts = sincos_polar(ths), sps = sincos_ray(phs);
+    const d3 nrm = normalized(world(mkd(sts.x * sps.y, sts.y, sts.x * sps.x), hf));
+    __builtin_amdgcn_sched_barrier(0);
     double th;
     if (SUN == OM_SUN_PILLBOX) {
         th = b.pillbox * sqrt(u0);
     } else {
-        const double v = u0 * b.A;
-        if (v < b.int465) {
+        if (head) {
             // the largest lo with cdf_y[lo] <= v (max_less_index, scene_instance_process.cu:
             // 158-173), from the guide entry of v
-            int lo = b.guide[min(max((int)(v * b.gscale), 0), b.nbins - 1)];
             while (lo > 0 && b.cdf_y[lo] > v) --lo;
             while (b.cdf_y[lo + 1] <= v) ++lo;
             th = (v - b.cdf_y[lo]) * b.slope[lo] + b.cdf_x[lo];
@@ -360,10 +370,6 @@ __device__ Ray64 trace_ray64(const TraceArgs &a, const Trace64Args &b, const dfr
     const double ph = 6.283185307179586 * u1;
     const double2 st = sincos_polar(th), sp = sincos_ray(ph);
     const d3 to_sun = neg(world(mkd(st.x * sp.y, st.y, st.x * sp.x), sf));
-    const double ths = erfinv_ray(2.0 * u2 - 1.0) * 1.4142135623730951 * b.sigma;
-    const double phs = 6.283185307179586 * u3;
-    const double2 sts = sincos_polar(ths), sps = sincos_ray(phs);
-    const d3 nrm = normalized(world(mkd(sts.x * sps.y, sts.y, sts.x * sps.x), hf));
     r.shade = clear ? -1 : walk64(a, b, orig, c0, to_sun, self);
     if (r.shade >= 0) return r;
     const d3 refl = normalized(reflected(neg(to_sun), nrm));
@@ -1123,12 +1129,19 @@ __global__ __launch_bounds__(WG) void trace_redo_kernel(MixLaunch)
         atomicAdd(L.x.redo_n + 1, cnt);
         if (full) atomicAdd(L.x.redo_n + 2, 1u);
     }
-    if (full) return;
+    // The list lies in the trace's workspace, whose words a later trace may take as packed
+    // copies, which it adds to: every entry written is cleared once read.
+    if (full) {
+        for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < (uint32_t)L.x.redo_cap; i += gridDim.x * WG)
+            L.x.redo[i] = make_uint2(0u, 0u);
+        return;
+    }
     const uint32_t n = cnt;
     unsigned long long *part = copy_of(a);
     const unsigned long long one = 1ull << a.pack_shift;
     for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
         const uint2 e = L.x.redo[i];
+        L.x.redo[i] = make_uint2(0u, 0u);
         const Ray64 o = ray64_at<SUN, POISSON>(a, L.b, (int)e.x, (int)e.y);
         if (o.px >= 0) {
             const unsigned long long w = one + o.q;

@@ -172,3 +172,33 @@ def test_differentiable_mode_refuses_mixed(om):
     theta = torch.zeros((g.info["n_helio"], 4), dtype=torch.float32, device="cuda")
     with pytest.raises(om.OMError, match="MIXED"):
         diff.forward(g, theta)
+
+
+def test_workspace_is_clean_after_a_mixed_trace(om):
+    """A mixed trace's redo list and the window workgroups' queue numbers lie in the trace
+    workspace, past the packed copies of its map.  A later trace of a larger map on the same
+    context takes those words as packed copies and adds to them, so they must be left
+    cleared: the whole 5,000-heliostat field in mixed precision (its list and sector-free
+    window), then the cylinder-map heliostat slice of the 25,000-heliostat field in FP32,
+    whose maps must equal the oracle's; and again after a mixed trace of a cylinder field
+    with sector windows."""
+    import torch
+    d, p = F.config_mid()
+    g = om.SolarScene(d, _mixed(p, om))
+    g.trace_plain()
+    torch.cuda.synchronize()
+    assert g.redo_count() > 0
+    dl, pl = F.config_large()
+    big = om.SolarScene(dl, pl, g.ctx)
+    want = oracle.OracleScene(dl, pl).trace_f32(12000, 12006)
+    for rep in range(2):
+        maps = big.new_maps(smooth=False)
+        big.trace_heliostats(12000, 12006, maps)
+        torch.cuda.synchronize()
+        assert np.array_equal(maps.counts.cpu().numpy().view(np.uint32), want["counts"]), rep
+        assert np.array_equal(maps.energy.cpu().numpy().view(np.uint64), want["energy"]), rep
+        dc, pc = F.config_small_cylinder()
+        cyl = om.SolarScene(dict(dc, recv_res_x=320, recv_res_y=160), _mixed(pc, om), g.ctx)
+        cyl.trace_plain()
+        torch.cuda.synchronize()
+        assert cyl.redo_count() > 0
